@@ -1,0 +1,155 @@
+"""Benchmark: plain wavefront path tracer on one MI355X, 1M-triangle terrain + icosphere scene.
+
+Workload (one step = one full frame): seeded fBm terrain (1000x500 grid, 997,002 triangles) plus
+200 icospheres (subdivision 3, 256,000 triangles), 50% Lambert / 50% GGX conductor, 16 emissive
+quads, 1920x1080 at 64 spp, max depth 8, Russian roulette after depth 3, FP32 film. Scene data is
+resident in HBM before the timed region.
+
+Prints one JSON line: Mpaths/s, the extension-traversal kernel's roofline (algorithmic bytes per
+launch / mean launch time, HIP events on the library's stream), and the CPU oracle timed on this
+host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, threads, budget_s=12.0):
+    """CPU oracle (the C++ restatement of the reference CPU path) on a crop of the same frame."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleScene
+
+    t0 = time.perf_counter()
+    o = OracleScene(sc)
+    log(f"cpu oracle: BVH built in {time.perf_counter() - t0:.1f}s")
+    w, h = sc["width"], sc["height"]
+
+    def run(cw, ch, spp):
+        rect = ((w - cw) // 2, (h - ch) // 2, cw, ch)
+        t = time.perf_counter()
+        o.render(rect=rect, spp_range=(0, spp), threads=threads, film=True)
+        return time.perf_counter() - t, rect
+
+    dt, _ = run(192, 108, 1)  # calibration
+    target = 192 * 108 / max(dt, 1e-6) * budget_s  # paths that fit the budget
+    if target >= w * h:
+        cw, ch, spp = w, h, max(1, min(64, int(target // (w * h))))
+    else:
+        scale = (target / (w * h)) ** 0.5
+        cw, ch, spp = max(64, int(w * scale) // 8 * 8), max(36, int(h * scale) // 8 * 8), 1
+    dt, rect = run(cw, ch, spp)
+    paths = cw * ch * spp
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, own binned-SAH BVH2) on a {cw}x{ch} crop at {spp} spp "
+                      f"({paths} paths, {dt:.1f} s) of the same 1920x1080 frame"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--wavefront-paths", type=int, default=0)
+    ap.add_argument("--bvh-width", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the render shards by tiles or "
+              "scenes with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import piper2_amd as pg
+    from piper2_amd import scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.terrain_spheres(spp=args.spp)
+    log(f"scene generated in {time.perf_counter() - t0:.1f}s: {sc['indices'].shape[0]} triangles")
+    dev = pg.Device(0)
+    t0 = time.perf_counter()
+    scene = pg.Scene(dev, sc, bvh_width=args.bvh_width)
+    info = scene.bvh_info()
+    log(f"device {dev.platform}; BVH{info['width']} built in {time.perf_counter() - t0:.1f}s: {info}")
+    film = pg.Film(dev, sc["width"], sc["height"])
+    params = pg.render_params(sc, wavefront_paths=args.wavefront_paths)
+
+    # traversal work per extension ray on a 4-sample slice (counting perturbs timing: untimed)
+    cnt = pg.render_params(sc, spp_range=(0, 4), flags=pg.PG_COUNT_TRAVERSAL,
+                           wavefront_paths=args.wavefront_paths)
+    st = scene.render(cnt, None)
+    nodes_per_ray = st["nodes_visited"] / st["extension_rays"]
+    tris_per_ray = st["tris_tested"] / st["extension_rays"]
+    log(f"traversal: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles per extension ray")
+
+    for i in range(args.warmup):
+        film.clear()
+        st = scene.render(params, film)
+        log(f"warmup {i}: {st['ms_total']:.1f} ms")
+    totals = {k: 0.0 for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_mis", "ms_film", "ms_raygen")}
+    rays = {"extension_rays": 0, "shadow_rays": 0, "mis_rays": 0, "extend_launches": 0, "paths": 0}
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        film.clear()
+        st = scene.render(params, film)
+        for k in totals:
+            totals[k] += st[k]
+        for k in rays:
+            rays[k] += st[k]
+        log(f"step {i}: {st['ms_total']:.1f} ms, {st['paths'] / st['ms_total'] / 1e3:.1f} Mpaths/s")
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / args.steps
+    paths = sc["width"] * sc["height"] * args.spp * args.steps
+    mpaths = paths / elapsed / 1e6
+    total_rays = rays["extension_rays"] + rays["shadow_rays"] + rays["mis_rays"]
+
+    bytes_per_ray = 32 + 20 + nodes_per_ray * info["node_bytes"] + tris_per_ray * info["tri_bytes"]
+    launches = max(rays["extend_launches"], 1)
+    bytes_per_launch = rays["extension_rays"] / launches * bytes_per_ray
+    mean_launch_s = totals["ms_extend"] / launches / 1e3
+    achieved = bytes_per_launch / mean_launch_s / 1e9
+
+    result = {
+        "metric": "Mpaths/s (plain GPU wavefront path tracer)",
+        "value": round(mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "terrain_spheres_1920x1080_64spp_depth8_rr3", "triangles": int(sc["indices"].shape[0]),
+                   "lights": 16, "spp": args.spp, "bvh_width": info["width"],
+                   "wavefront_paths": args.wavefront_paths or (1 << 21)},
+        "mrays_per_s": round(total_rays / elapsed / 1e6, 2),
+        "rays_per_path": round(total_rays / paths, 3),
+        "stage_ms_per_step": {k[3:]: round(v / args.steps, 2) for k, v in totals.items()},
+        "roofline": {"bound": "hbm", "kernel": "kExtend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
+                     "tris_per_ray": round(tris_per_ray, 3), "mean_launch_ms": round(mean_launch_s * 1e3, 4),
+                     "launches": launches},
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(sc, threads)
+        log(f"cpu baseline: {result['cpu_baseline']}")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

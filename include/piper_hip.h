@@ -1,0 +1,166 @@
+/* piper_hip.h - C ABI of the MI355X (gfx950) path-tracing library libpiper_hip.so.
+ *
+ * This is the drop-in boundary for Piper2's GPU render path. Piper2 renders through two plugin
+ * interfaces, `Tracer` (Src/Interface/BuiltinComponent/Tracer.hpp:140-159) and `Accelerator`
+ * (Src/Interface/Infrastructure/Accelerator.hpp:130-161); the C++ plugin pair in
+ * piper2_amd/plugin/ implements those interfaces on top of the entry points below, so every
+ * entry point names the reference interface it replaces.
+ *
+ * Conventions: every call returns PG_OK (0) or a negative PG_ERR_* code and never throws.
+ * pg_last_error() returns a thread-local message for the last failure. The library owns device
+ * memory behind opaque handles; the caller owns every host array it passes. One HIP stream per
+ * device handle; calls on one handle are serialised by an internal mutex (the reference admits
+ * one live TraceLauncher per pipeline, Tracer/Embree/Embree.cpp:993-996).
+ */
+#ifndef PIPER_HIP_H
+#define PIPER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_ERR_INVALID (-1)     /* bad argument or size */
+#define PG_ERR_DEVICE (-2)      /* HIP runtime error, or a forced device fault (pg_debug_inject_fault) */
+#define PG_ERR_OOM (-3)         /* device allocation failed */
+#define PG_ERR_UNSUPPORTED (-4) /* feature or symbol not implemented by this library */
+
+/* Reference-compatibility bits (pg_render_params.flags). Default 0 = corrected ("spec") math. */
+#define PG_COMPAT_D1 0x1u  /* light CDF remap with the upper bound (Kernel/Sampling.hpp:61) */
+#define PG_COMPAT_D3 0x2u  /* unnormalised shading tangents (Tracer/Embree/Kernel.cpp:180-186) */
+#define PG_COMPAT_D9 0x4u  /* u2 drawn before u1 (Tracer/Embree/Kernel.cpp:117,145, MS ABI order) */
+#define PG_COMPAT_D13 0x8u /* light pass-through decrements depth (PathTracing/Kernel.cpp:143) */
+#define PG_FILM_FP64 0x100u /* accumulate into the film's FP64 RGBW buffer instead of FP32 */
+#define PG_COUNT_TRAVERSAL 0x200u /* count BVH nodes visited / triangles tested by extension rays (slower) */
+
+#define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
+#define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
+
+#define PG_HIT_MISS 0xFFFFFFFFu
+#define PG_HIT_QUAD 0x80000000u /* prim = PG_HIT_QUAD | quad index for a light-quad hit */
+
+typedef struct pg_device pg_device;
+typedef struct pg_scene pg_scene;
+typedef struct pg_film pg_film;
+
+typedef struct pg_material {
+    uint32_t type;      /* PG_MAT_* */
+    float albedo[3];    /* Lambert rho (ConstantData, ProceduralTexture/Kernel.cpp:21-26) or conductor F0 */
+    float alpha;        /* GGX alpha = roughness^2 (conductor only) */
+    float reserved[3];
+} pg_material;
+
+typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22-28, AreaLight/Shared.hpp:22-26) */
+    float origin[3], u[3], v[3];
+    float radiance[3];
+} pg_quad;
+
+typedef struct pg_camera { /* PCData (Sensor/PerspectiveCamera/Shared.hpp:22-29) plus the node position */
+    float position[3], look_at[3], up[3];
+    float sensor_size[2];  /* metres */
+    float focal_length;    /* metres */
+    float aperture_radius; /* metres; focal_length / (2 FStop), 0 = pinhole */
+} pg_camera;
+
+typedef struct pg_scene_desc {
+    /* Triangle mesh buffer as TriangleIndexedGeometryDesc (Interface/BuiltinComponent/Tracer.hpp:51-63):
+     * uint3 index[tri] + float3 vertex[vert]; instances are passed flattened to world space. */
+    const float* vertices;
+    uint32_t num_vertices;
+    const uint32_t* indices;
+    uint32_t num_triangles;
+    const uint32_t* tri_object;      /* object id per triangle */
+    const uint32_t* object_material; /* material id per object */
+    uint32_t num_objects;
+    const pg_material* materials;
+    uint32_t num_materials;
+    const pg_quad* quads; /* area lights; light 0 is the environment (Embree.cpp:1175-1188) */
+    uint32_t num_quads;
+    pg_camera camera;
+    float env_radiance[3]; /* constant environment (BasicLight/Kernel.cpp:59-89) */
+} pg_scene_desc;
+
+typedef struct pg_bvh_params {
+    uint32_t width;     /* 4 or 8 children per node */
+    uint32_t leaf_size; /* max triangles per leaf, 1..8 */
+    uint32_t sah_bins;  /* 8..64 */
+    uint32_t reserved;
+} pg_bvh_params;
+
+typedef struct pg_render_params {
+    uint32_t width, height;      /* film resolution */
+    uint32_t spp;                /* samples per pixel; rounded up to a power of two (ScrambledSobolSampler.cpp:43-50) */
+    uint32_t rect[4];            /* render rect x, y, w, h (RenderRECT) */
+    uint32_t spp_begin, spp_end; /* sample range [begin, end) rendered by this call */
+    uint32_t max_depth, min_rr;  /* MaxTraceDepth, MinRussianRouletteDepth (PathTracing.cpp:38-39) */
+    uint32_t flags;              /* PG_COMPAT_*, PG_FILM_FP64 */
+    float ndc[4];                /* SensorNDCAffineTransform ox, oy, sx, sy (Sensor.hpp:38-74) */
+    uint32_t wavefront_paths;    /* paths in flight per wave of work; 0 = library default */
+    uint32_t reserved[3];
+} pg_render_params;
+
+typedef struct pg_stats {
+    uint64_t paths;
+    uint64_t extension_rays, shadow_rays, mis_rays;
+    uint64_t extend_launches;
+    double ms_total;   /* host wall time of the call, stream synchronised */
+    double ms_raygen, ms_extend, ms_shade, ms_shadow, ms_mis, ms_film; /* HIP-event time per stage */
+    uint64_t nodes_visited, tris_tested; /* extension rays, with PG_COUNT_TRAVERSAL only */
+} pg_stats;
+
+/* ---- device (Accelerator: enumerateContexts / getNativePlatform, Accelerator.hpp:130-161) ---- */
+int pg_device_create(int ordinal, pg_device** out);
+void pg_device_destroy(pg_device* dev);
+const char* pg_device_platform(pg_device* dev); /* "AMD HIP gfx950" (getNativePlatform) */
+const char* pg_last_error(void);
+
+/* ---- scene (Tracer::buildAcceleration / buildNode, Tracer.hpp:140-159; Embree.cpp:497-655) ---- */
+int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out);
+int pg_scene_build_bvh(pg_scene* scene, const pg_bvh_params* params);
+void pg_scene_destroy(pg_scene* scene);
+/* BVH shape after a build: {nodes, leaves, max depth, width, node bytes, triangle bytes}. */
+int pg_scene_bvh_info(pg_scene* scene, uint64_t info[6]);
+/* Copy the built BVH to the host: nodes (node bytes each), triangles (48 B each: v0,prim | v1 | v2). */
+int pg_scene_bvh_download(pg_scene* scene, void* nodes, uint64_t node_bytes, void* tris, uint64_t tri_bytes);
+
+/* ---- film (FixedSampler RGBW film, RenderDriver/FixedSampler/Kernel.cpp:22-52) ---- */
+int pg_film_create(pg_device* dev, uint32_t width, uint32_t height, pg_film** out);
+int pg_film_clear(pg_film* film);
+void pg_film_destroy(pg_film* film);
+/* rgbw: float[4*w*h] from the FP32 buffer; rgbw64: double[4*w*h] from the FP64 buffer (either may be NULL). */
+int pg_film_download(pg_film* film, float* rgbw, double* rgbw64);
+
+/* ---- render (TraceLauncher::launch + RenderDriver::renderFrame, Tracer.hpp:100-119,
+ *      FixedSampler.cpp:68-133; per-sample kernel piperMain, Tracer/Embree/Kernel.cpp:47-100) ---- */
+int pg_render(pg_scene* scene, const pg_render_params* params, pg_film* film, pg_stats* stats);
+/* Same render, but also writes each path's radiance (rgb float) to path_L (host), path order
+ * ((py-rect.y)*rect.w + (px-rect.x))*(spp_end-spp_begin) + (s-spp_begin). Parity aid. */
+int pg_render_paths(pg_scene* scene, const pg_render_params* params, pg_film* film, float* path_L,
+                    pg_stats* stats);
+
+/* ---- parity entry points (Embree rtcIntersect1 / rtcOccluded1, Embree.cpp:300,384; sampler
+ *      sobolStart / sobolGenerate, ScrambledSobolSampler/Kernel.cpp:52-67) ---- */
+/* rays: float[8*n] = {o xyz, d xyz, tmin, tmax}; outputs t[n], prim[n], uv[2n] (host arrays). */
+int pg_trace(pg_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv);
+int pg_occluded(pg_scene* scene, const float* rays, uint32_t n, uint8_t* occluded);
+/* Camera rays for (px, py, sample) triples as piperMain makes them; rays as in pg_trace, film[2n]. */
+int pg_camera_rays(pg_scene* scene, const pg_render_params* params, const uint32_t* pix, uint32_t n,
+                   float* rays, float* film);
+/* Sobol values: out[n*dims] for samples (px, py, s) at resolution 2^m: dims 0..dims-1 of index
+ * sobolIntervalToIndex(m, s, px, py); index_out[n] (optional) receives the indices. */
+int pg_sobol(pg_device* dev, uint32_t m, uint32_t dims, const uint32_t* pix, uint32_t n, float* out,
+             uint64_t* index_out);
+/* Generator matrices, uint32[1024*52] (the reference's SobolMatrix.dat layout, Shared.hpp:22-23). */
+int pg_sobol_matrices(uint32_t* out);
+
+/* ---- diagnostics ---- */
+/* Make the next device call on this handle fail with PG_ERR_DEVICE (host-forced fault path test). */
+int pg_debug_inject_fault(pg_device* dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIPER_HIP_H */

@@ -1,0 +1,1029 @@
+// CPU oracle: a plain C++ restatement of Piper2's CPU path-tracing path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker that the HIP product path is compared
+// against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load the
+// library built from it. The product library never links or calls it.
+//
+// Each function cites the reference file:line it restates (paths relative to the Piper2 tree).
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off, so no FMA contraction: the camera, the
+// watertight triangle test and the Sobol float conversion must match the GPU bit for bit).
+//
+// Reference defects (spec mode corrects them; the compat bits reproduce the reference):
+//   D1  light CDF remap uses the upper bound (Kernel/Sampling.hpp:61)          -> OR_COMPAT_D1
+//   D3  shading T/B not normalised (Tracer/Embree/Kernel.cpp:180-186)           -> OR_COMPAT_D3
+//   D9  u2 drawn before u1 under the MS ABI (Tracer/Embree/Kernel.cpp:117,145)  -> OR_COMPAT_D9
+//   D13 light pass-through decrements depth, wrapping at 0
+//       (Integrator/PathTracing/Kernel.cpp:143)                                 -> OR_COMPAT_D13
+// GGX sampling uses exact visible-normal sampling with its exact pdf in both modes (the reference's
+// pdf omits D(wh), Surface/Classic/Kernel.cpp:405-412, so its plain estimator is biased).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Sobol (Sampler/ScrambledSobolSampler/Kernel.cpp:23-67, ScrambledSobolSampler.cpp:66-72)
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kSobolDims = 1024;  // Shared.hpp:22
+constexpr uint32_t kSobolCols = 52;    // Shared.hpp:23
+
+struct JoeKuo {
+    uint32_t s, a;
+    uint32_t m[18];
+};
+const JoeKuo kJoeKuo[kSobolDims] = {
+#include "../piper2_amd/csrc/joe_kuo_1024.inc"
+};
+
+uint32_t g_sobol[kSobolDims * kSobolCols];
+// Interval-to-index tables for resolution 2^m, m = 1..16: column c of the inverse of the
+// (2m x 2m) GF(2) map from the low 2m index bits to (top m bits of dim 0, top m bits of dim 1),
+// and the contribution of sample-index bit c (index bit 2m+c) to those 2m output bits.
+uint64_t g_inv[17][32];
+uint64_t g_delta[17][64];
+
+// Bratley-Fox / Joe-Kuo recurrence at 64-bit precision; column j of dimension d keeps the high
+// 32 bits (v_j = m_j / 2^j).
+void buildSobolMatrices() {
+    for (uint32_t d = 0; d < kSobolDims; ++d) {
+        uint64_t v[kSobolCols];
+        if (d == 0) {
+            for (uint32_t j = 0; j < kSobolCols; ++j) v[j] = 1ULL << (63 - j);
+        } else {
+            const uint32_t s = kJoeKuo[d].s, a = kJoeKuo[d].a;
+            for (uint32_t j = 0; j < kSobolCols; ++j) {
+                if (j < s) {
+                    v[j] = static_cast<uint64_t>(kJoeKuo[d].m[j]) << (63 - j);
+                } else {
+                    uint64_t x = v[j - s] ^ (v[j - s] >> s);
+                    for (uint32_t k = 1; k < s; ++k)
+                        if ((a >> (s - 1 - k)) & 1) x ^= v[j - k];
+                    v[j] = x;
+                }
+            }
+        }
+        for (uint32_t j = 0; j < kSobolCols; ++j) g_sobol[d * kSobolCols + j] = static_cast<uint32_t>(v[j] >> 32);
+    }
+}
+
+// Output bits (top m of dim 0 << m | top m of dim 1) produced by index bit j.
+uint64_t topBits(uint32_t m, uint32_t j) {
+    const uint32_t x = g_sobol[0 * kSobolCols + j] >> (32 - m);
+    const uint32_t y = g_sobol[1 * kSobolCols + j] >> (32 - m);
+    return (static_cast<uint64_t>(x) << m) | y;
+}
+
+void buildIntervalTables() {
+    for (uint32_t m = 1; m <= 16; ++m) {
+        const uint32_t n = 2 * m;
+        // Gauss-Jordan over GF(2): columns A[j] = topBits(j), solve A x = e_i for each unit e_i.
+        uint64_t rows[32];  // augmented: row r has bits of A's row r (n bits) | identity (n bits) << n
+        for (uint32_t r = 0; r < n; ++r) {
+            uint64_t row = 0;
+            for (uint32_t j = 0; j < n; ++j)
+                if ((topBits(m, j) >> r) & 1) row |= 1ULL << j;
+            rows[r] = row | (1ULL << (n + r));
+        }
+        for (uint32_t c = 0; c < n; ++c) {
+            uint32_t p = c;
+            while (p < n && !((rows[p] >> c) & 1)) ++p;
+            if (p == n) std::abort();  // singular: not a (0,2)-sequence
+            std::swap(rows[p], rows[c]);
+            for (uint32_t r = 0; r < n; ++r)
+                if (r != c && ((rows[r] >> c) & 1)) rows[r] ^= rows[c];
+        }
+        // rows[j] = (e_j | inverse row j); inverse column i = bits i of rows[*] >> n
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t col = 0;
+            for (uint32_t j = 0; j < n; ++j)
+                if ((rows[j] >> (n + i)) & 1) col |= 1ULL << j;
+            g_inv[m][i] = col;
+        }
+        for (uint32_t c = 0; c + n < kSobolCols && c < 64; ++c) g_delta[m][c] = topBits(m, n + c);
+    }
+}
+
+struct SobolInit {
+    SobolInit() {
+        buildSobolMatrices();
+        buildIntervalTables();
+    }
+} g_sobolInit;
+
+// sobolSampleImpl, Kernel.cpp:23-29 (scramble is always 0, ScrambledSobolSampler.cpp:52-53)
+float sobolSample(uint64_t index, uint32_t dim) {
+    if (dim >= kSobolDims) {
+        // Beyond the table the reference draws from mt19937_64 (Tracer/Embree/Kernel.cpp:37-45);
+        // both sides of this build use a splitmix64 hash of (index, dim) instead.
+        uint64_t z = index ^ (static_cast<uint64_t>(dim) * 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return static_cast<float>(static_cast<uint32_t>(z >> 40)) * 0x1p-24f;
+    }
+    uint32_t v = 0;
+    for (uint32_t i = dim * kSobolCols; index; index >>= 1, ++i)
+        if (index & 1) v ^= g_sobol[i];
+    return static_cast<float>(v) * 0x1p-32f;
+}
+
+// sobolIntervalToIndex, Kernel.cpp:31-50: the unique index whose top bits are `sample` and whose
+// first two coordinates fall in pixel (px, py) at resolution 2^m.
+uint64_t intervalToIndex(uint32_t m, uint32_t sample, uint32_t px, uint32_t py) {
+    if (m == 0) return 0;
+    const uint32_t n = 2 * m;
+    uint64_t index = static_cast<uint64_t>(sample) << n;
+    uint64_t b = (static_cast<uint64_t>(px) << m) | py;
+    for (uint32_t c = 0; sample; sample >>= 1, ++c)
+        if (sample & 1) b ^= g_delta[m][c];
+    for (uint32_t c = 0; b; b >>= 1, ++c)
+        if (b & 1) index ^= g_inv[m][c];
+    return index;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Small vector math (Kernel/Transform.hpp)
+// ---------------------------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+// Normal's normalising constructor, Transform.hpp:244-248
+inline V3 normalize(V3 a) {
+    const float l = length(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+inline float maxc(V3 a) { return std::fmax(a.x, std::fmax(a.y, a.z)); }
+inline bool valid(V3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }  // Protocol.hpp Spectrum::valid
+inline float comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+
+// ---------------------------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------------------------
+enum : uint32_t { kMatLambert = 0, kMatConductor = 1 };
+struct Material {
+    uint32_t type;
+    V3 albedo;  // Lambert rho, or conductor F0
+    float alpha;
+};
+struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
+    V3 origin, u, v, normal, radiance;
+    float area;
+};
+struct Camera {  // Sensor/PerspectiveCamera/Shared.hpp:22-29
+    V3 base, lookAt, up;
+    float sizeX, sizeY, focal, aperture;
+};
+struct Tri {
+    V3 v0, v1, v2;
+};
+struct BNode {
+    float lo[3], hi[3];
+    uint32_t left, count;  // count > 0: leaf over prims [left, left+count)
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<uint32_t> triObj, objMat;
+    std::vector<Material> mats;
+    std::vector<Quad> quads;
+    Camera cam;
+    V3 env;
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> order;  // BVH leaf order -> triangle index
+};
+
+// Binned-SAH BVH2 over triangle boxes (the oracle's own acceleration structure; only the hit
+// result matters, which is defined independently of the tree by the closest-(t, prim) rule).
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) lo[k] = std::fmin(lo[k], b.lo[k]), hi[k] = std::fmax(hi[k], b.hi[k]);
+    }
+    void grow(V3 p) {
+        const float c[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) lo[k] = std::fmin(lo[k], c[k]), hi[k] = std::fmax(hi[k], c[k]);
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+void buildBVH(Scene& s) {
+    const uint32_t n = static_cast<uint32_t>(s.tris.size());
+    s.nodes.clear();
+    s.order.resize(n);
+    if (n == 0) return;
+    std::vector<Box> boxes(n);
+    std::vector<float> cen(3 * n);
+    float maxAbs = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        boxes[i].grow(s.tris[i].v0), boxes[i].grow(s.tris[i].v1), boxes[i].grow(s.tris[i].v2);
+        for (int k = 0; k < 3; ++k) {
+            cen[3 * i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+            maxAbs = std::fmax(maxAbs, std::fmax(std::fabs(boxes[i].lo[k]), std::fabs(boxes[i].hi[k])));
+        }
+        s.order[i] = i;
+    }
+    const float pad = 1e-5f * std::fmax(maxAbs, 1.0f);
+    struct Item {
+        uint32_t node, begin, end;
+    };
+    s.nodes.push_back({});
+    std::vector<Item> work{{0, 0, n}};
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        Box b, cb;
+        for (uint32_t i = it.begin; i < it.end; ++i) {
+            const uint32_t t = s.order[i];
+            b.grow(boxes[t]);
+            cb.grow(V3{cen[3 * t], cen[3 * t + 1], cen[3 * t + 2]});
+        }
+        BNode& node = s.nodes[it.node];
+        for (int k = 0; k < 3; ++k) node.lo[k] = b.lo[k] - pad, node.hi[k] = b.hi[k] + pad;
+        const uint32_t cnt = it.end - it.begin;
+        int bestAxis = -1;
+        uint32_t bestSplit = 0;
+        float bestCost = static_cast<float>(cnt) * b.area();
+        constexpr int kBins = 16;
+        if (cnt > 2) {
+            for (int k = 0; k < 3; ++k) {
+                const float ext = cb.hi[k] - cb.lo[k];
+                if (!(ext > 0.0f)) continue;
+                Box bb[kBins];
+                uint32_t bc[kBins] = {};
+                for (uint32_t i = it.begin; i < it.end; ++i) {
+                    const uint32_t t = s.order[i];
+                    int bi = static_cast<int>((cen[3 * t + k] - cb.lo[k]) / ext * kBins);
+                    bi = std::min(std::max(bi, 0), kBins - 1);
+                    bb[bi].grow(boxes[t]);
+                    ++bc[bi];
+                }
+                float rightArea[kBins];
+                uint32_t rightCnt[kBins];
+                Box acc;
+                uint32_t ac = 0;
+                for (int i = kBins - 1; i > 0; --i) {
+                    acc.grow(bb[i]), ac += bc[i];
+                    rightArea[i] = acc.area(), rightCnt[i] = ac;
+                }
+                Box lacc;
+                uint32_t lc = 0;
+                for (int i = 0; i < kBins - 1; ++i) {
+                    lacc.grow(bb[i]), lc += bc[i];
+                    if (lc == 0 || rightCnt[i + 1] == 0) continue;
+                    const float cost = 0.125f * b.area() + lacc.area() * lc + rightArea[i + 1] * rightCnt[i + 1];
+                    if (cost < bestCost) bestCost = cost, bestAxis = k, bestSplit = static_cast<uint32_t>(i + 1);
+                }
+            }
+        }
+        uint32_t mid;
+        if (bestAxis >= 0) {
+            const float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+            const auto it2 = std::partition(s.order.begin() + it.begin, s.order.begin() + it.end, [&](uint32_t t) {
+                int bi = static_cast<int>((cen[3 * t + bestAxis] - cb.lo[bestAxis]) / ext * kBins);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                return static_cast<uint32_t>(bi) < bestSplit;
+            });
+            mid = static_cast<uint32_t>(it2 - s.order.begin());
+        } else if (cnt > 4) {
+            mid = it.begin + cnt / 2;  // SAH prefers a leaf but it is too large: median split
+        } else {
+            node.left = it.begin, node.count = cnt;
+            continue;
+        }
+        const uint32_t l = static_cast<uint32_t>(s.nodes.size());
+        s.nodes.push_back({});
+        s.nodes.push_back({});
+        s.nodes[it.node].left = l;
+        s.nodes[it.node].count = 0;
+        work.push_back({l, it.begin, mid});
+        work.push_back({l + 1, mid, it.end});
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ray casting (Tracer/Embree/Embree.cpp:267-391)
+// ---------------------------------------------------------------------------------------------
+struct Ray {
+    V3 o, d;
+};
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kQuadBit = 0x80000000u;
+struct Hit {
+    float t;
+    uint32_t prim;  // triangle index, kQuadBit|quad, or kMiss
+    float u, v;     // barycentrics of v1 and v2 (Embree convention, Embree/Kernel.cpp:176-178)
+};
+
+struct RayPre {  // per-ray constants of the watertight test (Woop, Benthin, Wald, JCGT 2013)
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float inv[3];
+};
+RayPre precompute(const Ray& r) {
+    RayPre p;
+    const float ad[3] = {std::fabs(r.d.x), std::fabs(r.d.y), std::fabs(r.d.z)};
+    p.kz = (ad[0] >= ad[1] && ad[0] >= ad[2]) ? 0 : (ad[1] >= ad[2] ? 1 : 2);
+    p.kx = p.kz == 2 ? 0 : p.kz + 1;
+    p.ky = p.kx == 2 ? 0 : p.kx + 1;
+    if (comp(r.d, p.kz) < 0.0f) std::swap(p.kx, p.ky);
+    p.sx = comp(r.d, p.kx) / comp(r.d, p.kz);
+    p.sy = comp(r.d, p.ky) / comp(r.d, p.kz);
+    p.sz = 1.0f / comp(r.d, p.kz);
+    for (int k = 0; k < 3; ++k) {
+        float dk = comp(r.d, k);
+        if (dk == 0.0f) dk = 1e-20f;  // box test only: avoids 0 * inf
+        p.inv[k] = 1.0f / dk;
+    }
+    return p;
+}
+
+// Watertight ray/triangle test. Returns t (or NaN on miss) and barycentrics (u: v1, v: v2).
+bool intersectTri(const Ray& r, const RayPre& p, const Tri& tri, float& t, float& bu, float& bv) {
+    const V3 a = tri.v0 - r.o, b = tri.v1 - r.o, c = tri.v2 - r.o;
+    const float az = comp(a, p.kz), bz = comp(b, p.kz), cz = comp(c, p.kz);
+    const float ax = comp(a, p.kx) - p.sx * az, ay = comp(a, p.ky) - p.sy * az;
+    const float bx = comp(b, p.kx) - p.sx * bz, by = comp(b, p.ky) - p.sy * bz;
+    const float cx = comp(c, p.kx) - p.sx * cz, cy = comp(c, p.ky) - p.sy * cz;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {  // exact edge: redo the 2D edge tests in double
+        U = static_cast<float>(static_cast<double>(cx) * by - static_cast<double>(cy) * bx);
+        V = static_cast<float>(static_cast<double>(ax) * cy - static_cast<double>(ay) * cx);
+        W = static_cast<float>(static_cast<double>(bx) * ay - static_cast<double>(by) * ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (p.sz * az) + V * (p.sz * bz) + W * (p.sz * cz);
+    t = T / det;
+    bu = V / det;
+    bv = W / det;
+    return true;
+}
+
+// planeIntersect, BasicShapes/Kernel.cpp:30-74 (calcTexCoord + parallelogram bounds)
+bool intersectQuad(const Ray& r, const Quad& q, float tNear, float tFar, float& tOut) {
+    const V3 delta = q.origin - r.o;
+    const float b = dot(delta, q.normal);
+    const float k = dot(r.d, q.normal);
+    const float t = b / k;
+    if (!(tNear < t && t < tFar)) return false;
+    const V3 pos = r.d * t - delta;
+    const V3 det3 = cross(q.u, q.v);
+    const float a0 = std::fabs(det3.x), a1 = std::fabs(det3.y), a2 = std::fabs(det3.z);
+    const int sel = (a0 >= a1 && a0 >= a2) ? 0 : ((a1 >= a0 && a1 >= a2) ? 1 : 2);
+    float invDet, x, y;
+    if (sel == 0) {
+        invDet = 1.0f / (q.u.y * q.v.z - q.v.y * q.u.z);
+        x = pos.y * q.v.z - q.v.y * pos.z;
+        y = q.u.y * pos.z - pos.y * q.u.z;
+    } else if (sel == 1) {
+        invDet = 1.0f / (q.u.z * q.v.x - q.v.z * q.u.x);
+        x = pos.z * q.v.x - q.v.z * pos.x;
+        y = q.u.z * pos.x - pos.z * q.u.x;
+    } else {
+        invDet = 1.0f / (q.u.x * q.v.y - q.v.x * q.u.y);
+        x = pos.x * q.v.y - q.v.x * pos.y;
+        y = q.u.x * pos.y - pos.x * q.u.y;
+    }
+    const float u = x * invDet, v = y * invDet;
+    if (!(0.0f <= u && u <= 1.0f && 0.0f <= v && v <= 1.0f)) return false;
+    tOut = t;
+    return true;
+}
+
+// Closest-hit rule: accept t in (tmin, best.t], ties broken by the smaller primitive index.
+inline void consider(Hit& best, float t, uint32_t prim, float u, float v, float tmin) {
+    if (t > tmin && (t < best.t || (t == best.t && prim < best.prim))) best = {t, prim, u, v};
+}
+
+bool boxHit(const BNode& n, const Ray& r, const RayPre& p, float tmin, float tmax) {
+    float t0 = tmin, t1 = tmax;
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    for (int k = 0; k < 3; ++k) {
+        float a = (n.lo[k] - o[k]) * p.inv[k], b = (n.hi[k] - o[k]) * p.inv[k];
+        if (a > b) std::swap(a, b);
+        t0 = std::fmax(t0, a);
+        t1 = std::fmin(t1, b * 1.0000004f);
+    }
+    return t0 <= t1;
+}
+
+// rtcIntersect1 with mask gsm|areaLight (Embree.cpp:267-355): closest triangle or light quad.
+Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brute, uint64_t* counts) {
+    Hit best{tmax, kMiss, 0.0f, 0.0f};
+    const RayPre p = precompute(r);
+    float t, u, v;
+    if (brute) {
+        for (uint32_t i = 0; i < s.tris.size(); ++i)
+            if (intersectTri(r, p, s.tris[i], t, u, v)) consider(best, t, i, u, v, tmin);
+    } else if (!s.nodes.empty()) {
+        uint32_t stack[128];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const BNode& n = s.nodes[stack[--sp]];
+            if (counts) ++counts[0];
+            if (!boxHit(n, r, p, tmin, best.t * 1.00001f)) continue;
+            if (n.count) {
+                for (uint32_t i = n.left; i < n.left + n.count; ++i) {
+                    const uint32_t ti = s.order[i];
+                    if (counts) ++counts[1];
+                    if (intersectTri(r, p, s.tris[ti], t, u, v)) consider(best, t, ti, u, v, tmin);
+                }
+            } else {
+                stack[sp++] = n.left + 1;
+                stack[sp++] = n.left;
+            }
+        }
+    }
+    // light quads after triangles; a quad must be strictly closer to win
+    for (uint32_t q = 0; q < s.quads.size(); ++q)
+        if (intersectQuad(r, s.quads[q], tmin, best.t, t)) best = {t, kQuadBit | q, 0.0f, 0.0f};
+    if (best.prim == kMiss) best.t = tmax;
+    return best;
+}
+
+// rtcOccluded1 with mask gsm only (Embree.cpp:357-391): lights never occlude.
+bool occluded(const Scene& s, const Ray& r, float tmin, float tmax) {
+    if (s.nodes.empty()) return false;
+    const RayPre p = precompute(r);
+    float t, u, v;
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const BNode& n = s.nodes[stack[--sp]];
+        if (!boxHit(n, r, p, tmin, tmax * 1.00001f)) continue;
+        if (n.count) {
+            for (uint32_t i = n.left; i < n.left + n.count; ++i)
+                if (intersectTri(r, p, s.tris[s.order[i]], t, u, v) && t > tmin && t < tmax) return true;
+        } else {
+            stack[sp++] = n.left + 1;
+            stack[sp++] = n.left;
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Sampling helpers (Kernel/Sampling.hpp:23-68)
+// ---------------------------------------------------------------------------------------------
+void concentricDisk(float u1, float u2, float& x, float& y) {  // Sampling.hpp:31-42
+    u1 = 2.0f * u1 - 1.0f;
+    u2 = 2.0f * u2 - 1.0f;
+    const float au1 = std::fabs(u1), au2 = std::fabs(u2);
+    if (std::fmin(au1, au2) < 1e-4f) {
+        x = y = 0.0f;
+        return;
+    }
+    const float rad = au1 > au2 ? u1 : u2;
+    const float theta = au1 > au2 ? (kPi / 4.0f) * (u2 / u1) : (kPi / 2.0f) - (kPi / 4.0f) * (u1 / u2);
+    x = std::cos(theta) * rad;
+    y = std::sin(theta) * rad;
+}
+V3 cosineHemisphere(float u1, float u2) {  // Sampling.hpp:44-50
+    float x, y;
+    concentricDisk(u1, u2, x, y);
+    const float z = std::sqrt(std::fmax(0.0f, 1.0f - (x * x + y * y)));
+    return {x, y, z};
+}
+
+// ---------------------------------------------------------------------------------------------
+// BxDFs (Surface/Classic/Kernel.cpp). Shading space: z = shading normal.
+// ---------------------------------------------------------------------------------------------
+struct BsdfSample {
+    V3 wi, f;
+    float pdf;
+};
+
+// GGX (Trowbridge-Reitz) isotropic distribution, Classic/Kernel.cpp:289-382, alpha = roughness^2.
+float ggxLambda(V3 w, float alpha) {  // Kernel.cpp:326-341
+    const float cos2 = w.z * w.z;
+    const float sin2 = std::fmax(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (std::isinf(tan2)) return 0.0f;
+    return (-1.0f + std::sqrt(1.0f + alpha * alpha * tan2)) * 0.5f;
+}
+float ggxD(V3 wh, float alpha) {  // Kernel.cpp:342-357
+    const float cos2 = wh.z * wh.z;
+    const float sin2 = std::fmax(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (std::isinf(tan2)) return 0.0f;
+    const float e = cos2 * (1.0f + tan2 / (alpha * alpha));
+    return kInvPi / (alpha * alpha * e * e);
+}
+// Visible-normal sampling (Heitz, "Sampling the GGX Distribution of Visible Normals", JCGT 2018).
+V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2) {
+    const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / std::sqrt(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    const V3 t2 = cross(vh, t1);
+    const float r = std::sqrt(u1);
+    const float phi = 2.0f * kPi * u2;
+    const float p1 = r * std::cos(phi);
+    float p2 = r * std::sin(phi);
+    const float sm = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - sm) * std::sqrt(std::fmax(0.0f, 1.0f - p1 * p1)) + sm * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::fmax(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return normalize(V3{alpha * nh.x, alpha * nh.y, std::fmax(0.0f, nh.z)});
+}
+V3 schlick(V3 f0, float c) {
+    const float m = 1.0f - std::fmin(1.0f, std::fmax(0.0f, c));
+    const float m5 = (m * m) * (m * m) * m;
+    return V3{f0.x + (1.0f - f0.x) * m5, f0.y + (1.0f - f0.y) * m5, f0.z + (1.0f - f0.z) * m5};
+}
+// MicrofacetReflection::evaluate, Kernel.cpp:398-404, with a Schlick conductor Fresnel.
+V3 conductorF(const Material& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+    const V3 wh = normalize(wo + wi);
+    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+    const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+    return schlick(m.albedo, dot(wi, wh)) * k;
+}
+float conductorPdf(const Material& m, V3 wo, V3 wi) {  // D(wh) G1(wo) / (4 cos(theta_o))
+    if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
+    const V3 wh = normalize(wo + wi);
+    return ggxD(wh, m.alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, m.alpha)));
+}
+
+// Reflection/transmission side test of the evaluate path, Classic/Kernel.cpp:220 and :211-215
+inline bool sameSide(V3 wo, V3 wi, V3 ng) { return dot(wo, ng) * dot(wi, ng) > 0.0f; }
+
+V3 bsdfEval(const Material& m, V3 wo, V3 wi, V3 ng) {
+    if (!sameSide(wo, wi, ng)) return {0.0f, 0.0f, 0.0f};
+    if (m.type == kMatLambert) return m.albedo * kInvPi;  // LambertianReflection, Kernel.cpp:81-92
+    return conductorF(m, wo, wi);
+}
+float bsdfPdf(const Material& m, V3 wo, V3 wi) {
+    if (m.type == kMatLambert) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
+    return conductorPdf(m, wo, wi);
+}
+// sample(), Kernel.cpp:180-202 with a single BxDF (count == 1): f comes from the BxDF itself.
+BsdfSample bsdfSample(const Material& m, V3 wo, float u1, float u2) {
+    BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
+    if (m.type == kMatLambert) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
+        s.wi = cosineHemisphere(u1, u2);
+        s.f = m.albedo * kInvPi;
+        s.pdf = s.wi.z > 0.0f ? std::fabs(s.wi.z) * kInvPi : 0.0f;
+        return s;
+    }
+    if (wo.z <= 0.0f) return s;  // MicrofacetReflection::sample, Kernel.cpp:413-433
+    const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+    const float c = dot(wo, wh);
+    if (c < 0.0f) return s;
+    s.wi = wh * (2.0f * c) - wo;  // reflect, Transform.hpp:405-408
+    if (s.wi.z <= 0.0f) return s;
+    s.f = conductorF(m, wo, s.wi);
+    s.pdf = conductorPdf(m, wo, s.wi);
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Integrator (Integrator/PathTracing/Kernel.cpp) and per-sample driver (Tracer/Embree/Kernel.cpp)
+// ---------------------------------------------------------------------------------------------
+enum : uint32_t { OR_COMPAT_D1 = 1, OR_COMPAT_D3 = 2, OR_COMPAT_D9 = 4, OR_COMPAT_D13 = 8 };
+
+struct Params {
+    uint32_t width, height, spp, maxDepth, minRR, flags;
+    uint32_t rectX, rectY, rectW, rectH;
+    uint32_t sppBegin, sppEnd;
+    float ndc[4];  // SensorNDCAffineTransform {ox, oy, sx, sy}
+};
+
+struct Sampler {  // PerSampleContext dimension counter, Embree/Kernel.cpp:37-45
+    uint64_t index;
+    uint32_t dim;
+    float next() { return sobolSample(index, dim++); }
+    void next2(bool d9, float& u1, float& u2) {
+        const float a = next(), b = next();
+        if (d9) u2 = a, u1 = b;
+        else u1 = a, u2 = b;
+    }
+};
+
+struct Frame {  // SurfaceIntersectionInfo, Protocol.hpp:47-66
+    V3 t, b, n;
+    V3 toLocal(V3 v) const { return {dot(t, v), dot(b, v), dot(n, v)}; }
+    V3 toWorld(V3 v) const {
+        return {t.x * v.x + b.x * v.y + n.x * v.z, t.y * v.x + b.y * v.y + n.y * v.z, t.z * v.x + b.z * v.y + n.z * v.z};
+    }
+};
+
+// Hit post-process: calcTriangleMeshSurface (Embree/Kernel.cpp:170-192) and planeSurface
+// (BasicShapes/Kernel.cpp:99-111). Returns the world normal facing the incoming ray.
+V3 hitNormal(const Scene& s, const Hit& h, V3 dir) {
+    if (h.prim & kQuadBit) {
+        const Quad& q = s.quads[h.prim & ~kQuadBit];
+        return dot(dir, q.normal) > 0.0f ? -q.normal : q.normal;  // face Front -> -normal
+    }
+    const Tri& tri = s.tris[h.prim];
+    const V3 ng = normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0));
+    return dot(dir, ng) < 0.0f ? ng : -ng;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
+}
+Frame shadingFrame(V3 n, bool d3) {
+    Frame f;
+    f.n = n;
+    const V3 ex{1.0f, 0.0f, 0.0f}, ey{0.0f, 1.0f, 0.0f};
+    f.t = std::fabs(dot(n, ex)) < std::fabs(dot(n, ey)) ? cross(n, ex) : cross(n, ey);
+    if (!d3) f.t = normalize(f.t);
+    f.b = cross(n, f.t);
+    return f;
+}
+
+// uniformSelect (Light/BasicLight/Kernel.cpp:91-98) over {environment, quad lights...}
+struct LightSel {
+    uint32_t light;  // 0 = environment, 1 + q = quad q
+    float pdf;
+};
+LightSel selectLight(const Scene& s, float u) {
+    const uint32_t count = 1 + static_cast<uint32_t>(s.quads.size());
+    const uint32_t sel = std::min(static_cast<uint32_t>(std::floor(u * static_cast<float>(count))), count - 1);
+    return {sel, 1.0f / static_cast<float>(count)};
+}
+
+struct LightSample {
+    V3 dir, rad;
+    float dist, pdf;
+};
+// diffuseSample (AreaLight/Kernel.cpp:24-41) over planeSample (BasicShapes/Kernel.cpp:113-134)
+// for a one-primitive plane; SESample (BasicLight/Kernel.cpp:63-76) for the environment.
+LightSample sampleLight(const Scene& s, uint32_t light, V3 hit, float u1, float u2, bool d1) {
+    LightSample ls{{0, 0, 0}, {0, 0, 0}, 0.0f, 0.0f};
+    if (light == 0) {
+        const float theta = u1 * kPi, phi = u2 * kPi;
+        const float st = std::sin(theta);
+        ls.dir = {st * std::cos(phi), st * std::sin(phi), std::cos(theta)};
+        ls.rad = s.env;
+        ls.pdf = st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
+        ls.dist = 1e5f;
+        return ls;
+    }
+    const Quad& q = s.quads[light - 1];
+    // select() over a one-entry CDF {1}: spec remaps with the lower bound (u unchanged),
+    // D1 remaps with the upper bound: u - 1.
+    const float uu = d1 ? (u1 - 1.0f) / 1.0f : u1;
+    const V3 src = q.origin + q.u * uu + q.v * u2;
+    V3 n = q.normal;
+    if (dot(hit - src, n) <= 0.0f) n = -n;
+    const float pdfA = 1.0f / q.area;
+    const V3 delta = src - hit;
+    if (dot(delta, n) >= 0.0f) return ls;
+    const float dis2 = dot(delta, delta);
+    ls.rad = q.radiance;
+    ls.dist = std::sqrt(dis2);
+    ls.dir = {delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
+    ls.pdf = pdfA * dis2 / std::fabs(dot(n, ls.dir));
+    return ls;
+}
+
+inline float powerHeuristic(float a, float b) {  // PathTracing/Kernel.cpp:27-31
+    const float sa = a * a, sb = b * b;
+    return sa / (sa + sb);
+}
+
+struct PathStats {
+    uint64_t ext = 0, shadow = 0, mis = 0;
+};
+
+// trace() + multipleImportanceSampling(), PathTracing/Kernel.cpp:23-193
+V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& st) {
+    const bool d1 = p.flags & OR_COMPAT_D1, d3 = p.flags & OR_COMPAT_D3, d9 = p.flags & OR_COMPAT_D9,
+               d13 = p.flags & OR_COMPAT_D13;
+    V3 pf{1.0f, 1.0f, 1.0f}, L{0.0f, 0.0f, 0.0f};
+    bool specular = true;
+    uint32_t depth = 0;
+    while (true) {
+        const Hit h = traceClosest(s, ray, 1e-3f, 1e5f, false, nullptr);
+        ++st.ext;
+        const bool isLight = h.prim != kMiss && (h.prim & kQuadBit);
+        if (specular && (isLight || h.prim == kMiss)) {  // :119-140
+            const V3 rad = isLight ? s.quads[h.prim & ~kQuadBit].radiance : s.env;  // two-sided emission
+            L = L + mul(pf, rad);
+        }
+        if (isLight) {  // :142-146 pass through the light
+            if (d13) --depth;
+            ray.o = ray.o + ray.d * (h.t + 0.001f);
+            continue;
+        }
+        if (h.prim == kMiss || depth >= p.maxDepth) break;  // :148
+
+        const Material& mat = s.mats[s.objMat[s.triObj[h.prim]]];
+        const V3 n = hitNormal(s, h, ray.d);
+        const Frame fr = shadingFrame(n, d3);
+        const V3 wo = fr.toLocal(-ray.d);
+        const V3 ng = fr.toLocal(n);
+        const V3 hit = ray.o + ray.d * h.t;
+
+        // multipleImportanceSampling, :23-107
+        {
+            const LightSel sel = selectLight(s, smp.next());
+            V3 partA{0, 0, 0}, partB{0, 0, 0};
+            float u1, u2;
+            smp.next2(d9, u1, u2);
+            const LightSample ls = sampleLight(s, sel.light, hit, u1, u2, d1);
+            if (ls.pdf > 0.0f && valid(ls.rad)) {
+                const V3 wi = fr.toLocal(ls.dir);
+                const V3 f = bsdfEval(mat, wo, wi, ng);
+                if (valid(f)) {
+                    ++st.shadow;
+                    if (!occluded(s, Ray{hit, ls.dir}, std::fmin(1e-3f, ls.dist), ls.dist)) {
+                        float w = std::fabs(wi.z) / ls.pdf;
+                        w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
+                        partA = mul(f, ls.rad) * w;
+                    }
+                }
+            }
+            smp.next2(d9, u1, u2);
+            const BsdfSample bs = bsdfSample(mat, wo, u1, u2);
+            if (bs.pdf > 0.0f && valid(bs.f)) {
+                float weight = std::fabs(bs.wi.z) / bs.pdf;
+                const V3 dir = fr.toWorld(bs.wi);
+                const Hit lh = traceClosest(s, Ray{hit, dir}, 1e-3f, 1e5f, false, nullptr);
+                ++st.mis;
+                V3 rad{0, 0, 0};
+                float lpdf = 0.0f;
+                if (lh.prim == kMiss) {  // environment: SEEvaluate / SEPdf (BasicLight/Kernel.cpp:77-89)
+                    rad = s.env;
+                    const float theta = std::acos(std::fmin(1.0f, std::fmax(-1.0f, dir.z)));
+                    const float st2 = std::sin(theta);
+                    lpdf = st2 == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st2);
+                } else if ((lh.prim & kQuadBit) && sel.light == 1 + (lh.prim & ~kQuadBit)) {
+                    const Quad& q = s.quads[lh.prim & ~kQuadBit];
+                    const V3 qn = hitNormal(s, lh, dir);
+                    rad = dot(qn, dir) < 0.0f ? q.radiance : V3{0, 0, 0};  // diffuseEvaluate
+                    lpdf = lh.t * lh.t / (std::fabs(dot(qn, dir)) * q.area);  // calcGeometrySamplePdf
+                }
+                if (valid(rad) && lpdf > 0.0f) {
+                    weight = weight * powerHeuristic(bs.pdf, lpdf);
+                    partB = mul(rad, bs.f) * weight;
+                }
+            }
+            L = L + mul(pf, (partA + partB) * (1.0f / sel.pdf));
+        }
+
+        float u1, u2;
+        smp.next2(d9, u1, u2);
+        const BsdfSample ss = bsdfSample(mat, wo, u1, u2);  // :164-169
+        if (!(ss.pdf > 0.0f && valid(ss.f))) break;
+        pf = mul(pf, ss.f * (std::fabs(ss.wi.z) / ss.pdf));
+        if (depth > p.minRR) {  // :177-182
+            const float q = std::fmax(0.05f, 1.0f - maxc(pf));
+            if (smp.next() < q) break;
+            pf = pf * (1.0f / (1.0f - q));
+        }
+        ray.d = fr.toWorld(ss.wi);
+        ray.o = hit + n * 0.001f;  // reflection only: both BxDFs reflect
+        specular = false;
+        ++depth;
+    }
+    return L;
+}
+
+// rayGen, Sensor/PerspectiveCamera/Kernel.cpp:22-48 (base = camera position)
+Ray cameraRay(const Camera& c, float ndcX, float ndcY, float u1, float u2) {
+    const V3 forward = normalize(c.lookAt - c.base);
+    const V3 right = cross(forward, c.up);
+    const V3 up = cross(right, forward);
+    const V3 filmHit = c.base + right * (c.sizeX * (0.5f - ndcX)) + up * (c.sizeY * (ndcY - 0.5f));
+    const float focalDistance = dot(c.lookAt - c.base, forward);
+    const float filmDistance = 1.0f / (1.0f / c.focal - 1.0f / focalDistance);
+    const V3 lensCenter = c.base + forward * filmDistance;
+    const float ang = u1 * (2.0f * kPi);  // sampleUniformDisk, Sampling.hpp:23-29
+    const float rad = std::sqrt(std::fmax(0.0f, u2));
+    const float lx = rad * std::cos(ang), ly = rad * std::sin(ang);
+    const V3 lensHit = lensCenter + right * (c.aperture * lx) + up * (c.aperture * ly);
+    const V3 dir = lensCenter - filmHit;
+    const V3 focusHit = lensCenter + dir * (focalDistance / dot(dir, forward));
+    return Ray{lensHit, normalize(focusHit - lensHit)};
+}
+
+// One (pixel, sample): piperMain, Tracer/Embree/Kernel.cpp:47-100.
+V3 renderSample(const Scene& s, const Params& p, uint32_t m, uint32_t px, uint32_t py, uint32_t si, float& fx,
+                float& fy, PathStats& st) {
+    Sampler smp{intervalToIndex(m, si, px, py), 5};
+    const float res = static_cast<float>(1u << m);
+    // sobolStart, Sampler/ScrambledSobolSampler/Kernel.cpp:52-62
+    fx = std::fmin(std::fmax(sobolSample(smp.index, 0) * res - static_cast<float>(px), 0.0f), 1.0f);
+    fy = std::fmin(std::fmax(sobolSample(smp.index, 1) * res - static_cast<float>(py), 0.0f), 1.0f);
+    fx += static_cast<float>(px);
+    fy += static_cast<float>(py);
+    const float ndcX = p.ndc[0] + p.ndc[2] * fx / static_cast<float>(p.width);
+    const float ndcY = p.ndc[1] + p.ndc[3] * fy / static_cast<float>(p.height);
+    float u1 = sobolSample(smp.index, 3), u2 = sobolSample(smp.index, 4);
+    const Ray ray = cameraRay(s.cam, ndcX, ndcY, u1, u2);
+    return tracePath(s, p, smp, ray, st);
+}
+
+// accumulate with the box filter (RenderDriver/FixedSampler/Kernel.cpp:30-58)
+template <typename F>
+void splat(F* rgbw, uint32_t w, uint32_t h, float fx, float fy, V3 L, std::vector<std::atomic<uint32_t>>* locks) {
+    const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = static_cast<uint32_t>(bx + (k & 1)), y = static_cast<uint32_t>(by + (k >> 1));
+        if (!(x < w && y < h)) continue;
+        F* px = rgbw + 4 * (static_cast<size_t>(y) * w + x);
+        std::atomic<uint32_t>& lk = (*locks)[(static_cast<size_t>(y) * w + x) & (locks->size() - 1)];
+        while (lk.exchange(1, std::memory_order_acquire)) {
+        }
+        px[0] += L.x, px[1] += L.y, px[2] += L.z, px[3] += 1.0f;
+        lk.store(0, std::memory_order_release);
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI for ctypes (tests / bench cpu_baseline only)
+// =============================================================================================
+extern "C" {
+
+void or_sobol_matrices(uint32_t* out) { std::memcpy(out, g_sobol, sizeof(g_sobol)); }
+float or_sobol_sample(uint64_t index, uint32_t dim) { return sobolSample(index, dim); }
+uint64_t or_interval_to_index(uint32_t m, uint32_t sample, uint32_t px, uint32_t py) {
+    return intervalToIndex(m, sample, px, py);
+}
+
+// Scene arrays:
+//   verts[3*nv]; idx[3*nt]; triObj[nt]; objMat[nobj];
+//   mats[8*nmat] = {type, r, g, b, alpha, 0, 0, 0};
+//   quads[12*nq] = {origin xyz, U xyz, V xyz, radiance rgb};
+//   cam[14] = {base xyz, lookAt xyz, up xyz, sizeX, sizeY, focal, aperture, 0}; env[3]
+void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* triObj,
+                      const uint32_t* objMat, uint32_t nobj, const float* mats, uint32_t nmat, const float* quads,
+                      uint32_t nq, const float* cam, const float* env) {
+    (void)nv;
+    (void)nmat;
+    auto* s = new Scene;
+    s->tris.resize(nt);
+    for (uint32_t i = 0; i < nt; ++i) {
+        auto vtx = [&](uint32_t k) {
+            const uint32_t j = idx[3 * i + k];
+            return V3{verts[3 * j], verts[3 * j + 1], verts[3 * j + 2]};
+        };
+        s->tris[i] = {vtx(0), vtx(1), vtx(2)};
+    }
+    s->triObj.assign(triObj, triObj + nt);
+    s->objMat.assign(objMat, objMat + nobj);
+    for (uint32_t i = 0; i < nmat; ++i) {
+        const float* m = mats + 8 * i;
+        s->mats.push_back({static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]});
+    }
+    for (uint32_t i = 0; i < nq; ++i) {
+        const float* q = quads + 12 * i;
+        Quad Q;
+        Q.origin = {q[0], q[1], q[2]};
+        Q.u = {q[3], q[4], q[5]};
+        Q.v = {q[6], q[7], q[8]};
+        Q.radiance = {q[9], q[10], q[11]};
+        const V3 det = cross(Q.u, Q.v);  // BasicShapes.cpp:76-90
+        Q.area = length(det);
+        Q.normal = {det.x / Q.area, det.y / Q.area, det.z / Q.area};
+        s->quads.push_back(Q);
+    }
+    s->cam = {{cam[0], cam[1], cam[2]}, {cam[3], cam[4], cam[5]}, {cam[6], cam[7], cam[8]},
+              cam[9], cam[10], cam[11], cam[12]};
+    s->env = {env[0], env[1], env[2]};
+    buildBVH(*s);
+    return s;
+}
+void or_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
+
+// rays[8*n] = {o xyz, d xyz, tmin, tmax}; outputs t, prim (kMiss / kQuadBit|q / tri), u, v.
+// brute != 0 tests every triangle instead of using the BVH. counts (optional) gets
+// {nodes visited, triangles tested} summed over the rays.
+void or_trace(void* sp, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv, int brute,
+              uint64_t* counts) {
+    const Scene& s = *static_cast<Scene*>(sp);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        const Hit h = traceClosest(s, Ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}}, r[6], r[7], brute != 0, counts);
+        t[i] = h.t, prim[i] = h.prim, uv[2 * i] = h.u, uv[2 * i + 1] = h.v;
+    }
+}
+void or_occluded(void* sp, const float* rays, uint32_t n, uint8_t* out) {
+    const Scene& s = *static_cast<Scene*>(sp);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        out[i] = occluded(s, Ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}}, r[6], r[7]);
+    }
+}
+
+// Camera rays exactly as the renderer makes them; rays[8*n] as in or_trace (tmin 1e-3, tmax 1e5).
+// pix[3*n] = {px, py, sample}.
+void or_camera_rays(void* sp, const uint32_t* iparams /*width,height,spp*/, const float* ndc, const uint32_t* pix,
+                    uint32_t n, float* rays, float* film) {
+    const Scene& s = *static_cast<Scene*>(sp);
+    const uint32_t w = iparams[0], h = iparams[1];
+    uint32_t m = 0;
+    while ((1u << m) < std::max(w, h)) ++m;
+    const float res = static_cast<float>(1u << m);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t px = pix[3 * i], py = pix[3 * i + 1], si = pix[3 * i + 2];
+        const uint64_t idx = intervalToIndex(m, si, px, py);
+        float fx = std::fmin(std::fmax(sobolSample(idx, 0) * res - static_cast<float>(px), 0.0f), 1.0f);
+        float fy = std::fmin(std::fmax(sobolSample(idx, 1) * res - static_cast<float>(py), 0.0f), 1.0f);
+        fx += static_cast<float>(px);
+        fy += static_cast<float>(py);
+        const float ndcX = ndc[0] + ndc[2] * fx / static_cast<float>(w);
+        const float ndcY = ndc[1] + ndc[3] * fy / static_cast<float>(h);
+        const Ray r = cameraRay(s.cam, ndcX, ndcY, sobolSample(idx, 3), sobolSample(idx, 4));
+        float* o = rays + 8 * i;
+        o[0] = r.o.x, o[1] = r.o.y, o[2] = r.o.z, o[3] = r.d.x, o[4] = r.d.y, o[5] = r.d.z, o[6] = 1e-3f, o[7] = 1e5f;
+        if (film) film[2 * i] = fx, film[2 * i + 1] = fy;
+    }
+}
+
+// Render a rect over samples [sppBegin, sppEnd).
+//   iparams = {width, height, spp, maxDepth, minRR, flags, rectX, rectY, rectW, rectH, sppBegin, sppEnd}
+//   ndc = {ox, oy, sx, sy}; film = RGBW double [height*width*4] (accumulated into, not cleared);
+//   pathL (optional) = per-path radiance [rectH*rectW*(sppEnd-sppBegin)*3], path order
+//   ((py-rectY)*rectW + (px-rectX))*(sppEnd-sppBegin) + (s-sppBegin);
+//   rayCounts (optional) = {extension, shadow, mis} ray totals.
+void or_render(void* sp, const uint32_t* iparams, const float* ndc, double* film, float* pathL, int threads,
+               uint64_t* rayCounts) {
+    const Scene& s = *static_cast<Scene*>(sp);
+    Params p;
+    p.width = iparams[0], p.height = iparams[1], p.spp = iparams[2], p.maxDepth = iparams[3], p.minRR = iparams[4];
+    p.flags = iparams[5], p.rectX = iparams[6], p.rectY = iparams[7], p.rectW = iparams[8], p.rectH = iparams[9];
+    p.sppBegin = iparams[10], p.sppEnd = iparams[11];
+    std::memcpy(p.ndc, ndc, sizeof(p.ndc));
+    uint32_t m = 0;
+    while ((1u << m) < std::max(p.width, p.height)) ++m;  // ScrambledSobolSampler.cpp:66-72
+    const uint32_t ns = p.sppEnd - p.sppBegin;
+    // 8x8 tiles handed to worker threads, samples looped serially per pixel
+    // (FixedSampler.cpp:73-94, Parallel.cpp:475-491)
+    const uint32_t tx = (p.rectW + 7) / 8, ty = (p.rectH + 7) / 8;
+    std::atomic<uint32_t> next{0};
+    std::vector<std::atomic<uint32_t>> locks(1 << 16);
+    for (auto& l : locks) l.store(0);
+    std::vector<PathStats> stats(std::max(threads, 1));
+    auto worker = [&](int tid) {
+        PathStats& st = stats[tid];
+        for (uint32_t tile; (tile = next.fetch_add(1)) < tx * ty;) {
+            const uint32_t bx = tile % tx, by = tile / tx;
+            for (uint32_t yy = by * 8; yy < std::min(p.rectH, by * 8 + 8); ++yy)
+                for (uint32_t xx = bx * 8; xx < std::min(p.rectW, bx * 8 + 8); ++xx)
+                    for (uint32_t si = p.sppBegin; si < p.sppEnd; ++si) {
+                        float fx, fy;
+                        const uint32_t px = p.rectX + xx, py = p.rectY + yy;
+                        const V3 L = renderSample(s, p, m, px, py, si, fx, fy, st);
+                        if (pathL) {
+                            float* o = pathL + 3 * ((static_cast<size_t>(yy) * p.rectW + xx) * ns + (si - p.sppBegin));
+                            o[0] = L.x, o[1] = L.y, o[2] = L.z;
+                        }
+                        if (film) splat(film, p.width, p.height, fx, fy, L, &locks);
+                    }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < threads; ++i) pool.emplace_back(worker, i);
+    worker(0);
+    for (auto& t : pool) t.join();
+    if (rayCounts) {
+        rayCounts[0] = rayCounts[1] = rayCounts[2] = 0;
+        for (auto& st : stats) rayCounts[0] += st.ext, rayCounts[1] += st.shadow, rayCounts[2] += st.mis;
+    }
+}
+
+// Furnace-style and analytic helpers used by the oracle's own pinning tests.
+void or_bsdf_eval(const float* mat, const float* wo, const float* wi, float* f, float* pdf) {
+    const Material m{static_cast<uint32_t>(mat[0]), {mat[1], mat[2], mat[3]}, mat[4]};
+    const V3 o{wo[0], wo[1], wo[2]}, i{wi[0], wi[1], wi[2]};
+    const V3 r = bsdfEval(m, o, i, V3{0, 0, 1});
+    f[0] = r.x, f[1] = r.y, f[2] = r.z;
+    *pdf = bsdfPdf(m, o, i);
+}
+void or_bsdf_sample(const float* mat, const float* wo, float u1, float u2, float* wi, float* f, float* pdf) {
+    const Material m{static_cast<uint32_t>(mat[0]), {mat[1], mat[2], mat[3]}, mat[4]};
+    const BsdfSample s = bsdfSample(m, V3{wo[0], wo[1], wo[2]}, u1, u2);
+    wi[0] = s.wi.x, wi[1] = s.wi.y, wi[2] = s.wi.z;
+    f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
+    *pdf = s.pdf;
+}
+float or_ggx_d(float cosTheta, float alpha) {
+    const float s = std::sqrt(std::fmax(0.0f, 1.0f - cosTheta * cosTheta));
+    return ggxD(V3{s, 0.0f, cosTheta}, alpha);
+}
+}

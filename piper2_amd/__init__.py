@@ -1,0 +1,326 @@
+"""piper2_amd: MI355X (gfx950) path-tracing hot path of Piper2, behind a C ABI.
+
+The product is libpiper_hip.so (include/piper_hip.h). This module is a thin ctypes layer over it
+for tests and benchmarks; it fails loudly when the library has not been built, and it never falls
+back to any CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpiper_hip.so")
+
+PG_OK = 0
+PG_ERR_INVALID = -1
+PG_ERR_DEVICE = -2
+PG_ERR_OOM = -3
+PG_ERR_UNSUPPORTED = -4
+
+PG_COMPAT_D1 = 0x1
+PG_COMPAT_D3 = 0x2
+PG_COMPAT_D9 = 0x4
+PG_COMPAT_D13 = 0x8
+PG_FILM_FP64 = 0x100
+PG_COUNT_TRAVERSAL = 0x200
+
+PG_MAT_LAMBERT = 0
+PG_MAT_CONDUCTOR = 1
+PG_HIT_MISS = 0xFFFFFFFF
+PG_HIT_QUAD = 0x80000000
+
+F3 = ctypes.c_float * 3
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("albedo", F3), ("alpha", ctypes.c_float), ("reserved", F3)]
+
+
+class Quad(ctypes.Structure):
+    _fields_ = [("origin", F3), ("u", F3), ("v", F3), ("radiance", F3)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", F3), ("look_at", F3), ("up", F3), ("sensor_size", ctypes.c_float * 2),
+                ("focal_length", ctypes.c_float), ("aperture_radius", ctypes.c_float)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32),
+        ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32),
+        ("tri_object", ctypes.c_void_p), ("object_material", ctypes.c_void_p), ("num_objects", ctypes.c_uint32),
+        ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
+        ("quads", ctypes.c_void_p), ("num_quads", ctypes.c_uint32),
+        ("camera", Camera), ("env_radiance", F3),
+    ]
+
+
+class BvhParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32), ("sah_bins", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+        ("rect", ctypes.c_uint32 * 4), ("spp_begin", ctypes.c_uint32), ("spp_end", ctypes.c_uint32),
+        ("max_depth", ctypes.c_uint32), ("min_rr", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+        ("ndc", ctypes.c_float * 4), ("wavefront_paths", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("paths", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+        ("mis_rays", ctypes.c_uint64), ("extend_launches", ctypes.c_uint64),
+        ("ms_total", ctypes.c_double), ("ms_raygen", ctypes.c_double), ("ms_extend", ctypes.c_double),
+        ("ms_shade", ctypes.c_double), ("ms_shadow", ctypes.c_double), ("ms_mis", ctypes.c_double),
+        ("ms_film", ctypes.c_double), ("nodes_visited", ctypes.c_uint64), ("tris_tested", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# Every symbol the C ABI declares, with (restype, argtypes).
+_VP = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+SIGNATURES = {
+    "pg_device_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_VP)]),
+    "pg_device_destroy": (None, [_VP]),
+    "pg_device_platform": (ctypes.c_char_p, [_VP]),
+    "pg_last_error": (ctypes.c_char_p, []),
+    "pg_scene_create": (ctypes.c_int, [_VP, ctypes.POINTER(SceneDesc), ctypes.POINTER(_VP)]),
+    "pg_scene_build_bvh": (ctypes.c_int, [_VP, ctypes.POINTER(BvhParams)]),
+    "pg_scene_destroy": (None, [_VP]),
+    "pg_scene_bvh_info": (ctypes.c_int, [_VP, _VP]),
+    "pg_scene_bvh_download": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]),
+    "pg_film_create": (ctypes.c_int, [_VP, _U32, _U32, ctypes.POINTER(_VP)]),
+    "pg_film_clear": (ctypes.c_int, [_VP]),
+    "pg_film_destroy": (None, [_VP]),
+    "pg_film_download": (ctypes.c_int, [_VP, _VP, _VP]),
+    "pg_render": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
+    "pg_render_paths": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _VP, ctypes.POINTER(Stats)]),
+    "pg_trace": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
+    "pg_occluded": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "pg_camera_rays": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _U32, _VP, _VP]),
+    "pg_sobol": (ctypes.c_int, [_VP, _U32, _U32, _VP, _U32, _VP, _VP]),
+    "pg_sobol_matrices": (ctypes.c_int, [_VP]),
+    "pg_debug_inject_fault": (ctypes.c_int, [_VP]),
+}
+
+_lib = None
+
+
+class PgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"piper_hip error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpiper_hip.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != PG_OK:
+        raise PgError(code, lib().pg_last_error().decode())
+    return code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        self.h = ctypes.c_void_p()
+        check(lib().pg_device_create(ordinal, ctypes.byref(self.h)))
+
+    @property
+    def platform(self):
+        return lib().pg_device_platform(self.h).decode()
+
+    def close(self):
+        if self.h:
+            lib().pg_device_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sobol(self, m, dims, pix):
+        pix = np.ascontiguousarray(pix, dtype=np.uint32).reshape(-1, 3)
+        n = pix.shape[0]
+        out = np.zeros((n, dims), dtype=np.float32)
+        idx = np.zeros(n, dtype=np.uint64)
+        check(lib().pg_sobol(self.h, m, dims, _ptr(pix), n, _ptr(out), _ptr(idx)))
+        return out, idx
+
+    def inject_fault(self):
+        check(lib().pg_debug_inject_fault(self.h))
+
+
+def sobol_matrices():
+    out = np.zeros(1024 * 52, dtype=np.uint32)
+    check(lib().pg_sobol_matrices(_ptr(out)))
+    return out.reshape(1024, 52)
+
+
+class Scene:
+    """A scene on the device (see piper2_amd.scenes for the seeded generators)."""
+
+    def __init__(self, device, sc, bvh_width=8, leaf_size=4, bins=32):
+        self.device = device
+        self._keep = sc
+        desc = scene_desc(sc)
+        self.h = ctypes.c_void_p()
+        check(lib().pg_scene_create(device.h, ctypes.byref(desc), ctypes.byref(self.h)))
+        if (bvh_width, leaf_size, bins) != (8, 4, 32):
+            self.build_bvh(bvh_width, leaf_size, bins)
+
+    def build_bvh(self, width=8, leaf_size=4, bins=32):
+        p = BvhParams(width, leaf_size, bins, 0)
+        check(lib().pg_scene_build_bvh(self.h, ctypes.byref(p)))
+
+    def bvh_info(self):
+        info = np.zeros(6, dtype=np.uint64)
+        check(lib().pg_scene_bvh_info(self.h, _ptr(info)))
+        keys = ["nodes", "leaves", "max_depth", "width", "node_bytes", "tri_bytes"]
+        return {k: int(v) for k, v in zip(keys, info)}
+
+    def bvh_download(self):
+        info = self.bvh_info()
+        nodes = np.zeros(info["nodes"] * info["node_bytes"] // 4, dtype=np.float32)
+        tris = np.zeros(self._keep["indices"].shape[0] * 12, dtype=np.float32)
+        check(lib().pg_scene_bvh_download(self.h, _ptr(nodes), nodes.nbytes, _ptr(tris), tris.nbytes))
+        return nodes, tris
+
+    def close(self):
+        if self.h:
+            lib().pg_scene_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.uint32)
+        uv = np.zeros((n, 2), np.float32)
+        check(lib().pg_trace(self.h, _ptr(rays), n, _ptr(t), _ptr(prim), _ptr(uv)))
+        return t, prim, uv
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint8)
+        check(lib().pg_occluded(self.h, _ptr(rays), rays.shape[0], _ptr(out)))
+        return out
+
+    def camera_rays(self, params, pix):
+        pix = np.ascontiguousarray(pix, dtype=np.uint32).reshape(-1, 3)
+        n = pix.shape[0]
+        rays = np.zeros((n, 8), np.float32)
+        film = np.zeros((n, 2), np.float32)
+        check(lib().pg_camera_rays(self.h, ctypes.byref(params), _ptr(pix), n, _ptr(rays), _ptr(film)))
+        return rays, film
+
+    def render(self, params, film=None, path_l=False):
+        st = Stats()
+        if path_l:
+            n = params.rect[2] * params.rect[3] * (params.spp_end - params.spp_begin)
+            out = np.zeros((n, 3), np.float32)
+            check(lib().pg_render_paths(self.h, ctypes.byref(params), film.h if film else None, _ptr(out),
+                                        ctypes.byref(st)))
+            return st.as_dict(), out
+        check(lib().pg_render(self.h, ctypes.byref(params), film.h if film else None, ctypes.byref(st)))
+        return st.as_dict()
+
+
+class Film:
+    def __init__(self, device, width, height):
+        self.width, self.height = width, height
+        self.h = ctypes.c_void_p()
+        check(lib().pg_film_create(device.h, width, height, ctypes.byref(self.h)))
+
+    def clear(self):
+        check(lib().pg_film_clear(self.h))
+
+    def download(self, fp64=False):
+        if fp64:
+            out = np.zeros((self.height, self.width, 4), np.float64)
+            check(lib().pg_film_download(self.h, None, _ptr(out)))
+        else:
+            out = np.zeros((self.height, self.width, 4), np.float32)
+            check(lib().pg_film_download(self.h, _ptr(out), None))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().pg_film_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_desc(sc):
+    """Build a SceneDesc that points into the numpy arrays of a generated scene dict."""
+    for k in ("vertices", "indices", "tri_object", "object_material", "materials", "quads"):
+        assert sc[k].flags["C_CONTIGUOUS"], k
+    mats = sc["materials"]  # (n, 8) float32: type, r, g, b, alpha, 0, 0, 0
+    mat_arr = (Material * len(mats))()
+    for i, m in enumerate(mats):
+        mat_arr[i] = Material(int(m[0]), F3(*m[1:4]), float(m[4]), F3(0, 0, 0))
+    quads = sc["quads"]  # (n, 12)
+    quad_arr = (Quad * max(len(quads), 1))()
+    for i, q in enumerate(quads):
+        quad_arr[i] = Quad(F3(*q[0:3]), F3(*q[3:6]), F3(*q[6:9]), F3(*q[9:12]))
+    sc["_mat_arr"], sc["_quad_arr"] = mat_arr, quad_arr
+    c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
+    cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))
+    return SceneDesc(
+        _ptr(sc["vertices"]), sc["vertices"].shape[0], _ptr(sc["indices"]), sc["indices"].shape[0],
+        _ptr(sc["tri_object"]), _ptr(sc["object_material"]), sc["object_material"].shape[0],
+        ctypes.cast(mat_arr, ctypes.c_void_p), len(mats), ctypes.cast(quad_arr, ctypes.c_void_p), len(quads),
+        cam, F3(*sc["env"]))
+
+
+def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0):
+    """RenderParams for a generated scene: film size, depth and RR from its config."""
+    w, h = sc["width"], sc["height"]
+    spp = spp or sc["spp"]
+    rect = rect or (0, 0, w, h)
+    s0, s1 = spp_range or (0, spp)
+    p = RenderParams()
+    p.width, p.height, p.spp = w, h, spp
+    p.rect[:] = list(rect)
+    p.spp_begin, p.spp_end = s0, s1
+    p.max_depth, p.min_rr, p.flags = sc["max_depth"], sc["min_rr"], flags
+    p.ndc[:] = list(sc["ndc"])
+    p.wavefront_paths = wavefront_paths
+    return p

@@ -1,0 +1,667 @@
+// C ABI of libpiper_hip.so: devices, scenes, films, renders and parity entry points.
+// Declarations and the reference interfaces they replace: include/piper_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/piper_hip.h"
+#include "pg_bvh.hpp"
+#include "pg_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct PgError : std::runtime_error {
+    int code;
+    PgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PG_HIP(call)                                                                                  \
+    do {                                                                                              \
+        const hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                         \
+            throw PgError(e_ == hipErrorOutOfMemory ? PG_ERR_OOM : PG_ERR_DEVICE,                     \
+                          std::string(#call) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PG_OK;
+    } catch (const PgError& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return PG_ERR_INVALID;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return PG_ERR_DEVICE;
+    }
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw PgError(PG_ERR_INVALID, msg);
+}
+
+// Device buffer owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        PG_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+void upload(DevBuf& b, const void* host, size_t n, hipStream_t st) {
+    b.reserve(std::max<size_t>(n, 16));
+    if (n) PG_HIP(hipMemcpyAsync(b.p, host, n, hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
+struct pg_device {
+    int ordinal = 0;
+    int numCUs = 256;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    bool injectFault = false;
+    std::string platform;
+    DevBuf sobol, sobolInv, sobolDelta;
+    // render workspace
+    uint32_t capacity = 0;
+    DevBuf o, d, thr, L, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
+    size_t spillBytes = 0;
+    std::vector<hipEvent_t> events;
+    size_t eventsUsed = 0;
+
+    hipEvent_t event() {
+        if (eventsUsed == events.size()) {
+            hipEvent_t e;
+            PG_HIP(hipEventCreate(&e));
+            events.push_back(e);
+        }
+        return events[eventsUsed++];
+    }
+    void checkFault() {
+        if (injectFault) {
+            injectFault = false;
+            throw PgError(PG_ERR_DEVICE, "injected device fault");
+        }
+    }
+};
+
+struct pg_scene {
+    pg_device* dev = nullptr;
+    std::vector<float> triVerts;   // 9 floats per triangle, world space
+    std::vector<uint32_t> triMat;  // material per original triangle
+    pg::WideBvh bvh;
+    pg::DevScene ds{};
+    DevBuf tris, triMatLeaf, nodes, mats, quads;
+    uint32_t spillPerLane = 1;
+};
+
+struct pg_film {
+    pg_device* dev = nullptr;
+    uint32_t width = 0, height = 0;
+    DevBuf rgbw32, rgbw64;
+};
+
+namespace {
+
+void buildBvh(pg_scene* s, const pg_bvh_params* params) {
+    pg::BvhParams bp;
+    if (params) {
+        bp.width = params->width;
+        bp.leafSize = params->leaf_size;
+        bp.bins = params->sah_bins;
+    }
+    const uint32_t nt = static_cast<uint32_t>(s->triMat.size());
+    pg::buildWideBvh(s->triVerts.data(), nt, bp, s->bvh);
+    hipStream_t st = s->dev->stream;
+    std::vector<uint32_t> leafMat(std::max<uint32_t>(nt, 1));
+    for (uint32_t i = 0; i < nt; ++i) leafMat[i] = s->triMat[s->bvh.order[i]];
+    upload(s->tris, s->bvh.tris.data(), s->bvh.tris.size() * 4, st);
+    upload(s->triMatLeaf, leafMat.data(), leafMat.size() * 4, st);
+    upload(s->nodes, s->bvh.nodes.data(), s->bvh.nodes.size() * 4, st);
+    PG_HIP(hipStreamSynchronize(st));
+    s->ds.tris = s->tris.as<float4>();
+    s->ds.triMat = s->triMatLeaf.as<uint32_t>();
+    s->ds.nodes = s->nodes.p;
+    s->ds.numTris = nt;
+    s->ds.numNodes = s->bvh.numNodes;
+    s->ds.width = s->bvh.width;
+    s->ds.hasBvh = nt > 0 ? 1u : 0u;
+    const int need = static_cast<int>((s->bvh.width - 1) * s->bvh.maxDepth + 2) - pg::kLdsStack;
+    s->spillPerLane = static_cast<uint32_t>(std::max(need, 1));
+}
+
+uint32_t log2Ceil(uint32_t v) {
+    uint32_t m = 0;
+    while ((1u << m) < v) ++m;
+    return m;
+}
+
+pg::RenderConsts makeConsts(const pg_render_params& p) {
+    require(p.width > 0 && p.height > 0, "film size must be positive");
+    require(p.rect[2] > 0 && p.rect[3] > 0 && p.rect[0] + p.rect[2] <= p.width && p.rect[1] + p.rect[3] <= p.height,
+            "render rect must lie inside the film");
+    require(p.spp_end > p.spp_begin, "empty sample range");
+    require(p.max_depth <= 127, "max_depth above 127 would exceed the 1024 Sobol dimensions");
+    pg::RenderConsts rc{};
+    rc.width = p.width, rc.height = p.height;
+    uint32_t spp = p.spp ? p.spp : 1;
+    spp = 1u << log2Ceil(spp);  // ScrambledSobolSampler.cpp:43-50
+    require(p.spp_end <= spp, "sample range beyond samples per pixel");
+    rc.spp = spp;
+    rc.rectX = p.rect[0], rc.rectY = p.rect[1], rc.rectW = p.rect[2], rc.rectH = p.rect[3];
+    rc.sppBegin = p.spp_begin, rc.sppCount = p.spp_end - p.spp_begin;
+    rc.maxDepth = p.max_depth, rc.minRR = p.min_rr, rc.flags = p.flags;
+    rc.log2Res = log2Ceil(std::max(p.width, p.height));  // ScrambledSobolSampler.cpp:66-72
+    require(rc.log2Res <= pg::kMaxLog2Res, "film larger than 65536");
+    const uint64_t paths = static_cast<uint64_t>(rc.rectW) * rc.rectH * rc.sppCount;
+    require(paths < (1ull << 32), "more than 2^32 paths in one call: split the sample range");
+    require(static_cast<uint64_t>(rc.spp) << (2 * rc.log2Res) < (1ull << 52),
+            "sample index beyond the 52 Sobol matrix columns");
+    std::memcpy(rc.ndc, p.ndc, sizeof(rc.ndc));
+    return rc;
+}
+
+void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
+    if (cap > dev->capacity) {
+        const size_t P = cap;
+        dev->o.reserve(P * 16), dev->d.reserve(P * 16), dev->thr.reserve(P * 16), dev->L.reserve(P * 16);
+        dev->hit.reserve(P * 16), dev->idx.reserve(P * 8), dev->meta.reserve(P * 16), dev->film.reserve(P * 8);
+        dev->qext0.reserve(P * 4), dev->qext1.reserve(P * 4), dev->qdone.reserve(P * 4);
+        dev->qshadow.reserve(P * sizeof(pg::ShadowRec)), dev->qmis.reserve(P * sizeof(pg::MisRec));
+        dev->capacity = cap;
+    }
+    dev->counts.reserve(pg::kCntNum * 4);
+    dev->totals.reserve(pg::kTotNum * 8);
+    dev->travCounts.reserve(64);
+    dev->spill.reserve(std::max<size_t>(spillBytes, 64));
+}
+
+pg::PathState pathState(pg_device* dev) {
+    return pg::PathState{dev->o.as<float4>(),   dev->d.as<float4>(),    dev->thr.as<float4>(), dev->L.as<float4>(),
+                         dev->hit.as<float4>(), dev->idx.as<uint64_t>(), dev->meta.as<uint4>(), dev->film.as<float2>()};
+}
+pg::Queues queues(pg_device* dev) {
+    return pg::Queues{{dev->qext0.as<uint32_t>(), dev->qext1.as<uint32_t>()},
+                      dev->qshadow.as<pg::ShadowRec>(),
+                      dev->qmis.as<pg::MisRec>(),
+                      dev->qdone.as<uint32_t>(),
+                      dev->counts.as<uint32_t>()};
+}
+
+uint32_t traversalLanes(pg_device* dev, uint64_t work) {
+    const uint64_t full = static_cast<uint64_t>(dev->numCUs) * 16 * 64;  // 16 waves per CU
+    const uint64_t need = (work + 63) / 64 * 64;
+    return static_cast<uint32_t>(std::max<uint64_t>(64, std::min(full, need)));
+}
+uint32_t shadeThreads(pg_device* dev, uint64_t work) {
+    const uint64_t full = static_cast<uint64_t>(dev->numCUs) * 8 * 256;
+    const uint64_t need = (work + 255) / 256 * 256;
+    return static_cast<uint32_t>(std::max<uint64_t>(256, std::min(full, need)));
+}
+
+enum Stage { kRaygen, kExtend, kShade, kShadow, kMis, kFilm, kStages };
+
+void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats) {
+    require(s && params, "null scene or params");
+    pg_device* dev = s->dev;
+    std::lock_guard<std::mutex> lock(dev->mu);
+    dev->checkFault();
+    const pg::RenderConsts rc = makeConsts(*params);
+    require(!film || (film->width == rc.width && film->height == rc.height), "film size differs from params");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = dev->stream;
+    const uint64_t total = static_cast<uint64_t>(rc.rectW) * rc.rectH * rc.sppCount;
+    uint32_t cap = params->wavefront_paths ? params->wavefront_paths : (1u << 21);
+    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, total));
+    const uint32_t lanes = traversalLanes(dev, cap);
+    const uint32_t threads = shadeThreads(dev, cap);
+    ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * s->spillPerLane * 8);
+    const pg::PathState ps = pathState(dev);
+    const pg::Queues q = queues(dev);
+    DevBuf pathL;
+    if (pathLHost) pathL.reserve(total * 12);
+    float* film32 = nullptr;
+    double* film64 = nullptr;
+    if (film) {
+        if (rc.flags & pg::kFilmFp64) {
+            if (!film->rgbw64.p) {
+                film->rgbw64.reserve(static_cast<size_t>(film->width) * film->height * 32);
+                PG_HIP(hipMemsetAsync(film->rgbw64.p, 0, film->rgbw64.bytes, st));
+            }
+            film64 = film->rgbw64.as<double>();
+        } else {
+            film32 = film->rgbw32.as<float>();
+        }
+    }
+    PG_HIP(hipMemsetAsync(dev->totals.p, 0, pg::kTotNum * 8, st));
+    const bool countTrav = rc.flags & pg::kCountTraversal;
+    uint32_t* trav = countTrav ? dev->travCounts.as<uint32_t>() : nullptr;
+    if (countTrav) PG_HIP(hipMemsetAsync(dev->travCounts.p, 0, 64, st));
+    dev->eventsUsed = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[kStages];
+    const bool timed = stats != nullptr;
+    auto timeStage = [&](Stage sg, auto&& launch) {
+        if (!timed) {
+            launch();
+            return;
+        }
+        hipEvent_t a = dev->event(), b = dev->event();
+        PG_HIP(hipEventRecord(a, st));
+        launch();
+        PG_HIP(hipEventRecord(b, st));
+        marks[sg].push_back({a, b});
+    };
+    uint64_t extendLaunches = 0;
+    uint2* spill = dev->spill.as<uint2>();
+    for (uint64_t first = 0; first < total; first += cap) {
+        const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(cap, total - first));
+        timeStage(kRaygen, [&] { pg::launchRaygen(st, s->ds, ps, q, rc, first, n); });
+        uint32_t cur = 0;
+        for (uint32_t iter = 0;; ++iter) {
+            if (iter > rc.maxDepth) {
+                uint32_t left = 0;
+                PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                if (left == 0) break;
+            }
+            const uint32_t nxt = cur ^ 1u;
+            pg::launchBounceBegin(st, q, nxt);
+            timeStage(kExtend, [&] {
+                pg::launchExtend(st, s->ds, ps, q, cur, lanes, spill, s->spillPerLane, trav);
+            });
+            ++extendLaunches;
+            timeStage(kShade, [&] { pg::launchShade(st, s->ds, ps, q, rc, cur, nxt, threads, dev->totals.as<uint64_t>()); });
+            timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
+            timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
+            timeStage(kFilm, [&] {
+                pg::launchFilm(st, ps, q, rc, threads, film32, film64, pathLHost ? pathL.as<float>() : nullptr);
+            });
+            PG_HIP(hipGetLastError());
+            cur = nxt;
+        }
+    }
+    PG_HIP(hipStreamSynchronize(st));
+    if (pathLHost) PG_HIP(hipMemcpy(pathLHost, pathL.p, total * 12, hipMemcpyDeviceToHost));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        uint64_t tot[pg::kTotNum];
+        PG_HIP(hipMemcpy(tot, dev->totals.p, sizeof(tot), hipMemcpyDeviceToHost));
+        stats->paths = total;
+        stats->extension_rays = total + tot[pg::kTotExt];
+        stats->shadow_rays = tot[pg::kTotShadow];
+        stats->mis_rays = tot[pg::kTotMis];
+        stats->extend_launches = extendLaunches;
+        if (countTrav) {
+            unsigned long long tc[2];
+            PG_HIP(hipMemcpy(tc, dev->travCounts.p, sizeof(tc), hipMemcpyDeviceToHost));
+            stats->nodes_visited = tc[0];
+            stats->tris_tested = tc[1];
+        }
+        double* ms[kStages] = {&stats->ms_raygen, &stats->ms_extend, &stats->ms_shade,
+                               &stats->ms_shadow, &stats->ms_mis,    &stats->ms_film};
+        for (int sg = 0; sg < kStages; ++sg)
+            for (auto& m : marks[sg]) {
+                float e = 0.0f;
+                PG_HIP(hipEventElapsedTime(&e, m.first, m.second));
+                *ms[sg] += e;
+            }
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pg_last_error(void) { return g_lastError.c_str(); }
+
+int pg_device_create(int ordinal, pg_device** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output");
+        int n = 0;
+        PG_HIP(hipGetDeviceCount(&n));
+        require(ordinal >= 0 && ordinal < n, "no such device");
+        auto dev = std::make_unique<pg_device>();
+        dev->ordinal = ordinal;
+        PG_HIP(hipSetDevice(ordinal));
+        hipDeviceProp_t prop;
+        PG_HIP(hipGetDeviceProperties(&prop, ordinal));
+        dev->numCUs = prop.multiProcessorCount;
+        dev->platform = std::string("AMD HIP ") + prop.gcnArchName;
+        PG_HIP(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+        std::vector<uint32_t> m;
+        std::vector<uint64_t> inv, delta;
+        pg::buildSobolTables(m, inv, delta);
+        upload(dev->sobol, m.data(), m.size() * 4, dev->stream);
+        upload(dev->sobolInv, inv.data(), inv.size() * 8, dev->stream);
+        upload(dev->sobolDelta, delta.data(), delta.size() * 8, dev->stream);
+        PG_HIP(hipStreamSynchronize(dev->stream));
+        *out = dev.release();
+    });
+}
+
+void pg_device_destroy(pg_device* dev) {
+    if (!dev) return;
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    for (auto e : dev->events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(dev->stream);
+    delete dev;
+}
+
+const char* pg_device_platform(pg_device* dev) { return dev ? dev->platform.c_str() : ""; }
+
+int pg_debug_inject_fault(pg_device* dev) {
+    return guarded([&] {
+        require(dev != nullptr, "null device");
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->injectFault = true;
+    });
+}
+
+int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
+    return guarded([&] {
+        require(dev && desc && out, "null argument");
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        const uint32_t nt = desc->num_triangles;
+        require(nt == 0 || (desc->vertices && desc->indices && desc->tri_object && desc->object_material),
+                "missing mesh arrays");
+        require(desc->num_materials > 0 && desc->materials, "at least one material is required");
+        require(desc->num_quads <= 4096 && (desc->num_quads == 0 || desc->quads), "bad light quads");
+        auto s = std::make_unique<pg_scene>();
+        s->dev = dev;
+        s->triVerts.resize(static_cast<size_t>(nt) * 9);
+        s->triMat.resize(nt);
+        for (uint32_t i = 0; i < nt; ++i) {
+            for (int v = 0; v < 3; ++v) {
+                const uint32_t j = desc->indices[3 * i + v];
+                require(j < desc->num_vertices, "triangle index out of range");
+                for (int k = 0; k < 3; ++k) s->triVerts[9 * i + 3 * v + k] = desc->vertices[3 * j + k];
+            }
+            const uint32_t obj = desc->tri_object[i];
+            require(obj < desc->num_objects, "object id out of range");
+            const uint32_t mat = desc->object_material[obj];
+            require(mat < desc->num_materials, "material id out of range");
+            s->triMat[i] = mat;
+        }
+        std::vector<pg::DevMaterial> mats(desc->num_materials);
+        for (uint32_t i = 0; i < desc->num_materials; ++i) {
+            const pg_material& m = desc->materials[i];
+            require(m.type == PG_MAT_LAMBERT || m.type == PG_MAT_CONDUCTOR, "unknown material type");
+            require(m.type != PG_MAT_CONDUCTOR || m.alpha > 0.0f, "conductor alpha must be positive");
+            mats[i] = pg::DevMaterial{m.type, {m.albedo[0], m.albedo[1], m.albedo[2]}, m.alpha, {0, 0, 0}};
+        }
+        std::vector<pg::DevQuad> quads(desc->num_quads);
+        for (uint32_t i = 0; i < desc->num_quads; ++i) {
+            const pg_quad& q = desc->quads[i];
+            pg::DevQuad& d = quads[i];
+            std::memcpy(d.origin, q.origin, 12), std::memcpy(d.u, q.u, 12), std::memcpy(d.v, q.v, 12);
+            std::memcpy(d.radiance, q.radiance, 12);
+            // BasicShapes.cpp:70-90: normal = U x V / |U x V|, area, largest-determinant axis
+            const float det[3] = {q.u[1] * q.v[2] - q.u[2] * q.v[1], q.u[2] * q.v[0] - q.u[0] * q.v[2],
+                                  q.u[0] * q.v[1] - q.u[1] * q.v[0]};
+            d.area = std::sqrt(det[0] * det[0] + det[1] * det[1] + det[2] * det[2]);
+            require(d.area > 0.0f, "degenerate light quad");
+            for (int k = 0; k < 3; ++k) d.normal[k] = det[k] / d.area;
+            const float a0 = std::fabs(det[0]), a1 = std::fabs(det[1]), a2 = std::fabs(det[2]);
+            d.maxDetComp = (a0 >= a1 && a0 >= a2) ? 0u : ((a1 >= a0 && a1 >= a2) ? 1u : 2u);
+        }
+        hipStream_t st = dev->stream;
+        upload(s->mats, mats.data(), mats.size() * sizeof(pg::DevMaterial), st);
+        upload(s->quads, quads.data(), quads.size() * sizeof(pg::DevQuad), st);
+        pg::DevScene& ds = s->ds;
+        ds.mats = s->mats.as<pg::DevMaterial>();
+        ds.quads = s->quads.as<pg::DevQuad>();
+        ds.numQuads = desc->num_quads;
+        ds.sobol = dev->sobol.as<uint32_t>();
+        ds.sobolInv = dev->sobolInv.as<uint64_t>();
+        ds.sobolDelta = dev->sobolDelta.as<uint64_t>();
+        const pg_camera& c = desc->camera;
+        std::memcpy(ds.cam.base, c.position, 12), std::memcpy(ds.cam.lookAt, c.look_at, 12);
+        std::memcpy(ds.cam.up, c.up, 12);
+        ds.cam.sizeX = c.sensor_size[0], ds.cam.sizeY = c.sensor_size[1];
+        ds.cam.focal = c.focal_length, ds.cam.aperture = c.aperture_radius;
+        std::memcpy(ds.env, desc->env_radiance, 12);
+        buildBvh(s.get(), nullptr);
+        *out = s.release();
+    });
+}
+
+int pg_scene_build_bvh(pg_scene* s, const pg_bvh_params* params) {
+    return guarded([&] {
+        require(s && params, "null argument");
+        std::lock_guard<std::mutex> lock(s->dev->mu);
+        s->dev->checkFault();
+        buildBvh(s, params);
+    });
+}
+
+void pg_scene_destroy(pg_scene* s) {
+    if (!s) return;
+    std::lock_guard<std::mutex> lock(s->dev->mu);
+    (void)hipStreamSynchronize(s->dev->stream);
+    delete s;
+}
+
+int pg_scene_bvh_info(pg_scene* s, uint64_t info[6]) {
+    return guarded([&] {
+        require(s && info, "null argument");
+        info[0] = s->bvh.numNodes;
+        info[1] = s->bvh.numLeaves;
+        info[2] = s->bvh.maxDepth;
+        info[3] = s->bvh.width;
+        info[4] = 7ull * 4 * s->bvh.width;
+        info[5] = 48;
+    });
+}
+
+int pg_scene_bvh_download(pg_scene* s, void* nodes, uint64_t nodeBytes, void* tris, uint64_t triBytes) {
+    return guarded([&] {
+        require(s != nullptr, "null scene");
+        require(nodeBytes >= s->bvh.nodes.size() * 4 && triBytes >= s->bvh.tris.size() * 4, "buffers too small");
+        if (nodes) std::memcpy(nodes, s->bvh.nodes.data(), s->bvh.nodes.size() * 4);
+        if (tris) std::memcpy(tris, s->bvh.tris.data(), s->bvh.tris.size() * 4);
+    });
+}
+
+int pg_film_create(pg_device* dev, uint32_t width, uint32_t height, pg_film** out) {
+    return guarded([&] {
+        require(dev && out && width > 0 && height > 0, "bad film arguments");
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        auto f = std::make_unique<pg_film>();
+        f->dev = dev;
+        f->width = width, f->height = height;
+        f->rgbw32.reserve(static_cast<size_t>(width) * height * 16);
+        PG_HIP(hipMemsetAsync(f->rgbw32.p, 0, f->rgbw32.bytes, dev->stream));
+        PG_HIP(hipStreamSynchronize(dev->stream));
+        *out = f.release();
+    });
+}
+
+int pg_film_clear(pg_film* f) {
+    return guarded([&] {
+        require(f != nullptr, "null film");
+        std::lock_guard<std::mutex> lock(f->dev->mu);
+        f->dev->checkFault();
+        PG_HIP(hipMemsetAsync(f->rgbw32.p, 0, f->rgbw32.bytes, f->dev->stream));
+        if (f->rgbw64.p) PG_HIP(hipMemsetAsync(f->rgbw64.p, 0, f->rgbw64.bytes, f->dev->stream));
+        PG_HIP(hipStreamSynchronize(f->dev->stream));
+    });
+}
+
+void pg_film_destroy(pg_film* f) {
+    if (!f) return;
+    std::lock_guard<std::mutex> lock(f->dev->mu);
+    (void)hipStreamSynchronize(f->dev->stream);
+    delete f;
+}
+
+int pg_film_download(pg_film* f, float* rgbw, double* rgbw64) {
+    return guarded([&] {
+        require(f != nullptr, "null film");
+        std::lock_guard<std::mutex> lock(f->dev->mu);
+        f->dev->checkFault();
+        const size_t px = static_cast<size_t>(f->width) * f->height;
+        PG_HIP(hipStreamSynchronize(f->dev->stream));
+        if (rgbw) PG_HIP(hipMemcpy(rgbw, f->rgbw32.p, px * 16, hipMemcpyDeviceToHost));
+        if (rgbw64) {
+            if (f->rgbw64.p) PG_HIP(hipMemcpy(rgbw64, f->rgbw64.p, px * 32, hipMemcpyDeviceToHost));
+            else std::memset(rgbw64, 0, px * 32);
+        }
+    });
+}
+
+int pg_render(pg_scene* s, const pg_render_params* params, pg_film* film, pg_stats* stats) {
+    return guarded([&] { render(s, params, film, nullptr, stats); });
+}
+
+int pg_render_paths(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathL, pg_stats* stats) {
+    return guarded([&] {
+        require(pathL != nullptr, "null path_L");
+        render(s, params, film, pathL, stats);
+    });
+}
+
+int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv) {
+    return guarded([&] {
+        require(s && (n == 0 || (rays && t && prim && uv)), "null argument");
+        if (n == 0) return;
+        pg_device* dev = s->dev;
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        hipStream_t st = dev->stream;
+        DevBuf dr, dt, dp, duv, spill;
+        upload(dr, rays, static_cast<size_t>(n) * 32, st);
+        dt.reserve(static_cast<size_t>(n) * 4), dp.reserve(static_cast<size_t>(n) * 4), duv.reserve(static_cast<size_t>(n) * 8);
+        const uint32_t lanes = traversalLanes(dev, n);
+        spill.reserve(static_cast<size_t>(lanes) * s->spillPerLane * 8);
+        pg::launchTrace(st, s->ds, dr.as<float>(), n, dt.as<float>(), dp.as<uint32_t>(), duv.as<float>(), nullptr, lanes,
+                        spill.as<uint2>(), s->spillPerLane);
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(t, dt.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(prim, dp.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(uv, duv.p, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_occluded(pg_scene* s, const float* rays, uint32_t n, uint8_t* occ) {
+    return guarded([&] {
+        require(s && (n == 0 || (rays && occ)), "null argument");
+        if (n == 0) return;
+        pg_device* dev = s->dev;
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        hipStream_t st = dev->stream;
+        DevBuf dr, docc, spill;
+        upload(dr, rays, static_cast<size_t>(n) * 32, st);
+        docc.reserve(n);
+        const uint32_t lanes = traversalLanes(dev, n);
+        spill.reserve(static_cast<size_t>(lanes) * s->spillPerLane * 8);
+        pg::launchTrace(st, s->ds, dr.as<float>(), n, nullptr, nullptr, nullptr, docc.as<uint8_t>(), lanes,
+                        spill.as<uint2>(), s->spillPerLane);
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(occ, docc.p, n, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_camera_rays(pg_scene* s, const pg_render_params* params, const uint32_t* pix, uint32_t n, float* rays,
+                   float* film) {
+    return guarded([&] {
+        require(s && params && (n == 0 || (pix && rays && film)), "null argument");
+        if (n == 0) return;
+        const pg::RenderConsts rc = makeConsts(*params);
+        for (uint32_t i = 0; i < n; ++i)
+            require(pix[3 * i] < rc.width && pix[3 * i + 1] < rc.height && pix[3 * i + 2] < rc.spp, "pixel out of range");
+        pg_device* dev = s->dev;
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        hipStream_t st = dev->stream;
+        DevBuf dpix, drays, dfilm;
+        upload(dpix, pix, static_cast<size_t>(n) * 12, st);
+        drays.reserve(static_cast<size_t>(n) * 32), dfilm.reserve(static_cast<size_t>(n) * 8);
+        pg::launchCameraRays(st, s->ds, rc, dpix.as<uint32_t>(), n, drays.as<float>(), dfilm.as<float>());
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(rays, drays.p, static_cast<size_t>(n) * 32, hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(film, dfilm.p, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_sobol(pg_device* dev, uint32_t m, uint32_t dims, const uint32_t* pix, uint32_t n, float* out,
+             uint64_t* indexOut) {
+    return guarded([&] {
+        require(dev && (n == 0 || (pix && out)), "null argument");
+        require(m <= pg::kMaxLog2Res && dims <= pg::kSobolDims, "resolution or dimension out of range");
+        if (n == 0) return;
+        for (uint32_t i = 0; i < n; ++i) {
+            require(pix[3 * i] < (1u << m) && pix[3 * i + 1] < (1u << m), "pixel outside the 2^m grid");
+            require((static_cast<uint64_t>(pix[3 * i + 2]) << (2 * m)) < (1ull << pg::kSobolCols),
+                    "sample index beyond the Sobol matrix columns");
+        }
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        hipStream_t st = dev->stream;
+        DevBuf dpix, dout, didx;
+        upload(dpix, pix, static_cast<size_t>(n) * 12, st);
+        dout.reserve(std::max<size_t>(static_cast<size_t>(n) * dims * 4, 4));
+        didx.reserve(static_cast<size_t>(n) * 8);
+        pg::DevScene ds{};
+        ds.sobol = dev->sobol.as<uint32_t>();
+        ds.sobolInv = dev->sobolInv.as<uint64_t>();
+        ds.sobolDelta = dev->sobolDelta.as<uint64_t>();
+        pg::launchSobol(st, ds, m, dims, dpix.as<uint32_t>(), n, dout.as<float>(), didx.as<uint64_t>());
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        if (dims) PG_HIP(hipMemcpy(out, dout.p, static_cast<size_t>(n) * dims * 4, hipMemcpyDeviceToHost));
+        if (indexOut) PG_HIP(hipMemcpy(indexOut, didx.p, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_sobol_matrices(uint32_t* out) {
+    return guarded([&] {
+        require(out != nullptr, "null output");
+        std::vector<uint32_t> m;
+        std::vector<uint64_t> inv, delta;
+        pg::buildSobolTables(m, inv, delta);
+        std::memcpy(out, m.data(), m.size() * 4);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,511 @@
+// Device-side building blocks of the gfx950 path tracer: vector math, Sobol, camera, watertight
+// triangle test, light quads, wide-BVH traversal, BxDFs and lights.
+//
+// The whole translation unit is compiled with -ffp-contract=off: the camera, the triangle test and
+// the Sobol conversion must produce the same float bits as the CPU restatement of the reference.
+// Where speed matters and exactness does not (the ray/box slab test), fused multiply-adds are
+// written out explicitly.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pg_shared.hpp"
+
+namespace pg {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+// Normal's normalising constructor (Kernel/Transform.hpp:244-248): divide each component.
+__device__ __forceinline__ V3 normalize(V3 a) {
+    const float l = length(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+// Spectrum::valid (Kernel/Protocol.hpp:92-94)
+__device__ __forceinline__ bool valid(V3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+__device__ __forceinline__ float comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+
+// ------------------------------------------------------------------------------------------
+// Sobol: sobolSampleImpl (ScrambledSobolSampler/Kernel.cpp:23-29) walking only the set bits.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float sobolSample(const uint32_t* __restrict__ m, uint64_t index, uint32_t dim) {
+    if (dim >= kSobolDims) {
+        // Past the table the reference switches to a per-path pseudo-random engine
+        // (Tracer/Embree/Kernel.cpp:37-45); here a splitmix64 hash of (index, dim).
+        uint64_t z = index ^ (static_cast<uint64_t>(dim) * 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return static_cast<float>(static_cast<uint32_t>(z >> 40)) * 0x1p-24f;
+    }
+    const uint32_t* col = m + dim * kSobolCols;
+    uint32_t v = 0;
+    while (index) {
+        v ^= col[__builtin_ctzll(index)];
+        index &= index - 1;
+    }
+    return static_cast<float>(v) * 0x1p-32f;
+}
+
+// sobolIntervalToIndex (ScrambledSobolSampler/Kernel.cpp:31-50) with tables from a GF(2) inverse.
+__device__ __forceinline__ uint64_t intervalToIndex(const DevScene& s, uint32_t m, uint32_t sample, uint32_t px,
+                                                    uint32_t py) {
+    if (m == 0) return 0;
+    const uint32_t n = 2 * m;
+    uint64_t index = static_cast<uint64_t>(sample) << n;
+    uint64_t b = (static_cast<uint64_t>(px) << m) | py;
+    const uint64_t* delta = s.sobolDelta + m * 64;
+    const uint64_t* inv = s.sobolInv + m * 32;
+    while (sample) {
+        b ^= delta[__builtin_ctz(sample)];
+        sample &= sample - 1;
+    }
+    while (b) {
+        index ^= inv[__builtin_ctzll(b)];
+        b &= b - 1;
+    }
+    return index;
+}
+
+struct Ray {
+    V3 o, d;
+};
+
+// rayGen (Sensor/PerspectiveCamera/Kernel.cpp:22-48), same operation order as the CPU oracle.
+__device__ __forceinline__ Ray cameraRay(const DevCamera& c, float ndcX, float ndcY, float u1, float u2) {
+    const V3 base = ld3(c.base), lookAt = ld3(c.lookAt), upRef = ld3(c.up);
+    const V3 forward = normalize(lookAt - base);
+    const V3 right = cross(forward, upRef);
+    const V3 up = cross(right, forward);
+    const V3 filmHit = base + right * (c.sizeX * (0.5f - ndcX)) + up * (c.sizeY * (ndcY - 0.5f));
+    const float focalDistance = dot(lookAt - base, forward);
+    const float filmDistance = 1.0f / (1.0f / c.focal - 1.0f / focalDistance);
+    const V3 lensCenter = base + forward * filmDistance;
+    const float ang = u1 * (2.0f * kPi);  // sampleUniformDisk, Kernel/Sampling.hpp:23-29
+    const float rad = sqrtf(fmaxf(0.0f, u2));
+    const float lx = rad * cosf(ang), ly = rad * sinf(ang);
+    const V3 lensHit = lensCenter + right * (c.aperture * lx) + up * (c.aperture * ly);
+    const V3 dir = lensCenter - filmHit;
+    const V3 focusHit = lensCenter + dir * (focalDistance / dot(dir, forward));
+    return Ray{lensHit, normalize(focusHit - lensHit)};
+}
+
+// ------------------------------------------------------------------------------------------
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), replacing Embree's triangle
+// intersector behind rtcIntersect1 (Tracer/Embree/Embree.cpp:300).
+// ------------------------------------------------------------------------------------------
+struct RayPre {
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float ix, iy, iz;     // 1/d with zero components replaced (box test only)
+    float oix, oiy, oiz;  // o * (1/d)
+};
+__device__ __forceinline__ RayPre precompute(const Ray& r) {
+    RayPre p;
+    const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    p.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    p.kx = p.kz == 2 ? 0 : p.kz + 1;
+    p.ky = p.kx == 2 ? 0 : p.kx + 1;
+    const float dz = comp(r.d, p.kz);
+    if (dz < 0.0f) {
+        const int t = p.kx;
+        p.kx = p.ky;
+        p.ky = t;
+    }
+    p.sx = comp(r.d, p.kx) / dz;
+    p.sy = comp(r.d, p.ky) / dz;
+    p.sz = 1.0f / dz;
+    p.ix = 1.0f / (r.d.x == 0.0f ? 1e-20f : r.d.x);
+    p.iy = 1.0f / (r.d.y == 0.0f ? 1e-20f : r.d.y);
+    p.iz = 1.0f / (r.d.z == 0.0f ? 1e-20f : r.d.z);
+    p.oix = r.o.x * p.ix;
+    p.oiy = r.o.y * p.iy;
+    p.oiz = r.o.z * p.iz;
+    return p;
+}
+
+__device__ __forceinline__ bool intersectTri(const Ray& r, const RayPre& p, V3 v0, V3 v1, V3 v2, float& t, float& bu,
+                                             float& bv) {
+    const V3 a = v0 - r.o, b = v1 - r.o, c = v2 - r.o;
+    const float az = comp(a, p.kz), bz = comp(b, p.kz), cz = comp(c, p.kz);
+    const float ax = comp(a, p.kx) - p.sx * az, ay = comp(a, p.ky) - p.sy * az;
+    const float bx = comp(b, p.kx) - p.sx * bz, by = comp(b, p.ky) - p.sy * bz;
+    const float cx = comp(c, p.kx) - p.sx * cz, cy = comp(c, p.ky) - p.sy * cz;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {  // exact edge: redo the 2D edge tests in double
+        U = static_cast<float>(static_cast<double>(cx) * by - static_cast<double>(cy) * bx);
+        V = static_cast<float>(static_cast<double>(ax) * cy - static_cast<double>(ay) * cx);
+        W = static_cast<float>(static_cast<double>(bx) * ay - static_cast<double>(by) * ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (p.sz * az) + V * (p.sz * bz) + W * (p.sz * cz);
+    t = T / det;
+    bu = V / det;
+    bv = W / det;
+    return true;
+}
+
+// planeIntersect + calcTexCoord (Geometry/BasicShapes/Kernel.cpp:30-74)
+__device__ __forceinline__ bool intersectQuad(const Ray& r, const DevQuad& q, float tNear, float tFar, float& tOut) {
+    const V3 qo = ld3(q.origin), qu = ld3(q.u), qv = ld3(q.v), qn = ld3(q.normal);
+    const V3 delta = qo - r.o;
+    const float b = dot(delta, qn);
+    const float k = dot(r.d, qn);
+    const float t = b / k;
+    if (!(tNear < t && t < tFar)) return false;
+    const V3 pos = r.d * t - delta;
+    float invDet, x, y;
+    if (q.maxDetComp == 0) {
+        invDet = 1.0f / (qu.y * qv.z - qv.y * qu.z);
+        x = pos.y * qv.z - qv.y * pos.z;
+        y = qu.y * pos.z - pos.y * qu.z;
+    } else if (q.maxDetComp == 1) {
+        invDet = 1.0f / (qu.z * qv.x - qv.z * qu.x);
+        x = pos.z * qv.x - qv.z * pos.x;
+        y = qu.z * pos.x - pos.z * qu.x;
+    } else {
+        invDet = 1.0f / (qu.x * qv.y - qv.x * qu.y);
+        x = pos.x * qv.y - qv.x * pos.y;
+        y = qu.x * pos.y - pos.x * qu.y;
+    }
+    const float u = x * invDet, v = y * invDet;
+    if (!(0.0f <= u && u <= 1.0f && 0.0f <= v && v <= 1.0f)) return false;
+    tOut = t;
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Wide-BVH traversal, one ray per lane. Each lane keeps a stack of (child reference, entry
+// distance) pairs: the first kLdsStack entries live in the wave's LDS slice (lane-minor, so the
+// 64 lanes of a wave hit distinct banks), deeper entries spill to a per-lane HBM region.
+// Closest hit: nearest child first, the rest pushed and culled on pop against the best t.
+// ------------------------------------------------------------------------------------------
+
+
+struct HitRec {
+    float t;
+    uint32_t pos;   // leaf-order triangle position, kHitQuad | quad, or kHitMiss
+    uint32_t prim;  // original triangle index (tie rule), or as pos for quads/miss
+    float u, v;
+};
+
+__device__ __forceinline__ void stackPush(uint2* ls, uint2* gs, int& sp, uint32_t ref, float t) {
+    const uint2 e = make_uint2(ref, __float_as_uint(t));
+    if (sp < kLdsStack) ls[sp * 64] = e;
+    else gs[sp - kLdsStack] = e;
+    ++sp;
+}
+__device__ __forceinline__ uint2 stackPop(const uint2* ls, const uint2* gs, int& sp) {
+    --sp;
+    return sp < kLdsStack ? ls[sp * 64] : gs[sp - kLdsStack];
+}
+
+// ANY = false: closest hit of triangles in (tmin, h.t] (tie: smaller original prim).
+// ANY = true : returns true at the first triangle with tmin < t < tmax.
+template <int W, bool ANY>
+__device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const RayPre& p, float tmin, float tmax,
+                                         HitRec& h, uint2* ls, uint2* gs, uint32_t* nodeCount, uint32_t* triCount) {
+    const WideNode<W>* __restrict__ nodes = static_cast<const WideNode<W>*>(s.nodes);
+    const float4* __restrict__ tris = s.tris;
+    int sp = 0;
+    uint32_t ref = 0;  // root is node 0
+    while (true) {
+        if (!(ref & kLeafBit)) {
+            if (nodeCount) ++*nodeCount;
+            const WideNode<W> n = nodes[ref];
+            const float tcull = ANY ? tmax : h.t * 1.00001f;
+            float tn[W];
+            uint32_t mask = 0;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const float ax = __builtin_fmaf(n.lox[c], p.ix, -p.oix), bx = __builtin_fmaf(n.hix[c], p.ix, -p.oix);
+                const float ay = __builtin_fmaf(n.loy[c], p.iy, -p.oiy), by = __builtin_fmaf(n.hiy[c], p.iy, -p.oiy);
+                const float az = __builtin_fmaf(n.loz[c], p.iz, -p.oiz), bz = __builtin_fmaf(n.hiz[c], p.iz, -p.oiz);
+                const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+                const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tcull)) * 1.0000004f;
+                tn[c] = t0;
+                if (n.child[c] != kInvalidChild && t0 <= t1) mask |= 1u << c;
+            }
+            if (mask) {
+                int best = __builtin_ctz(mask);
+                float bt = tn[best];
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (((mask >> c) & 1) && tn[c] < bt) bt = tn[c], best = c;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (((mask >> c) & 1) && c != best) stackPush(ls, gs, sp, n.child[c], tn[c]);
+                ref = n.child[best];
+                continue;
+            }
+        } else {
+            const uint32_t first = ref & kLeafFirstMask;
+            const uint32_t cnt = ((ref >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                if (triCount) ++*triCount;
+                const float4 a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
+                float t, u, v;
+                if (!intersectTri(r, p, xyz(a), xyz(b), xyz(c), t, u, v)) continue;
+                if (ANY) {
+                    if (t > tmin && t < tmax) return true;
+                } else {
+                    const uint32_t prim = __float_as_uint(a.w);
+                    if (t > tmin && (t < h.t || (t == h.t && prim < h.prim))) h = HitRec{t, k, prim, u, v};
+                }
+            }
+        }
+        // pop the next entry that can still hold a closer hit
+        bool found = false;
+        while (sp > 0) {
+            const uint2 e = stackPop(ls, gs, sp);
+            if (ANY || __uint_as_float(e.y) <= h.t * 1.00001f) {
+                ref = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return ANY ? false : h.pos != kHitMiss;
+}
+
+// Closest hit of triangles and light quads (rtcIntersect1 with mask gsm|areaLight, Embree.cpp:292).
+template <int W>
+__device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls,
+                                             uint2* gs, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
+    HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
+    const RayPre p = precompute(r);
+    if (s.hasBvh) traverse<W, false>(s, r, p, tmin, tmax, h, ls, gs, nodeCount, triCount);
+    for (uint32_t q = 0; q < s.numQuads; ++q) {
+        float t;
+        if (intersectQuad(r, s.quads[q], tmin, h.t, t)) h = HitRec{t, kHitQuad | q, kHitQuad | q, 0.0f, 0.0f};
+    }
+    if (h.pos == kHitMiss) h.t = tmax;
+    return h;
+}
+
+// rtcOccluded1 with mask gsm (Embree.cpp:357-391): light quads never occlude.
+template <int W>
+__device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls, uint2* gs) {
+    if (!s.hasBvh) return false;
+    HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
+    const RayPre p = precompute(r);
+    return traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, nullptr, nullptr);
+}
+
+// ------------------------------------------------------------------------------------------
+// Shading frame and hit normal (calcTriangleMeshSurface, Tracer/Embree/Kernel.cpp:170-192;
+// planeSurface, BasicShapes/Kernel.cpp:99-111; SurfaceIntersectionInfo, Protocol.hpp:47-66)
+// ------------------------------------------------------------------------------------------
+struct Frame {
+    V3 t, b, n;
+    __device__ __forceinline__ V3 toLocal(V3 v) const { return {dot(t, v), dot(b, v), dot(n, v)}; }
+    __device__ __forceinline__ V3 toWorld(V3 v) const {
+        return {t.x * v.x + b.x * v.y + n.x * v.z, t.y * v.x + b.y * v.y + n.y * v.z, t.z * v.x + b.z * v.y + n.z * v.z};
+    }
+};
+__device__ __forceinline__ Frame shadingFrame(V3 n, bool d3) {
+    Frame f;
+    f.n = n;
+    const V3 ex{1.0f, 0.0f, 0.0f}, ey{0.0f, 1.0f, 0.0f};
+    f.t = fabsf(dot(n, ex)) < fabsf(dot(n, ey)) ? cross(n, ex) : cross(n, ey);
+    if (!d3) f.t = normalize(f.t);
+    f.b = cross(n, f.t);
+    return f;
+}
+__device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir) {
+    const V3 v0 = xyz(s.tris[3 * pos]), v1 = xyz(s.tris[3 * pos + 1]), v2 = xyz(s.tris[3 * pos + 2]);
+    const V3 ng = normalize(cross(v1 - v0, v2 - v0));
+    return dot(dir, ng) < 0.0f ? ng : -ng;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
+}
+__device__ __forceinline__ V3 quadNormal(const DevQuad& q, V3 dir) {
+    const V3 n = ld3(q.normal);
+    return dot(dir, n) > 0.0f ? -n : n;
+}
+
+// ------------------------------------------------------------------------------------------
+// Sampling helpers (Kernel/Sampling.hpp:31-50) and BxDFs (Surface/Classic/Kernel.cpp)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ V3 cosineHemisphere(float u1, float u2) {
+    u1 = 2.0f * u1 - 1.0f;
+    u2 = 2.0f * u2 - 1.0f;
+    const float au1 = fabsf(u1), au2 = fabsf(u2);
+    float x = 0.0f, y = 0.0f;
+    if (!(fminf(au1, au2) < 1e-4f)) {
+        const float rad = au1 > au2 ? u1 : u2;
+        const float theta = au1 > au2 ? (kPi / 4.0f) * (u2 / u1) : (kPi / 2.0f) - (kPi / 4.0f) * (u1 / u2);
+        x = cosf(theta) * rad;
+        y = sinf(theta) * rad;
+    }
+    return {x, y, sqrtf(fmaxf(0.0f, 1.0f - (x * x + y * y)))};
+}
+
+__device__ __forceinline__ float ggxLambda(V3 w, float alpha) {  // Classic/Kernel.cpp:326-341
+    const float cos2 = w.z * w.z;
+    const float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (isinf(tan2)) return 0.0f;
+    return (-1.0f + sqrtf(1.0f + alpha * alpha * tan2)) * 0.5f;
+}
+__device__ __forceinline__ float ggxD(V3 wh, float alpha) {  // Classic/Kernel.cpp:342-357
+    const float cos2 = wh.z * wh.z;
+    const float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (isinf(tan2)) return 0.0f;
+    const float e = cos2 * (1.0f + tan2 / (alpha * alpha));
+    return kInvPi / (alpha * alpha * e * e);
+}
+// Visible-normal sampling (Heitz, JCGT 2018).
+__device__ __forceinline__ V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2) {
+    const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / sqrtf(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    const V3 t2 = cross(vh, t1);
+    const float r = sqrtf(u1);
+    const float phi = 2.0f * kPi * u2;
+    const float p1 = r * cosf(phi);
+    float p2 = r * sinf(phi);
+    const float sm = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - sm) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return normalize(V3{alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)});
+}
+__device__ __forceinline__ V3 schlick(V3 f0, float c) {
+    const float m = 1.0f - fminf(1.0f, fmaxf(0.0f, c));
+    const float m5 = (m * m) * (m * m) * m;
+    return V3{f0.x + (1.0f - f0.x) * m5, f0.y + (1.0f - f0.y) * m5, f0.z + (1.0f - f0.z) * m5};
+}
+__device__ __forceinline__ V3 conductorF(V3 f0, float alpha, V3 wo, V3 wi) {  // MicrofacetReflection::evaluate
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+    const V3 wh = normalize(wo + wi);
+    const float g = 1.0f + ggxLambda(wo, alpha) + ggxLambda(wi, alpha);
+    const float k = ggxD(wh, alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    return schlick(f0, dot(wi, wh)) * k;
+}
+__device__ __forceinline__ float conductorPdf(float alpha, V3 wo, V3 wi) {
+    if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
+    const V3 wh = normalize(wo + wi);
+    return ggxD(wh, alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, alpha)));
+}
+
+struct Mat {
+    uint32_t type;
+    V3 albedo;
+    float alpha;
+};
+__device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id) {
+    const DevMaterial& m = s.mats[id];
+    return Mat{m.type, ld3(m.albedo), m.alpha};
+}
+__device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
+    if (!(dot(wo, ng) * dot(wi, ng) > 0.0f)) return {0.0f, 0.0f, 0.0f};  // Classic/Kernel.cpp:220
+    if (m.type == 0) return m.albedo * kInvPi;
+    return conductorF(m.albedo, m.alpha, wo, wi);
+}
+__device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
+    if (m.type == 0) return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f;
+    return conductorPdf(m.alpha, wo, wi);
+}
+struct BsdfSample {
+    V3 wi, f;
+    float pdf;
+};
+__device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, float u1, float u2) {
+    BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
+    if (m.type == 0) {
+        s.wi = cosineHemisphere(u1, u2);
+        s.f = m.albedo * kInvPi;
+        s.pdf = s.wi.z > 0.0f ? fabsf(s.wi.z) * kInvPi : 0.0f;
+        return s;
+    }
+    if (wo.z <= 0.0f) return s;
+    const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+    const float c = dot(wo, wh);
+    if (c < 0.0f) return s;
+    s.wi = wh * (2.0f * c) - wo;
+    if (s.wi.z <= 0.0f) return s;
+    s.f = conductorF(m.albedo, m.alpha, wo, s.wi);
+    s.pdf = conductorPdf(m.alpha, wo, s.wi);
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// Lights: uniformSelect (BasicLight/Kernel.cpp:91-98), diffuseSample over planeSample
+// (AreaLight/Kernel.cpp:24-41, BasicShapes/Kernel.cpp:113-134), SESample (BasicLight :63-76)
+// ------------------------------------------------------------------------------------------
+struct LightSample {
+    V3 dir, rad;
+    float dist, pdf;
+};
+__device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t light, V3 hit, float u1, float u2,
+                                                   bool d1) {
+    LightSample ls{{0, 0, 0}, {0, 0, 0}, 0.0f, 0.0f};
+    if (light == 0) {
+        const float theta = u1 * kPi, phi = u2 * kPi;
+        const float st = sinf(theta);
+        ls.dir = {st * cosf(phi), st * sinf(phi), cosf(theta)};
+        ls.rad = ld3(s.env);
+        ls.pdf = st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
+        ls.dist = 1e5f;
+        return ls;
+    }
+    const DevQuad& q = s.quads[light - 1];
+    const float uu = d1 ? (u1 - 1.0f) / 1.0f : u1;
+    const V3 src = ld3(q.origin) + ld3(q.u) * uu + ld3(q.v) * u2;
+    V3 n = ld3(q.normal);
+    if (dot(hit - src, n) <= 0.0f) n = -n;
+    const float pdfA = 1.0f / q.area;
+    const V3 delta = src - hit;
+    if (dot(delta, n) >= 0.0f) return ls;
+    const float dis2 = dot(delta, delta);
+    ls.rad = ld3(q.radiance);
+    ls.dist = sqrtf(dis2);
+    ls.dir = {delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
+    ls.pdf = pdfA * dis2 / fabsf(dot(n, ls.dir));
+    return ls;
+}
+__device__ __forceinline__ float envPdf(V3 dir) {  // SEPdf, BasicLight/Kernel.cpp:82-89
+    const float theta = acosf(fminf(1.0f, fmaxf(-1.0f, dir.z)));
+    const float st = sinf(theta);
+    return st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
+}
+__device__ __forceinline__ float powerHeuristic(float a, float b) {  // PathTracing/Kernel.cpp:27-31
+    const float sa = a * a, sb = b * b;
+    return sa / (sa + sb);
+}
+
+// Wave-level compaction: every lane of the wave calls this together; lanes with pred get
+// consecutive slots from one atomic per wave.
+__device__ __forceinline__ uint32_t wavePush(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(mask));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader));
+    return base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+}  // namespace pg

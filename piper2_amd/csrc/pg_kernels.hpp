@@ -1,0 +1,49 @@
+// Launchers of the gfx950 kernels (pg_kernels.hip) used by the host library (pg_api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pg_shared.hpp"
+
+namespace pg {
+
+// Path state, structure of arrays over the wave's path slots.
+struct PathState {
+    float4* o;     // ray origin
+    float4* d;     // ray direction
+    float4* thr;   // throughput
+    float4* L;     // radiance so far
+    float4* hit;   // t, hit position bits, u, v
+    uint64_t* idx; // Sobol index
+    uint4* meta;   // next Sobol dim, depth, flags (bit 0: specular), path number within the call
+    float2* film;  // film point
+};
+
+struct Queues {
+    uint32_t* ext[2];  // extension queues (current / next bounce)
+    ShadowRec* shadow;
+    MisRec* mis;
+    uint32_t* done;
+    uint32_t* counts;  // kCntNum u32
+};
+
+void launchRaygen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                  uint64_t first, uint32_t n);
+void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot);
+void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
+                  uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts);
+void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals);
+void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
+                  uint2* spill, uint32_t spillPerLane);
+void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
+               uint2* spill, uint32_t spillPerLane);
+void launchFilm(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
+                float* film32, double* film64, float* pathL);
+void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
+                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane);
+void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
+                      float* rays, float* film);
+void launchSobol(hipStream_t st, const DevScene& s, uint32_t m, uint32_t dims, const uint32_t* pix, uint32_t n,
+                 float* out, uint64_t* indexOut);
+
+}  // namespace pg

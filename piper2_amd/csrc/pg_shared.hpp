@@ -1,0 +1,109 @@
+// Layouts shared by the host library code and the gfx950 kernels of libpiper_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pg {
+
+constexpr uint32_t kSobolDims = 1024;  // ScrambledSobolSampler/Shared.hpp:22
+constexpr uint32_t kSobolCols = 52;    // ScrambledSobolSampler/Shared.hpp:23
+constexpr uint32_t kMaxLog2Res = 16;   // interval-to-index tables cover resolutions up to 2^16
+
+constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;  // child = kLeafBit | (count-1) << 27 | first triangle
+constexpr uint32_t kLeafCountShift = 27;
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+constexpr uint32_t kMaxLeafSize = 8;
+// Traversal stack entries per lane kept in LDS; deeper entries spill to a per-lane HBM region.
+constexpr int kLdsStack = 16;
+
+constexpr uint32_t kHitMiss = 0xFFFFFFFFu;
+constexpr uint32_t kHitQuad = 0x80000000u;
+
+// Wide BVH node, structure-of-arrays inside the node so that one lane streams the whole node with
+// 16-byte loads: W boxes as six planes of W floats, then W child references.
+template <int W>
+struct alignas(16) WideNode {
+    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
+    uint32_t child[W];
+};
+static_assert(sizeof(WideNode<8>) == 224, "BVH8 node is 224 B");
+static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 B");
+
+struct DevMaterial {  // 32 B
+    uint32_t type;
+    float albedo[3];
+    float alpha;
+    float reserved[3];
+};
+
+struct DevQuad {  // Plane primitive (PerPlaneData) + DiffuseAreaLightData, 80 B
+    float origin[3], u[3], v[3];
+    float normal[3];
+    float radiance[3];
+    float area;
+    uint32_t maxDetComp;  // PerPlaneData::maxDetComp, BasicShapes.cpp:70-75
+    uint32_t reserved[3];
+};
+
+struct DevCamera {
+    float base[3], lookAt[3], up[3];
+    float sizeX, sizeY, focal, aperture;
+};
+
+struct DevScene {
+    const float4* tris;      // 3 float4 per triangle in leaf order: (v0, prim bits) (v1, 0) (v2, 0)
+    const uint32_t* triMat;  // material per leaf-order triangle
+    const void* nodes;       // WideNode<width>[]
+    const DevMaterial* mats;
+    const DevQuad* quads;
+    const uint32_t* sobol;       // kSobolDims * kSobolCols
+    const uint64_t* sobolInv;    // [kMaxLog2Res+1][32]
+    const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
+    uint32_t numTris, numNodes, numQuads, width;
+    DevCamera cam;
+    float env[3];
+    uint32_t hasBvh;
+};
+
+// Queue records written by the shade stage.
+struct alignas(16) ShadowRec {  // 48 B
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+    float cr, cg, cb;
+    uint32_t path;
+};
+struct alignas(16) MisRec {  // 48 B
+    float ox, oy, oz, bsdfPdf;
+    float dx, dy, dz;
+    uint32_t light;  // selected light (0 = environment, 1 + q = quad q)
+    float cr, cg, cb;
+    uint32_t path;
+};
+
+// Counters in device memory (one block of u32, then u64 totals).
+enum CounterSlot : uint32_t {
+    kCntExtCur = 0,
+    kCntExtNext = 1,
+    kCntShadow = 2,
+    kCntMis = 3,
+    kCntDone = 4,
+    kCntNum = 8
+};
+enum TotalSlot : uint32_t { kTotExt = 0, kTotShadow = 1, kTotMis = 2, kTotNum = 4 };
+
+struct RenderConsts {
+    uint32_t width, height, spp;
+    uint32_t rectX, rectY, rectW, rectH;
+    uint32_t sppBegin, sppCount;
+    uint32_t maxDepth, minRR, flags;
+    uint32_t log2Res;
+    float ndc[4];
+};
+
+constexpr uint32_t kCompatD1 = 0x1u, kCompatD3 = 0x2u, kCompatD9 = 0x4u, kCompatD13 = 0x8u;
+constexpr uint32_t kFilmFp64 = 0x100u;
+constexpr uint32_t kCountTraversal = 0x200u;
+
+}  // namespace pg

@@ -1,0 +1,230 @@
+"""Seeded scene generators for the benchmark and parity workloads.
+
+Each generator returns a dict of numpy arrays in the layout of pg_scene_desc (include/piper_hip.h):
+  vertices (V,3) f32, indices (T,3) u32, tri_object (T,) u32, object_material (O,) u32,
+  materials (M,8) f32 = [type, r, g, b, alpha, 0, 0, 0], quads (Q,12) f32 = [origin, U, V, radiance],
+  camera (14,) f32 = [position, look_at, up, sensor_w, sensor_h, focal, aperture, 0], env (3,) f32,
+plus the render settings (width, height, spp, max_depth, min_rr, ndc).
+
+Instances (the boxes and icospheres) are flattened to world space. The camera's up vector is made
+orthogonal to the view direction, so the reference camera's unnormalised right/up vectors
+(Sensor/PerspectiveCamera/Kernel.cpp:31-32) stay unit length.
+"""
+import math
+
+import numpy as np
+
+LAMBERT, CONDUCTOR = 0, 1
+
+
+def calc_render_rect(width, height, device_aspect, fit_fill=True):
+    """calcRenderRECT (Interface/BuiltinComponent/Sensor.hpp:38-74): NDC affine transform and rect."""
+    iar = np.float32(width) / np.float32(height)
+    iiar = np.float32(1.0) / iar
+    idar = np.float32(1.0) / np.float32(device_aspect)
+    dar = np.float32(device_aspect)
+    if fit_fill:
+        rect = (0, 0, width, height)
+        if iar > dar:
+            ndc = (0.0, (idar - iiar) * 0.5 * dar, 1.0, iiar * dar)
+        else:
+            ndc = ((dar - iar) * 0.5 * idar, 0.0, iar * idar, 1.0)
+        return tuple(float(np.float32(v)) for v in ndc), rect
+    raise NotImplementedError("OverScan fit mode")
+
+
+def make_camera(pos, look_at, up_hint, vfov_deg, aspect, focal=0.035, fstop=math.inf):
+    pos, look_at = np.asarray(pos, np.float64), np.asarray(look_at, np.float64)
+    fwd = look_at - pos
+    focus = np.linalg.norm(fwd)
+    fwd /= focus
+    up = np.asarray(up_hint, np.float64)
+    up = up - fwd * np.dot(up, fwd)
+    up /= np.linalg.norm(up)
+    film_dist = 1.0 / (1.0 / focal - 1.0 / focus)
+    sh = 2.0 * film_dist * math.tan(math.radians(vfov_deg) / 2.0)
+    sw = sh * aspect
+    aperture = 0.0 if math.isinf(fstop) else focal / (2.0 * fstop)
+    return np.array([*pos, *look_at, *up, sw, sh, focal, aperture, 0.0], np.float32)
+
+
+def _quad_tris(p0, p1, p2, p3):
+    return [p0, p1, p2, p3], [[0, 1, 2], [0, 2, 3]]
+
+
+def _box(center_xz, y0, size, yaw):
+    """Axis-aligned box of `size` (sx, sy, sz) rotated by yaw about +y, base at height y0: 12 tris."""
+    sx, sy, sz = size
+    c, s = math.cos(yaw), math.sin(yaw)
+    local = np.array([[x, y, z] for y in (0.0, sy) for z in (-sz / 2, sz / 2) for x in (-sx / 2, sx / 2)])
+    rot = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    v = local @ rot.T + np.array([center_xz[0], y0, center_xz[1]])
+    # vertex index = y*4 + z*2 + x
+    faces = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 2, 6, 4], [1, 5, 7, 3], [0, 4, 5, 1], [2, 3, 7, 6]]
+    tris = []
+    for f in faces:
+        tris += [[f[0], f[1], f[2]], [f[0], f[2], f[3]]]
+    return v, np.array(tris)
+
+
+def icosphere(subdiv):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    v = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[key] = len(v) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return np.array(v), np.array(f)
+
+
+def _value_noise(rng, x, z, cells_x, cells_z):
+    lattice = rng.uniform(-1.0, 1.0, size=(cells_z + 2, cells_x + 2))
+    gx, gz = x * cells_x, z * cells_z
+    ix, iz = np.floor(gx).astype(int), np.floor(gz).astype(int)
+    fx, fz = gx - ix, gz - iz
+    sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+    a = lattice[iz, ix] * (1 - sx) + lattice[iz, ix + 1] * sx
+    b = lattice[iz + 1, ix] * (1 - sx) + lattice[iz + 1, ix + 1] * sx
+    return a * (1 - sz) + b * sz
+
+
+def fbm_height(rng, u, v, octaves=4, amplitude=0.2, base=(8, 4)):
+    """fBm value noise over the unit square (u, v in [0, 1]), peak amplitude `amplitude`."""
+    h = np.zeros_like(u)
+    norm = 0.0
+    for o in range(octaves):
+        w = 0.5 ** o
+        h += w * _value_noise(rng, u, v, base[0] << o, base[1] << o)
+        norm += w
+    return amplitude * h / norm
+
+
+class _Builder:
+    def __init__(self):
+        self.v, self.t, self.obj, self.objmat, self.mats = [], [], [], [], []
+        self.nv = 0
+
+    def material(self, kind, albedo, alpha=0.0):
+        self.mats.append([kind, *albedo, alpha, 0, 0, 0])
+        return len(self.mats) - 1
+
+    def add(self, verts, tris, mat):
+        oid = len(self.objmat)
+        self.objmat.append(mat)
+        self.v.append(np.asarray(verts, np.float64))
+        self.t.append(np.asarray(tris, np.int64) + self.nv)
+        self.obj.append(np.full(len(tris), oid, np.uint32))
+        self.nv += len(verts)
+        return oid
+
+    def finish(self, quads, camera, **settings):
+        sc = {
+            "vertices": np.ascontiguousarray(np.concatenate(self.v).astype(np.float32)),
+            "indices": np.ascontiguousarray(np.concatenate(self.t).astype(np.uint32)),
+            "tri_object": np.ascontiguousarray(np.concatenate(self.obj)),
+            "object_material": np.ascontiguousarray(np.array(self.objmat, np.uint32)),
+            "materials": np.ascontiguousarray(np.array(self.mats, np.float32)),
+            "quads": np.ascontiguousarray(np.array(quads, np.float32).reshape(-1, 12)),
+            "camera": camera,
+            "env": np.zeros(3, np.float32),
+        }
+        sc.update(settings)
+        return sc
+
+
+def cornell(seed=1, width=256, height=256, spp=16, max_depth=5):
+    """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    walls = {
+        "floor": [[-1, -1, -1], [1, -1, -1], [1, -1, 1], [-1, -1, 1]],
+        "ceiling": [[-1, 1, -1], [-1, 1, 1], [1, 1, 1], [1, 1, -1]],
+        "back": [[-1, -1, -1], [-1, 1, -1], [1, 1, -1], [1, -1, -1]],
+        "left": [[-1, -1, -1], [-1, -1, 1], [-1, 1, 1], [-1, 1, -1]],
+        "right": [[1, -1, -1], [1, 1, -1], [1, 1, 1], [1, -1, 1]],
+    }
+    for name, corners in walls.items():
+        alb = np.sort(rng.uniform(0.1, 0.8, 3))
+        if name == "left":  # red-dominant
+            alb = np.array([alb[2], alb[0], alb[1]])
+        elif name == "right":  # green-dominant
+            alb = np.array([alb[0], alb[2], alb[1]])
+        else:
+            rng.shuffle(alb)
+        v, t = _quad_tris(*corners)
+        b.add(v, t, b.material(LAMBERT, alb))
+    for _ in range(2):
+        size = rng.uniform(0.3, 0.6, 3)
+        yaw = rng.uniform(0.0, math.pi / 2)
+        pos = rng.uniform(-0.6, 0.6, 2)
+        v, t = _box(pos, -1.0, size, yaw)
+        b.add(v, t, b.material(LAMBERT, rng.uniform(0.1, 0.8, 3)))
+    quads = [[-0.25, 0.999, -0.25, 0.5, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0]]
+    aspect = width / height
+    cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
+    ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
+    return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
+                    ndc=ndc, name="cornell")
+
+
+def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500),
+                    n_spheres=200, n_lights=16):
+    """Config 2: fBm height-field terrain + 200 icospheres (subdivision 3), 16 emissive quads."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    gx, gz = grid
+    u, v = np.meshgrid(np.linspace(0.0, 1.0, gx), np.linspace(0.0, 1.0, gz))
+    hgt = fbm_height(rng, u, v)
+    verts = np.stack([-10.0 + 20.0 * u, hgt, -5.0 + 10.0 * v], axis=-1).reshape(-1, 3)
+    i = np.arange(gz - 1)[:, None] * gx + np.arange(gx - 1)[None, :]
+    i = i.reshape(-1)
+    tris = np.concatenate([np.stack([i, i + gx, i + 1], -1), np.stack([i + 1, i + gx, i + gx + 1], -1)])
+    b.add(verts, tris, b.material(LAMBERT, rng.uniform(0.1, 0.9, 3)))
+
+    sv, st = icosphere(3)
+    for _ in range(n_spheres):
+        r = rng.uniform(0.1, 0.4)
+        x, z = rng.uniform(-9.0, 9.0), rng.uniform(-4.5, 4.5)
+        cx = int(round((x + 10.0) / 20.0 * (gx - 1)))
+        cz = int(round((z + 5.0) / 10.0 * (gz - 1)))
+        y = hgt[cz, cx] + 0.7 * r
+        if rng.uniform() < 0.5:
+            mat = b.material(LAMBERT, rng.uniform(0.1, 0.9, 3))
+        else:
+            rough = rng.uniform(0.05, 0.6)
+            mat = b.material(CONDUCTOR, rng.uniform(0.5, 1.0, 3), rough * rough)  # alpha = roughness^2
+        b.add(sv * r + np.array([x, y, z]), st, mat)
+
+    quads = []
+    for _ in range(n_lights):
+        sx, sz = rng.uniform(0.5, 1.0, 2)
+        x, z, y = rng.uniform(-8.0, 8.0), rng.uniform(-4.0, 4.0), rng.uniform(2.0, 3.0)
+        rad = rng.uniform(5.0, 50.0, 3)
+        quads.append([x - sx / 2, y, z - sz / 2, sx, 0.0, 0.0, 0.0, 0.0, sz, *rad])
+    aspect = width / height
+    cam = make_camera([0.0, 3.5, 11.0], [0.0, 0.0, 0.0], [0, 1, 0], 45.0, aspect)
+    ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
+    return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=min_rr,
+                    ndc=ndc, name="terrain_spheres")
+
+
+def world_triangles(sc):
+    """(T, 3, 3) float32 world-space triangle vertices."""
+    return sc["vertices"][sc["indices"]]

@@ -1,0 +1,112 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so). Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_VP = ctypes.c_void_p
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError("oracle/liboracle.so missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(ORACLE_PATH)
+        L.or_sobol_sample.restype = ctypes.c_float
+        L.or_sobol_sample.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+        L.or_interval_to_index.restype = ctypes.c_uint64
+        L.or_interval_to_index.argtypes = [ctypes.c_uint32] * 4
+        L.or_sobol_matrices.argtypes = [_VP]
+        L.or_scene_create.restype = _VP
+        L.or_scene_create.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint32, _VP,
+                                      ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]
+        L.or_scene_destroy.argtypes = [_VP]
+        L.or_trace.argtypes = [_VP, _VP, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int, _VP]
+        L.or_occluded.argtypes = [_VP, _VP, ctypes.c_uint32, _VP]
+        L.or_camera_rays.argtypes = [_VP, _VP, _VP, _VP, ctypes.c_uint32, _VP, _VP]
+        L.or_render.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_int, _VP]
+        L.or_bsdf_eval.argtypes = [_VP, _VP, _VP, _VP, _VP]
+        L.or_bsdf_sample.argtypes = [_VP, _VP, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP]
+        L.or_ggx_d.restype = ctypes.c_float
+        L.or_ggx_d.argtypes = [ctypes.c_float, ctypes.c_float]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_VP) if a is not None else None
+
+
+def sobol_matrices():
+    out = np.zeros(1024 * 52, np.uint32)
+    lib().or_sobol_matrices(_p(out))
+    return out.reshape(1024, 52)
+
+
+def sobol_sample(index, dim):
+    return lib().or_sobol_sample(int(index), int(dim))
+
+
+def interval_to_index(m, s, px, py):
+    return lib().or_interval_to_index(m, s, px, py)
+
+
+class OracleScene:
+    def __init__(self, sc):
+        self.sc = sc
+        self._arrs = [np.ascontiguousarray(sc[k]) for k in
+                      ("vertices", "indices", "tri_object", "object_material", "materials", "quads", "camera", "env")]
+        v, i, to, om, m, q, c, e = self._arrs
+        self.h = lib().or_scene_create(_p(v), v.shape[0], _p(i), i.shape[0], _p(to), _p(om), om.shape[0], _p(m),
+                                       m.shape[0], _p(q), q.shape[0], _p(c), _p(e))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def trace(self, rays, brute=False, counts=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t, prim, uv = np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros((n, 2), np.float32)
+        cnt = np.zeros(2, np.uint64)
+        lib().or_trace(self.h, _p(rays), n, _p(t), _p(prim), _p(uv), int(brute), _p(cnt) if counts else None)
+        return (t, prim, uv, cnt) if counts else (t, prim, uv)
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint8)
+        lib().or_occluded(self.h, _p(rays), rays.shape[0], _p(out))
+        return out
+
+    def camera_rays(self, pix):
+        pix = np.ascontiguousarray(pix, np.uint32).reshape(-1, 3)
+        sc = self.sc
+        ip = np.array([sc["width"], sc["height"], sc["spp"]], np.uint32)
+        ndc = np.array(sc["ndc"], np.float32)
+        rays = np.zeros((pix.shape[0], 8), np.float32)
+        film = np.zeros((pix.shape[0], 2), np.float32)
+        lib().or_camera_rays(self.h, _p(ip), _p(ndc), _p(pix), pix.shape[0], _p(rays), _p(film))
+        return rays, film
+
+    def render(self, rect=None, spp_range=None, flags=0, threads=None, path_l=False, film=True, spp=None,
+               max_depth=None, min_rr=None):
+        sc = self.sc
+        w, h = sc["width"], sc["height"]
+        spp = spp or sc["spp"]
+        rect = rect or (0, 0, w, h)
+        s0, s1 = spp_range or (0, spp)
+        md = sc["max_depth"] if max_depth is None else max_depth
+        rr = sc["min_rr"] if min_rr is None else min_rr
+        ip = np.array([w, h, spp, md, rr, flags, *rect, s0, s1], np.uint32)
+        ndc = np.array(sc["ndc"], np.float32)
+        f = np.zeros((h, w, 4), np.float64) if film else None
+        pl = np.zeros((rect[2] * rect[3] * (s1 - s0), 3), np.float32) if path_l else None
+        rc = np.zeros(3, np.uint64)
+        lib().or_render(self.h, _p(ip), _p(ndc), _p(f), _p(pl), int(threads or os.cpu_count() or 1), _p(rc))
+        return {"film": f, "path_l": pl, "rays": rc}
