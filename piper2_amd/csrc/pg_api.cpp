@@ -87,6 +87,10 @@ void upload(DevBuf& b, const void* host, size_t n, hipStream_t st) {
 }  // namespace
 
 struct pg_device {
+    // Lifetime: scenes and films keep their device alive; pg_device_destroy only marks it closed
+    // while any of them remains, and the last child to go frees it.
+    int children = 0;
+    bool closed = false;
     int ordinal = 0;
     int numCUs = 256;
     hipStream_t stream = nullptr;
@@ -134,6 +138,24 @@ struct pg_film {
 };
 
 namespace {
+
+std::mutex g_lifetime;
+
+void freeDevice(pg_device* dev) {
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    for (auto e : dev->events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(dev->stream);
+    delete dev;
+}
+void addChild(pg_device* dev) {
+    std::lock_guard<std::mutex> g(g_lifetime);
+    ++dev->children;
+}
+void releaseChild(pg_device* dev) {
+    std::lock_guard<std::mutex> g(g_lifetime);
+    if (--dev->children == 0 && dev->closed) freeDevice(dev);
+}
 
 void buildBvh(pg_scene* s, const pg_bvh_params* params) {
     pg::BvhParams bp;
@@ -375,11 +397,9 @@ int pg_device_create(int ordinal, pg_device** out) {
 
 void pg_device_destroy(pg_device* dev) {
     if (!dev) return;
-    (void)hipSetDevice(dev->ordinal);
-    (void)hipStreamSynchronize(dev->stream);
-    for (auto e : dev->events) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(dev->stream);
-    delete dev;
+    std::lock_guard<std::mutex> g(g_lifetime);
+    dev->closed = true;
+    if (dev->children == 0) freeDevice(dev);
 }
 
 const char* pg_device_platform(pg_device* dev) { return dev ? dev->platform.c_str() : ""; }
@@ -457,6 +477,7 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         ds.cam.focal = c.focal_length, ds.cam.aperture = c.aperture_radius;
         std::memcpy(ds.env, desc->env_radiance, 12);
         buildBvh(s.get(), nullptr);
+        addChild(dev);
         *out = s.release();
     });
 }
@@ -472,9 +493,13 @@ int pg_scene_build_bvh(pg_scene* s, const pg_bvh_params* params) {
 
 void pg_scene_destroy(pg_scene* s) {
     if (!s) return;
-    std::lock_guard<std::mutex> lock(s->dev->mu);
-    (void)hipStreamSynchronize(s->dev->stream);
-    delete s;
+    pg_device* dev = s->dev;
+    {
+        std::lock_guard<std::mutex> lock(dev->mu);
+        (void)hipStreamSynchronize(dev->stream);
+        delete s;
+    }
+    releaseChild(dev);
 }
 
 int pg_scene_bvh_info(pg_scene* s, uint64_t info[6]) {
@@ -509,6 +534,7 @@ int pg_film_create(pg_device* dev, uint32_t width, uint32_t height, pg_film** ou
         f->rgbw32.reserve(static_cast<size_t>(width) * height * 16);
         PG_HIP(hipMemsetAsync(f->rgbw32.p, 0, f->rgbw32.bytes, dev->stream));
         PG_HIP(hipStreamSynchronize(dev->stream));
+        addChild(dev);
         *out = f.release();
     });
 }
@@ -526,9 +552,13 @@ int pg_film_clear(pg_film* f) {
 
 void pg_film_destroy(pg_film* f) {
     if (!f) return;
-    std::lock_guard<std::mutex> lock(f->dev->mu);
-    (void)hipStreamSynchronize(f->dev->stream);
-    delete f;
+    pg_device* dev = f->dev;
+    {
+        std::lock_guard<std::mutex> lock(dev->mu);
+        (void)hipStreamSynchronize(dev->stream);
+        delete f;
+    }
+    releaseChild(dev);
 }
 
 int pg_film_download(pg_film* f, float* rgbw, double* rgbw64) {

@@ -139,7 +139,8 @@ def _path_parity(sc, dev, flags=0, rect=None, spp=None, frac=0.99):
     ig = fg[..., :3][m] / fg[..., 3:][m]
     ic = fc[..., :3][m] / fc[..., 3:][m]
     assert rel_mse(ig, ic) < 1e-4
-    assert st["extension_rays"] == res["rays"][0]
+    # the few paths that diverge by an ulp may trace a different number of rays
+    assert abs(int(st["extension_rays"]) - int(res["rays"][0])) <= 1e-4 * int(res["rays"][0])
     return st
 
 
