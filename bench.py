@@ -83,6 +83,8 @@ def main():
     log(f"device {dev.platform}; BVH{info['width']} built in {time.perf_counter() - t0:.1f}s: {info}")
     film = pg.Film(dev, sc["width"], sc["height"])
     params = pg.render_params(sc, wavefront_paths=args.wavefront_paths)
+    copy_gbps = dev.copy_bandwidth(1 << 30, 3)  # kCopy: 1 GiB read + 1 GiB written per launch
+    log(f"plain copy kernel: {copy_gbps:.0f} GB/s")
 
     # traversal work per extension ray on a 4-sample slice (counting perturbs timing: untimed)
     cnt = pg.render_params(sc, spp_range=(0, 4), flags=pg.PG_COUNT_TRAVERSAL,
@@ -136,6 +138,7 @@ def main():
                    "lights": 16, "spp": args.spp, "bvh_width": info["width"],
                    "wavefront_paths": args.wavefront_paths or (1 << 21)},
         "mrays_per_s": round(total_rays / elapsed / 1e6, 2),
+        "copy_kernel_gbps": round(copy_gbps, 1),
         "rays_per_path": round(total_rays / paths, 3),
         "stage_ms_per_step": {k[3:]: round(v / args.steps, 2) for k, v in totals.items()},
         "roofline": {"bound": "hbm", "kernel": "kExtend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

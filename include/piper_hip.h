@@ -158,6 +158,9 @@ int pg_sobol_matrices(uint32_t* out);
 /* ---- diagnostics ---- */
 /* Make the next device call on this handle fail with PG_ERR_DEVICE (host-forced fault path test). */
 int pg_debug_inject_fault(pg_device* dev);
+/* Run a plain device copy kernel of `bytes` (read + write) `reps` times; *gbps receives the
+ * (read + write) bandwidth. Calibrates HBM byte counters; no reference counterpart. */
+int pg_debug_copy(pg_device* dev, uint64_t bytes, uint32_t reps, double* gbps);
 
 #ifdef __cplusplus
 }

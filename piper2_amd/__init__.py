@@ -109,6 +109,7 @@ SIGNATURES = {
     "pg_sobol": (ctypes.c_int, [_VP, _U32, _U32, _VP, _U32, _VP, _VP]),
     "pg_sobol_matrices": (ctypes.c_int, [_VP]),
     "pg_debug_inject_fault": (ctypes.c_int, [_VP]),
+    "pg_debug_copy": (ctypes.c_int, [_VP, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -175,6 +176,12 @@ class Device:
 
     def inject_fault(self):
         check(lib().pg_debug_inject_fault(self.h))
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=5):
+        """GB/s (read + write) of a plain device copy kernel over nbytes."""
+        g = ctypes.c_double()
+        check(lib().pg_debug_copy(self.h, nbytes, reps, ctypes.byref(g)))
+        return g.value
 
 
 def sobol_matrices():

@@ -97,10 +97,10 @@ struct pg_device {
     std::mutex mu;
     bool injectFault = false;
     std::string platform;
-    DevBuf sobol, sobolInv, sobolDelta;
+    DevBuf sobol, sobolT, sobolInv, sobolDelta;
     // render workspace
     uint32_t capacity = 0;
-    DevBuf o, d, thr, L, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
+    DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
     size_t spillBytes = 0;
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
@@ -218,10 +218,10 @@ pg::RenderConsts makeConsts(const pg_render_params& p) {
 void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
     if (cap > dev->capacity) {
         const size_t P = cap;
-        dev->o.reserve(P * 16), dev->d.reserve(P * 16), dev->thr.reserve(P * 16), dev->L.reserve(P * 16);
+        dev->o.reserve(P * 16), dev->d.reserve(P * 16), dev->thr.reserve(P * 16), dev->L.reserve(P * 16), dev->Lmis.reserve(P * 16);
         dev->hit.reserve(P * 16), dev->idx.reserve(P * 8), dev->meta.reserve(P * 16), dev->film.reserve(P * 8);
         dev->qext0.reserve(P * 4), dev->qext1.reserve(P * 4), dev->qdone.reserve(P * 4);
-        dev->qshadow.reserve(P * sizeof(pg::ShadowRec)), dev->qmis.reserve(P * sizeof(pg::MisRec));
+        dev->qshadow.reserve(2 * P * sizeof(pg::ShadowRec)), dev->qmis.reserve(P * sizeof(pg::MisRec));
         dev->capacity = cap;
     }
     dev->counts.reserve(pg::kCntNum * 4);
@@ -231,8 +231,9 @@ void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
 }
 
 pg::PathState pathState(pg_device* dev) {
-    return pg::PathState{dev->o.as<float4>(),   dev->d.as<float4>(),    dev->thr.as<float4>(), dev->L.as<float4>(),
-                         dev->hit.as<float4>(), dev->idx.as<uint64_t>(), dev->meta.as<uint4>(), dev->film.as<float2>()};
+    return pg::PathState{dev->o.as<float4>(),    dev->d.as<float4>(),   dev->thr.as<float4>(),
+                         dev->L.as<float4>(),    dev->Lmis.as<float4>(), dev->hit.as<float4>(),
+                         dev->idx.as<uint64_t>(), dev->meta.as<uint4>(), dev->film.as<float2>()};
 }
 pg::Queues queues(pg_device* dev) {
     return pg::Queues{{dev->qext0.as<uint32_t>(), dev->qext1.as<uint32_t>()},
@@ -326,7 +327,10 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
             ++extendLaunches;
             timeStage(kShade, [&] { pg::launchShade(st, s->ds, ps, q, rc, cur, nxt, threads, dev->totals.as<uint64_t>()); });
             timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-            timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
+            // With a black environment the shade stage resolves MIS-BSDF rays as shadow-queue
+            // occlusion queries, and the closest-hit MIS queue stays empty.
+            const bool envBlack = !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
+            if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
             timeStage(kFilm, [&] {
                 pg::launchFilm(st, ps, q, rc, threads, film32, film64, pathLHost ? pathL.as<float>() : nullptr);
             });
@@ -388,6 +392,10 @@ int pg_device_create(int ordinal, pg_device** out) {
         std::vector<uint64_t> inv, delta;
         pg::buildSobolTables(m, inv, delta);
         upload(dev->sobol, m.data(), m.size() * 4, dev->stream);
+        std::vector<uint32_t> mt(m.size());
+        for (uint32_t d = 0; d < pg::kSobolDims; ++d)
+            for (uint32_t j = 0; j < pg::kSobolCols; ++j) mt[j * pg::kSobolDims + d] = m[d * pg::kSobolCols + j];
+        upload(dev->sobolT, mt.data(), mt.size() * 4, dev->stream);
         upload(dev->sobolInv, inv.data(), inv.size() * 8, dev->stream);
         upload(dev->sobolDelta, delta.data(), delta.size() * 8, dev->stream);
         PG_HIP(hipStreamSynchronize(dev->stream));
@@ -468,6 +476,7 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         ds.quads = s->quads.as<pg::DevQuad>();
         ds.numQuads = desc->num_quads;
         ds.sobol = dev->sobol.as<uint32_t>();
+        ds.sobolT = dev->sobolT.as<uint32_t>();
         ds.sobolInv = dev->sobolInv.as<uint64_t>();
         ds.sobolDelta = dev->sobolDelta.as<uint64_t>();
         const pg_camera& c = desc->camera;
@@ -681,6 +690,29 @@ int pg_sobol(pg_device* dev, uint32_t m, uint32_t dims, const uint32_t* pix, uin
         PG_HIP(hipStreamSynchronize(st));
         if (dims) PG_HIP(hipMemcpy(out, dout.p, static_cast<size_t>(n) * dims * 4, hipMemcpyDeviceToHost));
         if (indexOut) PG_HIP(hipMemcpy(indexOut, didx.p, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_debug_copy(pg_device* dev, uint64_t bytes, uint32_t reps, double* gbps) {
+    return guarded([&] {
+        require(dev && bytes >= 16 && bytes % 16 == 0 && reps > 0, "bad copy arguments");
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        hipStream_t st = dev->stream;
+        DevBuf a, b;
+        a.reserve(bytes), b.reserve(bytes);
+        PG_HIP(hipMemsetAsync(a.p, 1, bytes, st));
+        hipEvent_t e0 = dev->event(), e1 = dev->event();
+        pg::launchCopy(st, a.p, b.p, bytes);  // warm-up
+        PG_HIP(hipEventRecord(e0, st));
+        for (uint32_t r = 0; r < reps; ++r) pg::launchCopy(st, a.p, b.p, bytes);
+        PG_HIP(hipEventRecord(e1, st));
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        float ms = 0.0f;
+        PG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        dev->eventsUsed = 0;
+        if (gbps) *gbps = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
     });
 }
 

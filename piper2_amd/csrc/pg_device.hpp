@@ -64,6 +64,27 @@ __device__ __forceinline__ float sobolSample(const uint32_t* __restrict__ m, uin
     return static_cast<float>(v) * 0x1p-32f;
 }
 
+// Dimensions d0 .. d0+7 of one index at once, from the bit-major copy of the matrices
+// (sobolT[bit * kSobolDims + dim]): each set index bit costs 8 independent loads from one 32-byte
+// row segment instead of 8 separate dependent chains. Bit-identical to sobolSample.
+__device__ __forceinline__ void sobolWindow(const DevScene& s, uint64_t index, uint32_t d0, float out[8]) {
+    if (d0 + 8 > kSobolDims) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = sobolSample(s.sobol, index, d0 + k);
+        return;
+    }
+    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t* __restrict__ t = s.sobolT + d0;
+    while (index) {
+        const uint32_t* row = t + __builtin_ctzll(index) * kSobolDims;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] ^= row[k];
+        index &= index - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = static_cast<float>(v[k]) * 0x1p-32f;
+}
+
 // sobolIntervalToIndex (ScrambledSobolSampler/Kernel.cpp:31-50) with tables from a GF(2) inverse.
 __device__ __forceinline__ uint64_t intervalToIndex(const DevScene& s, uint32_t m, uint32_t sample, uint32_t px,
                                                     uint32_t py) {
@@ -172,6 +193,10 @@ __device__ __forceinline__ bool intersectQuad(const Ray& r, const DevQuad& q, fl
     const V3 delta = qo - r.o;
     const float b = dot(delta, qn);
     const float k = dot(r.d, qn);
+    // Division-free rejection that never rejects a hit of the exact test below (tNear >= 0):
+    // t > 0 needs b and k of one sign; |b| >= |k| tFar (1 + 1e-6) guarantees fl(b / k) >= tFar.
+    if (!((b > 0.0f && k > 0.0f) || (b < 0.0f && k < 0.0f))) return false;
+    if (fabsf(b) >= fabsf(k) * (tFar * 1.000001f)) return false;
     const float t = b / k;
     if (!(tNear < t && t < tFar)) return false;
     const V3 pos = r.d * t - delta;
@@ -223,16 +248,21 @@ __device__ __forceinline__ uint2 stackPop(const uint2* ls, const uint2* gs, int&
 
 // ANY = false: closest hit of triangles in (tmin, h.t] (tie: smaller original prim).
 // ANY = true : returns true at the first triangle with tmin < t < tmax.
-template <int W, bool ANY>
+struct TravCount {
+    uint32_t nodes = 0, tris = 0;
+};
+
+// COUNT = true also counts wide nodes visited and triangles tested (roofline bookkeeping).
+template <int W, bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const RayPre& p, float tmin, float tmax,
-                                         HitRec& h, uint2* ls, uint2* gs, uint32_t* nodeCount, uint32_t* triCount) {
+                                         HitRec& h, uint2* ls, uint2* gs, TravCount& cnt) {
     const WideNode<W>* __restrict__ nodes = static_cast<const WideNode<W>*>(s.nodes);
     const float4* __restrict__ tris = s.tris;
     int sp = 0;
     uint32_t ref = 0;  // root is node 0
     while (true) {
         if (!(ref & kLeafBit)) {
-            if (nodeCount) ++*nodeCount;
+            if (COUNT) ++cnt.nodes;
             const WideNode<W> n = nodes[ref];
             const float tcull = ANY ? tmax : h.t * 1.00001f;
             float tn[W];
@@ -248,22 +278,25 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
                 if (n.child[c] != kInvalidChild && t0 <= t1) mask |= 1u << c;
             }
             if (mask) {
-                int best = __builtin_ctz(mask);
-                float bt = tn[best];
+                // Every index into the node is a compile-time constant (no dynamic indexing), so
+                // the node stays in registers instead of being spilled to scratch memory.
+                int best = 0;
+                float bt = INFINITY;
+                uint32_t bestRef = 0;
 #pragma unroll
                 for (int c = 0; c < W; ++c)
-                    if (((mask >> c) & 1) && tn[c] < bt) bt = tn[c], best = c;
+                    if (((mask >> c) & 1) && tn[c] < bt) bt = tn[c], best = c, bestRef = n.child[c];
 #pragma unroll
                 for (int c = 0; c < W; ++c)
                     if (((mask >> c) & 1) && c != best) stackPush(ls, gs, sp, n.child[c], tn[c]);
-                ref = n.child[best];
+                ref = bestRef;
                 continue;
             }
         } else {
             const uint32_t first = ref & kLeafFirstMask;
-            const uint32_t cnt = ((ref >> kLeafCountShift) & 0xFu) + 1u;
-            for (uint32_t k = first; k < first + cnt; ++k) {
-                if (triCount) ++*triCount;
+            const uint32_t count = ((ref >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + count; ++k) {
+                if (COUNT) ++cnt.tris;
                 const float4 a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
                 float t, u, v;
                 if (!intersectTri(r, p, xyz(a), xyz(b), xyz(c), t, u, v)) continue;
@@ -291,12 +324,12 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
 }
 
 // Closest hit of triangles and light quads (rtcIntersect1 with mask gsm|areaLight, Embree.cpp:292).
-template <int W>
+template <int W, bool COUNT = false>
 __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls,
-                                             uint2* gs, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
+                                             uint2* gs, TravCount& cnt) {
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
-    if (s.hasBvh) traverse<W, false>(s, r, p, tmin, tmax, h, ls, gs, nodeCount, triCount);
+    if (s.hasBvh) traverse<W, false, COUNT>(s, r, p, tmin, tmax, h, ls, gs, cnt);
     for (uint32_t q = 0; q < s.numQuads; ++q) {
         float t;
         if (intersectQuad(r, s.quads[q], tmin, h.t, t)) h = HitRec{t, kHitQuad | q, kHitQuad | q, 0.0f, 0.0f};
@@ -311,7 +344,8 @@ __device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float 
     if (!s.hasBvh) return false;
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
-    return traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, nullptr, nullptr);
+    TravCount cnt;
+    return traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, cnt);
 }
 
 // ------------------------------------------------------------------------------------------

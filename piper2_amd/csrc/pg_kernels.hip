@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(256) kRaygen(DevScene s, PathState ps, Queues 
     ps.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
     ps.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     ps.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ps.Lmis[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps.idx[i] = idx;
     ps.meta[i] = make_uint4(5u, 0u, 1u, g);  // sample dims start at 5 (Embree/Kernel.cpp:68); specular
     ps.film[i] = make_float2(fp.x, fp.y);
@@ -77,7 +78,7 @@ __global__ void kBounceBegin(Queues q, uint32_t nextSlot) {
 // ------------------------------------------------------------------------------------------
 // extend: closest hit for every queued path
 // ------------------------------------------------------------------------------------------
-template <int W>
+template <int W, bool COUNT>
 __global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const uint32_t* __restrict__ queue,
                                               const uint32_t* __restrict__ count, uint2* spill, uint32_t spillPerLane,
                                               unsigned long long* travCounts) {
@@ -86,18 +87,17 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const ui
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
-    uint32_t nodes = 0, tris = 0;
+    TravCount cnt;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const uint32_t path = queue[i];
         const float4 o = ps.o[path], d = ps.d[path];
         const Ray r{xyz(o), xyz(d)};
-        const HitRec h = closestHit<W>(s, r, 1e-3f, 1e5f, lstack + lane, gs, travCounts ? &nodes : nullptr,
-                                       travCounts ? &tris : nullptr);
+        const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt);
         ps.hit[path] = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
     }
-    if (travCounts) {
-        atomicAdd(&travCounts[0], static_cast<unsigned long long>(nodes));
-        atomicAdd(&travCounts[1], static_cast<unsigned long long>(tris));
+    if (COUNT) {
+        atomicAdd(&travCounts[0], static_cast<unsigned long long>(cnt.nodes));
+        atomicAdd(&travCounts[1], static_cast<unsigned long long>(cnt.tris));
     }
 }
 
@@ -105,17 +105,6 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const ui
 // shade: one vertex of trace() (PathTracing/Kernel.cpp:115-190) including
 // multipleImportanceSampling (:23-107); shadow and MIS-BSDF rays go to their queues.
 // ------------------------------------------------------------------------------------------
-struct SobolStream {
-    const uint32_t* m;
-    uint64_t idx;
-    uint32_t dim;
-    __device__ __forceinline__ float next() { return sobolSample(m, idx, dim++); }
-    __device__ __forceinline__ void next2(bool d9, float& u1, float& u2) {
-        const float a = next(), b = next();
-        if (d9) u1 = b, u2 = a;
-        else u1 = a, u2 = b;
-    }
-};
 
 __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q, RenderConsts rc, uint32_t curSlot,
                                               uint32_t nextSlot, uint64_t* totals) {
@@ -124,12 +113,13 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
     uint32_t* qnext = q.ext[nextSlot];
     const bool d1 = rc.flags & kCompatD1, d3 = rc.flags & kCompatD3, d9 = rc.flags & kCompatD9,
                d13 = rc.flags & kCompatD13;
+    const bool envBlack = !valid(ld3(s.env));
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop
         const uint32_t i = i0 + threadIdx.x;
-        bool pushExt = false, pushShadow = false, pushMis = false, pushDone = false;
+        bool pushExt = false, pushShadow = false, pushMis = false, pushMisQuery = false, pushDone = false;
         uint32_t path = 0;
-        ShadowRec sr;
+        ShadowRec sr, mq;
         MisRec mr;
         if (i < n) {
             path = qcur[i];
@@ -138,7 +128,9 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
             uint4 meta = ps.meta[path];
             Ray ray{xyz(ps.o[path]), xyz(ps.d[path])};
             V3 thr = xyz(ps.thr[path]);
-            V3 L = xyz(ps.L[path]);
+            // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
+            V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
+            ps.Lmis[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const bool specular = meta.z & 1u;
             const bool isLight = pos != kHitMiss && (pos & kHitQuad);
             if (specular && (isLight || pos == kHitMiss)) {  // :119-140, two-sided emission
@@ -159,16 +151,17 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                 const V3 wo = fr.toLocal(-ray.d);
                 const V3 ng = fr.toLocal(nrm);
                 const V3 hp = ray.o + ray.d * hit4.x;
-                SobolStream smp{s.sobol, ps.idx[path], meta.x};
+                // The vertex draws at most 8 consecutive dimensions: light select, light (2),
+                // MIS BSDF (2), continuation (2), Russian roulette (1), in that order.
+                float u[8];
+                sobolWindow(s, ps.idx[path], meta.x, u);
+                uint32_t used = 7;
 
                 // light selection: uniformSelect over {environment, quads}
                 const uint32_t count = 1u + s.numQuads;
-                const uint32_t sel =
-                    min(static_cast<uint32_t>(floorf(smp.next() * static_cast<float>(count))), count - 1u);
+                const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
                 const float selInv = 1.0f / (1.0f / static_cast<float>(count));
-                float u1, u2;
-                smp.next2(d9, u1, u2);
-                const LightSample ls = sampleLight(s, sel, hp, u1, u2, d1);
+                const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
                 if (ls.pdf > 0.0f && valid(ls.rad)) {
                     const V3 wi = fr.toLocal(ls.dir);
                     const V3 f = bsdfEval(mat, wo, wi, ng);
@@ -181,18 +174,50 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                         pushShadow = true;
                     }
                 }
-                smp.next2(d9, u1, u2);
-                const BsdfSample bs = bsdfSample(mat, wo, u1, u2);
+                const BsdfSample bs = bsdfSample(mat, wo, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
                 if (bs.pdf > 0.0f && valid(bs.f)) {
                     const float weight = fabsf(bs.wi.z) / bs.pdf;
                     const V3 dir = fr.toWorld(bs.wi);
                     const V3 c = mul(thr, bs.f * weight * selInv);
-                    mr = MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
-                    pushMis = true;
+                    if (!envBlack) {
+                        mr = MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
+                        pushMis = true;
+                    } else if (sel != 0u) {
+                        // With a black environment the BSDF ray contributes only if the selected
+                        // quad is its closest hit: the quad must be hit, beat every other quad in
+                        // (t, index) order (closestHit's quad loop), and no triangle may lie in
+                        // (tmin, tq] (triangles win ties). The first two are decided here; the
+                        // triangle test becomes an occlusion query on the shadow queue.
+                        const Ray mray{hp, dir};
+                        const uint32_t qi = sel - 1u;
+                        const DevQuad& qd = s.quads[qi];
+                        float tq;
+                        if (intersectQuad(mray, qd, 1e-3f, 1e5f, tq)) {
+                            bool first = true;
+                            for (uint32_t k = 0; k < s.numQuads && first; ++k) {
+                                float t2;
+                                if (k != qi && intersectQuad(mray, s.quads[k], 1e-3f, 1e5f, t2) &&
+                                    (t2 < tq || (t2 == tq && k < qi)))
+                                    first = false;
+                            }
+                            if (first) {
+                                const V3 qn = quadNormal(qd, dir);
+                                const V3 rad = dot(qn, dir) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
+                                const float lpdf = tq * tq / (fabsf(dot(qn, dir)) * qd.area);
+                                if (valid(rad) && lpdf > 0.0f) {
+                                    const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
+                                    // tq > 0: the next float up makes the range (tmin, tq] inclusive
+                                    const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
+                                    mq = ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
+                                                   cm.x, cm.y, cm.z, path | kMisFlag};
+                                    pushMisQuery = true;
+                                }
+                            }
+                        }
+                    }
                 }
 
-                smp.next2(d9, u1, u2);  // continuation, :164-190
-                const BsdfSample ss = bsdfSample(mat, wo, u1, u2);
+                const BsdfSample ss = bsdfSample(mat, wo, d9 ? u[6] : u[5], d9 ? u[5] : u[6]);  // :164-190
                 if (!(ss.pdf > 0.0f && valid(ss.f))) {
                     pushDone = true;
                 } else {
@@ -200,7 +225,8 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                     bool alive = true;
                     if (meta.y > rc.minRR) {
                         const float qrr = fmaxf(0.05f, 1.0f - maxc(thr));
-                        if (smp.next() < qrr) alive = false;
+                        used = 8;
+                        if (u[7] < qrr) alive = false;
                         else thr = thr * (1.0f / (1.0f - qrr));
                     }
                     if (alive) {
@@ -216,7 +242,7 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                         pushDone = true;
                     }
                 }
-                meta.x = smp.dim;
+                meta.x += used;
             }
             ps.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
             ps.meta[path] = meta;
@@ -225,12 +251,14 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
         if (pushExt) qnext[se] = path;
         const uint32_t ss = wavePush(&q.counts[kCntShadow], pushShadow);
         if (pushShadow) q.shadow[ss] = sr;
+        const uint32_t sq = wavePush(&q.counts[kCntShadow], pushMisQuery);
+        if (pushMisQuery) q.shadow[sq] = mq;
         const uint32_t sm = wavePush(&q.counts[kCntMis], pushMis);
         if (pushMis) q.mis[sm] = mr;
         const uint32_t sd = wavePush(&q.counts[kCntDone], pushDone);
         if (pushDone) q.done[sd] = path;
         if (totals) {
-            const uint64_t me = __ballot(pushExt), msh = __ballot(pushShadow), mm = __ballot(pushMis);
+            const uint64_t me = __ballot(pushExt), msh = __ballot(pushShadow), mm = __ballot(pushMis || pushMisQuery);
             if (__lane_id() == 0) {
                 if (me) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotExt]), __popcll(me));
                 if (msh) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotShadow]), __popcll(msh));
@@ -254,9 +282,13 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s, PathState ps, Queues q
         const ShadowRec r = q.shadow[i];
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs)) {
-            float4 L = ps.L[r.path];
-            L.x += r.cr, L.y += r.cg, L.z += r.cb;
-            ps.L[r.path] = L;
+            if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
+                ps.Lmis[r.path & ~kMisFlag] = make_float4(r.cr, r.cg, r.cb, 0.0f);
+            } else {
+                float4 L = ps.L[r.path];
+                L.x += r.cr, L.y += r.cg, L.z += r.cb;
+                ps.L[r.path] = L;
+            }
         }
     }
 }
@@ -272,20 +304,46 @@ __global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, u
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
+    const bool envBlack = !valid(ld3(s.env));
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const MisRec r = q.mis[i];
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
-        const HitRec h = closestHit<W>(s, ray, 1e-3f, 1e5f, lstack + lane, gs);
         V3 rad{0.0f, 0.0f, 0.0f};
         float lpdf = 0.0f;
-        if (h.pos == kHitMiss) {
-            rad = ld3(s.env);
-            lpdf = envPdf(ray.d);
-        } else if ((h.pos & kHitQuad) && r.light == 1u + (h.pos & ~kHitQuad)) {
-            const DevQuad& qd = s.quads[h.pos & ~kHitQuad];
-            const V3 qn = quadNormal(qd, ray.d);
-            rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
-            lpdf = h.t * h.t / (fabsf(dot(qn, ray.d)) * qd.area);
+        if (envBlack && r.light != 0u) {
+            // With a black environment the ray contributes only if the selected quad is its
+            // closest hit. Same decision as closestHit(): the quad must be hit, beat every other
+            // quad in (t, index) order, and no triangle may lie in (tmin, t_q] (triangles win ties).
+            const uint32_t sel = r.light - 1u;
+            const DevQuad& qd = s.quads[sel];
+            float tq;
+            if (intersectQuad(ray, qd, 1e-3f, 1e5f, tq)) {
+                bool first = true;
+                for (uint32_t k = 0; k < s.numQuads && first; ++k) {
+                    float t2;
+                    if (k != sel && intersectQuad(ray, s.quads[k], 1e-3f, 1e5f, t2) && (t2 < tq || (t2 == tq && k < sel)))
+                        first = false;
+                }
+                // tq > 0: the next float up makes the occlusion range (tmin, tq] inclusive
+                const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
+                if (first && !occluded<W>(s, ray, 1e-3f, tqUp, lstack + lane, gs)) {
+                    const V3 qn = quadNormal(qd, ray.d);
+                    rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
+                    lpdf = tq * tq / (fabsf(dot(qn, ray.d)) * qd.area);
+                }
+            }
+        } else {
+            TravCount tc;
+            const HitRec h = closestHit<W>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc);
+            if (h.pos == kHitMiss) {
+                rad = ld3(s.env);
+                lpdf = envPdf(ray.d);
+            } else if ((h.pos & kHitQuad) && r.light == 1u + (h.pos & ~kHitQuad)) {
+                const DevQuad& qd = s.quads[h.pos & ~kHitQuad];
+                const V3 qn = quadNormal(qd, ray.d);
+                rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
+                lpdf = h.t * h.t / (fabsf(dot(qn, ray.d)) * qd.area);
+            }
         }
         if (valid(rad) && lpdf > 0.0f) {
             const float w = powerHeuristic(r.bsdfPdf, lpdf);
@@ -300,37 +358,88 @@ __global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, u
 // ------------------------------------------------------------------------------------------
 // film: accumulate with the box filter, a 2x2 splat into RGBW (FixedSampler/Kernel.cpp:30-58)
 // ------------------------------------------------------------------------------------------
+// Box-filter splat of an aggregated group: `n` samples whose 2x2 footprint starts at (bx, by)
+// add their summed radiance and a weight of n to each covered pixel.
 template <typename F>
-__device__ __forceinline__ void splat(F* rgbw, uint32_t w, uint32_t h, float fx, float fy, V3 L) {
-    const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
+__device__ __forceinline__ void splatGroup(F* rgbw, uint32_t w, uint32_t h, int32_t bx, int32_t by, F r, F g, F b,
+                                           F n) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t x = static_cast<uint32_t>(bx + (k & 1)), y = static_cast<uint32_t>(by + (k >> 1));
         if (!(x < w && y < h)) continue;
         F* p = rgbw + 4 * (static_cast<size_t>(y) * w + x);
-        unsafeAtomicAdd(p + 0, static_cast<F>(L.x));
-        unsafeAtomicAdd(p + 1, static_cast<F>(L.y));
-        unsafeAtomicAdd(p + 2, static_cast<F>(L.z));
-        unsafeAtomicAdd(p + 3, static_cast<F>(1));
+        unsafeAtomicAdd(p + 0, r);
+        unsafeAtomicAdd(p + 1, g);
+        unsafeAtomicAdd(p + 2, b);
+        unsafeAtomicAdd(p + 3, n);
     }
+}
+
+template <typename F>
+__device__ __forceinline__ F waveSum(F v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// accumulate() with the box filter (FixedSampler/Kernel.cpp:30-58): every sample adds (L, 1) to the
+// 2x2 pixels from (int)(p - 0.5). Samples of one pixel sit in neighbouring lanes, so lanes that
+// share a footprint are summed across the wave first and one lane issues the atomics (at most
+// kFilmGroups rounds per wave; any lanes left after that add their own sample).
+constexpr int kFilmGroups = 8;
+
+template <typename F>
+__device__ __forceinline__ void filmAccumulate(F* rgbw, uint32_t w, uint32_t h, bool active, float fx, float fy, V3 L) {
+    const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
+    const uint32_t key = active ? static_cast<uint32_t>(by) * (w + 1u) + static_cast<uint32_t>(bx) : 0xFFFFFFFFu;
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    for (int it = 0; it < kFilmGroups && pending; ++it) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t lk = __shfl(key, leader);
+        const bool match = active && key == lk;
+        const uint64_t mask = __ballot(match);
+        const F r = waveSum<F>(match ? static_cast<F>(L.x) : F(0));
+        const F g = waveSum<F>(match ? static_cast<F>(L.y) : F(0));
+        const F b = waveSum<F>(match ? static_cast<F>(L.z) : F(0));
+        if (lane == static_cast<uint32_t>(leader))
+            splatGroup<F>(rgbw, w, h, bx, by, r, g, b, static_cast<F>(__popcll(mask)));
+        pending &= ~mask;
+    }
+    if ((pending >> lane) & 1ull)
+        splatGroup<F>(rgbw, w, h, bx, by, static_cast<F>(L.x), static_cast<F>(L.y), static_cast<F>(L.z), F(1));
 }
 
 __global__ void __launch_bounds__(256) kFilm(PathState ps, Queues q, RenderConsts rc, float* film32, double* film64,
                                              float* pathL) {
     const uint32_t n = q.counts[kCntDone];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t path = q.done[i];
-        const V3 L = xyz(ps.L[path]);
-        const float2 fp = ps.film[path];
-        if (film64) splat(film64, rc.width, rc.height, fp.x, fp.y, L);
-        else if (film32) splat(film32, rc.width, rc.height, fp.x, fp.y, L);
-        if (pathL) {
-            const uint32_t g = ps.meta[path].w;
-            pathL[3 * static_cast<size_t>(g)] = L.x;
-            pathL[3 * static_cast<size_t>(g) + 1] = L.y;
-            pathL[3 * static_cast<size_t>(g) + 2] = L.z;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {  // block-uniform loop
+        const uint32_t i = i0 + threadIdx.x;
+        const bool active = i < n;
+        uint32_t path = 0;
+        V3 L{0.0f, 0.0f, 0.0f};
+        float2 fp = make_float2(0.0f, 0.0f);
+        if (active) {
+            path = q.done[i];
+            L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
+            fp = ps.film[path];
+            if (pathL) {
+                const uint32_t g = ps.meta[path].w;
+                pathL[3 * static_cast<size_t>(g)] = L.x;
+                pathL[3 * static_cast<size_t>(g) + 1] = L.y;
+                pathL[3 * static_cast<size_t>(g) + 2] = L.z;
+            }
         }
+        if (film64) filmAccumulate<double>(film64, rc.width, rc.height, active, fp.x, fp.y, L);
+        else if (film32) filmAccumulate<float>(film32, rc.width, rc.height, active, fp.x, fp.y, L);
     }
+}
+
+// Plain copy kernel of known size, used to scale HBM byte counters.
+__global__ void __launch_bounds__(256) kCopy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -349,7 +458,8 @@ __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint
         if (occ) {
             occ[i] = occluded<W>(s, ray, r[6], r[7], lstack + lane, gs) ? 1 : 0;
         } else {
-            const HitRec h = closestHit<W>(s, ray, r[6], r[7], lstack + lane, gs);
+            TravCount tc;
+            const HitRec h = closestHit<W>(s, ray, r[6], r[7], lstack + lane, gs, tc);
             t[i] = h.t;
             prim[i] = h.prim;
             uv[2 * i] = h.u;
@@ -397,12 +507,15 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts) {
     const dim3 grid(gridLanes / 64);
     auto* tc = reinterpret_cast<unsigned long long*>(travCounts);
-    if (s.width == 8)
-        hipLaunchKernelGGL(kExtend<8>, grid, dim3(64), 0, st, s, ps, q.ext[curSlot], q.counts + curSlot, spill,
-                           spillPerLane, tc);
-    else
-        hipLaunchKernelGGL(kExtend<4>, grid, dim3(64), 0, st, s, ps, q.ext[curSlot], q.counts + curSlot, spill,
-                           spillPerLane, tc);
+    const uint32_t* queue = q.ext[curSlot];
+    const uint32_t* count = q.counts + curSlot;
+    if (s.width == 8) {
+        if (tc) hipLaunchKernelGGL((kExtend<8, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
+        else hipLaunchKernelGGL((kExtend<8, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
+    } else {
+        if (tc) hipLaunchKernelGGL((kExtend<4, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
+        else hipLaunchKernelGGL((kExtend<4, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
+    }
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals) {
@@ -439,6 +552,11 @@ void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t 
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film) {
     hipLaunchKernelGGL(kCameraRays, dim3(blocksFor(n, 256, 1u << 30)), dim3(256), 0, st, s, rc, pix, n, rays, film);
+}
+void launchCopy(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(kCopy, dim3(blocksFor(n, 256, 8192)), dim3(256), 0, st, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n);
 }
 void launchSobol(hipStream_t st, const DevScene& s, uint32_t m, uint32_t dims, const uint32_t* pix, uint32_t n,
                  float* out, uint64_t* indexOut) {

@@ -12,6 +12,7 @@ struct PathState {
     float4* d;     // ray direction
     float4* thr;   // throughput
     float4* L;     // radiance so far
+    float4* Lmis;  // MIS-BSDF light hit of the current bounce, added to L after its NEE term
     float4* hit;   // t, hit position bits, u, v
     uint64_t* idx; // Sobol index
     uint4* meta;   // next Sobol dim, depth, flags (bit 0: specular), path number within the call
@@ -43,6 +44,7 @@ void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t 
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane);
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film);
+void launchCopy(hipStream_t st, const void* src, void* dst, size_t bytes);
 void launchSobol(hipStream_t st, const DevScene& s, uint32_t m, uint32_t dims, const uint32_t* pix, uint32_t n,
                  float* out, uint64_t* indexOut);
 

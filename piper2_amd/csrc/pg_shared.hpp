@@ -20,6 +20,7 @@ constexpr int kLdsStack = 16;
 
 constexpr uint32_t kHitMiss = 0xFFFFFFFFu;
 constexpr uint32_t kHitQuad = 0x80000000u;
+constexpr uint32_t kMisFlag = 0x80000000u;  // ShadowRec::path bit: MIS-BSDF light query, not NEE
 
 // Wide BVH node, structure-of-arrays inside the node so that one lane streams the whole node with
 // 16-byte loads: W boxes as six planes of W floats, then W child references.
@@ -59,6 +60,7 @@ struct DevScene {
     const DevMaterial* mats;
     const DevQuad* quads;
     const uint32_t* sobol;       // kSobolDims * kSobolCols
+    const uint32_t* sobolT;      // the same matrices bit-major: kSobolCols * kSobolDims
     const uint64_t* sobolInv;    // [kMaxLog2Res+1][32]
     const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
     uint32_t numTris, numNodes, numQuads, width;

@@ -154,6 +154,13 @@ def test_cornell_compat_flags_match_oracle(dev, cornell):
     _path_parity(cornell, dev, flags=flags, rect=(32, 32, 128, 128), spp=8)
 
 
+def test_lit_environment_paths_match_oracle(dev):
+    # a non-black environment takes the closest-hit MIS-BSDF kernel and the environment light
+    sc = scenes.cornell(width=64, height=64, spp=16)
+    sc["env"] = np.array([0.3, 0.4, 0.5], np.float32)
+    _path_parity(sc, dev)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)
