@@ -97,9 +97,10 @@ struct pg_device {
     std::mutex mu;
     bool injectFault = false;
     std::string platform;
-    DevBuf sobol, sobolT, sobolInv, sobolDelta;
+    DevBuf sobol, sobolT, sobolB, sobolInv, sobolDelta;
     // render workspace
     uint32_t capacity = 0;
+    DevBuf recL, recY, quad;  // film records of one segment and per-pixel footprint sums
     DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
     size_t spillBytes = 0;
     std::vector<hipEvent_t> events;
@@ -256,6 +257,9 @@ uint32_t shadeThreads(pg_device* dev, uint64_t work) {
 
 enum Stage { kRaygen, kExtend, kShade, kShadow, kMis, kFilm, kStages };
 
+// Paths whose film records one segment holds; larger sample ranges are rendered in segments.
+constexpr uint64_t kMaxSegmentPaths = 1ull << 28;
+
 void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats) {
     require(s && params, "null scene or params");
     pg_device* dev = s->dev;
@@ -265,20 +269,24 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     require(!film || (film->width == rc.width && film->height == rc.height), "film size differs from params");
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = dev->stream;
-    const uint64_t total = static_cast<uint64_t>(rc.rectW) * rc.rectH * rc.sppCount;
+    const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
+    const uint64_t total = pixels * rc.sppCount;
+    const uint32_t segSpp = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(rc.sppCount, kMaxSegmentPaths / pixels)));
+    const uint64_t segMax = pixels * segSpp;
     uint32_t cap = params->wavefront_paths ? params->wavefront_paths : (1u << 21);
-    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, total));
+    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, segMax));
     const uint32_t lanes = traversalLanes(dev, cap);
     const uint32_t threads = shadeThreads(dev, cap);
     ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * s->spillPerLane * 8);
+    const bool fp64 = rc.flags & pg::kFilmFp64;
+    dev->recL.reserve(segMax * 16), dev->recY.reserve(segMax * 4);
+    dev->quad.reserve(pixels * 16 * (fp64 ? 8 : 4));
     const pg::PathState ps = pathState(dev);
     const pg::Queues q = queues(dev);
-    DevBuf pathL;
-    if (pathLHost) pathL.reserve(total * 12);
     float* film32 = nullptr;
     double* film64 = nullptr;
     if (film) {
-        if (rc.flags & pg::kFilmFp64) {
+        if (fp64) {
             if (!film->rgbw64.p) {
                 film->rgbw64.reserve(static_cast<size_t>(film->width) * film->height * 32);
                 PG_HIP(hipMemsetAsync(film->rgbw64.p, 0, film->rgbw64.bytes, st));
@@ -306,40 +314,59 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
         PG_HIP(hipEventRecord(b, st));
         marks[sg].push_back({a, b});
     };
+    // With a black environment the shade stage resolves MIS-BSDF rays as shadow-queue occlusion
+    // queries, and the closest-hit MIS queue stays empty.
+    const bool envBlack = !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
     uint64_t extendLaunches = 0;
     uint2* spill = dev->spill.as<uint2>();
-    for (uint64_t first = 0; first < total; first += cap) {
-        const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(cap, total - first));
-        timeStage(kRaygen, [&] { pg::launchRaygen(st, s->ds, ps, q, rc, first, n); });
-        uint32_t cur = 0;
-        for (uint32_t iter = 0;; ++iter) {
-            if (iter > rc.maxDepth) {
-                uint32_t left = 0;
-                PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
-                PG_HIP(hipStreamSynchronize(st));
-                if (left == 0) break;
+    float4* recL = dev->recL.as<float4>();
+    float* recY = dev->recY.as<float>();
+    std::vector<float> segHost;
+    for (uint32_t s0 = rc.sppBegin; s0 < rc.sppBegin + rc.sppCount; s0 += segSpp) {
+        pg::RenderConsts seg = rc;
+        seg.sppBegin = s0;
+        seg.sppCount = std::min(segSpp, rc.sppBegin + rc.sppCount - s0);
+        const uint64_t segTotal = pixels * seg.sppCount;
+        for (uint64_t first = 0; first < segTotal; first += cap) {
+            const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(cap, segTotal - first));
+            timeStage(kRaygen, [&] { pg::launchRaygen(st, s->ds, ps, q, seg, first, n); });
+            uint32_t cur = 0;
+            for (uint32_t iter = 0;; ++iter) {
+                if (iter > seg.maxDepth) {
+                    uint32_t left = 0;
+                    PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
+                    PG_HIP(hipStreamSynchronize(st));
+                    if (left == 0) break;
+                }
+                const uint32_t nxt = cur ^ 1u;
+                pg::launchBounceBegin(st, q, nxt);
+                timeStage(kExtend, [&] { pg::launchExtend(st, s->ds, ps, q, cur, lanes, spill, s->spillPerLane, trav); });
+                ++extendLaunches;
+                timeStage(kShade, [&] {
+                    pg::launchShade(st, s->ds, ps, q, seg, cur, nxt, threads, dev->totals.as<uint64_t>());
+                });
+                timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
+                if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
+                timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY); });
+                PG_HIP(hipGetLastError());
+                cur = nxt;
             }
-            const uint32_t nxt = cur ^ 1u;
-            pg::launchBounceBegin(st, q, nxt);
-            timeStage(kExtend, [&] {
-                pg::launchExtend(st, s->ds, ps, q, cur, lanes, spill, s->spillPerLane, trav);
-            });
-            ++extendLaunches;
-            timeStage(kShade, [&] { pg::launchShade(st, s->ds, ps, q, rc, cur, nxt, threads, dev->totals.as<uint64_t>()); });
-            timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-            // With a black environment the shade stage resolves MIS-BSDF rays as shadow-queue
-            // occlusion queries, and the closest-hit MIS queue stays empty.
-            const bool envBlack = !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
-            if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-            timeStage(kFilm, [&] {
-                pg::launchFilm(st, ps, q, rc, threads, film32, film64, pathLHost ? pathL.as<float>() : nullptr);
-            });
-            PG_HIP(hipGetLastError());
-            cur = nxt;
+        }
+        if (film) timeStage(kFilm, [&] { pg::launchResolve(st, seg, recL, recY, dev->quad.p, film32, film64); });
+        PG_HIP(hipGetLastError());
+        if (pathLHost) {  // scatter the segment's records into the caller's path order
+            segHost.resize(segTotal * 4);
+            PG_HIP(hipMemcpyAsync(segHost.data(), recL, segTotal * 16, hipMemcpyDeviceToHost, st));
+            PG_HIP(hipStreamSynchronize(st));
+            for (uint64_t p = 0; p < pixels; ++p)
+                for (uint32_t k = 0; k < seg.sppCount; ++k) {
+                    const float* r = &segHost[4 * (p * seg.sppCount + k)];
+                    float* o = pathLHost + 3 * (p * rc.sppCount + (s0 - rc.sppBegin) + k);
+                    o[0] = r[0], o[1] = r[1], o[2] = r[2];
+                }
         }
     }
     PG_HIP(hipStreamSynchronize(st));
-    if (pathLHost) PG_HIP(hipMemcpy(pathLHost, pathL.p, total * 12, hipMemcpyDeviceToHost));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         uint64_t tot[pg::kTotNum];
@@ -396,6 +423,9 @@ int pg_device_create(int ordinal, pg_device** out) {
         for (uint32_t d = 0; d < pg::kSobolDims; ++d)
             for (uint32_t j = 0; j < pg::kSobolCols; ++j) mt[j * pg::kSobolDims + d] = m[d * pg::kSobolCols + j];
         upload(dev->sobolT, mt.data(), mt.size() * 4, dev->stream);
+        std::vector<uint32_t> mb;
+        pg::buildSobolByteTable(m, mb);
+        upload(dev->sobolB, mb.data(), mb.size() * 4, dev->stream);
         upload(dev->sobolInv, inv.data(), inv.size() * 8, dev->stream);
         upload(dev->sobolDelta, delta.data(), delta.size() * 8, dev->stream);
         PG_HIP(hipStreamSynchronize(dev->stream));
@@ -477,6 +507,7 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         ds.numQuads = desc->num_quads;
         ds.sobol = dev->sobol.as<uint32_t>();
         ds.sobolT = dev->sobolT.as<uint32_t>();
+        ds.sobolB = dev->sobolB.as<uint32_t>();
         ds.sobolInv = dev->sobolInv.as<uint64_t>();
         ds.sobolDelta = dev->sobolDelta.as<uint64_t>();
         const pg_camera& c = desc->camera;

@@ -27,5 +27,7 @@ void buildWideBvh(const float* triVerts, uint32_t numTris, const BvhParams& para
 // Sobol generator matrices (kSobolDims x kSobolCols, u32) and interval-to-index tables
 // inv[(kMaxLog2Res+1) x 32], delta[(kMaxLog2Res+1) x 64].
 void buildSobolTables(std::vector<uint32_t>& matrices, std::vector<uint64_t>& inv, std::vector<uint64_t>& delta);
+// Byte-sliced table (kByteChunks x 256 x kByteDims) from the matrices.
+void buildSobolByteTable(const std::vector<uint32_t>& matrices, std::vector<uint32_t>& table);
 
 }  // namespace pg

@@ -68,6 +68,20 @@ __device__ __forceinline__ float sobolSample(const uint32_t* __restrict__ m, uin
 // (sobolT[bit * kSobolDims + dim]): each set index bit costs 8 independent loads from one 32-byte
 // row segment instead of 8 separate dependent chains. Bit-identical to sobolSample.
 __device__ __forceinline__ void sobolWindow(const DevScene& s, uint64_t index, uint32_t d0, float out[8]) {
+    if (d0 + 8 <= kByteDims) {
+        // Byte-sliced table: one row of 8 dimensions per index byte (4 rows for a 28-bit index),
+        // all loads independent.
+        uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint32_t* __restrict__ t = s.sobolB + d0;
+        for (uint32_t k = 0; index; ++k, index >>= 8) {
+            const uint32_t* row = t + (k * 256 + static_cast<uint32_t>(index & 255u)) * kByteDims;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] ^= row[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[j] = static_cast<float>(v[j]) * 0x1p-32f;
+        return;
+    }
     if (d0 + 8 > kSobolDims) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) out[k] = sobolSample(s.sobol, index, d0 + k);

@@ -31,16 +31,18 @@ __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts&
                                           uint32_t si, uint64_t& idx, FilmPoint& fp) {
     idx = intervalToIndex(s, rc.log2Res, si, px, py);
     const float res = static_cast<float>(1u << rc.log2Res);
+    float u[8];
+    sobolWindow(s, idx, 0, u);
     // sobolStart (ScrambledSobolSampler/Kernel.cpp:52-62)
-    float fx = fminf(fmaxf(sobolSample(s.sobol, idx, 0) * res - static_cast<float>(px), 0.0f), 1.0f);
-    float fy = fminf(fmaxf(sobolSample(s.sobol, idx, 1) * res - static_cast<float>(py), 0.0f), 1.0f);
+    float fx = fminf(fmaxf(u[0] * res - static_cast<float>(px), 0.0f), 1.0f);
+    float fy = fminf(fmaxf(u[1] * res - static_cast<float>(py), 0.0f), 1.0f);
     fx += static_cast<float>(px);
     fy += static_cast<float>(py);
     fp = FilmPoint{fx, fy};
     const float ndcX = rc.ndc[0] + rc.ndc[2] * fx / static_cast<float>(rc.width);
     const float ndcY = rc.ndc[1] + rc.ndc[3] * fy / static_cast<float>(rc.height);
     // dim 2 is the time sample (unused by static scenes); dims 3, 4 drive the lens
-    return cameraRay(s.cam, ndcX, ndcY, sobolSample(s.sobol, idx, 3), sobolSample(s.sobol, idx, 4));
+    return cameraRay(s.cam, ndcX, ndcY, u[3], u[4]);
 }
 
 __global__ void __launch_bounds__(256) kRaygen(DevScene s, PathState ps, Queues q, RenderConsts rc, uint64_t first,
@@ -358,81 +360,82 @@ __global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, u
 // ------------------------------------------------------------------------------------------
 // film: accumulate with the box filter, a 2x2 splat into RGBW (FixedSampler/Kernel.cpp:30-58)
 // ------------------------------------------------------------------------------------------
-// Box-filter splat of an aggregated group: `n` samples whose 2x2 footprint starts at (bx, by)
-// add their summed radiance and a weight of n to each covered pixel.
-template <typename F>
-__device__ __forceinline__ void splatGroup(F* rgbw, uint32_t w, uint32_t h, int32_t bx, int32_t by, F r, F g, F b,
-                                           F n) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t x = static_cast<uint32_t>(bx + (k & 1)), y = static_cast<uint32_t>(by + (k >> 1));
-        if (!(x < w && y < h)) continue;
-        F* p = rgbw + 4 * (static_cast<size_t>(y) * w + x);
-        unsafeAtomicAdd(p + 0, r);
-        unsafeAtomicAdd(p + 1, g);
-        unsafeAtomicAdd(p + 2, b);
-        unsafeAtomicAdd(p + 3, n);
-    }
-}
-
-template <typename F>
-__device__ __forceinline__ F waveSum(F v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// accumulate() with the box filter (FixedSampler/Kernel.cpp:30-58): every sample adds (L, 1) to the
-// 2x2 pixels from (int)(p - 0.5). Samples of one pixel sit in neighbouring lanes, so lanes that
-// share a footprint are summed across the wave first and one lane issues the atomics (at most
-// kFilmGroups rounds per wave; any lanes left after that add their own sample).
-constexpr int kFilmGroups = 8;
-
-template <typename F>
-__device__ __forceinline__ void filmAccumulate(F* rgbw, uint32_t w, uint32_t h, bool active, float fx, float fy, V3 L) {
-    const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
-    const uint32_t key = active ? static_cast<uint32_t>(by) * (w + 1u) + static_cast<uint32_t>(bx) : 0xFFFFFFFFu;
-    const uint32_t lane = __lane_id();
-    uint64_t pending = __ballot(active);
-    for (int it = 0; it < kFilmGroups && pending; ++it) {
-        const int leader = __builtin_ctzll(pending);
-        const uint32_t lk = __shfl(key, leader);
-        const bool match = active && key == lk;
-        const uint64_t mask = __ballot(match);
-        const F r = waveSum<F>(match ? static_cast<F>(L.x) : F(0));
-        const F g = waveSum<F>(match ? static_cast<F>(L.y) : F(0));
-        const F b = waveSum<F>(match ? static_cast<F>(L.z) : F(0));
-        if (lane == static_cast<uint32_t>(leader))
-            splatGroup<F>(rgbw, w, h, bx, by, r, g, b, static_cast<F>(__popcll(mask)));
-        pending &= ~mask;
-    }
-    if ((pending >> lane) & 1ull)
-        splatGroup<F>(rgbw, w, h, bx, by, static_cast<F>(L.x), static_cast<F>(L.y), static_cast<F>(L.z), F(1));
-}
-
-__global__ void __launch_bounds__(256) kFilm(PathState ps, Queues q, RenderConsts rc, float* film32, double* film64,
-                                             float* pathL) {
+// Film: accumulate() with the box filter (RenderDriver/FixedSampler/Kernel.cpp:30-58) adds (L, 1)
+// to the 2x2 pixels starting at (int)(p - 0.5) for every sample. Instead of atomics, each finished
+// path writes a record (L, film point) at its path number; after the render, kQuadSums reduces
+// each source pixel's samples into the four 2x2 footprints they can start at, and kResolve has
+// every film pixel gather the footprints that cover it from its 3x3 source neighbourhood. Every
+// sum runs in a fixed order, so the film is bit-reproducible.
+__global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* recL, float* recY) {
     const uint32_t n = q.counts[kCntDone];
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {  // block-uniform loop
-        const uint32_t i = i0 + threadIdx.x;
-        const bool active = i < n;
-        uint32_t path = 0;
-        V3 L{0.0f, 0.0f, 0.0f};
-        float2 fp = make_float2(0.0f, 0.0f);
-        if (active) {
-            path = q.done[i];
-            L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
-            fp = ps.film[path];
-            if (pathL) {
-                const uint32_t g = ps.meta[path].w;
-                pathL[3 * static_cast<size_t>(g)] = L.x;
-                pathL[3 * static_cast<size_t>(g) + 1] = L.y;
-                pathL[3 * static_cast<size_t>(g) + 2] = L.z;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t path = q.done[i];
+        const V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
+        const float2 fp = ps.film[path];
+        const uint32_t g = ps.meta[path].w;
+        recL[g] = make_float4(L.x, L.y, L.z, fp.x);
+        recY[g] = fp.y;
+    }
+}
+
+// Per source pixel of the rect: sums over its samples for each footprint origin
+// (bx, by) in {px-1, px} x {py-1, py}; quad[4 * pixel + 2 * (by - py + 1) + (bx - px + 1)] = (rgb, count).
+template <typename F>
+__global__ void __launch_bounds__(256) kQuadSums(RenderConsts rc, const float4* __restrict__ recL,
+                                                 const float* __restrict__ recY, F* quad) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= rc.rectW * rc.rectH) return;
+    const int32_t px = static_cast<int32_t>(rc.rectX + pix % rc.rectW);
+    const int32_t py = static_cast<int32_t>(rc.rectY + pix / rc.rectW);
+    F acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = F(0);
+    const size_t base = static_cast<size_t>(pix) * rc.sppCount;
+    for (uint32_t si = 0; si < rc.sppCount; ++si) {
+        const float4 r = recL[base + si];
+        const int32_t bx = static_cast<int32_t>(r.w - 0.5f), by = static_cast<int32_t>(recY[base + si] - 0.5f);
+        const int q = 2 * (by - py + 1) + (bx - px + 1);  // 0..3 (bx in {px-1, px}, by in {py-1, py})
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (q == k) {
+                acc[4 * k] += static_cast<F>(r.x);
+                acc[4 * k + 1] += static_cast<F>(r.y);
+                acc[4 * k + 2] += static_cast<F>(r.z);
+                acc[4 * k + 3] += F(1);
+            }
+    }
+    F* o = quad + 16 * static_cast<size_t>(pix);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) o[k] = acc[k];
+}
+
+// Per film pixel around the rect (rect grown by one pixel, clipped to the film).
+template <typename F>
+__global__ void __launch_bounds__(256) kResolve(RenderConsts rc, const F* __restrict__ quad, F* film) {
+    const int32_t x0 = max(static_cast<int32_t>(rc.rectX) - 1, 0), y0 = max(static_cast<int32_t>(rc.rectY) - 1, 0);
+    const int32_t x1 = min(static_cast<int32_t>(rc.rectX + rc.rectW) + 1, static_cast<int32_t>(rc.width));
+    const int32_t y1 = min(static_cast<int32_t>(rc.rectY + rc.rectH) + 1, static_cast<int32_t>(rc.height));
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = static_cast<uint32_t>(x1 - x0);
+    if (i >= w * static_cast<uint32_t>(y1 - y0)) return;
+    const int32_t tx = x0 + static_cast<int32_t>(i % w), ty = y0 + static_cast<int32_t>(i / w);
+    F r = F(0), g = F(0), b = F(0), n = F(0);
+    for (int32_t sy = ty - 1; sy <= ty + 1; ++sy) {
+        if (sy < static_cast<int32_t>(rc.rectY) || sy >= static_cast<int32_t>(rc.rectY + rc.rectH)) continue;
+        for (int32_t sx = tx - 1; sx <= tx + 1; ++sx) {
+            if (sx < static_cast<int32_t>(rc.rectX) || sx >= static_cast<int32_t>(rc.rectX + rc.rectW)) continue;
+            const F* q = quad + 16 * (static_cast<size_t>(sy - rc.rectY) * rc.rectW + (sx - rc.rectX));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int32_t bx = sx - 1 + (k & 1), by = sy - 1 + (k >> 1);
+                if (tx >= bx && tx <= bx + 1 && ty >= by && ty <= by + 1) {
+                    r += q[4 * k], g += q[4 * k + 1], b += q[4 * k + 2], n += q[4 * k + 3];
+                }
             }
         }
-        if (film64) filmAccumulate<double>(film64, rc.width, rc.height, active, fp.x, fp.y, L);
-        else if (film32) filmAccumulate<float>(film32, rc.width, rc.height, active, fp.x, fp.y, L);
     }
+    F* p = film + 4 * (static_cast<size_t>(ty) * rc.width + tx);
+    p[0] += r, p[1] += g, p[2] += b, p[3] += n;
 }
 
 // Plain copy kernel of known size, used to scale HBM byte counters.
@@ -537,9 +540,25 @@ void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Que
     else
         hipLaunchKernelGGL(kMis<4>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
 }
-void launchFilm(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
-                float* film32, double* film64, float* pathL) {
-    hipLaunchKernelGGL(kFilm, dim3(gridThreads / 256), dim3(256), 0, st, ps, q, rc, film32, film64, pathL);
+void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
+                  float* recY) {
+    hipLaunchKernelGGL(kRecord, dim3(gridThreads / 256), dim3(256), 0, st, ps, q, recL, recY);
+}
+void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
+                   float* film32, double* film64) {
+    const uint32_t pixels = rc.rectW * rc.rectH;
+    const uint32_t targets = (rc.rectW + 2) * (rc.rectH + 2);
+    if (film64) {
+        hipLaunchKernelGGL(kQuadSums<double>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, recL, recY,
+                           static_cast<double*>(quad));
+        hipLaunchKernelGGL(kResolve<double>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
+                           static_cast<const double*>(quad), film64);
+    } else {
+        hipLaunchKernelGGL(kQuadSums<float>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, recL, recY,
+                           static_cast<float*>(quad));
+        hipLaunchKernelGGL(kResolve<float>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
+                           static_cast<const float*>(quad), film32);
+    }
 }
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane) {

@@ -38,8 +38,11 @@ void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const 
                   uint2* spill, uint32_t spillPerLane);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                uint2* spill, uint32_t spillPerLane);
-void launchFilm(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
-                float* film32, double* film64, float* pathL);
+void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
+                  float* recY);
+// quad: 16 floats (or doubles with film64) per rect pixel
+void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
+                   float* film32, double* film64);
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane);
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,

@@ -9,6 +9,10 @@ namespace pg {
 constexpr uint32_t kSobolDims = 1024;  // ScrambledSobolSampler/Shared.hpp:22
 constexpr uint32_t kSobolCols = 52;    // ScrambledSobolSampler/Shared.hpp:23
 constexpr uint32_t kMaxLog2Res = 16;   // interval-to-index tables cover resolutions up to 2^16
+// Byte-sliced Sobol table for the first kByteDims dimensions: entry [k][b][d] is the XOR of the
+// columns 8k..8k+7 of dimension d selected by the bits of byte value b.
+constexpr uint32_t kByteDims = 128;
+constexpr uint32_t kByteChunks = (kSobolCols + 7) / 8;
 
 constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;  // child = kLeafBit | (count-1) << 27 | first triangle
@@ -61,6 +65,7 @@ struct DevScene {
     const DevQuad* quads;
     const uint32_t* sobol;       // kSobolDims * kSobolCols
     const uint32_t* sobolT;      // the same matrices bit-major: kSobolCols * kSobolDims
+    const uint32_t* sobolB;      // byte-sliced: kByteChunks * 256 * kByteDims
     const uint64_t* sobolInv;    // [kMaxLog2Res+1][32]
     const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
     uint32_t numTris, numNodes, numQuads, width;

@@ -72,4 +72,16 @@ void buildSobolTables(std::vector<uint32_t>& mat, std::vector<uint64_t>& inv, st
     }
 }
 
+void buildSobolByteTable(const std::vector<uint32_t>& mat, std::vector<uint32_t>& table) {
+    table.assign(static_cast<size_t>(kByteChunks) * 256 * kByteDims, 0u);
+    for (uint32_t k = 0; k < kByteChunks; ++k)
+        for (uint32_t b = 0; b < 256; ++b)
+            for (uint32_t d = 0; d < kByteDims; ++d) {
+                uint32_t v = 0;
+                for (uint32_t j = 0; j < 8; ++j)
+                    if (((b >> j) & 1u) && 8 * k + j < kSobolCols) v ^= mat[d * kSobolCols + 8 * k + j];
+                table[(static_cast<size_t>(k) * 256 + b) * kByteDims + d] = v;
+            }
+}
+
 }  // namespace pg

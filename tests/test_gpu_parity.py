@@ -180,13 +180,16 @@ def test_fp64_film_and_sample_ranges(dev, cornell):
     assert np.allclose(a, b, rtol=1e-9, atol=1e-9)
 
 
-def test_small_wavefront_capacity_same_paths(dev, cornell):
+def test_small_wavefront_capacity_same_paths_and_film(dev, cornell):
+    # per-path radiance and the resolved film are bit-identical whatever the wave size
     s = pg.Scene(dev, cornell)
     p = pg.render_params(cornell, spp=4)
-    _, a = s.render(p, None, path_l=True)
+    fa, fb = pg.Film(dev, 256, 256), pg.Film(dev, 256, 256)
+    _, a = s.render(p, fa, path_l=True)
     p.wavefront_paths = 1000  # many small waves
-    _, b = s.render(p, None, path_l=True)
+    _, b = s.render(p, fb, path_l=True)
     assert np.array_equal(a, b)
+    assert np.array_equal(fa.download(), fb.download())
 
 
 def test_injected_fault_is_reported(dev, cornell):
