@@ -155,6 +155,32 @@ int pg_sobol(pg_device* dev, uint32_t m, uint32_t dims, const uint32_t* pix, uin
 /* Generator matrices, uint32[1024*52] (the reference's SobolMatrix.dat layout, Shared.hpp:22-23). */
 int pg_sobol_matrices(uint32_t* out);
 
+/* ---- online neural path guiding (no reference counterpart: the reference has no guiding) ----
+ * A 144-64-64-64-64-40 MLP (BF16 weights on the matrix cores, FP32 accumulation) maps a hit's
+ * position (scene box -> [0,1]^3), shading normal and outgoing direction to an 8-lobe von
+ * Mises-Fisher mixture, sampled with probability mis_fraction next to the BSDF (one-sample MIS). */
+typedef struct pg_guide pg_guide;
+typedef struct pg_guide_params {
+    uint32_t seed;           /* weight initialisation */
+    float lr, beta1, beta2, eps; /* Adam */
+    float mis_fraction;      /* probability of drawing the continuation from the guide */
+    uint32_t train_batch;    /* records per Adam step (>= 64) */
+    uint32_t train_records;  /* records a guided render collects for training (0: none) */
+} pg_guide_params;
+int pg_guide_create(pg_device* dev, const pg_guide_params* params, pg_guide** out);
+void pg_guide_destroy(pg_guide* guide);
+uint32_t pg_guide_param_count(void); /* 24880 FP32 parameters: per layer W[out][in] then b[out] */
+int pg_guide_get_params(pg_guide* guide, float* theta, uint32_t n);
+int pg_guide_set_params(pg_guide* guide, const float* theta, uint32_t n); /* also resets Adam */
+/* in[12 n] = per sample (pos01 xyz, -, normal xyz, -, wo xyz, -); mix[48 n] = 8 x (mu xyz, kappa),
+ * then 8 lobe weights, then 8 unused. */
+int pg_guide_infer(pg_guide* guide, const float* in, uint32_t n, float* mix);
+/* Training records rec[16 n] = (pos01 xyz, weight c, normal, -, wo, -, sampled direction, -).
+ * Loss = -sum c log p(direction) / sum c. gradients: FP32 dLoss/dtheta, no update. */
+int pg_guide_gradients(pg_guide* guide, const float* rec, uint32_t n, float* loss, float* grads);
+/* Adam steps over rec in batches of train_batch; *loss = loss of the last batch. */
+int pg_guide_train(pg_guide* guide, const float* rec, uint32_t n, float* loss);
+
 /* ---- diagnostics ---- */
 /* Make the next device call on this handle fail with PG_ERR_DEVICE (host-forced fault path test). */
 int pg_debug_inject_fault(pg_device* dev);

@@ -110,6 +110,14 @@ SIGNATURES = {
     "pg_sobol_matrices": (ctypes.c_int, [_VP]),
     "pg_debug_inject_fault": (ctypes.c_int, [_VP]),
     "pg_debug_copy": (ctypes.c_int, [_VP, ctypes.c_uint64, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "pg_guide_create": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(_VP)]),
+    "pg_guide_destroy": (None, [_VP]),
+    "pg_guide_param_count": (_U32, []),
+    "pg_guide_get_params": (ctypes.c_int, [_VP, _VP, _U32]),
+    "pg_guide_set_params": (ctypes.c_int, [_VP, _VP, _U32]),
+    "pg_guide_infer": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "pg_guide_gradients": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP]),
+    "pg_guide_train": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
 }
 
 _lib = None
@@ -263,6 +271,72 @@ class Scene:
             return st.as_dict(), out
         check(lib().pg_render(self.h, ctypes.byref(params), film.h if film else None, ctypes.byref(st)))
         return st.as_dict()
+
+
+class GuideParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("mis_fraction", ctypes.c_float),
+                ("train_batch", ctypes.c_uint32), ("train_records", ctypes.c_uint32)]
+
+
+class Guide:
+    """Online path-guiding network on the device (pg_guide_*)."""
+
+    def __init__(self, device, seed=0, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8, mis_fraction=0.5,
+                 train_batch=1 << 14, train_records=1 << 18):
+        self.device = device
+        self.params = GuideParams(seed, lr, beta1, beta2, eps, mis_fraction, train_batch, train_records)
+        self.h = ctypes.c_void_p()
+        check(lib().pg_guide_create(device.h, ctypes.byref(self.params), ctypes.byref(self.h)))
+        self.n = lib().pg_guide_param_count()
+
+    def get_params(self):
+        out = np.zeros(self.n, np.float32)
+        check(lib().pg_guide_get_params(self.h, _ptr(out), self.n))
+        return out
+
+    def set_params(self, theta):
+        theta = np.ascontiguousarray(theta, np.float32)
+        check(lib().pg_guide_set_params(self.h, _ptr(theta), self.n))
+
+    def infer(self, pos01, normal, wo):
+        n = pos01.shape[0]
+        inp = np.zeros((n, 12), np.float32)
+        inp[:, 0:3], inp[:, 4:7], inp[:, 8:11] = pos01, normal, wo
+        mix = np.zeros((n, 48), np.float32)
+        check(lib().pg_guide_infer(self.h, _ptr(inp), n, _ptr(mix)))
+        return mix
+
+    @staticmethod
+    def records(pos01, normal, wo, direction, c):
+        n = pos01.shape[0]
+        rec = np.zeros((n, 16), np.float32)
+        rec[:, 0:3], rec[:, 3], rec[:, 4:7], rec[:, 8:11], rec[:, 12:15] = pos01, c, normal, wo, direction
+        return rec
+
+    def gradients(self, rec):
+        rec = np.ascontiguousarray(rec, np.float32)
+        g = np.zeros(self.n, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().pg_guide_gradients(self.h, _ptr(rec), rec.shape[0], _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    def train(self, rec):
+        rec = np.ascontiguousarray(rec, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().pg_guide_train(self.h, _ptr(rec), rec.shape[0], _ptr(loss)))
+        return float(loss[0])
+
+    def close(self):
+        if self.h:
+            lib().pg_guide_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Film:

@@ -14,6 +14,7 @@
 
 #include "../../include/piper_hip.h"
 #include "pg_bvh.hpp"
+#include "pg_guide_kernels.hpp"
 #include "pg_kernels.hpp"
 
 namespace {
@@ -138,6 +139,16 @@ struct pg_film {
     DevBuf rgbw32, rgbw64;
 };
 
+struct pg_guide {
+    pg_device* dev = nullptr;
+    pg_guide_params p{};
+    DevBuf theta, m, v, wb;                  // FP32 masters, Adam moments, BF16 copy
+    DevBuf scratch, partial, tiles, csum, loss, grads;
+    const void* tilesFor = nullptr;           // scratch pointer the uploaded tile table refers to
+    uint32_t tilesNpad = 0;
+    uint32_t step = 0;                        // Adam steps taken
+};
+
 namespace {
 
 std::mutex g_lifetime;
@@ -157,6 +168,65 @@ void releaseChild(pg_device* dev) {
     std::lock_guard<std::mutex> g(g_lifetime);
     if (--dev->children == 0 && dev->closed) freeDevice(dev);
 }
+
+// Guide initialisation: uniform(+-sqrt(6 / (in + out))) weights from a splitmix64 stream, zero
+// biases, zero padding (input features 139..143, output rows 40..47).
+std::vector<float> guideInitParams(uint32_t seed) {
+    std::vector<float> t(pg::kGParamCount, 0.0f);
+    uint64_t state = 0x9E3779B97F4A7C15ull * (seed + 1);
+    auto next = [&] {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return static_cast<double>(z >> 11) * 0x1p-53;
+    };
+    const int outs[5] = {64, 64, 64, 64, 48}, ins[5] = {144, 64, 64, 64, 64};
+    size_t k = 0;
+    for (int l = 0; l < 5; ++l) {
+        const double lim = std::sqrt(6.0 / (ins[l] + outs[l]));
+        for (int o = 0; o < outs[l]; ++o)
+            for (int i = 0; i < ins[l]; ++i) {
+                const double u = (2.0 * next() - 1.0) * lim;
+                const bool pad = (l == 0 && i >= 139) || (l == 4 && o >= 40);
+                t[k++] = pad ? 0.0f : static_cast<float>(u);
+            }
+        k += outs[l];  // biases stay zero
+    }
+    return t;
+}
+
+// One gradient evaluation over n device records; with update, one Adam step.
+void guideStep(pg_guide* g, const float* rec, uint32_t n, bool update) {
+    hipStream_t st = g->dev->stream;
+    const pg::GuideBatchLayout b = pg::guideBatchLayout(n);
+    g->scratch.reserve(pg::guideTrainScratchBytes(b));
+    g->partial.reserve(static_cast<size_t>(b.chunks) * pg::kGParamCount * 4);
+    if (g->tilesFor != g->scratch.p || g->tilesNpad != b.npad) {
+        std::vector<pg::WGradTile> tiles(pg::kGTiles);
+        pg::buildGuideTiles(g->scratch.p, b, tiles.data());
+        g->tiles.reserve(tiles.size() * sizeof(pg::WGradTile));
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(g->tiles.p, tiles.data(), tiles.size() * sizeof(pg::WGradTile), hipMemcpyHostToDevice));
+        g->tilesFor = g->scratch.p, g->tilesNpad = b.npad;
+    }
+    pg::guideConfigureLds();
+    PG_HIP(hipMemsetAsync(g->loss.p, 0, 4, st));
+    pg::launchGuideGradients(st, rec, n, b, g->wb.as<uint16_t>(), g->theta.as<float>(), g->scratch.p, g->csum.as<float>(),
+                             g->partial.as<float>(), g->loss.as<float>(), g->tiles.as<pg::WGradTile>(),
+                             static_cast<uint32_t>(g->dev->numCUs));
+    pg::AdamParams ap{g->p.lr, g->p.beta1, g->p.beta2, g->p.eps, 1.0f, 1.0f, update ? 1u : 0u};
+    if (update) {
+        ++g->step;
+        ap.c1 = 1.0f - std::pow(g->p.beta1, static_cast<float>(g->step));
+        ap.c2 = 1.0f - std::pow(g->p.beta2, static_cast<float>(g->step));
+    }
+    pg::launchGuideAdam(st, g->partial.as<float>(), b.chunks, g->theta.as<float>(), g->m.as<float>(), g->v.as<float>(),
+                        g->wb.as<uint16_t>(), g->grads.as<float>(), ap);
+    PG_HIP(hipGetLastError());
+}
+
+void guideSetLds() { pg::guideConfigureLds(); }
 
 void buildBvh(pg_scene* s, const pg_bvh_params* params) {
     pg::BvhParams bp;
@@ -744,6 +814,125 @@ int pg_debug_copy(pg_device* dev, uint64_t bytes, uint32_t reps, double* gbps) {
         PG_HIP(hipEventElapsedTime(&ms, e0, e1));
         dev->eventsUsed = 0;
         if (gbps) *gbps = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// guiding network
+// ---------------------------------------------------------------------------------------------
+int pg_guide_create(pg_device* dev, const pg_guide_params* params, pg_guide** out) {
+    return guarded([&] {
+        require(dev && params && out, "null argument");
+        require(params->mis_fraction >= 0.0f && params->mis_fraction < 1.0f, "mis_fraction must be in [0, 1)");
+        require(params->lr > 0.0f && params->train_batch >= 64, "bad learning rate or batch");
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        auto g = std::make_unique<pg_guide>();
+        g->dev = dev;
+        g->p = *params;
+        const std::vector<float> theta = guideInitParams(params->seed);
+        upload(g->theta, theta.data(), theta.size() * 4, dev->stream);
+        g->m.reserve(pg::kGParamCount * 4), g->v.reserve(pg::kGParamCount * 4), g->wb.reserve(pg::kGParamCount * 2);
+        PG_HIP(hipMemsetAsync(g->m.p, 0, g->m.bytes, dev->stream));
+        PG_HIP(hipMemsetAsync(g->v.p, 0, g->v.bytes, dev->stream));
+        pg::launchGuideToBf16(dev->stream, g->theta.as<float>(), g->wb.as<uint16_t>());
+        g->csum.reserve(16), g->loss.reserve(16), g->grads.reserve(pg::kGParamCount * 4);
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(dev->stream));
+        addChild(dev);
+        *out = g.release();
+    });
+}
+
+void pg_guide_destroy(pg_guide* g) {
+    if (!g) return;
+    pg_device* dev = g->dev;
+    {
+        std::lock_guard<std::mutex> lock(dev->mu);
+        (void)hipStreamSynchronize(dev->stream);
+        delete g;
+    }
+    releaseChild(dev);
+}
+
+uint32_t pg_guide_param_count(void) { return pg::kGParamCount; }
+
+int pg_guide_get_params(pg_guide* g, float* theta, uint32_t n) {
+    return guarded([&] {
+        require(g && theta && n == pg::kGParamCount, "bad arguments");
+        std::lock_guard<std::mutex> lock(g->dev->mu);
+        g->dev->checkFault();
+        PG_HIP(hipStreamSynchronize(g->dev->stream));
+        PG_HIP(hipMemcpy(theta, g->theta.p, n * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_guide_set_params(pg_guide* g, const float* theta, uint32_t n) {
+    return guarded([&] {
+        require(g && theta && n == pg::kGParamCount, "bad arguments");
+        std::lock_guard<std::mutex> lock(g->dev->mu);
+        g->dev->checkFault();
+        hipStream_t st = g->dev->stream;
+        upload(g->theta, theta, n * 4, st);
+        PG_HIP(hipMemsetAsync(g->m.p, 0, g->m.bytes, st));
+        PG_HIP(hipMemsetAsync(g->v.p, 0, g->v.bytes, st));
+        pg::launchGuideToBf16(st, g->theta.as<float>(), g->wb.as<uint16_t>());
+        g->step = 0;
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int pg_guide_infer(pg_guide* g, const float* in, uint32_t n, float* mix) {
+    return guarded([&] {
+        require(g && (n == 0 || (in && mix)), "null argument");
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lock(g->dev->mu);
+        g->dev->checkFault();
+        hipStream_t st = g->dev->stream;
+        DevBuf din, dmix;
+        upload(din, in, static_cast<size_t>(n) * 48, st);
+        dmix.reserve(static_cast<size_t>(n) * 48 * 4);
+        guideSetLds();
+        pg::launchGuideInferArray(st, din.as<float>(), n, g->wb.as<uint16_t>(), g->theta.as<float>(), dmix.as<float>(),
+                                  static_cast<uint32_t>(g->dev->numCUs) * 2);
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(mix, dmix.p, static_cast<size_t>(n) * 48 * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_guide_gradients(pg_guide* g, const float* rec, uint32_t n, float* loss, float* grads) {
+    return guarded([&] {
+        require(g && rec && n > 0 && grads, "bad arguments");
+        std::lock_guard<std::mutex> lock(g->dev->mu);
+        g->dev->checkFault();
+        hipStream_t st = g->dev->stream;
+        DevBuf drec;
+        upload(drec, rec, static_cast<size_t>(n) * 64, st);
+        guideStep(g, drec.as<float>(), n, false);
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(grads, g->grads.p, pg::kGParamCount * 4, hipMemcpyDeviceToHost));
+        if (loss) PG_HIP(hipMemcpy(loss, g->loss.p, 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_guide_train(pg_guide* g, const float* rec, uint32_t n, float* loss) {
+    return guarded([&] {
+        require(g && rec && n > 0, "bad arguments");
+        std::lock_guard<std::mutex> lock(g->dev->mu);
+        g->dev->checkFault();
+        hipStream_t st = g->dev->stream;
+        DevBuf drec;
+        upload(drec, rec, static_cast<size_t>(n) * 64, st);
+        float last = 0.0f;
+        for (uint32_t b = 0; b < n; b += g->p.train_batch) {
+            const uint32_t cnt = std::min(g->p.train_batch, n - b);
+            guideStep(g, drec.as<float>() + static_cast<size_t>(b) * 16, cnt, true);
+        }
+        PG_HIP(hipStreamSynchronize(st));
+        PG_HIP(hipMemcpy(&last, g->loss.p, 4, hipMemcpyDeviceToHost));
+        if (loss) *loss = last;
     });
 }
 

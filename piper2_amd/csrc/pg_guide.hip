@@ -1,0 +1,436 @@
+// Guiding network kernels for gfx950: inference, fused forward+backward, split-K weight gradients
+// on the matrix cores, and a deterministic reduce + Adam. Network layout: pg_guide.hpp.
+#include "pg_device.hpp"
+#include "pg_guide.hpp"
+#include "pg_guide_kernels.hpp"
+
+namespace pg {
+
+// ------------------------------------------------------------------------------------------
+// Inputs of one sample
+// ------------------------------------------------------------------------------------------
+// From an explicit array: 3 float4 per sample = (pos01, -) (normal, -) (wo, -).
+struct ArraySource {
+    const float4* in;
+    __device__ __forceinline__ GuideInput get(uint32_t i) const {
+        const float4 p = in[3 * i], n = in[3 * i + 1], w = in[3 * i + 2];
+        return guideInput(make_float3(p.x, p.y, p.z), make_float3(n.x, n.y, n.z), make_float3(w.x, w.y, w.z));
+    }
+};
+// From the path state of an extension-queue entry (hit point, shading normal, -ray direction).
+struct PathSource {
+    DevScene s;
+    const float4* o;
+    const float4* d;
+    const float4* hit;
+    const uint32_t* queue;
+    float3 lo, invExt;
+    __device__ __forceinline__ GuideInput get(uint32_t i) const {
+        const uint32_t path = queue[i];
+        const float4 h = hit[path];
+        const uint32_t pos = __float_as_uint(h.y);
+        if (pos == kHitMiss || (pos & kHitQuad)) return guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
+        const float4 oo = o[path], dd = d[path];
+        const V3 dir{dd.x, dd.y, dd.z};
+        const V3 p = V3{oo.x, oo.y, oo.z} + dir * h.x;
+        const V3 n = triNormal(s, pos, dir);
+        const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
+                                       fminf(fmaxf((p.y - lo.y) * invExt.y, 0.0f), 1.0f),
+                                       fminf(fmaxf((p.z - lo.z) * invExt.z, 0.0f), 1.0f));
+        return guideInput(p01, make_float3(n.x, n.y, n.z), make_float3(-dir.x, -dir.y, -dir.z));
+    }
+};
+
+// Per-lane mixture of lobes 2g, 2g+1 of sample s from the output row o[0..47] in LDS.
+struct LobePair {
+    float mx[2], my[2], mz[2], mn[2], r[2], kappa[2], pi[2];
+};
+__device__ __forceinline__ LobePair lobesFromRow(const float* row, int grp) {
+    LobePair L;
+    float a[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const float* o = row + 10 * grp + 5 * q;
+        const float n = fmaxf(sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]), 1e-6f);
+        L.mn[q] = n;
+        L.mx[q] = o[0] / n, L.my[q] = o[1] / n, L.mz[q] = o[2] / n;
+        L.r[q] = o[3];
+        L.kappa[q] = __expf(fminf(fmaxf(o[3], kGKappaLo), kGKappaHi));
+        a[q] = o[4];
+    }
+    // softmax over the 8 lobes held by the 4 lanes of this sample (lanes s, s+16, s+32, s+48)
+    float m = fmaxf(a[0], a[1]);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    const float e0 = __expf(a[0] - m), e1 = __expf(a[1] - m);
+    float sum = e0 + e1;
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    L.pi[0] = e0 / sum, L.pi[1] = e1 / sum;
+    return L;
+}
+
+constexpr int kGRow = kGOut + 1;  // LDS transpose row stride (floats)
+
+// ------------------------------------------------------------------------------------------
+// inference: mixture records for n samples
+// ------------------------------------------------------------------------------------------
+template <typename Src>
+__global__ void __launch_bounds__(256) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
+                                                   const uint16_t* __restrict__ wb, const float* __restrict__ theta,
+                                                   float* __restrict__ mix) {
+    extern __shared__ unsigned char smem[];
+    const uint32_t n = nDev ? *nDev : nHost;
+    const GuideLds L = guideLdsLayout(smem);
+    float* scratch = reinterpret_cast<float*>(smem + kGLdsWeightBytes) + (threadIdx.x / 64) * 16 * kGRow;
+    guideLoadLds(L, wb, theta);
+    __syncthreads();
+    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
+    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {  // block-uniform
+        const uint32_t i = base + (threadIdx.x / 64) * 16 + s;
+        const bool valid = i < n;
+        const GuideInput in = valid ? src.get(i) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
+        bf16x4 x0[9];
+#pragma unroll
+        for (int kt = 0; kt < 9; ++kt) x0[kt] = guideFeatures4(in, kt, grp);
+        f32x4 out[3];
+        guideForwardTile(L, x0, out);
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) scratch[s * kGRow + mt * 16 + 4 * grp + j] = out[mt][j];
+        __syncthreads();
+        const LobePair lp = lobesFromRow(scratch + s * kGRow, grp);
+        if (valid) {
+            float* o = mix + static_cast<size_t>(i) * kGMixFloats;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                reinterpret_cast<float4*>(o)[2 * grp + q] = make_float4(lp.mx[q], lp.my[q], lp.mz[q], lp.kappa[q]);
+                o[32 + 2 * grp + q] = lp.pi[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// training, part 1: forward with saved activations, loss gradient, backward data path.
+// Training record (4 float4): (pos01, c) (normal, -) (wo, -) (sampled direction, -).
+// Stores X_l (layer inputs) and dZ_l (pre-activation gradients) feature-major, BF16, [rows][npad].
+// ------------------------------------------------------------------------------------------
+struct TrainBuffers {
+    uint16_t* x0;  // 144 rows
+    uint16_t* xh;  // 4 x 64 rows: inputs of layers 1..4
+    uint16_t* dz;  // 4 x 64 rows (layers 0..3) + 48 rows (layer 4)
+    uint32_t npad;
+};
+
+__device__ __forceinline__ void storeRows(uint16_t* dst, uint32_t npad, int row0, int grp, uint32_t col, bf16x4 v) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dst[static_cast<size_t>(row0 + 4 * grp + j) * npad + col] = static_cast<uint16_t>(v[j]);
+}
+
+// LDS: forward weights, then transposed W1..W4 ([in = 64][out + 4]), then per-wave scratch.
+constexpr int kGLdsTransBytes = (3 * kGHid * kGLdH + kGHid * kGLdOT) * 2;
+
+__global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ rec, uint32_t n,
+                                                     const uint16_t* __restrict__ wb, const float* __restrict__ theta,
+                                                     const float* __restrict__ csum, TrainBuffers tb, float* lossAcc) {
+    extern __shared__ unsigned char smem[];
+    const GuideLds L = guideLdsLayout(smem);
+    uint16_t* wt = reinterpret_cast<uint16_t*>(smem + kGLdsWeightBytes);  // W1T, W2T, W3T [64][kGLdH], W4T [64][kGLdOT]
+    float* scratch = reinterpret_cast<float*>(smem + kGLdsWeightBytes + kGLdsTransBytes) + (threadIdx.x / 64) * 16 * kGRow;
+    guideLoadLds(L, wb, theta);
+    for (int l = 0; l < 3; ++l) {
+        const int off = l == 0 ? kGW1 : (l == 1 ? kGW2 : kGW3);
+        for (int i = threadIdx.x; i < kGHid * kGHid; i += blockDim.x) {  // W[o][k] -> WT[k][o]
+            const int o = i / kGHid, k = i % kGHid;
+            wt[l * kGHid * kGLdH + k * kGLdH + o] = wb[off + i];
+        }
+    }
+    for (int i = threadIdx.x; i < kGOut * kGHid; i += blockDim.x) {
+        const int o = i / kGHid, k = i % kGHid;
+        wt[3 * kGHid * kGLdH + k * kGLdOT + o] = wb[kGW4 + i];
+    }
+    __syncthreads();
+    const float invC = 1.0f / fmaxf(*csum, 1e-30f);
+    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
+    float lossPart = 0.0f;
+    for (uint32_t base = blockIdx.x * 64; base < tb.npad; base += gridDim.x * 64) {  // block-uniform
+        const uint32_t i = base + (threadIdx.x / 64) * 16 + s;
+        const bool valid = i < n;
+        float4 r0 = make_float4(0, 0, 0, 0), r1 = make_float4(0, 0, 1, 0), r2 = make_float4(0, 0, 1, 0),
+               r3 = make_float4(0, 0, 1, 0);
+        if (valid) r0 = rec[4 * i], r1 = rec[4 * i + 1], r2 = rec[4 * i + 2], r3 = rec[4 * i + 3];
+        const GuideInput in = guideInput(make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z),
+                                         make_float3(r2.x, r2.y, r2.z));
+        const float c = valid ? r0.w * invC : 0.0f;
+        bf16x4 x0[9];
+#pragma unroll
+        for (int kt = 0; kt < 9; ++kt) {
+            x0[kt] = guideFeatures4(in, kt, grp);
+            storeRows(tb.x0, tb.npad, kt * 16, grp, i, x0[kt]);
+        }
+        bf16x4 h[4][4];
+        f32x4 out[3];
+        guideForwardTile(L, x0, out, h);
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) storeRows(tb.xh + static_cast<size_t>(l) * kGHid * tb.npad, tb.npad, mt * 16, grp, i, h[l][mt]);
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) scratch[s * kGRow + mt * 16 + 4 * grp + j] = out[mt][j];
+        __syncthreads();
+        // mixture, loss and gradient with respect to the 40 outputs (lobes 2g, 2g+1 in this lane)
+        const LobePair lp = lobesFromRow(scratch + s * kGRow, grp);
+        const float wx = r3.x, wy = r3.y, wz = r3.z;
+        float v[2], cs[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cs[q] = lp.mx[q] * wx + lp.my[q] * wy + lp.mz[q] * wz;
+            v[q] = vmfPdf(lp.kappa[q], cs[q]);
+        }
+        float p = lp.pi[0] * v[0] + lp.pi[1] * v[1];
+        p += __shfl_xor(p, 16);
+        p += __shfl_xor(p, 32);
+        p = fmaxf(p, 1e-30f);
+        if (grp == 0) lossPart += -c * logf(p);
+        const float dLdp = -c / p;
+        // softmax: dL/da_i = pi_i (g_i - sum_j pi_j g_j), g_i = dL/dp v_i
+        float gpi = lp.pi[0] * dLdp * v[0] + lp.pi[1] * dLdp * v[1];
+        gpi += __shfl_xor(gpi, 16);
+        gpi += __shfl_xor(gpi, 32);
+        __syncthreads();
+        float* row = scratch + s * kGRow;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float dv = dLdp * lp.pi[q];
+            const float k = lp.kappa[q];
+            const float dlogC = 1.0f / k - 2.0f / expm1f(2.0f * k);
+            const float dk = dv * v[q] * (dlogC + cs[q] - 1.0f);
+            const float dr = (lp.r[q] > kGKappaLo && lp.r[q] < kGKappaHi) ? dk * k : 0.0f;
+            const float sc = dv * v[q] * k;  // dL/dmu = sc * w
+            const float dmx = sc * wx, dmy = sc * wy, dmz = sc * wz;
+            const float proj = lp.mx[q] * dmx + lp.my[q] * dmy + lp.mz[q] * dmz;
+            float* o = row + 10 * grp + 5 * q;
+            o[0] = (dmx - lp.mx[q] * proj) / lp.mn[q];
+            o[1] = (dmy - lp.my[q] * proj) / lp.mn[q];
+            o[2] = (dmz - lp.mz[q] * proj) / lp.mn[q];
+            o[3] = dr;
+            o[4] = lp.pi[q] * (dLdp * v[q] - gpi);
+        }
+        if (grp == 0)
+            for (int k = kGOutUsed; k < kGOut; ++k) row[k] = 0.0f;
+        __syncthreads();
+        // backward data path: dZ4 -> dZ3 ... dZ0
+        bf16x4 dz[4];
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt) {
+            f32x4 t;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) t[j] = scratch[s * kGRow + mt * 16 + 4 * grp + j];
+            dz[mt] = pack(t);
+            storeRows(tb.dz + 4ull * kGHid * tb.npad, tb.npad, mt * 16, grp, i, dz[mt]);
+        }
+#pragma unroll
+        for (int l = 3; l >= 0; --l) {
+            // dH_l (64 x 16) = W_{l+1}^T dZ_{l+1}; W^T rows = this layer's units
+            const uint16_t* w = wt + l * kGHid * kGLdH;
+            const int ld = l == 3 ? kGLdOT : kGLdH;
+            const int kts = l == 3 ? 3 : 4;
+            f32x4 acc[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt)
+                    if (kt < kts) acc[mt] = mfma16(ldsA(w, ld, mt * 16 + s, kt * 16 + 4 * grp), dz[kt], acc[mt]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                f32x4 t;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)  // ReLU'(z) from h = relu(z): nonzero magnitude
+                    t[j] = (static_cast<uint16_t>(h[l][mt][j]) & 0x7FFFu) != 0 ? acc[mt][j] : 0.0f;
+                dz[mt] = pack(t);
+                storeRows(tb.dz + static_cast<size_t>(l) * kGHid * tb.npad, tb.npad, mt * 16, grp, i, dz[mt]);
+            }
+        }
+    }
+    if (lossAcc) {
+        float t = lossPart;
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (lane == 0) atomicAdd(lossAcc, t);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// training, part 2: dW_l = dZ_l X_l^T over the batch (split K = samples), bias sums; partials per
+// sample chunk. One wave = one 16x16 output tile over one chunk.
+// ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(64) kGuideWGrad(const WGradTile* __restrict__ tiles, uint32_t npad, uint32_t chunk,
+                                                  float* __restrict__ partial) {
+    const WGradTile t = tiles[blockIdx.x];
+    const uint32_t c = blockIdx.y;
+    const int lane = threadIdx.x, s = lane & 15, grp = lane >> 4;
+    const uint16_t* dzRow = t.dz + static_cast<size_t>(t.mo * 16 + s) * npad;
+    const uint16_t* xRow = t.x + static_cast<size_t>(t.ki * 16 + s) * npad;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    const uint32_t n0 = c * chunk, n1 = n0 + chunk;
+    for (uint32_t n = n0; n < n1; n += 16) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(dzRow + n + 4 * grp);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(xRow + n + 4 * grp);
+        acc = mfma16(a, b, acc);
+        if (t.ki == 0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bsum += fromBf16(static_cast<uint16_t>(a[j]));
+    }
+    float* out = partial + static_cast<size_t>(c) * kGParams;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[t.wOff + (t.mo * 16 + 4 * grp + j) * t.cols + t.ki * 16 + s] = acc[j];
+    if (t.ki == 0) {
+        bsum += __shfl_xor(bsum, 16);
+        bsum += __shfl_xor(bsum, 32);
+        if (grp == 0) out[t.bOff + t.mo * 16 + s] = bsum;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// training, part 3: sum the chunk partials in a fixed order; optionally Adam-update the FP32
+// masters and refresh the BF16 copy.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kGuideAdam(const float* __restrict__ partial, uint32_t chunks, float* theta,
+                                                  float* m, float* v, uint16_t* wb, float* gradOut, AdamParams ap) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kGParams) return;
+    float g = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) g += partial[static_cast<size_t>(c) * kGParams + p];
+    if (gradOut) gradOut[p] = g;
+    if (!ap.update) return;
+    const float mm = ap.b1 * m[p] + (1.0f - ap.b1) * g;
+    const float vv = ap.b2 * v[p] + (1.0f - ap.b2) * g * g;
+    m[p] = mm, v[p] = vv;
+    const float mh = mm / ap.c1, vh = vv / ap.c2;
+    const float th = theta[p] - ap.lr * mh / (sqrtf(vh) + ap.eps);
+    theta[p] = th;
+    wb[p] = toBf16(th);
+}
+
+__global__ void kGuideToBf16(const float* theta, uint16_t* wb) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kGParams) wb[p] = toBf16(theta[p]);
+}
+
+// Deterministic sum of the records' weights c (one block).
+__global__ void __launch_bounds__(256) kGuideSumC(const float4* rec, uint32_t n, float* out) {
+    __shared__ float part[256];
+    float t = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) t += rec[4 * i].w;
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+// ------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------
+constexpr size_t kInferLds = kGLdsWeightBytes + 4 * 16 * kGRow * 4;
+constexpr size_t kTrainLds = kGLdsWeightBytes + kGLdsTransBytes + 4 * 16 * kGRow * 4;
+
+static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {
+    const uint32_t b = (n + per - 1) / per;
+    return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const uint16_t* wb, const float* theta,
+                           float* mix, uint32_t maxBlocks) {
+    hipLaunchKernelGGL(kGuideInfer<ArraySource>, dim3(gridFor(n, 64, maxBlocks)), dim3(256), kInferLds, st,
+                       ArraySource{reinterpret_cast<const float4*>(in)}, n, nullptr, wb, theta, mix);
+}
+
+void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
+                           const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
+                           const float* theta, float* mix, uint32_t maxBlocks) {
+    PathSource src{s, ps.o, ps.d, ps.hit, queue, lo, invExt};
+    hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, 64, maxBlocks)), dim3(256), kInferLds, st, src,
+                       capacity, count, wb, theta, mix);
+}
+
+GuideBatchLayout guideBatchLayout(uint32_t n) {
+    const uint32_t n64 = (n + 63) / 64 * 64;
+    const uint32_t perChunk = (n64 + kGMaxChunks - 1) / kGMaxChunks;
+    GuideBatchLayout b;
+    b.chunk = (perChunk + 63) / 64 * 64;
+    if (b.chunk == 0) b.chunk = 64;
+    b.chunks = (n64 + b.chunk - 1) / b.chunk;
+    if (b.chunks == 0) b.chunks = 1;
+    b.npad = b.chunks * b.chunk;
+    return b;
+}
+
+size_t guideTrainScratchBytes(const GuideBatchLayout& b) {
+    const size_t rows = kGIn + 4 * kGHid + 4 * kGHid + kGOut;
+    return rows * b.npad * 2;
+}
+
+void buildGuideTiles(void* scratch, const GuideBatchLayout& b, WGradTile* host) {
+    const uint32_t npad = b.npad;
+    uint16_t* x0 = static_cast<uint16_t*>(scratch);
+    uint16_t* xh = x0 + static_cast<size_t>(kGIn) * npad;
+    uint16_t* dz = xh + static_cast<size_t>(4 * kGHid) * npad;
+    int t = 0;
+    const uint32_t wOff[5] = {kGW0, kGW1, kGW2, kGW3, kGW4}, bOff[5] = {kGB0, kGB1, kGB2, kGB3, kGB4};
+    for (int l = 0; l < 5; ++l) {
+        const int rows = l == 4 ? kGOut : kGHid, cols = l == 0 ? kGIn : kGHid;
+        const uint16_t* x = l == 0 ? x0 : xh + static_cast<size_t>(l - 1) * kGHid * npad;
+        const uint16_t* d = dz + static_cast<size_t>(l) * kGHid * npad;
+        for (int mo = 0; mo < rows / 16; ++mo)
+            for (int ki = 0; ki < cols / 16; ++ki)
+                host[t++] = WGradTile{static_cast<uint16_t>(rows), static_cast<uint16_t>(cols), static_cast<uint16_t>(mo),
+                                      static_cast<uint16_t>(ki), wOff[l], bOff[l], d, x};
+    }
+}
+
+void launchGuideGradients(hipStream_t st, const float* rec, uint32_t n, const GuideBatchLayout& b, const uint16_t* wb,
+                          const float* theta, void* scratch, float* csum, float* partial, float* lossAcc,
+                          const WGradTile* tilesDev, uint32_t maxBlocks) {
+    uint16_t* base = static_cast<uint16_t*>(scratch);
+    TrainBuffers tb{base, base + static_cast<size_t>(kGIn) * b.npad, base + static_cast<size_t>(kGIn + 4 * kGHid) * b.npad,
+                    b.npad};
+    hipLaunchKernelGGL(kGuideSumC, dim3(1), dim3(256), 0, st, reinterpret_cast<const float4*>(rec), n, csum);
+    hipLaunchKernelGGL(kGuideTrainFB, dim3(gridFor(b.npad, 64, maxBlocks)), dim3(256), kTrainLds, st,
+                       reinterpret_cast<const float4*>(rec), n, wb, theta, csum, tb, lossAcc);
+    hipLaunchKernelGGL(kGuideWGrad, dim3(kGTiles, b.chunks), dim3(64), 0, st, tilesDev, b.npad, b.chunk, partial);
+}
+
+void launchGuideAdam(hipStream_t st, const float* partial, uint32_t chunks, float* theta, float* m, float* v,
+                     uint16_t* wb, float* gradOut, const AdamParams& ap) {
+    hipLaunchKernelGGL(kGuideAdam, dim3((kGParams + 255) / 256), dim3(256), 0, st, partial, chunks, theta, m, v, wb,
+                       gradOut, ap);
+}
+
+void guideConfigureLds() {
+    static bool done = false;
+    if (done) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<ArraySource>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<PathSource>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideTrainFB), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(kTrainLds));
+    done = true;
+}
+
+void launchGuideToBf16(hipStream_t st, const float* theta, uint16_t* wb) {
+    hipLaunchKernelGGL(kGuideToBf16, dim3((kGParams + 255) / 256), dim3(256), 0, st, theta, wb);
+}
+
+}  // namespace pg

@@ -1,0 +1,57 @@
+// Launchers of the guiding-network kernels (pg_guide.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pg_kernels.hpp"
+#include "pg_shared.hpp"
+
+namespace pg {
+
+constexpr uint32_t kGTiles = 96;       // 16x16 weight-gradient tiles over the five layers
+constexpr uint32_t kGMaxChunks = 64;   // sample chunks of the split-K weight gradient
+constexpr uint32_t kGParamCount = 24880;
+
+struct WGradTile {
+    uint16_t rows, cols, mo, ki;  // layer shape and tile coordinates
+    uint32_t wOff, bOff;          // parameter offsets
+    const uint16_t* dz;
+    const uint16_t* x;
+};
+
+struct AdamParams {
+    float lr, b1, b2, eps;
+    float c1, c2;  // bias corrections 1 - b1^t, 1 - b2^t
+    uint32_t update;
+};
+
+// Sample layout of one training batch of n records: row stride npad (a multiple of 64) split into
+// `chunks` chunks of `chunk` samples.
+struct GuideBatchLayout {
+    uint32_t npad, chunk, chunks;
+};
+GuideBatchLayout guideBatchLayout(uint32_t n);
+size_t guideTrainScratchBytes(const GuideBatchLayout& b);
+void buildGuideTiles(void* scratch, const GuideBatchLayout& b, WGradTile* host);
+
+// Mixture records (48 floats per sample) for n samples given as 3 float4 each (pos01, normal, wo).
+void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const uint16_t* wb, const float* theta,
+                           float* mix, uint32_t maxBlocks);
+// Mixture records for the surface hits of an extension queue (count read on the device); record i
+// belongs to queue entry i.
+void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
+                           const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
+                           const float* theta, float* mix, uint32_t maxBlocks);
+// Forward + backward over n training records (4 float4 each: (pos01, c) (normal) (wo) (direction)),
+// then the split-K weight gradients into partial[chunks][kGParamCount].
+void launchGuideGradients(hipStream_t st, const float* rec, uint32_t n, const GuideBatchLayout& b, const uint16_t* wb,
+                          const float* theta, void* scratch, float* csum, float* partial, float* lossAcc,
+                          const WGradTile* tilesDev, uint32_t maxBlocks);
+void launchGuideAdam(hipStream_t st, const float* partial, uint32_t chunks, float* theta, float* m, float* v,
+                     uint16_t* wb, float* gradOut, const AdamParams& ap);
+void launchGuideToBf16(hipStream_t st, const float* theta, uint16_t* wb);
+// Raise the dynamic-LDS limit of the network kernels (they use 64-96 KB per block); idempotent.
+void guideConfigureLds();
+
+}  // namespace pg

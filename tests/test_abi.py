@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "piper_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(pg_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char\*)\s+(pg_\w+)\s*\(", src, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
