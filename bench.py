@@ -55,6 +55,68 @@ def cpu_baseline(sc, threads, budget_s=12.0):
                       f"({paths} paths, {dt:.1f} s) of the same 1920x1080 frame"}
 
 
+def _img(f):
+    import numpy as np
+    return f[..., :3] / np.maximum(f[..., 3:], 1e-30)
+
+
+def _rel_mse(a, b):
+    import numpy as np
+    return float(np.mean((a - b) ** 2 / (b * b + 0.01)))
+
+
+def time_to_relmse(sc, scene, film, ref, render, target=0.01, max_level=14):
+    """Progressive render with doubling sample counts (1, 2, 4, ... spp accumulated in one film);
+    returns (seconds to reach `target` relMSE, interpolated in log-log between levels, trace)."""
+    import math
+    film.clear()
+    t, prev, trace = 0.0, None, []
+    for k in range(max_level + 1):
+        s0, s1 = (0, 1) if k == 0 else (1 << (k - 1), 1 << k)
+        st = render(pg_params(sc, (s0, s1)))
+        t += st["ms_total"] / 1e3
+        e = _rel_mse(_img(film.download()), ref)
+        trace.append((s1, round(t, 4), round(e, 5)))
+        if e <= target:
+            if prev is None or prev[1] <= e:
+                return t, trace
+            (t0, e0) = prev
+            f = (math.log(target) - math.log(e0)) / (math.log(e) - math.log(e0))
+            return math.exp(math.log(t0) + f * (math.log(t) - math.log(t0))), trace
+        prev = (t, e)
+    return None, trace
+
+
+def pg_params(sc, spp_range):
+    import piper2_amd as pg
+    return pg.render_params(sc, spp=1 << 16, spp_range=spp_range)
+
+
+def guided_vs_plain(dev, sc, ref_spp, label):
+    """Time to 1% relMSE for the plain tracer and the online-trained guided tracer on one scene."""
+    import piper2_amd as pg
+    scene = pg.Scene(dev, sc)
+    film = pg.Film(dev, sc["width"], sc["height"])
+    t0 = time.perf_counter()
+    for s0 in range(0, ref_spp, 1024):  # reference: plain, FP64 film
+        scene.render(pg.render_params(sc, spp=1 << 16, spp_range=(s0, min(s0 + 1024, ref_spp)), flags=pg.PG_FILM_FP64), film)
+    ref = _img(film.download(fp64=True))
+    log(f"{label}: {ref_spp}-spp FP64 reference in {time.perf_counter() - t0:.1f}s")
+    film.clear()
+    t_plain, tr_plain = time_to_relmse(sc, scene, film, ref, lambda p: scene.render(p, film))
+    guide = pg.Guide(dev, seed=1, lr=1e-2, mis_fraction=0.5, train_batch=1 << 14, train_records=1 << 18)
+    t_guided, tr_guided = time_to_relmse(sc, scene, film, ref, lambda p: scene.render_guided(guide, p, film))
+    log(f"{label}: plain {t_plain} s {tr_plain}")
+    log(f"{label}: guided {t_guided} s {tr_guided}")
+    out = {"plain_s": t_plain and round(t_plain, 4), "guided_s": t_guided and round(t_guided, 4),
+           "reference_spp": ref_spp, "resolution": [sc["width"], sc["height"]],
+           "plain_trace_spp_s_relmse": tr_plain, "guided_trace_spp_s_relmse": tr_guided}
+    if t_plain and t_guided:
+        out["guided_speedup"] = round(t_plain / t_guided, 3)
+    guide.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +126,8 @@ def main():
     ap.add_argument("--wavefront-paths", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-guided", action="store_true")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the render shards by tiles or "
@@ -147,6 +211,14 @@ def main():
                      "tris_per_ray": round(tris_per_ray, 3), "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "launches": launches},
     }
+    if not args.no_guided:
+        # time to 1% relMSE, guided (online training, one-sample MIS at 0.5) vs plain
+        result["time_to_1pct_relmse"] = {
+            "hard_room_256x256_depth12": guided_vs_plain(dev, scenes.hard_room(seed=0, width=256, height=256),
+                                                         args.ref_spp, "hard room"),
+            "terrain_slit_480x270_depth8": guided_vs_plain(dev, scenes.terrain_slit(width=480, height=270),
+                                                           args.ref_spp // 4, "terrain slit"),
+        }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(sc, threads)

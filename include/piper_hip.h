@@ -108,6 +108,8 @@ typedef struct pg_stats {
     double ms_total;   /* host wall time of the call, stream synchronised */
     double ms_raygen, ms_extend, ms_shade, ms_shadow, ms_mis, ms_film; /* HIP-event time per stage */
     uint64_t nodes_visited, tris_tested; /* extension rays, with PG_COUNT_TRAVERSAL only */
+    double ms_guide, ms_train;           /* guided renders: network inference, online training */
+    uint64_t train_records;              /* guided renders: records trained on after the render */
 } pg_stats;
 
 /* ---- device (Accelerator: enumerateContexts / getNativePlatform, Accelerator.hpp:130-161) ---- */
@@ -180,6 +182,12 @@ int pg_guide_infer(pg_guide* guide, const float* in, uint32_t n, float* mix);
 int pg_guide_gradients(pg_guide* guide, const float* rec, uint32_t n, float* loss, float* grads);
 /* Adam steps over rec in batches of train_batch; *loss = loss of the last batch. */
 int pg_guide_train(pg_guide* guide, const float* rec, uint32_t n, float* loss);
+/* pg_render with guided continuation sampling (one-sample MIS between the guide mixture, with
+ * probability mis_fraction, and the BSDF). With train_records > 0 the render also collects that
+ * many training records from its own paths and takes Adam steps on them after the render, so a
+ * progressive driver trains online pass by pass while every pass stays unbiased. */
+int pg_render_guided(pg_scene* scene, pg_guide* guide, const pg_render_params* params, pg_film* film,
+                     pg_stats* stats);
 
 /* ---- diagnostics ---- */
 /* Make the next device call on this handle fail with PG_ERR_DEVICE (host-forced fault path test). */

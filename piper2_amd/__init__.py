@@ -78,6 +78,7 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double), ("ms_raygen", ctypes.c_double), ("ms_extend", ctypes.c_double),
         ("ms_shade", ctypes.c_double), ("ms_shadow", ctypes.c_double), ("ms_mis", ctypes.c_double),
         ("ms_film", ctypes.c_double), ("nodes_visited", ctypes.c_uint64), ("tris_tested", ctypes.c_uint64),
+        ("ms_guide", ctypes.c_double), ("ms_train", ctypes.c_double), ("train_records", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -118,6 +119,7 @@ SIGNATURES = {
     "pg_guide_infer": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "pg_guide_gradients": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP]),
     "pg_guide_train": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "pg_render_guided": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
 }
 
 _lib = None
@@ -270,6 +272,12 @@ class Scene:
                                         ctypes.byref(st)))
             return st.as_dict(), out
         check(lib().pg_render(self.h, ctypes.byref(params), film.h if film else None, ctypes.byref(st)))
+        return st.as_dict()
+
+    def render_guided(self, guide, params, film=None):
+        st = Stats()
+        check(lib().pg_render_guided(self.h, guide.h, ctypes.byref(params), film.h if film else None,
+                                     ctypes.byref(st)))
         return st.as_dict()
 
 

@@ -102,6 +102,7 @@ struct pg_device {
     // render workspace
     uint32_t capacity = 0;
     DevBuf recL, recY, quad;  // film records of one segment and per-pixel footprint sums
+    DevBuf mix, hist, trainRec, trainCount;  // guiding: mixtures, vertex history, training records
     DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
     size_t spillBytes = 0;
     std::vector<hipEvent_t> events;
@@ -131,6 +132,7 @@ struct pg_scene {
     pg::DevScene ds{};
     DevBuf tris, triMatLeaf, nodes, mats, quads;
     uint32_t spillPerLane = 1;
+    float lo[3] = {0, 0, 0}, invExt[3] = {1, 1, 1};  // scene box (triangles and light quads) -> [0,1]^3
 };
 
 struct pg_film {
@@ -325,14 +327,16 @@ uint32_t shadeThreads(pg_device* dev, uint64_t work) {
     return static_cast<uint32_t>(std::max<uint64_t>(256, std::min(full, need)));
 }
 
-enum Stage { kRaygen, kExtend, kShade, kShadow, kMis, kFilm, kStages };
+enum Stage { kRaygen, kExtend, kShade, kShadow, kMis, kFilm, kGuide, kTrain, kStages };
 
 // Paths whose film records one segment holds; larger sample ranges are rendered in segments.
 constexpr uint64_t kMaxSegmentPaths = 1ull << 28;
 
-void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats) {
+void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats,
+            pg_guide* guide = nullptr) {
     require(s && params, "null scene or params");
     pg_device* dev = s->dev;
+    require(!guide || guide->dev == dev, "guide and scene live on different devices");
     std::lock_guard<std::mutex> lock(dev->mu);
     dev->checkFault();
     const pg::RenderConsts rc = makeConsts(*params);
@@ -353,6 +357,31 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     dev->quad.reserve(pixels * 16 * (fp64 ? 8 : 4));
     const pg::PathState ps = pathState(dev);
     const pg::Queues q = queues(dev);
+    // guiding: mixture per extension-queue entry, per-slot vertex history of training paths
+    pg::GuideShade gs{};
+    pg::GuideRecordOut tr{};
+    if (guide) {
+        gs.enabled = 1;
+        gs.alpha = guide->p.mis_fraction;
+        gs.lo = make_float3(s->lo[0], s->lo[1], s->lo[2]);
+        gs.invExt = make_float3(s->invExt[0], s->invExt[1], s->invExt[2]);
+        dev->mix.reserve(static_cast<size_t>(cap) * pg::kGMixFloatsHost * 4);
+        gs.mix = dev->mix.as<float>();
+        if (guide->p.train_records > 0) {
+            gs.train = 1;
+            gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(rc.maxDepth, 255));
+            // about three guided vertices per path: pick every stride-th path for training
+            gs.trainStride = static_cast<uint32_t>(std::max<uint64_t>(1, total * 3 / guide->p.train_records));
+            gs.trainPhase = gs.trainStride / 2;
+            dev->hist.reserve(static_cast<size_t>(cap) * gs.histDepth * 6 * 16);
+            gs.hist = dev->hist.as<float4>();
+            dev->trainRec.reserve(static_cast<size_t>(guide->p.train_records) * 64);
+            dev->trainCount.reserve(16);
+            PG_HIP(hipMemsetAsync(dev->trainCount.p, 0, 4, st));
+            tr = pg::GuideRecordOut{dev->trainRec.as<float4>(), dev->trainCount.as<uint32_t>(), guide->p.train_records};
+        }
+        pg::guideConfigureLds();
+    }
     float* film32 = nullptr;
     double* film64 = nullptr;
     if (film) {
@@ -412,12 +441,18 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
                 pg::launchBounceBegin(st, q, nxt);
                 timeStage(kExtend, [&] { pg::launchExtend(st, s->ds, ps, q, cur, lanes, spill, s->spillPerLane, trav); });
                 ++extendLaunches;
+                if (guide)
+                    timeStage(kGuide, [&] {
+                        pg::launchGuideInferPaths(st, s->ds, ps, q.ext[cur], q.counts + cur, n, gs.lo, gs.invExt,
+                                                  guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
+                                                  static_cast<uint32_t>(dev->numCUs) * 2);
+                    });
                 timeStage(kShade, [&] {
-                    pg::launchShade(st, s->ds, ps, q, seg, cur, nxt, threads, dev->totals.as<uint64_t>());
+                    pg::launchShade(st, s->ds, ps, q, seg, cur, nxt, threads, dev->totals.as<uint64_t>(), gs);
                 });
                 timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
                 if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-                timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY); });
+                timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY, gs, tr); });
                 PG_HIP(hipGetLastError());
                 cur = nxt;
             }
@@ -436,9 +471,21 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
                 }
         }
     }
+    uint32_t trained = 0;
+    if (gs.train) {  // online training on the records this render collected
+        uint32_t got = 0;
+        PG_HIP(hipMemcpyAsync(&got, tr.count, 4, hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        trained = std::min(got, tr.capacity);
+        for (uint32_t b = 0; b + 64 <= trained; b += guide->p.train_batch) {
+            const uint32_t cnt = std::min(guide->p.train_batch, trained - b);
+            timeStage(kTrain, [&] { guideStep(guide, dev->trainRec.as<float>() + static_cast<size_t>(b) * 16, cnt, true); });
+        }
+    }
     PG_HIP(hipStreamSynchronize(st));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
+        stats->train_records = trained;
         uint64_t tot[pg::kTotNum];
         PG_HIP(hipMemcpy(tot, dev->totals.p, sizeof(tot), hipMemcpyDeviceToHost));
         stats->paths = total;
@@ -452,8 +499,8 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
             stats->nodes_visited = tc[0];
             stats->tris_tested = tc[1];
         }
-        double* ms[kStages] = {&stats->ms_raygen, &stats->ms_extend, &stats->ms_shade,
-                               &stats->ms_shadow, &stats->ms_mis,    &stats->ms_film};
+        double* ms[kStages] = {&stats->ms_raygen, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow,
+                               &stats->ms_mis,    &stats->ms_film,   &stats->ms_guide, &stats->ms_train};
         for (int sg = 0; sg < kStages; ++sg)
             for (auto& m : marks[sg]) {
                 float e = 0.0f;
@@ -585,6 +632,24 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         std::memcpy(ds.cam.up, c.up, 12);
         ds.cam.sizeX = c.sensor_size[0], ds.cam.sizeY = c.sensor_size[1];
         ds.cam.focal = c.focal_length, ds.cam.aperture = c.aperture_radius;
+        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        auto grow = [&](const float* p) {
+            for (int k = 0; k < 3; ++k) blo[k] = std::min(blo[k], p[k]), bhi[k] = std::max(bhi[k], p[k]);
+        };
+        for (size_t i = 0; i < s->triVerts.size(); i += 3) grow(&s->triVerts[i]);
+        for (const auto& d : quads) {
+            float c4[3];
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    for (int k = 0; k < 3; ++k) c4[k] = d.origin[k] + a * d.u[k] + b * d.v[k];
+                    grow(c4);
+                }
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (!(bhi[k] > blo[k])) blo[k] = 0.0f, bhi[k] = 1.0f;
+            s->lo[k] = blo[k];
+            s->invExt[k] = 1.0f / (bhi[k] - blo[k]);
+        }
         std::memcpy(ds.env, desc->env_radiance, 12);
         buildBvh(s.get(), nullptr);
         addChild(dev);
@@ -688,6 +753,13 @@ int pg_film_download(pg_film* f, float* rgbw, double* rgbw64) {
 
 int pg_render(pg_scene* s, const pg_render_params* params, pg_film* film, pg_stats* stats) {
     return guarded([&] { render(s, params, film, nullptr, stats); });
+}
+
+int pg_render_guided(pg_scene* s, pg_guide* guide, const pg_render_params* params, pg_film* film, pg_stats* stats) {
+    return guarded([&] {
+        require(guide != nullptr, "null guide");
+        render(s, params, film, nullptr, stats, guide);
+    });
 }
 
 int pg_render_paths(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathL, pg_stats* stats) {

@@ -12,6 +12,7 @@ namespace pg {
 constexpr uint32_t kGTiles = 96;       // 16x16 weight-gradient tiles over the five layers
 constexpr uint32_t kGMaxChunks = 64;   // sample chunks of the split-K weight gradient
 constexpr uint32_t kGParamCount = 24880;
+constexpr uint32_t kGMixFloatsHost = 48;  // mixture record floats per sample
 
 struct WGradTile {
     uint16_t rows, cols, mo, ki;  // layer shape and tile coordinates

@@ -8,6 +8,7 @@
 // multipleImportanceSampling() of Integrator/PathTracing/Kernel.cpp:23-193; the per-sample driver
 // restates piperMain (Tracer/Embree/Kernel.cpp:47-100).
 #include "pg_device.hpp"
+#include "pg_guide.hpp"
 #include "pg_kernels.hpp"
 
 namespace pg {
@@ -108,8 +109,16 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const ui
 // multipleImportanceSampling (:23-107); shadow and MIS-BSDF rays go to their queues.
 // ------------------------------------------------------------------------------------------
 
+// Training history record of one guided vertex (6 float4): (pos01, mixture pdf) (normal) (wo)
+// (sampled direction) (throughput after the bounce) (L once the vertex's direct light is in).
+constexpr uint32_t kHistFloat4 = 6;
+
+__device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
+    return gs.train && (g % gs.trainStride) == gs.trainPhase;
+}
+
 __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q, RenderConsts rc, uint32_t curSlot,
-                                              uint32_t nextSlot, uint64_t* totals) {
+                                              uint32_t nextSlot, uint64_t* totals, GuideShade gs) {
     const uint32_t n = q.counts[curSlot];
     const uint32_t* qcur = q.ext[curSlot];
     uint32_t* qnext = q.ext[nextSlot];
@@ -133,6 +142,11 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
             V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
             ps.Lmis[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (meta.z & kMetaPending) {  // the previous guided vertex's direct light is now in L
+                const uint32_t r = ((meta.z >> 8) & 0xFFu) - 1u;
+                gs.hist[(static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4 + 5] = make_float4(L.x, L.y, L.z, 0.0f);
+                meta.z &= ~kMetaPending;
+            }
             const bool specular = meta.z & 1u;
             const bool isLight = pos != kHitMiss && (pos & kHitQuad);
             if (specular && (isLight || pos == kHitMiss)) {  // :119-140, two-sided emission
@@ -219,7 +233,35 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                     }
                 }
 
-                const BsdfSample ss = bsdfSample(mat, wo, d9 ? u[6] : u[5], d9 ? u[5] : u[6]);  // :164-190
+                // continuation (:164-190). With a guide: one-sample MIS that draws from the guide
+                // mixture with probability alpha, else from the BSDF, and weights by the combined
+                // density alpha p_guide + (1 - alpha) p_bsdf, so the estimator stays unbiased.
+                const float uc1 = d9 ? u[6] : u[5], uc2 = d9 ? u[5] : u[6];
+                BsdfSample ss;
+                float pdfMix = 0.0f;
+                if (!gs.enabled) {
+                    ss = bsdfSample(mat, wo, uc1, uc2);
+                } else {
+                    const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
+                    const float a = gs.alpha;
+                    bool ok = true;
+                    if (uc1 < a) {
+                        const float3 w = mixtureSample(mix, uc1 / a, uc2);
+                        ss.wi = fr.toLocal(V3{w.x, w.y, w.z});
+                    } else {
+                        const BsdfSample b = bsdfSample(mat, wo, (uc1 - a) / (1.0f - a), uc2);
+                        ss.wi = b.wi;
+                        ok = b.pdf > 0.0f;
+                    }
+                    ss.f = V3{0.0f, 0.0f, 0.0f};
+                    ss.pdf = 0.0f;
+                    if (ok) {
+                        const V3 ww = fr.toWorld(ss.wi);
+                        pdfMix = a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z)) + (1.0f - a) * bsdfPdf(mat, wo, ss.wi);
+                        ss.f = bsdfEval(mat, wo, ss.wi, ng);
+                        ss.pdf = pdfMix;
+                    }
+                }
                 if (!(ss.pdf > 0.0f && valid(ss.f))) {
                     pushDone = true;
                 } else {
@@ -238,8 +280,21 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
                         ps.d[path] = make_float4(nd.x, nd.y, nd.z, 0.0f);
                         ps.thr[path] = make_float4(thr.x, thr.y, thr.z, 0.0f);
                         meta.y += 1u;
-                        meta.z = 0u;
+                        meta.z &= ~1u;  // no longer a camera / specular ray
                         pushExt = true;
+                        const uint32_t r = (meta.z >> 8) & 0xFFu;
+                        if (gs.enabled && isTrainPath(gs, meta.w) && r < gs.histDepth) {
+                            float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
+                            const float px = fminf(fmaxf((hp.x - gs.lo.x) * gs.invExt.x, 0.0f), 1.0f);
+                            const float py = fminf(fmaxf((hp.y - gs.lo.y) * gs.invExt.y, 0.0f), 1.0f);
+                            const float pz = fminf(fmaxf((hp.z - gs.lo.z) * gs.invExt.z, 0.0f), 1.0f);
+                            h[0] = make_float4(px, py, pz, pdfMix);
+                            h[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+                            h[2] = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, 0.0f);
+                            h[3] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                            h[4] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                            meta.z = (meta.z & ~0xFF00u) | ((r + 1u) << 8) | kMetaPending;
+                        }
                     } else {
                         pushDone = true;
                     }
@@ -366,15 +421,35 @@ __global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, u
 // each source pixel's samples into the four 2x2 footprints they can start at, and kResolve has
 // every film pixel gather the footprints that cover it from its 3x3 source neighbourhood. Every
 // sum runs in a fixed order, so the film is bit-reproducible.
-__global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* recL, float* recY) {
+__global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* recL, float* recY, GuideShade gs,
+                                               GuideRecordOut tr) {
     const uint32_t n = q.counts[kCntDone];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t path = q.done[i];
         const V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
         const float2 fp = ps.film[path];
-        const uint32_t g = ps.meta[path].w;
+        const uint4 meta = ps.meta[path];
+        const uint32_t g = meta.w;
         recL[g] = make_float4(L.x, L.y, L.z, fp.x);
         recY[g] = fp.y;
+        if (!isTrainPath(gs, g)) continue;
+        // Each recorded vertex learns from the radiance that reached it along its sampled
+        // direction: (L_final - L after its direct light) / throughput after the bounce.
+        const uint32_t cnt = (meta.z >> 8) & 0xFFu;
+        for (uint32_t r = 0; r < cnt; ++r) {
+            const float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
+            const float4 h0 = h[0], t = h[4], snap = h[5];
+            const float lr = t.x > 0.0f ? (L.x - snap.x) / t.x : 0.0f;
+            const float lg = t.y > 0.0f ? (L.y - snap.y) / t.y : 0.0f;
+            const float lb = t.z > 0.0f ? (L.z - snap.z) / t.z : 0.0f;
+            const float c = fmaxf((lr + lg + lb) * (1.0f / 3.0f), 0.0f) / h0.w;
+            const uint32_t slot = atomicAdd(tr.count, 1u);
+            if (slot >= tr.capacity) break;
+            tr.rec[4 * slot] = make_float4(h0.x, h0.y, h0.z, c);
+            tr.rec[4 * slot + 1] = h[1];
+            tr.rec[4 * slot + 2] = h[2];
+            tr.rec[4 * slot + 3] = h[3];
+        }
     }
 }
 
@@ -521,8 +596,8 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
     }
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals) {
-    hipLaunchKernelGGL(kShade, dim3(gridThreads / 256), dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals);
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs) {
+    hipLaunchKernelGGL(kShade, dim3(gridThreads / 256), dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs);
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane) {
@@ -541,8 +616,8 @@ void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Que
         hipLaunchKernelGGL(kMis<4>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
 }
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY) {
-    hipLaunchKernelGGL(kRecord, dim3(gridThreads / 256), dim3(256), 0, st, ps, q, recL, recY);
+                  float* recY, const GuideShade& gs, const GuideRecordOut& tr) {
+    hipLaunchKernelGGL(kRecord, dim3(gridThreads / 256), dim3(256), 0, st, ps, q, recL, recY, gs, tr);
 }
 void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
                    float* film32, double* film64) {

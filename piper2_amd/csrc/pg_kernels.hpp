@@ -19,6 +19,27 @@ struct PathState {
     float2* film;  // film point
 };
 
+// meta.z bits: 0 camera/specular ray, 1 a guided vertex awaits its radiance snapshot,
+// 8..15 guided vertices recorded for training
+constexpr uint32_t kMetaPending = 0x2u;
+
+// Guiding state handed to the shade and record stages.
+struct GuideShade {
+    const float* mix;  // 48 floats per extension-queue entry (pg_guide.hpp layout)
+    float4* hist;      // per path slot: histDepth records of 6 float4
+    float alpha;       // probability of sampling the guide
+    float3 lo, invExt; // scene box -> [0,1]^3
+    uint32_t histDepth, trainStride, trainPhase;
+    uint32_t enabled, train;
+};
+
+// Training records emitted when a training path finishes.
+struct GuideRecordOut {
+    float4* rec;       // 4 float4 per record
+    uint32_t* count;
+    uint32_t capacity;
+};
+
 struct Queues {
     uint32_t* ext[2];  // extension queues (current / next bounce)
     ShadowRec* shadow;
@@ -33,13 +54,13 @@ void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot);
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts);
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals);
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs);
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                uint2* spill, uint32_t spillPerLane);
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY);
+                  float* recY, const GuideShade& gs, const GuideRecordOut& tr);
 // quad: 16 floats (or doubles with film64) per rect pixel
 void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
                    float* film32, double* film64);

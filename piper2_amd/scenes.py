@@ -225,6 +225,97 @@ def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr
                     ndc=ndc, name="terrain_spheres")
 
 
+def terrain_slit(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500)):
+    """Moderate indirect (guided config 4): the terrain + icosphere scene with its 16 lights moved
+    behind a wall whose only opening is a 0.3-wide vertical slit at x = 0."""
+    sc = terrain_spheres(seed=seed, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=min_rr,
+                         grid=grid)
+    rng = np.random.default_rng(seed + 1000)
+    b = _Builder()
+    b.v, b.t, b.obj = [sc["vertices"].astype(np.float64)], [sc["indices"].astype(np.int64)], [sc["tri_object"]]
+    b.objmat, b.mats, b.nv = list(sc["object_material"]), [list(m) for m in sc["materials"]], sc["vertices"].shape[0]
+    wall = b.material(LAMBERT, [0.6, 0.6, 0.6])
+    for x0, x1 in ((-10.5, -0.15), (0.15, 10.5)):
+        v, t = _box(((x0 + x1) / 2, -2.5), -1.0, (x1 - x0, 6.0, 0.2), 0.0)
+        b.add(v, t, wall)
+    quads = []
+    for _ in range(16):
+        sx, sz = rng.uniform(0.5, 1.0, 2)
+        x, z, y = rng.uniform(-8.0, 8.0), rng.uniform(-4.8, -3.2), rng.uniform(2.0, 3.0)
+        quads.append([x - sx / 2, y, z - sz / 2, sx, 0.0, 0.0, 0.0, 0.0, sz, *rng.uniform(5.0, 50.0, 3)])
+    out = b.finish(quads, sc["camera"], width=width, height=height, spp=spp, max_depth=max_depth, min_rr=min_rr,
+                   ndc=sc["ndc"], name="terrain_slit")
+    return out
+
+
+def _grid_quad(origin, u, v, n):
+    """A parallelogram tessellated into n x n cells (2 n^2 triangles)."""
+    origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))
+    s = np.linspace(0.0, 1.0, n + 1)
+    S, T = np.meshgrid(s, s, indexing="ij")
+    verts = origin + S.reshape(-1, 1) * u + T.reshape(-1, 1) * v
+    i = (np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]).reshape(-1)
+    tris = np.concatenate([np.stack([i, i + n + 1, i + 1], -1), np.stack([i + 1, i + n + 1, i + n + 2], -1)])
+    return verts, tris
+
+
+def hard_room(seed=0, width=512, height=512, spp=64, max_depth=12, tess=204):
+    """Hard indirect (guided config 5): a closed room (walls tessellated to ~500k triangles, Lambert
+    U[0.3, 0.7]) with one 0.05 x 0.05 light of radiance 500 inside an open-top lamp housing; the
+    light reaches the room only through a GGX conductor reflector (roughness 0.1) above it."""
+    rng = np.random.default_rng(100 + seed)
+    b = _Builder()
+    W, H, D = 5.0, 4.0, 5.0
+    walls = [
+        ([-W, 0, -D], [2 * W, 0, 0], [0, 0, 2 * D]),    # floor
+        ([-W, H, -D], [0, 0, 2 * D], [2 * W, 0, 0]),    # ceiling
+        ([-W, 0, -D], [0, H, 0], [2 * W, 0, 0]),        # back
+        ([-W, 0, D], [2 * W, 0, 0], [0, H, 0]),         # front (behind the camera)
+        ([-W, 0, -D], [0, 0, 2 * D], [0, H, 0]),        # left
+        ([W, 0, -D], [0, H, 0], [0, 0, 2 * D]),         # right
+    ]
+    for o, u, v in walls:
+        verts, tris = _grid_quad(o, u, v, tess)
+        b.add(verts, tris, b.material(LAMBERT, rng.uniform(0.3, 0.7, 3)))
+    cx, cz = rng.uniform(-2.0, 2.0, 2)
+    y0, hs, hh = 1.0, 0.2, 0.3  # housing base height, half size, height
+    housing = b.material(LAMBERT, [0.5, 0.5, 0.5])
+    sides = [
+        ([cx - hs, y0, cz - hs], [2 * hs, 0, 0], [0, 0, 2 * hs]),
+        ([cx - hs, y0, cz - hs], [0, hh, 0], [2 * hs, 0, 0]),
+        ([cx - hs, y0, cz + hs], [2 * hs, 0, 0], [0, hh, 0]),
+        ([cx - hs, y0, cz - hs], [0, 0, 2 * hs], [0, hh, 0]),
+        ([cx + hs, y0, cz - hs], [0, hh, 0], [0, 0, 2 * hs]),
+    ]
+    for o, u, v in sides:
+        verts, tris = _grid_quad(o, u, v, 1)
+        b.add(verts, tris, housing)
+    # reflector: GGX conductor panel near the ceiling above the housing, slightly tilted
+    tilt = rng.uniform(-0.3, 0.3)
+    ry = H - 0.3
+    r0 = [cx - 0.6, ry - 0.6 * tilt, cz - 0.6]
+    verts, tris = _grid_quad(r0, [1.2, 1.2 * tilt, 0.0], [0.0, 0.0, 1.2], 1)
+    b.add(verts, tris, b.material(CONDUCTOR, [0.95, 0.95, 0.95], 0.1 * 0.1))
+    quads = [[cx - 0.025, y0 + 0.02, cz - 0.025, 0.05, 0.0, 0.0, 0.0, 0.0, 0.05, 500.0, 500.0, 500.0]]
+    cam = make_camera([0.0, 2.0, 4.6], [0.0, 1.5, 0.0], [0, 1, 0], 60.0, width / height)
+    ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
+    return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=3, ndc=ndc,
+                    name=f"hard_room_{seed}")
+
+
+def hidden_cornell(width=64, height=64, spp=64, max_depth=6):
+    """Small hard-indirect test scene: the Cornell box with a plate under the ceiling light, so the
+    room is lit only after the light bounces off the ceiling."""
+    sc = cornell(width=width, height=height, spp=spp, max_depth=max_depth)
+    b = _Builder()
+    b.v, b.t, b.obj = [sc["vertices"].astype(np.float64)], [sc["indices"].astype(np.int64)], [sc["tri_object"]]
+    b.objmat, b.mats, b.nv = list(sc["object_material"]), [list(m) for m in sc["materials"]], sc["vertices"].shape[0]
+    verts, tris = _grid_quad([-0.45, 0.7, -0.45], [0.9, 0, 0], [0, 0, 0.9], 1)
+    b.add(verts, tris, b.material(LAMBERT, [0.5, 0.5, 0.5]))
+    return b.finish(sc["quads"], sc["camera"], width=width, height=height, spp=spp, max_depth=max_depth,
+                    min_rr=max_depth, ndc=sc["ndc"], name="hidden_cornell")
+
+
 def world_triangles(sc):
     """(T, 3, 3) float32 world-space triangle vertices."""
     return sc["vertices"][sc["indices"]]
