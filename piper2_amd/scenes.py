@@ -277,26 +277,29 @@ def hard_room(seed=0, width=512, height=512, spp=64, max_depth=12, tess=204):
     for o, u, v in walls:
         verts, tris = _grid_quad(o, u, v, tess)
         b.add(verts, tris, b.material(LAMBERT, rng.uniform(0.3, 0.7, 3)))
-    cx, cz = rng.uniform(-2.0, 2.0, 2)
-    y0, hs, hh = 1.0, 0.2, 0.3  # housing base height, half size, height
+    # Fixture (0.6 x 0.4 x 0.6 box at a seeded spot, opening toward +z): the light lies on the floor
+    # of the back half; a shelf over the front half and the closed lower front wall hide it from
+    # every straight line through the opening, so it reaches the room only after the 45-degree GGX
+    # reflector in the back-top corner (or bounces inside the housing).
+    cx, cz = rng.uniform(-2.5, 2.5), rng.uniform(-3.5, 0.0)
+    y0 = rng.uniform(1.2, 2.2)
     housing = b.material(LAMBERT, [0.5, 0.5, 0.5])
-    sides = [
-        ([cx - hs, y0, cz - hs], [2 * hs, 0, 0], [0, 0, 2 * hs]),
-        ([cx - hs, y0, cz - hs], [0, hh, 0], [2 * hs, 0, 0]),
-        ([cx - hs, y0, cz + hs], [2 * hs, 0, 0], [0, hh, 0]),
-        ([cx - hs, y0, cz - hs], [0, 0, 2 * hs], [0, hh, 0]),
-        ([cx + hs, y0, cz - hs], [0, hh, 0], [0, 0, 2 * hs]),
+    x0, x1, z0, z1 = cx - 0.3, cx + 0.3, cz - 0.3, cz + 0.3
+    panels = [
+        ([x0, y0, z0], [0.6, 0, 0], [0, 0, 0.6]),          # bottom
+        ([x0, y0 + 0.4, z0], [0, 0, 0.6], [0.6, 0, 0]),    # top
+        ([x0, y0, z0], [0, 0.4, 0], [0.6, 0, 0]),          # back
+        ([x0, y0, z0], [0, 0, 0.6], [0, 0.4, 0]),          # left
+        ([x1, y0, z0], [0, 0.4, 0], [0, 0, 0.6]),          # right
+        ([x0, y0, z1], [0.6, 0, 0], [0, 0.2, 0]),          # lower half of the front
+        ([x0, y0 + 0.2, cz], [0.6, 0, 0], [0, 0, 0.3]),    # shelf over the front half
     ]
-    for o, u, v in sides:
+    for o, u, v in panels:
         verts, tris = _grid_quad(o, u, v, 1)
         b.add(verts, tris, housing)
-    # reflector: GGX conductor panel near the ceiling above the housing, slightly tilted
-    tilt = rng.uniform(-0.3, 0.3)
-    ry = H - 0.3
-    r0 = [cx - 0.6, ry - 0.6 * tilt, cz - 0.6]
-    verts, tris = _grid_quad(r0, [1.2, 1.2 * tilt, 0.0], [0.0, 0.0, 1.2], 1)
+    verts, tris = _grid_quad([x0, y0 + 0.15, z0], [0.6, 0, 0], [0, 0.25, 0.25], 1)
     b.add(verts, tris, b.material(CONDUCTOR, [0.95, 0.95, 0.95], 0.1 * 0.1))
-    quads = [[cx - 0.025, y0 + 0.02, cz - 0.025, 0.05, 0.0, 0.0, 0.0, 0.0, 0.05, 500.0, 500.0, 500.0]]
+    quads = [[cx - 0.025, y0 + 0.02, cz - 0.245, 0.05, 0.0, 0.0, 0.0, 0.0, 0.05, 500.0, 500.0, 500.0]]
     cam = make_camera([0.0, 2.0, 4.6], [0.0, 1.5, 0.0], [0, 1, 0], 60.0, width / height)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=3, ndc=ndc,
