@@ -117,6 +117,59 @@ def guided_vs_plain(dev, sc, ref_spp, label):
     return out
 
 
+def batched(dev, n_scenes, steps):
+    """Batched workload: n seeded scenes (log-uniform 10k-2M triangles, mixed resolutions, spp
+    4/16/64, depth 8) in one pg_render_batch call; aggregate Mrays/s and per-scene latency
+    (device time from the batch's start until the scene's film is complete). The same scenes
+    rendered one pg_render call after another are timed beside it."""
+    import numpy as np
+
+    import piper2_amd as pg
+    from piper2_amd import scenes
+    t0 = time.perf_counter()
+    scs = [scenes.batch_scene(i) for i in range(n_scenes)]
+    log(f"batch: {n_scenes} scenes generated in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    objs = [pg.Scene(dev, sc) for sc in scs]
+    log(f"batch: BVHs built in {time.perf_counter() - t0:.1f}s")
+    params = [pg.render_params(sc, flags=pg.PG_NO_STAGE_TIMING) for sc in scs]
+    films = [pg.Film(dev, sc["width"], sc["height"]) for sc in scs]
+    st, lat = pg.render_batch(dev, objs, params, films)  # warmup
+    log(f"batch warmup: {st['ms_total']:.0f} ms")
+    rays = paths = 0
+    lats = []
+    t_start = time.perf_counter()
+    for i in range(steps):
+        for f in films:
+            f.clear()
+        st, lat = pg.render_batch(dev, objs, params, films)
+        rays += st["extension_rays"] + st["shadow_rays"] + st["mis_rays"]
+        paths += st["paths"]
+        lats.append(lat)
+        log(f"batch step {i}: {st['ms_total']:.0f} ms, {st['paths'] / st['ms_total'] / 1e3:.1f} Mpaths/s")
+    elapsed = time.perf_counter() - t_start
+    lats = np.concatenate(lats)
+    t_seq = 0.0
+    for s, p, f in zip(objs, params, films):  # the same work as one render call per scene
+        f.clear()
+        t = time.perf_counter()
+        s.render(p, f)
+        t_seq += time.perf_counter() - t
+    out = {"scenes": n_scenes, "triangles_total": int(sum(sc["indices"].shape[0] for sc in scs)),
+           "paths_per_batch": int(paths // steps), "steps": steps,
+           "ms_per_batch": round(elapsed * 1e3 / steps, 1),
+           "mrays_per_s": round(rays / elapsed / 1e6, 1), "mpaths_per_s": round(paths / elapsed / 1e6, 1),
+           "latency_ms_p50": round(float(np.percentile(lats, 50)), 1),
+           "latency_ms_p99": round(float(np.percentile(lats, 99)), 1),
+           "sequential_ms": round(t_seq * 1e3, 1),
+           "speedup_vs_sequential": round(t_seq / (elapsed / steps), 3)}
+    for f in films:
+        f.close()
+    for s in objs:
+        s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +180,8 @@ def main():
     ap.add_argument("--bvh-width", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--ref-spp", type=int, default=16384, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -211,6 +266,9 @@ def main():
                      "tris_per_ray": round(tris_per_ray, 3), "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "launches": launches},
     }
+    if args.batch_scenes > 0:
+        result["batched"] = batched(dev, args.batch_scenes, args.batch_steps)
+        log(f"batched: {result['batched']}")
     if not args.no_guided:
         # time to 1% relMSE, guided (online training, one-sample MIS at 0.5) vs plain
         result["time_to_1pct_relmse"] = {

@@ -34,6 +34,7 @@ extern "C" {
 #define PG_COMPAT_D13 0x8u /* light pass-through decrements depth (PathTracing/Kernel.cpp:143) */
 #define PG_FILM_FP64 0x100u /* accumulate into the film's FP64 RGBW buffer instead of FP32 */
 #define PG_COUNT_TRAVERSAL 0x200u /* count BVH nodes visited / triangles tested by extension rays (slower) */
+#define PG_NO_STAGE_TIMING 0x400u /* stats without per-stage device timing (no events around launches) */
 
 #define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
 #define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
@@ -141,6 +142,15 @@ int pg_render(pg_scene* scene, const pg_render_params* params, pg_film* film, pg
  * ((py-rect.y)*rect.w + (px-rect.x))*(spp_end-spp_begin) + (s-spp_begin). Parity aid. */
 int pg_render_paths(pg_scene* scene, const pg_render_params* params, pg_film* film, float* path_L,
                     pg_stats* stats);
+/* Batched render (no reference counterpart: the reference renders one pipeline at a time under a
+ * lock, Tracer/Embree/Embree.cpp:993-996; it replaces n successive RenderDriver::renderFrame
+ * calls, FixedSampler.cpp:68-133). The paths of n scenes (own resolutions, rects, sample
+ * ranges, depths and flags) run through one wavefront pipeline; each film ends up exactly as
+ * pg_render(scenes[i], &params[i], films[i]) leaves it. All scenes live on `dev` and share one
+ * BVH width; films[i] may be NULL. latency_ms (optional, n entries): device time from the start
+ * of the batch until scene i's film is complete. */
+int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
+                    pg_film* const* films, double* latency_ms, pg_stats* stats);
 
 /* ---- parity entry points (Embree rtcIntersect1 / rtcOccluded1, Embree.cpp:300,384; sampler
  *      sobolStart / sobolGenerate, ScrambledSobolSampler/Kernel.cpp:52-67) ---- */

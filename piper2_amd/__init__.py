@@ -24,6 +24,7 @@ PG_COMPAT_D9 = 0x4
 PG_COMPAT_D13 = 0x8
 PG_FILM_FP64 = 0x100
 PG_COUNT_TRAVERSAL = 0x200
+PG_NO_STAGE_TIMING = 0x400
 
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
@@ -120,6 +121,8 @@ SIGNATURES = {
     "pg_guide_gradients": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP]),
     "pg_guide_train": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "pg_render_guided": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
+    "pg_render_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _U32, _VP, _VP,
+                                       ctypes.POINTER(Stats)]),
 }
 
 _lib = None
@@ -375,6 +378,21 @@ class Film:
             self.close()
         except Exception:
             pass
+
+
+def render_batch(device, scenes, params, films=None):
+    """One batched render of several scenes (pg_render_batch). films: list of Film or None entries.
+    Returns (stats dict, per-scene latency in ms as a numpy array)."""
+    n = len(scenes)
+    assert len(params) == n and (films is None or len(films) == n)
+    hs = (ctypes.c_void_p * n)(*[s.h.value for s in scenes])
+    ps = (RenderParams * n)(*params)
+    fs = (ctypes.c_void_p * n)(*[(f.h.value if f else None) for f in films]) if films else None
+    lat = np.zeros(n, np.float64)
+    st = Stats()
+    check(lib().pg_render_batch(device.h, ctypes.cast(hs, _VP), ps, n, ctypes.cast(fs, _VP) if fs else None,
+                                _ptr(lat), ctypes.byref(st)))
+    return st.as_dict(), lat
 
 
 def scene_desc(sc):

@@ -104,6 +104,7 @@ struct pg_device {
     DevBuf recL, recY, quad;  // film records of one segment and per-pixel footprint sums
     DevBuf mix, hist, trainRec, trainCount;  // guiding: mixtures, vertex history, training records
     DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
+    DevBuf batchScenes, batchRc, batchOffsets, batchRecBase;  // work-item tables of a batched render
     size_t spillBytes = 0;
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
@@ -329,8 +330,46 @@ uint32_t shadeThreads(pg_device* dev, uint64_t work) {
 
 enum Stage { kRaygen, kExtend, kShade, kShadow, kMis, kFilm, kGuide, kTrain, kStages };
 
-// Paths whose film records one segment holds; larger sample ranges are rendered in segments.
-constexpr uint64_t kMaxSegmentPaths = 1ull << 28;
+// Per-stage device time: a pair of events around every launch of a stage (off: no events).
+struct StageTimer {
+    pg_device* dev;
+    hipStream_t st;
+    bool on;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[kStages];
+
+    template <typename F>
+    void operator()(Stage sg, F&& launch) {
+        if (!on) {
+            launch();
+            return;
+        }
+        hipEvent_t a = dev->event(), b = dev->event();
+        PG_HIP(hipEventRecord(a, st));
+        launch();
+        PG_HIP(hipEventRecord(b, st));
+        marks[sg].push_back({a, b});
+    }
+    // after the stream has drained
+    void collect(pg_stats* stats) const {
+        double* ms[kStages] = {&stats->ms_raygen, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow,
+                               &stats->ms_mis,    &stats->ms_film,   &stats->ms_guide, &stats->ms_train};
+        for (int sg = 0; sg < kStages; ++sg)
+            for (auto& m : marks[sg]) {
+                float e = 0.0f;
+                PG_HIP(hipEventElapsedTime(&e, m.first, m.second));
+                *ms[sg] += e;
+            }
+    }
+};
+
+// Paths whose film records one segment holds; larger sample ranges are rendered in segments
+// (whole pixels x a sample sub-range), each resolved into the film when it is done.
+constexpr uint64_t kMaxSegmentPaths = 1ull << 25;
+
+uint32_t segmentSpp(const pg::RenderConsts& rc) {
+    const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(rc.sppCount, kMaxSegmentPaths / pixels)));
+}
 
 void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats,
             pg_guide* guide = nullptr) {
@@ -345,7 +384,7 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     hipStream_t st = dev->stream;
     const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
     const uint64_t total = pixels * rc.sppCount;
-    const uint32_t segSpp = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(rc.sppCount, kMaxSegmentPaths / pixels)));
+    const uint32_t segSpp = segmentSpp(rc);
     const uint64_t segMax = pixels * segSpp;
     uint32_t cap = params->wavefront_paths ? params->wavefront_paths : (1u << 21);
     cap = static_cast<uint32_t>(std::min<uint64_t>(cap, segMax));
@@ -400,19 +439,7 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     uint32_t* trav = countTrav ? dev->travCounts.as<uint32_t>() : nullptr;
     if (countTrav) PG_HIP(hipMemsetAsync(dev->travCounts.p, 0, 64, st));
     dev->eventsUsed = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[kStages];
-    const bool timed = stats != nullptr;
-    auto timeStage = [&](Stage sg, auto&& launch) {
-        if (!timed) {
-            launch();
-            return;
-        }
-        hipEvent_t a = dev->event(), b = dev->event();
-        PG_HIP(hipEventRecord(a, st));
-        launch();
-        PG_HIP(hipEventRecord(b, st));
-        marks[sg].push_back({a, b});
-    };
+    StageTimer timeStage{dev, st, stats != nullptr && !(rc.flags & pg::kNoStageTiming), {}};
     // With a black environment the shade stage resolves MIS-BSDF rays as shadow-queue occlusion
     // queries, and the closest-hit MIS queue stays empty.
     const bool envBlack = !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
@@ -499,14 +526,195 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
             stats->nodes_visited = tc[0];
             stats->tris_tested = tc[1];
         }
-        double* ms[kStages] = {&stats->ms_raygen, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow,
-                               &stats->ms_mis,    &stats->ms_film,   &stats->ms_guide, &stats->ms_train};
-        for (int sg = 0; sg < kStages; ++sg)
-            for (auto& m : marks[sg]) {
-                float e = 0.0f;
-                PG_HIP(hipEventElapsedTime(&e, m.first, m.second));
-                *ms[sg] += e;
+        timeStage.collect(stats);
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+// Work item of a batched render: one scene's segment (as render() splits it), its path numbers
+// [first, first + count) in the batch and its film records at ring slots [slot, slot + count).
+struct BatchItem {
+    uint32_t scene;
+    pg::RenderConsts rc;
+    uint64_t first, count, slot;
+};
+
+// Places the items' film records in a ring of `ring` slots. An item is resolved after the wave
+// that holds its last path, so item k may reuse the slots of every item that ends at or before
+// the first path of the wave that starts k. With ring >= 4 * (largest item) + 2 * cap the live
+// items never fill the ring; the check below guards that bound.
+void placeBatchRecords(std::vector<BatchItem>& items, uint64_t cap, uint64_t ring) {
+    uint64_t pos = 0;
+    size_t live = 0;  // oldest item still unresolved when the current item's first wave starts
+    for (size_t k = 0; k < items.size(); ++k) {
+        BatchItem& it = items[k];
+        if (pos + it.count > ring) pos = 0;
+        const uint64_t waveStart = it.first / cap * cap;
+        while (live < k && items[live].first + items[live].count <= waveStart) ++live;
+        for (size_t j = live; j < k; ++j)
+            if (pos < items[j].slot + items[j].count && items[j].slot < pos + it.count)
+                throw PgError(PG_ERR_DEVICE, "batched render: film record ring overrun");
+        it.slot = pos;
+        pos += it.count;
+    }
+}
+
+// Batched render: the paths of all scenes run through one wavefront pipeline (raygen numbers
+// them item after item), so a wave mixes scenes, depths and resolutions and small scenes no
+// longer leave the device idle. Per path the arithmetic is that of render(), and every scene is
+// resolved segment by segment in the same order, so each film is bit-identical to its own render.
+void renderBatch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
+                 pg_film* const* films, double* latencyMs, pg_stats* stats) {
+    require(dev && scenes && params && n > 0, "empty batch");
+    std::lock_guard<std::mutex> lock(dev->mu);
+    dev->checkFault();
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = dev->stream;
+    std::vector<BatchItem> items;
+    uint64_t total = 0, maxItem = 0;
+    uint32_t maxDepth = 0, spillPerLane = 1;
+    bool envBlack = true;
+    size_t quadBytes = 16;
+    for (uint32_t i = 0; i < n; ++i) {
+        const pg_scene* s = scenes[i];
+        require(s && s->dev == dev, "every batched scene must live on the batch's device");
+        require(s->ds.width == scenes[0]->ds.width, "batched scenes must share one BVH width");
+        const pg::RenderConsts rc = makeConsts(params[i]);
+        require(!(rc.flags & pg::kCountTraversal), "traversal counting is not available in batched renders");
+        const pg_film* f = films ? films[i] : nullptr;
+        require(!f || (f->dev == dev && f->width == rc.width && f->height == rc.height),
+                "film size differs from params");
+        const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
+        const uint32_t segSpp = segmentSpp(rc);
+        for (uint32_t s0 = rc.sppBegin; s0 < rc.sppBegin + rc.sppCount; s0 += segSpp) {
+            BatchItem it{i, rc, total, 0, 0};
+            it.rc.sppBegin = s0;
+            it.rc.sppCount = std::min(segSpp, rc.sppBegin + rc.sppCount - s0);
+            it.count = pixels * it.rc.sppCount;
+            total += it.count;
+            maxItem = std::max(maxItem, it.count);
+            items.push_back(it);
+        }
+        maxDepth = std::max(maxDepth, rc.maxDepth);
+        spillPerLane = std::max(spillPerLane, s->spillPerLane);
+        envBlack = envBlack && !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
+        quadBytes = std::max<size_t>(quadBytes, pixels * 16 * ((rc.flags & pg::kFilmFp64) ? 8 : 4));
+    }
+    require(items.size() <= pg::kMaxBatchItems, "too many segments in one batch: split it");
+    uint32_t cap = params[0].wavefront_paths ? params[0].wavefront_paths : (1u << 21);
+    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, total));
+    const uint64_t ring = std::min<uint64_t>(total, 4 * maxItem + 2 * static_cast<uint64_t>(cap));
+    placeBatchRecords(items, cap, ring);
+
+    const uint32_t lanes = traversalLanes(dev, cap);
+    const uint32_t threads = shadeThreads(dev, cap);
+    ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * spillPerLane * 8);
+    dev->recL.reserve(ring * 16), dev->recY.reserve(ring * 4);
+    dev->quad.reserve(quadBytes);
+    std::vector<pg::DevScene> tabScene(items.size());
+    std::vector<pg::RenderConsts> tabRc(items.size());
+    std::vector<uint64_t> tabOffsets(items.size() + 1), tabRecBase(items.size());
+    std::vector<size_t> lastItem(n);
+    for (size_t k = 0; k < items.size(); ++k) {
+        tabScene[k] = scenes[items[k].scene]->ds;
+        tabRc[k] = items[k].rc;
+        tabOffsets[k] = items[k].first;
+        tabRecBase[k] = items[k].slot;
+        lastItem[items[k].scene] = k;
+    }
+    tabOffsets[items.size()] = total;
+    upload(dev->batchScenes, tabScene.data(), tabScene.size() * sizeof(pg::DevScene), st);
+    upload(dev->batchRc, tabRc.data(), tabRc.size() * sizeof(pg::RenderConsts), st);
+    upload(dev->batchOffsets, tabOffsets.data(), tabOffsets.size() * 8, st);
+    upload(dev->batchRecBase, tabRecBase.data(), tabRecBase.size() * 8, st);
+    const pg::BatchTable bt{dev->batchScenes.as<pg::DevScene>(), dev->batchRc.as<pg::RenderConsts>(),
+                            dev->batchOffsets.as<uint64_t>(), dev->batchRecBase.as<uint64_t>(),
+                            static_cast<uint32_t>(items.size())};
+    for (uint32_t i = 0; i < n; ++i) {  // FP64 films get their buffer on first use
+        pg_film* f = films ? films[i] : nullptr;
+        if (f && (params[i].flags & pg::kFilmFp64) && !f->rgbw64.p) {
+            f->rgbw64.reserve(static_cast<size_t>(f->width) * f->height * 32);
+            PG_HIP(hipMemsetAsync(f->rgbw64.p, 0, f->rgbw64.bytes, st));
+        }
+    }
+    const pg::PathState ps = pathState(dev);
+    const pg::Queues q = queues(dev);
+    const pg::GuideShade gs{};
+    const pg::GuideRecordOut tr{};
+    const pg::DevScene& ds0 = scenes[0]->ds;  // BVH width of the batch; per-path scenes come from bt
+    const pg::RenderConsts& rc0 = items[0].rc;
+    PG_HIP(hipMemsetAsync(dev->totals.p, 0, pg::kTotNum * 8, st));
+    dev->eventsUsed = 0;
+    StageTimer timeStage{dev, st, stats != nullptr && !(params[0].flags & pg::kNoStageTiming), {}};
+    hipEvent_t start = dev->event();
+    std::vector<hipEvent_t> done(n, nullptr);
+    PG_HIP(hipEventRecord(start, st));
+    uint64_t extendLaunches = 0;
+    uint2* spill = dev->spill.as<uint2>();
+    float4* recL = dev->recL.as<float4>();
+    float* recY = dev->recY.as<float>();
+    size_t nextResolve = 0;
+    for (uint64_t first = 0; first < total; first += cap) {
+        const uint32_t wn = static_cast<uint32_t>(std::min<uint64_t>(cap, total - first));
+        timeStage(kRaygen, [&] { pg::launchRaygen(st, ds0, ps, q, rc0, first, wn, &bt); });
+        uint32_t cur = 0;
+        for (uint32_t iter = 0;; ++iter) {
+            if (iter > maxDepth) {
+                uint32_t left = 0;
+                PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
+                PG_HIP(hipStreamSynchronize(st));
+                if (left == 0) break;
             }
+            const uint32_t nxt = cur ^ 1u;
+            pg::launchBounceBegin(st, q, nxt);
+            timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, nullptr, &bt); });
+            ++extendLaunches;
+            timeStage(kShade, [&] {
+                pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, &bt);
+            });
+            timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, &bt); });
+            if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, &bt); });
+            timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY, gs, tr, &bt); });
+            PG_HIP(hipGetLastError());
+            cur = nxt;
+        }
+        // resolve every item this wave finished, in item order
+        while (nextResolve < items.size() && items[nextResolve].first + items[nextResolve].count <= first + wn) {
+            const BatchItem& it = items[nextResolve];
+            pg_film* f = films ? films[it.scene] : nullptr;
+            if (f) {
+                const bool fp64 = it.rc.flags & pg::kFilmFp64;
+                timeStage(kFilm, [&] {
+                    pg::launchResolve(st, it.rc, recL + it.slot, recY + it.slot, dev->quad.p,
+                                      fp64 ? nullptr : f->rgbw32.as<float>(), fp64 ? f->rgbw64.as<double>() : nullptr);
+                });
+            }
+            if (lastItem[it.scene] == nextResolve) {
+                done[it.scene] = dev->event();
+                PG_HIP(hipEventRecord(done[it.scene], st));
+            }
+            ++nextResolve;
+        }
+        PG_HIP(hipGetLastError());
+    }
+    PG_HIP(hipStreamSynchronize(st));
+    if (latencyMs)
+        for (uint32_t i = 0; i < n; ++i) {
+            float e = 0.0f;
+            PG_HIP(hipEventElapsedTime(&e, start, done[i]));
+            latencyMs[i] = e;
+        }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        uint64_t tot[pg::kTotNum];
+        PG_HIP(hipMemcpy(tot, dev->totals.p, sizeof(tot), hipMemcpyDeviceToHost));
+        stats->paths = total;
+        stats->extension_rays = total + tot[pg::kTotExt];
+        stats->shadow_rays = tot[pg::kTotShadow];
+        stats->mis_rays = tot[pg::kTotMis];
+        stats->extend_launches = extendLaunches;
+        timeStage.collect(stats);
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -767,6 +975,11 @@ int pg_render_paths(pg_scene* s, const pg_render_params* params, pg_film* film, 
         require(pathL != nullptr, "null path_L");
         render(s, params, film, pathL, stats);
     });
+}
+
+int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
+                    pg_film* const* films, double* latency_ms, pg_stats* stats) {
+    return guarded([&] { renderBatch(dev, scenes, params, n, films, latency_ms, stats); });
 }
 
 int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv) {

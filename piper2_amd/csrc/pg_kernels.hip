@@ -46,8 +46,12 @@ __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts&
     return cameraRay(s.cam, ndcX, ndcY, u[3], u[4]);
 }
 
-__global__ void __launch_bounds__(256) kRaygen(DevScene s, PathState ps, Queues q, RenderConsts rc, uint64_t first,
-                                               uint32_t n) {
+// Work item of a path in a batched render (meta.z bits 16..31; 0 otherwise).
+__device__ __forceinline__ uint32_t sceneOf(uint32_t metaZ) { return metaZ >> 16; }
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) kRaygen(DevScene s0, PathState ps, Queues q, RenderConsts rc0, BatchTable bt,
+                                               uint64_t first, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         q.counts[kCntExtCur] = n;
@@ -55,7 +59,20 @@ __global__ void __launch_bounds__(256) kRaygen(DevScene s, PathState ps, Queues 
         q.counts[kCntShadow] = q.counts[kCntMis] = q.counts[kCntDone] = 0;
     }
     if (i >= n) return;
-    const uint32_t g = static_cast<uint32_t>(first + i);
+    const uint64_t G = first + i;
+    uint32_t sid = 0;
+    if (BATCH) {  // last scene whose first path number is <= G
+        uint32_t lo = 0, hi = bt.n;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bt.pathOffsets[mid] <= G) lo = mid;
+            else hi = mid;
+        }
+        sid = lo;
+    }
+    const DevScene& s = BATCH ? bt.scenes[sid] : s0;
+    const RenderConsts& rc = BATCH ? bt.rc[sid] : rc0;
+    const uint32_t g = static_cast<uint32_t>(BATCH ? G - bt.pathOffsets[sid] : G);
     uint32_t px, py, si;
     pathPixel(rc, g, px, py, si);
     uint64_t idx;
@@ -67,7 +84,7 @@ __global__ void __launch_bounds__(256) kRaygen(DevScene s, PathState ps, Queues 
     ps.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps.Lmis[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps.idx[i] = idx;
-    ps.meta[i] = make_uint4(5u, 0u, 1u, g);  // sample dims start at 5 (Embree/Kernel.cpp:68); specular
+    ps.meta[i] = make_uint4(5u, 0u, 1u | (sid << 16), g);  // sample dims start at 5 (Embree/Kernel.cpp:68); specular
     ps.film[i] = make_float2(fp.x, fp.y);
     q.ext[0][i] = i;
 }
@@ -81,10 +98,10 @@ __global__ void kBounceBegin(Queues q, uint32_t nextSlot) {
 // ------------------------------------------------------------------------------------------
 // extend: closest hit for every queued path
 // ------------------------------------------------------------------------------------------
-template <int W, bool COUNT>
-__global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const uint32_t* __restrict__ queue,
+template <int W, bool COUNT, bool BATCH>
+__global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const uint32_t* __restrict__ queue,
                                               const uint32_t* __restrict__ count, uint2* spill, uint32_t spillPerLane,
-                                              unsigned long long* travCounts) {
+                                              unsigned long long* travCounts, BatchTable bt) {
     __shared__ uint2 lstack[kLdsStack * 64];
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x;
@@ -93,6 +110,7 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s, PathState ps, const ui
     TravCount cnt;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const uint32_t path = queue[i];
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[path].z)] : s0;
         const float4 o = ps.o[path], d = ps.d[path];
         const Ray r{xyz(o), xyz(d)};
         const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt);
@@ -117,14 +135,12 @@ __device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
     return gs.train && (g % gs.trainStride) == gs.trainPhase;
 }
 
-__global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q, RenderConsts rc, uint32_t curSlot,
-                                              uint32_t nextSlot, uint64_t* totals, GuideShade gs) {
+template <bool BATCH>
+__global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot,
+                                              uint32_t nextSlot, uint64_t* totals, GuideShade gs, BatchTable bt) {
     const uint32_t n = q.counts[curSlot];
     const uint32_t* qcur = q.ext[curSlot];
     uint32_t* qnext = q.ext[nextSlot];
-    const bool d1 = rc.flags & kCompatD1, d3 = rc.flags & kCompatD3, d9 = rc.flags & kCompatD9,
-               d13 = rc.flags & kCompatD13;
-    const bool envBlack = !valid(ld3(s.env));
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop
         const uint32_t i = i0 + threadIdx.x;
@@ -137,6 +153,11 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
             const float4 hit4 = ps.hit[path];
             const uint32_t pos = __float_as_uint(hit4.y);
             uint4 meta = ps.meta[path];
+            const DevScene& s = BATCH ? bt.scenes[sceneOf(meta.z)] : s0;
+            const RenderConsts& rc = BATCH ? bt.rc[sceneOf(meta.z)] : rc0;
+            const bool d1 = rc.flags & kCompatD1, d3 = rc.flags & kCompatD3, d9 = rc.flags & kCompatD9,
+                       d13 = rc.flags & kCompatD13;
+            const bool envBlack = !valid(ld3(s.env));
             Ray ray{xyz(ps.o[path]), xyz(ps.d[path])};
             V3 thr = xyz(ps.thr[path]);
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
@@ -328,8 +349,9 @@ __global__ void __launch_bounds__(256) kShade(DevScene s, PathState ps, Queues q
 // ------------------------------------------------------------------------------------------
 // shadow: occlusion test of NEE rays (sampleLightSource, PathTracing/Kernel.cpp:40-62)
 // ------------------------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(64) kShadow(DevScene s, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane) {
+template <int W, bool BATCH>
+__global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
+                                              BatchTable bt) {
     __shared__ uint2 lstack[kLdsStack * 64];
     const uint32_t n = q.counts[kCntShadow];
     const uint32_t lane = threadIdx.x;
@@ -337,6 +359,7 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s, PathState ps, Queues q
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const ShadowRec r = q.shadow[i];
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[r.path & ~kMisFlag].z)] : s0;
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs)) {
             if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
@@ -354,16 +377,18 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s, PathState ps, Queues q
 // mis: the separate BSDF-sampled ray of sampleBSDF (PathTracing/Kernel.cpp:63-102); counts only
 // if it hits the selected light (or escapes to the environment)
 // ------------------------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane) {
+template <int W, bool BATCH>
+__global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
+                                           BatchTable bt) {
     __shared__ uint2 lstack[kLdsStack * 64];
     const uint32_t n = q.counts[kCntMis];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
-    const bool envBlack = !valid(ld3(s.env));
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const MisRec r = q.mis[i];
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[r.path].z)] : s0;
+        const bool envBlack = !valid(ld3(s.env));
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         V3 rad{0.0f, 0.0f, 0.0f};
         float lpdf = 0.0f;
@@ -421,8 +446,9 @@ __global__ void __launch_bounds__(64) kMis(DevScene s, PathState ps, Queues q, u
 // each source pixel's samples into the four 2x2 footprints they can start at, and kResolve has
 // every film pixel gather the footprints that cover it from its 3x3 source neighbourhood. Every
 // sum runs in a fixed order, so the film is bit-reproducible.
+template <bool BATCH>
 __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* recL, float* recY, GuideShade gs,
-                                               GuideRecordOut tr) {
+                                               GuideRecordOut tr, BatchTable bt) {
     const uint32_t n = q.counts[kCntDone];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t path = q.done[i];
@@ -430,9 +456,10 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* r
         const float2 fp = ps.film[path];
         const uint4 meta = ps.meta[path];
         const uint32_t g = meta.w;
-        recL[g] = make_float4(L.x, L.y, L.z, fp.x);
-        recY[g] = fp.y;
-        if (!isTrainPath(gs, g)) continue;
+        const size_t slot = BATCH ? bt.recBase[sceneOf(meta.z)] + g : g;
+        recL[slot] = make_float4(L.x, L.y, L.z, fp.x);
+        recY[slot] = fp.y;
+        if (BATCH || !isTrainPath(gs, g)) continue;
         // Each recorded vertex learns from the radiance that reached it along its sampled
         // direction: (L_final - L after its direct light) / throughput after the bounce.
         const uint32_t cnt = (meta.z >> 8) & 0xFFu;
@@ -575,49 +602,81 @@ static inline uint32_t blocksFor(uint64_t n, uint32_t bs, uint32_t cap) {
 }
 
 void launchRaygen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                  uint64_t first, uint32_t n) {
-    hipLaunchKernelGGL(kRaygen, dim3(blocksFor(n, 256, 1u << 30)), dim3(256), 0, st, s, ps, q, rc, first, n);
+                  uint64_t first, uint32_t n, const BatchTable* bt) {
+    const dim3 grid(blocksFor(n, 256, 1u << 30));
+    if (bt) hipLaunchKernelGGL(kRaygen<true>, grid, dim3(256), 0, st, s, ps, q, rc, *bt, first, n);
+    else hipLaunchKernelGGL(kRaygen<false>, grid, dim3(256), 0, st, s, ps, q, rc, BatchTable{}, first, n);
 }
 void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot) {
     hipLaunchKernelGGL(kBounceBegin, dim3(1), dim3(1), 0, st, q, nextSlot);
 }
+
+template <int W>
+static void extendW(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
+                    const uint32_t* count, dim3 grid, uint2* spill, uint32_t spillPerLane, unsigned long long* tc,
+                    const BatchTable* bt) {
+    if (bt) {
+        if (tc) hipLaunchKernelGGL((kExtend<W, true, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
+                                   spillPerLane, tc, *bt);
+        else hipLaunchKernelGGL((kExtend<W, false, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
+                                spillPerLane, tc, *bt);
+    } else {
+        if (tc) hipLaunchKernelGGL((kExtend<W, true, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
+                                   spillPerLane, tc, BatchTable{});
+        else hipLaunchKernelGGL((kExtend<W, false, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
+                                spillPerLane, tc, BatchTable{});
+    }
+}
+
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
-                  uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts) {
+                  uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
+                  const BatchTable* bt) {
     const dim3 grid(gridLanes / 64);
     auto* tc = reinterpret_cast<unsigned long long*>(travCounts);
     const uint32_t* queue = q.ext[curSlot];
     const uint32_t* count = q.counts + curSlot;
-    if (s.width == 8) {
-        if (tc) hipLaunchKernelGGL((kExtend<8, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
-        else hipLaunchKernelGGL((kExtend<8, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
-    } else {
-        if (tc) hipLaunchKernelGGL((kExtend<4, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
-        else hipLaunchKernelGGL((kExtend<4, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill, spillPerLane, tc);
-    }
+    if (s.width == 8) extendW<8>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
+    else extendW<4>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs) {
-    hipLaunchKernelGGL(kShade, dim3(gridThreads / 256), dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs);
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
+                 const BatchTable* bt) {
+    const dim3 grid(gridThreads / 256);
+    if (bt)
+        hipLaunchKernelGGL(kShade<true>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs, *bt);
+    else
+        hipLaunchKernelGGL(kShade<false>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
+                           BatchTable{});
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-                  uint2* spill, uint32_t spillPerLane) {
+                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
     const dim3 grid(gridLanes / 64);
-    if (s.width == 8)
-        hipLaunchKernelGGL(kShadow<8>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
-    else
-        hipLaunchKernelGGL(kShadow<4>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
+    const BatchTable b = bt ? *bt : BatchTable{};
+    if (s.width == 8) {
+        if (bt) hipLaunchKernelGGL((kShadow<8, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kShadow<8, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+    } else {
+        if (bt) hipLaunchKernelGGL((kShadow<4, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kShadow<4, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+    }
 }
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-               uint2* spill, uint32_t spillPerLane) {
+               uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
     const dim3 grid(gridLanes / 64);
-    if (s.width == 8)
-        hipLaunchKernelGGL(kMis<8>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
-    else
-        hipLaunchKernelGGL(kMis<4>, grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane);
+    const BatchTable b = bt ? *bt : BatchTable{};
+    if (s.width == 8) {
+        if (bt) hipLaunchKernelGGL((kMis<8, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kMis<8, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+    } else {
+        if (bt) hipLaunchKernelGGL((kMis<4, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kMis<4, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+    }
 }
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY, const GuideShade& gs, const GuideRecordOut& tr) {
-    hipLaunchKernelGGL(kRecord, dim3(gridThreads / 256), dim3(256), 0, st, ps, q, recL, recY, gs, tr);
+                  float* recY, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt) {
+    const dim3 grid(gridThreads / 256);
+    if (bt) hipLaunchKernelGGL(kRecord<true>, grid, dim3(256), 0, st, ps, q, recL, recY, gs, tr, *bt);
+    else hipLaunchKernelGGL(kRecord<false>, grid, dim3(256), 0, st, ps, q, recL, recY, gs, tr, BatchTable{});
 }
 void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
                    float* film32, double* film64) {

@@ -20,7 +20,7 @@ struct PathState {
 };
 
 // meta.z bits: 0 camera/specular ray, 1 a guided vertex awaits its radiance snapshot,
-// 8..15 guided vertices recorded for training
+// 8..15 guided vertices recorded for training, 16..31 work item of a batched render
 constexpr uint32_t kMetaPending = 0x2u;
 
 // Guiding state handed to the shade and record stages.
@@ -48,19 +48,35 @@ struct Queues {
     uint32_t* counts;  // kCntNum u32
 };
 
+// Device tables of a batched render. Its work items (a scene and a sample sub-range each) have
+// consecutive path numbers, item k owning [pathOffsets[k], pathOffsets[k + 1]); a path carries
+// its item id in meta.z bits 16..31, and item k's film records start at record slot recBase[k].
+constexpr uint32_t kMaxBatchItems = 1u << 16;
+struct BatchTable {
+    const DevScene* scenes;       // per item: its scene
+    const RenderConsts* rc;       // per item: rect, sample sub-range, depth, flags
+    const uint64_t* pathOffsets;  // n + 1 entries
+    const uint64_t* recBase;
+    uint32_t n;
+};
+
+// `bt` selects the batched kernels (nullptr: one scene, `s` and `rc` in kernel arguments).
+// Batched launches still take the first scene as `s`: all scenes share its BVH width.
 void launchRaygen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                  uint64_t first, uint32_t n);
+                  uint64_t first, uint32_t n, const BatchTable* bt = nullptr);
 void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot);
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
-                  uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts);
+                  uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
+                  const BatchTable* bt = nullptr);
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs);
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
+                 const BatchTable* bt = nullptr);
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-                  uint2* spill, uint32_t spillPerLane);
+                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-               uint2* spill, uint32_t spillPerLane);
+               uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY, const GuideShade& gs, const GuideRecordOut& tr);
+                  float* recY, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt = nullptr);
 // quad: 16 floats (or doubles with film64) per rect pixel
 void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
                    float* film32, double* film64);

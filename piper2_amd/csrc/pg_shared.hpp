@@ -112,5 +112,6 @@ struct RenderConsts {
 constexpr uint32_t kCompatD1 = 0x1u, kCompatD3 = 0x2u, kCompatD9 = 0x4u, kCompatD13 = 0x8u;
 constexpr uint32_t kFilmFp64 = 0x100u;
 constexpr uint32_t kCountTraversal = 0x200u;
+constexpr uint32_t kNoStageTiming = 0x400u;
 
 }  // namespace pg

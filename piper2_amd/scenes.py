@@ -248,6 +248,29 @@ def terrain_slit(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3,
     return out
 
 
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 768), (1920, 1080))
+BATCH_SPP = (4, 16, 64)
+
+
+def batch_scene(index, max_depth=8, min_rr=3):
+    """Scene `index` of the batched workload (seed 3 + index): a displaced fBm grid plus instanced
+    icospheres (about 30% of the triangles), log-uniform in [1e4, 2e6] triangles in all, with the
+    terrain scene's materials and 16 lights; resolution drawn from BATCH_RESOLUTIONS, samples
+    per pixel from BATCH_SPP, max depth 8."""
+    rng = np.random.default_rng(3 + index)
+    total = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
+    width, height = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+    spp = BATCH_SPP[int(rng.integers(len(BATCH_SPP)))]
+    n_spheres = max(1, int(round(0.3 * total / 1280)))
+    grid_tris = max(total - 1280 * n_spheres, 64.0)
+    cz = max(2, int(round(math.sqrt(grid_tris / 4))))  # 2 cx cz cells' triangles, cx = 2 cz
+    cx = max(2, int(grid_tris // (2 * cz)))
+    sc = terrain_spheres(seed=int(rng.integers(1 << 31)), width=width, height=height, spp=spp, max_depth=max_depth,
+                         min_rr=min_rr, grid=(cx + 1, cz + 1), n_spheres=n_spheres)
+    sc["name"] = f"batch_{index}"
+    return sc
+
+
 def _grid_quad(origin, u, v, n):
     """A parallelogram tessellated into n x n cells (2 n^2 triangles)."""
     origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))
