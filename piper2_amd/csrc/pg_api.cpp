@@ -85,6 +85,11 @@ void upload(DevBuf& b, const void* host, size_t n, hipStream_t st) {
     if (n) PG_HIP(hipMemcpyAsync(b.p, host, n, hipMemcpyHostToDevice, st));
 }
 
+// Render progress the host reads back: the pool counters, then the recorded-path counts of up to
+// kStatusItems work items from the oldest unresolved one.
+constexpr size_t kStatusItems = 64;
+constexpr size_t kStatusBytes = pg::kCountsBytes + kStatusItems * 4;
+
 }  // namespace
 
 struct pg_device {
@@ -101,10 +106,12 @@ struct pg_device {
     DevBuf sobol, sobolT, sobolB, sobolInv, sobolDelta;
     // render workspace
     uint32_t capacity = 0;
-    DevBuf recL, recY, quad;  // film records of one segment and per-pixel footprint sums
+    DevBuf recL, recY, quad;  // film record ring and per-pixel footprint sums
+    DevBuf itemDone;          // paths recorded per work item
     DevBuf mix, hist, trainRec, trainCount;  // guiding: mixtures, vertex history, training records
     DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
-    DevBuf batchScenes, batchRc, batchOffsets, batchRecBase;  // work-item tables of a batched render
+    DevBuf batchScenes, batchRc, batchOffsets;  // work-item tables of a batched render
+    void* status = nullptr;   // pinned host copy of the pool counters and item progress
     size_t spillBytes = 0;
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
@@ -160,6 +167,7 @@ void freeDevice(pg_device* dev) {
     (void)hipSetDevice(dev->ordinal);
     (void)hipStreamSynchronize(dev->stream);
     for (auto e : dev->events) (void)hipEventDestroy(e);
+    if (dev->status) (void)hipHostFree(dev->status);
     (void)hipStreamDestroy(dev->stream);
     delete dev;
 }
@@ -298,7 +306,7 @@ void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
         dev->qshadow.reserve(2 * P * sizeof(pg::ShadowRec)), dev->qmis.reserve(P * sizeof(pg::MisRec));
         dev->capacity = cap;
     }
-    dev->counts.reserve(pg::kCntNum * 4);
+    dev->counts.reserve(pg::kCountsBytes);
     dev->totals.reserve(pg::kTotNum * 8);
     dev->travCounts.reserve(64);
     dev->spill.reserve(std::max<size_t>(spillBytes, 64));
@@ -314,7 +322,8 @@ pg::Queues queues(pg_device* dev) {
                       dev->qshadow.as<pg::ShadowRec>(),
                       dev->qmis.as<pg::MisRec>(),
                       dev->qdone.as<uint32_t>(),
-                      dev->counts.as<uint32_t>()};
+                      dev->counts.as<uint32_t>(),
+                      reinterpret_cast<pg::PoolPaths*>(dev->counts.as<char>() + pg::kCntNum * 4)};
 }
 
 uint32_t traversalLanes(pg_device* dev, uint64_t work) {
@@ -371,35 +380,116 @@ uint32_t segmentSpp(const pg::RenderConsts& rc) {
     return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(rc.sppCount, kMaxSegmentPaths / pixels)));
 }
 
-void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats,
-            pg_guide* guide = nullptr) {
-    require(s && params, "null scene or params");
-    pg_device* dev = s->dev;
-    require(!guide || guide->dev == dev, "guide and scene live on different devices");
-    std::lock_guard<std::mutex> lock(dev->mu);
-    dev->checkFault();
-    const pg::RenderConsts rc = makeConsts(*params);
-    require(!film || (film->width == rc.width && film->height == rc.height), "film size differs from params");
+// Work item: one scene's whole rect x a sample sub-range (a film segment), with path numbers
+// [first, first + count) in its render call.
+struct WorkItem {
+    uint32_t scene;
+    pg::RenderConsts rc;  // rect, the item's sample sub-range, depth, flags
+    uint64_t first, count;
+};
+
+// Splits a scene's render into segments of rc.segSpp samples per pixel, appended to `items`.
+void appendSegments(uint32_t scene, const pg::RenderConsts& rc, std::vector<WorkItem>& items, uint64_t& total) {
+    const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
+    for (uint32_t s0 = rc.sppBegin; s0 < rc.sppBegin + rc.sppCount; s0 += rc.segSpp) {
+        WorkItem it{scene, rc, total, 0};
+        it.rc.sppBegin = s0;
+        it.rc.sppCount = std::min(rc.segSpp, rc.sppBegin + rc.sppCount - s0);
+        it.rc.segSpp = it.rc.sppCount;
+        it.count = pixels * it.rc.sppCount;
+        total += it.count;
+        items.push_back(it);
+    }
+}
+
+struct RunOptions {
+    bool batch = false;           // per-item scene and consts from device tables
+    pg::RenderConsts rc0{};       // single scene: the call's consts (segments by rc0.segSpp)
+    pg_film* const* films = nullptr;  // per scene (entries may be null)
+    uint32_t capacity = 1u << 21;
+    pg_guide* guide = nullptr;    // single scene only
+    float* pathLHost = nullptr;   // single scene only
+    double* latencyMs = nullptr;  // per scene
+    pg_stats* stats = nullptr;
+    bool countTraversal = false;
+    bool timed = false;
+};
+
+// Iterations between two reads of the render's progress (each read waits for the stream).
+constexpr uint32_t kCheckEvery = 4;
+// Iterations without any progress after which a render is abandoned as stuck.
+constexpr uint32_t kStallLimit = 1u << 16;
+
+// Renders work items through one pool of path slots. Every iteration restarts the slots of
+// finished paths with the next path numbers (in order), then runs one bounce for every live
+// path: extend, (guide inference), shade, shadow, (MIS), record. Film records go to a ring
+// indexed by path number; an item is resolved into its film as soon as all its paths are
+// recorded, and new paths may only take ring slots that no unresolved item still needs.
+void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const std::vector<WorkItem>& items,
+              uint64_t total, const RunOptions& opt) {
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = dev->stream;
-    const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
-    const uint64_t total = pixels * rc.sppCount;
-    const uint32_t segSpp = segmentSpp(rc);
-    const uint64_t segMax = pixels * segSpp;
-    uint32_t cap = params->wavefront_paths ? params->wavefront_paths : (1u << 21);
-    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, segMax));
+    require(!items.empty() && total > 0, "nothing to render");
+    require(items.size() <= pg::kMaxBatchItems, "too many film segments in one call: split it");
+    uint64_t maxItem = 0;
+    uint32_t maxDepth = 0, spillPerLane = 1;
+    bool envBlack = true;
+    size_t quadBytes = 16;
+    for (const WorkItem& it : items) {
+        maxItem = std::max(maxItem, it.count);
+        maxDepth = std::max(maxDepth, it.rc.maxDepth);
+        quadBytes = std::max<size_t>(quadBytes, static_cast<size_t>(it.rc.rectW) * it.rc.rectH * 16 *
+                                                    ((it.rc.flags & pg::kFilmFp64) ? 8 : 4));
+    }
+    for (uint32_t i = 0; i < nScenes; ++i) {
+        spillPerLane = std::max(spillPerLane, scenes[i]->spillPerLane);
+        const float* env = scenes[i]->ds.env;
+        envBlack = envBlack && !(env[0] > 0.0f || env[1] > 0.0f || env[2] > 0.0f);
+    }
+    const uint32_t cap = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(opt.capacity, total)));
     const uint32_t lanes = traversalLanes(dev, cap);
     const uint32_t threads = shadeThreads(dev, cap);
-    ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * s->spillPerLane * 8);
-    const bool fp64 = rc.flags & pg::kFilmFp64;
-    dev->recL.reserve(segMax * 16), dev->recY.reserve(segMax * 4);
-    dev->quad.reserve(pixels * 16 * (fp64 ? 8 : 4));
+    ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * spillPerLane * 8);
+    // Ring: the largest item plus room for the spread of live path numbers (paths finish out of
+    // order). A ring too small only throttles the pool; one item must always fit.
+    const uint64_t ringSize =
+        opt.pathLHost ? total : std::min(total, 2 * maxItem + 8 * static_cast<uint64_t>(cap));
+    dev->recL.reserve(ringSize * 16), dev->recY.reserve(ringSize * 4);
+    dev->quad.reserve(quadBytes);
+    dev->itemDone.reserve(std::max<size_t>(items.size(), kStatusItems) * 4);
+    PG_HIP(hipMemsetAsync(dev->itemDone.p, 0, items.size() * 4, st));
+    const pg::RecordRing ring{dev->recL.as<float4>(), dev->recY.as<float>(), ringSize, dev->itemDone.as<uint32_t>()};
+
+    pg::BatchTable bt{};
+    std::vector<pg::DevScene> tabScene;
+    std::vector<pg::RenderConsts> tabRc;
+    std::vector<uint64_t> tabOffsets;
+    if (opt.batch) {
+        tabScene.resize(items.size()), tabRc.resize(items.size()), tabOffsets.resize(items.size() + 1);
+        for (size_t k = 0; k < items.size(); ++k) {
+            tabScene[k] = scenes[items[k].scene]->ds;
+            tabRc[k] = items[k].rc;
+            tabOffsets[k] = items[k].first;
+        }
+        tabOffsets[items.size()] = total;
+        upload(dev->batchScenes, tabScene.data(), tabScene.size() * sizeof(pg::DevScene), st);
+        upload(dev->batchRc, tabRc.data(), tabRc.size() * sizeof(pg::RenderConsts), st);
+        upload(dev->batchOffsets, tabOffsets.data(), tabOffsets.size() * 8, st);
+        bt = pg::BatchTable{dev->batchScenes.as<pg::DevScene>(), dev->batchRc.as<pg::RenderConsts>(),
+                            dev->batchOffsets.as<uint64_t>(), static_cast<uint32_t>(items.size())};
+    }
+    const pg::BatchTable* btp = opt.batch ? &bt : nullptr;
+    const pg::DevScene& ds0 = scenes[0]->ds;  // the batch's BVH width; per-path scenes come from bt
+    const pg::RenderConsts& rc0 = opt.batch ? items[0].rc : opt.rc0;
+
     const pg::PathState ps = pathState(dev);
     const pg::Queues q = queues(dev);
     // guiding: mixture per extension-queue entry, per-slot vertex history of training paths
     pg::GuideShade gs{};
     pg::GuideRecordOut tr{};
+    pg_guide* guide = opt.guide;
     if (guide) {
+        const pg_scene* s = scenes[0];
         gs.enabled = 1;
         gs.alpha = guide->p.mis_fraction;
         gs.lo = make_float3(s->lo[0], s->lo[1], s->lo[2]);
@@ -408,7 +498,7 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
         gs.mix = dev->mix.as<float>();
         if (guide->p.train_records > 0) {
             gs.train = 1;
-            gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(rc.maxDepth, 255));
+            gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(rc0.maxDepth, 255));
             // about three guided vertices per path: pick every stride-th path for training
             gs.trainStride = static_cast<uint32_t>(std::max<uint64_t>(1, total * 3 / guide->p.train_records));
             gs.trainPhase = gs.trainStride / 2;
@@ -421,79 +511,114 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
         }
         pg::guideConfigureLds();
     }
-    float* film32 = nullptr;
-    double* film64 = nullptr;
-    if (film) {
-        if (fp64) {
-            if (!film->rgbw64.p) {
-                film->rgbw64.reserve(static_cast<size_t>(film->width) * film->height * 32);
-                PG_HIP(hipMemsetAsync(film->rgbw64.p, 0, film->rgbw64.bytes, st));
-            }
-            film64 = film->rgbw64.as<double>();
-        } else {
-            film32 = film->rgbw32.as<float>();
+    for (uint32_t i = 0; i < nScenes; ++i) {  // FP64 films get their buffer on first use
+        pg_film* f = opt.films ? opt.films[i] : nullptr;
+        bool fp64 = false;
+        for (const WorkItem& it : items) fp64 = fp64 || (it.scene == i && (it.rc.flags & pg::kFilmFp64));
+        if (f && fp64 && !f->rgbw64.p) {
+            f->rgbw64.reserve(static_cast<size_t>(f->width) * f->height * 32);
+            PG_HIP(hipMemsetAsync(f->rgbw64.p, 0, f->rgbw64.bytes, st));
         }
     }
     PG_HIP(hipMemsetAsync(dev->totals.p, 0, pg::kTotNum * 8, st));
-    const bool countTrav = rc.flags & pg::kCountTraversal;
-    uint32_t* trav = countTrav ? dev->travCounts.as<uint32_t>() : nullptr;
-    if (countTrav) PG_HIP(hipMemsetAsync(dev->travCounts.p, 0, 64, st));
+    uint32_t* trav = opt.countTraversal ? dev->travCounts.as<uint32_t>() : nullptr;
+    if (trav) PG_HIP(hipMemsetAsync(dev->travCounts.p, 0, 64, st));
     dev->eventsUsed = 0;
-    StageTimer timeStage{dev, st, stats != nullptr && !(rc.flags & pg::kNoStageTiming), {}};
-    // With a black environment the shade stage resolves MIS-BSDF rays as shadow-queue occlusion
-    // queries, and the closest-hit MIS queue stays empty.
-    const bool envBlack = !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
-    uint64_t extendLaunches = 0;
-    uint2* spill = dev->spill.as<uint2>();
-    float4* recL = dev->recL.as<float4>();
-    float* recY = dev->recY.as<float>();
-    std::vector<float> segHost;
-    for (uint32_t s0 = rc.sppBegin; s0 < rc.sppBegin + rc.sppCount; s0 += segSpp) {
-        pg::RenderConsts seg = rc;
-        seg.sppBegin = s0;
-        seg.sppCount = std::min(segSpp, rc.sppBegin + rc.sppCount - s0);
-        const uint64_t segTotal = pixels * seg.sppCount;
-        for (uint64_t first = 0; first < segTotal; first += cap) {
-            const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(cap, segTotal - first));
-            timeStage(kRaygen, [&] { pg::launchRaygen(st, s->ds, ps, q, seg, first, n); });
-            uint32_t cur = 0;
-            for (uint32_t iter = 0;; ++iter) {
-                if (iter > seg.maxDepth) {
-                    uint32_t left = 0;
-                    PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
-                    PG_HIP(hipStreamSynchronize(st));
-                    if (left == 0) break;
-                }
-                const uint32_t nxt = cur ^ 1u;
-                pg::launchBounceBegin(st, q, nxt);
-                timeStage(kExtend, [&] { pg::launchExtend(st, s->ds, ps, q, cur, lanes, spill, s->spillPerLane, trav); });
-                ++extendLaunches;
-                if (guide)
-                    timeStage(kGuide, [&] {
-                        pg::launchGuideInferPaths(st, s->ds, ps, q.ext[cur], q.counts + cur, n, gs.lo, gs.invExt,
-                                                  guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
-                                                  static_cast<uint32_t>(dev->numCUs) * 2);
-                    });
-                timeStage(kShade, [&] {
-                    pg::launchShade(st, s->ds, ps, q, seg, cur, nxt, threads, dev->totals.as<uint64_t>(), gs);
-                });
-                timeStage(kShadow, [&] { pg::launchShadow(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-                if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, s->ds, ps, q, lanes, spill, s->spillPerLane); });
-                timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY, gs, tr); });
-                PG_HIP(hipGetLastError());
-                cur = nxt;
-            }
+    StageTimer timeStage{dev, st, opt.timed, {}};
+    hipEvent_t start = nullptr;
+    std::vector<hipEvent_t> sceneDone(nScenes, nullptr);
+    std::vector<size_t> lastItem(nScenes, 0);
+    for (size_t k = 0; k < items.size(); ++k) lastItem[items[k].scene] = k;
+    if (opt.latencyMs) {
+        start = dev->event();
+        PG_HIP(hipEventRecord(start, st));
+    }
+
+    size_t nextResolve = 0;
+    auto resolveItem = [&](size_t k) {
+        const WorkItem& it = items[k];
+        pg_film* f = opt.films ? opt.films[it.scene] : nullptr;
+        if (f) {
+            const bool fp64 = it.rc.flags & pg::kFilmFp64;
+            timeStage(kFilm, [&] {
+                pg::launchResolve(st, it.rc, ring, it.first, dev->quad.p, fp64 ? nullptr : f->rgbw32.as<float>(),
+                                  fp64 ? f->rgbw64.as<double>() : nullptr);
+            });
         }
-        if (film) timeStage(kFilm, [&] { pg::launchResolve(st, seg, recL, recY, dev->quad.p, film32, film64); });
+        if (opt.latencyMs && lastItem[it.scene] == k) {
+            sceneDone[it.scene] = dev->event();
+            PG_HIP(hipEventRecord(sceneDone[it.scene], st));
+        }
+    };
+    // new paths may take ring slots up to the oldest unresolved item's first path number + ring
+    auto limitNow = [&] {
+        return nextResolve < items.size() ? std::min(total, items[nextResolve].first + ringSize) : total;
+    };
+
+    timeStage(kRaygen, [&] { pg::launchPoolInit(st, q, cap); });
+    uint2* spill = dev->spill.as<uint2>();
+    uint64_t iterations = 0, lastNext = ~0ull;
+    uint32_t lastFree = ~0u, stalled = 0;
+    uint32_t cur = 0;
+    auto* statusCounts = static_cast<const uint32_t*>(dev->status);
+    auto* statusPool = reinterpret_cast<const pg::PoolPaths*>(static_cast<const char*>(dev->status) + pg::kCntNum * 4);
+    auto* statusItems = reinterpret_cast<const uint32_t*>(static_cast<const char*>(dev->status) + pg::kCountsBytes);
+    for (;;) {
+        const uint32_t nxt = cur ^ 1u;
+        const uint64_t limit = limitNow();
+        timeStage(kRaygen, [&] {
+            pg::launchIterBegin(st, q, cur, limit);
+            pg::launchRegen(st, ds0, ps, q, rc0, cur, threads, btp);
+        });
+        timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, trav, btp); });
+        if (guide)
+            timeStage(kGuide, [&] {
+                pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
+                                          guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
+                                          static_cast<uint32_t>(dev->numCUs) * 2);
+            });
+        timeStage(kShade, [&] {
+            pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp);
+        });
+        timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
+        if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
+        timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, rc0, threads, ring, gs, tr, btp); });
         PG_HIP(hipGetLastError());
-        if (pathLHost) {  // scatter the segment's records into the caller's path order
-            segHost.resize(segTotal * 4);
-            PG_HIP(hipMemcpyAsync(segHost.data(), recL, segTotal * 16, hipMemcpyDeviceToHost, st));
-            PG_HIP(hipStreamSynchronize(st));
+        cur = nxt;
+        ++iterations;
+        if (iterations % kCheckEvery != 0) continue;
+        // progress: pool counters and the recorded-path counts of the oldest unresolved items
+        const size_t window = std::min(kStatusItems, items.size() - std::min(nextResolve, items.size()));
+        PG_HIP(hipMemcpyAsync(dev->status, dev->counts.p, pg::kCountsBytes, hipMemcpyDeviceToHost, st));
+        if (window)
+            PG_HIP(hipMemcpyAsync(static_cast<char*>(dev->status) + pg::kCountsBytes,
+                                  dev->itemDone.as<uint32_t>() + nextResolve, window * 4, hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        const size_t base = nextResolve;
+        while (nextResolve < items.size() && nextResolve - base < window &&
+               statusItems[nextResolve - base] == items[nextResolve].count)
+            resolveItem(nextResolve++);
+        const uint64_t next = statusPool->next;
+        const uint32_t freeSlots = statusCounts[pg::kCntDone];
+        if (next == total && freeSlots == cap) break;  // every path started, finished and recorded
+        stalled = (next == lastNext && freeSlots == lastFree) ? stalled + kCheckEvery : 0;
+        if (stalled > kStallLimit) throw PgError(PG_ERR_DEVICE, "render made no progress");
+        lastNext = next, lastFree = freeSlots;
+    }
+    while (nextResolve < items.size()) resolveItem(nextResolve++);  // all recorded by now
+    PG_HIP(hipGetLastError());
+
+    if (opt.pathLHost) {  // the ring holds every record (ringSize == total): scatter to caller order
+        std::vector<float> recHost(total * 4);
+        PG_HIP(hipMemcpyAsync(recHost.data(), ring.L, total * 16, hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        const pg::RenderConsts& rc = opt.rc0;
+        for (const WorkItem& it : items) {
+            const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
             for (uint64_t p = 0; p < pixels; ++p)
-                for (uint32_t k = 0; k < seg.sppCount; ++k) {
-                    const float* r = &segHost[4 * (p * seg.sppCount + k)];
-                    float* o = pathLHost + 3 * (p * rc.sppCount + (s0 - rc.sppBegin) + k);
+                for (uint32_t k = 0; k < it.rc.sppCount; ++k) {
+                    const float* r = &recHost[4 * (it.first + p * it.rc.sppCount + k)];
+                    float* o = opt.pathLHost + 3 * (p * rc.sppCount + (it.rc.sppBegin - rc.sppBegin) + k);
                     o[0] = r[0], o[1] = r[1], o[2] = r[2];
                 }
         }
@@ -510,7 +635,13 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
         }
     }
     PG_HIP(hipStreamSynchronize(st));
-    if (stats) {
+    if (opt.latencyMs)
+        for (uint32_t i = 0; i < nScenes; ++i) {
+            float e = 0.0f;
+            PG_HIP(hipEventElapsedTime(&e, start, sceneDone[i]));
+            opt.latencyMs[i] = e;
+        }
+    if (pg_stats* stats = opt.stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->train_records = trained;
         uint64_t tot[pg::kTotNum];
@@ -519,8 +650,8 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
         stats->extension_rays = total + tot[pg::kTotExt];
         stats->shadow_rays = tot[pg::kTotShadow];
         stats->mis_rays = tot[pg::kTotMis];
-        stats->extend_launches = extendLaunches;
-        if (countTrav) {
+        stats->extend_launches = iterations;
+        if (trav) {
             unsigned long long tc[2];
             PG_HIP(hipMemcpy(tc, dev->travCounts.p, sizeof(tc), hipMemcpyDeviceToHost));
             stats->nodes_visited = tc[0];
@@ -532,192 +663,63 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     }
 }
 
-// Work item of a batched render: one scene's segment (as render() splits it), its path numbers
-// [first, first + count) in the batch and its film records at ring slots [slot, slot + count).
-struct BatchItem {
-    uint32_t scene;
-    pg::RenderConsts rc;
-    uint64_t first, count, slot;
-};
-
-// Places the items' film records in a ring of `ring` slots. An item is resolved after the wave
-// that holds its last path, so item k may reuse the slots of every item that ends at or before
-// the first path of the wave that starts k. With ring >= 4 * (largest item) + 2 * cap the live
-// items never fill the ring; the check below guards that bound.
-void placeBatchRecords(std::vector<BatchItem>& items, uint64_t cap, uint64_t ring) {
-    uint64_t pos = 0;
-    size_t live = 0;  // oldest item still unresolved when the current item's first wave starts
-    for (size_t k = 0; k < items.size(); ++k) {
-        BatchItem& it = items[k];
-        if (pos + it.count > ring) pos = 0;
-        const uint64_t waveStart = it.first / cap * cap;
-        while (live < k && items[live].first + items[live].count <= waveStart) ++live;
-        for (size_t j = live; j < k; ++j)
-            if (pos < items[j].slot + items[j].count && items[j].slot < pos + it.count)
-                throw PgError(PG_ERR_DEVICE, "batched render: film record ring overrun");
-        it.slot = pos;
-        pos += it.count;
-    }
+void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* pathLHost, pg_stats* stats,
+            pg_guide* guide = nullptr) {
+    require(s && params, "null scene or params");
+    pg_device* dev = s->dev;
+    require(!guide || guide->dev == dev, "guide and scene live on different devices");
+    std::lock_guard<std::mutex> lock(dev->mu);
+    dev->checkFault();
+    pg::RenderConsts rc = makeConsts(*params);
+    require(!film || (film->width == rc.width && film->height == rc.height), "film size differs from params");
+    rc.segSpp = segmentSpp(rc);
+    std::vector<WorkItem> items;
+    uint64_t total = 0;
+    appendSegments(0, rc, items, total);
+    RunOptions opt;
+    opt.rc0 = rc;
+    pg_film* films[1] = {film};
+    opt.films = films;
+    opt.capacity = params->wavefront_paths ? params->wavefront_paths : (1u << 21);
+    opt.guide = guide;
+    opt.pathLHost = pathLHost;
+    opt.stats = stats;
+    opt.countTraversal = rc.flags & pg::kCountTraversal;
+    opt.timed = stats != nullptr && !(rc.flags & pg::kNoStageTiming);
+    runPaths(dev, &s, 1, items, total, opt);
 }
 
-// Batched render: the paths of all scenes run through one wavefront pipeline (raygen numbers
-// them item after item), so a wave mixes scenes, depths and resolutions and small scenes no
-// longer leave the device idle. Per path the arithmetic is that of render(), and every scene is
-// resolved segment by segment in the same order, so each film is bit-identical to its own render.
+// Batched render: the paths of all scenes run through one path pool (their segments numbered
+// one after another), so the pool mixes scenes, depths and resolutions and small scenes no longer
+// leave the device idle. Per path the arithmetic is that of render(), and every scene is resolved
+// segment by segment in the same order, so each film is bit-identical to its own render.
 void renderBatch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
                  pg_film* const* films, double* latencyMs, pg_stats* stats) {
     require(dev && scenes && params && n > 0, "empty batch");
     std::lock_guard<std::mutex> lock(dev->mu);
     dev->checkFault();
-    const auto t0 = std::chrono::steady_clock::now();
-    hipStream_t st = dev->stream;
-    std::vector<BatchItem> items;
-    uint64_t total = 0, maxItem = 0;
-    uint32_t maxDepth = 0, spillPerLane = 1;
-    bool envBlack = true;
-    size_t quadBytes = 16;
+    std::vector<WorkItem> items;
+    uint64_t total = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const pg_scene* s = scenes[i];
         require(s && s->dev == dev, "every batched scene must live on the batch's device");
         require(s->ds.width == scenes[0]->ds.width, "batched scenes must share one BVH width");
-        const pg::RenderConsts rc = makeConsts(params[i]);
+        pg::RenderConsts rc = makeConsts(params[i]);
         require(!(rc.flags & pg::kCountTraversal), "traversal counting is not available in batched renders");
         const pg_film* f = films ? films[i] : nullptr;
         require(!f || (f->dev == dev && f->width == rc.width && f->height == rc.height),
                 "film size differs from params");
-        const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
-        const uint32_t segSpp = segmentSpp(rc);
-        for (uint32_t s0 = rc.sppBegin; s0 < rc.sppBegin + rc.sppCount; s0 += segSpp) {
-            BatchItem it{i, rc, total, 0, 0};
-            it.rc.sppBegin = s0;
-            it.rc.sppCount = std::min(segSpp, rc.sppBegin + rc.sppCount - s0);
-            it.count = pixels * it.rc.sppCount;
-            total += it.count;
-            maxItem = std::max(maxItem, it.count);
-            items.push_back(it);
-        }
-        maxDepth = std::max(maxDepth, rc.maxDepth);
-        spillPerLane = std::max(spillPerLane, s->spillPerLane);
-        envBlack = envBlack && !(s->ds.env[0] > 0.0f || s->ds.env[1] > 0.0f || s->ds.env[2] > 0.0f);
-        quadBytes = std::max<size_t>(quadBytes, pixels * 16 * ((rc.flags & pg::kFilmFp64) ? 8 : 4));
+        rc.segSpp = segmentSpp(rc);
+        appendSegments(i, rc, items, total);
     }
-    require(items.size() <= pg::kMaxBatchItems, "too many segments in one batch: split it");
-    uint32_t cap = params[0].wavefront_paths ? params[0].wavefront_paths : (1u << 21);
-    cap = static_cast<uint32_t>(std::min<uint64_t>(cap, total));
-    const uint64_t ring = std::min<uint64_t>(total, 4 * maxItem + 2 * static_cast<uint64_t>(cap));
-    placeBatchRecords(items, cap, ring);
-
-    const uint32_t lanes = traversalLanes(dev, cap);
-    const uint32_t threads = shadeThreads(dev, cap);
-    ensureWorkspace(dev, cap, static_cast<size_t>(lanes) * spillPerLane * 8);
-    dev->recL.reserve(ring * 16), dev->recY.reserve(ring * 4);
-    dev->quad.reserve(quadBytes);
-    std::vector<pg::DevScene> tabScene(items.size());
-    std::vector<pg::RenderConsts> tabRc(items.size());
-    std::vector<uint64_t> tabOffsets(items.size() + 1), tabRecBase(items.size());
-    std::vector<size_t> lastItem(n);
-    for (size_t k = 0; k < items.size(); ++k) {
-        tabScene[k] = scenes[items[k].scene]->ds;
-        tabRc[k] = items[k].rc;
-        tabOffsets[k] = items[k].first;
-        tabRecBase[k] = items[k].slot;
-        lastItem[items[k].scene] = k;
-    }
-    tabOffsets[items.size()] = total;
-    upload(dev->batchScenes, tabScene.data(), tabScene.size() * sizeof(pg::DevScene), st);
-    upload(dev->batchRc, tabRc.data(), tabRc.size() * sizeof(pg::RenderConsts), st);
-    upload(dev->batchOffsets, tabOffsets.data(), tabOffsets.size() * 8, st);
-    upload(dev->batchRecBase, tabRecBase.data(), tabRecBase.size() * 8, st);
-    const pg::BatchTable bt{dev->batchScenes.as<pg::DevScene>(), dev->batchRc.as<pg::RenderConsts>(),
-                            dev->batchOffsets.as<uint64_t>(), dev->batchRecBase.as<uint64_t>(),
-                            static_cast<uint32_t>(items.size())};
-    for (uint32_t i = 0; i < n; ++i) {  // FP64 films get their buffer on first use
-        pg_film* f = films ? films[i] : nullptr;
-        if (f && (params[i].flags & pg::kFilmFp64) && !f->rgbw64.p) {
-            f->rgbw64.reserve(static_cast<size_t>(f->width) * f->height * 32);
-            PG_HIP(hipMemsetAsync(f->rgbw64.p, 0, f->rgbw64.bytes, st));
-        }
-    }
-    const pg::PathState ps = pathState(dev);
-    const pg::Queues q = queues(dev);
-    const pg::GuideShade gs{};
-    const pg::GuideRecordOut tr{};
-    const pg::DevScene& ds0 = scenes[0]->ds;  // BVH width of the batch; per-path scenes come from bt
-    const pg::RenderConsts& rc0 = items[0].rc;
-    PG_HIP(hipMemsetAsync(dev->totals.p, 0, pg::kTotNum * 8, st));
-    dev->eventsUsed = 0;
-    StageTimer timeStage{dev, st, stats != nullptr && !(params[0].flags & pg::kNoStageTiming), {}};
-    hipEvent_t start = dev->event();
-    std::vector<hipEvent_t> done(n, nullptr);
-    PG_HIP(hipEventRecord(start, st));
-    uint64_t extendLaunches = 0;
-    uint2* spill = dev->spill.as<uint2>();
-    float4* recL = dev->recL.as<float4>();
-    float* recY = dev->recY.as<float>();
-    size_t nextResolve = 0;
-    for (uint64_t first = 0; first < total; first += cap) {
-        const uint32_t wn = static_cast<uint32_t>(std::min<uint64_t>(cap, total - first));
-        timeStage(kRaygen, [&] { pg::launchRaygen(st, ds0, ps, q, rc0, first, wn, &bt); });
-        uint32_t cur = 0;
-        for (uint32_t iter = 0;; ++iter) {
-            if (iter > maxDepth) {
-                uint32_t left = 0;
-                PG_HIP(hipMemcpyAsync(&left, dev->counts.as<uint32_t>() + cur, 4, hipMemcpyDeviceToHost, st));
-                PG_HIP(hipStreamSynchronize(st));
-                if (left == 0) break;
-            }
-            const uint32_t nxt = cur ^ 1u;
-            pg::launchBounceBegin(st, q, nxt);
-            timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, nullptr, &bt); });
-            ++extendLaunches;
-            timeStage(kShade, [&] {
-                pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, &bt);
-            });
-            timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, &bt); });
-            if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, &bt); });
-            timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, threads, recL, recY, gs, tr, &bt); });
-            PG_HIP(hipGetLastError());
-            cur = nxt;
-        }
-        // resolve every item this wave finished, in item order
-        while (nextResolve < items.size() && items[nextResolve].first + items[nextResolve].count <= first + wn) {
-            const BatchItem& it = items[nextResolve];
-            pg_film* f = films ? films[it.scene] : nullptr;
-            if (f) {
-                const bool fp64 = it.rc.flags & pg::kFilmFp64;
-                timeStage(kFilm, [&] {
-                    pg::launchResolve(st, it.rc, recL + it.slot, recY + it.slot, dev->quad.p,
-                                      fp64 ? nullptr : f->rgbw32.as<float>(), fp64 ? f->rgbw64.as<double>() : nullptr);
-                });
-            }
-            if (lastItem[it.scene] == nextResolve) {
-                done[it.scene] = dev->event();
-                PG_HIP(hipEventRecord(done[it.scene], st));
-            }
-            ++nextResolve;
-        }
-        PG_HIP(hipGetLastError());
-    }
-    PG_HIP(hipStreamSynchronize(st));
-    if (latencyMs)
-        for (uint32_t i = 0; i < n; ++i) {
-            float e = 0.0f;
-            PG_HIP(hipEventElapsedTime(&e, start, done[i]));
-            latencyMs[i] = e;
-        }
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        uint64_t tot[pg::kTotNum];
-        PG_HIP(hipMemcpy(tot, dev->totals.p, sizeof(tot), hipMemcpyDeviceToHost));
-        stats->paths = total;
-        stats->extension_rays = total + tot[pg::kTotExt];
-        stats->shadow_rays = tot[pg::kTotShadow];
-        stats->mis_rays = tot[pg::kTotMis];
-        stats->extend_launches = extendLaunches;
-        timeStage.collect(stats);
-        stats->ms_total =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
+    RunOptions opt;
+    opt.batch = true;
+    opt.films = films;
+    opt.capacity = params[0].wavefront_paths ? params[0].wavefront_paths : (1u << 21);
+    opt.latencyMs = latencyMs;
+    opt.stats = stats;
+    opt.timed = stats != nullptr && !(params[0].flags & pg::kNoStageTiming);
+    runPaths(dev, scenes, n, items, total, opt);
 }
 
 }  // namespace
@@ -740,6 +742,7 @@ int pg_device_create(int ordinal, pg_device** out) {
         dev->numCUs = prop.multiProcessorCount;
         dev->platform = std::string("AMD HIP ") + prop.gcnArchName;
         PG_HIP(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+        PG_HIP(hipHostMalloc(&dev->status, kStatusBytes, hipHostMallocDefault));
         std::vector<uint32_t> m;
         std::vector<uint64_t> inv, delta;
         pg::buildSobolTables(m, inv, delta);

@@ -20,12 +20,22 @@ struct FilmPoint {
     float x, y;
 };
 
-__device__ __forceinline__ void pathPixel(const RenderConsts& rc, uint32_t g, uint32_t& px, uint32_t& py,
-                                          uint32_t& si) {
-    si = rc.sppBegin + g % rc.sppCount;
-    const uint32_t pix = g / rc.sppCount;
+// Path g of a segment: pixel-major over the rect, the segment's samples of one pixel adjacent.
+__device__ __forceinline__ void pathPixel(const RenderConsts& rc, uint32_t sppBegin, uint32_t sppCount, uint32_t g,
+                                          uint32_t& px, uint32_t& py, uint32_t& si) {
+    si = sppBegin + g % sppCount;
+    const uint32_t pix = g / sppCount;
     px = rc.rectX + pix % rc.rectW;
     py = rc.rectY + pix / rc.rectW;
+}
+__device__ __forceinline__ void pathPixel(const RenderConsts& rc, uint32_t g, uint32_t& px, uint32_t& py,
+                                          uint32_t& si) {
+    pathPixel(rc, rc.sppBegin, rc.sppCount, g, px, py, si);
+}
+
+// Paths of one segment of a single-scene render.
+__device__ __forceinline__ uint64_t segmentPaths(const RenderConsts& rc) {
+    return static_cast<uint64_t>(rc.rectW) * rc.rectH * rc.segSpp;
 }
 
 __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts& rc, uint32_t px, uint32_t py,
@@ -46,53 +56,98 @@ __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts&
     return cameraRay(s.cam, ndcX, ndcY, u[3], u[4]);
 }
 
-// Work item of a path in a batched render (meta.z bits 16..31; 0 otherwise).
+// Work item of a path (meta.z bits 16..31).
 __device__ __forceinline__ uint32_t sceneOf(uint32_t metaZ) { return metaZ >> 16; }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(256) kRaygen(DevScene s0, PathState ps, Queues q, RenderConsts rc0, BatchTable bt,
-                                               uint64_t first, uint32_t n) {
+// Path pool. The slots of finished paths form a stack (done[0, counts[kCntDone])); the shade
+// stage pushes finished paths on top, the record stage writes their film records, and the next
+// iteration restarts the slots with the next path numbers. The extension queue therefore stays
+// full until the last paths of the render drain, and no wave has to finish before the next starts.
+__global__ void __launch_bounds__(256) kPoolInit(Queues q, uint32_t capacity) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < capacity) q.done[i] = i;
     if (i == 0) {
-        q.counts[kCntExtCur] = n;
-        q.counts[kCntExtNext] = 0;
-        q.counts[kCntShadow] = q.counts[kCntMis] = q.counts[kCntDone] = 0;
+        q.counts[kCntExtCur] = q.counts[kCntExtNext] = q.counts[kCntShadow] = q.counts[kCntMis] = 0;
+        q.counts[kCntDone] = q.counts[kCntRecorded] = capacity;
+        q.counts[kCntRegenBase] = q.counts[kCntRegenCount] = 0;
+        q.pool->next = 0;
+        q.pool->regenFirst = 0;
     }
-    if (i >= n) return;
-    const uint64_t G = first + i;
-    uint32_t sid = 0;
-    if (BATCH) {  // last scene whose first path number is <= G
-        uint32_t lo = 0, hi = bt.n;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (bt.pathOffsets[mid] <= G) lo = mid;
-            else hi = mid;
-        }
-        sid = lo;
-    }
-    const DevScene& s = BATCH ? bt.scenes[sid] : s0;
-    const RenderConsts& rc = BATCH ? bt.rc[sid] : rc0;
-    const uint32_t g = static_cast<uint32_t>(BATCH ? G - bt.pathOffsets[sid] : G);
-    uint32_t px, py, si;
-    pathPixel(rc, g, px, py, si);
-    uint64_t idx;
-    FilmPoint fp;
-    const Ray r = primaryRay(s, rc, px, py, si, idx, fp);
-    ps.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-    ps.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-    ps.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    ps.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ps.Lmis[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ps.idx[i] = idx;
-    ps.meta[i] = make_uint4(5u, 0u, 1u | (sid << 16), g);  // sample dims start at 5 (Embree/Kernel.cpp:68); specular
-    ps.film[i] = make_float2(fp.x, fp.y);
-    q.ext[0][i] = i;
 }
 
-// Counter reset at the start of a bounce (a kernel, not a memset, so the sequence can be captured).
-__global__ void kBounceBegin(Queues q, uint32_t nextSlot) {
-    q.counts[nextSlot] = 0;
-    q.counts[kCntShadow] = q.counts[kCntMis] = q.counts[kCntDone] = 0;
+// One thread at the start of an iteration: new path numbers for the free slots (up to `limit`),
+// and the counter resets of the bounce (a kernel, not a memset, so the sequence can be captured).
+__global__ void kIterBegin(Queues q, uint32_t curSlot, unsigned long long limit) {
+    const uint32_t top = q.counts[kCntDone];
+    const unsigned long long next = q.pool->next;
+    const unsigned long long avail = limit > next ? limit - next : 0ull;
+    const uint32_t m = avail < top ? static_cast<uint32_t>(avail) : top;
+    q.counts[kCntRegenBase] = top - m;
+    q.counts[kCntRegenCount] = m;
+    q.counts[kCntDone] = top - m;
+    q.counts[kCntRecorded] = top - m;
+    q.pool->regenFirst = next;
+    q.pool->next = next + m;
+    q.counts[curSlot ^ 1u] = 0;
+    q.counts[kCntShadow] = q.counts[kCntMis] = 0;
+}
+
+// raygen for the new paths (piperMain up to rayGen, Tracer/Embree/Kernel.cpp:47-90); their slots
+// join the current extension queue behind the paths that continue from the last bounce.
+template <bool BATCH>
+__global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot,
+                                              BatchTable bt) {
+    const uint32_t m = q.counts[kCntRegenCount];
+    const uint32_t base = q.counts[kCntRegenBase];
+    const uint64_t first = q.pool->regenFirst;
+    uint32_t* ext = q.ext[curSlot];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < m; i0 += stride) {  // block-uniform loop (wavePush)
+        const uint32_t i = i0 + threadIdx.x;
+        const bool act = i < m;
+        uint32_t slot = 0;
+        if (act) {
+            slot = q.done[base + i];
+            const uint64_t G = first + i;
+            uint32_t item, g, sppBegin, sppCount;
+            if (BATCH) {  // last item whose first path number is <= G
+                uint32_t lo = 0, hi = bt.n;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (bt.pathOffsets[mid] <= G) lo = mid;
+                    else hi = mid;
+                }
+                item = lo;
+                g = static_cast<uint32_t>(G - bt.pathOffsets[item]);
+                sppBegin = bt.rc[item].sppBegin;
+                sppCount = bt.rc[item].sppCount;
+            } else {
+                const uint64_t segPaths = segmentPaths(rc0);
+                item = static_cast<uint32_t>(G / segPaths);
+                g = static_cast<uint32_t>(G - static_cast<uint64_t>(item) * segPaths);
+                sppBegin = rc0.sppBegin + item * rc0.segSpp;
+                sppCount = min(rc0.segSpp, rc0.sppBegin + rc0.sppCount - sppBegin);
+            }
+            const DevScene& s = BATCH ? bt.scenes[item] : s0;
+            const RenderConsts& rc = BATCH ? bt.rc[item] : rc0;
+            uint32_t px, py, si;
+            pathPixel(rc, sppBegin, sppCount, g, px, py, si);
+            uint64_t idx;
+            FilmPoint fp;
+            const Ray r = primaryRay(s, rc, px, py, si, idx, fp);
+            ps.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+            ps.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+            ps.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            ps.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.Lmis[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.idx[slot] = idx;
+            // sample dims start at 5 (Embree/Kernel.cpp:68); a camera ray counts as specular
+            ps.meta[slot] = make_uint4(5u, 0u, 1u | (item << 16), g);
+            ps.film[slot] = make_float2(fp.x, fp.y);
+        }
+        const uint32_t e = wavePush(&q.counts[curSlot], act);
+        if (act) ext[e] = slot;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -135,8 +190,10 @@ __device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
     return gs.train && (g % gs.trainStride) == gs.trainPhase;
 }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot,
+// Waves per SIMD: the plain kernel fits 5 (<= 96 VGPRs) without spilling; the guided and batched
+// ones keep what their extra state needs.
+template <bool BATCH, bool GUIDED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : (BATCH ? 4 : 5)))) kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot,
                                               uint32_t nextSlot, uint64_t* totals, GuideShade gs, BatchTable bt) {
     const uint32_t n = q.counts[curSlot];
     const uint32_t* qcur = q.ext[curSlot];
@@ -146,8 +203,6 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
         const uint32_t i = i0 + threadIdx.x;
         bool pushExt = false, pushShadow = false, pushMis = false, pushMisQuery = false, pushDone = false;
         uint32_t path = 0;
-        ShadowRec sr, mq;
-        MisRec mr;
         if (i < n) {
             path = qcur[i];
             const float4 hit4 = ps.hit[path];
@@ -163,7 +218,7 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
             V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
             ps.Lmis[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (meta.z & kMetaPending) {  // the previous guided vertex's direct light is now in L
+            if (GUIDED && (meta.z & kMetaPending)) {  // the previous guided vertex's direct light is now in L
                 const uint32_t r = ((meta.z >> 8) & 0xFFu) - 1u;
                 gs.hist[(static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4 + 5] = make_float4(L.x, L.y, L.z, 0.0f);
                 meta.z &= ~kMetaPending;
@@ -206,8 +261,11 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
                         float w = fabsf(wi.z) / ls.pdf;
                         w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
                         const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
-                        sr = ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y, ls.dir.z, ls.dist,
-                                       c.x, c.y, c.z, path};
+                        // queued where it is made (the active lanes share one atomic), so the
+                        // record's registers are free for the rest of the vertex
+                        q.shadow[wavePush(&q.counts[kCntShadow], true)] =
+                            ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y, ls.dir.z, ls.dist,
+                                      c.x, c.y, c.z, path};
                         pushShadow = true;
                     }
                 }
@@ -217,7 +275,8 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
                     const V3 dir = fr.toWorld(bs.wi);
                     const V3 c = mul(thr, bs.f * weight * selInv);
                     if (!envBlack) {
-                        mr = MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
+                        q.mis[wavePush(&q.counts[kCntMis], true)] =
+                            MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
                         pushMis = true;
                     } else if (sel != 0u) {
                         // With a black environment the BSDF ray contributes only if the selected
@@ -245,8 +304,9 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
                                     const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
                                     // tq > 0: the next float up makes the range (tmin, tq] inclusive
                                     const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                                    mq = ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
-                                                   cm.x, cm.y, cm.z, path | kMisFlag};
+                                    q.shadow[wavePush(&q.counts[kCntShadow], true)] =
+                                        ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
+                                                  cm.x, cm.y, cm.z, path | kMisFlag};
                                     pushMisQuery = true;
                                 }
                             }
@@ -260,7 +320,7 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
                 const float uc1 = d9 ? u[6] : u[5], uc2 = d9 ? u[5] : u[6];
                 BsdfSample ss;
                 float pdfMix = 0.0f;
-                if (!gs.enabled) {
+                if (!GUIDED) {
                     ss = bsdfSample(mat, wo, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
@@ -304,7 +364,7 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
                         meta.z &= ~1u;  // no longer a camera / specular ray
                         pushExt = true;
                         const uint32_t r = (meta.z >> 8) & 0xFFu;
-                        if (gs.enabled && isTrainPath(gs, meta.w) && r < gs.histDepth) {
+                        if (GUIDED && isTrainPath(gs, meta.w) && r < gs.histDepth) {
                             float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
                             const float px = fminf(fmaxf((hp.x - gs.lo.x) * gs.invExt.x, 0.0f), 1.0f);
                             const float py = fminf(fmaxf((hp.y - gs.lo.y) * gs.invExt.y, 0.0f), 1.0f);
@@ -327,12 +387,6 @@ __global__ void __launch_bounds__(256) kShade(DevScene s0, PathState ps, Queues 
         }
         const uint32_t se = wavePush(&q.counts[nextSlot], pushExt);
         if (pushExt) qnext[se] = path;
-        const uint32_t ss = wavePush(&q.counts[kCntShadow], pushShadow);
-        if (pushShadow) q.shadow[ss] = sr;
-        const uint32_t sq = wavePush(&q.counts[kCntShadow], pushMisQuery);
-        if (pushMisQuery) q.shadow[sq] = mq;
-        const uint32_t sm = wavePush(&q.counts[kCntMis], pushMis);
-        if (pushMis) q.mis[sm] = mr;
         const uint32_t sd = wavePush(&q.counts[kCntDone], pushDone);
         if (pushDone) q.done[sd] = path;
         if (totals) {
@@ -442,40 +496,60 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
 // ------------------------------------------------------------------------------------------
 // Film: accumulate() with the box filter (RenderDriver/FixedSampler/Kernel.cpp:30-58) adds (L, 1)
 // to the 2x2 pixels starting at (int)(p - 0.5) for every sample. Instead of atomics, each finished
-// path writes a record (L, film point) at its path number; after the render, kQuadSums reduces
-// each source pixel's samples into the four 2x2 footprints they can start at, and kResolve has
-// every film pixel gather the footprints that cover it from its 3x3 source neighbourhood. Every
-// sum runs in a fixed order, so the film is bit-reproducible.
+// path writes a record (L, film point) at its path number (in a ring of record slots); once all
+// paths of a film segment are recorded, kQuadSums reduces each source pixel's samples into the
+// four 2x2 footprints they can start at, and kResolve has every film pixel gather the footprints
+// that cover it from its 3x3 source neighbourhood. Every sum runs in a fixed order, so the film is
+// bit-reproducible whatever order the paths finish in.
 template <bool BATCH>
-__global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* recL, float* recY, GuideShade gs,
-                                               GuideRecordOut tr, BatchTable bt) {
-    const uint32_t n = q.counts[kCntDone];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t path = q.done[i];
-        const V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
-        const float2 fp = ps.film[path];
-        const uint4 meta = ps.meta[path];
-        const uint32_t g = meta.w;
-        const size_t slot = BATCH ? bt.recBase[sceneOf(meta.z)] + g : g;
-        recL[slot] = make_float4(L.x, L.y, L.z, fp.x);
-        recY[slot] = fp.y;
-        if (BATCH || !isTrainPath(gs, g)) continue;
-        // Each recorded vertex learns from the radiance that reached it along its sampled
-        // direction: (L_final - L after its direct light) / throughput after the bounce.
-        const uint32_t cnt = (meta.z >> 8) & 0xFFu;
-        for (uint32_t r = 0; r < cnt; ++r) {
-            const float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
-            const float4 h0 = h[0], t = h[4], snap = h[5];
-            const float lr = t.x > 0.0f ? (L.x - snap.x) / t.x : 0.0f;
-            const float lg = t.y > 0.0f ? (L.y - snap.y) / t.y : 0.0f;
-            const float lb = t.z > 0.0f ? (L.z - snap.z) / t.z : 0.0f;
-            const float c = fmaxf((lr + lg + lb) * (1.0f / 3.0f), 0.0f) / h0.w;
-            const uint32_t slot = atomicAdd(tr.count, 1u);
-            if (slot >= tr.capacity) break;
-            tr.rec[4 * slot] = make_float4(h0.x, h0.y, h0.z, c);
-            tr.rec[4 * slot + 1] = h[1];
-            tr.rec[4 * slot + 2] = h[2];
-            tr.rec[4 * slot + 3] = h[3];
+__global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderConsts rc0, RecordRing ring,
+                                               GuideShade gs, GuideRecordOut tr, BatchTable bt) {
+    const uint32_t lo = q.counts[kCntRecorded];
+    const uint32_t n = q.counts[kCntDone] - lo;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop (ballots)
+        const uint32_t i = i0 + threadIdx.x;
+        const bool act = i < n;
+        uint32_t item = 0;
+        if (act) {
+            const uint32_t path = q.done[lo + i];
+            const V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
+            const float2 fp = ps.film[path];
+            const uint4 meta = ps.meta[path];
+            const uint32_t g = meta.w;
+            item = sceneOf(meta.z);
+            const uint64_t G = (BATCH ? bt.pathOffsets[item] : static_cast<uint64_t>(item) * segmentPaths(rc0)) + g;
+            const uint64_t slot = G % ring.size;
+            ring.L[slot] = make_float4(L.x, L.y, L.z, fp.x);
+            ring.y[slot] = fp.y;
+            if (!BATCH && isTrainPath(gs, g)) {
+                // Each recorded vertex learns from the radiance that reached it along its sampled
+                // direction: (L_final - L after its direct light) / throughput after the bounce.
+                const uint32_t cnt = (meta.z >> 8) & 0xFFu;
+                for (uint32_t r = 0; r < cnt; ++r) {
+                    const float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
+                    const float4 h0 = h[0], t = h[4], snap = h[5];
+                    const float lr = t.x > 0.0f ? (L.x - snap.x) / t.x : 0.0f;
+                    const float lg = t.y > 0.0f ? (L.y - snap.y) / t.y : 0.0f;
+                    const float lb = t.z > 0.0f ? (L.z - snap.z) / t.z : 0.0f;
+                    const float c = fmaxf((lr + lg + lb) * (1.0f / 3.0f), 0.0f) / h0.w;
+                    const uint32_t k = atomicAdd(tr.count, 1u);
+                    if (k >= tr.capacity) break;
+                    tr.rec[4 * k] = make_float4(h0.x, h0.y, h0.z, c);
+                    tr.rec[4 * k + 1] = h[1];
+                    tr.rec[4 * k + 2] = h[2];
+                    tr.rec[4 * k + 3] = h[3];
+                }
+            }
+        }
+        // paths recorded per item: one atomic per distinct item among the wave's lanes
+        uint64_t pending = __ballot(act);
+        while (pending) {
+            const int leader = __builtin_ctzll(pending);
+            const uint32_t it = __shfl(item, leader);
+            const uint64_t same = __ballot(act && item == it);
+            if (__lane_id() == static_cast<uint32_t>(leader)) atomicAdd(&ring.itemDone[it], static_cast<uint32_t>(__popcll(same)));
+            pending &= ~same;
         }
     }
 }
@@ -484,7 +558,8 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, float4* r
 // (bx, by) in {px-1, px} x {py-1, py}; quad[4 * pixel + 2 * (by - py + 1) + (bx - px + 1)] = (rgb, count).
 template <typename F>
 __global__ void __launch_bounds__(256) kQuadSums(RenderConsts rc, const float4* __restrict__ recL,
-                                                 const float* __restrict__ recY, F* quad) {
+                                                 const float* __restrict__ recY, uint64_t start, uint64_t ringSize,
+                                                 F* quad) {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= rc.rectW * rc.rectH) return;
     const int32_t px = static_cast<int32_t>(rc.rectX + pix % rc.rectW);
@@ -492,10 +567,12 @@ __global__ void __launch_bounds__(256) kQuadSums(RenderConsts rc, const float4* 
     F acc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = F(0);
-    const size_t base = static_cast<size_t>(pix) * rc.sppCount;
+    const uint64_t base = start + static_cast<uint64_t>(pix) * rc.sppCount;  // start < ringSize
     for (uint32_t si = 0; si < rc.sppCount; ++si) {
-        const float4 r = recL[base + si];
-        const int32_t bx = static_cast<int32_t>(r.w - 0.5f), by = static_cast<int32_t>(recY[base + si] - 0.5f);
+        uint64_t j = base + si;
+        if (j >= ringSize) j -= ringSize;  // a segment is never longer than the ring
+        const float4 r = recL[j];
+        const int32_t bx = static_cast<int32_t>(r.w - 0.5f), by = static_cast<int32_t>(recY[j] - 0.5f);
         const int q = 2 * (by - py + 1) + (bx - px + 1);  // 0..3 (bx in {px-1, px}, by in {py-1, py})
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -601,14 +678,17 @@ static inline uint32_t blocksFor(uint64_t n, uint32_t bs, uint32_t cap) {
     return static_cast<uint32_t>(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-void launchRaygen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                  uint64_t first, uint32_t n, const BatchTable* bt) {
-    const dim3 grid(blocksFor(n, 256, 1u << 30));
-    if (bt) hipLaunchKernelGGL(kRaygen<true>, grid, dim3(256), 0, st, s, ps, q, rc, *bt, first, n);
-    else hipLaunchKernelGGL(kRaygen<false>, grid, dim3(256), 0, st, s, ps, q, rc, BatchTable{}, first, n);
+void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity) {
+    hipLaunchKernelGGL(kPoolInit, dim3(blocksFor(capacity, 256, 1u << 30)), dim3(256), 0, st, q, capacity);
 }
-void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot) {
-    hipLaunchKernelGGL(kBounceBegin, dim3(1), dim3(1), 0, st, q, nextSlot);
+void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot, uint64_t limit) {
+    hipLaunchKernelGGL(kIterBegin, dim3(1), dim3(1), 0, st, q, curSlot, static_cast<unsigned long long>(limit));
+}
+void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                 uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt) {
+    const dim3 grid(gridThreads / 256);
+    if (bt) hipLaunchKernelGGL(kRegen<true>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, *bt);
+    else hipLaunchKernelGGL(kRegen<false>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, BatchTable{});
 }
 
 template <int W>
@@ -642,10 +722,14 @@ void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Q
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
                  const BatchTable* bt) {
     const dim3 grid(gridThreads / 256);
-    if (bt)
-        hipLaunchKernelGGL(kShade<true>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs, *bt);
+    if (bt)  // batched renders are never guided
+        hipLaunchKernelGGL((kShade<true, false>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
+                           *bt);
+    else if (gs.enabled)
+        hipLaunchKernelGGL((kShade<false, true>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
+                           BatchTable{});
     else
-        hipLaunchKernelGGL(kShade<false>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
+        hipLaunchKernelGGL((kShade<false, false>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
                            BatchTable{});
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
@@ -672,24 +756,25 @@ void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Que
         else hipLaunchKernelGGL((kMis<4, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
     }
 }
-void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt) {
+void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
+                  const RecordRing& ring, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt) {
     const dim3 grid(gridThreads / 256);
-    if (bt) hipLaunchKernelGGL(kRecord<true>, grid, dim3(256), 0, st, ps, q, recL, recY, gs, tr, *bt);
-    else hipLaunchKernelGGL(kRecord<false>, grid, dim3(256), 0, st, ps, q, recL, recY, gs, tr, BatchTable{});
+    if (bt) hipLaunchKernelGGL(kRecord<true>, grid, dim3(256), 0, st, ps, q, rc, ring, gs, tr, *bt);
+    else hipLaunchKernelGGL(kRecord<false>, grid, dim3(256), 0, st, ps, q, rc, ring, gs, tr, BatchTable{});
 }
-void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
+void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& ring, uint64_t first, void* quad,
                    float* film32, double* film64) {
     const uint32_t pixels = rc.rectW * rc.rectH;
     const uint32_t targets = (rc.rectW + 2) * (rc.rectH + 2);
+    const uint64_t start = first % ring.size;
     if (film64) {
-        hipLaunchKernelGGL(kQuadSums<double>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, recL, recY,
-                           static_cast<double*>(quad));
+        hipLaunchKernelGGL(kQuadSums<double>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, ring.L,
+                           ring.y, start, ring.size, static_cast<double*>(quad));
         hipLaunchKernelGGL(kResolve<double>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
                            static_cast<const double*>(quad), film64);
     } else {
-        hipLaunchKernelGGL(kQuadSums<float>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, recL, recY,
-                           static_cast<float*>(quad));
+        hipLaunchKernelGGL(kQuadSums<float>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, ring.L,
+                           ring.y, start, ring.size, static_cast<float*>(quad));
         hipLaunchKernelGGL(kResolve<float>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
                            static_cast<const float*>(quad), film32);
     }

@@ -44,27 +44,42 @@ struct Queues {
     uint32_t* ext[2];  // extension queues (current / next bounce)
     ShadowRec* shadow;
     MisRec* mis;
-    uint32_t* done;
+    uint32_t* done;    // free path slots, with finished paths pushed on top (kCntDone)
     uint32_t* counts;  // kCntNum u32
+    PoolPaths* pool;
 };
 
-// Device tables of a batched render. Its work items (a scene and a sample sub-range each) have
-// consecutive path numbers, item k owning [pathOffsets[k], pathOffsets[k + 1]); a path carries
-// its item id in meta.z bits 16..31, and item k's film records start at record slot recBase[k].
+// Work items: film segments (whole rect x a sample sub-range) with consecutive path numbers.
+// A single-scene render splits its sample range into segments of rc.segSpp samples, so item k
+// owns path numbers [k * P, (k + 1) * P) with P = rectW * rectH * segSpp. A batched render lists
+// its items in device tables, item k owning [pathOffsets[k], pathOffsets[k + 1]). Either way a
+// path carries its item id in meta.z bits 16..31 and its number within the item in meta.w.
 constexpr uint32_t kMaxBatchItems = 1u << 16;
 struct BatchTable {
     const DevScene* scenes;       // per item: its scene
     const RenderConsts* rc;       // per item: rect, sample sub-range, depth, flags
     const uint64_t* pathOffsets;  // n + 1 entries
-    const uint64_t* recBase;
     uint32_t n;
+};
+
+// Film records live in a ring of `ring` slots: path number G writes slot G mod ring.
+struct RecordRing {
+    float4* L;       // (L, film x)
+    float* y;        // film y
+    uint64_t size;
+    uint32_t* itemDone;  // paths recorded per item
 };
 
 // `bt` selects the batched kernels (nullptr: one scene, `s` and `rc` in kernel arguments).
 // Batched launches still take the first scene as `s`: all scenes share its BVH width.
-void launchRaygen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                  uint64_t first, uint32_t n, const BatchTable* bt = nullptr);
-void launchBounceBegin(hipStream_t st, const Queues& q, uint32_t nextSlot);
+//
+// Path pool: `capacity` slots. launchPoolInit marks every slot free. Each iteration,
+// launchIterBegin (one thread) takes up to `limit` - next new path numbers for the free slots,
+// then launchRegen starts those paths and appends their slots to extension queue curSlot.
+void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity);
+void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot, uint64_t limit);
+void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                 uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt = nullptr);
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
                   const BatchTable* bt = nullptr);
@@ -75,10 +90,13 @@ void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const 
                   uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
-void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, uint32_t gridThreads, float4* recL,
-                  float* recY, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt = nullptr);
-// quad: 16 floats (or doubles with film64) per rect pixel
-void launchResolve(hipStream_t st, const RenderConsts& rc, const float4* recL, const float* recY, void* quad,
+// Film records of the paths that finished this iteration (done entries above kCntRecorded).
+void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
+                  const RecordRing& ring, const GuideShade& gs, const GuideRecordOut& tr,
+                  const BatchTable* bt = nullptr);
+// Resolve one item (rc: its rect and sample sub-range; its first path number `first`) into the
+// film. quad: 16 floats (or doubles with film64) per rect pixel.
+void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& ring, uint64_t first, void* quad,
                    float* film32, double* film64);
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane);

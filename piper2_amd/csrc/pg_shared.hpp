@@ -89,15 +89,23 @@ struct alignas(16) MisRec {  // 48 B
     uint32_t path;
 };
 
-// Counters in device memory (one block of u32, then u64 totals).
+// Counters in device memory: a block of u32, then the pool's u64 path numbers.
 enum CounterSlot : uint32_t {
-    kCntExtCur = 0,
+    kCntExtCur = 0,     // extension queues (two, alternating by bounce)
     kCntExtNext = 1,
     kCntShadow = 2,
     kCntMis = 3,
-    kCntDone = 4,
+    kCntDone = 4,       // free path slots: done[0, count); finished paths are pushed on top
+    kCntRecorded = 5,   // done entries below this have had their film records written
+    kCntRegenBase = 6,  // this iteration's new paths take the slots done[base, base + count)
+    kCntRegenCount = 7,
     kCntNum = 8
 };
+struct PoolPaths {
+    unsigned long long next;        // number of the next path to start
+    unsigned long long regenFirst;  // number of this iteration's first new path
+};
+constexpr uint32_t kCountsBytes = kCntNum * 4 + sizeof(PoolPaths);
 enum TotalSlot : uint32_t { kTotExt = 0, kTotShadow = 1, kTotMis = 2, kTotNum = 4 };
 
 struct RenderConsts {
@@ -106,6 +114,7 @@ struct RenderConsts {
     uint32_t sppBegin, sppCount;
     uint32_t maxDepth, minRR, flags;
     uint32_t log2Res;
+    uint32_t segSpp;  // samples per pixel of one film segment (path numbers run segment by segment)
     float ndc[4];
 };
 
