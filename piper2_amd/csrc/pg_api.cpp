@@ -109,7 +109,7 @@ struct pg_device {
     DevBuf recL, recY, quad;  // film record ring and per-pixel footprint sums
     DevBuf itemDone;          // paths recorded per work item
     DevBuf mix, hist, trainRec, trainCount;  // guiding: mixtures, vertex history, training records
-    DevBuf o, d, thr, L, Lmis, hit, idx, meta, film, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
+    DevBuf paths, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
     DevBuf batchScenes, batchRc, batchOffsets;  // work-item tables of a batched render
     void* status = nullptr;   // pinned host copy of the pool counters and item progress
     size_t spillBytes = 0;
@@ -300,8 +300,7 @@ pg::RenderConsts makeConsts(const pg_render_params& p) {
 void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
     if (cap > dev->capacity) {
         const size_t P = cap;
-        dev->o.reserve(P * 16), dev->d.reserve(P * 16), dev->thr.reserve(P * 16), dev->L.reserve(P * 16), dev->Lmis.reserve(P * 16);
-        dev->hit.reserve(P * 16), dev->idx.reserve(P * 8), dev->meta.reserve(P * 16), dev->film.reserve(P * 8);
+        dev->paths.reserve(P * sizeof(pg::PathRec));
         dev->qext0.reserve(P * 4), dev->qext1.reserve(P * 4), dev->qdone.reserve(P * 4);
         dev->qshadow.reserve(2 * P * sizeof(pg::ShadowRec)), dev->qmis.reserve(P * sizeof(pg::MisRec));
         dev->capacity = cap;
@@ -313,9 +312,7 @@ void ensureWorkspace(pg_device* dev, uint32_t cap, size_t spillBytes) {
 }
 
 pg::PathState pathState(pg_device* dev) {
-    return pg::PathState{dev->o.as<float4>(),    dev->d.as<float4>(),   dev->thr.as<float4>(),
-                         dev->L.as<float4>(),    dev->Lmis.as<float4>(), dev->hit.as<float4>(),
-                         dev->idx.as<uint64_t>(), dev->meta.as<uint4>(), dev->film.as<float2>()};
+    return pg::PathState{dev->paths.as<pg::PathRec>()};
 }
 pg::Queues queues(pg_device* dev) {
     return pg::Queues{{dev->qext0.as<uint32_t>(), dev->qext1.as<uint32_t>()},

@@ -20,17 +20,15 @@ struct ArraySource {
 // From the path state of an extension-queue entry (hit point, shading normal, -ray direction).
 struct PathSource {
     DevScene s;
-    const float4* o;
-    const float4* d;
-    const float4* hit;
+    const PathRec* paths;
     const uint32_t* queue;
     float3 lo, invExt;
     __device__ __forceinline__ GuideInput get(uint32_t i) const {
         const uint32_t path = queue[i];
-        const float4 h = hit[path];
+        const float4 h = paths[path].hit;
         const uint32_t pos = __float_as_uint(h.y);
         if (pos == kHitMiss || (pos & kHitQuad)) return guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
-        const float4 oo = o[path], dd = d[path];
+        const float4 oo = paths[path].o, dd = paths[path].d;
         const V3 dir{dd.x, dd.y, dd.z};
         const V3 p = V3{oo.x, oo.y, oo.z} + dir * h.x;
         const V3 n = triNormal(s, pos, dir);
@@ -359,7 +357,7 @@ void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const ui
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks) {
-    PathSource src{s, ps.o, ps.d, ps.hit, queue, lo, invExt};
+    PathSource src{s, ps.p, queue, lo, invExt};
     hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, 64, maxBlocks)), dim3(256), kInferLds, st, src,
                        capacity, count, wb, theta, mix);
 }

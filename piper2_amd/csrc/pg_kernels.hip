@@ -135,15 +135,16 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
             uint64_t idx;
             FilmPoint fp;
             const Ray r = primaryRay(s, rc, px, py, si, idx, fp);
-            ps.o[slot] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-            ps.d[slot] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-            ps.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            ps.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.Lmis[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.idx[slot] = idx;
+            ps.p[slot].o = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+            ps.p[slot].d = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+            ps.p[slot].thr = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            ps.p[slot].L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.p[slot].Lmis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.p[slot].hit = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // the whole 128 B line is written
+            ps.p[slot].idx = idx;
             // sample dims start at 5 (Embree/Kernel.cpp:68); a camera ray counts as specular
-            ps.meta[slot] = make_uint4(5u, 0u, 1u | (item << 16), g);
-            ps.film[slot] = make_float2(fp.x, fp.y);
+            ps.p[slot].meta = make_uint4(5u, 0u, 1u | (item << 16), g);
+            ps.p[slot].film = make_float2(fp.x, fp.y);
         }
         const uint32_t e = wavePush(&q.counts[curSlot], act);
         if (act) ext[e] = slot;
@@ -165,11 +166,11 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const u
     TravCount cnt;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const uint32_t path = queue[i];
-        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[path].z)] : s0;
-        const float4 o = ps.o[path], d = ps.d[path];
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[path].meta.z)] : s0;
+        const float4 o = ps.p[path].o, d = ps.p[path].d;
         const Ray r{xyz(o), xyz(d)};
         const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt);
-        ps.hit[path] = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
+        ps.p[path].hit = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
     }
     if (COUNT) {
         atomicAdd(&travCounts[0], static_cast<unsigned long long>(cnt.nodes));
@@ -205,19 +206,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
         uint32_t path = 0;
         if (i < n) {
             path = qcur[i];
-            const float4 hit4 = ps.hit[path];
+            const float4 hit4 = ps.p[path].hit;
             const uint32_t pos = __float_as_uint(hit4.y);
-            uint4 meta = ps.meta[path];
+            uint4 meta = ps.p[path].meta;
             const DevScene& s = BATCH ? bt.scenes[sceneOf(meta.z)] : s0;
             const RenderConsts& rc = BATCH ? bt.rc[sceneOf(meta.z)] : rc0;
             const bool d1 = rc.flags & kCompatD1, d3 = rc.flags & kCompatD3, d9 = rc.flags & kCompatD9,
                        d13 = rc.flags & kCompatD13;
             const bool envBlack = !valid(ld3(s.env));
-            Ray ray{xyz(ps.o[path]), xyz(ps.d[path])};
-            V3 thr = xyz(ps.thr[path]);
+            Ray ray{xyz(ps.p[path].o), xyz(ps.p[path].d)};
+            V3 thr = xyz(ps.p[path].thr);
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
-            V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
-            ps.Lmis[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            V3 L = xyz(ps.p[path].L) + xyz(ps.p[path].Lmis);
+            ps.p[path].Lmis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (GUIDED && (meta.z & kMetaPending)) {  // the previous guided vertex's direct light is now in L
                 const uint32_t r = ((meta.z >> 8) & 0xFFu) - 1u;
                 gs.hist[(static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4 + 5] = make_float4(L.x, L.y, L.z, 0.0f);
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
             if (isLight) {  // :142-146 pass through the light
                 if (d13) meta.y -= 1u;
                 ray.o = ray.o + ray.d * (hit4.x + 0.001f);
-                ps.o[path] = make_float4(ray.o.x, ray.o.y, ray.o.z, 0.0f);
+                ps.p[path].o = make_float4(ray.o.x, ray.o.y, ray.o.z, 0.0f);
                 pushExt = true;
             } else if (pos == kHitMiss || meta.y >= rc.maxDepth) {  // :148
                 pushDone = true;
@@ -246,7 +247,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                 // The vertex draws at most 8 consecutive dimensions: light select, light (2),
                 // MIS BSDF (2), continuation (2), Russian roulette (1), in that order.
                 float u[8];
-                sobolWindow(s, ps.idx[path], meta.x, u);
+                sobolWindow(s, ps.p[path].idx, meta.x, u);
                 uint32_t used = 7;
 
                 // light selection: uniformSelect over {environment, quads}
@@ -357,9 +358,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                     if (alive) {
                         const V3 nd = fr.toWorld(ss.wi);
                         const V3 no = hp + nrm * 0.001f;
-                        ps.o[path] = make_float4(no.x, no.y, no.z, 0.0f);
-                        ps.d[path] = make_float4(nd.x, nd.y, nd.z, 0.0f);
-                        ps.thr[path] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                        ps.p[path].o = make_float4(no.x, no.y, no.z, 0.0f);
+                        ps.p[path].d = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                        ps.p[path].thr = make_float4(thr.x, thr.y, thr.z, 0.0f);
                         meta.y += 1u;
                         meta.z &= ~1u;  // no longer a camera / specular ray
                         pushExt = true;
@@ -382,8 +383,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                 }
                 meta.x += used;
             }
-            ps.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-            ps.meta[path] = meta;
+            ps.p[path].L = make_float4(L.x, L.y, L.z, 0.0f);
+            ps.p[path].meta = meta;
         }
         const uint32_t se = wavePush(&q.counts[nextSlot], pushExt);
         if (pushExt) qnext[se] = path;
@@ -413,15 +414,15 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const ShadowRec r = q.shadow[i];
-        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[r.path & ~kMisFlag].z)] : s0;
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path & ~kMisFlag].meta.z)] : s0;
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs)) {
             if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
-                ps.Lmis[r.path & ~kMisFlag] = make_float4(r.cr, r.cg, r.cb, 0.0f);
+                ps.p[r.path & ~kMisFlag].Lmis = make_float4(r.cr, r.cg, r.cb, 0.0f);
             } else {
-                float4 L = ps.L[r.path];
+                float4 L = ps.p[r.path].L;
                 L.x += r.cr, L.y += r.cg, L.z += r.cb;
-                ps.L[r.path] = L;
+                ps.p[r.path].L = L;
             }
         }
     }
@@ -441,7 +442,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const MisRec r = q.mis[i];
-        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.meta[r.path].z)] : s0;
+        const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path].meta.z)] : s0;
         const bool envBlack = !valid(ld3(s.env));
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         V3 rad{0.0f, 0.0f, 0.0f};
@@ -484,9 +485,9 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
         if (valid(rad) && lpdf > 0.0f) {
             const float w = powerHeuristic(r.bsdfPdf, lpdf);
             const V3 c = mul(V3{r.cr, r.cg, r.cb}, rad) * w;
-            float4 L = ps.L[r.path];
+            float4 L = ps.p[r.path].L;
             L.x += c.x, L.y += c.y, L.z += c.z;
-            ps.L[r.path] = L;
+            ps.p[r.path].L = L;
         }
     }
 }
@@ -513,9 +514,9 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
         uint32_t item = 0;
         if (act) {
             const uint32_t path = q.done[lo + i];
-            const V3 L = xyz(ps.L[path]) + xyz(ps.Lmis[path]);
-            const float2 fp = ps.film[path];
-            const uint4 meta = ps.meta[path];
+            const V3 L = xyz(ps.p[path].L) + xyz(ps.p[path].Lmis);
+            const float2 fp = ps.p[path].film;
+            const uint4 meta = ps.p[path].meta;
             const uint32_t g = meta.w;
             item = sceneOf(meta.z);
             const uint64_t G = (BATCH ? bt.pathOffsets[item] : static_cast<uint64_t>(item) * segmentPaths(rc0)) + g;

@@ -6,17 +6,24 @@
 
 namespace pg {
 
-// Path state, structure of arrays over the wave's path slots.
+// State of one path slot: exactly one 128 B cache line. The pool hands slots out in whatever
+// order paths finish, so the stages touch slots at random; one line per slot keeps each such
+// access to a single line instead of one line per field.
+struct alignas(128) PathRec {
+    float4 o;       // ray origin
+    float4 d;       // ray direction
+    float4 thr;     // throughput
+    float4 L;       // radiance so far
+    float4 Lmis;    // MIS-BSDF light hit of the current bounce, added to L after its NEE term
+    float4 hit;     // t, hit position bits, u, v
+    uint4 meta;     // next Sobol dim, depth, flags, path number within its work item
+    uint64_t idx;   // Sobol index
+    float2 film;    // film point
+};
+static_assert(sizeof(PathRec) == 128, "one cache line per path slot");
+
 struct PathState {
-    float4* o;     // ray origin
-    float4* d;     // ray direction
-    float4* thr;   // throughput
-    float4* L;     // radiance so far
-    float4* Lmis;  // MIS-BSDF light hit of the current bounce, added to L after its NEE term
-    float4* hit;   // t, hit position bits, u, v
-    uint64_t* idx; // Sobol index
-    uint4* meta;   // next Sobol dim, depth, flags (bit 0: specular), path number within the call
-    float2* film;  // film point
+    PathRec* p;  // capacity slots
 };
 
 // meta.z bits: 0 camera/specular ray, 1 a guided vertex awaits its radiance snapshot,
