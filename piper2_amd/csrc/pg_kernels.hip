@@ -84,6 +84,8 @@ __global__ void kIterBegin(Queues q, uint32_t curSlot, unsigned long long limit)
     const uint32_t m = avail < top ? static_cast<uint32_t>(avail) : top;
     q.counts[kCntRegenBase] = top - m;
     q.counts[kCntRegenCount] = m;
+    q.counts[kCntRegenExt] = q.counts[curSlot];  // behind the paths that continue
+    q.counts[curSlot] += m;
     q.counts[kCntDone] = top - m;
     q.counts[kCntRecorded] = top - m;
     q.pool->regenFirst = next;
@@ -100,14 +102,12 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
     const uint32_t m = q.counts[kCntRegenCount];
     const uint32_t base = q.counts[kCntRegenBase];
     const uint64_t first = q.pool->regenFirst;
-    uint32_t* ext = q.ext[curSlot];
+    uint32_t* ext = q.ext[curSlot] + q.counts[kCntRegenExt];
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < m; i0 += stride) {  // block-uniform loop (wavePush)
-        const uint32_t i = i0 + threadIdx.x;
-        const bool act = i < m;
-        uint32_t slot = 0;
-        if (act) {
-            slot = q.done[base + i];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        {
+            const uint32_t slot = q.done[base + i];
+            ext[i] = slot;
             const uint64_t G = first + i;
             uint32_t item, g, sppBegin, sppCount;
             if (BATCH) {  // last item whose first path number is <= G
@@ -146,8 +146,6 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
             ps.p[slot].meta = make_uint4(5u, 0u, 1u | (item << 16), g);
             ps.p[slot].film = make_float2(fp.x, fp.y);
         }
-        const uint32_t e = wavePush(&q.counts[curSlot], act);
-        if (act) ext[e] = slot;
     }
 }
 
@@ -191,17 +189,34 @@ __device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
     return gs.train && (g % gs.trainStride) == gs.trainPhase;
 }
 
+// Queue records of one shade block, staged in LDS until the block knows where its records go.
+// Device-scope atomics on one counter serialise across the whole chip, so the shade stage takes
+// one per block and queue (not one per wave); the staged records then leave as coalesced stores.
+constexpr uint32_t kShadeBlock = 256;
+struct ShadeLds {
+    ShadowRec nee[kShadeBlock];   // NEE shadow ray
+    ShadowRec rec2[kShadeBlock];  // MIS-BSDF occlusion query (ShadowRec) or MIS ray (MisRec)
+    uint32_t cnt[5][kShadeBlock / 64];  // per wave: ext, done, nee, MIS query, MIS ray
+    uint32_t base[4];                   // block's first entry: ext, done, shadow, mis queue
+};
+static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging slots");
+
 // Waves per SIMD: the plain kernel fits 5 (<= 96 VGPRs) without spilling; the guided and batched
 // ones keep what their extra state needs.
 template <bool BATCH, bool GUIDED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : (BATCH ? 4 : 5)))) kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot,
-                                              uint32_t nextSlot, uint64_t* totals, GuideShade gs, BatchTable bt) {
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : (BATCH ? 4 : 5))))
+kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
+       GuideShade gs, BatchTable bt) {
+    __shared__ ShadeLds lds;
     const uint32_t n = q.counts[curSlot];
     const uint32_t* qcur = q.ext[curSlot];
     uint32_t* qnext = q.ext[nextSlot];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop
-        const uint32_t i = i0 + threadIdx.x;
+    const uint32_t tid = threadIdx.x, wave = tid / 64, lane = __lane_id();
+    const uint64_t lowerLanes = (1ull << lane) - 1ull;
+    uint64_t totExt = 0, totNee = 0, totMis = 0;  // thread 0: this block's ray counts
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    for (uint32_t i0 = blockIdx.x * kShadeBlock; i0 < n; i0 += stride) {  // block-uniform loop (barriers)
+        const uint32_t i = i0 + tid;
         bool pushExt = false, pushShadow = false, pushMis = false, pushMisQuery = false, pushDone = false;
         uint32_t path = 0;
         if (i < n) {
@@ -262,11 +277,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                         float w = fabsf(wi.z) / ls.pdf;
                         w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
                         const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
-                        // queued where it is made (the active lanes share one atomic), so the
-                        // record's registers are free for the rest of the vertex
-                        q.shadow[wavePush(&q.counts[kCntShadow], true)] =
-                            ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y, ls.dir.z, ls.dist,
-                                      c.x, c.y, c.z, path};
+                        // staged where it is made, so its registers are free for the rest of the vertex
+                        lds.nee[tid] = ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y,
+                                                 ls.dir.z, ls.dist, c.x, c.y, c.z, path};
                         pushShadow = true;
                     }
                 }
@@ -276,7 +289,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                     const V3 dir = fr.toWorld(bs.wi);
                     const V3 c = mul(thr, bs.f * weight * selInv);
                     if (!envBlack) {
-                        q.mis[wavePush(&q.counts[kCntMis], true)] =
+                        *reinterpret_cast<MisRec*>(&lds.rec2[tid]) =
                             MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
                         pushMis = true;
                     } else if (sel != 0u) {
@@ -305,9 +318,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
                                     const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
                                     // tq > 0: the next float up makes the range (tmin, tq] inclusive
                                     const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                                    q.shadow[wavePush(&q.counts[kCntShadow], true)] =
-                                        ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
-                                                  cm.x, cm.y, cm.z, path | kMisFlag};
+                                    lds.rec2[tid] = ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
+                                                              cm.x, cm.y, cm.z, path | kMisFlag};
                                     pushMisQuery = true;
                                 }
                             }
@@ -386,18 +398,52 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED
             ps.p[path].L = make_float4(L.x, L.y, L.z, 0.0f);
             ps.p[path].meta = meta;
         }
-        const uint32_t se = wavePush(&q.counts[nextSlot], pushExt);
-        if (pushExt) qnext[se] = path;
-        const uint32_t sd = wavePush(&q.counts[kCntDone], pushDone);
-        if (pushDone) q.done[sd] = path;
-        if (totals) {
-            const uint64_t me = __ballot(pushExt), msh = __ballot(pushShadow), mm = __ballot(pushMis || pushMisQuery);
-            if (__lane_id() == 0) {
-                if (me) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotExt]), __popcll(me));
-                if (msh) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotShadow]), __popcll(msh));
-                if (mm) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotMis]), __popcll(mm));
-            }
+        // block-wide compaction: per-wave counts, one atomic per queue, then every lane's slot
+        const uint64_t mExt = __ballot(pushExt), mDone = __ballot(pushDone), mNee = __ballot(pushShadow),
+                       mMq = __ballot(pushMisQuery), mMis = __ballot(pushMis);
+        if (lane == 0) {
+            lds.cnt[0][wave] = __popcll(mExt);
+            lds.cnt[1][wave] = __popcll(mDone);
+            lds.cnt[2][wave] = __popcll(mNee);
+            lds.cnt[3][wave] = __popcll(mMq);
+            lds.cnt[4][wave] = __popcll(mMis);
         }
+        __syncthreads();
+        if (tid < 4) {  // queue tid: ext, done, shadow (NEE + MIS queries), mis
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < kShadeBlock / 64; ++w)
+                t += tid == 2 ? lds.cnt[2][w] + lds.cnt[3][w] : lds.cnt[tid == 3 ? 4 : tid][w];
+            uint32_t* ctr = tid == 0 ? &q.counts[nextSlot]
+                            : tid == 1 ? &q.counts[kCntDone]
+                            : tid == 2 ? &q.counts[kCntShadow]
+                                       : &q.counts[kCntMis];
+            lds.base[tid] = t ? atomicAdd(ctr, t) : 0u;
+        }
+        if (tid == 0)
+            for (uint32_t w = 0; w < kShadeBlock / 64; ++w) {
+                totExt += lds.cnt[0][w];
+                totNee += lds.cnt[2][w];
+                totMis += lds.cnt[3][w] + lds.cnt[4][w];
+            }
+        __syncthreads();
+        uint32_t oExt = lds.base[0], oDone = lds.base[1], oSh = lds.base[2], oMis = lds.base[3];
+        for (uint32_t w = 0; w < wave; ++w) {
+            oExt += lds.cnt[0][w];
+            oDone += lds.cnt[1][w];
+            oSh += lds.cnt[2][w] + lds.cnt[3][w];
+            oMis += lds.cnt[4][w];
+        }
+        if (pushExt) qnext[oExt + __popcll(mExt & lowerLanes)] = path;
+        if (pushDone) q.done[oDone + __popcll(mDone & lowerLanes)] = path;
+        if (pushShadow) q.shadow[oSh + __popcll(mNee & lowerLanes)] = lds.nee[tid];
+        if (pushMisQuery) q.shadow[oSh + __popcll(mNee) + __popcll(mMq & lowerLanes)] = lds.rec2[tid];
+        if (pushMis) q.mis[oMis + __popcll(mMis & lowerLanes)] = *reinterpret_cast<const MisRec*>(&lds.rec2[tid]);
+        __syncthreads();  // the staging slots and counts are reused by the next iteration
+    }
+    if (totals && tid == 0) {
+        if (totExt) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotExt]), totExt);
+        if (totNee) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotShadow]), totNee);
+        if (totMis) atomicAdd(reinterpret_cast<unsigned long long*>(&totals[kTotMis]), totMis);
     }
 }
 
@@ -507,8 +553,10 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
                                                GuideShade gs, GuideRecordOut tr, BatchTable bt) {
     const uint32_t lo = q.counts[kCntRecorded];
     const uint32_t n = q.counts[kCntDone] - lo;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    uint32_t wItem = 0, wCount = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop (ballots)
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // block-uniform loop (ballots, barrier)
         const uint32_t i = i0 + threadIdx.x;
         const bool act = i < n;
         uint32_t item = 0;
@@ -543,16 +591,34 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
                 }
             }
         }
-        // paths recorded per item: one atomic per distinct item among the wave's lanes
+        // paths recorded per item, counted per wave (wave-uniform); a change of item flushes
         uint64_t pending = __ballot(act);
         while (pending) {
             const int leader = __builtin_ctzll(pending);
             const uint32_t it = __shfl(item, leader);
             const uint64_t same = __ballot(act && item == it);
-            if (__lane_id() == static_cast<uint32_t>(leader)) atomicAdd(&ring.itemDone[it], static_cast<uint32_t>(__popcll(same)));
+            const uint32_t c = static_cast<uint32_t>(__popcll(same));
+            if (wCount != 0 && it != wItem) {
+                if (lane == 0) atomicAdd(&ring.itemDone[wItem], wCount);
+                wCount = 0;
+            }
+            wItem = it;
+            wCount += c;
             pending &= ~same;
         }
     }
+    // one device atomic per block and item (atomics on one counter serialise across the chip)
+    __shared__ uint32_t sItem[4], sCount[4];
+    if (lane == 0) sItem[wave] = wItem, sCount[wave] = wCount;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+            if (sCount[w] == 0) continue;
+            uint32_t c = sCount[w];
+            for (uint32_t w2 = w + 1; w2 < blockDim.x / 64; ++w2)
+                if (sCount[w2] != 0 && sItem[w2] == sItem[w]) c += sCount[w2], sCount[w2] = 0;
+            atomicAdd(&ring.itemDone[sItem[w]], c);
+        }
 }
 
 // Per source pixel of the rect: sums over its samples for each footprint origin

@@ -99,7 +99,8 @@ enum CounterSlot : uint32_t {
     kCntRecorded = 5,   // done entries below this have had their film records written
     kCntRegenBase = 6,  // this iteration's new paths take the slots done[base, base + count)
     kCntRegenCount = 7,
-    kCntNum = 8
+    kCntRegenExt = 8,   // ... and the extension queue entries [ext, ext + count)
+    kCntNum = 12
 };
 struct PoolPaths {
     unsigned long long next;        // number of the next path to start
