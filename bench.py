@@ -117,7 +117,7 @@ def guided_vs_plain(dev, sc, ref_spp, label):
     return out
 
 
-def batched(dev, n_scenes, steps):
+def batched(dev, n_scenes, steps, bvh_width, wavefront_paths):
     """Batched workload: n seeded scenes (log-uniform 10k-2M triangles, mixed resolutions, spp
     4/16/64, depth 8) in one pg_render_batch call; aggregate Mrays/s and per-scene latency
     (device time from the batch's start until the scene's film is complete). The same scenes
@@ -130,9 +130,9 @@ def batched(dev, n_scenes, steps):
     scs = [scenes.batch_scene(i) for i in range(n_scenes)]
     log(f"batch: {n_scenes} scenes generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    objs = [pg.Scene(dev, sc) for sc in scs]
-    log(f"batch: BVHs built in {time.perf_counter() - t0:.1f}s")
-    params = [pg.render_params(sc, flags=pg.PG_NO_STAGE_TIMING) for sc in scs]
+    objs = [pg.Scene(dev, sc, bvh_width=bvh_width) for sc in scs]
+    log(f"batch: BVH{bvh_width}s built in {time.perf_counter() - t0:.1f}s")
+    params = [pg.render_params(sc, flags=pg.PG_NO_STAGE_TIMING, wavefront_paths=wavefront_paths) for sc in scs]
     films = [pg.Film(dev, sc["width"], sc["height"]) for sc in scs]
     st, lat = pg.render_batch(dev, objs, params, films)  # warmup
     log(f"batch warmup: {st['ms_total']:.0f} ms")
@@ -156,6 +156,7 @@ def batched(dev, n_scenes, steps):
         s.render(p, f)
         t_seq += time.perf_counter() - t
     out = {"scenes": n_scenes, "triangles_total": int(sum(sc["indices"].shape[0] for sc in scs)),
+           "bvh_width": bvh_width, "wavefront_paths": wavefront_paths,
            "paths_per_batch": int(paths // steps), "steps": steps,
            "ms_per_batch": round(elapsed * 1e3 / steps, 1),
            "mrays_per_s": round(rays / elapsed / 1e6, 1), "mpaths_per_s": round(paths / elapsed / 1e6, 1),
@@ -176,8 +177,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=64)
-    ap.add_argument("--wavefront-paths", type=int, default=0)
-    ap.add_argument("--bvh-width", type=int, default=8)
+    ap.add_argument("--wavefront-paths", type=int, default=0, help="path-pool capacity (0: autotuned)")
+    ap.add_argument("--bvh-width", type=int, default=0, help="4 or 8 (0: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
@@ -197,9 +198,18 @@ def main():
     log(f"scene generated in {time.perf_counter() - t0:.1f}s: {sc['indices'].shape[0]} triangles")
     dev = pg.Device(0)
     t0 = time.perf_counter()
-    scene = pg.Scene(dev, sc, bvh_width=args.bvh_width)
+    scene = pg.Scene(dev, sc, bvh_width=args.bvh_width or 8)
+    log(f"device {dev.platform}; BVH built in {time.perf_counter() - t0:.1f}s")
+    tune = None
+    if not (args.bvh_width and args.wavefront_paths):  # autotuner: BVH width x path-pool capacity
+        t0 = time.perf_counter()
+        tune = scene.autotune(pg.render_params(sc))
+        log(f"autotune in {time.perf_counter() - t0:.1f}s: {tune}")
+        if args.bvh_width and args.bvh_width != tune["bvh_width"]:
+            scene.build_bvh(width=args.bvh_width)
+        args.wavefront_paths = args.wavefront_paths or tune["wavefront_paths"]
     info = scene.bvh_info()
-    log(f"device {dev.platform}; BVH{info['width']} built in {time.perf_counter() - t0:.1f}s: {info}")
+    log(f"BVH{info['width']}: {info}; path pool {args.wavefront_paths}")
     film = pg.Film(dev, sc["width"], sc["height"])
     params = pg.render_params(sc, wavefront_paths=args.wavefront_paths)
     copy_gbps = dev.copy_bandwidth(1 << 30, 3)  # kCopy: 1 GiB read + 1 GiB written per launch
@@ -229,6 +239,16 @@ def main():
             rays[k] += st[k]
         log(f"step {i}: {st['ms_total']:.1f} ms, {st['paths'] / st['ms_total'] / 1e3:.1f} Mpaths/s")
     elapsed = time.perf_counter() - t_start
+    # the same frames with the bounce loop replayed from a captured graph (no stage events)
+    gparams = pg.render_params(sc, wavefront_paths=args.wavefront_paths, flags=pg.PG_USE_GRAPH)
+    film.clear()
+    scene.render(gparams, film)
+    t_graph = time.perf_counter()
+    for i in range(args.steps):
+        film.clear()
+        scene.render(gparams, film)
+    graph_ms = (time.perf_counter() - t_graph) * 1e3 / args.steps
+    log(f"graph replay: {graph_ms:.1f} ms per frame")
     ms_per_step = elapsed * 1e3 / args.steps
     paths = sc["width"] * sc["height"] * args.spp * args.steps
     mpaths = paths / elapsed / 1e6
@@ -255,7 +275,9 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": "terrain_spheres_1920x1080_64spp_depth8_rr3", "triangles": int(sc["indices"].shape[0]),
                    "lights": 16, "spp": args.spp, "bvh_width": info["width"],
-                   "wavefront_paths": args.wavefront_paths or (1 << 21)},
+                   "wavefront_paths": args.wavefront_paths},
+        "autotune": tune,
+        "graph_ms_per_step": round(graph_ms, 2),
         "mrays_per_s": round(total_rays / elapsed / 1e6, 2),
         "copy_kernel_gbps": round(copy_gbps, 1),
         "rays_per_path": round(total_rays / paths, 3),
@@ -267,7 +289,7 @@ def main():
                      "launches": launches},
     }
     if args.batch_scenes > 0:
-        result["batched"] = batched(dev, args.batch_scenes, args.batch_steps)
+        result["batched"] = batched(dev, args.batch_scenes, args.batch_steps, info["width"], args.wavefront_paths)
         log(f"batched: {result['batched']}")
     if not args.no_guided:
         # time to 1% relMSE, guided (online training, one-sample MIS at 0.5) vs plain

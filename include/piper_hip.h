@@ -35,6 +35,8 @@ extern "C" {
 #define PG_FILM_FP64 0x100u /* accumulate into the film's FP64 RGBW buffer instead of FP32 */
 #define PG_COUNT_TRAVERSAL 0x200u /* count BVH nodes visited / triangles tested by extension rays (slower) */
 #define PG_NO_STAGE_TIMING 0x400u /* stats without per-stage device timing (no events around launches) */
+#define PG_USE_GRAPH 0x800u /* capture the bounce loop's kernels once per call, replay them as hipGraph
+                               launches (implies PG_NO_STAGE_TIMING) */
 
 #define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
 #define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
@@ -151,6 +153,20 @@ int pg_render_paths(pg_scene* scene, const pg_render_params* params, pg_film* fi
  * of the batch until scene i's film is complete. */
 int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
                     pg_film* const* films, double* latency_ms, pg_stats* stats);
+
+/* Autotuner (no reference counterpart: the reference has no tunable launch shape). Times a slice
+ * of `params` (the whole rect, the first samples of its range, about sample_paths paths) at BVH
+ * widths 4 and 8 and path-pool capacities 2^21..2^24, leaves the scene built at the fastest width
+ * and returns the fastest capacity; pass it as params->wavefront_paths. The films are untouched. */
+typedef struct pg_tune_result {
+    uint32_t bvh_width;        /* the scene's BVH width from now on */
+    uint32_t wavefront_paths;  /* fastest path-pool capacity */
+    uint64_t sample_paths;     /* paths per timed trial */
+    uint32_t trials;
+    uint32_t reserved;
+    double mpaths_per_s;       /* throughput of the best trial */
+} pg_tune_result;
+int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample_paths, pg_tune_result* out);
 
 /* ---- parity entry points (Embree rtcIntersect1 / rtcOccluded1, Embree.cpp:300,384; sampler
  *      sobolStart / sobolGenerate, ScrambledSobolSampler/Kernel.cpp:52-67) ---- */

@@ -25,6 +25,7 @@ PG_COMPAT_D13 = 0x8
 PG_FILM_FP64 = 0x100
 PG_COUNT_TRAVERSAL = 0x200
 PG_NO_STAGE_TIMING = 0x400
+PG_USE_GRAPH = 0x800
 
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
@@ -86,6 +87,15 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("bvh_width", ctypes.c_uint32), ("wavefront_paths", ctypes.c_uint32),
+                ("sample_paths", ctypes.c_uint64), ("trials", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("mpaths_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 # Every symbol the C ABI declares, with (restype, argtypes).
 _VP = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -123,6 +133,7 @@ SIGNATURES = {
     "pg_render_guided": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
     "pg_render_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _U32, _VP, _VP,
                                        ctypes.POINTER(Stats)]),
+    "pg_autotune": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), ctypes.c_uint64, ctypes.POINTER(TuneResult)]),
 }
 
 _lib = None
@@ -276,6 +287,12 @@ class Scene:
             return st.as_dict(), out
         check(lib().pg_render(self.h, ctypes.byref(params), film.h if film else None, ctypes.byref(st)))
         return st.as_dict()
+
+    def autotune(self, params, sample_paths=1 << 25):
+        """pg_autotune: rebuilds the BVH at the fastest width; returns the chosen configuration."""
+        out = TuneResult()
+        check(lib().pg_autotune(self.h, ctypes.byref(params), sample_paths, ctypes.byref(out)))
+        return out.as_dict()
 
     def render_guided(self, guide, params, film=None):
         st = Stats()

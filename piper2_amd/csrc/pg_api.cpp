@@ -89,6 +89,19 @@ void upload(DevBuf& b, const void* host, size_t n, hipStream_t st) {
 // kStatusItems work items from the oldest unresolved one.
 constexpr size_t kStatusItems = 64;
 constexpr size_t kStatusBytes = pg::kCountsBytes + kStatusItems * 4;
+// ... followed by the pool's generation limit on its way to the device (8 B, padded).
+constexpr size_t kPinnedBytes = kStatusBytes + 64;
+
+// Instantiated graph, destroyed with its owner.
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec&) = delete;
+    GraphExec& operator=(const GraphExec&) = delete;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
 
 }  // namespace
 
@@ -137,6 +150,7 @@ struct pg_scene {
     std::vector<float> triVerts;   // 9 floats per triangle, world space
     std::vector<uint32_t> triMat;  // material per original triangle
     pg::WideBvh bvh;
+    pg::BvhParams bvhParams;  // of the current build
     pg::DevScene ds{};
     DevBuf tris, triMatLeaf, nodes, mats, quads;
     uint32_t spillPerLane = 1;
@@ -248,6 +262,7 @@ void buildBvh(pg_scene* s, const pg_bvh_params* params) {
     }
     const uint32_t nt = static_cast<uint32_t>(s->triMat.size());
     pg::buildWideBvh(s->triVerts.data(), nt, bp, s->bvh);
+    s->bvhParams = bp;
     hipStream_t st = s->dev->stream;
     std::vector<uint32_t> leafMat(std::max<uint32_t>(nt, 1));
     for (uint32_t i = 0; i < nt; ++i) leafMat[i] = s->triMat[s->bvh.order[i]];
@@ -409,7 +424,8 @@ struct RunOptions {
     double* latencyMs = nullptr;  // per scene
     pg_stats* stats = nullptr;
     bool countTraversal = false;
-    bool timed = false;
+    bool timed = false;           // per-stage events (not with graph)
+    bool graph = false;           // replay captured groups of iterations
 };
 
 // Iterations between two reads of the render's progress (each read waits for the stream).
@@ -554,17 +570,18 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
 
     timeStage(kRaygen, [&] { pg::launchPoolInit(st, q, cap); });
     uint2* spill = dev->spill.as<uint2>();
-    uint64_t iterations = 0, lastNext = ~0ull;
-    uint32_t lastFree = ~0u, stalled = 0;
-    uint32_t cur = 0;
-    auto* statusCounts = static_cast<const uint32_t*>(dev->status);
-    auto* statusPool = reinterpret_cast<const pg::PoolPaths*>(static_cast<const char*>(dev->status) + pg::kCntNum * 4);
-    auto* statusItems = reinterpret_cast<const uint32_t*>(static_cast<const char*>(dev->status) + pg::kCountsBytes);
-    for (;;) {
+    // generation limit: written to device memory from pinned host memory after every progress read
+    auto* limitHost = reinterpret_cast<uint64_t*>(static_cast<char*>(dev->status) + kStatusBytes);
+    auto pushLimit = [&] {
+        *limitHost = limitNow();
+        PG_HIP(hipMemcpyAsync(&q.pool->limit, limitHost, 8, hipMemcpyHostToDevice, st));
+    };
+    pushLimit();
+    // one bounce for every live path, after restarting the free slots
+    auto iterate = [&](uint32_t cur) {
         const uint32_t nxt = cur ^ 1u;
-        const uint64_t limit = limitNow();
         timeStage(kRaygen, [&] {
-            pg::launchIterBegin(st, q, cur, limit);
+            pg::launchIterBegin(st, q, cur);
             pg::launchRegen(st, ds0, ps, q, rc0, cur, threads, btp);
         });
         timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, trav, btp); });
@@ -580,10 +597,39 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
         timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
         if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
         timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, rc0, threads, ring, gs, tr, btp); });
+    };
+    // Iterations run in groups of kCheckEvery (even, so every group starts on extension queue 0).
+    // With opt.graph the group is captured once and replayed as one graph launch.
+    GraphExec group;
+    if (opt.graph) {
+        require(!opt.timed, "stage timing is not available with graph launches");
+        PG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            for (uint32_t k = 0; k < kCheckEvery; ++k) iterate(k & 1u);
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        PG_HIP(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&group.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PG_HIP(e);
+    }
+    uint64_t iterations = 0, lastNext = ~0ull;
+    uint32_t lastFree = ~0u, stalled = 0;
+    auto* statusCounts = static_cast<const uint32_t*>(dev->status);
+    auto* statusPool = reinterpret_cast<const pg::PoolPaths*>(static_cast<const char*>(dev->status) + pg::kCntNum * 4);
+    auto* statusItems = reinterpret_cast<const uint32_t*>(static_cast<const char*>(dev->status) + pg::kCountsBytes);
+    for (;;) {
+        if (group.exec) {
+            PG_HIP(hipGraphLaunch(group.exec, st));
+        } else {
+            for (uint32_t k = 0; k < kCheckEvery; ++k) iterate(k & 1u);
+        }
         PG_HIP(hipGetLastError());
-        cur = nxt;
-        ++iterations;
-        if (iterations % kCheckEvery != 0) continue;
+        iterations += kCheckEvery;
         // progress: pool counters and the recorded-path counts of the oldest unresolved items
         const size_t window = std::min(kStatusItems, items.size() - std::min(nextResolve, items.size()));
         PG_HIP(hipMemcpyAsync(dev->status, dev->counts.p, pg::kCountsBytes, hipMemcpyDeviceToHost, st));
@@ -601,6 +647,7 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
         stalled = (next == lastNext && freeSlots == lastFree) ? stalled + kCheckEvery : 0;
         if (stalled > kStallLimit) throw PgError(PG_ERR_DEVICE, "render made no progress");
         lastNext = next, lastFree = freeSlots;
+        pushLimit();
     }
     while (nextResolve < items.size()) resolveItem(nextResolve++);  // all recorded by now
     PG_HIP(hipGetLastError());
@@ -682,7 +729,8 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
     opt.pathLHost = pathLHost;
     opt.stats = stats;
     opt.countTraversal = rc.flags & pg::kCountTraversal;
-    opt.timed = stats != nullptr && !(rc.flags & pg::kNoStageTiming);
+    opt.graph = rc.flags & pg::kUseGraph;
+    opt.timed = stats != nullptr && !(rc.flags & pg::kNoStageTiming) && !opt.graph;
     runPaths(dev, &s, 1, items, total, opt);
 }
 
@@ -715,8 +763,58 @@ void renderBatch(pg_device* dev, pg_scene* const* scenes, const pg_render_params
     opt.capacity = params[0].wavefront_paths ? params[0].wavefront_paths : (1u << 21);
     opt.latencyMs = latencyMs;
     opt.stats = stats;
-    opt.timed = stats != nullptr && !(params[0].flags & pg::kNoStageTiming);
+    opt.graph = params[0].flags & pg::kUseGraph;
+    opt.timed = stats != nullptr && !(params[0].flags & pg::kNoStageTiming) && !opt.graph;
     runPaths(dev, scenes, n, items, total, opt);
+}
+
+// Autotuner: times a slice of the render (the whole rect, the first samples of its range, about
+// samplePaths paths) for every BVH width and path-pool capacity it tries, leaves the scene built
+// at the fastest width and reports the fastest capacity. Each trial runs twice and the second run
+// is timed, so workspace growth is not counted.
+void autotune(pg_scene* s, const pg_render_params* params, uint64_t samplePaths, pg_tune_result* out) {
+    require(s && params && out, "null argument");
+    const pg::RenderConsts rc = makeConsts(*params);
+    const uint64_t pixels = static_cast<uint64_t>(rc.rectW) * rc.rectH;
+    const uint32_t k = static_cast<uint32_t>(
+        std::max<uint64_t>(1, std::min<uint64_t>(rc.sppCount, std::max<uint64_t>(samplePaths, 1) / pixels)));
+    pg_render_params p = *params;
+    p.spp_end = p.spp_begin + k;
+    p.flags &= ~(pg::kCountTraversal | pg::kFilmFp64);
+    const uint64_t paths = pixels * k;
+    const uint32_t widths[2] = {4, 8};
+    const uint32_t caps[4] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    double bestMs = 1e300;
+    uint32_t bestWidth = s->ds.width, bestCap = caps[0], trials = 0;
+    for (uint32_t w : widths) {
+        if (s->ds.width != w && !s->triMat.empty()) {
+            std::lock_guard<std::mutex> lock(s->dev->mu);
+            pg_bvh_params bp{w, s->bvhParams.leafSize, s->bvhParams.bins, 0};
+            buildBvh(s, &bp);
+        }
+        for (uint32_t c : caps) {
+            if (c > caps[0] && c / 2 > paths) continue;  // a pool larger than the work changes nothing
+            p.wavefront_paths = c;
+            double ms = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {
+                const auto t0 = std::chrono::steady_clock::now();
+                render(s, &p, nullptr, nullptr, nullptr);
+                ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
+            ++trials;
+            if (ms < bestMs) bestMs = ms, bestWidth = s->ds.width, bestCap = c;
+        }
+    }
+    if (s->ds.width != bestWidth && !s->triMat.empty()) {
+        std::lock_guard<std::mutex> lock(s->dev->mu);
+        pg_bvh_params bp{bestWidth, s->bvhParams.leafSize, s->bvhParams.bins, 0};
+        buildBvh(s, &bp);
+    }
+    out->bvh_width = bestWidth;
+    out->wavefront_paths = bestCap;
+    out->sample_paths = paths;
+    out->trials = trials;
+    out->mpaths_per_s = static_cast<double>(paths) / bestMs / 1e3;
 }
 
 }  // namespace
@@ -739,7 +837,7 @@ int pg_device_create(int ordinal, pg_device** out) {
         dev->numCUs = prop.multiProcessorCount;
         dev->platform = std::string("AMD HIP ") + prop.gcnArchName;
         PG_HIP(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
-        PG_HIP(hipHostMalloc(&dev->status, kStatusBytes, hipHostMallocDefault));
+        PG_HIP(hipHostMalloc(&dev->status, kPinnedBytes, hipHostMallocDefault));
         std::vector<uint32_t> m;
         std::vector<uint64_t> inv, delta;
         pg::buildSobolTables(m, inv, delta);
@@ -980,6 +1078,10 @@ int pg_render_paths(pg_scene* s, const pg_render_params* params, pg_film* film, 
 int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
                     pg_film* const* films, double* latency_ms, pg_stats* stats) {
     return guarded([&] { renderBatch(dev, scenes, params, n, films, latency_ms, stats); });
+}
+
+int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample_paths, pg_tune_result* out) {
+    return guarded([&] { autotune(scene, params, sample_paths, out); });
 }
 
 int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv) {

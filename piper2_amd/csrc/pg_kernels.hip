@@ -75,11 +75,13 @@ __global__ void __launch_bounds__(256) kPoolInit(Queues q, uint32_t capacity) {
     }
 }
 
-// One thread at the start of an iteration: new path numbers for the free slots (up to `limit`),
-// and the counter resets of the bounce (a kernel, not a memset, so the sequence can be captured).
-__global__ void kIterBegin(Queues q, uint32_t curSlot, unsigned long long limit) {
+// One thread at the start of an iteration: new path numbers for the free slots (up to the limit
+// the host keeps in device memory), and the counter resets of the bounce (a kernel, not a memset,
+// so the sequence can be captured in a graph).
+__global__ void kIterBegin(Queues q, uint32_t curSlot) {
     const uint32_t top = q.counts[kCntDone];
     const unsigned long long next = q.pool->next;
+    const unsigned long long limit = q.pool->limit;
     const unsigned long long avail = limit > next ? limit - next : 0ull;
     const uint32_t m = avail < top ? static_cast<uint32_t>(avail) : top;
     q.counts[kCntRegenBase] = top - m;
@@ -748,8 +750,8 @@ static inline uint32_t blocksFor(uint64_t n, uint32_t bs, uint32_t cap) {
 void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity) {
     hipLaunchKernelGGL(kPoolInit, dim3(blocksFor(capacity, 256, 1u << 30)), dim3(256), 0, st, q, capacity);
 }
-void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot, uint64_t limit) {
-    hipLaunchKernelGGL(kIterBegin, dim3(1), dim3(1), 0, st, q, curSlot, static_cast<unsigned long long>(limit));
+void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot) {
+    hipLaunchKernelGGL(kIterBegin, dim3(1), dim3(1), 0, st, q, curSlot);
 }
 void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt) {

@@ -81,10 +81,10 @@ struct RecordRing {
 // Batched launches still take the first scene as `s`: all scenes share its BVH width.
 //
 // Path pool: `capacity` slots. launchPoolInit marks every slot free. Each iteration,
-// launchIterBegin (one thread) takes up to `limit` - next new path numbers for the free slots,
-// then launchRegen starts those paths and appends their slots to extension queue curSlot.
+// launchIterBegin (one thread) takes up to pool->limit - pool->next new path numbers for the free
+// slots, then launchRegen starts those paths and appends their slots to extension queue curSlot.
 void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity);
-void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot, uint64_t limit);
+void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot);
 void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt = nullptr);
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,

@@ -105,6 +105,7 @@ enum CounterSlot : uint32_t {
 struct PoolPaths {
     unsigned long long next;        // number of the next path to start
     unsigned long long regenFirst;  // number of this iteration's first new path
+    unsigned long long limit;       // paths numbered below this may start (set by the host)
 };
 constexpr uint32_t kCountsBytes = kCntNum * 4 + sizeof(PoolPaths);
 enum TotalSlot : uint32_t { kTotExt = 0, kTotShadow = 1, kTotMis = 2, kTotNum = 4 };
@@ -123,5 +124,6 @@ constexpr uint32_t kCompatD1 = 0x1u, kCompatD3 = 0x2u, kCompatD9 = 0x4u, kCompat
 constexpr uint32_t kFilmFp64 = 0x100u;
 constexpr uint32_t kCountTraversal = 0x200u;
 constexpr uint32_t kNoStageTiming = 0x400u;
+constexpr uint32_t kUseGraph = 0x800u;
 
 }  // namespace pg

@@ -101,6 +101,43 @@ def test_batch_without_films_and_stats(dev):
     assert lat.shape == (2,) and lat[0] <= lat[1]
 
 
+def test_graph_replay_matches_eager(dev):
+    """PG_USE_GRAPH captures a group of bounce iterations and replays it (tens of replays per render,
+    with host progress reads in between): films must equal the eager launches' bit for bit, for
+    single and batched renders, on every one of several consecutive renders."""
+    items = mixed_batch()[:4]
+    objs = [pg.Scene(dev, sc) for sc, _ in items]
+    for (sc, p), s in zip(items, objs):
+        p.wavefront_paths = 1 << 12  # small pool: many replays per render
+        f = pg.Film(dev, sc["width"], sc["height"])
+        s.render(p, f)
+        eager = download(f, p)
+        g = pg.render_params(sc, rect=tuple(p.rect), spp_range=(p.spp_begin, p.spp_end),
+                             flags=p.flags | pg.PG_USE_GRAPH, wavefront_paths=1 << 12)
+        for _ in range(3):
+            f.clear()
+            st = s.render(g, f)
+            assert st["extend_launches"] >= 8
+            assert np.array_equal(download(f, p), eager)
+        f.close()
+    params = [p for _, p in items]
+    singles = []
+    for (sc, p), s in zip(items, objs):
+        f = pg.Film(dev, sc["width"], sc["height"])
+        s.render(p, f)
+        singles.append(download(f, p))
+        f.close()
+    gparams = [pg.render_params(sc, rect=tuple(p.rect), spp_range=(p.spp_begin, p.spp_end),
+                                flags=p.flags | (pg.PG_USE_GRAPH if k == 0 else 0), wavefront_paths=1 << 12)
+               for k, (sc, p) in enumerate(items)]
+    for _ in range(2):
+        films = [pg.Film(dev, sc["width"], sc["height"]) for sc, _ in items]
+        pg.render_batch(dev, objs, gparams, films)
+        for k, (f, p) in enumerate(zip(films, params)):
+            assert np.array_equal(download(f, p), singles[k]), k
+            f.close()
+
+
 def test_batch_rejects_mixed_bvh_width(dev):
     sc = scenes.cornell(width=16, height=16, spp=1)
     a, b = pg.Scene(dev, sc, bvh_width=8), pg.Scene(dev, sc, bvh_width=4)
