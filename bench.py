@@ -181,6 +181,7 @@ def main():
     ap.add_argument("--bvh-width", type=int, default=0, help="4 or 8 (0: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--ref-spp", type=int, default=16384, help="reference samples per pixel of the guided comparison")
@@ -239,16 +240,17 @@ def main():
             rays[k] += st[k]
         log(f"step {i}: {st['ms_total']:.1f} ms, {st['paths'] / st['ms_total'] / 1e3:.1f} Mpaths/s")
     elapsed = time.perf_counter() - t_start
-    # the same frames with the bounce loop replayed from a captured graph (no stage events)
-    gparams = pg.render_params(sc, wavefront_paths=args.wavefront_paths, flags=pg.PG_USE_GRAPH)
-    film.clear()
-    scene.render(gparams, film)
-    t_graph = time.perf_counter()
-    for i in range(args.steps):
+    graph_ms = None
+    if not args.no_graph:  # the same frames with the bounce loop replayed from a captured graph
+        gparams = pg.render_params(sc, wavefront_paths=args.wavefront_paths, flags=pg.PG_USE_GRAPH)
         film.clear()
         scene.render(gparams, film)
-    graph_ms = (time.perf_counter() - t_graph) * 1e3 / args.steps
-    log(f"graph replay: {graph_ms:.1f} ms per frame")
+        t_graph = time.perf_counter()
+        for i in range(args.steps):
+            film.clear()
+            scene.render(gparams, film)
+        graph_ms = round((time.perf_counter() - t_graph) * 1e3 / args.steps, 2)
+        log(f"graph replay: {graph_ms} ms per frame")
     ms_per_step = elapsed * 1e3 / args.steps
     paths = sc["width"] * sc["height"] * args.spp * args.steps
     mpaths = paths / elapsed / 1e6
@@ -277,7 +279,7 @@ def main():
                    "lights": 16, "spp": args.spp, "bvh_width": info["width"],
                    "wavefront_paths": args.wavefront_paths},
         "autotune": tune,
-        "graph_ms_per_step": round(graph_ms, 2),
+        "graph_ms_per_step": graph_ms,
         "mrays_per_s": round(total_rays / elapsed / 1e6, 2),
         "copy_kernel_gbps": round(copy_gbps, 1),
         "rays_per_path": round(total_rays / paths, 3),
