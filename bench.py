@@ -55,6 +55,20 @@ def cpu_baseline(sc, threads, budget_s=12.0):
                       f"({paths} paths, {dt:.1f} s) of the same 1920x1080 frame"}
 
 
+def measured_traffic(bvh_width, wavefront_paths):
+    """HBM bytes per kExtend launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    this workload (profiles/), when they were taken at the same BVH width and pool size."""
+    path = os.path.join(ROOT, "profiles", "kextend_hbm_traffic_terrain_spheres_1080p64.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t["bvh_width"] != bvh_width or t["wavefront_paths"] != wavefront_paths:
+        return None
+    return t["hbm_bytes_per_launch"]
+
+
 def _img(f):
     import numpy as np
     return f[..., :3] / np.maximum(f[..., 3:], 1e-30)
@@ -285,7 +299,9 @@ def main():
         "rays_per_path": round(total_rays / paths, 3),
         "stage_ms_per_step": {k[3:]: round(v / args.steps, 2) for k, v in totals.items()},
         "roofline": {"bound": "hbm", "kernel": "kExtend", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(info["width"], args.wavefront_paths),
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch),
                      "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
                      "tris_per_ray": round(tris_per_ray, 3), "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "launches": launches},
