@@ -85,11 +85,13 @@ typedef struct pg_scene_desc {
     float env_radiance[3]; /* constant environment (BasicLight/Kernel.cpp:59-89) */
 } pg_scene_desc;
 
+#define PG_BVH_HOST 0u   /* binned SAH on the host's cores (multithreaded) */
+#define PG_BVH_DEVICE 1u /* binned SAH on the GPU (bins capped at 32) */
 typedef struct pg_bvh_params {
     uint32_t width;     /* 4 or 8 children per node */
     uint32_t leaf_size; /* max triangles per leaf, 1..8 */
     uint32_t sah_bins;  /* 8..64 */
-    uint32_t reserved;
+    uint32_t builder;   /* PG_BVH_HOST or PG_BVH_DEVICE */
 } pg_bvh_params;
 
 typedef struct pg_render_params {

@@ -26,6 +26,8 @@ PG_FILM_FP64 = 0x100
 PG_COUNT_TRAVERSAL = 0x200
 PG_NO_STAGE_TIMING = 0x400
 PG_USE_GRAPH = 0x800
+PG_BVH_HOST = 0
+PG_BVH_DEVICE = 1
 
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
@@ -61,7 +63,7 @@ class SceneDesc(ctypes.Structure):
 
 class BvhParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32), ("sah_bins", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("builder", ctypes.c_uint32)]
 
 
 class RenderParams(ctypes.Structure):
@@ -217,17 +219,17 @@ def sobol_matrices():
 class Scene:
     """A scene on the device (see piper2_amd.scenes for the seeded generators)."""
 
-    def __init__(self, device, sc, bvh_width=8, leaf_size=4, bins=32):
+    def __init__(self, device, sc, bvh_width=8, leaf_size=4, bins=32, builder=None):
         self.device = device
         self._keep = sc
         desc = scene_desc(sc)
         self.h = ctypes.c_void_p()
         check(lib().pg_scene_create(device.h, ctypes.byref(desc), ctypes.byref(self.h)))
-        if (bvh_width, leaf_size, bins) != (8, 4, 32):
-            self.build_bvh(bvh_width, leaf_size, bins)
+        if (bvh_width, leaf_size, bins) != (8, 4, 32) or builder is not None:
+            self.build_bvh(bvh_width, leaf_size, bins, builder)
 
-    def build_bvh(self, width=8, leaf_size=4, bins=32):
-        p = BvhParams(width, leaf_size, bins, 0)
+    def build_bvh(self, width=8, leaf_size=4, bins=32, builder=None):
+        p = BvhParams(width, leaf_size, bins, PG_BVH_HOST if builder is None else builder)
         check(lib().pg_scene_build_bvh(self.h, ctypes.byref(p)))
 
     def bvh_info(self):

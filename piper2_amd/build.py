@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpiper_hip.so")
-SOURCES = ["pg_api.cpp", "pg_kernels.hip", "pg_guide.hip", "pg_bvh.cpp", "pg_sobol.cpp"]
+SOURCES = ["pg_api.cpp", "pg_kernels.hip", "pg_guide.hip", "pg_bvh_gpu.hip", "pg_bvh.cpp", "pg_sobol.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

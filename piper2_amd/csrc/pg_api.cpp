@@ -151,6 +151,7 @@ struct pg_scene {
     std::vector<uint32_t> triMat;  // material per original triangle
     pg::WideBvh bvh;
     pg::BvhParams bvhParams;  // of the current build
+    uint32_t builder = PG_BVH_HOST;
     pg::DevScene ds{};
     DevBuf tris, triMatLeaf, nodes, mats, quads;
     uint32_t spillPerLane = 1;
@@ -261,15 +262,45 @@ void buildBvh(pg_scene* s, const pg_bvh_params* params) {
         bp.bins = params->sah_bins;
     }
     const uint32_t nt = static_cast<uint32_t>(s->triMat.size());
-    pg::buildWideBvh(s->triVerts.data(), nt, bp, s->bvh);
-    s->bvhParams = bp;
     hipStream_t st = s->dev->stream;
-    std::vector<uint32_t> leafMat(std::max<uint32_t>(nt, 1));
-    for (uint32_t i = 0; i < nt; ++i) leafMat[i] = s->triMat[s->bvh.order[i]];
-    upload(s->tris, s->bvh.tris.data(), s->bvh.tris.size() * 4, st);
-    upload(s->triMatLeaf, leafMat.data(), leafMat.size() * 4, st);
-    upload(s->nodes, s->bvh.nodes.data(), s->bvh.nodes.size() * 4, st);
-    PG_HIP(hipStreamSynchronize(st));
+    const bool onDevice = params && params->builder == PG_BVH_DEVICE && nt > 0;
+    require(!params || params->builder == PG_BVH_HOST || params->builder == PG_BVH_DEVICE, "unknown BVH builder");
+    if (onDevice) {  // binned SAH on the GPU; host keeps only the shape
+        require(bp.width == 4 || bp.width == 8, "BVH width must be 4 or 8");
+        require(bp.leafSize >= 1 && bp.leafSize <= pg::kMaxLeafSize, "leaf size must be 1..8");
+        DevBuf verts, mats;
+        upload(verts, s->triVerts.data(), static_cast<size_t>(nt) * 36, st);
+        upload(mats, s->triMat.data(), static_cast<size_t>(nt) * 4, st);
+        pg::DeviceBvh db;
+        db.alloc = [&](pg::DeviceBvh::Buffer b, size_t bytes) -> void* {
+            DevBuf& d = b == pg::DeviceBvh::kNodes ? s->nodes : b == pg::DeviceBvh::kTris ? s->tris : s->triMatLeaf;
+            d.reserve(bytes);
+            return d.p;
+        };
+        try {
+            pg::buildWideBvhDevice(st, verts.as<float>(), mats.as<uint32_t>(), nt, bp, db);
+        } catch (const std::runtime_error& e) {
+            throw PgError(PG_ERR_DEVICE, e.what());
+        }
+        s->bvh = pg::WideBvh{};
+        s->bvh.width = db.width;
+        s->bvh.numNodes = db.numNodes;
+        s->bvh.numLeaves = db.numLeaves;
+        s->bvh.maxDepth = db.maxDepth;
+        s->bvh.pad = db.pad;
+    } else {
+        pg::buildWideBvh(s->triVerts.data(), nt, bp, s->bvh);
+        std::vector<uint32_t> leafMat(std::max<uint32_t>(nt, 1));
+        for (uint32_t i = 0; i < nt; ++i) leafMat[i] = s->triMat[s->bvh.order[i]];
+        upload(s->tris, s->bvh.tris.data(), s->bvh.tris.size() * 4, st);
+        upload(s->triMatLeaf, leafMat.data(), leafMat.size() * 4, st);
+        upload(s->nodes, s->bvh.nodes.data(), s->bvh.nodes.size() * 4, st);
+        PG_HIP(hipStreamSynchronize(st));
+        s->bvh.nodes.clear(), s->bvh.nodes.shrink_to_fit();  // the device copy is the one in use
+        s->bvh.tris.clear(), s->bvh.tris.shrink_to_fit();
+    }
+    s->bvhParams = bp;
+    s->builder = onDevice ? PG_BVH_DEVICE : PG_BVH_HOST;
     s->ds.tris = s->tris.as<float4>();
     s->ds.triMat = s->triMatLeaf.as<uint32_t>();
     s->ds.nodes = s->nodes.p;
@@ -998,9 +1029,12 @@ int pg_scene_bvh_info(pg_scene* s, uint64_t info[6]) {
 int pg_scene_bvh_download(pg_scene* s, void* nodes, uint64_t nodeBytes, void* tris, uint64_t triBytes) {
     return guarded([&] {
         require(s != nullptr, "null scene");
-        require(nodeBytes >= s->bvh.nodes.size() * 4 && triBytes >= s->bvh.tris.size() * 4, "buffers too small");
-        if (nodes) std::memcpy(nodes, s->bvh.nodes.data(), s->bvh.nodes.size() * 4);
-        if (tris) std::memcpy(tris, s->bvh.tris.data(), s->bvh.tris.size() * 4);
+        std::lock_guard<std::mutex> lock(s->dev->mu);
+        const size_t nb = static_cast<size_t>(s->bvh.numNodes) * 7 * 4 * s->bvh.width;
+        const size_t tb = s->triMat.size() * 48;
+        require((!nodes || nodeBytes >= nb) && (!tris || triBytes >= tb), "buffers too small");
+        if (nodes && nb) PG_HIP(hipMemcpy(nodes, s->nodes.p, nb, hipMemcpyDeviceToHost));
+        if (tris && tb) PG_HIP(hipMemcpy(tris, s->tris.p, tb, hipMemcpyDeviceToHost));
     });
 }
 
