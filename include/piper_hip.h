@@ -124,6 +124,8 @@ const char* pg_device_platform(pg_device* dev); /* "AMD HIP gfx950" (getNativePl
 const char* pg_last_error(void);
 
 /* ---- scene (Tracer::buildAcceleration / buildNode, Tracer.hpp:140-159; Embree.cpp:497-655) ---- */
+/* Copies the scene to the device and builds its BVH there (width 8, leaves of <= 4, 32 bins,
+ * PG_BVH_DEVICE); pg_scene_build_bvh rebuilds it with other parameters or on the host. */
 int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out);
 int pg_scene_build_bvh(pg_scene* scene, const pg_bvh_params* params);
 void pg_scene_destroy(pg_scene* scene);

@@ -229,7 +229,8 @@ class Scene:
             self.build_bvh(bvh_width, leaf_size, bins, builder)
 
     def build_bvh(self, width=8, leaf_size=4, bins=32, builder=None):
-        p = BvhParams(width, leaf_size, bins, PG_BVH_HOST if builder is None else builder)
+        """Rebuild the BVH (builder: PG_BVH_DEVICE by default, or PG_BVH_HOST)."""
+        p = BvhParams(width, leaf_size, bins, PG_BVH_DEVICE if builder is None else builder)
         check(lib().pg_scene_build_bvh(self.h, ctypes.byref(p)))
 
     def bvh_info(self):

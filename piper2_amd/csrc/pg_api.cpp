@@ -820,7 +820,7 @@ void autotune(pg_scene* s, const pg_render_params* params, uint64_t samplePaths,
     for (uint32_t w : widths) {
         if (s->ds.width != w && !s->triMat.empty()) {
             std::lock_guard<std::mutex> lock(s->dev->mu);
-            pg_bvh_params bp{w, s->bvhParams.leafSize, s->bvhParams.bins, 0};
+            pg_bvh_params bp{w, s->bvhParams.leafSize, s->bvhParams.bins, s->builder};
             buildBvh(s, &bp);
         }
         for (uint32_t c : caps) {
@@ -838,7 +838,7 @@ void autotune(pg_scene* s, const pg_render_params* params, uint64_t samplePaths,
     }
     if (s->ds.width != bestWidth && !s->triMat.empty()) {
         std::lock_guard<std::mutex> lock(s->dev->mu);
-        pg_bvh_params bp{bestWidth, s->bvhParams.leafSize, s->bvhParams.bins, 0};
+        pg_bvh_params bp{bestWidth, s->bvhParams.leafSize, s->bvhParams.bins, s->builder};
         buildBvh(s, &bp);
     }
     out->bvh_width = bestWidth;
@@ -988,7 +988,8 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
             s->invExt[k] = 1.0f / (bhi[k] - blo[k]);
         }
         std::memcpy(ds.env, desc->env_radiance, 12);
-        buildBvh(s.get(), nullptr);
+        const pg_bvh_params defaults{8, 4, 32, PG_BVH_DEVICE};  // BVH8, leaves of <= 4, 32 bins, built on the GPU
+        buildBvh(s.get(), &defaults);
         addChild(dev);
         *out = s.release();
     });
