@@ -29,6 +29,11 @@ def _compile(src: str) -> str:
     return obj
 
 
+PLUGIN_SRC = os.path.join(HERE, "plugin", "PiperHIP.cpp")
+PLUGIN_LIB = os.path.join(HERE, "libpiper_hip_plugin.so")
+CXX = os.environ.get("CXX", "g++")
+
+
 def build() -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -36,6 +41,11 @@ def build() -> str:
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, LIB)
+    # C++ component pair over the C ABI (host code only: plain C++17, no HIP headers)
+    tmp = PLUGIN_LIB + ".tmp"
+    subprocess.run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", PLUGIN_SRC, "-L" + HERE,
+                    "-lpiper_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], check=True)
+    os.replace(tmp, PLUGIN_LIB)
     return LIB
 
 
