@@ -620,7 +620,7 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
             timeStage(kGuide, [&] {
                 pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
                                           guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
-                                          static_cast<uint32_t>(dev->numCUs) * 2);
+                                          static_cast<uint32_t>(dev->numCUs) * 3);  // 3 blocks per CU fit (registers, LDS)
             });
         timeStage(kShade, [&] {
             pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp);

@@ -19,7 +19,7 @@ struct ArraySource {
 };
 // From the path state of an extension-queue entry (hit point, shading normal, -ray direction).
 struct PathSource {
-    DevScene s;
+    const float4* tris;  // leaf-order triangles of the scene (only the hit triangle's normal is needed)
     const PathRec* paths;
     const uint32_t* queue;
     float3 lo, invExt;
@@ -31,7 +31,9 @@ struct PathSource {
         const float4 oo = paths[path].o, dd = paths[path].d;
         const V3 dir{dd.x, dd.y, dd.z};
         const V3 p = V3{oo.x, oo.y, oo.z} + dir * h.x;
-        const V3 n = triNormal(s, pos, dir);
+        const V3 v0 = xyz(tris[3 * pos]), v1 = xyz(tris[3 * pos + 1]), v2 = xyz(tris[3 * pos + 2]);
+        const V3 ng = normalize(cross(v1 - v0, v2 - v0));
+        const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;  // as triNormal
         const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
                                        fminf(fmaxf((p.y - lo.y) * invExt.y, 0.0f), 1.0f),
                                        fminf(fmaxf((p.z - lo.z) * invExt.z, 0.0f), 1.0f));
@@ -70,6 +72,14 @@ __device__ __forceinline__ LobePair lobesFromRow(const float* row, int grp) {
 
 constexpr int kGRow = kGOut + 1;  // LDS transpose row stride (floats)
 
+// Orders one wave's LDS stores before its lanes' later LDS loads (no block barrier: other waves of
+// the block keep running).
+__device__ __forceinline__ void waveLdsSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------------------------------
 // inference: mixture records for n samples
 // ------------------------------------------------------------------------------------------
@@ -82,22 +92,22 @@ __global__ void __launch_bounds__(256) kGuideInfer(Src src, uint32_t nHost, cons
     const GuideLds L = guideLdsLayout(smem);
     float* scratch = reinterpret_cast<float*>(smem + kGLdsWeightBytes) + (threadIdx.x / 64) * 16 * kGRow;
     guideLoadLds(L, wb, theta);
-    __syncthreads();
+    __syncthreads();  // the weights are shared by the block's waves; after this each wave runs alone
     const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
-    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {  // block-uniform
-        const uint32_t i = base + (threadIdx.x / 64) * 16 + s;
+    const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t base = wave * 16; base < n; base += waves * 16) {  // wave-uniform: 16 samples per wave
+        const uint32_t i = base + s;
         const bool valid = i < n;
         const GuideInput in = valid ? src.get(i) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
         bf16x4 x0[9];
-#pragma unroll
-        for (int kt = 0; kt < 9; ++kt) x0[kt] = guideFeatures4(in, kt, grp);
+        guideFeatureTiles(in, grp, x0);
         f32x4 out[3];
         guideForwardTile(L, x0, out);
 #pragma unroll
         for (int mt = 0; mt < 3; ++mt)
 #pragma unroll
             for (int j = 0; j < 4; ++j) scratch[s * kGRow + mt * 16 + 4 * grp + j] = out[mt][j];
-        __syncthreads();
+        waveLdsSync();  // the wave's own transpose slice: written and read by its lanes only
         const LobePair lp = lobesFromRow(scratch + s * kGRow, grp);
         if (valid) {
             float* o = mix + static_cast<size_t>(i) * kGMixFloats;
@@ -107,7 +117,7 @@ __global__ void __launch_bounds__(256) kGuideInfer(Src src, uint32_t nHost, cons
                 o[32 + 2 * grp + q] = lp.pi[q];
             }
         }
-        __syncthreads();
+        waveLdsSync();  // reads done before the next iteration overwrites the slice
     }
 }
 
@@ -164,11 +174,9 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
                                          make_float3(r2.x, r2.y, r2.z));
         const float c = valid ? r0.w * invC : 0.0f;
         bf16x4 x0[9];
+        guideFeatureTiles(in, grp, x0);
 #pragma unroll
-        for (int kt = 0; kt < 9; ++kt) {
-            x0[kt] = guideFeatures4(in, kt, grp);
-            storeRows(tb.x0, tb.npad, kt * 16, grp, i, x0[kt]);
-        }
+        for (int kt = 0; kt < 9; ++kt) storeRows(tb.x0, tb.npad, kt * 16, grp, i, x0[kt]);
         bf16x4 h[4][4];
         f32x4 out[3];
         guideForwardTile(L, x0, out, h);
@@ -357,7 +365,7 @@ void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const ui
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks) {
-    PathSource src{s, ps.p, queue, lo, invExt};
+    PathSource src{s.tris, ps.p, queue, lo, invExt};
     hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, 64, maxBlocks)), dim3(256), kInferLds, st, src,
                        capacity, count, wb, theta, mix);
 }

@@ -41,10 +41,12 @@ constexpr int kGLd0 = kGIn + 4, kGLdH = kGHid + 4, kGLdOT = kGOut + 4;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t toBf16(float f) {  // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
+// Round to nearest even: gfx950's v_cvt_pk_bf16_f32 (what a __bf16 cast lowers to).
+__device__ __forceinline__ uint16_t toBf16(float f) {
+    const __bf16 b = static_cast<__bf16>(f);
+    uint16_t u;
+    __builtin_memcpy(&u, &b, 2);
+    return u;
 }
 __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
@@ -139,12 +141,48 @@ __device__ __forceinline__ float guideFeature(const GuideInput& g, int k) {
     }
     return 0.0f;
 }
-// B operand of layer 0 for K tile kt: features kt*16 + 4g + j of this lane's sample.
-__device__ __forceinline__ bf16x4 guideFeatures4(const GuideInput& g, int kt, int grp) {
-    bf16x4 r;
+// B operands of layer 0 (all 9 K tiles): features kt*16 + 4*grp + j of this lane's sample.
+// The 72 frequency features sin/cos(2^f pi p) come from one sincospi per coordinate and the
+// double-angle recurrence (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)); the
+// phase error doubles per step, about 2^11 ulp at f = 11, far below BF16 rounding. Every lane
+// forms all 72 (the same instructions across the wave) and keeps the ones its slots hold.
+__device__ __forceinline__ void guideFeatureTiles(const GuideInput& g, int grp, bf16x4 (&x0)[9]) {
+    float v[9][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) r[j] = static_cast<short>(toBf16(guideFeature(g, kt * 16 + 4 * grp + j)));
-    return r;
+    for (int kt = 0; kt < 9; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[kt][j] = 0.0f;
+    float s[3], c[3];
+    sincospif(g.px, &s[0], &c[0]);
+    sincospif(g.py, &s[1], &c[1]);
+    sincospif(g.pz, &s[2], &c[2]);
+#pragma unroll
+    for (int f = 0; f < kGFreq; ++f) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int k = 6 * f + j;
+            if (((k >> 2) & 3) == grp) v[k >> 4][k & 3] = j < 3 ? s[j] : c[j - 3];
+        }
+        if (f + 1 < kGFreq)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float ns = 2.0f * s[i] * c[i];
+                c[i] = (c[i] - s[i]) * (c[i] + s[i]);
+                s[i] = ns;
+            }
+    }
+    // normal (72..74) and the one-blob encodings (75..138) of this lane's slots
+#pragma unroll
+    for (int kt = 4; kt < 9; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kt * 16 + 4 * grp + j;
+            if (k >= 6 * kGFreq) v[kt][j] = guideFeature(g, k);
+        }
+#pragma unroll
+    for (int kt = 0; kt < 9; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x0[kt][j] = static_cast<short>(toBf16(v[kt][j]));
 }
 
 // Forward pass of one 16-sample tile. x0: layer-0 B operands (9 K tiles). out: 3 D tiles of the
