@@ -172,6 +172,7 @@ struct pg_guide {
     const void* tilesFor = nullptr;           // scratch pointer the uploaded tile table refers to
     uint32_t tilesNpad = 0;
     uint32_t step = 0;                        // Adam steps taken
+    double recordsPerPath = 3.0;              // training records a training path gave last render
 };
 
 namespace {
@@ -543,8 +544,10 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
         if (guide->p.train_records > 0) {
             gs.train = 1;
             gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(rc0.maxDepth, 255));
-            // about three guided vertices per path: pick every stride-th path for training
-            gs.trainStride = static_cast<uint32_t>(std::max<uint64_t>(1, total * 3 / guide->p.train_records));
+            // Only vertices that found light give records, so the share of paths that train
+            // follows the record yield of the previous render: every stride-th path trains.
+            const double want = static_cast<double>(total) * guide->recordsPerPath / guide->p.train_records;
+            gs.trainStride = static_cast<uint32_t>(std::min(std::max(want, 1.0), 1e9));
             gs.trainPhase = gs.trainStride / 2;
             dev->hist.reserve(static_cast<size_t>(cap) * gs.histDepth * 6 * 16);
             gs.hist = dev->hist.as<float4>();
@@ -704,6 +707,9 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
         PG_HIP(hipMemcpyAsync(&got, tr.count, 4, hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         trained = std::min(got, tr.capacity);
+        const double trainPaths = static_cast<double>((total + gs.trainStride - 1 - gs.trainPhase) / gs.trainStride);
+        if (trainPaths > 0.0)
+            guide->recordsPerPath = std::min(std::max(static_cast<double>(got) / trainPaths, 1e-6), 16.0);
         for (uint32_t b = 0; b + 64 <= trained; b += guide->p.train_batch) {
             const uint32_t cnt = std::min(guide->p.train_batch, trained - b);
             timeStage(kTrain, [&] { guideStep(guide, dev->trainRec.as<float>() + static_cast<size_t>(b) * 16, cnt, true); });

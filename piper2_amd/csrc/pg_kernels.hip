@@ -191,6 +191,14 @@ __device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
     return gs.train && (g % gs.trainStride) == gs.trainPhase;
 }
 
+// GGX lobes narrower than this (alpha = roughness^2 < 0.04) are sampled better by the BSDF than by
+// any 8-lobe mixture the network can express: such vertices draw from the BSDF alone (the guide
+// fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
+constexpr float kGuideMinAlpha = 0.04f;
+__device__ __forceinline__ float guideFraction(const GuideShade& gs, const Mat& m) {
+    return (m.type == 0u || m.alpha >= kGuideMinAlpha) ? gs.alpha : 0.0f;
+}
+
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
 // Device-scope atomics on one counter serialise across the whole chip, so the shade stage takes
 // one per block and queue (not one per wave); the staged records then leave as coalesced stores.
@@ -334,12 +342,12 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 // density alpha p_guide + (1 - alpha) p_bsdf, so the estimator stays unbiased.
                 const float uc1 = d9 ? u[6] : u[5], uc2 = d9 ? u[5] : u[6];
                 BsdfSample ss;
-                float pdfMix = 0.0f;
+                float pdfMix = 0.0f, a = 0.0f;
                 if (!GUIDED) {
                     ss = bsdfSample(mat, wo, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
-                    const float a = gs.alpha;
+                    a = guideFraction(gs, mat);
                     bool ok = true;
                     if (uc1 < a) {
                         const float3 w = mixtureSample(mix, uc1 / a, uc2);
@@ -353,7 +361,8 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     ss.pdf = 0.0f;
                     if (ok) {
                         const V3 ww = fr.toWorld(ss.wi);
-                        pdfMix = a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z)) + (1.0f - a) * bsdfPdf(mat, wo, ss.wi);
+                        pdfMix = (1.0f - a) * bsdfPdf(mat, wo, ss.wi);
+                        if (a > 0.0f) pdfMix += a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z));
                         ss.f = bsdfEval(mat, wo, ss.wi, ng);
                         ss.pdf = pdfMix;
                     }
@@ -379,7 +388,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         meta.z &= ~1u;  // no longer a camera / specular ray
                         pushExt = true;
                         const uint32_t r = (meta.z >> 8) & 0xFFu;
-                        if (GUIDED && isTrainPath(gs, meta.w) && r < gs.histDepth) {
+                        if (GUIDED && a > 0.0f && isTrainPath(gs, meta.w) && r < gs.histDepth) {
                             float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
                             const float px = fminf(fmaxf((hp.x - gs.lo.x) * gs.invExt.x, 0.0f), 1.0f);
                             const float py = fminf(fmaxf((hp.y - gs.lo.y) * gs.invExt.y, 0.0f), 1.0f);
@@ -576,6 +585,8 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
             if (!BATCH && isTrainPath(gs, g)) {
                 // Each recorded vertex learns from the radiance that reached it along its sampled
                 // direction: (L_final - L after its direct light) / throughput after the bounce.
+                // The loss weights every record by that radiance over its pdf, so records that
+                // found no light carry no gradient and are not written.
                 const uint32_t cnt = (meta.z >> 8) & 0xFFu;
                 for (uint32_t r = 0; r < cnt; ++r) {
                     const float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
@@ -584,6 +595,7 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
                     const float lg = t.y > 0.0f ? (L.y - snap.y) / t.y : 0.0f;
                     const float lb = t.z > 0.0f ? (L.z - snap.z) / t.z : 0.0f;
                     const float c = fmaxf((lr + lg + lb) * (1.0f / 3.0f), 0.0f) / h0.w;
+                    if (!(c > 0.0f)) continue;
                     const uint32_t k = atomicAdd(tr.count, 1u);
                     if (k >= tr.capacity) break;
                     tr.rec[4 * k] = make_float4(h0.x, h0.y, h0.z, c);

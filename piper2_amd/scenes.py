@@ -283,12 +283,17 @@ def _grid_quad(origin, u, v, n):
 
 
 def hard_room(seed=0, width=512, height=512, spp=64, max_depth=12, tess=204):
-    """Hard indirect (guided config 5): a closed room (walls tessellated to ~500k triangles, Lambert
-    U[0.3, 0.7]) with one 0.05 x 0.05 light of radiance 500 inside an open-top lamp housing; the
-    light reaches the room only through a GGX conductor reflector (roughness 0.1) above it."""
+    """Hard indirect (guided config 5): a closed 2 x 1.6 x 2 room (walls tessellated to ~500k
+    triangles, Lambert U[0.3, 0.7]) with one 0.05 x 0.05 light of radiance 500 inside a lamp
+    housing. No straight line joins the light and the housing's opening: light leaves only after a
+    GGX conductor reflector (roughness 0.1, alpha 0.01) or the housing's inner walls. The opening
+    faces the back wall 0.35-0.65 away, so the reflected beam lights a patch there and the rest of
+    the room sees the light only through that patch. The camera stands at the front wall, behind
+    the closed back of the housing. The room has the Cornell box's scale, so the 7.9 W light
+    leaves pixel values of 0.01-0.5 rather than values lost under relMSE's 0.01 floor."""
     rng = np.random.default_rng(100 + seed)
     b = _Builder()
-    W, H, D = 5.0, 4.0, 5.0
+    W, H, D = 1.0, 1.6, 1.0
     walls = [
         ([-W, 0, -D], [2 * W, 0, 0], [0, 0, 2 * D]),    # floor
         ([-W, H, -D], [0, 0, 2 * D], [2 * W, 0, 0]),    # ceiling
@@ -300,30 +305,39 @@ def hard_room(seed=0, width=512, height=512, spp=64, max_depth=12, tess=204):
     for o, u, v in walls:
         verts, tris = _grid_quad(o, u, v, tess)
         b.add(verts, tris, b.material(LAMBERT, rng.uniform(0.3, 0.7, 3)))
-    # Fixture (0.6 x 0.4 x 0.6 box at a seeded spot, opening toward +z): the light lies on the floor
-    # of the back half; a shelf over the front half and the closed lower front wall hide it from
-    # every straight line through the opening, so it reaches the room only after the 45-degree GGX
-    # reflector in the back-top corner (or bounces inside the housing).
-    cx, cz = rng.uniform(-2.5, 2.5), rng.uniform(-3.5, 0.0)
-    y0 = rng.uniform(1.2, 2.2)
+    # Housing: 0.3 x 0.2 x 0.3 box at a seeded spot; local depth q runs from its closed back
+    # (q = -0.15, toward the camera) to its front (q = +0.15, toward the back wall), z = cz - q.
+    # The light lies on the floor of the rear half; a shelf over the front half and the closed
+    # lower front hide it from every straight line through the opening (the upper front), so it
+    # reaches the room only after the 45-degree reflector in the rear top corner or the walls.
+    cx, cz = rng.uniform(-0.5, 0.5), rng.uniform(-0.5, -0.2)
+    y0 = rng.uniform(0.7, 1.1)
+    x0, x1 = cx - 0.15, cx + 0.15
+
+    def at(x, y, q):
+        return [x, y, cz - q]
+
+    def along(dx, dy, dq):
+        return [dx, dy, -dq]
+
     housing = b.material(LAMBERT, [0.5, 0.5, 0.5])
-    x0, x1, z0, z1 = cx - 0.3, cx + 0.3, cz - 0.3, cz + 0.3
     panels = [
-        ([x0, y0, z0], [0.6, 0, 0], [0, 0, 0.6]),          # bottom
-        ([x0, y0 + 0.4, z0], [0, 0, 0.6], [0.6, 0, 0]),    # top
-        ([x0, y0, z0], [0, 0.4, 0], [0.6, 0, 0]),          # back
-        ([x0, y0, z0], [0, 0, 0.6], [0, 0.4, 0]),          # left
-        ([x1, y0, z0], [0, 0.4, 0], [0, 0, 0.6]),          # right
-        ([x0, y0, z1], [0.6, 0, 0], [0, 0.2, 0]),          # lower half of the front
-        ([x0, y0 + 0.2, cz], [0.6, 0, 0], [0, 0, 0.3]),    # shelf over the front half
+        (at(x0, y0, -0.15), along(0.3, 0, 0), along(0, 0, 0.3)),            # bottom
+        (at(x0, y0 + 0.2, -0.15), along(0, 0, 0.3), along(0.3, 0, 0)),      # top
+        (at(x0, y0, -0.15), along(0, 0.2, 0), along(0.3, 0, 0)),            # back
+        (at(x0, y0, -0.15), along(0, 0, 0.3), along(0, 0.2, 0)),            # left
+        (at(x1, y0, -0.15), along(0, 0.2, 0), along(0, 0, 0.3)),            # right
+        (at(x0, y0, 0.15), along(0.3, 0, 0), along(0, 0.1, 0)),             # lower half of the front
+        (at(x0, y0 + 0.1, 0.0), along(0.3, 0, 0), along(0, 0, 0.15)),       # shelf over the front half
     ]
     for o, u, v in panels:
         verts, tris = _grid_quad(o, u, v, 1)
         b.add(verts, tris, housing)
-    verts, tris = _grid_quad([x0, y0 + 0.15, z0], [0.6, 0, 0], [0, 0.25, 0.25], 1)
+    verts, tris = _grid_quad(at(x0, y0 + 0.075, -0.15), along(0.3, 0, 0), along(0, 0.125, 0.125), 1)
     b.add(verts, tris, b.material(CONDUCTOR, [0.95, 0.95, 0.95], 0.1 * 0.1))
-    quads = [[cx - 0.025, y0 + 0.02, cz - 0.245, 0.05, 0.0, 0.0, 0.0, 0.0, 0.05, 500.0, 500.0, 500.0]]
-    cam = make_camera([0.0, 2.0, 4.6], [0.0, 1.5, 0.0], [0, 1, 0], 60.0, width / height)
+    lo = at(cx - 0.025, y0 + 0.01, -0.125)
+    quads = [lo + [0.05, 0.0, 0.0] + along(0, 0, 0.05) + [500.0, 500.0, 500.0]]
+    cam = make_camera([0.0, 0.85, 0.95], [0.0, 0.75, -1.0], [0, 1, 0], 60.0, width / height)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=3, ndc=ndc,
                     name=f"hard_room_{seed}")
