@@ -5,8 +5,10 @@ kernels implement, so that their forward pass, loss and gradients can be checked
 
   inputs  : position p (scene box mapped to [0,1]^3) -> 12-frequency sin/cos encoding (72),
             shading normal n (3), outgoing direction wo as (theta, phi) one-blob, 32 bins each (64);
-            139 features, zero-padded to 144.
-  network : 144 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 40 (linear), padded to 48 rows.
+            139 features and 5 zeros, 144 in all, in the order encode() gives (each lane group of
+            the kernels computes exactly the features it feeds to the matrix cores).
+  network : 144 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 40 (linear) in 48 rows (output_row() gives
+            where each lobe parameter sits; PAD_OUTPUTS are zero rows).
   output  : 8 von Mises-Fisher lobes, each (mu raw 3, log kappa 1, weight logit 1):
             mu = m / |m|, kappa = exp(clamp(r, KLO, KHI)), pi = softmax(logits).
   pdf     : p(w) = sum_i pi_i * kappa_i / (2 pi (1 - exp(-2 kappa_i))) * exp(kappa_i (mu_i . w - 1)).
@@ -52,26 +54,55 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+def feature_index(g, t):
+    """Input feature fed by slot t (0..35) of lane group g (0..3): the MFMA B-operand position
+    16 kt + 4 g + j with t = 4 kt + j."""
+    return (t // 4) * 16 + 4 * g + t % 4
+
+
+# Input features that are always zero (their first-layer weights start at zero).
+PAD_FEATURES = sorted([feature_index(1, 35), feature_index(2, 34), feature_index(2, 35),
+                       feature_index(3, 34), feature_index(3, 35)])
+
+
+def output_row(l, k):
+    """Output row of lobe l (0..7), parameter k (0..4: mu x, mu y, mu z, log kappa, weight logit):
+    lane group g = l // 2 holds rows 16 mt + 4 g + j, and its slot u = 4 mt + j = 5 (l % 2) + k."""
+    g, u = l // 2, 5 * (l % 2) + k
+    return 16 * (u // 4) + 4 * g + u % 4
+
+
+LOBE_ROWS = np.array([[output_row(l, k) for k in range(5)] for l in range(NLOBES)])  # (8, 5)
+# Output rows that are always zero (slots 10 and 11 of every lane group).
+PAD_OUTPUTS = sorted(set(range(NOUT_PAD)) - set(LOBE_ROWS.reshape(-1).tolist()))
+
+
 def encode(pos01, normal, wo):
-    """Feature vector (N, 144) for positions in [0,1]^3, unit normals and unit directions."""
+    """Feature vector (N, 144) for positions in [0,1]^3, unit normals and unit directions.
+
+    Slot t of lane group g (feature feature_index(g, t)) holds
+      t = 0..17 : position item P = 9 g + t // 2 (coordinate P // 12, frequency f = P % 12):
+                  sin(2^f pi p) for even t, cos(2^f pi p) for odd t;
+      t = 18..33: one-blob bin b = 16 (g % 2) + t - 18 of theta = atan2(|wo.xy|, wo.z) / pi
+                  (g < 2) or phi = atan2(wo.y, wo.x) / 2 pi mod 1 (g >= 2): exp(-(32 a - b - 0.5)^2 / 2);
+      t = 34, 35: n.x, n.y (g = 0); n.z, 0 (g = 1); 0, 0 (g = 2, 3).
+    """
     n = pos01.shape[0]
     feats = np.zeros((n, NIN))
-    col = 0
-    for f in range(NFREQ):
-        s = (2.0 ** f) * np.pi * pos01
-        feats[:, col:col + 3] = np.sin(s)
-        feats[:, col + 3:col + 6] = np.cos(s)
-        col += 6
-    feats[:, col:col + 3] = normal
-    col += 3
-    theta = np.arccos(np.clip(wo[:, 2], -1.0, 1.0)) / np.pi
+    theta = np.arctan2(np.hypot(wo[:, 0], wo[:, 1]), wo[:, 2]) / np.pi
     phi = (np.arctan2(wo[:, 1], wo[:, 0]) / (2 * np.pi)) % 1.0
-    centers = (np.arange(NBLOB) + 0.5) / NBLOB
-    for v in (theta, phi):
-        d = v[:, None] - centers[None, :]
-        feats[:, col:col + NBLOB] = np.exp(-0.5 * (d * NBLOB) ** 2)
-        col += NBLOB
-    assert col == 139
+    for g in range(4):
+        for t in range(18):
+            P = 9 * g + t // 2
+            x = (2.0 ** (P % 12)) * np.pi * pos01[:, P // 12]
+            feats[:, feature_index(g, t)] = np.sin(x) if t % 2 == 0 else np.cos(x)
+        a = theta if g < 2 else phi
+        for t in range(18, 34):
+            b = 16 * (g % 2) + t - 18
+            feats[:, feature_index(g, t)] = np.exp(-0.5 * (a * NBLOB - b - 0.5) ** 2)
+    feats[:, feature_index(0, 34)] = normal[:, 0]
+    feats[:, feature_index(0, 35)] = normal[:, 1]
+    feats[:, feature_index(1, 34)] = normal[:, 2]
     return feats
 
 
@@ -92,7 +123,7 @@ def forward(theta, feats, quant=True):
 
 def mixture(out):
     """Lobe parameters from raw outputs: mu (N,8,3), kappa (N,8), pi (N,8), and intermediates."""
-    o = out[:, :NOUT].reshape(-1, NLOBES, 5)
+    o = out[:, LOBE_ROWS]                                # (N, 8, 5)
     m = o[:, :, 0:3]
     mnorm = np.maximum(np.linalg.norm(m, axis=2), 1e-6)
     mu = m / mnorm[:, :, None]
@@ -143,11 +174,9 @@ def loss_and_grad(theta, feats, w, c, quant=True):
     dmu = (dv * v * kappa)[:, :, None] * w[:, None, :]
     dm = (dmu - mu * np.einsum("nlk,nlk->nl", mu, dmu)[:, :, None]) / mnorm[:, :, None]
     dout = np.zeros_like(out)
-    o = dout[:, :NOUT].reshape(-1, NLOBES, 5)
-    o[:, :, 0:3] = dm
-    o[:, :, 3] = dr
-    o[:, :, 4] = da
-    dout[:, :NOUT] = o.reshape(-1, NOUT)
+    dout[:, LOBE_ROWS[:, 0:3]] = dm
+    dout[:, LOBE_ROWS[:, 3]] = dr
+    dout[:, LOBE_ROWS[:, 4]] = da
     grads = []
     dz = dout
     for li in range(len(layers) - 1, -1, -1):
@@ -159,11 +188,11 @@ def loss_and_grad(theta, feats, w, c, quant=True):
         dz = dz @ q(W)
     grads.reverse()
     flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
-    # padding rows of the output layer (40..47) are never trained
+    # the zero rows of the output layer are never trained
     k = sum(o * i + o for o, i in LAYERS[:-1])
     Wlast = flat[k:k + NOUT_PAD * NHID].reshape(NOUT_PAD, NHID)
-    Wlast[NOUT:] = 0.0
-    flat[k + NOUT_PAD * NHID + NOUT:] = 0.0
+    Wlast[PAD_OUTPUTS] = 0.0
+    flat[k + NOUT_PAD * NHID + np.array(PAD_OUTPUTS)] = 0.0
     return L, flat
 
 
@@ -175,9 +204,9 @@ def init_params(seed=0):
         lim = np.sqrt(6.0 / (i + o))
         W = rng.uniform(-lim, lim, (o, i))
         if li == 0:
-            W[:, 139:] = 0.0
+            W[:, PAD_FEATURES] = 0.0
         if li == len(LAYERS) - 1:
-            W[NOUT:] = 0.0
+            W[PAD_OUTPUTS] = 0.0
         parts += [W.reshape(-1), np.zeros(o)]
     return np.concatenate(parts)
 

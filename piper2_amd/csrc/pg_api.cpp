@@ -197,7 +197,8 @@ void releaseChild(pg_device* dev) {
 }
 
 // Guide initialisation: uniform(+-sqrt(6 / (in + out))) weights from a splitmix64 stream, zero
-// biases, zero padding (input features 139..143, output rows 40..47).
+// biases, zero padding (the always-zero input features 135, 138, 139, 142, 143 and output rows
+// 34, 35, 38, 39, 42, 43, 46, 47).
 std::vector<float> guideInitParams(uint32_t seed) {
     std::vector<float> t(pg::kGParamCount, 0.0f);
     uint64_t state = 0x9E3779B97F4A7C15ull * (seed + 1);
@@ -215,7 +216,9 @@ std::vector<float> guideInitParams(uint32_t seed) {
         for (int o = 0; o < outs[l]; ++o)
             for (int i = 0; i < ins[l]; ++i) {
                 const double u = (2.0 * next() - 1.0) * lim;
-                const bool pad = (l == 0 && i >= 139) || (l == 4 && o >= 40);
+                const bool padIn = i == 135 || i == 138 || i == 139 || i == 142 || i == 143;
+                const bool padOut = o >= 32 && (o & 3) >= 2;
+                const bool pad = (l == 0 && padIn) || (l == 4 && padOut);
                 t[k++] = pad ? 0.0f : static_cast<float>(u);
             }
         k += outs[l];  // biases stay zero
@@ -623,7 +626,7 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
             timeStage(kGuide, [&] {
                 pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
                                           guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
-                                          static_cast<uint32_t>(dev->numCUs) * 3);  // 3 blocks per CU fit (registers, LDS)
+                                          static_cast<uint32_t>(dev->numCUs) * 2);  // 2 blocks of 8 waves per CU fit (LDS)
             });
         timeStage(kShade, [&] {
             pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp);

@@ -41,24 +41,27 @@ struct PathSource {
     }
 };
 
-// Per-lane mixture of lobes 2g, 2g+1 of sample s from the output row o[0..47] in LDS.
+// Mixture of lobes 2g, 2g+1 of this lane's sample, decoded from the lane's own output accumulators
+// (output slot u = 4 mt + j is row 16 mt + 4 g + j; slots 5q..5q+4 are lobe 2g + q's mu xyz, log
+// kappa and weight logit). Only the softmax over the 8 weights crosses lanes (s, s+16, s+32, s+48).
 struct LobePair {
     float mx[2], my[2], mz[2], mn[2], r[2], kappa[2], pi[2];
 };
-__device__ __forceinline__ LobePair lobesFromRow(const float* row, int grp) {
+__device__ __forceinline__ float outSlot(const f32x4 (&out)[3], int u) { return out[u >> 2][u & 3]; }
+__device__ __forceinline__ LobePair lobesFromTile(const f32x4 (&out)[3]) {
     LobePair L;
     float a[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const float* o = row + 10 * grp + 5 * q;
-        const float n = fmaxf(sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]), 1e-6f);
+        const float ox = outSlot(out, 5 * q), oy = outSlot(out, 5 * q + 1), oz = outSlot(out, 5 * q + 2);
+        const float n = fmaxf(sqrtf(ox * ox + oy * oy + oz * oz), 1e-6f);
+        const float inv = __builtin_amdgcn_rcpf(n);  // 1 ulp: the stored lobe is what sampling and pdf use
         L.mn[q] = n;
-        L.mx[q] = o[0] / n, L.my[q] = o[1] / n, L.mz[q] = o[2] / n;
-        L.r[q] = o[3];
-        L.kappa[q] = __expf(fminf(fmaxf(o[3], kGKappaLo), kGKappaHi));
-        a[q] = o[4];
+        L.mx[q] = ox * inv, L.my[q] = oy * inv, L.mz[q] = oz * inv;
+        L.r[q] = outSlot(out, 5 * q + 3);
+        L.kappa[q] = __expf(fminf(fmaxf(L.r[q], kGKappaLo), kGKappaHi));
+        a[q] = outSlot(out, 5 * q + 4);
     }
-    // softmax over the 8 lobes held by the 4 lanes of this sample (lanes s, s+16, s+32, s+48)
     float m = fmaxf(a[0], a[1]);
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
@@ -66,11 +69,10 @@ __device__ __forceinline__ LobePair lobesFromRow(const float* row, int grp) {
     float sum = e0 + e1;
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
-    L.pi[0] = e0 / sum, L.pi[1] = e1 / sum;
+    const float rs = __builtin_amdgcn_rcpf(sum);
+    L.pi[0] = e0 * rs, L.pi[1] = e1 * rs;
     return L;
 }
-
-constexpr int kGRow = kGOut + 1;  // LDS transpose row stride (floats)
 
 // Orders one wave's LDS stores before its lanes' later LDS loads (no block barrier: other waves of
 // the block keep running).
@@ -83,41 +85,51 @@ __device__ __forceinline__ void waveLdsSync() {
 // ------------------------------------------------------------------------------------------
 // inference: mixture records for n samples
 // ------------------------------------------------------------------------------------------
+// Each wave takes 64 samples at a time: every lane gathers one sample's inputs (one dependent load
+// chain per 64 samples) into the wave's LDS stage, then the wave runs four 16-sample MFMA tiles.
+constexpr int kInferStageBytes = 64 * 3 * 16;  // per wave: 64 samples x 3 float4
+// 8 waves share one LDS copy of the weights: 37 KB + 8 x 3 KB, two blocks per CU (4 waves per SIMD)
+constexpr int kInferBlock = 512;
+
 template <typename Src>
-__global__ void __launch_bounds__(256) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
+__global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
                                                    const uint16_t* __restrict__ wb, const float* __restrict__ theta,
                                                    float* __restrict__ mix) {
     extern __shared__ unsigned char smem[];
     const uint32_t n = nDev ? *nDev : nHost;
     const GuideLds L = guideLdsLayout(smem);
-    float* scratch = reinterpret_cast<float*>(smem + kGLdsWeightBytes) + (threadIdx.x / 64) * 16 * kGRow;
+    float4* stage = reinterpret_cast<float4*>(smem + kGLdsWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
     guideLoadLds(L, wb, theta);
     __syncthreads();  // the weights are shared by the block's waves; after this each wave runs alone
     const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, waves = gridDim.x * (blockDim.x / 64);
-    for (uint32_t base = wave * 16; base < n; base += waves * 16) {  // wave-uniform: 16 samples per wave
-        const uint32_t i = base + s;
-        const bool valid = i < n;
-        const GuideInput in = valid ? src.get(i) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
-        bf16x4 x0[9];
-        guideFeatureTiles(in, grp, x0);
-        f32x4 out[3];
-        guideForwardTile(L, x0, out);
-#pragma unroll
-        for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) scratch[s * kGRow + mt * 16 + 4 * grp + j] = out[mt][j];
-        waveLdsSync();  // the wave's own transpose slice: written and read by its lanes only
-        const LobePair lp = lobesFromRow(scratch + s * kGRow, grp);
-        if (valid) {
-            float* o = mix + static_cast<size_t>(i) * kGMixFloats;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                reinterpret_cast<float4*>(o)[2 * grp + q] = make_float4(lp.mx[q], lp.my[q], lp.mz[q], lp.kappa[q]);
-                o[32 + 2 * grp + q] = lp.pi[q];
+    for (uint32_t base = wave * 64; base < n; base += waves * 64) {  // wave-uniform
+        {
+            const uint32_t i = base + lane;
+            const GuideInput in = i < n ? src.get(i) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
+            stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
+            stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
+            stage[3 * lane + 2] = make_float4(in.wz, 0.0f, 0.0f, 0.0f);
+        }
+        waveLdsSync();  // the wave's own stage: written and read by its lanes only
+#pragma unroll 1
+        for (uint32_t t = 0; t < 4 && base + 16 * t < n; ++t) {
+            const uint32_t k = 16 * t + s, i = base + k;
+            const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
+            const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
+            bf16x4 x0[9];
+            guideFeatureTiles(in, grp, x0);
+            f32x4 out[3];
+            guideForwardTile(L, x0, out);
+            const LobePair lp = lobesFromTile(out);
+            if (i < n) {
+                float* o = mix + static_cast<size_t>(i) * kGMixFloats;
+                reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
+                reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
+                reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
             }
         }
-        waveLdsSync();  // reads done before the next iteration overwrites the slice
+        waveLdsSync();  // stage reads done before the next samples overwrite it
     }
 }
 
@@ -138,7 +150,7 @@ __device__ __forceinline__ void storeRows(uint16_t* dst, uint32_t npad, int row0
     for (int j = 0; j < 4; ++j) dst[static_cast<size_t>(row0 + 4 * grp + j) * npad + col] = static_cast<uint16_t>(v[j]);
 }
 
-// LDS: forward weights, then transposed W1..W4 ([in = 64][out + 4]), then per-wave scratch.
+// LDS: forward weights, then transposed W1..W4 ([in = 64][out + 4]).
 constexpr int kGLdsTransBytes = (3 * kGHid * kGLdH + kGHid * kGLdOT) * 2;
 
 __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ rec, uint32_t n,
@@ -147,7 +159,6 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
     extern __shared__ unsigned char smem[];
     const GuideLds L = guideLdsLayout(smem);
     uint16_t* wt = reinterpret_cast<uint16_t*>(smem + kGLdsWeightBytes);  // W1T, W2T, W3T [64][kGLdH], W4T [64][kGLdOT]
-    float* scratch = reinterpret_cast<float*>(smem + kGLdsWeightBytes + kGLdsTransBytes) + (threadIdx.x / 64) * 16 * kGRow;
     guideLoadLds(L, wb, theta);
     for (int l = 0; l < 3; ++l) {
         const int off = l == 0 ? kGW1 : (l == 1 ? kGW2 : kGW3);
@@ -184,13 +195,8 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
         for (int l = 0; l < 4; ++l)
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) storeRows(tb.xh + static_cast<size_t>(l) * kGHid * tb.npad, tb.npad, mt * 16, grp, i, h[l][mt]);
-#pragma unroll
-        for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) scratch[s * kGRow + mt * 16 + 4 * grp + j] = out[mt][j];
-        __syncthreads();
-        // mixture, loss and gradient with respect to the 40 outputs (lobes 2g, 2g+1 in this lane)
-        const LobePair lp = lobesFromRow(scratch + s * kGRow, grp);
+        // mixture, loss and gradient with respect to this lane's output slots (lobes 2g, 2g+1)
+        const LobePair lp = lobesFromTile(out);
         const float wx = r3.x, wy = r3.y, wz = r3.z;
         float v[2], cs[2];
 #pragma unroll
@@ -208,8 +214,8 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
         float gpi = lp.pi[0] * dLdp * v[0] + lp.pi[1] * dLdp * v[1];
         gpi += __shfl_xor(gpi, 16);
         gpi += __shfl_xor(gpi, 32);
-        __syncthreads();
-        float* row = scratch + s * kGRow;
+        float gs[12];  // dL/d(output slot u); slots 10, 11 are zero rows
+        gs[10] = gs[11] = 0.0f;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const float dv = dLdp * lp.pi[q];
@@ -220,23 +226,18 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
             const float sc = dv * v[q] * k;  // dL/dmu = sc * w
             const float dmx = sc * wx, dmy = sc * wy, dmz = sc * wz;
             const float proj = lp.mx[q] * dmx + lp.my[q] * dmy + lp.mz[q] * dmz;
-            float* o = row + 10 * grp + 5 * q;
+            float* o = gs + 5 * q;
             o[0] = (dmx - lp.mx[q] * proj) / lp.mn[q];
             o[1] = (dmy - lp.my[q] * proj) / lp.mn[q];
             o[2] = (dmz - lp.mz[q] * proj) / lp.mn[q];
             o[3] = dr;
             o[4] = lp.pi[q] * (dLdp * v[q] - gpi);
         }
-        if (grp == 0)
-            for (int k = kGOutUsed; k < kGOut; ++k) row[k] = 0.0f;
-        __syncthreads();
         // backward data path: dZ4 -> dZ3 ... dZ0
         bf16x4 dz[4];
 #pragma unroll
         for (int mt = 0; mt < 3; ++mt) {
-            f32x4 t;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) t[j] = scratch[s * kGRow + mt * 16 + 4 * grp + j];
+            const f32x4 t{gs[4 * mt], gs[4 * mt + 1], gs[4 * mt + 2], gs[4 * mt + 3]};
             dz[mt] = pack(t);
             storeRows(tb.dz + 4ull * kGHid * tb.npad, tb.npad, mt * 16, grp, i, dz[mt]);
         }
@@ -348,8 +349,8 @@ __global__ void __launch_bounds__(256) kGuideSumC(const float4* rec, uint32_t n,
 // ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
-constexpr size_t kInferLds = kGLdsWeightBytes + 4 * 16 * kGRow * 4;
-constexpr size_t kTrainLds = kGLdsWeightBytes + kGLdsTransBytes + 4 * 16 * kGRow * 4;
+constexpr size_t kInferLds = kGLdsWeightBytes + (kInferBlock / 64) * kInferStageBytes;
+constexpr size_t kTrainLds = kGLdsWeightBytes + kGLdsTransBytes;
 
 static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {
     const uint32_t b = (n + per - 1) / per;
@@ -358,7 +359,7 @@ static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {
 
 void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const uint16_t* wb, const float* theta,
                            float* mix, uint32_t maxBlocks) {
-    hipLaunchKernelGGL(kGuideInfer<ArraySource>, dim3(gridFor(n, 64, maxBlocks)), dim3(256), kInferLds, st,
+    hipLaunchKernelGGL(kGuideInfer<ArraySource>, dim3(gridFor(n, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds, st,
                        ArraySource{reinterpret_cast<const float4*>(in)}, n, nullptr, wb, theta, mix);
 }
 
@@ -366,7 +367,7 @@ void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& p
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks) {
     PathSource src{s.tris, ps.p, queue, lo, invExt};
-    hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, 64, maxBlocks)), dim3(256), kInferLds, st, src,
+    hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds, st, src,
                        capacity, count, wb, theta, mix);
 }
 

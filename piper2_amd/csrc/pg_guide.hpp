@@ -1,9 +1,10 @@
 // Neural path guiding on gfx950 matrix cores: network layout, feature encoding, BF16 MFMA layers
 // and the 8-lobe von Mises-Fisher mixture used as a one-sample-MIS proposal next to the BSDF.
 //
-// Network: 144 input features -> 4 hidden layers of 64 (ReLU) -> 40 outputs (48 rows, the last 8
-// padding). Weights are row-major [out][in] FP32 masters with a BF16 copy; matrix products run on
-// v_mfma_f32_16x16x16_bf16 with FP32 accumulation.
+// Network: 144 input features -> 4 hidden layers of 64 (ReLU) -> 40 outputs in 48 rows (8 zero
+// rows). Weights are row-major [out][in] FP32 masters with a BF16 copy; matrix products run on
+// v_mfma_f32_16x16x16_bf16 with FP32 accumulation. Inputs and outputs are ordered so that each lane
+// computes the features it feeds and decodes the lobes it holds (guideFeatureTiles, output layout).
 //
 // MFMA operand layout used throughout (lane l, g = l / 16, s = l % 16, j = 0..3):
 //   A (16 x 16, rows m, cols k): lane holds A[s][4g + j]
@@ -21,8 +22,11 @@
 
 namespace pg {
 
-constexpr int kGIn = 144, kGHid = 64, kGOut = 48, kGOutUsed = 40, kGLobes = 8;
-constexpr int kGFreq = 12, kGBlob = 32, kGFeat = 139;
+constexpr int kGIn = 144, kGHid = 64, kGOut = 48, kGLobes = 8;
+// Output layout: lane group g holds rows 16 mt + 4 g + j (mt = 0..2, j = 0..3) of its sample; slot
+// u = 4 mt + j is lobe 2 g + u / 5, parameter u % 5 (mu x, mu y, mu z, log kappa, weight logit) for
+// u < 10. Slots 10 and 11 are the zero rows 34, 35, 38, 39, 42, 43, 46, 47.
+constexpr int kGFreq = 12, kGBlob = 32;
 // FP32 parameter offsets (floats), layer l: W then b
 constexpr int kGW0 = 0, kGB0 = kGW0 + kGHid * kGIn;
 constexpr int kGW1 = kGB0 + kGHid, kGB1 = kGW1 + kGHid * kGHid;
@@ -89,16 +93,30 @@ __device__ __forceinline__ GuideLds guideLdsLayout(unsigned char* base) {
     return L;
 }
 
+// Rows of BF16 weights from global (unpadded, 16-byte aligned) into padded LDS rows: 8 weights per
+// 16-byte load; the padded rows are 8-byte aligned, so each load becomes two 8-byte stores.
+template <int ROWS, int COLS, int LD>
+__device__ __forceinline__ void guideCopyRows(uint16_t* dst, const uint16_t* __restrict__ src) {
+    static_assert(COLS % 8 == 0 && LD % 4 == 0, "vector copy");
+    constexpr int kVec = COLS / 8;
+    for (int i = threadIdx.x; i < ROWS * kVec; i += blockDim.x) {
+        const int r = i / kVec, q = i - r * kVec;
+        const uint4 w = reinterpret_cast<const uint4*>(src + r * COLS)[q];
+        uint2* d = reinterpret_cast<uint2*>(dst + r * LD + 8 * q);
+        d[0] = make_uint2(w.x, w.y);
+        d[1] = make_uint2(w.z, w.w);
+    }
+}
+static_assert(kGW1 % 8 == 0 && kGW2 % 8 == 0 && kGW3 % 8 == 0 && kGW4 % 8 == 0, "16-byte aligned weight rows");
+
 // Cooperative copy of the BF16 weights (global, unpadded [out][in]) and FP32 biases into LDS.
 __device__ __forceinline__ void guideLoadLds(const GuideLds& L, const uint16_t* __restrict__ wb,
                                              const float* __restrict__ theta) {
-    for (int i = threadIdx.x; i < kGHid * kGIn; i += blockDim.x) L.w0[(i / kGIn) * kGLd0 + i % kGIn] = wb[kGW0 + i];
-    for (int l = 0; l < 3; ++l) {
-        const int off = l == 0 ? kGW1 : (l == 1 ? kGW2 : kGW3);
-        for (int i = threadIdx.x; i < kGHid * kGHid; i += blockDim.x)
-            L.wh[l * kGHid * kGLdH + (i / kGHid) * kGLdH + i % kGHid] = wb[off + i];
-    }
-    for (int i = threadIdx.x; i < kGOut * kGHid; i += blockDim.x) L.w4[(i / kGHid) * kGLdH + i % kGHid] = wb[kGW4 + i];
+    guideCopyRows<kGHid, kGIn, kGLd0>(L.w0, wb + kGW0);
+    guideCopyRows<kGHid, kGHid, kGLdH>(L.wh, wb + kGW1);
+    guideCopyRows<kGHid, kGHid, kGLdH>(L.wh + kGHid * kGLdH, wb + kGW2);
+    guideCopyRows<kGHid, kGHid, kGLdH>(L.wh + 2 * kGHid * kGLdH, wb + kGW3);
+    guideCopyRows<kGOut, kGHid, kGLdH>(L.w4, wb + kGW4);
     for (int i = threadIdx.x; i < kGHid; i += blockDim.x) {
         L.b[i] = theta[kGB0 + i];
         L.b[kGHid + i] = theta[kGB1 + i];
@@ -112,77 +130,60 @@ __device__ __forceinline__ bf16x4 ldsA(const uint16_t* w, int ld, int row, int c
     return *reinterpret_cast<const bf16x4*>(w + row * ld + col);
 }
 
-// Feature k of a sample (positions in [0,1]^3, unit normal, unit outgoing direction).
+// Inputs of one sample: position in [0,1]^3, unit shading normal, unit outgoing direction.
 struct GuideInput {
-    float px, py, pz, nx, ny, nz, theta01, phi01;
+    float px, py, pz, nx, ny, nz, wx, wy, wz;
 };
 __device__ __forceinline__ GuideInput guideInput(float3 p01, float3 n, float3 wo) {
-    GuideInput g;
-    g.px = p01.x, g.py = p01.y, g.pz = p01.z;
-    g.nx = n.x, g.ny = n.y, g.nz = n.z;
-    g.theta01 = acosf(fminf(1.0f, fmaxf(-1.0f, wo.z))) * 0.31830988618379067154f;
-    float phi = atan2f(wo.y, wo.x) * 0.15915494309189533577f;
-    g.phi01 = phi < 0.0f ? phi + 1.0f : phi;
-    return g;
+    return GuideInput{p01.x, p01.y, p01.z, n.x, n.y, n.z, wo.x, wo.y, wo.z};
 }
-__device__ __forceinline__ float guideFeature(const GuideInput& g, int k) {
-    if (k < 6 * kGFreq) {
-        const int f = k / 6, j = k % 6;
-        const float p = (j % 3) == 0 ? g.px : ((j % 3) == 1 ? g.py : g.pz);
-        const float x = ldexpf(p, f);  // 2^f p, exact
-        return j < 3 ? sinpif(x) : cospif(x);
-    }
-    if (k < 75) return k == 72 ? g.nx : (k == 73 ? g.ny : g.nz);
-    if (k < kGFeat) {
-        const bool th = k < 75 + kGBlob;
-        const int bin = th ? k - 75 : k - 75 - kGBlob;
-        const float d = ((th ? g.theta01 : g.phi01) - (bin + 0.5f) * (1.0f / kGBlob)) * kGBlob;
-        return __expf(-0.5f * d * d);
-    }
-    return 0.0f;
-}
-// B operands of layer 0 (all 9 K tiles): features kt*16 + 4*grp + j of this lane's sample.
-// The 72 frequency features sin/cos(2^f pi p) come from one sincospi per coordinate and the
-// double-angle recurrence (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)); the
-// phase error doubles per step, about 2^11 ulp at f = 11, far below BF16 rounding. Every lane
-// forms all 72 (the same instructions across the wave) and keeps the ones its slots hold.
+
+// Layer-0 input layout. Lane group g (lanes 16g..16g+15, one sample each) feeds the B-operand
+// slots k = 16 kt + 4 g + j (kt = 0..8, j = 0..3); slot t = 4 kt + j of group g holds
+//   t = 0..17  : position item P = 9 g + t / 2, coordinate P / 12 at frequency f = P % 12:
+//                sin(2^f pi p) for even t, cos(2^f pi p) for odd t (72 in all);
+//   t = 18..33 : one-blob bin b = 16 (g % 2) + t - 18 of theta (g < 2) or phi (g >= 2):
+//                exp(-(32 a - b - 0.5)^2 / 2), a = theta / pi or phi / 2 pi in [0, 1) (64 in all);
+//   t = 34, 35 : n.x, n.y (g = 0); n.z, 0 (g = 1); 0, 0 (g = 2, 3).
+// Every lane thus computes exactly the 36 features it feeds, with the same instructions as every
+// other lane (no divergence, nothing computed twice). Zero slots: features 135, 138, 139, 142, 143.
+// Frequencies come from one sincospi at the start of each coordinate run and the double-angle
+// recurrence (sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)(cos x + sin x)), at most 8 steps.
 __device__ __forceinline__ void guideFeatureTiles(const GuideInput& g, int grp, bf16x4 (&x0)[9]) {
-    float v[9][4];
+    float v[36];
+    // position items: a run of coordinate c0 from frequency f0, then coordinate c0 + 1 from 0
+    const int p0 = 9 * grp, c0 = p0 / 12, f0 = p0 % 12, brk = 12 - f0;
+    const float va = c0 == 0 ? g.px : (c0 == 1 ? g.py : g.pz);
+    const float vb = c0 == 0 ? g.py : g.pz;
+    float s, c, s1, c1;
+    sincospif(ldexpf(va, f0), &s, &c);  // 2^f0 va is exact
+    sincospif(vb, &s1, &c1);
 #pragma unroll
-    for (int kt = 0; kt < 9; ++kt)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v[kt][j] = 0.0f;
-    float s[3], c[3];
-    sincospif(g.px, &s[0], &c[0]);
-    sincospif(g.py, &s[1], &c[1]);
-    sincospif(g.pz, &s[2], &c[2]);
-#pragma unroll
-    for (int f = 0; f < kGFreq; ++f) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            const int k = 6 * f + j;
-            if (((k >> 2) & 3) == grp) v[k >> 4][k & 3] = j < 3 ? s[j] : c[j - 3];
-        }
-        if (f + 1 < kGFreq)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const float ns = 2.0f * s[i] * c[i];
-                c[i] = (c[i] - s[i]) * (c[i] + s[i]);
-                s[i] = ns;
-            }
+    for (int i = 0; i < 9; ++i) {
+        if (i == brk) s = s1, c = c1;
+        v[2 * i] = s;
+        v[2 * i + 1] = c;
+        const float ns = 2.0f * s * c;
+        c = (c - s) * (c + s);
+        s = ns;
     }
-    // normal (72..74) and the one-blob encodings (75..138) of this lane's slots
+    // one-blob of theta (groups 0, 1) or phi (groups 2, 3): both from one atan2
+    const bool th = grp < 2;
+    const float ay = th ? sqrtf(g.wx * g.wx + g.wy * g.wy) : g.wy, ax = th ? g.wz : g.wx;
+    float a = atan2f(ay, ax) * (th ? 0.31830988618379067154f : 0.15915494309189533577f);
+    if (!th && a < 0.0f) a += 1.0f;
+    const float u = a * kGBlob - static_cast<float>(16 * (grp & 1)) - 0.5f;
 #pragma unroll
-    for (int kt = 4; kt < 9; ++kt)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = kt * 16 + 4 * grp + j;
-            if (k >= 6 * kGFreq) v[kt][j] = guideFeature(g, k);
-        }
+    for (int i = 0; i < 16; ++i) {
+        const float d = u - static_cast<float>(i);
+        v[18 + i] = __builtin_amdgcn_exp2f(d * d * -0.72134752044448170368f);  // exp(-d^2 / 2)
+    }
+    v[34] = grp == 0 ? g.nx : (grp == 1 ? g.nz : 0.0f);
+    v[35] = grp == 0 ? g.ny : 0.0f;
 #pragma unroll
     for (int kt = 0; kt < 9; ++kt)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) x0[kt][j] = static_cast<short>(toBf16(v[kt][j]));
+        for (int j = 0; j < 4; ++j) x0[kt][j] = static_cast<short>(toBf16(v[4 * kt + j]));
 }
 
 // Forward pass of one 16-sample tile. x0: layer-0 B operands (9 K tiles). out: 3 D tiles of the

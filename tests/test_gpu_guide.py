@@ -40,7 +40,8 @@ def test_param_count_and_init_layout(dev):
     th = g.get_params().astype(np.float64)
     assert th.size == go.param_count() == 24880
     layers = go.unpack(th)
-    assert np.all(layers[0][0][:, 139:] == 0) and np.all(layers[4][0][40:] == 0)
+    assert np.all(layers[0][0][:, go.PAD_FEATURES] == 0) and np.all(layers[4][0][go.PAD_OUTPUTS] == 0)
+    assert np.count_nonzero(layers[0][0]) == 64 * (144 - len(go.PAD_FEATURES))
     assert all(np.all(b == 0) for _, b in layers)
 
 
@@ -52,8 +53,7 @@ def test_forward_mixture_matches_oracle(dev):
     layers = go.unpack(th)
     for _, b in layers:
         b[:] = rng.normal(0, 0.1, b.shape)
-    go.unpack(th)[4][1][40:] = 0.0
-    th[go.param_count() - 8:] = 0.0
+    go.unpack(th)[4][1][go.PAD_OUTPUTS] = 0.0
     g.set_params(th.astype(np.float32))
     th = g.get_params().astype(np.float64)
     n = 2000

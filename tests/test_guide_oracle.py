@@ -1,0 +1,97 @@
+"""CPU checks that pin the FP64 guiding oracle (oracle/guide_oracle.py) before the GPU kernels are
+compared with it: the input layout covers every feature once, the encodings satisfy their closed
+forms, the vMF mixture density integrates to one, and the analytic gradient of the loss agrees with
+central finite differences."""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import guide_oracle as go  # noqa: E402
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_feature_layout_is_a_permutation():
+    idx = [go.feature_index(g, t) for g in range(4) for t in range(36)]
+    assert sorted(idx) == list(range(go.NIN))
+    rng = np.random.default_rng(0)
+    f = go.encode(rng.uniform(0, 1, (64, 3)), _unit(rng, 64), _unit(rng, 64))
+    assert np.all(f[:, go.PAD_FEATURES] == 0)
+    live = np.setdiff1d(np.arange(go.NIN), go.PAD_FEATURES)
+    assert np.all(np.any(f[:, live] != 0, axis=0))
+
+
+def test_frequency_and_blob_closed_forms():
+    rng = np.random.default_rng(1)
+    p = rng.uniform(0, 1, (32, 3))
+    nrm, wo = _unit(rng, 32), _unit(rng, 32)
+    f = go.encode(p, nrm, wo)
+    # each (coordinate, frequency) item is a (sin, cos) pair on the unit circle, and doubling the
+    # frequency doubles the angle: sin(2x) = 2 sin x cos x
+    pairs = {}
+    for g in range(4):
+        for t in range(0, 18, 2):
+            P = 9 * g + t // 2
+            pairs[P] = (f[:, go.feature_index(g, t)], f[:, go.feature_index(g, t + 1)])
+    assert len(pairs) == 36
+    for P, (s, c) in pairs.items():
+        assert np.allclose(s * s + c * c, 1.0)
+        if P % 12 < 11:
+            s2, _ = pairs[P + 1]
+            assert np.allclose(s2, 2 * s * c, atol=1e-12)
+        if P % 12 == 0:
+            assert np.allclose(s, np.sin(np.pi * p[:, P // 12]))
+    assert np.allclose(f[:, [go.feature_index(0, 34), go.feature_index(0, 35), go.feature_index(1, 34)]], nrm)
+    # one-blob: a direction at the centre of theta bin b and phi bin c peaks there with value 1
+    b, c = 9, 27
+    th, ph = (b + 0.5) / 32 * np.pi, (c + 0.5) / 32 * 2 * np.pi
+    w = np.array([[np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]])
+    f = go.encode(np.zeros((1, 3)), np.array([[0, 0, 1.0]]), w)
+    tb = [f[0, go.feature_index(g, t)] for g in (0, 1) for t in range(18, 34)]
+    pb = [f[0, go.feature_index(g, t)] for g in (2, 3) for t in range(18, 34)]
+    assert np.argmax(tb) == b and np.argmax(pb) == c
+    assert np.isclose(tb[b], 1.0) and np.isclose(pb[c], 1.0)
+    assert np.isclose(tb[b + 1], np.exp(-0.5)) and np.isclose(pb[c - 2], np.exp(-2.0))
+
+
+def test_vmf_mixture_integrates_to_one():
+    rng = np.random.default_rng(2)
+    n_t, n_p = 2000, 4000
+    t = (np.arange(n_t) + 0.5) / n_t * np.pi
+    ph = (np.arange(n_p) + 0.5) / n_p * 2 * np.pi
+    T, Ph = np.meshgrid(t, ph, indexing="ij")
+    w = np.stack([np.sin(T) * np.cos(Ph), np.sin(T) * np.sin(Ph), np.cos(T)], -1).reshape(-1, 3)
+    dA = (np.pi / n_t) * (2 * np.pi / n_p) * np.sin(T).reshape(-1)
+    for kappa in (0.02, 1.0, 30.0, 500.0):
+        mu = _unit(rng, 1)[0]
+        total = (go.vmf(np.array(kappa), w @ mu) * dA).sum()
+        assert abs(total - 1.0) < 2e-3, (kappa, total)
+
+
+def test_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    theta = go.init_params(seed=3)
+    layers = go.unpack(theta)
+    for _, b in layers:
+        b[:] = rng.normal(0, 0.1, b.shape)
+    layers[4][1][go.NOUT:] = 0.0
+    n = 64
+    feats = go.encode(rng.uniform(0, 1, (n, 3)), _unit(rng, n), _unit(rng, n))
+    w = _unit(rng, n)
+    c = rng.uniform(0.0, 2.0, n)
+    _, grad = go.loss_and_grad(theta, feats, w, c, quant=False)
+    # parameters with a live gradient: every layer's weights and biases, sampled
+    live = np.flatnonzero(grad != 0)
+    pick = rng.choice(live, 60, replace=False)
+    h = 1e-6
+    for k in pick:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (go.loss(tp, feats, w, c, quant=False) - go.loss(tm, feats, w, c, quant=False)) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, grad[k])
