@@ -79,55 +79,112 @@ def _rel_mse(a, b):
     return float(np.mean((a - b) ** 2 / (b * b + 0.01)))
 
 
-def time_to_relmse(sc, scene, film, ref, render, target=0.01, max_level=14):
-    """Progressive render with doubling sample counts (1, 2, 4, ... spp accumulated in one film);
-    returns (seconds to reach `target` relMSE, interpolated in log-log between levels, trace)."""
+def time_to_relmse(sc, film, ref, render, target=0.01, max_level=14):
+    """Progressive render with doubling sample counts (1, 2, 4, ... 2^max_level spp accumulated in
+    one film); returns (seconds to reach `target` relMSE, interpolated in log-log between levels,
+    or None, and the whole trace of (spp, seconds, relMSE))."""
     import math
     film.clear()
-    t, prev, trace = 0.0, None, []
+    t, prev, trace, hit = 0.0, None, [], None
     for k in range(max_level + 1):
         s0, s1 = (0, 1) if k == 0 else (1 << (k - 1), 1 << k)
         st = render(pg_params(sc, (s0, s1)))
         t += st["ms_total"] / 1e3
         e = _rel_mse(_img(film.download()), ref)
-        trace.append((s1, round(t, 4), round(e, 5)))
-        if e <= target:
+        trace.append((s1, round(t, 4), round(e, 6)))
+        if hit is None and e <= target:
             if prev is None or prev[1] <= e:
-                return t, trace
-            (t0, e0) = prev
-            f = (math.log(target) - math.log(e0)) / (math.log(e) - math.log(e0))
-            return math.exp(math.log(t0) + f * (math.log(t) - math.log(t0))), trace
+                hit = t
+            else:
+                f = (math.log(target) - math.log(prev[1])) / (math.log(e) - math.log(prev[1]))
+                hit = math.exp(math.log(prev[0]) + f * (math.log(t) - math.log(prev[0])))
         prev = (t, e)
-    return None, trace
+    return hit, trace
 
 
-def pg_params(sc, spp_range):
+def equal_time(trace_a, trace_b):
+    """relMSE of trace b at the times of trace a's levels (log-log interpolation between b's levels),
+    as (seconds, relMSE a, relMSE b) for every time both traces cover."""
+    import math
+    out = []
+    for (_, t, e) in trace_a:
+        for (_, t0, e0), (_, t1, e1) in zip(trace_b, trace_b[1:]):
+            if t0 <= t <= t1 and min(e0, e1, t0) > 0:
+                f = (math.log(t) - math.log(t0)) / (math.log(t1) - math.log(t0))
+                out.append((round(t, 4), e, round(math.exp(math.log(e0) + f * (math.log(e1) - math.log(e0))), 6)))
+                break
+    return out
+
+
+SPP_DECLARED = 1 << 18  # sample index space: [0, 2^16) measured renders, [2^16, 2^17) reference, 2^17+ training
+
+
+def pg_params(sc, spp_range, flags=0):
     import piper2_amd as pg
-    return pg.render_params(sc, spp=1 << 16, spp_range=spp_range)
+    return pg.render_params(sc, spp=SPP_DECLARED, spp_range=spp_range, flags=flags)
 
 
-def guided_vs_plain(dev, sc, ref_spp, label):
-    """Time to 1% relMSE for the plain tracer and the online-trained guided tracer on one scene."""
+def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_level=14):
+    """Guided (online-trained, one-sample MIS at 0.5) against the plain tracer on one scene.
+
+    The reference R is an FP64-film render of ref_spp samples by the guided estimator with a guide
+    trained beforehand on samples no measured render uses, then frozen (the estimator is unbiased
+    for any guide; a trained one reaches a given error in fewer samples than the plain tracer). A
+    plain FP64 render of the same samples is compared with it as a cross-check. Reported: time to
+    1% relMSE from a fresh guide trained online during the progressive render (training included),
+    the plain tracer's time, and both estimators' relMSE at equal samples with the trained guide."""
     import piper2_amd as pg
     scene = pg.Scene(dev, sc)
     film = pg.Film(dev, sc["width"], sc["height"])
+    R = 1 << 16
+    chunk = max(1, min(4096, (1 << 31) // (sc["width"] * sc["height"])))  # < 2^32 paths per call
     t0 = time.perf_counter()
-    for s0 in range(0, ref_spp, 1024):  # reference: plain, FP64 film
-        scene.render(pg.render_params(sc, spp=1 << 16, spp_range=(s0, min(s0 + 1024, ref_spp)), flags=pg.PG_FILM_FP64), film)
+    trained = pg.Guide(dev, seed=1)
+    for p in range(train_passes):
+        scene.render_guided(trained, pg_params(sc, (2 * R + train_spp * p, 2 * R + train_spp * (p + 1))), None)
+    frozen = pg.Guide(dev, seed=1, train_records=0)
+    frozen.set_params(trained.get_params())
+    trained.close()
+    for s0 in range(R, R + ref_spp, chunk):
+        scene.render_guided(frozen, pg_params(sc, (s0, min(s0 + chunk, R + ref_spp)), pg.PG_FILM_FP64), film)
     ref = _img(film.download(fp64=True))
-    log(f"{label}: {ref_spp}-spp FP64 reference in {time.perf_counter() - t0:.1f}s")
     film.clear()
-    t_plain, tr_plain = time_to_relmse(sc, scene, film, ref, lambda p: scene.render(p, film))
+    for s0 in range(R, R + ref_spp, chunk):
+        scene.render(pg_params(sc, (s0, min(s0 + chunk, R + ref_spp)), pg.PG_FILM_FP64), film)
+    ref_plain = _img(film.download(fp64=True))
+    log(f"{label}: {ref_spp}-spp FP64 references in {time.perf_counter() - t0:.1f}s; "
+        f"relMSE(plain, guided) = {_rel_mse(ref_plain, ref):.5f}, mean {ref.mean():.5f}")
+    equal_spp = {}
+    for spp in (256, 1024):
+        film.clear()
+        scene.render(pg_params(sc, (0, spp)), film)
+        e_plain = _rel_mse(_img(film.download()), ref)
+        film.clear()
+        scene.render_guided(frozen, pg_params(sc, (0, spp)), film)
+        equal_spp[str(spp)] = {"plain": round(e_plain, 5), "trained_guide": round(_rel_mse(_img(film.download()), ref), 5)}
+    frozen.close()
+    t_plain, tr_plain = time_to_relmse(sc, film, ref, lambda p: scene.render(p, film), max_level=max_level)
     guide = pg.Guide(dev, seed=1, lr=1e-2, mis_fraction=0.5, train_batch=1 << 14, train_records=1 << 18)
-    t_guided, tr_guided = time_to_relmse(sc, scene, film, ref, lambda p: scene.render_guided(guide, p, film))
+    t_guided, tr_guided = time_to_relmse(sc, film, ref, lambda p: scene.render_guided(guide, p, film),
+                                         max_level=max_level)
     log(f"{label}: plain {t_plain} s {tr_plain}")
     log(f"{label}: guided {t_guided} s {tr_guided}")
     out = {"plain_s": t_plain and round(t_plain, 4), "guided_s": t_guided and round(t_guided, 4),
-           "reference_spp": ref_spp, "resolution": [sc["width"], sc["height"]],
+           "reference": f"{ref_spp}-spp FP64 guided render (trained guide, frozen)",
+           "reference_vs_plain_fp64_relmse": round(_rel_mse(ref_plain, ref), 5),
+           "reference_mean": round(float(ref.mean()), 5),
+           "resolution": [sc["width"], sc["height"]], "relmse_at_equal_spp": equal_spp,
            "plain_trace_spp_s_relmse": tr_plain, "guided_trace_spp_s_relmse": tr_guided}
     if t_plain and t_guided:
         out["guided_speedup"] = round(t_plain / t_guided, 3)
+    eq = equal_time(tr_plain, tr_guided)
+    if eq:
+        t, e_plain, e_guided = eq[-1]
+        out["equal_time"] = {"seconds": t, "plain_relmse": e_plain, "guided_relmse": e_guided,
+                             "plain_over_guided": round(e_plain / e_guided, 3)}
     guide.close()
+    film.close()
+    scene.close()
     return out
 
 
@@ -198,7 +255,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
-    ap.add_argument("--ref-spp", type=int, default=16384, help="reference samples per pixel of the guided comparison")
+    ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the render shards by tiles or "
@@ -314,8 +371,8 @@ def main():
         result["time_to_1pct_relmse"] = {
             "hard_room_256x256_depth12": guided_vs_plain(dev, scenes.hard_room(seed=0, width=256, height=256),
                                                          args.ref_spp, "hard room"),
-            "terrain_slit_480x270_depth8": guided_vs_plain(dev, scenes.terrain_slit(width=480, height=270),
-                                                           args.ref_spp // 4, "terrain slit"),
+            "terrain_slit_1920x1080_depth8": guided_vs_plain(dev, scenes.terrain_slit(), args.ref_spp // 16,
+                                                             "terrain slit", max_level=9),
         }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)

@@ -205,7 +205,7 @@ uint32_t pg_guide_param_count(void); /* 24880 FP32 parameters: per layer W[out][
 int pg_guide_get_params(pg_guide* guide, float* theta, uint32_t n);
 int pg_guide_set_params(pg_guide* guide, const float* theta, uint32_t n); /* also resets Adam */
 /* in[12 n] = per sample (pos01 xyz, -, normal xyz, -, wo xyz, -); mix[48 n] = 8 x (mu xyz, kappa),
- * then 8 lobe weights, then 8 unused. */
+ * then 8 lobe weights pi, then 8 x pi * kappa / (2 pi (1 - exp(-2 kappa))). */
 int pg_guide_infer(pg_guide* guide, const float* in, uint32_t n, float* mix);
 /* Training records rec[16 n] = (pos01 xyz, weight c, normal, -, wo, -, sampled direction, -).
  * Loss = -sum c log p(direction) / sum c. gradients: FP32 dLoss/dtheta, no update. */

@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHo
                 reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
                 reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
                 reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
+                reinterpret_cast<float2*>(o + 40)[grp] = make_float2(lp.pi[0] * vmfNorm(lp.kappa[0]), lp.pi[1] * vmfNorm(lp.kappa[1]));
             }
         }
         waveLdsSync();  // stage reads done before the next samples overwrite it

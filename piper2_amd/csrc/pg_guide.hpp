@@ -36,7 +36,8 @@ constexpr int kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGOut * kGHid;
 constexpr int kGParams = kGB4 + kGOut;  // 24880
 static_assert(kGParams == 24880, "guide parameter count");
 constexpr float kGKappaLo = -4.0f, kGKappaHi = 9.0f;  // log-kappa clamp
-constexpr int kGMixFloats = 48;  // per sample: 8 x (mu xyz, kappa), 8 x weight, 8 padding
+// Mixture record per sample: 8 x (mu xyz, kappa), 8 weights pi, 8 x pi * vMF normalisation
+constexpr int kGMixFloats = 48;
 
 // LDS row strides (bf16 elements): 4 extra elements keep the 16 rows of one A-operand read on
 // distinct banks.
@@ -234,17 +235,22 @@ __device__ __forceinline__ void guideForwardTile(const GuideLds& L, const bf16x4
 }
 
 // ---- von Mises-Fisher mixture ---------------------------------------------------------------
+// vMF density on the sphere: kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
+__device__ __forceinline__ float vmfNorm(float kappa) {
+    return kappa / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * kappa)));
+}
 __device__ __forceinline__ float vmfPdf(float kappa, float cosang) {
-    return kappa / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * kappa))) * __expf(kappa * (cosang - 1.0f));
+    return vmfNorm(kappa) * __expf(kappa * (cosang - 1.0f));
 }
 
-// Mixture record of one sample: lobe[i] = (mu xyz, kappa) for i < 8 at float4 i, weights at 32..39.
+// Mixture record of one sample: lobe[i] = (mu xyz, kappa) for i < 8 at float4 i, weights pi at
+// 32..39, pi_i times lobe i's normalisation at 40..47 (so the density costs one exp per lobe).
 __device__ __forceinline__ float mixturePdf(const float* __restrict__ mix, float3 w) {
     float p = 0.0f;
 #pragma unroll
     for (int i = 0; i < kGLobes; ++i) {
         const float4 l = reinterpret_cast<const float4*>(mix)[i];
-        p += mix[32 + i] * vmfPdf(l.w, l.x * w.x + l.y * w.y + l.z * w.z);
+        p += mix[40 + i] * __expf(l.w * (l.x * w.x + l.y * w.y + l.z * w.z - 1.0f));
     }
     return p;
 }

@@ -65,6 +65,8 @@ def test_forward_mixture_matches_oracle(dev):
     gk = mix[:, :32].reshape(n, 8, 4)[:, :, 3]
     gpi = mix[:, 32:40]
     assert np.allclose(gpi.sum(1), 1.0, atol=1e-5)
+    norm = gk / (2 * np.pi * -np.expm1(-2.0 * gk.astype(np.float64)))
+    assert np.allclose(mix[:, 40:48], gpi * norm, rtol=1e-5, atol=1e-30)
     assert np.median(np.abs(gk - kappa) / kappa) < 2e-2
     assert np.median(np.abs(gpi - pi)) < 2e-3
     cosmu = (gmu * mu).sum(-1)
