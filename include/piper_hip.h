@@ -218,6 +218,15 @@ int pg_guide_train(pg_guide* guide, const float* rec, uint32_t n, float* loss);
  * progressive driver trains online pass by pass while every pass stays unbiased. */
 int pg_render_guided(pg_scene* scene, pg_guide* guide, const pg_render_params* params, pg_film* film,
                      pg_stats* stats);
+/* pg_render_batch with guided sampling: scene i uses guides[i] (scenes may share a guide; at most
+ * 256 distinct guides). One persistent path pool runs every scene; each bounce groups the
+ * extension queue by guide so every inference block runs one guide's weights. Each guide with
+ * train_records > 0 trains after the batch on the records of its own scenes. With frozen guides
+ * (train_records = 0) each film is exactly what pg_render_guided(scenes[i], guides[i], ...)
+ * leaves. No reference counterpart (the reference has no guiding and no batching). */
+int pg_render_batch_guided(pg_device* dev, pg_scene* const* scenes, pg_guide* const* guides,
+                           const pg_render_params* params, uint32_t n, pg_film* const* films, double* latency_ms,
+                           pg_stats* stats);
 
 /* ---- diagnostics ---- */
 /* Make the next device call on this handle fail with PG_ERR_DEVICE (host-forced fault path test). */

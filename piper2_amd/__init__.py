@@ -135,6 +135,8 @@ SIGNATURES = {
     "pg_render_guided": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
     "pg_render_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(RenderParams), _U32, _VP, _VP,
                                        ctypes.POINTER(Stats)]),
+    "pg_render_batch_guided": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(RenderParams), _U32, _VP, _VP,
+                                              ctypes.POINTER(Stats)]),
     "pg_autotune": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), ctypes.c_uint64, ctypes.POINTER(TuneResult)]),
 }
 
@@ -400,18 +402,24 @@ class Film:
             pass
 
 
-def render_batch(device, scenes, params, films=None):
-    """One batched render of several scenes (pg_render_batch). films: list of Film or None entries.
+def render_batch(device, scenes, params, films=None, guides=None):
+    """One batched render of several scenes (pg_render_batch, or pg_render_batch_guided when
+    `guides` lists one Guide per scene). films: list of Film or None entries.
     Returns (stats dict, per-scene latency in ms as a numpy array)."""
     n = len(scenes)
-    assert len(params) == n and (films is None or len(films) == n)
+    assert len(params) == n and (films is None or len(films) == n) and (guides is None or len(guides) == n)
     hs = (ctypes.c_void_p * n)(*[s.h.value for s in scenes])
     ps = (RenderParams * n)(*params)
     fs = (ctypes.c_void_p * n)(*[(f.h.value if f else None) for f in films]) if films else None
     lat = np.zeros(n, np.float64)
     st = Stats()
-    check(lib().pg_render_batch(device.h, ctypes.cast(hs, _VP), ps, n, ctypes.cast(fs, _VP) if fs else None,
-                                _ptr(lat), ctypes.byref(st)))
+    if guides is None:
+        check(lib().pg_render_batch(device.h, ctypes.cast(hs, _VP), ps, n, ctypes.cast(fs, _VP) if fs else None,
+                                    _ptr(lat), ctypes.byref(st)))
+    else:
+        gs = (ctypes.c_void_p * n)(*[g.h.value for g in guides])
+        check(lib().pg_render_batch_guided(device.h, ctypes.cast(hs, _VP), ctypes.cast(gs, _VP), ps, n,
+                                           ctypes.cast(fs, _VP) if fs else None, _ptr(lat), ctypes.byref(st)))
     return st.as_dict(), lat
 
 

@@ -121,7 +121,8 @@ struct pg_device {
     uint32_t capacity = 0;
     DevBuf recL, recY, quad;  // film record ring and per-pixel footprint sums
     DevBuf itemDone;          // paths recorded per work item
-    DevBuf mix, hist, trainRec, trainCount;  // guiding: mixtures, vertex history, training records
+    DevBuf mix, hist;          // guiding: mixtures per queue entry, vertex history per path slot
+    DevBuf batchGuides, keyWb, keyTheta, groupBuf;  // guided batches: per-item guides, grouped inference
     DevBuf paths, qext0, qext1, qshadow, qmis, qdone, counts, totals, spill, travCounts;
     DevBuf batchScenes, batchRc, batchOffsets;  // work-item tables of a batched render
     void* status = nullptr;   // pinned host copy of the pool counters and item progress
@@ -169,6 +170,7 @@ struct pg_guide {
     pg_guide_params p{};
     DevBuf theta, m, v, wb;                  // FP32 masters, Adam moments, BF16 copy
     DevBuf scratch, partial, tiles, csum, loss, grads;
+    DevBuf trainRec, trainCount;              // training records of the current render
     const void* tilesFor = nullptr;           // scratch pointer the uploaded tile table refers to
     uint32_t tilesNpad = 0;
     uint32_t step = 0;                        // Adam steps taken
@@ -454,7 +456,8 @@ struct RunOptions {
     pg::RenderConsts rc0{};       // single scene: the call's consts (segments by rc0.segSpp)
     pg_film* const* films = nullptr;  // per scene (entries may be null)
     uint32_t capacity = 1u << 21;
-    pg_guide* guide = nullptr;    // single scene only
+    pg_guide* guide = nullptr;    // single scene
+    pg_guide* const* guides = nullptr;  // per scene (guided batch)
     float* pathLHost = nullptr;   // single scene only
     double* latencyMs = nullptr;  // per scene
     pg_stats* stats = nullptr;
@@ -532,32 +535,83 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
 
     const pg::PathState ps = pathState(dev);
     const pg::Queues q = queues(dev);
-    // guiding: mixture per extension-queue entry, per-slot vertex history of training paths
+    // Guiding: one guide (single scene) or one per scene (guided batch; scenes may share a guide).
+    // Mixtures per extension-queue entry, per-slot vertex history of training paths, training
+    // records per guide. Only vertices that found light give records, so the share of a guide's
+    // paths that train follows its record yield in the previous render: every stride-th path.
     pg::GuideShade gs{};
     pg::GuideRecordOut tr{};
-    pg_guide* guide = opt.guide;
-    if (guide) {
-        const pg_scene* s = scenes[0];
+    std::vector<pg_guide*> keys;                       // distinct guides of the call
+    std::vector<uint32_t> itemKey(items.size(), 0), keyStride;
+    auto guideOf = [&](uint32_t scene) { return opt.guides ? opt.guides[scene] : opt.guide; };
+    pg::GroupBuffers gb{};
+    const uint16_t* const* keyWb = nullptr;
+    const float* const* keyTheta = nullptr;
+    if (opt.guide || opt.guides) {
+        std::vector<uint64_t> keyPaths;
+        for (size_t k = 0; k < items.size(); ++k) {
+            pg_guide* g = guideOf(items[k].scene);
+            require(g && g->dev == dev, "every scene of a guided render needs a guide on the render's device");
+            const auto at = std::find(keys.begin(), keys.end(), g);
+            itemKey[k] = static_cast<uint32_t>(at - keys.begin());
+            if (at == keys.end()) keys.push_back(g), keyPaths.push_back(0);
+            keyPaths[itemKey[k]] += items[k].count;
+        }
+        require(keys.size() <= pg::kMaxBatchGuides, "too many distinct guides in one batch");
         gs.enabled = 1;
-        gs.alpha = guide->p.mis_fraction;
-        gs.lo = make_float3(s->lo[0], s->lo[1], s->lo[2]);
-        gs.invExt = make_float3(s->invExt[0], s->invExt[1], s->invExt[2]);
         dev->mix.reserve(static_cast<size_t>(cap) * pg::kGMixFloatsHost * 4);
         gs.mix = dev->mix.as<float>();
-        if (guide->p.train_records > 0) {
+        keyStride.assign(keys.size(), 1);
+        for (size_t k = 0; k < keys.size(); ++k) {
+            pg_guide* g = keys[k];
+            if (g->p.train_records == 0) continue;
             gs.train = 1;
-            gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(rc0.maxDepth, 255));
-            // Only vertices that found light give records, so the share of paths that train
-            // follows the record yield of the previous render: every stride-th path trains.
-            const double want = static_cast<double>(total) * guide->recordsPerPath / guide->p.train_records;
-            gs.trainStride = static_cast<uint32_t>(std::min(std::max(want, 1.0), 1e9));
-            gs.trainPhase = gs.trainStride / 2;
+            const double want = static_cast<double>(keyPaths[k]) * g->recordsPerPath / g->p.train_records;
+            keyStride[k] = static_cast<uint32_t>(std::min(std::max(want, 1.0), 1e9));
+            g->trainRec.reserve(static_cast<size_t>(g->p.train_records) * 64);
+            g->trainCount.reserve(16);
+            PG_HIP(hipMemsetAsync(g->trainCount.p, 0, 4, st));
+        }
+        if (gs.train) {
+            gs.histDepth = std::max<uint32_t>(1, std::min<uint32_t>(maxDepth, 255));
             dev->hist.reserve(static_cast<size_t>(cap) * gs.histDepth * 6 * 16);
             gs.hist = dev->hist.as<float4>();
-            dev->trainRec.reserve(static_cast<size_t>(guide->p.train_records) * 64);
-            dev->trainCount.reserve(16);
-            PG_HIP(hipMemsetAsync(dev->trainCount.p, 0, 4, st));
-            tr = pg::GuideRecordOut{dev->trainRec.as<float4>(), dev->trainCount.as<uint32_t>(), guide->p.train_records};
+        }
+        auto records = [](pg_guide* g) {
+            return g->p.train_records ? pg::GuideRecordOut{g->trainRec.as<float4>(), g->trainCount.as<uint32_t>(), g->p.train_records}
+                                      : pg::GuideRecordOut{nullptr, nullptr, 0};
+        };
+        if (!opt.batch) {
+            const pg_scene* s = scenes[0];
+            gs.alpha = keys[0]->p.mis_fraction;
+            gs.lo = make_float3(s->lo[0], s->lo[1], s->lo[2]);
+            gs.invExt = make_float3(s->invExt[0], s->invExt[1], s->invExt[2]);
+            gs.trainStride = keyStride[0];
+            gs.trainPhase = keyStride[0] / 2;
+            tr = records(keys[0]);
+        } else {
+            std::vector<pg::ItemGuide> tab(items.size());
+            for (size_t k = 0; k < items.size(); ++k) {
+                pg_guide* g = keys[itemKey[k]];
+                const pg_scene* s = scenes[items[k].scene];
+                const uint32_t stride = keyStride[itemKey[k]];
+                tab[k] = pg::ItemGuide{g->wb.as<uint16_t>(), g->theta.as<float>(), records(g),
+                                       make_float3(s->lo[0], s->lo[1], s->lo[2]),
+                                       make_float3(s->invExt[0], s->invExt[1], s->invExt[2]),
+                                       g->p.mis_fraction, itemKey[k], stride, stride / 2};
+            }
+            upload(dev->batchGuides, tab.data(), tab.size() * sizeof(pg::ItemGuide), st);
+            bt.guide = dev->batchGuides.as<pg::ItemGuide>();
+            std::vector<const uint16_t*> wbs;
+            std::vector<const float*> thetas;
+            for (pg_guide* g : keys) wbs.push_back(g->wb.as<uint16_t>()), thetas.push_back(g->theta.as<float>());
+            upload(dev->keyWb, wbs.data(), wbs.size() * sizeof(void*), st);
+            upload(dev->keyTheta, thetas.data(), thetas.size() * sizeof(void*), st);
+            keyWb = dev->keyWb.as<const uint16_t*>();
+            keyTheta = dev->keyTheta.as<const float*>();
+            dev->groupBuf.reserve(pg::groupBufferBytes(cap));
+            gb = pg::groupBuffers(dev->groupBuf.p, cap);
+            PG_HIP(hipMemsetAsync(gb.keyCount, 0, pg::kMaxBatchGuides * 4, st));  // the scan clears it afterwards
         }
         pg::guideConfigureLds();
     }
@@ -622,11 +676,17 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
             pg::launchRegen(st, ds0, ps, q, rc0, cur, threads, btp);
         });
         timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, trav, btp); });
-        if (guide)
+        if (!keys.empty())  // 2 inference blocks of 8 waves per CU fit (LDS)
             timeStage(kGuide, [&] {
-                pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
-                                          guide->wb.as<uint16_t>(), guide->theta.as<float>(), dev->mix.as<float>(),
-                                          static_cast<uint32_t>(dev->numCUs) * 2);  // 2 blocks of 8 waves per CU fit (LDS)
+                const uint32_t blocks = static_cast<uint32_t>(dev->numCUs) * 2;
+                if (opt.batch)
+                    pg::launchGuideInferGrouped(st, bt, ps, q.ext[cur], q.counts + cur, cap,
+                                                static_cast<uint32_t>(keys.size()), gb, keyWb, keyTheta,
+                                                dev->mix.as<float>(), blocks);
+                else
+                    pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
+                                              keys[0]->wb.as<uint16_t>(), keys[0]->theta.as<float>(),
+                                              dev->mix.as<float>(), blocks);
             });
         timeStage(kShade, [&] {
             pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp);
@@ -704,19 +764,24 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
                 }
         }
     }
-    uint32_t trained = 0;
-    if (gs.train) {  // online training on the records this render collected
+    uint64_t trained = 0;
+    for (size_t k = 0; k < keys.size(); ++k) {  // online training on the records this render collected
+        pg_guide* g = keys[k];
+        if (g->p.train_records == 0) continue;
         uint32_t got = 0;
-        PG_HIP(hipMemcpyAsync(&got, tr.count, 4, hipMemcpyDeviceToHost, st));
+        PG_HIP(hipMemcpyAsync(&got, g->trainCount.p, 4, hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
-        trained = std::min(got, tr.capacity);
-        const double trainPaths = static_cast<double>((total + gs.trainStride - 1 - gs.trainPhase) / gs.trainStride);
-        if (trainPaths > 0.0)
-            guide->recordsPerPath = std::min(std::max(static_cast<double>(got) / trainPaths, 1e-6), 16.0);
-        for (uint32_t b = 0; b + 64 <= trained; b += guide->p.train_batch) {
-            const uint32_t cnt = std::min(guide->p.train_batch, trained - b);
-            timeStage(kTrain, [&] { guideStep(guide, dev->trainRec.as<float>() + static_cast<size_t>(b) * 16, cnt, true); });
+        const uint32_t n = std::min(got, g->p.train_records);
+        const uint64_t stride = keyStride[k], phase = stride / 2;
+        double trainPaths = 0.0;  // paths g with g % stride == phase, over the guide's items
+        for (size_t i = 0; i < items.size(); ++i)
+            if (itemKey[i] == k) trainPaths += static_cast<double>((items[i].count + stride - 1 - phase) / stride);
+        if (trainPaths > 0.0) g->recordsPerPath = std::min(std::max(static_cast<double>(got) / trainPaths, 1e-6), 16.0);
+        for (uint32_t b = 0; b + 64 <= n; b += g->p.train_batch) {
+            const uint32_t cnt = std::min(g->p.train_batch, n - b);
+            timeStage(kTrain, [&] { guideStep(g, g->trainRec.as<float>() + static_cast<size_t>(b) * 16, cnt, true); });
         }
+        trained += n;
     }
     PG_HIP(hipStreamSynchronize(st));
     if (opt.latencyMs)
@@ -779,7 +844,7 @@ void render(pg_scene* s, const pg_render_params* params, pg_film* film, float* p
 // leave the device idle. Per path the arithmetic is that of render(), and every scene is resolved
 // segment by segment in the same order, so each film is bit-identical to its own render.
 void renderBatch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
-                 pg_film* const* films, double* latencyMs, pg_stats* stats) {
+                 pg_film* const* films, double* latencyMs, pg_stats* stats, pg_guide* const* guides = nullptr) {
     require(dev && scenes && params && n > 0, "empty batch");
     std::lock_guard<std::mutex> lock(dev->mu);
     dev->checkFault();
@@ -803,6 +868,7 @@ void renderBatch(pg_device* dev, pg_scene* const* scenes, const pg_render_params
     opt.capacity = params[0].wavefront_paths ? params[0].wavefront_paths : (1u << 21);
     opt.latencyMs = latencyMs;
     opt.stats = stats;
+    opt.guides = guides;
     opt.graph = params[0].flags & pg::kUseGraph;
     opt.timed = stats != nullptr && !(params[0].flags & pg::kNoStageTiming) && !opt.graph;
     runPaths(dev, scenes, n, items, total, opt);
@@ -1122,6 +1188,15 @@ int pg_render_paths(pg_scene* s, const pg_render_params* params, pg_film* film, 
 int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
                     pg_film* const* films, double* latency_ms, pg_stats* stats) {
     return guarded([&] { renderBatch(dev, scenes, params, n, films, latency_ms, stats); });
+}
+
+int pg_render_batch_guided(pg_device* dev, pg_scene* const* scenes, pg_guide* const* guides,
+                           const pg_render_params* params, uint32_t n, pg_film* const* films, double* latency_ms,
+                           pg_stats* stats) {
+    return guarded([&] {
+        require(guides != nullptr, "null guides");
+        renderBatch(dev, scenes, params, n, films, latency_ms, stats, guides);
+    });
 }
 
 int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample_paths, pg_tune_result* out) {

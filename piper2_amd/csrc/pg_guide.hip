@@ -90,47 +90,150 @@ __device__ __forceinline__ void waveLdsSync() {
 constexpr int kInferStageBytes = 64 * 3 * 16;  // per wave: 64 samples x 3 float4
 // 8 waves share one LDS copy of the weights: 37 KB + 8 x 3 KB, two blocks per CU (4 waves per SIMD)
 constexpr int kInferBlock = 512;
+static_assert(kInferBlock == static_cast<int>(kGroupUnit), "a grouped unit is one block's 8 x 64 samples");
+
+// Mixtures of one wave's 64 samples: lane l's sample is queue/array entry `entry` (~0u: none).
+// Wave-uniform; the stage is the wave's own LDS slice.
+template <typename Src>
+__device__ __forceinline__ void inferWave64(const Src& src, uint32_t entry, const GuideLds& L, float4* stage,
+                                            float* __restrict__ mix) {
+    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
+    const bool valid = entry != ~0u;
+    {
+        const GuideInput in = valid ? src.get(entry) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
+        stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
+        stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
+        stage[3 * lane + 2] = make_float4(in.wz, __uint_as_float(entry), 0.0f, 0.0f);
+    }
+    const uint64_t live = __ballot(valid);
+    waveLdsSync();  // the wave's own stage: written and read by its lanes only
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        if (((live >> (16 * t)) & 0xFFFFull) == 0) continue;  // wave-uniform
+        const int k = 16 * t + s;
+        const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
+        const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
+        const uint32_t i = __float_as_uint(c.y);
+        bf16x4 x0[9];
+        guideFeatureTiles(in, grp, x0);
+        f32x4 out[3];
+        guideForwardTile(L, x0, out);
+        const LobePair lp = lobesFromTile(out);
+        if (i != ~0u) {
+            float* o = mix + static_cast<size_t>(i) * kGMixFloats;
+            reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
+            reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
+            reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
+            reinterpret_cast<float2*>(o + 40)[grp] = make_float2(lp.pi[0] * vmfNorm(lp.kappa[0]), lp.pi[1] * vmfNorm(lp.kappa[1]));
+        }
+    }
+    waveLdsSync();  // stage reads done before the next samples overwrite it
+}
 
 template <typename Src>
 __global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
-                                                   const uint16_t* __restrict__ wb, const float* __restrict__ theta,
-                                                   float* __restrict__ mix) {
+                                                           const uint16_t* __restrict__ wb, const float* __restrict__ theta,
+                                                           float* __restrict__ mix) {
     extern __shared__ unsigned char smem[];
     const uint32_t n = nDev ? *nDev : nHost;
     const GuideLds L = guideLdsLayout(smem);
     float4* stage = reinterpret_cast<float4*>(smem + kGLdsWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
     guideLoadLds(L, wb, theta);
     __syncthreads();  // the weights are shared by the block's waves; after this each wave runs alone
-    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
+    const uint32_t lane = __lane_id();
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, waves = gridDim.x * (blockDim.x / 64);
-    for (uint32_t base = wave * 64; base < n; base += waves * 64) {  // wave-uniform
-        {
-            const uint32_t i = base + lane;
-            const GuideInput in = i < n ? src.get(i) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
-            stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
-            stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
-            stage[3 * lane + 2] = make_float4(in.wz, 0.0f, 0.0f, 0.0f);
+    for (uint32_t base = wave * 64; base < n; base += waves * 64)  // wave-uniform
+        inferWave64(src, base + lane < n ? base + lane : ~0u, L, stage, mix);
+}
+
+// ---- guided batches: inference grouped by guide --------------------------------------------------
+// Queue entries of a guided batch: path slot -> item -> its scene's triangles and box.
+struct BatchPathSource {
+    const DevScene* scenes;
+    const ItemGuide* guide;
+    const PathRec* paths;
+    const uint32_t* queue;
+    __device__ __forceinline__ uint32_t key(uint32_t i) const { return guide[paths[queue[i]].meta.z >> 16].key; }
+    __device__ __forceinline__ GuideInput get(uint32_t i) const {
+        const uint32_t path = queue[i];
+        const uint32_t item = paths[path].meta.z >> 16;
+        const PathSource one{scenes[item].tris, paths, queue, guide[item].lo, guide[item].invExt};
+        return one.get(i);
+    }
+};
+
+__global__ void __launch_bounds__(256) kGroupCount(BatchPathSource src, const uint32_t* count, uint32_t nKeys,
+                                                   uint32_t* keyCount) {
+    __shared__ uint32_t hist[kMaxBatchGuides];
+    for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&hist[src.key(i)], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x)
+        if (hist[k]) atomicAdd(&keyCount[k], hist[k]);
+}
+
+// One block: padded exclusive scan of the key counts; clears the counts for the next iteration.
+__global__ void __launch_bounds__(kMaxBatchGuides) kGroupScan(uint32_t nKeys, GroupBuffers gb) {
+    __shared__ uint32_t pad[kMaxBatchGuides];
+    const uint32_t k = threadIdx.x;
+    const uint32_t c = k < nKeys ? gb.keyCount[k] : 0u;
+    pad[k] = (c + kGroupUnit - 1) / kGroupUnit * kGroupUnit;
+    __syncthreads();
+    if (k == 0) {
+        uint32_t acc = 0;
+        for (uint32_t j = 0; j < nKeys; ++j) {
+            const uint32_t p = pad[j];
+            pad[j] = acc;
+            acc += p;
         }
-        waveLdsSync();  // the wave's own stage: written and read by its lanes only
-#pragma unroll 1
-        for (uint32_t t = 0; t < 4 && base + 16 * t < n; ++t) {
-            const uint32_t k = 16 * t + s, i = base + k;
-            const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
-            const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
-            bf16x4 x0[9];
-            guideFeatureTiles(in, grp, x0);
-            f32x4 out[3];
-            guideForwardTile(L, x0, out);
-            const LobePair lp = lobesFromTile(out);
-            if (i < n) {
-                float* o = mix + static_cast<size_t>(i) * kGMixFloats;
-                reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
-                reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
-                reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
-                reinterpret_cast<float2*>(o + 40)[grp] = make_float2(lp.pi[0] * vmfNorm(lp.kappa[0]), lp.pi[1] * vmfNorm(lp.kappa[1]));
-            }
+        gb.keyStart[nKeys] = acc;
+    }
+    __syncthreads();
+    if (k < nKeys) {
+        gb.keyStart[k] = pad[k];
+        gb.keyEnd[k] = pad[k] + c;
+        gb.cursor[k] = 0;
+        gb.keyCount[k] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource src, const uint32_t* count, GroupBuffers gb) {
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = src.key(i);
+        gb.perm[gb.keyStart[k] + atomicAdd(&gb.cursor[k], 1u)] = i;
+    }
+}
+
+// Each block takes a contiguous range of units (kGroupUnit entries of one guide each) and reloads
+// the weights only when the guide changes; the order of entries inside a group does not matter
+// (every mixture depends on its own entry only).
+__global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSource src, uint32_t nKeys, GroupBuffers gb,
+                                                                  const uint16_t* const* __restrict__ keyWb,
+                                                                  const float* const* __restrict__ keyTheta,
+                                                                  float* __restrict__ mix) {
+    extern __shared__ unsigned char smem[];
+    const GuideLds L = guideLdsLayout(smem);
+    float4* stage = reinterpret_cast<float4*>(smem + kGLdsWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
+    const uint32_t units = gb.keyStart[nKeys] / kGroupUnit;
+    const uint32_t per = (units + gridDim.x - 1) / gridDim.x;
+    const uint32_t u0 = blockIdx.x * per, u1 = min(units, u0 + per);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    uint32_t key = 0, loaded = ~0u;
+    for (uint32_t u = u0; u < u1; ++u) {  // block-uniform
+        const uint32_t pos0 = u * kGroupUnit;
+        while (key + 1 < nKeys && gb.keyStart[key + 1] <= pos0) ++key;
+        if (key != loaded) {
+            __syncthreads();  // every wave is done with the previous weights
+            guideLoadLds(L, keyWb[key], keyTheta[key]);
+            __syncthreads();
+            loaded = key;
         }
-        waveLdsSync();  // stage reads done before the next samples overwrite it
+        const uint32_t pos = pos0 + wave * 64 + lane;
+        inferWave64(src, pos < gb.keyEnd[key] ? gb.perm[pos] : ~0u, L, stage, mix);
     }
 }
 
@@ -372,6 +475,30 @@ void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& p
                        capacity, count, wb, theta, mix);
 }
 
+size_t groupBufferBytes(uint32_t capacity) {
+    return (4 * kMaxBatchGuides + 1 + static_cast<size_t>(capacity) + kMaxBatchGuides * kGroupUnit) * 4;
+}
+
+GroupBuffers groupBuffers(void* base, uint32_t capacity) {
+    uint32_t* p = static_cast<uint32_t*>(base);
+    (void)capacity;
+    return GroupBuffers{p, p + kMaxBatchGuides, p + 2 * kMaxBatchGuides + 1, p + 3 * kMaxBatchGuides + 1,
+                        p + 4 * kMaxBatchGuides + 1};
+}
+
+void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathState& ps, const uint32_t* queue,
+                             const uint32_t* count, uint32_t capacity, uint32_t nKeys, const GroupBuffers& gb,
+                             const uint16_t* const* keyWb, const float* const* keyTheta, float* mix,
+                             uint32_t maxBlocks) {
+    const BatchPathSource src{bt.scenes, bt.guide, ps.p, queue};
+    const uint32_t sortBlocks = gridFor(capacity, 256, maxBlocks * 2);
+    hipLaunchKernelGGL(kGroupCount, dim3(sortBlocks), dim3(256), 0, st, src, count, nKeys, gb.keyCount);
+    hipLaunchKernelGGL(kGroupScan, dim3(1), dim3(kMaxBatchGuides), 0, st, nKeys, gb);
+    hipLaunchKernelGGL(kGroupScatter, dim3(sortBlocks), dim3(256), 0, st, src, count, gb);
+    hipLaunchKernelGGL(kGuideInferGrouped, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds,
+                       st, src, nKeys, gb, keyWb, keyTheta, mix);
+}
+
 GuideBatchLayout guideBatchLayout(uint32_t n) {
     const uint32_t n64 = (n + 63) / 64 * 64;
     const uint32_t perChunk = (n64 + kGMaxChunks - 1) / kGMaxChunks;
@@ -431,6 +558,8 @@ void guideConfigureLds() {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<ArraySource>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<PathSource>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInferGrouped),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideTrainFB), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(kTrainLds));

@@ -44,6 +44,24 @@ void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const ui
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks);
+// Guided batches: the extension queue mixes scenes with different guides. Each iteration groups
+// the queue entries by guide key (count, scan with every group padded to kGroupUnit entries,
+// scatter of entry indices), then every block runs units of one guide with that guide's weights in
+// LDS. Record i still belongs to queue entry i. Workspace: GroupBuffers, sized for `capacity`.
+constexpr uint32_t kGroupUnit = 512;
+struct GroupBuffers {
+    uint32_t* keyCount;  // kMaxBatchGuides: entries per key (zeroed again by the scan)
+    uint32_t* keyStart;  // kMaxBatchGuides + 1: padded group starts; [nKeys] = padded total
+    uint32_t* keyEnd;    // kMaxBatchGuides: end of each group's real entries
+    uint32_t* cursor;    // kMaxBatchGuides
+    uint32_t* perm;      // capacity + kMaxBatchGuides * kGroupUnit queue-entry indices
+};
+size_t groupBufferBytes(uint32_t capacity);
+GroupBuffers groupBuffers(void* base, uint32_t capacity);
+void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathState& ps, const uint32_t* queue,
+                             const uint32_t* count, uint32_t capacity, uint32_t nKeys, const GroupBuffers& gb,
+                             const uint16_t* const* keyWb, const float* const* keyTheta, float* mix,
+                             uint32_t maxBlocks);
 // Forward + backward over n training records (4 float4 each: (pos01, c) (normal) (wo) (direction)),
 // then the split-K weight gradients into partial[chunks][kGParamCount].
 void launchGuideGradients(hipStream_t st, const float* rec, uint32_t n, const GuideBatchLayout& b, const uint16_t* wb,

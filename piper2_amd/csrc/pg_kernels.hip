@@ -190,13 +190,17 @@ constexpr uint32_t kHistFloat4 = 6;
 __device__ __forceinline__ bool isTrainPath(const GuideShade& gs, uint32_t g) {
     return gs.train && (g % gs.trainStride) == gs.trainPhase;
 }
+// The same for path g of an item of a guided batch.
+__device__ __forceinline__ bool isTrainPath(const ItemGuide& ig, uint32_t g) {
+    return ig.tr.capacity > 0 && (g % ig.trainStride) == ig.trainPhase;
+}
 
 // GGX lobes narrower than this (alpha = roughness^2 < 0.04) are sampled better by the BSDF than by
 // any 8-lobe mixture the network can express: such vertices draw from the BSDF alone (the guide
 // fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
 constexpr float kGuideMinAlpha = 0.04f;
-__device__ __forceinline__ float guideFraction(const GuideShade& gs, const Mat& m) {
-    return (m.type == 0u || m.alpha >= kGuideMinAlpha) ? gs.alpha : 0.0f;
+__device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
+    return (m.type == 0u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
 }
 
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
@@ -347,7 +351,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     ss = bsdfSample(mat, wo, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
-                    a = guideFraction(gs, mat);
+                    a = guideFraction(BATCH ? bt.guide[sceneOf(meta.z)].alpha : gs.alpha, mat);
                     bool ok = true;
                     if (uc1 < a) {
                         const float3 w = mixtureSample(mix, uc1 / a, uc2);
@@ -388,11 +392,14 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         meta.z &= ~1u;  // no longer a camera / specular ray
                         pushExt = true;
                         const uint32_t r = (meta.z >> 8) & 0xFFu;
-                        if (GUIDED && a > 0.0f && isTrainPath(gs, meta.w) && r < gs.histDepth) {
+                        const bool train = BATCH ? isTrainPath(bt.guide[sceneOf(meta.z)], meta.w) : isTrainPath(gs, meta.w);
+                        if (GUIDED && a > 0.0f && train && r < gs.histDepth) {
                             float4* h = gs.hist + (static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4;
-                            const float px = fminf(fmaxf((hp.x - gs.lo.x) * gs.invExt.x, 0.0f), 1.0f);
-                            const float py = fminf(fmaxf((hp.y - gs.lo.y) * gs.invExt.y, 0.0f), 1.0f);
-                            const float pz = fminf(fmaxf((hp.z - gs.lo.z) * gs.invExt.z, 0.0f), 1.0f);
+                            const float3 lo = BATCH ? bt.guide[sceneOf(meta.z)].lo : gs.lo;
+                            const float3 ie = BATCH ? bt.guide[sceneOf(meta.z)].invExt : gs.invExt;
+                            const float px = fminf(fmaxf((hp.x - lo.x) * ie.x, 0.0f), 1.0f);
+                            const float py = fminf(fmaxf((hp.y - lo.y) * ie.y, 0.0f), 1.0f);
+                            const float pz = fminf(fmaxf((hp.z - lo.z) * ie.z, 0.0f), 1.0f);
                             h[0] = make_float4(px, py, pz, pdfMix);
                             h[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
                             h[2] = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, 0.0f);
@@ -582,7 +589,9 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
             const uint64_t slot = G % ring.size;
             ring.L[slot] = make_float4(L.x, L.y, L.z, fp.x);
             ring.y[slot] = fp.y;
-            if (!BATCH && isTrainPath(gs, g)) {
+            const bool train = BATCH ? (bt.guide != nullptr && isTrainPath(bt.guide[item], g)) : isTrainPath(gs, g);
+            if (train) {
+                const GuideRecordOut out = BATCH ? bt.guide[item].tr : tr;
                 // Each recorded vertex learns from the radiance that reached it along its sampled
                 // direction: (L_final - L after its direct light) / throughput after the bounce.
                 // The loss weights every record by that radiance over its pdf, so records that
@@ -596,12 +605,12 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
                     const float lb = t.z > 0.0f ? (L.z - snap.z) / t.z : 0.0f;
                     const float c = fmaxf((lr + lg + lb) * (1.0f / 3.0f), 0.0f) / h0.w;
                     if (!(c > 0.0f)) continue;
-                    const uint32_t k = atomicAdd(tr.count, 1u);
-                    if (k >= tr.capacity) break;
-                    tr.rec[4 * k] = make_float4(h0.x, h0.y, h0.z, c);
-                    tr.rec[4 * k + 1] = h[1];
-                    tr.rec[4 * k + 2] = h[2];
-                    tr.rec[4 * k + 3] = h[3];
+                    const uint32_t k = atomicAdd(out.count, 1u);
+                    if (k >= out.capacity) break;
+                    out.rec[4 * k] = make_float4(h0.x, h0.y, h0.z, c);
+                    out.rec[4 * k + 1] = h[1];
+                    out.rec[4 * k + 2] = h[2];
+                    out.rec[4 * k + 3] = h[3];
                 }
             }
         }
@@ -803,7 +812,10 @@ void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Q
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
                  const BatchTable* bt) {
     const dim3 grid(gridThreads / 256);
-    if (bt)  // batched renders are never guided
+    if (bt && gs.enabled)  // guided batch: per-item guides in bt.guide
+        hipLaunchKernelGGL((kShade<true, true>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
+                           *bt);
+    else if (bt)
         hipLaunchKernelGGL((kShade<true, false>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
                            *bt);
     else if (gs.enabled)

@@ -47,6 +47,19 @@ struct GuideRecordOut {
     uint32_t capacity;
 };
 
+// Guiding of one item of a guided batch: its scene's guide and the scene's settings. Items of
+// scenes that share a guide share its key, its weights and its training records.
+constexpr uint32_t kMaxBatchGuides = 256;
+struct ItemGuide {
+    const uint16_t* wb;    // the guide's BF16 weights
+    const float* theta;    // and its FP32 parameters (biases)
+    GuideRecordOut tr;     // the guide's training records (capacity 0: no training)
+    float3 lo, invExt;     // the scene's box -> [0,1]^3
+    float alpha;           // probability of sampling the guide
+    uint32_t key;          // index of the guide among the batch's distinct guides
+    uint32_t trainStride, trainPhase;
+};
+
 struct Queues {
     uint32_t* ext[2];  // extension queues (current / next bounce)
     ShadowRec* shadow;
@@ -67,6 +80,7 @@ struct BatchTable {
     const RenderConsts* rc;       // per item: rect, sample sub-range, depth, flags
     const uint64_t* pathOffsets;  // n + 1 entries
     uint32_t n;
+    const ItemGuide* guide;       // per item, in guided batches (nullptr otherwise)
 };
 
 // Film records live in a ring of `ring` slots: path number G writes slot G mod ring.
