@@ -188,6 +188,102 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
     return out
 
 
+def guided_batch(dev, n, bvh_width, scale=8, ref_spp=1 << 15, train_passes=32, train_spp=16, max_level=14,
+                 pass_spp=16):
+    """Hard-indirect batch: n seeded hard rooms (512x512 to 1920x1080, depth 12), every batched call
+    one persistent launch with one guide per room.
+    Throughput: one pass_spp pass of all rooms at their own resolutions, plain batch vs guided batch
+    (guides trained beforehand, frozen). Convergence: the same rooms at 1/scale resolution (same
+    geometry and camera); references of ref_spp samples (FP64 films) by the frozen guides;
+    progressive batched renders (one batch call per doubling level) by the plain tracer and by
+    fresh guides trained online; per room, the batch time at which its film first reaches 1% relMSE."""
+    import math
+    import numpy as np
+    import piper2_amd as pg
+    from piper2_amd import scenes
+    t0 = time.perf_counter()
+    full = [scenes.hard_room_variant(k) for k in range(n)]
+    small = [scenes.hard_room_variant(k, scale=scale) for k in range(n)]
+    objs = [pg.Scene(dev, sc, bvh_width=bvh_width) for sc in full]  # small differs only in resolution
+    films = [pg.Film(dev, sc["width"], sc["height"]) for sc in small]
+    log(f"guided batch: {n} rooms, {sum(sc['indices'].shape[0] for sc in full)} triangles, built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    R = 1 << 16
+
+    def params(scs, rng, flags=0):
+        return [pg.render_params(sc, spp=1 << 18, spp_range=rng, flags=flags) for sc in scs]
+
+    guides = [pg.Guide(dev, seed=100 + k) for k in range(n)]
+    t0 = time.perf_counter()
+    for p in range(train_passes):
+        pg.render_batch(dev, objs, params(small, (2 * R + train_spp * p, 2 * R + train_spp * (p + 1))), None,
+                        guides=guides)
+    frozen = []
+    for g in guides:
+        f = pg.Guide(dev, seed=0, train_records=0)
+        f.set_params(g.get_params())
+        frozen.append(f)
+        g.close()
+    for s0 in range(R, R + ref_spp, 4096):
+        st, _ = pg.render_batch(dev, objs, params(small, (s0, min(s0 + 4096, R + ref_spp)), pg.PG_FILM_FP64),
+                                films, guides=frozen)
+        log(f"guided batch: reference samples {s0 - R}..{min(s0 + 4096, R + ref_spp) - R} in {st['ms_total']:.0f} ms")
+    refs = [_img(f.download(fp64=True)) for f in films]
+    log(f"guided batch: guides trained and {ref_spp}-spp references in {time.perf_counter() - t0:.1f}s")
+
+    def progressive(gs):
+        for f in films:
+            f.clear()
+        t, prev, hit = 0.0, [None] * n, [None] * n
+        for k in range(max_level + 1):
+            rng = (0, 1) if k == 0 else (1 << (k - 1), 1 << k)
+            st, _ = pg.render_batch(dev, objs, params(small, rng), films, guides=gs)
+            t += st["ms_total"] / 1e3
+            log(f"guided batch: {'plain' if gs is None else 'guided'} level {k} ({rng[1]} spp) at {t:.3f} s")
+            for i, (f, ref) in enumerate(zip(films, refs)):
+                e = _rel_mse(_img(f.download()), ref)
+                if hit[i] is None and e <= 0.01:
+                    if prev[i] is None:
+                        hit[i] = t
+                    else:
+                        (tp, ep) = prev[i]
+                        fr = (math.log(0.01) - math.log(ep)) / (math.log(e) - math.log(ep))
+                        hit[i] = math.exp(math.log(tp) + fr * (math.log(t) - math.log(tp)))
+                prev[i] = (t, e)
+            if all(h is not None for h in hit):
+                break
+        return hit, t
+
+    t_plain, clock_plain = progressive(None)
+    online = [pg.Guide(dev, seed=200 + k) for k in range(n)]
+    t_guided, clock_guided = progressive(online)
+    log(f"guided batch: time to 1% per room, plain {t_plain}; guided {t_guided}")
+    both = [(a, b) for a, b in zip(t_plain, t_guided) if a is not None and b is not None]
+    out = {"rooms": n, "resolutions": [[sc["width"], sc["height"]] for sc in full], "depth": 12,
+           "convergence_scale": f"1/{scale} resolution", "reference": f"{ref_spp}-spp FP64, frozen trained guides",
+           "plain_s_to_1pct": [None if v is None else round(v, 4) for v in t_plain],
+           "guided_s_to_1pct": [None if v is None else round(v, 4) for v in t_guided],
+           "plain_all_rooms_s": round(max(v for v in t_plain if v is not None), 4) if all(v is not None for v in t_plain) else None,
+           "guided_all_rooms_s": round(max(v for v in t_guided if v is not None), 4) if all(v is not None for v in t_guided) else None}
+    if both:
+        out["median_speedup"] = round(float(np.median([a / b for a, b in both])), 3)
+    # throughput at the rooms' own resolutions
+    ffull = [pg.Film(dev, sc["width"], sc["height"]) for sc in full]
+    tp = {}
+    for name, gs in (("plain", None), ("guided", frozen)):
+        st, lat = pg.render_batch(dev, objs, params(full, (0, pass_spp)), ffull, guides=gs)  # warm
+        st, lat = pg.render_batch(dev, objs, params(full, (pass_spp, 2 * pass_spp)), ffull, guides=gs)
+        tp[name] = {"mpaths_per_s": round(st["paths"] / st["ms_total"] / 1e3, 1), "ms": round(st["ms_total"], 2),
+                    "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+                    "latency_ms_p99": round(float(np.percentile(lat, 99)), 2)}
+    out["pass_spp"] = pass_spp
+    out["throughput"] = tp
+    log(f"guided batch: {out}")
+    for o in ffull + films + objs + frozen + online:
+        o.close()
+    return out
+
+
 def batched(dev, n_scenes, steps, bvh_width, wavefront_paths):
     """Batched workload: n seeded scenes (log-uniform 10k-2M triangles, mixed resolutions, spp
     4/16/64, depth 8) in one pg_render_batch call; aggregate Mrays/s and per-scene latency
@@ -255,6 +351,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--guided-batch", type=int, default=16, help="hard rooms of the guided batch leg (0: skip it)")
     ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -374,6 +471,8 @@ def main():
             "terrain_slit_1920x1080_depth8": guided_vs_plain(dev, scenes.terrain_slit(), args.ref_spp // 16,
                                                              "terrain slit", max_level=9),
         }
+    if args.guided_batch > 0:
+        result["guided_batch_hard_rooms"] = guided_batch(dev, args.guided_batch, info["width"])
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(sc, threads)

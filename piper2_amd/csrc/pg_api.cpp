@@ -611,7 +611,6 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
             keyTheta = dev->keyTheta.as<const float*>();
             dev->groupBuf.reserve(pg::groupBufferBytes(cap));
             gb = pg::groupBuffers(dev->groupBuf.p, cap);
-            PG_HIP(hipMemsetAsync(gb.keyCount, 0, pg::kMaxBatchGuides * 4, st));  // the scan clears it afterwards
         }
         pg::guideConfigureLds();
     }

@@ -162,50 +162,80 @@ struct BatchPathSource {
     }
 };
 
+// Contiguous share [lo, hi) of the n queue entries for sort block blockIdx.x (of kSortBlocks).
+__device__ __forceinline__ void sortShare(uint32_t n, uint32_t& lo, uint32_t& hi) {
+    const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+    lo = min(n, blockIdx.x * chunk);
+    hi = min(n, lo + chunk);
+}
+
 __global__ void __launch_bounds__(256) kGroupCount(BatchPathSource src, const uint32_t* count, uint32_t nKeys,
-                                                   uint32_t* keyCount) {
+                                                   GroupBuffers gb) {
     __shared__ uint32_t hist[kMaxBatchGuides];
     for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) hist[k] = 0;
     __syncthreads();
-    const uint32_t n = *count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        atomicAdd(&hist[src.key(i)], 1u);
+    uint32_t lo, hi;
+    sortShare(*count, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&hist[src.key(i)], 1u);  // LDS
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x)
-        if (hist[k]) atomicAdd(&keyCount[k], hist[k]);
+    for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) gb.blockCount[k * kSortBlocks + blockIdx.x] = hist[k];
 }
 
-// One block: padded exclusive scan of the key counts; clears the counts for the next iteration.
-__global__ void __launch_bounds__(kMaxBatchGuides) kGroupScan(uint32_t nKeys, GroupBuffers gb) {
-    __shared__ uint32_t pad[kMaxBatchGuides];
-    const uint32_t k = threadIdx.x;
-    const uint32_t c = k < nKeys ? gb.keyCount[k] : 0u;
-    pad[k] = (c + kGroupUnit - 1) / kGroupUnit * kGroupUnit;
-    __syncthreads();
-    if (k == 0) {
-        uint32_t acc = 0;
-        for (uint32_t j = 0; j < nKeys; ++j) {
-            const uint32_t p = pad[j];
-            pad[j] = acc;
-            acc += p;
+// One block of kSortBlocks threads (thread b = sort block b): per key, the exclusive scan of the
+// blocks' counts (wave scans, then the waves' totals); keys laid out one after another, each group
+// padded to kGroupUnit entries.
+__global__ void __launch_bounds__(kSortBlocks) kGroupScan(uint32_t nKeys, GroupBuffers gb) {
+    static_assert(kSortBlocks == 256 && kMaxBatchGuides == 256, "one thread per sort block and per key");
+    __shared__ uint32_t waveSum[4][kMaxBatchGuides];
+    __shared__ uint32_t keyBase[kMaxBatchGuides];
+    const uint32_t b = threadIdx.x, lane = __lane_id(), wave = b / 64;
+    for (uint32_t k = 0; k < nKeys; ++k) {
+        const uint32_t c = gb.blockCount[k * kSortBlocks + b];
+        uint32_t x = c;  // inclusive scan within the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= static_cast<uint32_t>(off)) x += y;
         }
-        gb.keyStart[nKeys] = acc;
+        gb.blockBase[k * kSortBlocks + b] = x - c;
+        if (lane == 63) waveSum[wave][k] = x;
     }
     __syncthreads();
-    if (k < nKeys) {
-        gb.keyStart[k] = pad[k];
-        gb.keyEnd[k] = pad[k] + c;
-        gb.cursor[k] = 0;
-        gb.keyCount[k] = 0;
+    if (b < nKeys) {  // thread k: the waves' offsets within key k, and key k's total
+        uint32_t acc = 0;
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t t = waveSum[w][b];
+            waveSum[w][b] = acc;
+            acc += t;
+        }
+        keyBase[b] = acc;
     }
+    __syncthreads();
+    if (b == 0) {  // group starts, each group padded to whole units
+        uint32_t start = 0;
+        for (uint32_t k = 0; k < nKeys; ++k) {
+            const uint32_t total = keyBase[k];
+            gb.keyStart[k] = start;
+            gb.keyEnd[k] = start + total;
+            keyBase[k] = start;
+            start += (total + kGroupUnit - 1) / kGroupUnit * kGroupUnit;
+        }
+        gb.keyStart[nKeys] = start;
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < nKeys; ++k) gb.blockBase[k * kSortBlocks + b] += keyBase[k] + waveSum[wave][k];
 }
 
-__global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource src, const uint32_t* count, GroupBuffers gb) {
-    const uint32_t n = *count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t k = src.key(i);
-        gb.perm[gb.keyStart[k] + atomicAdd(&gb.cursor[k], 1u)] = i;
-    }
+// The same shares as kGroupCount; inside a block the entries of one key take consecutive slots
+// from the block's base (LDS atomics: the order inside a group does not matter).
+__global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource src, const uint32_t* count, uint32_t nKeys,
+                                                     GroupBuffers gb) {
+    __shared__ uint32_t next[kMaxBatchGuides];
+    for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) next[k] = gb.blockBase[k * kSortBlocks + blockIdx.x];
+    __syncthreads();
+    uint32_t lo, hi;
+    sortShare(*count, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) gb.perm[atomicAdd(&next[src.key(i)], 1u)] = i;
 }
 
 // Each block takes a contiguous range of units (kGroupUnit entries of one guide each) and reloads
@@ -476,14 +506,16 @@ void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& p
 }
 
 size_t groupBufferBytes(uint32_t capacity) {
-    return (4 * kMaxBatchGuides + 1 + static_cast<size_t>(capacity) + kMaxBatchGuides * kGroupUnit) * 4;
+    return (2 * static_cast<size_t>(kMaxBatchGuides) * kSortBlocks + 2 * kMaxBatchGuides + 1 + capacity +
+            static_cast<size_t>(kMaxBatchGuides) * kGroupUnit) * 4;
 }
 
 GroupBuffers groupBuffers(void* base, uint32_t capacity) {
     uint32_t* p = static_cast<uint32_t*>(base);
     (void)capacity;
-    return GroupBuffers{p, p + kMaxBatchGuides, p + 2 * kMaxBatchGuides + 1, p + 3 * kMaxBatchGuides + 1,
-                        p + 4 * kMaxBatchGuides + 1};
+    const size_t table = static_cast<size_t>(kMaxBatchGuides) * kSortBlocks;
+    return GroupBuffers{p, p + table, p + 2 * table, p + 2 * table + kMaxBatchGuides + 1,
+                        p + 2 * table + 2 * kMaxBatchGuides + 1};
 }
 
 void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathState& ps, const uint32_t* queue,
@@ -491,10 +523,9 @@ void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathSta
                              const uint16_t* const* keyWb, const float* const* keyTheta, float* mix,
                              uint32_t maxBlocks) {
     const BatchPathSource src{bt.scenes, bt.guide, ps.p, queue};
-    const uint32_t sortBlocks = gridFor(capacity, 256, maxBlocks * 2);
-    hipLaunchKernelGGL(kGroupCount, dim3(sortBlocks), dim3(256), 0, st, src, count, nKeys, gb.keyCount);
-    hipLaunchKernelGGL(kGroupScan, dim3(1), dim3(kMaxBatchGuides), 0, st, nKeys, gb);
-    hipLaunchKernelGGL(kGroupScatter, dim3(sortBlocks), dim3(256), 0, st, src, count, gb);
+    hipLaunchKernelGGL(kGroupCount, dim3(kSortBlocks), dim3(256), 0, st, src, count, nKeys, gb);
+    hipLaunchKernelGGL(kGroupScan, dim3(1), dim3(kSortBlocks), 0, st, nKeys, gb);
+    hipLaunchKernelGGL(kGroupScatter, dim3(kSortBlocks), dim3(256), 0, st, src, count, nKeys, gb);
     hipLaunchKernelGGL(kGuideInferGrouped, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds,
                        st, src, nKeys, gb, keyWb, keyTheta, mix);
 }

@@ -45,16 +45,18 @@ void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& p
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks);
 // Guided batches: the extension queue mixes scenes with different guides. Each iteration groups
-// the queue entries by guide key (count, scan with every group padded to kGroupUnit entries,
-// scatter of entry indices), then every block runs units of one guide with that guide's weights in
-// LDS. Record i still belongs to queue entry i. Workspace: GroupBuffers, sized for `capacity`.
+// the queue entries by guide key without global atomics: kSortBlocks blocks count their
+// contiguous share of the queue per key, one block scans those counts (every group padded to
+// kGroupUnit entries), and the same blocks scatter their entry indices; then every inference block
+// runs units of one guide with that guide's weights in LDS. Record i still belongs to queue entry i.
 constexpr uint32_t kGroupUnit = 512;
+constexpr uint32_t kSortBlocks = 256;
 struct GroupBuffers {
-    uint32_t* keyCount;  // kMaxBatchGuides: entries per key (zeroed again by the scan)
-    uint32_t* keyStart;  // kMaxBatchGuides + 1: padded group starts; [nKeys] = padded total
-    uint32_t* keyEnd;    // kMaxBatchGuides: end of each group's real entries
-    uint32_t* cursor;    // kMaxBatchGuides
-    uint32_t* perm;      // capacity + kMaxBatchGuides * kGroupUnit queue-entry indices
+    uint32_t* blockCount;  // [kMaxBatchGuides][kSortBlocks]: entries of key k in sort block b
+    uint32_t* blockBase;   // same shape: where sort block b's entries of key k start in perm
+    uint32_t* keyStart;    // kMaxBatchGuides + 1: padded group starts; [nKeys] = padded total
+    uint32_t* keyEnd;      // kMaxBatchGuides: end of each group's real entries
+    uint32_t* perm;        // capacity + kMaxBatchGuides * kGroupUnit queue-entry indices
 };
 size_t groupBufferBytes(uint32_t capacity);
 GroupBuffers groupBuffers(void* base, uint32_t capacity);

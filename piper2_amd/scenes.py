@@ -343,6 +343,18 @@ def hard_room(seed=0, width=512, height=512, spp=64, max_depth=12, tess=204):
                     name=f"hard_room_{seed}")
 
 
+HARD_ROOM_RESOLUTIONS = ((512, 512), (1024, 768), (1280, 720), (1920, 1080))
+
+
+def hard_room_variant(index, scale=1, spp=64, tess=204):
+    """Variant `index` of the guided batch workload (16 seeded hard rooms): hard_room(seed=index) at a
+    resolution drawn from HARD_ROOM_RESOLUTIONS (512x512 to 1920x1080) by seed 500 + index, divided
+    by `scale` in both directions; max depth 12."""
+    rng = np.random.default_rng(500 + index)
+    width, height = HARD_ROOM_RESOLUTIONS[int(rng.integers(len(HARD_ROOM_RESOLUTIONS)))]
+    return hard_room(seed=index, width=width // scale, height=height // scale, spp=spp, max_depth=12, tess=tess)
+
+
 def hidden_cornell(width=64, height=64, spp=64, max_depth=6):
     """Small hard-indirect test scene: the Cornell box with a plate under the ceiling light, so the
     room is lit only after the light bounces off the ceiling."""
