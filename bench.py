@@ -102,6 +102,45 @@ def time_to_relmse(sc, film, ref, render, target=0.01, max_level=14):
     return hit, trace
 
 
+GUIDE_FLOP_PER_SAMPLE = 2 * (139 * 64 + 3 * 64 * 64 + 64 * 40)  # unpadded MLP forward: 47,488
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def guide_roofline(st):
+    """Guide inference of one guided render against MI355X's dense BF16 matrix-core peak: one
+    forward per extension-queue entry (= extension ray), algorithmic FLOPs over the event-timed
+    inference stage."""
+    if not st.get("ms_guide"):
+        return None
+    tflops = st["extension_rays"] * GUIDE_FLOP_PER_SAMPLE / (st["ms_guide"] / 1e3) / 1e12
+    return {"bound": "mfma", "kernel": "kGuideInfer", "achieved": round(tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4), "samples": st["extension_rays"],
+            "ms": round(st["ms_guide"], 3), "flop_per_sample": GUIDE_FLOP_PER_SAMPLE}
+
+
+def guided_profile():
+    """Profiling workload of the guided path (for rocprofv3): the 256x256 hard room, 16 online
+    training passes of 16 spp, then one 1024-spp render with the trained guide frozen and one plain
+    1024-spp render; prints one JSON line with their stage times."""
+    import piper2_amd as pg
+    from piper2_amd import scenes
+    dev = pg.Device(0)
+    sc = scenes.hard_room(seed=0, width=256, height=256)
+    scene = pg.Scene(dev, sc)
+    film = pg.Film(dev, 256, 256)
+    g = pg.Guide(dev, seed=1)
+    base = 1 << 17  # training samples: disjoint from the measured renders' [0, 1024)
+    for p in range(16):
+        scene.render_guided(g, pg_params(sc, (base + 16 * p, base + 16 * (p + 1))), None)
+    frozen = pg.Guide(dev, seed=1, train_records=0)
+    frozen.set_params(g.get_params())
+    guided = scene.render_guided(frozen, pg_params(sc, (0, 1024)), film)
+    plain = scene.render(pg_params(sc, (0, 1024)), film)
+    keys = ("ms_total", "ms_raygen", "ms_extend", "ms_guide", "ms_shade", "ms_shadow", "ms_film", "extension_rays")
+    print(json.dumps({"workload": "hard_room_256x256_1024spp", "guided": {k: guided[k] for k in keys},
+                      "plain": {k: plain[k] for k in keys}, "inference_roofline": guide_roofline(guided)}), flush=True)
+
+
 def equal_time(trace_a, trace_b):
     """relMSE of trace b at the times of trace a's levels (log-log interpolation between b's levels),
     as (seconds, relMSE a, relMSE b) for every time both traces cover."""
@@ -160,8 +199,9 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
         scene.render(pg_params(sc, (0, spp)), film)
         e_plain = _rel_mse(_img(film.download()), ref)
         film.clear()
-        scene.render_guided(frozen, pg_params(sc, (0, spp)), film)
+        st = scene.render_guided(frozen, pg_params(sc, (0, spp)), film)
         equal_spp[str(spp)] = {"plain": round(e_plain, 5), "trained_guide": round(_rel_mse(_img(film.download()), ref), 5)}
+    inference = guide_roofline(st)
     frozen.close()
     t_plain, tr_plain = time_to_relmse(sc, film, ref, lambda p: scene.render(p, film), max_level=max_level)
     guide = pg.Guide(dev, seed=1, lr=1e-2, mis_fraction=0.5, train_batch=1 << 14, train_records=1 << 18)
@@ -174,6 +214,7 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
            "reference_vs_plain_fp64_relmse": round(_rel_mse(ref_plain, ref), 5),
            "reference_mean": round(float(ref.mean()), 5),
            "resolution": [sc["width"], sc["height"]], "relmse_at_equal_spp": equal_spp,
+           "inference_roofline": inference,
            "plain_trace_spp_s_relmse": tr_plain, "guided_trace_spp_s_relmse": tr_guided}
     if t_plain and t_guided:
         out["guided_speedup"] = round(t_plain / t_guided, 3)
@@ -352,6 +393,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--guided-batch", type=int, default=16, help="hard rooms of the guided batch leg (0: skip it)")
+    ap.add_argument("--guided-profile", action="store_true", help="only the guided profiling workload (for rocprofv3)")
     ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -359,6 +401,9 @@ def main():
               "scenes with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.guided_profile:
+        guided_profile()
+        return
     import piper2_amd as pg
     from piper2_amd import scenes
 
