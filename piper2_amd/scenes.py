@@ -227,7 +227,9 @@ def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr
 
 def terrain_slit(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500)):
     """Moderate indirect (guided config 4): the terrain + icosphere scene with its 16 lights moved
-    behind a wall whose only opening is a 0.3-wide vertical slit at x = 0."""
+    behind a wall (z = -2.5, 6 high) whose only opening is a 0.3-high slot across its whole width at
+    height 1.0-1.3, below the lights (2-3 high): the light falls through it onto a strip of terrain
+    just in front of the wall, and the rest of the scene is lit mostly by that strip (one bounce)."""
     sc = terrain_spheres(seed=seed, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=min_rr,
                          grid=grid)
     rng = np.random.default_rng(seed + 1000)
@@ -235,8 +237,8 @@ def terrain_slit(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3,
     b.v, b.t, b.obj = [sc["vertices"].astype(np.float64)], [sc["indices"].astype(np.int64)], [sc["tri_object"]]
     b.objmat, b.mats, b.nv = list(sc["object_material"]), [list(m) for m in sc["materials"]], sc["vertices"].shape[0]
     wall = b.material(LAMBERT, [0.6, 0.6, 0.6])
-    for x0, x1 in ((-10.5, -0.15), (0.15, 10.5)):
-        v, t = _box(((x0 + x1) / 2, -2.5), -1.0, (x1 - x0, 6.0, 0.2), 0.0)
+    for y0, y1 in ((-1.0, 1.0), (1.3, 5.0)):
+        v, t = _box((0.0, -2.5), y0, (21.0, y1 - y0, 0.2), 0.0)
         b.add(v, t, wall)
     quads = []
     for _ in range(16):
