@@ -646,36 +646,49 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
 
 // Per source pixel of the rect: sums over its samples for each footprint origin
 // (bx, by) in {px-1, px} x {py-1, py}; quad[4 * pixel + 2 * (by - py + 1) + (bx - px + 1)] = (rgb, count).
+// A group of G lanes (G = sppCount rounded up to a power of two, at most 64) reduces one pixel:
+// lane l takes samples l, l + G, ... (the group reads consecutive records, so each load is
+// coalesced), then a butterfly over the group's lanes adds the partial sums in a fixed order.
 template <typename F>
 __global__ void __launch_bounds__(256) kQuadSums(RenderConsts rc, const float4* __restrict__ recL,
                                                  const float* __restrict__ recY, uint64_t start, uint64_t ringSize,
-                                                 F* quad) {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= rc.rectW * rc.rectH) return;
-    const int32_t px = static_cast<int32_t>(rc.rectX + pix % rc.rectW);
-    const int32_t py = static_cast<int32_t>(rc.rectY + pix / rc.rectW);
+                                                 uint32_t groupLanes, F* quad) {
+    const uint32_t G = groupLanes;
+    const uint32_t lane = __lane_id(), l = lane % G;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t pix = wave * (64 / G) + lane / G;
+    const bool act = pix < rc.rectW * rc.rectH;  // no early exit: every lane joins the butterfly
     F acc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = F(0);
-    const uint64_t base = start + static_cast<uint64_t>(pix) * rc.sppCount;  // start < ringSize
-    for (uint32_t si = 0; si < rc.sppCount; ++si) {
-        uint64_t j = base + si;
-        if (j >= ringSize) j -= ringSize;  // a segment is never longer than the ring
-        const float4 r = recL[j];
-        const int32_t bx = static_cast<int32_t>(r.w - 0.5f), by = static_cast<int32_t>(recY[j] - 0.5f);
-        const int q = 2 * (by - py + 1) + (bx - px + 1);  // 0..3 (bx in {px-1, px}, by in {py-1, py})
+    if (act) {
+        const int32_t px = static_cast<int32_t>(rc.rectX + pix % rc.rectW);
+        const int32_t py = static_cast<int32_t>(rc.rectY + pix / rc.rectW);
+        const uint64_t base = start + static_cast<uint64_t>(pix) * rc.sppCount;  // start < ringSize
+        for (uint32_t si = l; si < rc.sppCount; si += G) {
+            uint64_t j = base + si;
+            if (j >= ringSize) j -= ringSize;  // a segment is never longer than the ring
+            const float4 r = recL[j];
+            const int32_t bx = static_cast<int32_t>(r.w - 0.5f), by = static_cast<int32_t>(recY[j] - 0.5f);
+            const int q = 2 * (by - py + 1) + (bx - px + 1);  // 0..3 (bx in {px-1, px}, by in {py-1, py})
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (q == k) {
-                acc[4 * k] += static_cast<F>(r.x);
-                acc[4 * k + 1] += static_cast<F>(r.y);
-                acc[4 * k + 2] += static_cast<F>(r.z);
-                acc[4 * k + 3] += F(1);
-            }
+            for (int k = 0; k < 4; ++k)
+                if (q == k) {
+                    acc[4 * k] += static_cast<F>(r.x);
+                    acc[4 * k + 1] += static_cast<F>(r.y);
+                    acc[4 * k + 2] += static_cast<F>(r.z);
+                    acc[4 * k + 3] += F(1);
+                }
+        }
     }
-    F* o = quad + 16 * static_cast<size_t>(pix);
+    for (uint32_t off = G / 2; off > 0; off >>= 1)
 #pragma unroll
-    for (int k = 0; k < 16; ++k) o[k] = acc[k];
+        for (int k = 0; k < 16; ++k) acc[k] += __shfl_xor(acc[k], static_cast<int>(off));
+    if (act && l == 0) {
+        F* o = quad + 16 * static_cast<size_t>(pix);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) o[k] = acc[k];
+    }
 }
 
 // Per film pixel around the rect (rect grown by one pixel, clipped to the film).
@@ -860,14 +873,17 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
     const uint32_t pixels = rc.rectW * rc.rectH;
     const uint32_t targets = (rc.rectW + 2) * (rc.rectH + 2);
     const uint64_t start = first % ring.size;
+    uint32_t g = 1;  // lanes per pixel in kQuadSums
+    while (g < 64 && g < rc.sppCount) g <<= 1;
+    const uint32_t sumBlocks = static_cast<uint32_t>((static_cast<uint64_t>(pixels) * g + 255) / 256);
     if (film64) {
-        hipLaunchKernelGGL(kQuadSums<double>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, ring.L,
-                           ring.y, start, ring.size, static_cast<double*>(quad));
+        hipLaunchKernelGGL(kQuadSums<double>, dim3(sumBlocks), dim3(256), 0, st, rc, ring.L, ring.y, start, ring.size, g,
+                           static_cast<double*>(quad));
         hipLaunchKernelGGL(kResolve<double>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
                            static_cast<const double*>(quad), film64);
     } else {
-        hipLaunchKernelGGL(kQuadSums<float>, dim3(blocksFor(pixels, 256, 1u << 30)), dim3(256), 0, st, rc, ring.L,
-                           ring.y, start, ring.size, static_cast<float*>(quad));
+        hipLaunchKernelGGL(kQuadSums<float>, dim3(sumBlocks), dim3(256), 0, st, rc, ring.L, ring.y, start, ring.size, g,
+                           static_cast<float*>(quad));
         hipLaunchKernelGGL(kResolve<float>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
                            static_cast<const float*>(quad), film32);
     }
