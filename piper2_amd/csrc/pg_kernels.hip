@@ -245,9 +245,12 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             const bool envBlack = !valid(ld3(s.env));
             Ray ray{xyz(ps.p[path].o), xyz(ps.p[path].d)};
             V3 thr = xyz(ps.p[path].thr);
-            // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them)
-            V3 L = xyz(ps.p[path].L) + xyz(ps.p[path].Lmis);
-            ps.p[path].Lmis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them);
+            // L and Lmis are written back only when they change (adding a zero leaves L's bits)
+            const float4 lmis = ps.p[path].Lmis;
+            bool lDirty = lmis.x != 0.0f || lmis.y != 0.0f || lmis.z != 0.0f;
+            V3 L = xyz(ps.p[path].L) + xyz(lmis);
+            if (lDirty) ps.p[path].Lmis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (GUIDED && (meta.z & kMetaPending)) {  // the previous guided vertex's direct light is now in L
                 const uint32_t r = ((meta.z >> 8) & 0xFFu) - 1u;
                 gs.hist[(static_cast<size_t>(path) * gs.histDepth + r) * kHistFloat4 + 5] = make_float4(L.x, L.y, L.z, 0.0f);
@@ -258,6 +261,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             if (specular && (isLight || pos == kHitMiss)) {  // :119-140, two-sided emission
                 const V3 rad = isLight ? ld3(s.quads[pos & ~kHitQuad].radiance) : ld3(s.env);
                 L = L + mul(thr, rad);
+                lDirty = true;
             }
             if (isLight) {  // :142-146 pass through the light
                 if (d13) meta.y -= 1u;
@@ -413,7 +417,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 }
                 meta.x += used;
             }
-            ps.p[path].L = make_float4(L.x, L.y, L.z, 0.0f);
+            if (lDirty) ps.p[path].L = make_float4(L.x, L.y, L.z, 0.0f);
             ps.p[path].meta = meta;
         }
         // block-wide compaction: per-wave counts, one atomic per queue, then every lane's slot
