@@ -38,6 +38,10 @@ extern "C" {
 #define PG_USE_GRAPH 0x800u /* capture the bounce loop's kernels once per call, replay them as hipGraph
                                launches (implies PG_NO_STAGE_TIMING) */
 
+#define PG_FILTER_BOX 0u      /* weight 1 */
+#define PG_FILTER_TRIANGLE 1u /* max(0, 1 - |dx|) max(0, 1 - |dy|) */
+#define PG_FILTER_GAUSSIAN 2u /* max(exp(-a dx^2) - exp(-a), 0) max(exp(-a dy^2) - exp(-a), 0) */
+
 #define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
 #define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
 
@@ -103,7 +107,9 @@ typedef struct pg_render_params {
     uint32_t flags;              /* PG_COMPAT_*, PG_FILM_FP64 */
     float ndc[4];                /* SensorNDCAffineTransform ox, oy, sx, sy (Sensor.hpp:38-74) */
     uint32_t wavefront_paths;    /* paths in flight per wave of work; 0 = library default */
-    uint32_t reserved[3];
+    uint32_t filter;             /* PG_FILTER_*: the FixedSampler film filter (FixedSampler/Kernel.cpp:55-70) */
+    float filter_alpha;          /* PG_FILTER_GAUSSIAN: Alpha (GaussianFilterData, FixedSampler.cpp:40-44) */
+    uint32_t reserved;
 } pg_render_params;
 
 typedef struct pg_stats {

@@ -610,6 +610,8 @@ struct Params {
     uint32_t rectX, rectY, rectW, rectH;
     uint32_t sppBegin, sppEnd;
     float ndc[4];  // SensorNDCAffineTransform {ox, oy, sx, sy}
+    uint32_t filter;        // 0 box, 1 triangle, 2 gaussian (FixedSampler/Kernel.cpp:55-70)
+    float negAlpha, sub;    // GaussianFilterData (FixedSampler.cpp:40-44): -Alpha, exp(-Alpha)
 };
 
 struct Sampler {  // PerSampleContext dimension counter, Embree/Kernel.cpp:37-45
@@ -836,18 +838,30 @@ V3 renderSample(const Scene& s, const Params& p, uint32_t m, uint32_t px, uint32
     return tracePath(s, p, smp, ray, st);
 }
 
-// accumulate with the box filter (RenderDriver/FixedSampler/Kernel.cpp:30-58)
+// Filter weight of a sample at offset (dx, dy) from a pixel centre (FixedSampler/Kernel.cpp:55-70).
+float filterWeight(const Params& p, float dx, float dy) {
+    if (p.filter == 1) return std::fmax(0.0f, 1.0f - std::fabs(dx)) * std::fmax(0.0f, 1.0f - std::fabs(dy));
+    if (p.filter == 2) {
+        const auto w = [&](float d) { return std::fmax(std::exp(p.negAlpha * d * d) - p.sub, 0.0f); };
+        return w(dx) * w(dy);
+    }
+    return 1.0f;
+}
+
+// accumulate (RenderDriver/FixedSampler/Kernel.cpp:30-52): the sample splats to the 2x2 pixels from
+// (trunc(fx - 0.5), trunc(fy - 0.5)) with the filter's weight of its offset from each pixel centre.
 template <typename F>
-void splat(F* rgbw, uint32_t w, uint32_t h, float fx, float fy, V3 L, std::vector<std::atomic<uint32_t>>* locks) {
+void splat(F* rgbw, const Params& p, float fx, float fy, V3 L, std::vector<std::atomic<uint32_t>>* locks) {
     const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
     for (int k = 0; k < 4; ++k) {
         const uint32_t x = static_cast<uint32_t>(bx + (k & 1)), y = static_cast<uint32_t>(by + (k >> 1));
-        if (!(x < w && y < h)) continue;
-        F* px = rgbw + 4 * (static_cast<size_t>(y) * w + x);
-        std::atomic<uint32_t>& lk = (*locks)[(static_cast<size_t>(y) * w + x) & (locks->size() - 1)];
+        if (!(x < p.width && y < p.height)) continue;
+        const float w = filterWeight(p, fx - (static_cast<float>(x) + 0.5f), fy - (static_cast<float>(y) + 0.5f));
+        F* px = rgbw + 4 * (static_cast<size_t>(y) * p.width + x);
+        std::atomic<uint32_t>& lk = (*locks)[(static_cast<size_t>(y) * p.width + x) & (locks->size() - 1)];
         while (lk.exchange(1, std::memory_order_acquire)) {
         }
-        px[0] += L.x, px[1] += L.y, px[2] += L.z, px[3] += 1.0f;
+        px[0] += L.x * w, px[1] += L.y * w, px[2] += L.z * w, px[3] += w;
         lk.store(0, std::memory_order_release);
     }
 }
@@ -956,8 +970,9 @@ void or_camera_rays(void* sp, const uint32_t* iparams /*width,height,spp*/, cons
 }
 
 // Render a rect over samples [sppBegin, sppEnd).
-//   iparams = {width, height, spp, maxDepth, minRR, flags, rectX, rectY, rectW, rectH, sppBegin, sppEnd}
-//   ndc = {ox, oy, sx, sy}; film = RGBW double [height*width*4] (accumulated into, not cleared);
+//   iparams = {width, height, spp, maxDepth, minRR, flags, rectX, rectY, rectW, rectH, sppBegin, sppEnd,
+//              filter (0 box, 1 triangle, 2 gaussian)}
+//   ndc = {ox, oy, sx, sy, gaussian Alpha}; film = RGBW double [height*width*4] (accumulated into);
 //   pathL (optional) = per-path radiance [rectH*rectW*(sppEnd-sppBegin)*3], path order
 //   ((py-rectY)*rectW + (px-rectX))*(sppEnd-sppBegin) + (s-sppBegin);
 //   rayCounts (optional) = {extension, shadow, mis} ray totals.
@@ -969,6 +984,9 @@ void or_render(void* sp, const uint32_t* iparams, const float* ndc, double* film
     p.flags = iparams[5], p.rectX = iparams[6], p.rectY = iparams[7], p.rectW = iparams[8], p.rectH = iparams[9];
     p.sppBegin = iparams[10], p.sppEnd = iparams[11];
     std::memcpy(p.ndc, ndc, sizeof(p.ndc));
+    p.filter = iparams[12];
+    p.negAlpha = -ndc[4];
+    p.sub = std::exp(-ndc[4]);
     uint32_t m = 0;
     while ((1u << m) < std::max(p.width, p.height)) ++m;  // ScrambledSobolSampler.cpp:66-72
     const uint32_t ns = p.sppEnd - p.sppBegin;
@@ -993,7 +1011,7 @@ void or_render(void* sp, const uint32_t* iparams, const float* ndc, double* film
                             float* o = pathL + 3 * ((static_cast<size_t>(yy) * p.rectW + xx) * ns + (si - p.sppBegin));
                             o[0] = L.x, o[1] = L.y, o[2] = L.z;
                         }
-                        if (film) splat(film, p.width, p.height, fx, fy, L, &locks);
+                        if (film) splat(film, p, fx, fy, L, &locks);
                     }
         }
     };

@@ -28,6 +28,9 @@ PG_NO_STAGE_TIMING = 0x400
 PG_USE_GRAPH = 0x800
 PG_BVH_HOST = 0
 PG_BVH_DEVICE = 1
+PG_FILTER_BOX = 0
+PG_FILTER_TRIANGLE = 1
+PG_FILTER_GAUSSIAN = 2
 
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
@@ -71,7 +74,8 @@ class RenderParams(ctypes.Structure):
         ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
         ("rect", ctypes.c_uint32 * 4), ("spp_begin", ctypes.c_uint32), ("spp_end", ctypes.c_uint32),
         ("max_depth", ctypes.c_uint32), ("min_rr", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-        ("ndc", ctypes.c_float * 4), ("wavefront_paths", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3),
+        ("ndc", ctypes.c_float * 4), ("wavefront_paths", ctypes.c_uint32), ("filter", ctypes.c_uint32),
+        ("filter_alpha", ctypes.c_float), ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -445,8 +449,10 @@ def scene_desc(sc):
         cam, F3(*sc["env"]))
 
 
-def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0):
-    """RenderParams for a generated scene: film size, depth and RR from its config."""
+def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0, film_filter=0,
+                  filter_alpha=0.0):
+    """RenderParams for a generated scene: film size, depth and RR from its config; film_filter is
+    PG_FILTER_BOX, PG_FILTER_TRIANGLE or PG_FILTER_GAUSSIAN (with filter_alpha)."""
     w, h = sc["width"], sc["height"]
     spp = spp or sc["spp"]
     rect = rect or (0, 0, w, h)
@@ -458,4 +464,5 @@ def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_pa
     p.max_depth, p.min_rr, p.flags = sc["max_depth"], sc["min_rr"], flags
     p.ndc[:] = list(sc["ndc"])
     p.wavefront_paths = wavefront_paths
+    p.filter, p.filter_alpha = film_filter, filter_alpha
     return p

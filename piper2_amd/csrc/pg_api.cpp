@@ -346,6 +346,11 @@ pg::RenderConsts makeConsts(const pg_render_params& p) {
     require(static_cast<uint64_t>(rc.spp) << (2 * rc.log2Res) < (1ull << 52),
             "sample index beyond the 52 Sobol matrix columns");
     std::memcpy(rc.ndc, p.ndc, sizeof(rc.ndc));
+    require(p.filter <= pg::kFilterGaussian, "unknown film filter");
+    require(p.filter != pg::kFilterGaussian || p.filter_alpha > 0.0f, "gaussian filter needs alpha > 0");
+    rc.filter = p.filter;
+    rc.negAlpha = -p.filter_alpha;
+    rc.sub = p.filter == pg::kFilterGaussian ? std::exp(-p.filter_alpha) : 0.0f;
     return rc;
 }
 
@@ -489,7 +494,8 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
     for (const WorkItem& it : items) {
         maxItem = std::max(maxItem, it.count);
         maxDepth = std::max(maxDepth, it.rc.maxDepth);
-        quadBytes = std::max<size_t>(quadBytes, static_cast<size_t>(it.rc.rectW) * it.rc.rectH * 16 *
+        quadBytes = std::max<size_t>(quadBytes, static_cast<size_t>(it.rc.rectW) * it.rc.rectH *
+                                                    (it.rc.filter == pg::kFilterBox ? 16 : 36) *
                                                     ((it.rc.flags & pg::kFilmFp64) ? 8 : 4));
     }
     for (uint32_t i = 0; i < nScenes; ++i) {

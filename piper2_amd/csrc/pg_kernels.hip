@@ -695,6 +695,89 @@ __global__ void __launch_bounds__(256) kQuadSums(RenderConsts rc, const float4* 
     }
 }
 
+// Filter weight of a sample at offset (dx, dy) from a pixel centre (FixedSampler/Kernel.cpp:55-70).
+__device__ __forceinline__ float filterWeight(const RenderConsts& rc, float dx, float dy) {
+    if (rc.filter == kFilterTriangle) return fmaxf(0.0f, 1.0f - fabsf(dx)) * fmaxf(0.0f, 1.0f - fabsf(dy));
+    const float wx = fmaxf(expf(rc.negAlpha * dx * dx) - rc.sub, 0.0f);
+    const float wy = fmaxf(expf(rc.negAlpha * dy * dy) - rc.sub, 0.0f);
+    return wx * wy;
+}
+
+// Weighted filters (triangle, gaussian): per source pixel, the weighted sums (rgb w, w) its samples
+// give each pixel of its 3x3 neighbourhood: bins[36 * pixel + 4 * (3 (ty - py + 1) + tx - px + 1)].
+// Lane groups and the butterfly as in kQuadSums.
+template <typename F>
+__global__ void __launch_bounds__(256) kBinSums(RenderConsts rc, const float4* __restrict__ recL,
+                                                const float* __restrict__ recY, uint64_t start, uint64_t ringSize,
+                                                uint32_t groupLanes, F* bins) {
+    const uint32_t G = groupLanes;
+    const uint32_t lane = __lane_id(), l = lane % G;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t pix = wave * (64 / G) + lane / G;
+    const bool act = pix < rc.rectW * rc.rectH;  // no early exit: every lane joins the butterfly
+    F acc[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) acc[k] = F(0);
+    if (act) {
+        const int32_t px = static_cast<int32_t>(rc.rectX + pix % rc.rectW);
+        const int32_t py = static_cast<int32_t>(rc.rectY + pix / rc.rectW);
+        const uint64_t base = start + static_cast<uint64_t>(pix) * rc.sppCount;
+        for (uint32_t si = l; si < rc.sppCount; si += G) {
+            uint64_t j = base + si;
+            if (j >= ringSize) j -= ringSize;
+            const float4 r = recL[j];
+            const float fx = r.w, fy = recY[j];
+            const int32_t bx = static_cast<int32_t>(fx - 0.5f), by = static_cast<int32_t>(fy - 0.5f);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int32_t tx = bx + (k & 1), ty = by + (k >> 1);
+                const float w = filterWeight(rc, fx - (static_cast<float>(tx) + 0.5f), fy - (static_cast<float>(ty) + 0.5f));
+                const int bin = 3 * (ty - py + 1) + (tx - px + 1);  // 0..8
+#pragma unroll
+                for (int b = 0; b < 9; ++b)
+                    if (bin == b) {
+                        acc[4 * b] += static_cast<F>(r.x * w);
+                        acc[4 * b + 1] += static_cast<F>(r.y * w);
+                        acc[4 * b + 2] += static_cast<F>(r.z * w);
+                        acc[4 * b + 3] += static_cast<F>(w);
+                    }
+            }
+        }
+    }
+    for (uint32_t off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 36; ++k) acc[k] += __shfl_xor(acc[k], static_cast<int>(off));
+    if (act && l == 0) {
+        F* o = bins + 36 * static_cast<size_t>(pix);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) o[k] = acc[k];
+    }
+}
+
+// Per film pixel around the rect: the bins its 3x3 source neighbourhood holds for it.
+template <typename F>
+__global__ void __launch_bounds__(256) kResolveBins(RenderConsts rc, const F* __restrict__ bins, F* film) {
+    const int32_t x0 = max(static_cast<int32_t>(rc.rectX) - 1, 0), y0 = max(static_cast<int32_t>(rc.rectY) - 1, 0);
+    const int32_t x1 = min(static_cast<int32_t>(rc.rectX + rc.rectW) + 1, static_cast<int32_t>(rc.width));
+    const int32_t y1 = min(static_cast<int32_t>(rc.rectY + rc.rectH) + 1, static_cast<int32_t>(rc.height));
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = static_cast<uint32_t>(x1 - x0);
+    if (i >= w * static_cast<uint32_t>(y1 - y0)) return;
+    const int32_t tx = x0 + static_cast<int32_t>(i % w), ty = y0 + static_cast<int32_t>(i / w);
+    F r = F(0), g = F(0), b = F(0), n = F(0);
+    for (int32_t sy = ty - 1; sy <= ty + 1; ++sy) {
+        if (sy < static_cast<int32_t>(rc.rectY) || sy >= static_cast<int32_t>(rc.rectY + rc.rectH)) continue;
+        for (int32_t sx = tx - 1; sx <= tx + 1; ++sx) {
+            if (sx < static_cast<int32_t>(rc.rectX) || sx >= static_cast<int32_t>(rc.rectX + rc.rectW)) continue;
+            const F* q = bins + 36 * (static_cast<size_t>(sy - rc.rectY) * rc.rectW + (sx - rc.rectX)) +
+                         4 * (3 * (ty - sy + 1) + (tx - sx + 1));
+            r += q[0], g += q[1], b += q[2], n += q[3];
+        }
+    }
+    F* p = film + 4 * (static_cast<size_t>(ty) * rc.width + tx);
+    p[0] += r, p[1] += g, p[2] += b, p[3] += n;
+}
+
 // Per film pixel around the rect (rect grown by one pixel, clipped to the film).
 template <typename F>
 __global__ void __launch_bounds__(256) kResolve(RenderConsts rc, const F* __restrict__ quad, F* film) {
@@ -880,6 +963,20 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
     uint32_t g = 1;  // lanes per pixel in kQuadSums
     while (g < 64 && g < rc.sppCount) g <<= 1;
     const uint32_t sumBlocks = static_cast<uint32_t>((static_cast<uint64_t>(pixels) * g + 255) / 256);
+    if (rc.filter != kFilterBox) {  // weighted footprints
+        if (film64) {
+            hipLaunchKernelGGL(kBinSums<double>, dim3(sumBlocks), dim3(256), 0, st, rc, ring.L, ring.y, start, ring.size,
+                               g, static_cast<double*>(quad));
+            hipLaunchKernelGGL(kResolveBins<double>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
+                               static_cast<const double*>(quad), film64);
+        } else {
+            hipLaunchKernelGGL(kBinSums<float>, dim3(sumBlocks), dim3(256), 0, st, rc, ring.L, ring.y, start, ring.size,
+                               g, static_cast<float*>(quad));
+            hipLaunchKernelGGL(kResolveBins<float>, dim3(blocksFor(targets, 256, 1u << 30)), dim3(256), 0, st, rc,
+                               static_cast<const float*>(quad), film32);
+        }
+        return;
+    }
     if (film64) {
         hipLaunchKernelGGL(kQuadSums<double>, dim3(sumBlocks), dim3(256), 0, st, rc, ring.L, ring.y, start, ring.size, g,
                            static_cast<double*>(quad));

@@ -118,7 +118,10 @@ struct RenderConsts {
     uint32_t log2Res;
     uint32_t segSpp;  // samples per pixel of one film segment (path numbers run segment by segment)
     float ndc[4];
+    uint32_t filter;       // kFilterBox / kFilterTriangle / kFilterGaussian
+    float negAlpha, sub;   // gaussian: -Alpha and exp(-Alpha) (FixedSampler.cpp:40-44)
 };
+constexpr uint32_t kFilterBox = 0, kFilterTriangle = 1, kFilterGaussian = 2;
 
 constexpr uint32_t kCompatD1 = 0x1u, kCompatD3 = 0x2u, kCompatD9 = 0x4u, kCompatD13 = 0x8u;
 constexpr uint32_t kFilmFp64 = 0x100u;

@@ -192,7 +192,8 @@ public:
     };
 
     HIPTraceLauncher(std::shared_ptr<pg_scene> scene, uint32_t w, uint32_t h, RenderRECT rect,
-                     SensorNDCAffineTransform ndc, uint32_t spp, const Integrator& integrator)
+                     SensorNDCAffineTransform ndc, uint32_t spp, const Integrator& integrator, uint32_t filter,
+                     float filterAlpha)
         : mScene(std::move(scene)), mW(w), mH(h), mRect(rect), mShared(std::make_shared<Shared>()) {
         pg_render_params& p = mShared->base;
         p = pg_render_params{};
@@ -201,6 +202,7 @@ public:
         p.max_depth = integrator.maxDepth, p.min_rr = integrator.minDepthForRussianRoulette;
         p.flags = integrator.compatFlags | PG_NO_STAGE_TIMING;
         p.ndc[0] = ndc.ox, p.ndc[1] = ndc.oy, p.ndc[2] = ndc.sx, p.ndc[3] = ndc.sy;
+        p.filter = filter, p.filter_alpha = filterAlpha;
         mShared->scene = mScene.get();
     }
     [[nodiscard]] std::pair<uint32_t, uint32_t> getFilmResolution() const noexcept override { return {mW, mH}; }
@@ -279,8 +281,10 @@ private:
 
 class HIPPipeline final : public Pipeline {
 public:
-    HIPPipeline(pg_device* dev, FlatScene flat, const Integrator& integrator, std::shared_ptr<Sampler> sampler)
-        : mDevice(dev), mFlat(std::move(flat)), mIntegrator(integrator), mSampler(std::move(sampler)) {}
+    HIPPipeline(pg_device* dev, FlatScene flat, const Integrator& integrator, std::shared_ptr<Sampler> sampler,
+                uint32_t filter, float filterAlpha)
+        : mDevice(dev), mFlat(std::move(flat)), mIntegrator(integrator), mSampler(std::move(sampler)), mFilter(filter),
+          mFilterAlpha(filterAlpha) {}
 
     [[nodiscard]] std::string generateStatisticsReport() const override {
         uint64_t paths = 0, ext = 0, shadow = 0, mis = 0, calls = 0;
@@ -339,7 +343,7 @@ public:
         std::shared_ptr<pg_scene> scene(raw, pg_scene_destroy);
         const auto [ndc, rect] = calcRenderRECT(width, height, cam.getAspectRatio(), fitMode);
         auto launcher = std::make_shared<HIPTraceLauncher>(scene, width, height, rect, ndc, mSampler->samplesPerPixel,
-                                                           mIntegrator);
+                                                           mIntegrator, mFilter, mFilterAlpha);
         mLaunchers.push_back(launcher);
         return launcher;
     }
@@ -349,6 +353,8 @@ private:
     FlatScene mFlat;
     Integrator mIntegrator;
     std::shared_ptr<Sampler> mSampler;
+    uint32_t mFilter;
+    float mFilterAlpha;
     std::vector<std::weak_ptr<HIPTraceLauncher>> mLaunchers;
 };
 
@@ -522,11 +528,24 @@ std::shared_ptr<RTProgram> HIPTracer::buildProgram(const LinkableProgram& linkab
 std::unique_ptr<Pipeline> HIPTracer::buildPipeline(const std::shared_ptr<Node>& scene, Integrator& integrator,
                                                    RenderDriver& renderDriver, LightSampler& lightSampler,
                                                    std::shared_ptr<Sampler> sampler) {
-    (void)renderDriver, (void)lightSampler;  // box-filter FixedSampler and uniform light selection are built in
+    (void)lightSampler;  // uniform light selection is built in
     if (!scene || !sampler) raise("buildPipeline: scene and sampler are required");
+    // the FixedSampler's filter becomes the film filter of every launch (its RDData payload)
+    uint32_t filter = PG_FILTER_BOX;
+    float alpha = 0.0f;
+    if (const auto* fs = dynamic_cast<const FixedSampler*>(&renderDriver)) {
+        const std::string& t = fs->filter.filterType;
+        if (t == "triangle") filter = PG_FILTER_TRIANGLE;
+        else if (t == "gaussian") filter = PG_FILTER_GAUSSIAN, alpha = fs->filter.alpha;
+        else if (t != "box") raise("buildPipeline: unknown FilterType " + t);
+        if (filter == PG_FILTER_GAUSSIAN && !(alpha > 0.0f)) raise("buildPipeline: gaussian filter needs Alpha > 0");
+    } else {
+        raise("buildPipeline: only the FixedSampler render driver is supported");
+    }
     FlatScene flat;
     flatten(scene, identity34(), flat);
-    return std::make_unique<HIPPipeline>(mAccelerator->device(), std::move(flat), integrator, std::move(sampler));
+    return std::make_unique<HIPPipeline>(mAccelerator->device(), std::move(flat), integrator, std::move(sampler), filter,
+                                         alpha);
 }
 
 size_t HIPTracer::getAlignmentRequirement(AlignmentRequirement requirement) const noexcept {

@@ -286,10 +286,18 @@ private:
     std::shared_ptr<HIPAccelerator> mAccelerator;
 };
 
+// The FixedSampler's filter (SimpleFilter, RenderDriver/FixedSampler/FixedSampler.cpp:28-54):
+// FilterType "box", "triangle" or "gaussian" (Alpha > 0), weights as FixedSampler/Kernel.cpp:55-70.
+struct SimpleFilter {
+    std::string filterType = "box";
+    float alpha = 0.0f;
+};
+
 // RenderDriver/FixedSampler: 8x8 tiles, one launch each, then rgb / w per pixel
-// (FixedSampler.cpp:68-133; box filter).
+// (FixedSampler.cpp:68-133).
 class FixedSampler final : public RenderDriver {
 public:
+    SimpleFilter filter;
     [[nodiscard]] std::vector<std::array<float, 3>> renderFrame(Tracer& tracer, TraceLauncher& launcher) override;
     std::vector<float> lastFilm;  // RGBW of the last frame, for inspection
 };

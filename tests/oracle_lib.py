@@ -95,7 +95,7 @@ class OracleScene:
         return rays, film
 
     def render(self, rect=None, spp_range=None, flags=0, threads=None, path_l=False, film=True, spp=None,
-               max_depth=None, min_rr=None):
+               max_depth=None, min_rr=None, film_filter=0, filter_alpha=0.0):
         sc = self.sc
         w, h = sc["width"], sc["height"]
         spp = spp or sc["spp"]
@@ -103,8 +103,8 @@ class OracleScene:
         s0, s1 = spp_range or (0, spp)
         md = sc["max_depth"] if max_depth is None else max_depth
         rr = sc["min_rr"] if min_rr is None else min_rr
-        ip = np.array([w, h, spp, md, rr, flags, *rect, s0, s1], np.uint32)
-        ndc = np.array(sc["ndc"], np.float32)
+        ip = np.array([w, h, spp, md, rr, flags, *rect, s0, s1, film_filter], np.uint32)
+        ndc = np.array([*sc["ndc"], filter_alpha], np.float32)
         f = np.zeros((h, w, 4), np.float64) if film else None
         pl = np.zeros((rect[2] * rect[3] * (s1 - s0), 3), np.float32) if path_l else None
         rc = np.zeros(3, np.uint64)

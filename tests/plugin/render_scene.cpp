@@ -2,7 +2,8 @@
 // the way Piper2 drives a Tracer plugin (PiperRenderer builds the scene graph, Tracer::buildPipeline,
 // Pipeline::prepare, FixedSampler::renderFrame), and writes the film.
 //
-//   render_scene SCENE OUT.rgbw        render; OUT holds width * height RGBW floats
+//   render_scene SCENE OUT.rgbw [FILTER [ALPHA]]   render with the FixedSampler's FilterType (box,
+//                                      triangle, gaussian); OUT holds width * height RGBW floats
 //   render_scene --errors SCENE        exercise the error paths; prints one line per case
 //
 // Scene file (little endian): "PGSC", u32 version 1, u32 nv nt no nm nq width height spp maxDepth
@@ -130,7 +131,7 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
     return b;
 }
 
-int render(const char* scenePath, const char* outPath) {
+int render(const char* scenePath, const char* outPath, const std::string& filterType, float alpha) {
     const SceneFile s = load(scenePath);
     auto acc = std::make_shared<HIPAccelerator>(0);
     HIPTracer tracer(acc);
@@ -141,6 +142,8 @@ int render(const char* scenePath, const char* outPath) {
     auto sampler = std::make_shared<Sampler>();
     sampler->samplesPerPixel = s.spp;
     FixedSampler driver;
+    driver.filter.filterType = filterType;
+    driver.filter.alpha = alpha;
     LightSampler lights;
     auto pipeline = tracer.buildPipeline(b.root, integrator, driver, lights, sampler);
     auto launcher = pipeline->prepare(b.sensor, s.width, s.height, s.fit ? FitMode::OverScan : FitMode::Fill);
@@ -204,6 +207,15 @@ int errors(const char* scenePath) {
         const std::vector<std::shared_ptr<Resource>> res{acc->createBuffer(16, 16, nullptr)};
         (void)launcher->launch({0, 0, 8, 8}, [&](uint32_t) { return packSBTPayload(LaunchData{0, s.width, s.height}); }, res);
     });
+    expect("unknown filter", [&] {
+        const Built b = buildScene(tracer, s);
+        Integrator integrator;
+        auto sampler = std::make_shared<Sampler>();
+        FixedSampler driver;
+        driver.filter.filterType = "mitchell";
+        LightSampler lights;
+        (void)tracer.buildPipeline(b.root, integrator, driver, lights, sampler);
+    });
     std::cout << "raised " << raised << " of " << cases << "\n";
     return raised == cases ? 0 : 1;
 }
@@ -213,8 +225,9 @@ int errors(const char* scenePath) {
 int main(int argc, char** argv) {
     try {
         if (argc == 3 && std::string(argv[1]) == "--errors") return errors(argv[2]);
-        if (argc == 3) return render(argv[1], argv[2]);
-        std::cerr << "usage: render_scene SCENE OUT | render_scene --errors SCENE\n";
+        if (argc >= 3 && argc <= 5 && std::string(argv[1]) != "--errors")
+            return render(argv[1], argv[2], argc >= 4 ? argv[3] : "box", argc == 5 ? std::stof(argv[4]) : 0.0f);
+        std::cerr << "usage: render_scene SCENE OUT [FILTER [ALPHA]] | render_scene --errors SCENE\n";
         return 2;
     } catch (const PiperException& e) {
         std::cerr << "error: " << e.what() << "\n";

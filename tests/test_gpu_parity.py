@@ -212,3 +212,21 @@ def test_invalid_arguments_rejected(dev, cornell):
     p.max_depth = 500
     with pytest.raises(pg.PgError):
         s.render(p)
+
+
+@pytest.mark.parametrize("film_filter,alpha,fp64", [(1, 0.0, False), (2, 2.0, False), (2, 1.5, True)])
+def test_filtered_film_matches_oracle(dev, film_filter, alpha, fp64):
+    # FixedSampler's triangle and gaussian filters (FixedSampler/Kernel.cpp:55-70): the same per-sample
+    # weights as the oracle, summed in another fixed order, so weights agree to rounding
+    sc = scenes.cornell(width=64, height=48, spp=16)
+    s = pg.Scene(dev, sc)
+    film = pg.Film(dev, sc["width"], sc["height"])
+    flags = pg.PG_FILM_FP64 if fp64 else 0
+    s.render(pg.render_params(sc, flags=flags, film_filter=film_filter, filter_alpha=alpha), film)
+    fg = film.download(fp64=fp64).astype(np.float64)
+    fc = OracleScene(sc).render(threads=16, film_filter=film_filter, filter_alpha=alpha)["film"]
+    assert np.allclose(fg[..., 3], fc[..., 3], rtol=1e-5, atol=1e-6)
+    m = fc[..., 3] > 1e-3
+    assert rel_mse(fg[..., :3][m] / fg[..., 3:][m], fc[..., :3][m] / fc[..., 3:][m]) < 1e-4
+    # not the box film: the weights are fractional
+    assert not np.allclose(fg[..., 3], np.round(fg[..., 3]))

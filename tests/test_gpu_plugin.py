@@ -20,17 +20,17 @@ def dev():
     d.close()
 
 
-def direct_film(dev, sc, flags=0):
+def direct_film(dev, sc, flags=0, film_filter=0, alpha=0.0):
     s = pg.Scene(dev, sc)
     f = pg.Film(dev, sc["width"], sc["height"])
-    s.render(pg.render_params(sc, flags=flags), f)
+    s.render(pg.render_params(sc, flags=flags, film_filter=film_filter, filter_alpha=alpha), f)
     return f.download()
 
 
-def plugin_film(tmp_path, sc, name, **kw):
+def plugin_film(tmp_path, sc, name, filter_args=(), **kw):
     path, out = tmp_path / f"{name}.pgsc", tmp_path / f"{name}.rgbw"
     write_scene(sc, path, **kw)
-    r = run_driver(str(path), str(out))
+    r = run_driver(str(path), str(out), *filter_args)
     assert r.returncode == 0, r.stdout + r.stderr
     film = np.fromfile(out, np.float32).reshape(sc["height"], sc["width"], 4)
     return film, r.stdout
@@ -43,6 +43,17 @@ def test_fixed_sampler_frame_matches_pg_render(dev, tmp_path):
     # the 8x8 tiles of the frame were coalesced into one render call over the whole rect
     assert "1 render calls" in log and f"{100 * 68 * 8} paths" in log, log
     assert re.search(r"rect: 0 0 100 68", log)
+
+
+@pytest.mark.parametrize("name,code,alpha", [("triangle", pg.PG_FILTER_TRIANGLE, 0.0),
+                                             ("gaussian", pg.PG_FILTER_GAUSSIAN, 2.5)])
+def test_fixed_sampler_filters(dev, tmp_path, name, code, alpha):
+    # the FixedSampler's SimpleFilter (FilterType, Alpha) becomes the film filter of its launches
+    sc = scenes.cornell(width=72, height=56, spp=8)
+    args = (name,) if code == pg.PG_FILTER_TRIANGLE else (name, str(alpha))
+    film, _ = plugin_film(tmp_path, sc, name, filter_args=args)
+    assert np.array_equal(film, direct_film(dev, sc, film_filter=code, alpha=alpha))
+    assert not np.array_equal(film, direct_film(dev, sc))
 
 
 def test_conductors_compat_flags_and_environment(dev, tmp_path):
@@ -96,4 +107,4 @@ def test_error_paths_raise(tmp_path):
     write_scene(sc, path)
     r = run_driver("--errors", str(path))
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "raised 6 of 6" in r.stdout, r.stdout
+    assert "raised 7 of 7" in r.stdout, r.stdout

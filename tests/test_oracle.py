@@ -246,3 +246,28 @@ def test_russian_roulette_is_unbiased():
     b = o.render(threads=8, max_depth=5, min_rr=1)["film"]
     ia, ib = a[..., :3] / a[..., 3:], b[..., :3] / b[..., 3:]
     assert abs(ia.mean() - ib.mean()) / ia.mean() < 0.02
+
+
+def test_film_filters_partition_and_gaussian_profile():
+    """Triangle weights of the 2x2 splat sum to one per sample (a partition of unity), so the film's
+    total weight equals the sample count up to border losses; box weights count whole samples (4 per
+    sample); the gaussian's total weight relative to the triangle's matches the ratio of the mean
+    1D profiles max(exp(-a d^2) - exp(-a), 0) and max(0, 1 - |d|) over |d| < 1, squared."""
+    sc = scenes.cornell(width=16, height=16, spp=4, max_depth=1)
+    o = OracleScene(sc)
+    box = o.render(threads=4)["film"]
+    tri = o.render(threads=4, film_filter=1)["film"]
+    inner = np.s_[1:-1, 1:-1]
+    # every pixel's 4 box weights count whole samples; the triangle's share them out
+    assert np.all(box[..., 3] == np.round(box[..., 3]))
+    n = 16 * 16 * 4
+    assert abs(box[..., 3].sum() - 4 * n) < 4 * 2 * 16 * 4  # box: 4 per sample, minus border losses
+    assert abs(tri[..., 3].sum() - n) < 2 * 16 * 4           # triangle: 1 per sample, minus border losses
+    assert np.all(tri[inner][..., 3] > 0)
+    gau = o.render(threads=4, film_filter=2, filter_alpha=2.0)["film"]
+    ratio = gau[inner][..., 3].sum() / tri[inner][..., 3].sum()
+    # mean gaussian product weight over the 2x2 splat, relative to the triangle's 1/4 per pixel
+    d = np.linspace(-1, 1, 2001)
+    g1 = np.maximum(np.exp(-2.0 * d * d) - np.exp(-2.0), 0.0)
+    t1 = np.maximum(0.0, 1.0 - np.abs(d))
+    assert abs(ratio - (g1.mean() / t1.mean()) ** 2) < 0.05 * (g1.mean() / t1.mean()) ** 2
