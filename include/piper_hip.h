@@ -44,6 +44,9 @@ extern "C" {
 
 #define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
 #define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
+#define PG_MAT_PLASTIC 2u   /* plastic (Surface/Classic/Kernel.cpp:435-473): Lambert diffuse = albedo plus
+                               GGX with dielectric Fresnel (eta 1.46), specular = reserved[0..2],
+                               roughness = alpha (remapRoughness applied, :378-381) */
 
 #define PG_HIT_MISS 0xFFFFFFFFu
 #define PG_HIT_QUAD 0x80000000u /* prim = PG_HIT_QUAD | quad index for a light-quad hit */
@@ -55,8 +58,8 @@ typedef struct pg_film pg_film;
 typedef struct pg_material {
     uint32_t type;      /* PG_MAT_* */
     float albedo[3];    /* Lambert rho (ConstantData, ProceduralTexture/Kernel.cpp:21-26) or conductor F0 */
-    float alpha;        /* GGX alpha = roughness^2 (conductor only) */
-    float reserved[3];
+    float alpha;        /* conductor: GGX alpha = roughness^2; plastic: roughness */
+    float reserved[3];  /* plastic: specular rgb */
 } pg_material;
 
 typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22-28, AreaLight/Shared.hpp:22-26) */

@@ -175,11 +175,13 @@ constexpr float kInvPi = 0.31830988618379067154f;
 // ---------------------------------------------------------------------------------------------
 // Scene
 // ---------------------------------------------------------------------------------------------
-enum : uint32_t { kMatLambert = 0, kMatConductor = 1 };
+enum : uint32_t { kMatLambert = 0, kMatConductor = 1, kMatPlastic = 2 };
 struct Material {
     uint32_t type;
-    V3 albedo;  // Lambert rho, or conductor F0
-    float alpha;
+    V3 albedo;          // Lambert rho, conductor F0, plastic diffuse
+    float alpha;        // GGX alpha (plastic: remapRoughness of the scene's roughness)
+    V3 spec{0, 0, 0};   // plastic specular
+    bool front = true;  // plastic: the hit is on the triangle's front face
 };
 struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
     V3 origin, u, v, normal, radiance;
@@ -568,24 +570,71 @@ float conductorPdf(const Material& m, V3 wo, V3 wi) {  // D(wh) G1(wo) / (4 cos(
     return ggxD(wh, m.alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, m.alpha)));
 }
 
-// Reflection/transmission side test of the evaluate path, Classic/Kernel.cpp:220 and :211-215
-inline bool sameSide(V3 wo, V3 wi, V3 ng) { return dot(wo, ng) * dot(wi, ng) > 0.0f; }
+// MicrofacetDistribution::remapRoughness, Classic/Kernel.cpp:378-381 (a quartic in log roughness).
+float remapRoughness(float roughness) {
+    const float x = std::log(std::fmax(1e-3f, roughness));
+    const float c[5] = {1.62142f, 0.819955f, 0.1734f, 0.0171201f, 0.000640711f};
+    float r = c[4];
+    for (int k = 3; k >= 0; --k) r = c[k] + r * x;
+    return std::fmax(1e-3f, r);
+}
+// FresnelDielectric::operator(), Classic/Kernel.cpp:271-286 (unpolarised, Snell's law; total
+// internal reflection gives 1).
+float fresnelDielectric(float cosI, float etaI, float etaT) {
+    cosI = std::fmin(1.0f, std::fmax(0.0f, std::fabs(cosI)));
+    const float sinT = etaI / etaT * std::sqrt(std::fmax(0.0f, 1.0f - cosI * cosI));
+    if (sinT >= 1.0f) return 1.0f;
+    const float cosT = std::sqrt(std::fmax(0.0f, 1.0f - sinT * sinT));
+    const float rPar = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
+    const float rPerp = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+    return 0.5f * (rPar * rPar + rPerp * rPerp);
+}
 
-V3 bsdfEval(const Material& m, V3 wo, V3 wi, V3 ng) {
-    if (!sameSide(wo, wi, ng)) return {0.0f, 0.0f, 0.0f};
-    if (m.type == kMatLambert) return m.albedo * kInvPi;  // LambertianReflection, Kernel.cpp:81-92
-    return conductorF(m, wo, wi);
+// One BxDF of a material. Kinds: the Lambert lobe (LambertianReflection, Kernel.cpp:81-92), the
+// GGX lobe with a Schlick conductor Fresnel, and the GGX lobe with the dielectric Fresnel of
+// plasticInit (Kernel.cpp:441-471: eta 1.46 outside-in on the front face, swapped on the back).
+enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric };
+struct Lobe {
+    LobeKind kind;
+    V3 scale;
+};
+struct Lobes {
+    Lobe l[2];
+    uint32_t n = 0;
+};
+// The valid lobes in the order of the BSDF's members (PlasticBSDF: diffuse, then glossy);
+// plasticInit keeps only lobes whose colour is valid().
+Lobes lobesOf(const Material& m) {
+    Lobes r;
+    if (m.type == kMatLambert) {
+        r.l[r.n++] = {kLobeLambert, m.albedo};
+    } else if (m.type == kMatConductor) {
+        r.l[r.n++] = {kLobeConductor, m.albedo};
+    } else {
+        if (valid(m.albedo)) r.l[r.n++] = {kLobeLambert, m.albedo};
+        if (valid(m.spec)) r.l[r.n++] = {kLobeDielectric, m.spec};
+    }
+    return r;
 }
-float bsdfPdf(const Material& m, V3 wo, V3 wi) {
-    if (m.type == kMatLambert) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
-    return conductorPdf(m, wo, wi);
+V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
+    if (b.kind == kLobeLambert) return b.scale * kInvPi;
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+    if (b.kind == kLobeConductor) return conductorF(m, wo, wi);
+    const V3 wh = normalize(wo + wi);  // MicrofacetReflection::evaluate, Kernel.cpp:398-404
+    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+    const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+    const float fr = m.front ? fresnelDielectric(dot(wi, wh), 1.0f, 1.46f) : fresnelDielectric(dot(wi, wh), 1.46f, 1.0f);
+    return b.scale * (fr * k);
 }
-// sample(), Kernel.cpp:180-202 with a single BxDF (count == 1): f comes from the BxDF itself.
-BsdfSample bsdfSample(const Material& m, V3 wo, float u1, float u2) {
+float lobePdf(const Material& m, const Lobe& b, V3 wo, V3 wi) {
+    if (b.kind == kLobeLambert) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
+    return conductorPdf(m, wo, wi);  // the same visible-normal pdf for either Fresnel
+}
+BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
-    if (m.type == kMatLambert) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
+    if (b.kind == kLobeLambert) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
         s.wi = cosineHemisphere(u1, u2);
-        s.f = m.albedo * kInvPi;
+        s.f = b.scale * kInvPi;
         s.pdf = s.wi.z > 0.0f ? std::fabs(s.wi.z) * kInvPi : 0.0f;
         return s;
     }
@@ -595,8 +644,45 @@ BsdfSample bsdfSample(const Material& m, V3 wo, float u1, float u2) {
     if (c < 0.0f) return s;
     s.wi = wh * (2.0f * c) - wo;  // reflect, Transform.hpp:405-408
     if (s.wi.z <= 0.0f) return s;
-    s.f = conductorF(m, wo, s.wi);
-    s.pdf = conductorPdf(m, wo, s.wi);
+    s.f = lobeF(m, b, wo, s.wi);
+    s.pdf = lobePdf(m, b, wo, s.wi);
+    return s;
+}
+
+// Reflection/transmission side test of the evaluate path, Classic/Kernel.cpp:220 and :211-215;
+// every lobe here is a reflection lobe, so a pair on opposite sides of Ng evaluates to zero.
+inline bool sameSide(V3 wo, V3 wi, V3 ng) { return dot(wo, ng) * dot(wi, ng) > 0.0f; }
+
+// The mixture over a material's lobes (Classic/Kernel.cpp:144-237): evaluate sums the lobes,
+// pdf is their mean, and sample draws lobe floor(u1 count), rescales u1, then adds the other
+// lobes' values (same side only) and pdfs and divides the pdf by the count.
+V3 bsdfEval(const Material& m, V3 wo, V3 wi, V3 ng) {
+    if (!sameSide(wo, wi, ng)) return {0.0f, 0.0f, 0.0f};
+    const Lobes ls = lobesOf(m);
+    V3 f{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < ls.n; ++k) f = f + lobeF(m, ls.l[k], wo, wi);
+    return f;
+}
+float bsdfPdf(const Material& m, V3 wo, V3 wi) {
+    const Lobes ls = lobesOf(m);
+    float p = 0.0f;
+    for (uint32_t k = 0; k < ls.n; ++k) p += lobePdf(m, ls.l[k], wo, wi);
+    return ls.n > 1 ? p / static_cast<float>(ls.n) : p;
+}
+BsdfSample bsdfSample(const Material& m, V3 wo, V3 ng, float u1, float u2) {
+    const Lobes ls = lobesOf(m);
+    if (ls.n == 0) return BsdfSample{{0, 0, 0}, {0, 0, 0}, 0.0f};
+    const float cnt = static_cast<float>(ls.n);
+    const uint32_t pick = std::min(static_cast<uint32_t>(std::floor(u1 * cnt)), ls.n - 1);
+    BsdfSample s = lobeSample(m, ls.l[pick], wo, u1 * cnt - static_cast<float>(pick), u2);
+    if (s.pdf <= 0.0f) return s;
+    const bool same = sameSide(wo, s.wi, ng);
+    for (uint32_t k = 0; k < ls.n; ++k) {
+        if (k == pick) continue;
+        if (same) s.f = s.f + lobeF(m, ls.l[k], wo, s.wi);
+        s.pdf += lobePdf(m, ls.l[k], wo, s.wi);
+    }
+    if (ls.n > 1) s.pdf = s.pdf / cnt;
     return s;
 }
 
@@ -635,13 +721,13 @@ struct Frame {  // SurfaceIntersectionInfo, Protocol.hpp:47-66
 
 // Hit post-process: calcTriangleMeshSurface (Embree/Kernel.cpp:170-192) and planeSurface
 // (BasicShapes/Kernel.cpp:99-111). Returns the world normal facing the incoming ray.
+V3 triNg(const Tri& tri) { return normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0)); }
 V3 hitNormal(const Scene& s, const Hit& h, V3 dir) {
     if (h.prim & kQuadBit) {
         const Quad& q = s.quads[h.prim & ~kQuadBit];
         return dot(dir, q.normal) > 0.0f ? -q.normal : q.normal;  // face Front -> -normal
     }
-    const Tri& tri = s.tris[h.prim];
-    const V3 ng = normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0));
+    const V3 ng = triNg(s.tris[h.prim]);
     return dot(dir, ng) < 0.0f ? ng : -ng;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
 }
 Frame shadingFrame(V3 n, bool d3) {
@@ -731,8 +817,9 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
         }
         if (h.prim == kMiss || depth >= p.maxDepth) break;  // :148
 
-        const Material& mat = s.mats[s.objMat[s.triObj[h.prim]]];
+        Material mat = s.mats[s.objMat[s.triObj[h.prim]]];
         const V3 n = hitNormal(s, h, ray.d);
+        mat.front = dot(ray.d, triNg(s.tris[h.prim])) < 0.0f;  // Embree.cpp:336
         const Frame fr = shadingFrame(n, d3);
         const V3 wo = fr.toLocal(-ray.d);
         const V3 ng = fr.toLocal(n);
@@ -758,7 +845,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
                 }
             }
             smp.next2(d9, u1, u2);
-            const BsdfSample bs = bsdfSample(mat, wo, u1, u2);
+            const BsdfSample bs = bsdfSample(mat, wo, ng, u1, u2);
             if (bs.pdf > 0.0f && valid(bs.f)) {
                 float weight = std::fabs(bs.wi.z) / bs.pdf;
                 const V3 dir = fr.toWorld(bs.wi);
@@ -787,7 +874,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
 
         float u1, u2;
         smp.next2(d9, u1, u2);
-        const BsdfSample ss = bsdfSample(mat, wo, u1, u2);  // :164-169
+        const BsdfSample ss = bsdfSample(mat, wo, ng, u1, u2);  // :164-169
         if (!(ss.pdf > 0.0f && valid(ss.f))) break;
         pf = mul(pf, ss.f * (std::fabs(ss.wi.z) / ss.pdf));
         if (depth > p.minRR) {  // :177-182
@@ -871,6 +958,16 @@ void splat(F* rgbw, const Params& p, float fx, float fy, V3 L, std::vector<std::
 // =============================================================================================
 // C ABI for ctypes (tests / bench cpu_baseline only)
 // =============================================================================================
+// One material row of or_scene_create's mats array.
+Material materialOf(const float* m) {
+    Material r{static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]};
+    if (r.type == kMatPlastic) {
+        r.alpha = remapRoughness(m[4]);
+        r.spec = {m[5], m[6], m[7]};
+    }
+    return r;
+}
+
 extern "C" {
 
 void or_sobol_matrices(uint32_t* out) { std::memcpy(out, g_sobol, sizeof(g_sobol)); }
@@ -881,7 +978,7 @@ uint64_t or_interval_to_index(uint32_t m, uint32_t sample, uint32_t px, uint32_t
 
 // Scene arrays:
 //   verts[3*nv]; idx[3*nt]; triObj[nt]; objMat[nobj];
-//   mats[8*nmat] = {type, r, g, b, alpha, 0, 0, 0};
+//   mats[8*nmat] = {type, r, g, b, alpha, spec r, g, b} (plastic: alpha slot holds the roughness);
 //   quads[12*nq] = {origin xyz, U xyz, V xyz, radiance rgb};
 //   cam[14] = {base xyz, lookAt xyz, up xyz, sizeX, sizeY, focal, aperture, 0}; env[3]
 void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* triObj,
@@ -902,7 +999,7 @@ void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint
     s->objMat.assign(objMat, objMat + nobj);
     for (uint32_t i = 0; i < nmat; ++i) {
         const float* m = mats + 8 * i;
-        s->mats.push_back({static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]});
+        s->mats.push_back(materialOf(m));
     }
     for (uint32_t i = 0; i < nq; ++i) {
         const float* q = quads + 12 * i;
@@ -1026,20 +1123,26 @@ void or_render(void* sp, const uint32_t* iparams, const float* ndc, double* film
 }
 
 // Furnace-style and analytic helpers used by the oracle's own pinning tests.
-void or_bsdf_eval(const float* mat, const float* wo, const float* wi, float* f, float* pdf) {
-    const Material m{static_cast<uint32_t>(mat[0]), {mat[1], mat[2], mat[3]}, mat[4]};
+// mat: 8 floats as in or_scene_create; front: the face flag of a plastic hit. The shading frame is
+// the geometric one (Ng = +z).
+void or_bsdf_eval(const float* mat, int front, const float* wo, const float* wi, float* f, float* pdf) {
+    Material m = materialOf(mat);
+    m.front = front != 0;
     const V3 o{wo[0], wo[1], wo[2]}, i{wi[0], wi[1], wi[2]};
     const V3 r = bsdfEval(m, o, i, V3{0, 0, 1});
     f[0] = r.x, f[1] = r.y, f[2] = r.z;
     *pdf = bsdfPdf(m, o, i);
 }
-void or_bsdf_sample(const float* mat, const float* wo, float u1, float u2, float* wi, float* f, float* pdf) {
-    const Material m{static_cast<uint32_t>(mat[0]), {mat[1], mat[2], mat[3]}, mat[4]};
-    const BsdfSample s = bsdfSample(m, V3{wo[0], wo[1], wo[2]}, u1, u2);
+void or_bsdf_sample(const float* mat, int front, const float* wo, float u1, float u2, float* wi, float* f, float* pdf) {
+    Material m = materialOf(mat);
+    m.front = front != 0;
+    const BsdfSample s = bsdfSample(m, V3{wo[0], wo[1], wo[2]}, V3{0, 0, 1}, u1, u2);
     wi[0] = s.wi.x, wi[1] = s.wi.y, wi[2] = s.wi.z;
     f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
     *pdf = s.pdf;
 }
+float or_fresnel_dielectric(float cosI, float etaI, float etaT) { return fresnelDielectric(cosI, etaI, etaT); }
+float or_remap_roughness(float roughness) { return remapRoughness(roughness); }
 float or_ggx_d(float cosTheta, float alpha) {
     const float s = std::sqrt(std::fmax(0.0f, 1.0f - cosTheta * cosTheta));
     return ggxD(V3{s, 0.0f, cosTheta}, alpha);

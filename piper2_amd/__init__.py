@@ -34,6 +34,7 @@ PG_FILTER_GAUSSIAN = 2
 
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
+PG_MAT_PLASTIC = 2
 PG_HIT_MISS = 0xFFFFFFFF
 PG_HIT_QUAD = 0x80000000
 
@@ -431,10 +432,10 @@ def scene_desc(sc):
     """Build a SceneDesc that points into the numpy arrays of a generated scene dict."""
     for k in ("vertices", "indices", "tri_object", "object_material", "materials", "quads"):
         assert sc[k].flags["C_CONTIGUOUS"], k
-    mats = sc["materials"]  # (n, 8) float32: type, r, g, b, alpha, 0, 0, 0
+    mats = sc["materials"]  # (n, 8) float32: type, r, g, b, alpha (plastic: roughness), specular rgb
     mat_arr = (Material * len(mats))()
     for i, m in enumerate(mats):
-        mat_arr[i] = Material(int(m[0]), F3(*m[1:4]), float(m[4]), F3(0, 0, 0))
+        mat_arr[i] = Material(int(m[0]), F3(*m[1:4]), float(m[4]), F3(*m[5:8]))
     quads = sc["quads"]  # (n, 12)
     quad_arr = (Quad * max(len(quads), 1))()
     for i, q in enumerate(quads):

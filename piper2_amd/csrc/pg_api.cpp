@@ -1013,9 +1013,16 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         std::vector<pg::DevMaterial> mats(desc->num_materials);
         for (uint32_t i = 0; i < desc->num_materials; ++i) {
             const pg_material& m = desc->materials[i];
-            require(m.type == PG_MAT_LAMBERT || m.type == PG_MAT_CONDUCTOR, "unknown material type");
+            require(m.type == PG_MAT_LAMBERT || m.type == PG_MAT_CONDUCTOR || m.type == PG_MAT_PLASTIC,
+                    "unknown material type");
             require(m.type != PG_MAT_CONDUCTOR || m.alpha > 0.0f, "conductor alpha must be positive");
-            mats[i] = pg::DevMaterial{m.type, {m.albedo[0], m.albedo[1], m.albedo[2]}, m.alpha, {0, 0, 0}};
+            float alpha = m.alpha;
+            if (m.type == PG_MAT_PLASTIC) {  // MicrofacetDistribution::remapRoughness (Classic/Kernel.cpp:378-381)
+                const float x = std::log(std::fmax(1e-3f, m.alpha));
+                alpha = std::fmax(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+            }
+            mats[i] = pg::DevMaterial{m.type, {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha,
+                                      {m.reserved[0], m.reserved[1], m.reserved[2]}};
         }
         std::vector<pg::DevQuad> quads(desc->num_quads);
         for (uint32_t i = 0; i < desc->num_quads; ++i) {

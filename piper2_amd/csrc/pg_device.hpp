@@ -382,10 +382,12 @@ __device__ __forceinline__ Frame shadingFrame(V3 n, bool d3) {
     f.b = cross(n, f.t);
     return f;
 }
-__device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir) {
+__device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir, bool* front = nullptr) {
     const V3 v0 = xyz(s.tris[3 * pos]), v1 = xyz(s.tris[3 * pos + 1]), v2 = xyz(s.tris[3 * pos + 2]);
     const V3 ng = normalize(cross(v1 - v0, v2 - v0));
-    return dot(dir, ng) < 0.0f ? ng : -ng;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
+    const bool f = dot(dir, ng) < 0.0f;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
+    if (front) *front = f;
+    return f ? ng : -ng;
 }
 __device__ __forceinline__ V3 quadNormal(const DevQuad& q, V3 dir) {
     const V3 n = ld3(q.normal);
@@ -457,44 +459,115 @@ __device__ __forceinline__ float conductorPdf(float alpha, V3 wo, V3 wi) {
     return ggxD(wh, alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, alpha)));
 }
 
-struct Mat {
-    uint32_t type;
-    V3 albedo;
-    float alpha;
-};
-__device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id) {
-    const DevMaterial& m = s.mats[id];
-    return Mat{m.type, ld3(m.albedo), m.alpha};
+// FresnelDielectric (Classic/Kernel.cpp:271-286).
+__device__ __forceinline__ float fresnelDielectric(float cosI, float etaI, float etaT) {
+    cosI = fmaxf(0.0f, fminf(1.0f, fabsf(cosI)));
+    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - cosI * cosI));
+    const float sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0f) return 1.0f;
+    const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT * sinT));
+    const float p1 = etaT * cosI, p2 = etaI * cosT;
+    const float par = (p1 - p2) / (p1 + p2);
+    const float p3 = etaI * cosI, p4 = etaT * cosT;
+    const float perp = (p3 - p4) / (p3 + p4);
+    return 0.5f * (par * par + perp * perp);
 }
+
+struct Mat {
+    uint32_t type;  // 0 Lambert, 1 conductor, 2 plastic
+    V3 albedo;      // Lambert / plastic diffuse, conductor F0
+    float alpha;
+    V3 spec;        // plastic specular reflectance
+    bool front;     // plastic: the hit is on the triangle's front face (plasticInit's etaI, etaT)
+};
+__device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true) {
+    const DevMaterial& m = s.mats[id];
+    return Mat{m.type, ld3(m.albedo), m.alpha, ld3(m.spec), front};
+}
+
+// Plastic's glossy lobe: MicrofacetReflection<FresnelDielectric> (Classic/Kernel.cpp:384-433,
+// plasticInit :441-471, eta 1.46), with the exact visible-normal pdf as the conductor.
+__device__ __forceinline__ V3 plasticGlossyF(const Mat& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+    const V3 wh = normalize(wo + wi);
+    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+    const float k = ggxD(wh, m.alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    const float etaI = m.front ? 1.0f : 1.46f, etaT = m.front ? 1.46f : 1.0f;
+    return m.spec * (fresnelDielectric(dot(wi, wh), etaI, etaT) * k);
+}
+__device__ __forceinline__ float lambertPdf(V3 wi) { return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f; }
+
+// The generic BxDF mixture (Classic/Kernel.cpp:144-237) over plastic's valid lobes (diffuse,
+// glossy): evaluate sums them, pdf averages them, sample picks one by u1.
+__device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec) ? 2u : 0u); }
+
 __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     if (!(dot(wo, ng) * dot(wi, ng) > 0.0f)) return {0.0f, 0.0f, 0.0f};  // Classic/Kernel.cpp:220
     if (m.type == 0) return m.albedo * kInvPi;
-    return conductorF(m.albedo, m.alpha, wo, wi);
+    if (m.type == 1) return conductorF(m.albedo, m.alpha, wo, wi);
+    const uint32_t lobes = plasticLobes(m);
+    V3 f{0.0f, 0.0f, 0.0f};
+    if (lobes & 1u) f = f + m.albedo * kInvPi;
+    if (lobes & 2u) f = f + plasticGlossyF(m, wo, wi);
+    return f;
 }
 __device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
-    if (m.type == 0) return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f;
-    return conductorPdf(m.alpha, wo, wi);
+    if (m.type == 0) return lambertPdf(wi);
+    if (m.type == 1) return conductorPdf(m.alpha, wo, wi);
+    const uint32_t lobes = plasticLobes(m);
+    float p = 0.0f;
+    if (lobes & 1u) p = p + lambertPdf(wi);
+    if (lobes & 2u) p = p + conductorPdf(m.alpha, wo, wi);
+    return lobes == 3u ? p / 2.0f : p;
 }
 struct BsdfSample {
     V3 wi, f;
     float pdf;
 };
-__device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, float u1, float u2) {
+__device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, float alpha, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
-    if (m.type == 0) {
-        s.wi = cosineHemisphere(u1, u2);
-        s.f = m.albedo * kInvPi;
-        s.pdf = s.wi.z > 0.0f ? fabsf(s.wi.z) * kInvPi : 0.0f;
-        return s;
-    }
     if (wo.z <= 0.0f) return s;
-    const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+    const V3 wh = ggxSampleWh(wo, alpha, u1, u2);
     const float c = dot(wo, wh);
     if (c < 0.0f) return s;
     s.wi = wh * (2.0f * c) - wo;
     if (s.wi.z <= 0.0f) return s;
-    s.f = conductorF(m.albedo, m.alpha, wo, s.wi);
-    s.pdf = conductorPdf(m.alpha, wo, s.wi);
+    s.pdf = conductorPdf(alpha, wo, s.wi);
+    return s;
+}
+__device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, float u1, float u2) {
+    BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
+    if (m.type == 0) {
+        s.wi = cosineHemisphere(u1, u2);
+        s.f = m.albedo * kInvPi;
+        s.pdf = lambertPdf(s.wi);
+        return s;
+    }
+    if (m.type == 1) {
+        s = ggxReflectSample(wo, m.alpha, u1, u2);
+        if (s.pdf > 0.0f) s.f = conductorF(m.albedo, m.alpha, wo, s.wi);
+        return s;
+    }
+    const uint32_t lobes = plasticLobes(m);
+    const uint32_t count = __popc(lobes);
+    if (count == 0) return s;
+    const uint32_t select = min(static_cast<uint32_t>(floorf(u1 * static_cast<float>(count))), count - 1u);
+    u1 = u1 * static_cast<float>(count) - static_cast<float>(select);
+    const bool diffuse = (lobes & 1u) && select == 0;  // lobes in order: diffuse, glossy
+    if (diffuse) {
+        s.wi = cosineHemisphere(u1, u2);
+        s.f = m.albedo * kInvPi;
+        s.pdf = lambertPdf(s.wi);
+    } else {
+        s = ggxReflectSample(wo, m.alpha, u1, u2);
+        if (s.pdf > 0.0f) s.f = plasticGlossyF(m, wo, s.wi);
+    }
+    if (!(s.pdf > 0.0f)) return s;
+    if (count > 1) {  // the other lobe: its value on the same side, its pdf always
+        if (dot(wo, ng) * dot(s.wi, ng) > 0.0f) s.f = s.f + (diffuse ? plasticGlossyF(m, wo, s.wi) : m.albedo * kInvPi);
+        s.pdf = s.pdf + (diffuse ? conductorPdf(m.alpha, wo, s.wi) : lambertPdf(s.wi));
+        s.pdf = s.pdf / static_cast<float>(count);
+    }
     return s;
 }
 

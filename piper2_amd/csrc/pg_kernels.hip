@@ -200,7 +200,7 @@ __device__ __forceinline__ bool isTrainPath(const ItemGuide& ig, uint32_t g) {
 // fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
 constexpr float kGuideMinAlpha = 0.04f;
 __device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
-    return (m.type == 0u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
+    return (m.type != 1u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
 }
 
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
@@ -271,8 +271,9 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             } else if (pos == kHitMiss || meta.y >= rc.maxDepth) {  // :148
                 pushDone = true;
             } else {
-                const Mat mat = loadMat(s, s.triMat[pos]);
-                const V3 nrm = triNormal(s, pos, ray.d);
+                bool front;
+                const V3 nrm = triNormal(s, pos, ray.d, &front);
+                const Mat mat = loadMat(s, s.triMat[pos], front);
                 const Frame fr = shadingFrame(nrm, d3);
                 const V3 wo = fr.toLocal(-ray.d);
                 const V3 ng = fr.toLocal(nrm);
@@ -301,7 +302,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         pushShadow = true;
                     }
                 }
-                const BsdfSample bs = bsdfSample(mat, wo, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
+                const BsdfSample bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
                 if (bs.pdf > 0.0f && valid(bs.f)) {
                     const float weight = fabsf(bs.wi.z) / bs.pdf;
                     const V3 dir = fr.toWorld(bs.wi);
@@ -352,7 +353,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 BsdfSample ss;
                 float pdfMix = 0.0f, a = 0.0f;
                 if (!GUIDED) {
-                    ss = bsdfSample(mat, wo, uc1, uc2);
+                    ss = bsdfSample(mat, wo, ng, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
                     a = guideFraction(BATCH ? bt.guide[sceneOf(meta.z)].alpha : gs.alpha, mat);
@@ -361,7 +362,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         const float3 w = mixtureSample(mix, uc1 / a, uc2);
                         ss.wi = fr.toLocal(V3{w.x, w.y, w.z});
                     } else {
-                        const BsdfSample b = bsdfSample(mat, wo, (uc1 - a) / (1.0f - a), uc2);
+                        const BsdfSample b = bsdfSample(mat, wo, ng, (uc1 - a) / (1.0f - a), uc2);
                         ss.wi = b.wi;
                         ok = b.pdf > 0.0f;
                     }

@@ -37,10 +37,10 @@ static_assert(sizeof(WideNode<8>) == 224, "BVH8 node is 224 B");
 static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 B");
 
 struct DevMaterial {  // 32 B
-    uint32_t type;
-    float albedo[3];
-    float alpha;
-    float reserved[3];
+    uint32_t type;     // 0 Lambert, 1 conductor, 2 plastic
+    float albedo[3];   // Lambert / plastic diffuse reflectance, conductor F0
+    float alpha;       // GGX alpha (plastic: remapped roughness)
+    float spec[3];     // plastic specular reflectance
 };
 
 struct DevQuad {  // Plane primitive (PerPlaneData) + DiffuseAreaLightData, 80 B

@@ -136,9 +136,13 @@ struct FlatScene {
 
 uint32_t materialOf(FlatScene& f, const Surface& s) {
     pg_material m{};
-    m.type = s.kind == Surface::Kind::Matte ? PG_MAT_LAMBERT : PG_MAT_CONDUCTOR;
+    m.type = s.kind == Surface::Kind::Matte ? PG_MAT_LAMBERT
+             : s.kind == Surface::Kind::Conductor ? PG_MAT_CONDUCTOR
+                                                   : PG_MAT_PLASTIC;
     for (int k = 0; k < 3; ++k) m.albedo[k] = s.color[k];
     m.alpha = s.alpha;
+    if (s.kind == Surface::Kind::Plastic)
+        for (int k = 0; k < 3; ++k) m.reserved[k] = s.specular[k];
     f.materials.push_back(m);
     return static_cast<uint32_t>(f.materials.size() - 1);
 }
@@ -407,6 +411,12 @@ std::shared_ptr<Surface> Surface::conductor(std::array<float, 3> f0, float rough
     s->kind = Kind::Conductor, s->color = f0, s->alpha = roughness * roughness;
     return s;
 }
+std::shared_ptr<Surface> Surface::plastic(std::array<float, 3> diffuse, std::array<float, 3> specular, float roughness) {
+    if (!std::isfinite(roughness)) raise("plastic roughness must be finite");
+    auto s = std::make_shared<Surface>();
+    s->kind = Kind::Plastic, s->color = diffuse, s->alpha = roughness, s->specular = specular;
+    return s;
+}
 std::shared_ptr<Light> Light::areaPlane(std::array<float, 3> origin, std::array<float, 3> u, std::array<float, 3> v,
                                         std::array<float, 3> radiance) {
     auto l = std::make_shared<Light>();
@@ -423,7 +433,8 @@ const std::vector<std::string>& hipBuiltinSymbols() {
     // the per-sample and per-bounce functions the wavefront kernels implement natively
     static const std::vector<std::string> s = {
         "rayGen",      "sobolStart", "sobolGenerate", "trace",         "accumulate", "box",
-        "matteInit",   "conductorInit", "diffuseSample", "diffuseEvaluate", "diffusePdf", "planeSample",
+        "matteInit",   "conductorInit", "plasticInit", "plasticSample", "plasticEvaluate", "plasticPdf",
+        "diffuseSample", "diffuseEvaluate", "diffusePdf", "planeSample",
         "uniformSelect", "SEInit",   "SESample",      "SEEvaluate",    "SEPdf",      "constantTexture"};
     return s;
 }

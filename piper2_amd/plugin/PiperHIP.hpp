@@ -157,16 +157,20 @@ public:
 private:
     GeometryDesc mDesc;
 };
-// Surface/Classic: Matte (MatteData, Classic/Shared.hpp:51-54 with a constant texture) and the GGX
-// conductor this path adds (alpha = roughness^2, Schlick F0).
+// Surface/Classic: Matte (MatteData, Classic/Shared.hpp:51-54 with a constant texture), Plastic
+// (PlasticData, Classic/Shared.hpp:56-60 and Classic.cpp:123-147: Diffuse, Specular, Roughness as
+// constant textures) and the GGX conductor this path adds (alpha = roughness^2, Schlick F0).
 class Surface : public Object {
 public:
-    enum class Kind { Matte, Conductor };
+    enum class Kind { Matte, Conductor, Plastic };
     static std::shared_ptr<Surface> matte(std::array<float, 3> reflectance);
     static std::shared_ptr<Surface> conductor(std::array<float, 3> f0, float roughness);
+    // roughness as the reference's config gives it: the library applies remapRoughness
+    static std::shared_ptr<Surface> plastic(std::array<float, 3> diffuse, std::array<float, 3> specular, float roughness);
     Kind kind;
-    std::array<float, 3> color;
-    float alpha;
+    std::array<float, 3> color;          // matte reflectance, conductor F0, plastic diffuse
+    float alpha;                         // conductor GGX alpha, plastic roughness
+    std::array<float, 3> specular{};     // plastic
 };
 class Medium : public Object {};
 // Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V) and the environment

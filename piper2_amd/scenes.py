@@ -2,7 +2,7 @@
 
 Each generator returns a dict of numpy arrays in the layout of pg_scene_desc (include/piper_hip.h):
   vertices (V,3) f32, indices (T,3) u32, tri_object (T,) u32, object_material (O,) u32,
-  materials (M,8) f32 = [type, r, g, b, alpha, 0, 0, 0], quads (Q,12) f32 = [origin, U, V, radiance],
+  materials (M,8) f32 = [type, r, g, b, alpha, specular rgb] (plastic: alpha holds the roughness), quads (Q,12) f32 = [origin, U, V, radiance],
   camera (14,) f32 = [position, look_at, up, sensor_w, sensor_h, focal, aperture, 0], env (3,) f32,
 plus the render settings (width, height, spp, max_depth, min_rr, ndc).
 
@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-LAMBERT, CONDUCTOR = 0, 1
+LAMBERT, CONDUCTOR, PLASTIC = 0, 1, 2
 
 
 def calc_render_rect(width, height, device_aspect, fit_fill=True):
@@ -121,8 +121,8 @@ class _Builder:
         self.v, self.t, self.obj, self.objmat, self.mats = [], [], [], [], []
         self.nv = 0
 
-    def material(self, kind, albedo, alpha=0.0):
-        self.mats.append([kind, *albedo, alpha, 0, 0, 0])
+    def material(self, kind, albedo, alpha=0.0, spec=(0.0, 0.0, 0.0)):
+        self.mats.append([kind, *albedo, alpha, *spec])
         return len(self.mats) - 1
 
     def add(self, verts, tris, mat):
@@ -149,9 +149,14 @@ class _Builder:
         return sc
 
 
-def cornell(seed=1, width=256, height=256, spp=16, max_depth=5):
-    """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light."""
+def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False):
+    """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
+
+    plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
+    tilted one-quad plastic panel that paths reach on both faces; the walls and the light stay as
+    in the plain scene."""
     rng = np.random.default_rng(seed)
+    prng = np.random.default_rng(seed + 1000)
     b = _Builder()
     walls = {
         "floor": [[-1, -1, -1], [1, -1, -1], [1, -1, 1], [-1, -1, 1]],
@@ -175,13 +180,22 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5):
         yaw = rng.uniform(0.0, math.pi / 2)
         pos = rng.uniform(-0.6, 0.6, 2)
         v, t = _box(pos, -1.0, size, yaw)
-        b.add(v, t, b.material(LAMBERT, rng.uniform(0.1, 0.8, 3)))
+        alb = rng.uniform(0.1, 0.8, 3)
+        if plastic:
+            mat = b.material(PLASTIC, alb, prng.uniform(0.05, 0.6), prng.uniform(0.2, 0.9, 3))
+        else:
+            mat = b.material(LAMBERT, alb)
+        b.add(v, t, mat)
+    if plastic:
+        panel = [[-0.9, 0.1, -0.2], [-0.3, 0.1, -0.6], [-0.3, 0.7, -0.5], [-0.9, 0.7, -0.1]]
+        v, t = _quad_tris(*panel)
+        b.add(v, t, b.material(PLASTIC, [0.5, 0.3, 0.2], 0.2, [0.8, 0.8, 0.8]))
     quads = [[-0.25, 0.999, -0.25, 0.5, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0]]
     aspect = width / height
     cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
-                    ndc=ndc, name="cornell")
+                    ndc=ndc, name="cornell_plastic" if plastic else "cornell")
 
 
 def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500),

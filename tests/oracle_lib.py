@@ -30,8 +30,12 @@ def lib():
         L.or_occluded.argtypes = [_VP, _VP, ctypes.c_uint32, _VP]
         L.or_camera_rays.argtypes = [_VP, _VP, _VP, _VP, ctypes.c_uint32, _VP, _VP]
         L.or_render.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_int, _VP]
-        L.or_bsdf_eval.argtypes = [_VP, _VP, _VP, _VP, _VP]
-        L.or_bsdf_sample.argtypes = [_VP, _VP, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP]
+        L.or_bsdf_eval.argtypes = [_VP, ctypes.c_int, _VP, _VP, _VP, _VP]
+        L.or_bsdf_sample.argtypes = [_VP, ctypes.c_int, _VP, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP]
+        L.or_fresnel_dielectric.restype = ctypes.c_float
+        L.or_fresnel_dielectric.argtypes = [ctypes.c_float] * 3
+        L.or_remap_roughness.restype = ctypes.c_float
+        L.or_remap_roughness.argtypes = [ctypes.c_float]
         L.or_ggx_d.restype = ctypes.c_float
         L.or_ggx_d.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib = L

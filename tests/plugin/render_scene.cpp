@@ -8,7 +8,7 @@
 //
 // Scene file (little endian): "PGSC", u32 version 1, u32 nv nt no nm nq width height spp maxDepth
 // minRR compatFlags fitMode, f32 vertices[3 nv], u32 indices[3 nt], u32 triObject[nt],
-// u32 objectMaterial[no], f32 materials[8 nm] (type, rgb, alpha, 0, 0, 0), f32 quads[12 nq]
+// u32 objectMaterial[no], f32 materials[8 nm] (type, rgb, alpha, specular rgb), f32 quads[12 nq]
 // (origin, U, V, radiance), f32 camera[14] (position, lookAt, up, sensor w h, focal, aperture, 0),
 // f32 env[3], f32 srt[16 no] (TransformSRT per object, applied to its triangles).
 #include <cstdio>
@@ -100,10 +100,15 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
                                               buffer, size, 0, iOff};
         auto geometry = std::make_shared<Geometry>(gd);
         const float* m = &s.materials[8 * s.objectMaterial[o]];
-        auto surface = std::make_shared<Surface>();  // alpha as stored (Surface::conductor squares a roughness)
-        surface->kind = m[0] == 0.0f ? Surface::Kind::Matte : Surface::Kind::Conductor;
-        surface->color = {m[1], m[2], m[3]};
-        surface->alpha = m[4];
+        std::shared_ptr<Surface> surface;
+        if (m[0] == 2.0f) {
+            surface = Surface::plastic({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4]);
+        } else {
+            surface = std::make_shared<Surface>();  // alpha as stored (Surface::conductor squares a roughness)
+            surface->kind = m[0] == 0.0f ? Surface::Kind::Matte : Surface::Kind::Conductor;
+            surface->color = {m[1], m[2], m[3]};
+            surface->alpha = m[4];
+        }
         auto inst = tracer.buildGSMInstance(geometry, surface, nullptr);
         TransformSRT srt;
         std::memcpy(&srt, &s.srt[16 * o], sizeof(srt));

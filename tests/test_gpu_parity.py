@@ -161,6 +161,19 @@ def test_lit_environment_paths_match_oracle(dev):
     _path_parity(sc, dev)
 
 
+def test_plastic_paths_match_oracle(dev):
+    # plastic boxes and a tilted plastic panel reached on both faces: the Lambert + dielectric-GGX
+    # mixture with the front/back Fresnel swap
+    sc = scenes.cornell(width=128, height=128, spp=16, plastic=True)
+    _path_parity(sc, dev)
+
+
+def test_plastic_compat_flags_match_oracle(dev):
+    sc = scenes.cornell(width=128, height=128, spp=8, plastic=True)
+    flags = pg.PG_COMPAT_D1 | pg.PG_COMPAT_D3 | pg.PG_COMPAT_D9 | pg.PG_COMPAT_D13
+    _path_parity(sc, dev, flags=flags)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

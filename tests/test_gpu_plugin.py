@@ -56,6 +56,13 @@ def test_fixed_sampler_filters(dev, tmp_path, name, code, alpha):
     assert not np.array_equal(film, direct_film(dev, sc))
 
 
+def test_plastic_surfaces(dev, tmp_path):
+    # Surface::plastic (Diffuse, Specular, Roughness) reaches the library as PG_MAT_PLASTIC
+    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True)
+    film, _ = plugin_film(tmp_path, sc, "plastic")
+    assert np.array_equal(film, direct_film(dev, sc))
+
+
 def test_conductors_compat_flags_and_environment(dev, tmp_path):
     sc = scenes.terrain_spheres(width=64, height=40, spp=4, grid=(80, 40), n_spheres=10)
     sc["env"] = np.array([0.2, 0.3, 0.4], np.float32)

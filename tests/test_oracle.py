@@ -122,17 +122,18 @@ def test_watertight_closed_mesh_no_leaks():
     assert np.all(o.trace(r)[1] != 0xFFFFFFFF)
 
 
-def _bsdf(mat, wo, wi):
+def _bsdf(mat, wo, wi, front=True):
     f = np.zeros(3, np.float32)
     pdf = np.zeros(1, np.float32)
-    lib().or_bsdf_eval(mat.ctypes.data, np.ascontiguousarray(wo, np.float32).ctypes.data,
-                       np.ascontiguousarray(wi, np.float32).ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    wo, wi = np.ascontiguousarray(wo, np.float32), np.ascontiguousarray(wi, np.float32)  # held across the call
+    lib().or_bsdf_eval(mat.ctypes.data, int(front), wo.ctypes.data, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
     return f, float(pdf[0])
 
 
-def _sample(mat, wo, u1, u2):
+def _sample(mat, wo, u1, u2, front=True):
     wi, f, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
-    lib().or_bsdf_sample(mat.ctypes.data, np.ascontiguousarray(wo, np.float32).ctypes.data, np.float32(u1),
+    wo = np.ascontiguousarray(wo, np.float32)
+    lib().or_bsdf_sample(mat.ctypes.data, int(front), wo.ctypes.data, np.float32(u1),
                          np.float32(u2), wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
     return wi, f, float(pdf[0])
 
@@ -192,6 +193,119 @@ def test_conductor_weak_white_furnace():
         fs = np.array([_bsdf(mat, wo, d)[0][0] for d in dirs])
         albedo = float((fs * dirs[:, 2] * dw).sum())
         assert 0.85 < albedo <= 1.01, (alpha, albedo)
+
+
+def _fresnel(c, eta_i, eta_t):
+    return lib().or_fresnel_dielectric(np.float32(c), np.float32(eta_i), np.float32(eta_t))
+
+
+def test_dielectric_fresnel_textbook_forms():
+    """The plastic coat's Fresnel term against the sine/tangent forms of Fresnel's equations, the
+    normal-incidence closed form, grazing incidence, reciprocity and total internal reflection."""
+    n = 1.46
+    r0 = ((n - 1) / (n + 1)) ** 2
+    assert abs(_fresnel(1.0, 1.0, n) - r0) < 1e-6 and abs(_fresnel(1.0, n, 1.0) - r0) < 1e-6
+    assert abs(_fresnel(0.0, 1.0, n) - 1.0) < 1e-6
+    crit = math.asin(1 / n)
+    for th in np.linspace(0.02, math.pi / 2 - 0.02, 60):
+        tt = math.asin(math.sin(th) / n)
+        rs = -math.sin(th - tt) / math.sin(th + tt)
+        rp = math.tan(th - tt) / math.tan(th + tt)
+        ref = 0.5 * (rs * rs + rp * rp)
+        assert abs(_fresnel(math.cos(th), 1.0, n) - ref) < 3e-6, th
+        assert abs(_fresnel(-math.cos(th), 1.0, n) - ref) < 3e-6, th  # |cos|
+        # the same interface from inside (float32 cancellation in cos(theta_t) near the critical angle)
+        assert abs(_fresnel(math.cos(tt), n, 1.0) - ref) < 1e-4, th
+        if th > crit + 1e-4:
+            assert _fresnel(math.cos(th), n, 1.0) == 1.0, th
+
+
+def test_remap_roughness_quartic():
+    """Roughness to GGX alpha: the quartic in x = ln(roughness) of the reference (published as
+    pbrt-v3's RoughnessToAlpha), floored at 1e-3 in both the input and the output."""
+    c = [1.62142, 0.819955, 0.1734, 0.0171201, 0.000640711]
+    remap = lambda r: lib().or_remap_roughness(np.float32(r))
+    assert abs(remap(1.0) - c[0]) < 1e-6
+    for x in (-1.0, 1.0, -2.0, -4.0):
+        expect = sum(ck * x ** k for k, ck in enumerate(c))
+        assert abs(remap(math.exp(x)) - expect) < 2e-5 * abs(expect), x
+    assert remap(1e-5) == remap(1e-3)
+    rs = np.geomspace(1e-3, 1.0, 50)
+    assert np.all(np.diff([remap(r) for r in rs]) > 0)
+
+
+def _plastic(diffuse, spec, rough):
+    return np.array([2, *diffuse, rough, *spec], np.float32)
+
+
+def test_plastic_lobes_reduce_to_lambert_and_fresnel_weighted_ggx():
+    """With no specular colour plastic is exactly Lambert (one lobe: u1 passes through); with no
+    diffuse colour it is the GGX lobe of a white conductor (Schlick F = 1) times spec x the
+    dielectric Fresnel of the face (1 -> 1.46 on the front, 1.46 -> 1 on the back)."""
+    rough = 0.2
+    alpha = lib().or_remap_roughness(np.float32(rough))
+    lam = np.array([0, 0.3, 0.5, 0.7, 0, 0, 0, 0], np.float32)
+    cond = np.array([1, 1.0, 1.0, 1.0, alpha, 0, 0, 0], np.float32)
+    only_d = _plastic([0.3, 0.5, 0.7], [0, 0, 0], rough)
+    only_s = _plastic([0, 0, 0], [0.6, 0.7, 0.8], rough)
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        wo = rng.normal(size=3); wo[2] = abs(wo[2]); wo /= np.linalg.norm(wo)
+        wi = rng.normal(size=3); wi[2] = abs(wi[2]); wi /= np.linalg.norm(wi)
+        u1, u2 = rng.uniform(0, 1, 2)
+        for front in (True, False):
+            (f1, p1), (f2, p2) = _bsdf(only_d, wo, wi, front), _bsdf(lam, wo, wi)
+            assert np.array_equal(f1, f2) and p1 == p2
+            a, b = _sample(only_d, wo, u1, u2, front), _sample(lam, wo, u1, u2)
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+            fs, ps = _bsdf(only_s, wo, wi, front)
+            fc, pc = _bsdf(cond, wo, wi)
+            assert ps == pc  # one lobe: no division by the count
+            wh = (wo + wi) / np.linalg.norm(wo + wi)
+            fr = _fresnel(np.dot(wi, wh), 1.0, 1.46) if front else _fresnel(np.dot(wi, wh), 1.46, 1.0)
+            assert np.allclose(fs, np.array([0.6, 0.7, 0.8]) * fr * fc, rtol=1e-4, atol=1e-7)
+
+
+@pytest.mark.parametrize("front", [True, False])
+@pytest.mark.parametrize("rough", [0.05, 0.4])
+def test_plastic_mixture_sampling_matches_evaluate(front, rough):
+    """The two-lobe mixture: a sample's value and pdf equal the evaluate path's (the other lobe's
+    value and pdf added, the pdf halved), E[1/pdf] over the samples is the upper hemisphere's
+    solid angle (the sampled density is the returned pdf) and E[f cos / pdf] is the directional
+    albedo that quadrature of the evaluate path gives."""
+    mat = _plastic([0.4, 0.5, 0.6], [0.7, 0.8, 0.9], rough)
+    wo = np.array([0.45, -0.2, 0.0], np.float32)
+    wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+    rng = np.random.default_rng(17)
+    n, inv, est, picks = 20000, 0.0, np.zeros(3), 0
+    for u1, u2 in rng.uniform(0, 1, (n, 2)):
+        wi, f, pdf = _sample(mat, wo, u1, u2, front)
+        if pdf > 0:
+            f2, pdf2 = _bsdf(mat, wo, wi, front)
+            assert np.allclose(f, f2, rtol=1e-5, atol=1e-8) and abs(pdf - pdf2) <= 1e-5 * pdf
+            inv += 1.0 / pdf
+            est += f * wi[2] / pdf
+            picks += u1 < 0.5
+    assert abs(inv / n - 2 * np.pi) < 0.04 * 2 * np.pi
+    dirs, dw = _hemisphere_grid(300, 200)
+    albedo = sum(_bsdf(mat, wo, d, front)[0] * d[2] * w for d, w in zip(dirs, dw))
+    assert np.allclose(est / n, albedo, rtol=0.03), (est / n, albedo)
+    assert abs(picks / n - 0.5) < 0.02
+
+
+def test_plastic_without_specular_renders_as_lambert():
+    """A scene whose Lambert materials are restated as plastic with a black specular colour renders
+    the same film, bit for bit."""
+    sc = scenes.cornell(width=32, height=32, spp=8)
+    o1 = OracleScene(sc)
+    sc2 = dict(sc)
+    m = sc["materials"].copy()
+    m[:, 0] = 2
+    m[:, 4] = 0.3
+    sc2["materials"] = m
+    o2 = OracleScene(sc2)
+    a, b = o1.render(threads=4), o2.render(threads=4)
+    assert np.array_equal(a["film"], b["film"]) and np.array_equal(a["rays"], b["rays"])
 
 
 def _polygon_irradiance(p, n, poly, le):
