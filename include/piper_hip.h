@@ -42,7 +42,8 @@ extern "C" {
 #define PG_FILTER_TRIANGLE 1u /* max(0, 1 - |dx|) max(0, 1 - |dy|) */
 #define PG_FILTER_GAUSSIAN 2u /* max(exp(-a dx^2) - exp(-a), 0) max(exp(-a dy^2) - exp(-a), 0) */
 
-#define PG_MAT_LAMBERT 0u   /* matte with roughness 0 (Surface/Classic/Kernel.cpp:249-269) */
+#define PG_MAT_LAMBERT 0u   /* matte (Surface/Classic/Kernel.cpp:94-269): roughness = alpha in [0, 1];
+                               0 gives Lambert, > 0 Oren-Nayar with sigma = roughness x pi */
 #define PG_MAT_CONDUCTOR 1u /* GGX conductor, Schlick F0 (new: the reference has no conductor) */
 #define PG_MAT_PLASTIC 2u   /* plastic (Surface/Classic/Kernel.cpp:435-473): Lambert diffuse = albedo plus
                                GGX with dielectric Fresnel (eta 1.46), specular = reserved[0..2],
@@ -58,7 +59,7 @@ typedef struct pg_film pg_film;
 typedef struct pg_material {
     uint32_t type;      /* PG_MAT_* */
     float albedo[3];    /* Lambert rho (ConstantData, ProceduralTexture/Kernel.cpp:21-26) or conductor F0 */
-    float alpha;        /* conductor: GGX alpha = roughness^2; plastic: roughness */
+    float alpha;        /* matte: roughness (0 = Lambert); conductor: GGX alpha = roughness^2; plastic: roughness */
     float reserved[3];  /* plastic: specular rgb */
 } pg_material;
 

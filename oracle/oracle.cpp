@@ -182,6 +182,8 @@ struct Material {
     float alpha;        // GGX alpha (plastic: remapRoughness of the scene's roughness)
     V3 spec{0, 0, 0};   // plastic specular
     bool front = true;  // plastic: the hit is on the triangle's front face
+    float onA = 1.0f, onB = 0.0f;  // matte with roughness > 0: Oren-Nayar A and B
+    bool orenNayar = false;
 };
 struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
     V3 origin, u, v, normal, radiance;
@@ -593,7 +595,7 @@ float fresnelDielectric(float cosI, float etaI, float etaT) {
 // One BxDF of a material. Kinds: the Lambert lobe (LambertianReflection, Kernel.cpp:81-92), the
 // GGX lobe with a Schlick conductor Fresnel, and the GGX lobe with the dielectric Fresnel of
 // plasticInit (Kernel.cpp:441-471: eta 1.46 outside-in on the front face, swapped on the back).
-enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric };
+enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric, kLobeOrenNayar };
 struct Lobe {
     LobeKind kind;
     V3 scale;
@@ -606,8 +608,8 @@ struct Lobes {
 // plasticInit keeps only lobes whose colour is valid().
 Lobes lobesOf(const Material& m) {
     Lobes r;
-    if (m.type == kMatLambert) {
-        r.l[r.n++] = {kLobeLambert, m.albedo};
+    if (m.type == kMatLambert) {  // matteInit (Kernel.cpp:249-269): one lobe either way
+        r.l[r.n++] = {m.orenNayar ? kLobeOrenNayar : kLobeLambert, m.albedo};
     } else if (m.type == kMatConductor) {
         r.l[r.n++] = {kLobeConductor, m.albedo};
     } else {
@@ -616,8 +618,23 @@ Lobes lobesOf(const Material& m) {
     }
     return r;
 }
+// OrenNayar::evaluate (Kernel.cpp:108-131), in terms of the polar and azimuth angles of wo and wi:
+// (A + B max(0, cos(phi_i - phi_o)) sin(alpha) tan(beta)) / pi with alpha the larger polar angle and
+// beta the smaller; zero within sin(theta) < 1e-4 of the normal, as the reference has it.
+float orenNayarScale(const Material& m, V3 wo, V3 wi) {
+    const float sI = std::sqrt(std::fmax(0.0f, 1.0f - wi.z * wi.z));
+    const float sO = std::sqrt(std::fmax(0.0f, 1.0f - wo.z * wo.z));
+    if (sI < 1e-4f || sO < 1e-4f) return 0.0f;
+    auto clamp1 = [](float x) { return std::fmin(1.0f, std::fmax(-1.0f, x)); };
+    const float cosDphi = clamp1(wi.x / sI) * clamp1(wo.x / sO) + clamp1(wi.y / sI) * clamp1(wo.y / sO);
+    const float cI = std::fabs(wi.z), cO = std::fabs(wo.z);
+    const float sinAlpha = cI > cO ? sO : sI;       // alpha = max(theta_i, theta_o)
+    const float tanBeta = cI > cO ? sI / cI : sO / cO;  // beta = min(theta_i, theta_o)
+    return kInvPi * (m.onA + m.onB * std::fmax(0.0f, cosDphi) * sinAlpha * tanBeta);
+}
 V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     if (b.kind == kLobeLambert) return b.scale * kInvPi;
+    if (b.kind == kLobeOrenNayar) return b.scale * orenNayarScale(m, wo, wi);
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
     if (b.kind == kLobeConductor) return conductorF(m, wo, wi);
     const V3 wh = normalize(wo + wi);  // MicrofacetReflection::evaluate, Kernel.cpp:398-404
@@ -627,14 +644,14 @@ V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     return b.scale * (fr * k);
 }
 float lobePdf(const Material& m, const Lobe& b, V3 wo, V3 wi) {
-    if (b.kind == kLobeLambert) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
+    if (b.kind == kLobeLambert || b.kind == kLobeOrenNayar) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
     return conductorPdf(m, wo, wi);  // the same visible-normal pdf for either Fresnel
 }
 BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
-    if (b.kind == kLobeLambert) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
+    if (b.kind == kLobeLambert || b.kind == kLobeOrenNayar) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
         s.wi = cosineHemisphere(u1, u2);
-        s.f = b.scale * kInvPi;
+        s.f = lobeF(m, b, wo, s.wi);
         s.pdf = s.wi.z > 0.0f ? std::fabs(s.wi.z) * kInvPi : 0.0f;
         return s;
     }
@@ -965,6 +982,12 @@ Material materialOf(const float* m) {
         r.alpha = remapRoughness(m[4]);
         r.spec = {m[5], m[6], m[7]};
     }
+    if (r.type == kMatLambert && m[4] > 0.0f) {  // OrenNayar(reflection, sigma), Kernel.cpp:99-106
+        const float sigma = m[4] * kPi, s2 = sigma * sigma;
+        r.orenNayar = true;
+        r.onA = 1.0f - s2 / (2.0f * (s2 + 0.33f));
+        r.onB = 0.45f * s2 / (s2 + 0.09f);
+    }
     return r;
 }
 
@@ -978,7 +1001,8 @@ uint64_t or_interval_to_index(uint32_t m, uint32_t sample, uint32_t px, uint32_t
 
 // Scene arrays:
 //   verts[3*nv]; idx[3*nt]; triObj[nt]; objMat[nobj];
-//   mats[8*nmat] = {type, r, g, b, alpha, spec r, g, b} (plastic: alpha slot holds the roughness);
+//   mats[8*nmat] = {type, r, g, b, alpha, spec r, g, b} (matte and plastic: the alpha slot holds the
+//   roughness);
 //   quads[12*nq] = {origin xyz, U xyz, V xyz, radiance rgb};
 //   cam[14] = {base xyz, lookAt xyz, up xyz, sizeX, sizeY, focal, aperture, 0}; env[3]
 void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* triObj,

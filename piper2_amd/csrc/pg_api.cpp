@@ -1016,13 +1016,23 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
             require(m.type == PG_MAT_LAMBERT || m.type == PG_MAT_CONDUCTOR || m.type == PG_MAT_PLASTIC,
                     "unknown material type");
             require(m.type != PG_MAT_CONDUCTOR || m.alpha > 0.0f, "conductor alpha must be positive");
+            require(m.type != PG_MAT_LAMBERT || (m.alpha >= 0.0f && m.alpha <= 1.0f), "matte roughness must be in [0, 1]");
+            uint32_t type = m.type;
             float alpha = m.alpha;
+            float spec[3] = {m.reserved[0], m.reserved[1], m.reserved[2]};
             if (m.type == PG_MAT_PLASTIC) {  // MicrofacetDistribution::remapRoughness (Classic/Kernel.cpp:378-381)
                 const float x = std::log(std::fmax(1e-3f, m.alpha));
                 alpha = std::fmax(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+            } else if (m.type == PG_MAT_LAMBERT && m.alpha > 0.0f) {
+                // matteInit (Classic/Kernel.cpp:260-266) picks OrenNayar; its constructor (:99-106)
+                // takes sigma = roughness x pi radians
+                const float sigma = m.alpha * 3.14159265358979323846f;
+                const float s2 = sigma * sigma;
+                type = pg::kDevOrenNayar;
+                alpha = 1.0f - s2 / (2.0f * (s2 + 0.33f));
+                spec[0] = 0.45f * s2 / (s2 + 0.09f), spec[1] = 0.0f, spec[2] = 0.0f;
             }
-            mats[i] = pg::DevMaterial{m.type, {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha,
-                                      {m.reserved[0], m.reserved[1], m.reserved[2]}};
+            mats[i] = pg::DevMaterial{type, {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha, {spec[0], spec[1], spec[2]}};
         }
         std::vector<pg::DevQuad> quads(desc->num_quads);
         for (uint32_t i = 0; i < desc->num_quads; ++i) {

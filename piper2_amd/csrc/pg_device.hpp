@@ -474,10 +474,10 @@ __device__ __forceinline__ float fresnelDielectric(float cosI, float etaI, float
 }
 
 struct Mat {
-    uint32_t type;  // 0 Lambert, 1 conductor, 2 plastic
-    V3 albedo;      // Lambert / plastic diffuse, conductor F0
-    float alpha;
-    V3 spec;        // plastic specular reflectance
+    uint32_t type;  // 0 Lambert, 1 conductor, 2 plastic, 3 Oren-Nayar
+    V3 albedo;      // Lambert / Oren-Nayar / plastic diffuse, conductor F0
+    float alpha;    // GGX alpha; Oren-Nayar: A
+    V3 spec;        // plastic specular reflectance; Oren-Nayar: B in x
     bool front;     // plastic: the hit is on the triangle's front face (plasticInit's etaI, etaT)
 };
 __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true) {
@@ -497,6 +497,21 @@ __device__ __forceinline__ V3 plasticGlossyF(const Mat& m, V3 wo, V3 wi) {
 }
 __device__ __forceinline__ float lambertPdf(V3 wi) { return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f; }
 
+// OrenNayar::evaluate (Classic/Kernel.cpp:108-131): A and B are precomputed on upload; the value is
+// zero when either direction is within sin(theta) < 1e-4 of the normal, as the reference has it.
+__device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
+    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - wi.z * wi.z));
+    const float sinO = sqrtf(fmaxf(0.0f, 1.0f - wo.z * wo.z));
+    if (sinI < 1e-4f || sinO < 1e-4f) return {0.0f, 0.0f, 0.0f};
+    const float cosPhiI = fminf(1.0f, fmaxf(-1.0f, wi.x / sinI)), sinPhiI = fminf(1.0f, fmaxf(-1.0f, wi.y / sinI));
+    const float cosPhiO = fminf(1.0f, fmaxf(-1.0f, wo.x / sinO)), sinPhiO = fminf(1.0f, fmaxf(-1.0f, wo.y / sinO));
+    const float maxCos = fmaxf(0.0f, cosPhiI * cosPhiO + sinPhiI * sinPhiO);
+    const bool iSteeper = fabsf(wi.z) > fabsf(wo.z);
+    const float sinAlpha = iSteeper ? sinO : sinI;
+    const float tanBeta = iSteeper ? sinI / fabsf(wi.z) : sinO / fabsf(wo.z);
+    return m.albedo * (kInvPi * (m.alpha + m.spec.x * maxCos * sinAlpha * tanBeta));
+}
+
 // The generic BxDF mixture (Classic/Kernel.cpp:144-237) over plastic's valid lobes (diffuse,
 // glossy): evaluate sums them, pdf averages them, sample picks one by u1.
 __device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec) ? 2u : 0u); }
@@ -505,6 +520,7 @@ __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     if (!(dot(wo, ng) * dot(wi, ng) > 0.0f)) return {0.0f, 0.0f, 0.0f};  // Classic/Kernel.cpp:220
     if (m.type == 0) return m.albedo * kInvPi;
     if (m.type == 1) return conductorF(m.albedo, m.alpha, wo, wi);
+    if (m.type == kDevOrenNayar) return orenNayarF(m, wo, wi);
     const uint32_t lobes = plasticLobes(m);
     V3 f{0.0f, 0.0f, 0.0f};
     if (lobes & 1u) f = f + m.albedo * kInvPi;
@@ -512,7 +528,7 @@ __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     return f;
 }
 __device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
-    if (m.type == 0) return lambertPdf(wi);
+    if (m.type == 0 || m.type == kDevOrenNayar) return lambertPdf(wi);
     if (m.type == 1) return conductorPdf(m.alpha, wo, wi);
     const uint32_t lobes = plasticLobes(m);
     float p = 0.0f;
@@ -537,9 +553,9 @@ __device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, float alpha, float
 }
 __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
-    if (m.type == 0) {
+    if (m.type == 0 || m.type == kDevOrenNayar) {  // DefaultBxDFHelper::sample (Classic/Kernel.cpp:67-74)
         s.wi = cosineHemisphere(u1, u2);
-        s.f = m.albedo * kInvPi;
+        s.f = m.type == 0 ? m.albedo * kInvPi : orenNayarF(m, wo, s.wi);
         s.pdf = lambertPdf(s.wi);
         return s;
     }

@@ -400,9 +400,10 @@ std::pair<SensorNDCAffineTransform, RenderRECT> calcRenderRECT(uint32_t width, u
 
 TransformSRT identitySRT() { return {1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0}; }
 
-std::shared_ptr<Surface> Surface::matte(std::array<float, 3> reflectance) {
+std::shared_ptr<Surface> Surface::matte(std::array<float, 3> reflectance, float roughness) {
+    if (!(roughness >= 0.0f && roughness <= 1.0f)) raise("matte roughness must be in [0, 1]");
     auto s = std::make_shared<Surface>();
-    s->kind = Kind::Matte, s->color = reflectance, s->alpha = 0.0f;
+    s->kind = Kind::Matte, s->color = reflectance, s->alpha = roughness;
     return s;
 }
 std::shared_ptr<Surface> Surface::conductor(std::array<float, 3> f0, float roughness) {

@@ -163,13 +163,14 @@ private:
 class Surface : public Object {
 public:
     enum class Kind { Matte, Conductor, Plastic };
-    static std::shared_ptr<Surface> matte(std::array<float, 3> reflectance);
+    // roughness 0 = Lambert, in (0, 1] Oren-Nayar (matteInit)
+    static std::shared_ptr<Surface> matte(std::array<float, 3> reflectance, float roughness = 0.0f);
     static std::shared_ptr<Surface> conductor(std::array<float, 3> f0, float roughness);
     // roughness as the reference's config gives it: the library applies remapRoughness
     static std::shared_ptr<Surface> plastic(std::array<float, 3> diffuse, std::array<float, 3> specular, float roughness);
     Kind kind;
     std::array<float, 3> color;          // matte reflectance, conductor F0, plastic diffuse
-    float alpha;                         // conductor GGX alpha, plastic roughness
+    float alpha;                         // matte roughness, conductor GGX alpha, plastic roughness
     std::array<float, 3> specular{};     // plastic
 };
 class Medium : public Object {};

@@ -149,12 +149,13 @@ class _Builder:
         return sc
 
 
-def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False):
+def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
-    tilted one-quad plastic panel that paths reach on both faces; the walls and the light stay as
-    in the plain scene."""
+    tilted one-quad plastic panel that paths reach on both faces; rough_walls=True gives the walls
+    seeded matte roughness (Oren-Nayar). The light and the rest of the geometry stay as in the
+    plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -174,7 +175,7 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False):
         else:
             rng.shuffle(alb)
         v, t = _quad_tris(*corners)
-        b.add(v, t, b.material(LAMBERT, alb))
+        b.add(v, t, b.material(LAMBERT, alb, prng.uniform(0.2, 0.9) if rough_walls else 0.0))
     for _ in range(2):
         size = rng.uniform(0.3, 0.6, 3)
         yaw = rng.uniform(0.0, math.pi / 2)
@@ -195,7 +196,7 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False):
     cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
-                    ndc=ndc, name="cornell_plastic" if plastic else "cornell")
+                    ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else ""))
 
 
 def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500),

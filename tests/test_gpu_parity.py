@@ -174,6 +174,12 @@ def test_plastic_compat_flags_match_oracle(dev):
     _path_parity(sc, dev, flags=flags)
 
 
+def test_rough_matte_paths_match_oracle(dev):
+    # walls of matte with roughness > 0: the Oren-Nayar lobe
+    sc = scenes.cornell(width=128, height=128, spp=16, rough_walls=True)
+    _path_parity(sc, dev)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

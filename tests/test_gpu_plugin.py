@@ -56,9 +56,10 @@ def test_fixed_sampler_filters(dev, tmp_path, name, code, alpha):
     assert not np.array_equal(film, direct_film(dev, sc))
 
 
-def test_plastic_surfaces(dev, tmp_path):
-    # Surface::plastic (Diffuse, Specular, Roughness) reaches the library as PG_MAT_PLASTIC
-    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True)
+def test_plastic_and_rough_matte_surfaces(dev, tmp_path):
+    # Surface::plastic (Diffuse, Specular, Roughness) reaches the library as PG_MAT_PLASTIC, and a
+    # matte roughness as the Oren-Nayar lobe
+    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True)
     film, _ = plugin_film(tmp_path, sc, "plastic")
     assert np.array_equal(film, direct_film(dev, sc))
 

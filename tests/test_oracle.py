@@ -293,6 +293,62 @@ def test_plastic_mixture_sampling_matches_evaluate(front, rough):
     assert abs(picks / n - 0.5) < 0.02
 
 
+def _matte(rho, rough):
+    return np.array([0, *rho, rough, 0, 0, 0], np.float32)
+
+
+def _upper(rng):
+    w = rng.normal(size=3)
+    w[2] = abs(w[2])
+    return w / np.linalg.norm(w)
+
+
+def test_oren_nayar_angle_form_and_reciprocity():
+    """Matte with roughness > 0 against the Oren-Nayar qualitative model written with polar and
+    azimuth angles, rho/pi (A + B max(0, cos(phi_i - phi_o)) sin(max theta) tan(min theta)) with
+    A = 1 - s^2 / (2 (s^2 + 0.33)), B = 0.45 s^2 / (s^2 + 0.09), s = pi roughness; reciprocity; the
+    cosine pdf; roughness 0 = Lambert."""
+    rho = np.array([0.3, 0.5, 0.7])
+    rng = np.random.default_rng(8)
+    for rough in (0.1, 0.35, 0.8):
+        s2 = (math.pi * rough) ** 2
+        a, b = 1 - s2 / (2 * (s2 + 0.33)), 0.45 * s2 / (s2 + 0.09)
+        mat = _matte(rho, rough)
+        for _ in range(200):
+            wo, wi = _upper(rng), _upper(rng)
+            to, ti = math.acos(wo[2]), math.acos(wi[2])
+            dphi = math.atan2(wi[1], wi[0]) - math.atan2(wo[1], wo[0])
+            ref = rho / math.pi * (a + b * max(0.0, math.cos(dphi)) * math.sin(max(ti, to)) * math.tan(min(ti, to)))
+            f, pdf = _bsdf(mat, wo, wi)
+            assert np.allclose(f, ref, rtol=5e-5), (rough, wo, wi)
+            assert abs(pdf - wi[2] / math.pi) < 1e-6
+            assert np.allclose(_bsdf(mat, wi, wo)[0], f, rtol=1e-5)
+    lam = _matte(rho, 0.0)
+    for _ in range(20):
+        f, _ = _bsdf(lam, _upper(rng), _upper(rng))
+        assert np.allclose(f, rho / math.pi, rtol=1e-6)
+
+
+@pytest.mark.parametrize("rough", [0.2, 0.7])
+def test_oren_nayar_sampling_matches_evaluate(rough):
+    """Cosine sampling of the rough matte lobe: the sample's value and pdf equal the evaluate path's
+    and E[f cos / pdf] is the directional albedo by quadrature."""
+    mat = _matte([0.4, 0.6, 0.8], rough)
+    wo = np.array([0.6, 0.2, 0.0], np.float32)
+    wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+    rng = np.random.default_rng(23)
+    n, est = 20000, np.zeros(3)
+    for u1, u2 in rng.uniform(0, 1, (n, 2)):
+        wi, f, pdf = _sample(mat, wo, u1, u2)
+        if pdf > 0:
+            f2, pdf2 = _bsdf(mat, wo, wi)
+            assert np.allclose(f, f2, rtol=1e-6, atol=1e-9) and abs(pdf - pdf2) <= 1e-6 * pdf
+            est += f * wi[2] / pdf
+    dirs, dw = _hemisphere_grid(300, 200)
+    albedo = sum(_bsdf(mat, wo, d)[0] * d[2] * w for d, w in zip(dirs, dw))
+    assert np.allclose(est / n, albedo, rtol=0.02), (est / n, albedo)
+
+
 def test_plastic_without_specular_renders_as_lambert():
     """A scene whose Lambert materials are restated as plastic with a black specular colour renders
     the same film, bit for bit."""
