@@ -473,16 +473,23 @@ __device__ __forceinline__ float fresnelDielectric(float cosI, float etaI, float
     return 0.5f * (par * par + perp * perp);
 }
 
+// A hit's material. The plastic and Oren-Nayar extras (spec) stay in the scene's table and are
+// read where a lobe needs them, so they hold no registers across the vertex.
 struct Mat {
-    uint32_t type;  // 0 Lambert, 1 conductor, 2 plastic, 3 Oren-Nayar
+    uint32_t bits;  // type (0 Lambert, 1 conductor, 2 plastic, 3 Oren-Nayar) | front face << 8
     V3 albedo;      // Lambert / Oren-Nayar / plastic diffuse, conductor F0
     float alpha;    // GGX alpha; Oren-Nayar: A
-    V3 spec;        // plastic specular reflectance; Oren-Nayar: B in x
-    bool front;     // plastic: the hit is on the triangle's front face (plasticInit's etaI, etaT)
+    uint32_t id;
+    const DevMaterial* tab;
+    __device__ __forceinline__ uint32_t type() const { return bits & 0xFFu; }
+    // plastic: the hit is on the triangle's front face (plasticInit's etaI, etaT)
+    __device__ __forceinline__ bool front() const { return (bits >> 8) != 0u; }
+    // plastic specular reflectance; Oren-Nayar: B in x
+    __device__ __forceinline__ V3 spec() const { return ld3(tab[id].spec); }
 };
 __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true) {
     const DevMaterial& m = s.mats[id];
-    return Mat{m.type, ld3(m.albedo), m.alpha, ld3(m.spec), front};
+    return Mat{m.type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, id, s.mats};
 }
 
 // Plastic's glossy lobe: MicrofacetReflection<FresnelDielectric> (Classic/Kernel.cpp:384-433,
@@ -492,8 +499,8 @@ __device__ __forceinline__ V3 plasticGlossyF(const Mat& m, V3 wo, V3 wi) {
     const V3 wh = normalize(wo + wi);
     const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
     const float k = ggxD(wh, m.alpha) / (4.0f * g * fabsf(wo.z * wi.z));
-    const float etaI = m.front ? 1.0f : 1.46f, etaT = m.front ? 1.46f : 1.0f;
-    return m.spec * (fresnelDielectric(dot(wi, wh), etaI, etaT) * k);
+    const float etaI = m.front() ? 1.0f : 1.46f, etaT = m.front() ? 1.46f : 1.0f;
+    return m.spec() * (fresnelDielectric(dot(wi, wh), etaI, etaT) * k);
 }
 __device__ __forceinline__ float lambertPdf(V3 wi) { return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f; }
 
@@ -509,18 +516,18 @@ __device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
     const bool iSteeper = fabsf(wi.z) > fabsf(wo.z);
     const float sinAlpha = iSteeper ? sinO : sinI;
     const float tanBeta = iSteeper ? sinI / fabsf(wi.z) : sinO / fabsf(wo.z);
-    return m.albedo * (kInvPi * (m.alpha + m.spec.x * maxCos * sinAlpha * tanBeta));
+    return m.albedo * (kInvPi * (m.alpha + m.spec().x * maxCos * sinAlpha * tanBeta));
 }
 
 // The generic BxDF mixture (Classic/Kernel.cpp:144-237) over plastic's valid lobes (diffuse,
 // glossy): evaluate sums them, pdf averages them, sample picks one by u1.
-__device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec) ? 2u : 0u); }
+__device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
 
 __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     if (!(dot(wo, ng) * dot(wi, ng) > 0.0f)) return {0.0f, 0.0f, 0.0f};  // Classic/Kernel.cpp:220
-    if (m.type == 0) return m.albedo * kInvPi;
-    if (m.type == 1) return conductorF(m.albedo, m.alpha, wo, wi);
-    if (m.type == kDevOrenNayar) return orenNayarF(m, wo, wi);
+    if (m.type() == 0) return m.albedo * kInvPi;
+    if (m.type() == 1) return conductorF(m.albedo, m.alpha, wo, wi);
+    if (m.type() == kDevOrenNayar) return orenNayarF(m, wo, wi);
     const uint32_t lobes = plasticLobes(m);
     V3 f{0.0f, 0.0f, 0.0f};
     if (lobes & 1u) f = f + m.albedo * kInvPi;
@@ -528,8 +535,8 @@ __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     return f;
 }
 __device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
-    if (m.type == 0 || m.type == kDevOrenNayar) return lambertPdf(wi);
-    if (m.type == 1) return conductorPdf(m.alpha, wo, wi);
+    if (m.type() == 0 || m.type() == kDevOrenNayar) return lambertPdf(wi);
+    if (m.type() == 1) return conductorPdf(m.alpha, wo, wi);
     const uint32_t lobes = plasticLobes(m);
     float p = 0.0f;
     if (lobes & 1u) p = p + lambertPdf(wi);
@@ -553,13 +560,13 @@ __device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, float alpha, float
 }
 __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
-    if (m.type == 0 || m.type == kDevOrenNayar) {  // DefaultBxDFHelper::sample (Classic/Kernel.cpp:67-74)
+    if (m.type() == 0 || m.type() == kDevOrenNayar) {  // DefaultBxDFHelper::sample (Classic/Kernel.cpp:67-74)
         s.wi = cosineHemisphere(u1, u2);
-        s.f = m.type == 0 ? m.albedo * kInvPi : orenNayarF(m, wo, s.wi);
+        s.f = m.type() == 0 ? m.albedo * kInvPi : orenNayarF(m, wo, s.wi);
         s.pdf = lambertPdf(s.wi);
         return s;
     }
-    if (m.type == 1) {
+    if (m.type() == 1) {
         s = ggxReflectSample(wo, m.alpha, u1, u2);
         if (s.pdf > 0.0f) s.f = conductorF(m.albedo, m.alpha, wo, s.wi);
         return s;

@@ -200,7 +200,7 @@ __device__ __forceinline__ bool isTrainPath(const ItemGuide& ig, uint32_t g) {
 // fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
 constexpr float kGuideMinAlpha = 0.04f;
 __device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
-    return (m.type != 1u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
+    return (m.type() != 1u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
 }
 
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
@@ -215,10 +215,11 @@ struct ShadeLds {
 };
 static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging slots");
 
-// Waves per SIMD: the plain kernel fits 5 (<= 96 VGPRs) without spilling; the guided and batched
-// ones keep what their extra state needs.
+// Waves per SIMD: 4 (<= 128 VGPRs) for the plain and batched kernels: at 5 (<= 96 VGPRs) the
+// plastic / Oren-Nayar lobes spill 14 VGPRs and the stage takes 18.9 instead of 17.0 ms on the
+// 1080p64 terrain frame. The guided ones keep 3 for their extra state.
 template <bool BATCH, bool GUIDED>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : (BATCH ? 4 : 5))))
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
 kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
        GuideShade gs, BatchTable bt) {
     __shared__ ShadeLds lds;
