@@ -48,6 +48,11 @@ extern "C" {
 #define PG_MAT_PLASTIC 2u   /* plastic (Surface/Classic/Kernel.cpp:435-473): Lambert diffuse = albedo plus
                                GGX with dielectric Fresnel (eta 1.46), specular = reserved[0..2],
                                roughness = alpha (remapRoughness applied, :378-381) */
+#define PG_MAT_MIRROR 3u    /* mirror (Classic/Kernel.cpp:663-692): perfect reflection scaled by albedo */
+#define PG_MAT_GLASS 4u     /* smooth glass (glassInit with roughness 0, FresnelSpecular :499-541, eta 1.5):
+                               reflection = albedo, transmission = reserved[0..2]; alpha must be 0 */
+#define PG_MAT_SUBSTRATE 5u /* substrate (FresnelBlend, Classic/Kernel.cpp:694-779): diffuse = albedo,
+                               specular = reserved[0..2], roughness = alpha (remapRoughness applied) */
 
 #define PG_HIT_MISS 0xFFFFFFFFu
 #define PG_HIT_QUAD 0x80000000u /* prim = PG_HIT_QUAD | quad index for a light-quad hit */
@@ -58,9 +63,11 @@ typedef struct pg_film pg_film;
 
 typedef struct pg_material {
     uint32_t type;      /* PG_MAT_* */
-    float albedo[3];    /* Lambert rho (ConstantData, ProceduralTexture/Kernel.cpp:21-26) or conductor F0 */
-    float alpha;        /* matte: roughness (0 = Lambert); conductor: GGX alpha = roughness^2; plastic: roughness */
-    float reserved[3];  /* plastic: specular rgb */
+    float albedo[3];    /* diffuse rho (ConstantData, ProceduralTexture/Kernel.cpp:21-26), conductor F0, or
+                           mirror / glass reflection */
+    float alpha;        /* matte: roughness (0 = Lambert); conductor: GGX alpha = roughness^2; plastic and
+                           substrate: roughness; mirror, glass: 0 */
+    float reserved[3];  /* plastic, substrate: specular rgb; glass: transmission rgb */
 } pg_material;
 
 typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22-28, AreaLight/Shared.hpp:22-26) */

@@ -175,12 +175,12 @@ constexpr float kInvPi = 0.31830988618379067154f;
 // ---------------------------------------------------------------------------------------------
 // Scene
 // ---------------------------------------------------------------------------------------------
-enum : uint32_t { kMatLambert = 0, kMatConductor = 1, kMatPlastic = 2 };
+enum : uint32_t { kMatLambert = 0, kMatConductor = 1, kMatPlastic = 2, kMatMirror = 3, kMatGlass = 4, kMatSubstrate = 5 };
 struct Material {
     uint32_t type;
     V3 albedo;          // Lambert rho, conductor F0, plastic diffuse
     float alpha;        // GGX alpha (plastic: remapRoughness of the scene's roughness)
-    V3 spec{0, 0, 0};   // plastic specular
+    V3 spec{0, 0, 0};   // plastic / substrate specular, glass transmission
     bool front = true;  // plastic: the hit is on the triangle's front face
     float onA = 1.0f, onB = 0.0f;  // matte with roughness > 0: Oren-Nayar A and B
     bool orenNayar = false;
@@ -517,9 +517,11 @@ V3 cosineHemisphere(float u1, float u2) {  // Sampling.hpp:44-50
 // ---------------------------------------------------------------------------------------------
 // BxDFs (Surface/Classic/Kernel.cpp). Shading space: z = shading normal.
 // ---------------------------------------------------------------------------------------------
+enum : uint32_t { kPartReflection = 1, kPartTransmission = 2, kPartSpecular = 4 };  // BxDFPart subset
 struct BsdfSample {
     V3 wi, f;
     float pdf;
+    uint32_t part = kPartReflection;
 };
 
 // GGX (Trowbridge-Reitz) isotropic distribution, Classic/Kernel.cpp:289-382, alpha = roughness^2.
@@ -595,7 +597,7 @@ float fresnelDielectric(float cosI, float etaI, float etaT) {
 // One BxDF of a material. Kinds: the Lambert lobe (LambertianReflection, Kernel.cpp:81-92), the
 // GGX lobe with a Schlick conductor Fresnel, and the GGX lobe with the dielectric Fresnel of
 // plasticInit (Kernel.cpp:441-471: eta 1.46 outside-in on the front face, swapped on the back).
-enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric, kLobeOrenNayar };
+enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric, kLobeOrenNayar, kLobeMirror, kLobeGlass, kLobeBlend };
 struct Lobe {
     LobeKind kind;
     V3 scale;
@@ -612,6 +614,12 @@ Lobes lobesOf(const Material& m) {
         r.l[r.n++] = {m.orenNayar ? kLobeOrenNayar : kLobeLambert, m.albedo};
     } else if (m.type == kMatConductor) {
         r.l[r.n++] = {kLobeConductor, m.albedo};
+    } else if (m.type == kMatMirror) {  // mirrorInit, Kernel.cpp:667-676
+        r.l[r.n++] = {kLobeMirror, m.albedo};
+    } else if (m.type == kMatGlass) {  // glassInit with zero roughness: one FresnelSpecular lobe, Kernel.cpp:608-640
+        if (valid(m.albedo) || valid(m.spec)) r.l[r.n++] = {kLobeGlass, m.albedo};
+    } else if (m.type == kMatSubstrate) {  // substrateInit, Kernel.cpp:757-777
+        if (valid(m.albedo) || valid(m.spec)) r.l[r.n++] = {kLobeBlend, m.albedo};
     } else {
         if (valid(m.albedo)) r.l[r.n++] = {kLobeLambert, m.albedo};
         if (valid(m.spec)) r.l[r.n++] = {kLobeDielectric, m.spec};
@@ -632,8 +640,32 @@ float orenNayarScale(const Material& m, V3 wo, V3 wi) {
     const float tanBeta = cI > cO ? sI / cI : sO / cO;  // beta = min(theta_i, theta_o)
     return kInvPi * (m.onA + m.onB * std::fmax(0.0f, cosDphi) * sinAlpha * tanBeta);
 }
+// FresnelBlend (Kernel.cpp:694-751): Ashikhmin-Shirley's diffuse term and a Schlick-weighted GGX
+// term, diffuse = scale, specular = m.spec.
+float fifth(float v) { return v * v * v * v * v; }
+V3 blendF(const Material& m, const V3& rd, V3 wo, V3 wi) {
+    const V3 rs = m.spec;
+    const V3 oneMinus{1.0f - rs.x, 1.0f - rs.y, 1.0f - rs.z};
+    const float shape = (28.0f / (23.0f * kPi)) * (1.0f - fifth(1.0f - 0.5f * std::fabs(wi.z))) *
+                        (1.0f - fifth(1.0f - 0.5f * std::fabs(wo.z)));
+    const V3 wh = normalize(wo + wi);
+    const float c = dot(wi, wh);
+    const float tail = fifth(1.0f - c);  // Schlick: rs + (1 - rs) (1 - cos)^5
+    const V3 fres{rs.x + oneMinus.x * tail, rs.y + oneMinus.y * tail, rs.z + oneMinus.z * tail};
+    const float spec = ggxD(wh, m.alpha) / (4.0f * std::fmax(std::fabs(wi.z), std::fabs(wo.z)) * std::fabs(c));
+    return mul(rd, oneMinus) * shape + fres * spec;
+}
+// Half cosine, half visible-normal GGX (the reference's pdf divides by 1 + Lambda(wo) + Lambda(wh),
+// which is not the density its sampler draws from).
+float blendPdf(const Material& m, V3 wo, V3 wi) {
+    if (wi.z <= 0.0f) return 0.0f;
+    const V3 wh = normalize(wo + wi);
+    return 0.5f * (std::fabs(wi.z) * kInvPi + ggxD(wh, m.alpha) / (4.0f * std::fabs(wo.z) * (1.0f + ggxLambda(wo, m.alpha))));
+}
 V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     if (b.kind == kLobeLambert) return b.scale * kInvPi;
+    if (b.kind == kLobeMirror || b.kind == kLobeGlass) return {0.0f, 0.0f, 0.0f};  // delta lobes evaluate to 0
+    if (b.kind == kLobeBlend) return blendF(m, b.scale, wo, wi);
     if (b.kind == kLobeOrenNayar) return b.scale * orenNayarScale(m, wo, wi);
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
     if (b.kind == kLobeConductor) return conductorF(m, wo, wi);
@@ -644,11 +676,53 @@ V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     return b.scale * (fr * k);
 }
 float lobePdf(const Material& m, const Lobe& b, V3 wo, V3 wi) {
+    if (b.kind == kLobeMirror || b.kind == kLobeGlass) return 0.0f;
+    if (b.kind == kLobeBlend) return blendPdf(m, wo, wi);
     if (b.kind == kLobeLambert || b.kind == kLobeOrenNayar) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
     return conductorPdf(m, wo, wi);  // the same visible-normal pdf for either Fresnel
 }
 BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
+    if (b.kind == kLobeMirror) {  // mirrorSample, Kernel.cpp:678-692: pdf 1, f = R / |cos|
+        s.wi = V3{-wo.x, -wo.y, wo.z};
+        s.f = b.scale * (1.0f / std::fabs(s.wi.z));
+        s.pdf = 1.0f;
+        s.part = kPartReflection | kPartSpecular;
+        return s;
+    }
+    if (b.kind == kLobeGlass) {  // FresnelSpecular::sample, Kernel.cpp:522-540 (eta 1.5, face-swapped)
+        const float eI = m.front ? 1.0f : 1.5f, eT = m.front ? 1.5f : 1.0f;
+        const float fr = fresnelDielectric(wo.z, eI, eT);
+        if (u1 < fr) {
+            s.wi = V3{-wo.x, -wo.y, wo.z};
+            s.pdf = fr;
+            s.f = b.scale * (fr / std::fabs(s.wi.z));
+            s.part = kPartReflection | kPartSpecular;
+            return s;
+        }
+        // Snell's law about +z with eta = eI / eT (refract, Transform.hpp:410-421)
+        const float eta = eI / eT;
+        const float sin2T = eta * eta * std::fmax(0.0f, 1.0f - wo.z * wo.z);
+        if (sin2T >= 1.0f) return s;
+        const float cosT = std::sqrt(std::fmax(0.0f, 1.0f - sin2T));
+        s.wi = normalize(V3{-eta * wo.x, -eta * wo.y, (eta * wo.z - cosT) - eta * wo.z});
+        s.pdf = 1.0f - fr;
+        s.f = m.spec * (eta * eta * s.pdf / std::fabs(s.wi.z));  // radiance scaled by (eI / eT)^2
+        s.part = kPartTransmission | kPartSpecular;
+        return s;
+    }
+    if (b.kind == kLobeBlend) {  // FresnelBlend::sample, Kernel.cpp:734-750
+        if (u1 < 0.5f) {
+            s.wi = cosineHemisphere(std::fmin(2.0f * u1, 1.0f), u2);
+        } else {
+            const V3 wh = ggxSampleWh(wo, m.alpha, std::fmin(2.0f * (u1 - 0.5f), 1.0f), u2);
+            s.wi = wh * (2.0f * dot(wo, wh)) - wo;
+            if (s.wi.z <= 0.0f) return s;
+        }
+        s.pdf = blendPdf(m, wo, s.wi);
+        s.f = blendF(m, b.scale, wo, s.wi);
+        return s;
+    }
     if (b.kind == kLobeLambert || b.kind == kLobeOrenNayar) {  // DefaultBxDFHelper::sample, Kernel.cpp:67-74
         s.wi = cosineHemisphere(u1, u2);
         s.f = lobeF(m, b, wo, s.wi);
@@ -694,7 +768,7 @@ BsdfSample bsdfSample(const Material& m, V3 wo, V3 ng, float u1, float u2) {
     BsdfSample s = lobeSample(m, ls.l[pick], wo, u1 * cnt - static_cast<float>(pick), u2);
     if (s.pdf <= 0.0f) return s;
     const bool same = sameSide(wo, s.wi, ng);
-    for (uint32_t k = 0; k < ls.n; ++k) {
+    for (uint32_t k = 0; k < ls.n && !(s.part & kPartSpecular); ++k) {
         if (k == pick) continue;
         if (same) s.f = s.f + lobeF(m, ls.l[k], wo, s.wi);
         s.pdf += lobePdf(m, ls.l[k], wo, s.wi);
@@ -842,8 +916,8 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
         const V3 ng = fr.toLocal(n);
         const V3 hit = ray.o + ray.d * h.t;
 
-        // multipleImportanceSampling, :23-107
-        {
+        // multipleImportanceSampling, :23-107, only where init left noSpecular set (:158-163)
+        if (mat.type != kMatMirror && mat.type != kMatGlass) {
             const LightSel sel = selectLight(s, smp.next());
             V3 partA{0, 0, 0}, partB{0, 0, 0};
             float u1, u2;
@@ -900,8 +974,8 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
             pf = pf * (1.0f / (1.0f - q));
         }
         ray.d = fr.toWorld(ss.wi);
-        ray.o = hit + n * 0.001f;  // reflection only: both BxDFs reflect
-        specular = false;
+        ray.o = hit + n * ((ss.part & kPartReflection) ? 0.001f : -0.001f);  // :184-187
+        specular = (ss.part & kPartSpecular) != 0;  // :189
         ++depth;
     }
     return L;
@@ -978,10 +1052,8 @@ void splat(F* rgbw, const Params& p, float fx, float fy, V3 L, std::vector<std::
 // One material row of or_scene_create's mats array.
 Material materialOf(const float* m) {
     Material r{static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]};
-    if (r.type == kMatPlastic) {
-        r.alpha = remapRoughness(m[4]);
-        r.spec = {m[5], m[6], m[7]};
-    }
+    if (r.type == kMatPlastic || r.type == kMatSubstrate) r.alpha = remapRoughness(m[4]);
+    if (r.type == kMatPlastic || r.type == kMatSubstrate || r.type == kMatGlass) r.spec = {m[5], m[6], m[7]};
     if (r.type == kMatLambert && m[4] > 0.0f) {  // OrenNayar(reflection, sigma), Kernel.cpp:99-106
         const float sigma = m[4] * kPi, s2 = sigma * sigma;
         r.orenNayar = true;
@@ -1001,8 +1073,8 @@ uint64_t or_interval_to_index(uint32_t m, uint32_t sample, uint32_t px, uint32_t
 
 // Scene arrays:
 //   verts[3*nv]; idx[3*nt]; triObj[nt]; objMat[nobj];
-//   mats[8*nmat] = {type, r, g, b, alpha, spec r, g, b} (matte and plastic: the alpha slot holds the
-//   roughness);
+//   mats[8*nmat] = {type, r, g, b, alpha, spec r, g, b} (matte, plastic and substrate: the alpha
+//   slot holds the roughness; glass: spec is the transmission colour);
 //   quads[12*nq] = {origin xyz, U xyz, V xyz, radiance rgb};
 //   cam[14] = {base xyz, lookAt xyz, up xyz, sizeX, sizeY, focal, aperture, 0}; env[3]
 void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* triObj,

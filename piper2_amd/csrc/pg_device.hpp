@@ -519,6 +519,32 @@ __device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
     return m.albedo * (kInvPi * (m.alpha + m.spec().x * maxCos * sinAlpha * tanBeta));
 }
 
+// FresnelBlend::evaluate (Classic/Kernel.cpp:705-722): Ashikhmin-Shirley diffuse + Schlick-weighted
+// GGX specular (substrate: diffuse = albedo, specular = spec).
+__device__ __forceinline__ float pow5(float v) { return (v * v) * (v * v) * v; }
+__device__ __forceinline__ V3 fresnelBlendF(const Mat& m, V3 wo, V3 wi) {
+    const V3 rs = m.spec();
+    const V3 negRem = rs - V3{1.0f, 1.0f, 1.0f};
+    const float kd = (-28.0f / (23.0f * kPi)) * (1.0f - pow5(1.0f - 0.5f * fabsf(wi.z))) *
+                     (1.0f - pow5(1.0f - 0.5f * fabsf(wo.z)));
+    const V3 wh = normalize(wo + wi);
+    const float c = dot(wi, wh);
+    const V3 schlick = rs + negRem * pow5(c - 1.0f);
+    const float ks = ggxD(wh, m.alpha) / (4.0f * fmaxf(fabsf(wi.z), fabsf(wo.z)) * fabsf(c));
+    return mul(m.albedo, negRem) * kd + schlick * ks;
+}
+// FresnelBlend::pdf with the visible-normal density of its GGX half (the reference's pdf divides by
+// 1 + Lambda(wo) + Lambda(wh), which does not match the sampler it pairs with).
+__device__ __forceinline__ float fresnelBlendPdf(const Mat& m, V3 wo, V3 wi) {
+    if (wi.z <= 0.0f) return 0.0f;
+    const V3 wh = normalize(wo + wi);
+    return 0.5f * (fabsf(wi.z) * kInvPi + ggxD(wh, m.alpha) / (4.0f * fabsf(wo.z) * (1.0f + ggxLambda(wo, m.alpha))));
+}
+
+// Mirror and smooth glass are specular: their init clears noSpecular, so the vertex does no light
+// sampling and the next hit's emission counts (PathTracing/Kernel.cpp:119, 158-163, 189).
+__device__ __forceinline__ bool isSpecular(const Mat& m) { return m.type() == 3u || m.type() == 4u; }
+
 // The generic BxDF mixture (Classic/Kernel.cpp:144-237) over plastic's valid lobes (diffuse,
 // glossy): evaluate sums them, pdf averages them, sample picks one by u1.
 __device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
@@ -528,6 +554,8 @@ __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
     if (m.type() == 0) return m.albedo * kInvPi;
     if (m.type() == 1) return conductorF(m.albedo, m.alpha, wo, wi);
     if (m.type() == kDevOrenNayar) return orenNayarF(m, wo, wi);
+    if (m.type() == 5u) return fresnelBlendF(m, wo, wi);
+    if (m.type() != 2u) return {0.0f, 0.0f, 0.0f};  // specular: blackBodyEvaluate / no lobe to evaluate
     const uint32_t lobes = plasticLobes(m);
     V3 f{0.0f, 0.0f, 0.0f};
     if (lobes & 1u) f = f + m.albedo * kInvPi;
@@ -537,6 +565,8 @@ __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
 __device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
     if (m.type() == 0 || m.type() == kDevOrenNayar) return lambertPdf(wi);
     if (m.type() == 1) return conductorPdf(m.alpha, wo, wi);
+    if (m.type() == 5u) return fresnelBlendPdf(m, wo, wi);
+    if (m.type() != 2u) return 0.0f;
     const uint32_t lobes = plasticLobes(m);
     float p = 0.0f;
     if (lobes & 1u) p = p + lambertPdf(wi);
@@ -569,6 +599,45 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
     if (m.type() == 1) {
         s = ggxReflectSample(wo, m.alpha, u1, u2);
         if (s.pdf > 0.0f) s.f = conductorF(m.albedo, m.alpha, wo, s.wi);
+        return s;
+    }
+    if (m.type() == 3u) {  // mirrorSample (Classic/Kernel.cpp:678-692)
+        s.wi = V3{-wo.x, -wo.y, wo.z};
+        s.f = m.albedo * (1.0f / fabsf(s.wi.z));
+        s.pdf = 1.0f;
+        return s;
+    }
+    if (m.type() == 4u) {  // FresnelSpecular::sample (Classic/Kernel.cpp:522-540), glassInit eta 1.5 (:608-640)
+        const V3 rt = m.spec();
+        if (!valid(m.albedo) && !valid(rt)) return s;  // glassInit: mask 0
+        const float etaI = m.front() ? 1.0f : 1.5f, etaT = m.front() ? 1.5f : 1.0f;
+        const float fr = fresnelDielectric(wo.z, etaI, etaT);
+        if (u1 < fr) {
+            s.wi = V3{-wo.x, -wo.y, wo.z};
+            s.pdf = fr;
+            s.f = m.albedo * (fr / fabsf(s.wi.z));
+        } else {  // refract(wo, +z, etaI / etaT), Kernel/Transform.hpp:410-421
+            const float eta = etaI / etaT;
+            const float cosI = wo.z;
+            const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - cosI * cosI);
+            if (sin2T >= 1.0f) return s;
+            const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sin2T));
+            s.wi = normalize(V3{0.0f, 0.0f, eta * cosI - cosT} - wo * eta);
+            s.pdf = 1.0f - fr;
+            s.f = rt * (eta * eta * s.pdf / fabsf(s.wi.z));
+        }
+        return s;
+    }
+    if (m.type() == 5u) {  // FresnelBlend::sample (Classic/Kernel.cpp:734-750)
+        if (u1 < 0.5f) {
+            s.wi = cosineHemisphere(fminf(2.0f * u1, 1.0f), u2);
+        } else {
+            const V3 wh = ggxSampleWh(wo, m.alpha, fminf(2.0f * (u1 - 0.5f), 1.0f), u2);
+            s.wi = wh * (2.0f * dot(wo, wh)) - wo;
+            if (s.wi.z <= 0.0f) return s;
+        }
+        s.pdf = fresnelBlendPdf(m, wo, s.wi);
+        s.f = fresnelBlendF(m, wo, s.wi);
         return s;
     }
     const uint32_t lobes = plasticLobes(m);

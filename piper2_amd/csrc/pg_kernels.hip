@@ -280,67 +280,71 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 const V3 ng = fr.toLocal(nrm);
                 const V3 hp = ray.o + ray.d * hit4.x;
                 // The vertex draws at most 8 consecutive dimensions: light select, light (2),
-                // MIS BSDF (2), continuation (2), Russian roulette (1), in that order.
+                // MIS BSDF (2), continuation (2), Russian roulette (1), in that order. A specular
+                // vertex samples no light and draws only the last three.
+                const bool specV = isSpecular(mat);
                 float u[8];
                 sobolWindow(s, ps.p[path].idx, meta.x, u);
-                uint32_t used = 7;
+                uint32_t used = specV ? 2u : 7u;
 
-                // light selection: uniformSelect over {environment, quads}
-                const uint32_t count = 1u + s.numQuads;
-                const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
-                const float selInv = 1.0f / (1.0f / static_cast<float>(count));
-                const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
-                if (ls.pdf > 0.0f && valid(ls.rad)) {
-                    const V3 wi = fr.toLocal(ls.dir);
-                    const V3 f = bsdfEval(mat, wo, wi, ng);
-                    if (valid(f)) {
-                        float w = fabsf(wi.z) / ls.pdf;
-                        w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
-                        const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
-                        // staged where it is made, so its registers are free for the rest of the vertex
-                        lds.nee[tid] = ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y,
-                                                 ls.dir.z, ls.dist, c.x, c.y, c.z, path};
-                        pushShadow = true;
+                if (!specV) {  // multipleImportanceSampling (:23-107) runs only if noSpecular
+                    // light selection: uniformSelect over {environment, quads}
+                    const uint32_t count = 1u + s.numQuads;
+                    const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
+                    const float selInv = 1.0f / (1.0f / static_cast<float>(count));
+                    const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
+                    if (ls.pdf > 0.0f && valid(ls.rad)) {
+                        const V3 wi = fr.toLocal(ls.dir);
+                        const V3 f = bsdfEval(mat, wo, wi, ng);
+                        if (valid(f)) {
+                            float w = fabsf(wi.z) / ls.pdf;
+                            w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
+                            const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
+                            // staged where it is made, so its registers are free for the rest of the vertex
+                            lds.nee[tid] = ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y,
+                                                     ls.dir.z, ls.dist, c.x, c.y, c.z, path};
+                            pushShadow = true;
+                        }
                     }
-                }
-                const BsdfSample bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
-                if (bs.pdf > 0.0f && valid(bs.f)) {
-                    const float weight = fabsf(bs.wi.z) / bs.pdf;
-                    const V3 dir = fr.toWorld(bs.wi);
-                    const V3 c = mul(thr, bs.f * weight * selInv);
-                    if (!envBlack) {
-                        *reinterpret_cast<MisRec*>(&lds.rec2[tid]) =
-                            MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
-                        pushMis = true;
-                    } else if (sel != 0u) {
-                        // With a black environment the BSDF ray contributes only if the selected
-                        // quad is its closest hit: the quad must be hit, beat every other quad in
-                        // (t, index) order (closestHit's quad loop), and no triangle may lie in
-                        // (tmin, tq] (triangles win ties). The first two are decided here; the
-                        // triangle test becomes an occlusion query on the shadow queue.
-                        const Ray mray{hp, dir};
-                        const uint32_t qi = sel - 1u;
-                        const DevQuad& qd = s.quads[qi];
-                        float tq;
-                        if (intersectQuad(mray, qd, 1e-3f, 1e5f, tq)) {
-                            bool first = true;
-                            for (uint32_t k = 0; k < s.numQuads && first; ++k) {
-                                float t2;
-                                if (k != qi && intersectQuad(mray, s.quads[k], 1e-3f, 1e5f, t2) &&
-                                    (t2 < tq || (t2 == tq && k < qi)))
-                                    first = false;
-                            }
-                            if (first) {
-                                const V3 qn = quadNormal(qd, dir);
-                                const V3 rad = dot(qn, dir) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
-                                const float lpdf = tq * tq / (fabsf(dot(qn, dir)) * qd.area);
-                                if (valid(rad) && lpdf > 0.0f) {
-                                    const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
-                                    // tq > 0: the next float up makes the range (tmin, tq] inclusive
-                                    const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                                    lds.rec2[tid] = ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
-                                                              cm.x, cm.y, cm.z, path | kMisFlag};
-                                    pushMisQuery = true;
+                    const BsdfSample bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
+                    if (bs.pdf > 0.0f && valid(bs.f)) {
+                        const float weight = fabsf(bs.wi.z) / bs.pdf;
+                        const V3 dir = fr.toWorld(bs.wi);
+                        const V3 c = mul(thr, bs.f * weight * selInv);
+                        if (!envBlack) {
+                            *reinterpret_cast<MisRec*>(&lds.rec2[tid]) =
+                                MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
+                            pushMis = true;
+                        } else if (sel != 0u) {
+                            // With a black environment the BSDF ray contributes only if the selected
+                            // quad is its closest hit: the quad must be hit, beat every other quad in
+                            // (t, index) order (closestHit's quad loop), and no triangle may lie in
+                            // (tmin, tq] (triangles win ties). The first two are decided here; the
+                            // triangle test becomes an occlusion query on the shadow queue.
+                            const Ray mray{hp, dir};
+                            const uint32_t qi = sel - 1u;
+                            const DevQuad& qd = s.quads[qi];
+                            float tq;
+                            if (intersectQuad(mray, qd, 1e-3f, 1e5f, tq)) {
+                                bool first = true;
+                                for (uint32_t k = 0; k < s.numQuads && first; ++k) {
+                                    float t2;
+                                    if (k != qi && intersectQuad(mray, s.quads[k], 1e-3f, 1e5f, t2) &&
+                                        (t2 < tq || (t2 == tq && k < qi)))
+                                        first = false;
+                                }
+                                if (first) {
+                                    const V3 qn = quadNormal(qd, dir);
+                                    const V3 rad = dot(qn, dir) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
+                                    const float lpdf = tq * tq / (fabsf(dot(qn, dir)) * qd.area);
+                                    if (valid(rad) && lpdf > 0.0f) {
+                                        const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
+                                        // tq > 0: the next float up makes the range (tmin, tq] inclusive
+                                        const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
+                                        lds.rec2[tid] = ShadowRec{hp.x, hp.y, hp.z, 1e-3f, dir.x, dir.y, dir.z, tqUp,
+                                                                  cm.x, cm.y, cm.z, path | kMisFlag};
+                                        pushMisQuery = true;
+                                    }
                                 }
                             }
                         }
@@ -350,10 +354,11 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 // continuation (:164-190). With a guide: one-sample MIS that draws from the guide
                 // mixture with probability alpha, else from the BSDF, and weights by the combined
                 // density alpha p_guide + (1 - alpha) p_bsdf, so the estimator stays unbiased.
-                const float uc1 = d9 ? u[6] : u[5], uc2 = d9 ? u[5] : u[6];
+                const float c0 = specV ? u[0] : u[5], c1 = specV ? u[1] : u[6];
+                const float uc1 = d9 ? c1 : c0, uc2 = d9 ? c0 : c1;
                 BsdfSample ss;
                 float pdfMix = 0.0f, a = 0.0f;
-                if (!GUIDED) {
+                if (!GUIDED || specV) {
                     ss = bsdfSample(mat, wo, ng, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
@@ -384,18 +389,19 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     bool alive = true;
                     if (meta.y > rc.minRR) {
                         const float qrr = fmaxf(0.05f, 1.0f - maxc(thr));
-                        used = 8;
-                        if (u[7] < qrr) alive = false;
+                        used += 1u;
+                        if ((specV ? u[2] : u[7]) < qrr) alive = false;
                         else thr = thr * (1.0f / (1.0f - qrr));
                     }
                     if (alive) {
                         const V3 nd = fr.toWorld(ss.wi);
-                        const V3 no = hp + nrm * 0.001f;
+                        // :184-187: offset along +Ng for reflection, -Ng for transmission
+                        const V3 no = hp + nrm * (ss.wi.z < 0.0f ? -0.001f : 0.001f);
                         ps.p[path].o = make_float4(no.x, no.y, no.z, 0.0f);
                         ps.p[path].d = make_float4(nd.x, nd.y, nd.z, 0.0f);
                         ps.p[path].thr = make_float4(thr.x, thr.y, thr.z, 0.0f);
                         meta.y += 1u;
-                        meta.z &= ~1u;  // no longer a camera / specular ray
+                        meta.z = specV ? (meta.z | 1u) : (meta.z & ~1u);  // :189 specular = part & Specular
                         pushExt = true;
                         const uint32_t r = (meta.z >> 8) & 0xFFu;
                         const bool train = BATCH ? isTrainPath(bt.guide[sceneOf(meta.z)], meta.w) : isTrainPath(gs, meta.w);

@@ -36,12 +36,15 @@ struct alignas(16) WideNode {
 static_assert(sizeof(WideNode<8>) == 224, "BVH8 node is 224 B");
 static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 B");
 
-constexpr uint32_t kDevOrenNayar = 3;  // device-only type: matte with roughness > 0
+// Material types: the public PG_MAT_* values (0 Lambert, 1 conductor, 2 plastic, 3 mirror,
+// 4 smooth glass, 5 substrate) plus this device-only one.
+constexpr uint32_t kDevOrenNayar = 16;  // matte with roughness > 0
 struct DevMaterial {  // 32 B
-    uint32_t type;     // 0 Lambert, 1 conductor, 2 plastic, 3 Oren-Nayar
-    float albedo[3];   // Lambert / Oren-Nayar / plastic diffuse reflectance, conductor F0
-    float alpha;       // GGX alpha (plastic: remapped roughness); Oren-Nayar: A
-    float spec[3];     // plastic specular reflectance; Oren-Nayar: {B, 0, 0}
+    uint32_t type;
+    float albedo[3];   // diffuse reflectance (Lambert, Oren-Nayar, plastic, substrate), conductor F0,
+                       // mirror / glass reflection
+    float alpha;       // GGX alpha (plastic, substrate: remapped roughness); Oren-Nayar: A
+    float spec[3];     // plastic / substrate specular; glass transmission; Oren-Nayar: {B, 0, 0}
 };
 
 struct DevQuad {  // Plane primitive (PerPlaneData) + DiffuseAreaLightData, 80 B

@@ -136,13 +136,17 @@ struct FlatScene {
 
 uint32_t materialOf(FlatScene& f, const Surface& s) {
     pg_material m{};
-    m.type = s.kind == Surface::Kind::Matte ? PG_MAT_LAMBERT
-             : s.kind == Surface::Kind::Conductor ? PG_MAT_CONDUCTOR
-                                                   : PG_MAT_PLASTIC;
+    switch (s.kind) {
+        case Surface::Kind::Matte: m.type = PG_MAT_LAMBERT; break;
+        case Surface::Kind::Conductor: m.type = PG_MAT_CONDUCTOR; break;
+        case Surface::Kind::Plastic: m.type = PG_MAT_PLASTIC; break;
+        case Surface::Kind::Mirror: m.type = PG_MAT_MIRROR; break;
+        case Surface::Kind::Glass: m.type = PG_MAT_GLASS; break;
+        case Surface::Kind::Substrate: m.type = PG_MAT_SUBSTRATE; break;
+    }
     for (int k = 0; k < 3; ++k) m.albedo[k] = s.color[k];
     m.alpha = s.alpha;
-    if (s.kind == Surface::Kind::Plastic)
-        for (int k = 0; k < 3; ++k) m.reserved[k] = s.specular[k];
+    for (int k = 0; k < 3; ++k) m.reserved[k] = s.specular[k];
     f.materials.push_back(m);
     return static_cast<uint32_t>(f.materials.size() - 1);
 }
@@ -418,6 +422,26 @@ std::shared_ptr<Surface> Surface::plastic(std::array<float, 3> diffuse, std::arr
     s->kind = Kind::Plastic, s->color = diffuse, s->alpha = roughness, s->specular = specular;
     return s;
 }
+std::shared_ptr<Surface> Surface::mirror(std::array<float, 3> reflection) {
+    auto s = std::make_shared<Surface>();
+    s->kind = Kind::Mirror, s->color = reflection, s->alpha = 0.0f;
+    return s;
+}
+std::shared_ptr<Surface> Surface::glass(std::array<float, 3> reflection, std::array<float, 3> transmission,
+                                        float roughnessX, float roughnessY) {
+    if (roughnessX != 0.0f || roughnessY != 0.0f) raise("glass: only smooth glass (roughness 0) is supported");
+    auto s = std::make_shared<Surface>();
+    s->kind = Kind::Glass, s->color = reflection, s->alpha = 0.0f, s->specular = transmission;
+    return s;
+}
+std::shared_ptr<Surface> Surface::substrate(std::array<float, 3> diffuse, std::array<float, 3> specular,
+                                            float roughnessX, float roughnessY) {
+    if (roughnessX != roughnessY) raise("substrate: anisotropic roughness is not supported");
+    if (!std::isfinite(roughnessX)) raise("substrate roughness must be finite");
+    auto s = std::make_shared<Surface>();
+    s->kind = Kind::Substrate, s->color = diffuse, s->alpha = roughnessX, s->specular = specular;
+    return s;
+}
 std::shared_ptr<Light> Light::areaPlane(std::array<float, 3> origin, std::array<float, 3> u, std::array<float, 3> v,
                                         std::array<float, 3> radiance) {
     auto l = std::make_shared<Light>();
@@ -435,6 +459,8 @@ const std::vector<std::string>& hipBuiltinSymbols() {
     static const std::vector<std::string> s = {
         "rayGen",      "sobolStart", "sobolGenerate", "trace",         "accumulate", "box",
         "matteInit",   "conductorInit", "plasticInit", "plasticSample", "plasticEvaluate", "plasticPdf",
+        "mirrorInit",  "mirrorSample", "blackBodyEvaluate", "blackBodyPdf", "glassInit", "glassSample",
+        "glassEvaluate", "glassPdf", "substrateInit", "substrateSample", "substrateEvaluate", "substratePdf",
         "diffuseSample", "diffuseEvaluate", "diffusePdf", "planeSample",
         "uniformSelect", "SEInit",   "SESample",      "SEEvaluate",    "SEPdf",      "constantTexture"};
     return s;

@@ -162,16 +162,25 @@ private:
 // constant textures) and the GGX conductor this path adds (alpha = roughness^2, Schlick F0).
 class Surface : public Object {
 public:
-    enum class Kind { Matte, Conductor, Plastic };
+    enum class Kind { Matte, Conductor, Plastic, Mirror, Glass, Substrate };
     // roughness 0 = Lambert, in (0, 1] Oren-Nayar (matteInit)
     static std::shared_ptr<Surface> matte(std::array<float, 3> reflectance, float roughness = 0.0f);
     static std::shared_ptr<Surface> conductor(std::array<float, 3> f0, float roughness);
     // roughness as the reference's config gives it: the library applies remapRoughness
     static std::shared_ptr<Surface> plastic(std::array<float, 3> diffuse, std::array<float, 3> specular, float roughness);
+    // Mirror (Classic.cpp:148-170: Reflection)
+    static std::shared_ptr<Surface> mirror(std::array<float, 3> reflection);
+    // Glass (Reflection, Transmission, RoughnessX, RoughnessY): the smooth form only (both roughnesses 0)
+    static std::shared_ptr<Surface> glass(std::array<float, 3> reflection, std::array<float, 3> transmission,
+                                          float roughnessX = 0.0f, float roughnessY = 0.0f);
+    // Substrate (Classic.cpp:172-195: Diffuse, Specular, RoughnessX, RoughnessY): isotropic only
+    static std::shared_ptr<Surface> substrate(std::array<float, 3> diffuse, std::array<float, 3> specular,
+                                              float roughnessX, float roughnessY);
     Kind kind;
-    std::array<float, 3> color;          // matte reflectance, conductor F0, plastic diffuse
-    float alpha;                         // matte roughness, conductor GGX alpha, plastic roughness
-    std::array<float, 3> specular{};     // plastic
+    std::array<float, 3> color;          // matte reflectance, conductor F0, plastic / substrate diffuse,
+                                         // mirror / glass reflection
+    float alpha;                         // matte roughness, conductor GGX alpha, plastic / substrate roughness
+    std::array<float, 3> specular{};     // plastic / substrate specular, glass transmission
 };
 class Medium : public Object {};
 // Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V) and the environment

@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-LAMBERT, CONDUCTOR, PLASTIC = 0, 1, 2
+LAMBERT, CONDUCTOR, PLASTIC, MIRROR, GLASS, SUBSTRATE = 0, 1, 2, 3, 4, 5
 
 
 def calc_render_rect(width, height, device_aspect, fit_fill=True):
@@ -149,13 +149,16 @@ class _Builder:
         return sc
 
 
-def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False):
+def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
+            specular=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
     tilted one-quad plastic panel that paths reach on both faces; rough_walls=True gives the walls
-    seeded matte roughness (Oren-Nayar). The light and the rest of the geometry stay as in the
-    plain scene."""
+    seeded matte roughness (Oren-Nayar); specular=True makes the floor substrate (its albedo as the
+    diffuse colour) and adds a smooth glass sphere (outward winding, so rays enter through front
+    faces and leave through back faces) and a mirror panel before the back wall, tilted to show the
+    ceiling. The light and the rest of the geometry stay as in the plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -175,7 +178,10 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
         else:
             rng.shuffle(alb)
         v, t = _quad_tris(*corners)
-        b.add(v, t, b.material(LAMBERT, alb, prng.uniform(0.2, 0.9) if rough_walls else 0.0))
+        if specular and name == "floor":
+            b.add(v, t, b.material(SUBSTRATE, alb, 0.3, [0.25, 0.25, 0.25]))
+        else:
+            b.add(v, t, b.material(LAMBERT, alb, prng.uniform(0.2, 0.9) if rough_walls else 0.0))
     for _ in range(2):
         size = rng.uniform(0.3, 0.6, 3)
         yaw = rng.uniform(0.0, math.pi / 2)
@@ -191,12 +197,19 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
         panel = [[-0.9, 0.1, -0.2], [-0.3, 0.1, -0.6], [-0.3, 0.7, -0.5], [-0.9, 0.7, -0.1]]
         v, t = _quad_tris(*panel)
         b.add(v, t, b.material(PLASTIC, [0.5, 0.3, 0.2], 0.2, [0.8, 0.8, 0.8]))
+    if specular:
+        sv, st = icosphere(2)
+        b.add(sv * 0.28 + np.array([0.3, 0.05, 0.35]), st, b.material(GLASS, [1.0, 1.0, 1.0], 0.0, [0.95, 0.95, 0.95]))
+        mirror = [[-0.7, -0.3, -0.55], [-0.1, -0.3, -0.55], [-0.1, 0.393, -0.95], [-0.7, 0.393, -0.95]]  # tilted up 30 deg
+        v, t = _quad_tris(*mirror)
+        b.add(v, t, b.material(MIRROR, [0.9, 0.9, 0.9]))
     quads = [[-0.25, 0.999, -0.25, 0.5, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0]]
     aspect = width / height
     cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
-                    ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else ""))
+                    ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else "")
+                    + ("_specular" if specular else ""))
 
 
 def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500),

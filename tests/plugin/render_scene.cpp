@@ -103,6 +103,12 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
         std::shared_ptr<Surface> surface;
         if (m[0] == 2.0f) {
             surface = Surface::plastic({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4]);
+        } else if (m[0] == 3.0f) {
+            surface = Surface::mirror({m[1], m[2], m[3]});
+        } else if (m[0] == 4.0f) {
+            surface = Surface::glass({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4], m[4]);
+        } else if (m[0] == 5.0f) {
+            surface = Surface::substrate({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4], m[4]);
         } else {
             surface = std::make_shared<Surface>();  // alpha as stored (Surface::conductor squares a roughness)
             surface->kind = m[0] == 0.0f ? Surface::Kind::Matte : Surface::Kind::Conductor;

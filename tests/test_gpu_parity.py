@@ -180,6 +180,19 @@ def test_rough_matte_paths_match_oracle(dev):
     _path_parity(sc, dev)
 
 
+def test_specular_surfaces_paths_match_oracle(dev):
+    # substrate floor, smooth glass sphere (entry and exit faces, transmission spawn below the
+    # surface), tilted mirror: specular vertices sample no light and count the next hit's emission
+    sc = scenes.cornell(width=128, height=128, spp=16, specular=True)
+    _path_parity(sc, dev)
+
+
+def test_specular_surfaces_compat_flags_match_oracle(dev):
+    sc = scenes.cornell(width=128, height=128, spp=8, specular=True)
+    flags = pg.PG_COMPAT_D1 | pg.PG_COMPAT_D3 | pg.PG_COMPAT_D9 | pg.PG_COMPAT_D13
+    _path_parity(sc, dev, flags=flags)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

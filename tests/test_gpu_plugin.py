@@ -64,6 +64,12 @@ def test_plastic_and_rough_matte_surfaces(dev, tmp_path):
     assert np.array_equal(film, direct_film(dev, sc))
 
 
+def test_mirror_glass_substrate_surfaces(dev, tmp_path):
+    sc = scenes.cornell(width=64, height=48, spp=8, specular=True)
+    film, _ = plugin_film(tmp_path, sc, "specular")
+    assert np.array_equal(film, direct_film(dev, sc))
+
+
 def test_conductors_compat_flags_and_environment(dev, tmp_path):
     sc = scenes.terrain_spheres(width=64, height=40, spp=4, grid=(80, 40), n_spheres=10)
     sc["env"] = np.array([0.2, 0.3, 0.4], np.float32)

@@ -349,6 +349,97 @@ def test_oren_nayar_sampling_matches_evaluate(rough):
     assert np.allclose(est / n, albedo, rtol=0.02), (est / n, albedo)
 
 
+def test_mirror_reflects_with_unit_weight():
+    """Mirror: the sample is the mirror direction with f |cos| / pdf = R; evaluate and pdf are zero
+    (blackBodyEvaluate / blackBodyPdf)."""
+    mat = np.array([3, 0.9, 0.8, 0.7, 0, 0, 0, 0], np.float32)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        wo = _upper(rng)
+        wi, f, pdf = _sample(mat, wo, *rng.uniform(0, 1, 2))
+        assert np.allclose(wi, [-wo[0], -wo[1], wo[2]], atol=1e-7) and pdf == 1.0
+        assert np.allclose(f * abs(wi[2]) / pdf, [0.9, 0.8, 0.7], rtol=1e-6)
+        f2, p2 = _bsdf(mat, wo, _upper(rng))
+        assert not f2.any() and p2 == 0.0
+
+
+@pytest.mark.parametrize("front", [True, False])
+def test_smooth_glass_fresnel_split_and_snell(front):
+    """Smooth glass: u1 < F reflects (weight R), else refracts by Snell's law in the plane of
+    incidence with weight T (eta_i / eta_t)^2; the reflected share is the textbook Fresnel
+    reflectance of the face (1 -> 1.5 on the front, 1.5 -> 1 on the back, total internal reflection
+    past the critical angle)."""
+    R, T = np.array([0.9, 0.8, 0.7]), np.array([0.6, 0.5, 0.4])
+    mat = np.array([4, *R, 0, *T], np.float32)
+    ei, et = (1.0, 1.5) if front else (1.5, 1.0)
+    for th in np.linspace(0.05, 1.5, 12):
+        wo = np.array([math.sin(th) * 0.6, math.sin(th) * 0.8, math.cos(th)], np.float32)
+        st = ei / et * math.sin(th)
+        tt = math.asin(min(st, 1.0))
+        fresnel = 1.0 if st >= 1 else 0.5 * ((math.sin(th - tt) / math.sin(th + tt)) ** 2 + (math.tan(th - tt) / math.tan(th + tt)) ** 2)
+        us = (np.arange(400) + 0.5) / 400
+        refl = 0
+        for u1 in us:
+            wi, f, pdf = _sample(mat, wo, u1, 0.5, front)
+            assert pdf > 0
+            w = f * abs(wi[2]) / pdf
+            if wi[2] > 0:
+                refl += 1
+                assert np.allclose(wi, [-wo[0], -wo[1], wo[2]], atol=1e-6)
+                assert np.allclose(w, R, rtol=1e-5)
+            else:
+                assert abs(math.sqrt(wi[0] ** 2 + wi[1] ** 2) - st) < 1e-5  # sin(theta_t)
+                assert abs(wi[0] * wo[1] - wi[1] * wo[0]) < 1e-6 and wi[0] * wo[0] <= 0  # plane of incidence
+                assert np.allclose(w, T * (ei / et) ** 2, rtol=1e-5)
+        assert abs(refl / len(us) - fresnel) <= 1.5 / len(us), (th, refl / len(us), fresnel)
+
+
+@pytest.mark.parametrize("rough", [0.1, 0.5])
+def test_substrate_reciprocity_and_sampling(rough):
+    """Substrate (FresnelBlend): reciprocity f(wo, wi) = f(wi, wo); a sample's value and pdf equal
+    the evaluate path's; E[1/pdf] is the upper hemisphere's solid angle (the sampler draws from the
+    pdf it reports) and E[f cos / pdf] the quadrature albedo."""
+    mat = np.array([5, 0.5, 0.4, 0.3, rough, 0.2, 0.3, 0.4], np.float32)
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        wo, wi = _upper(rng), _upper(rng)
+        assert np.allclose(_bsdf(mat, wo, wi)[0], _bsdf(mat, wi, wo)[0], rtol=2e-5)
+    wo = np.array([0.5, 0.3, 0.0], np.float32)
+    wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+    n, inv, est = 20000, 0.0, np.zeros(3)
+    for u1, u2 in rng.uniform(0, 1, (n, 2)):
+        wi, f, pdf = _sample(mat, wo, u1, u2)
+        if pdf > 0:
+            f2, pdf2 = _bsdf(mat, wo, wi)
+            assert np.allclose(f, f2, rtol=1e-5, atol=1e-8) and abs(pdf - pdf2) <= 1e-5 * pdf
+            inv += 1.0 / pdf
+            est += f * wi[2] / pdf
+    assert abs(inv / n - 2 * np.pi) < 0.04 * 2 * np.pi
+    dirs, dw = _hemisphere_grid(300, 200)
+    albedo = sum(_bsdf(mat, wo, d)[0] * d[2] * w for d, w in zip(dirs, dw))
+    assert np.allclose(est / n, albedo, rtol=0.03), (est / n, albedo)
+
+
+@pytest.mark.parametrize("kind", ["glass", "mirror"])
+def test_lossless_specular_object_vanishes_in_a_white_furnace(kind):
+    """A white environment (radiance 1) around a lossless glass sphere (R = T = 1) or mirror sphere:
+    every path that leaves is worth exactly 1 (the (eta_i / eta_t)^2 factors cancel between entry
+    and exit), so the image is 1 except for the few paths still trapped at the depth limit."""
+    b = scenes._Builder()
+    v, t = scenes.icosphere(3)
+    code = 4 if kind == "glass" else 3
+    b.add(v * 0.5, t, b.material(code, [1.0, 1.0, 1.0], 0.0, [1.0, 1.0, 1.0] if kind == "glass" else [0, 0, 0]))
+    cam = scenes.make_camera([0, 0, 2.0], [0, 0, 0], [0, 1, 0], 40.0, 1.0)
+    ndc, _ = scenes.calc_render_rect(32, 32, float(cam[9] / cam[10]))
+    sc = b.finish(np.zeros((0, 12), np.float32), cam, width=32, height=32, spp=16, max_depth=24, min_rr=24, ndc=ndc)
+    sc["env"] = np.array([1.0, 1.0, 1.0], np.float32)
+    pl = OracleScene(sc).render(path_l=True, film=False, threads=4)["path_l"]
+    assert np.all(pl <= 1.0 + 1e-5)
+    escaped = np.all(np.abs(pl - 1.0) <= 1e-5, axis=1)
+    assert escaped.mean() > 0.99, escaped.mean()  # facets of the sphere trap a few paths by total internal reflection
+    assert np.all(pl[~escaped] == 0.0)
+
+
 def test_plastic_without_specular_renders_as_lambert():
     """A scene whose Lambert materials are restated as plastic with a black specular colour renders
     the same film, bit for bit."""
