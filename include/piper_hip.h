@@ -75,6 +75,11 @@ typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22
     float radiance[3];
 } pg_quad;
 
+typedef struct pg_point_light { /* PointLightData (Light/BasicLight/Shared.hpp:26-29): a delta light */
+    float position[3];  /* the node's world position (pointInit, Light/BasicLight/Kernel.cpp:27-33) */
+    float intensity[3]; /* radiant intensity; radiance at distance d is intensity / d^2 (:35-45) */
+} pg_point_light;
+
 typedef struct pg_camera { /* PCData (Sensor/PerspectiveCamera/Shared.hpp:22-29) plus the node position */
     float position[3], look_at[3], up[3];
     float sensor_size[2];  /* metres */
@@ -98,6 +103,10 @@ typedef struct pg_scene_desc {
     uint32_t num_quads;
     pg_camera camera;
     float env_radiance[3]; /* constant environment (BasicLight/Kernel.cpp:59-89) */
+    /* point lights; lights are selected uniformly over {environment, quads, point lights} in that
+     * order. Zero / NULL for none. */
+    const pg_point_light* point_lights;
+    uint32_t num_point_lights;
 } pg_scene_desc;
 
 #define PG_BVH_HOST 0u   /* binned SAH on the host's cores (multithreaded) */

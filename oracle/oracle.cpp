@@ -189,6 +189,9 @@ struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
     V3 origin, u, v, normal, radiance;
     float area;
 };
+struct PointLight {  // PointLightData, BasicLight/Shared.hpp:26-29 (position from the node)
+    V3 pos, intensity;
+};
 struct Camera {  // Sensor/PerspectiveCamera/Shared.hpp:22-29
     V3 base, lookAt, up;
     float sizeX, sizeY, focal, aperture;
@@ -206,6 +209,7 @@ struct Scene {
     std::vector<uint32_t> triObj, objMat;
     std::vector<Material> mats;
     std::vector<Quad> quads;
+    std::vector<PointLight> points;
     Camera cam;
     V3 env;
     std::vector<BNode> nodes;
@@ -833,13 +837,15 @@ Frame shadingFrame(V3 n, bool d3) {
 
 // uniformSelect (Light/BasicLight/Kernel.cpp:91-98) over {environment, quad lights...}
 struct LightSel {
-    uint32_t light;  // 0 = environment, 1 + q = quad q
+    uint32_t light;  // 0 = environment, 1 + q = quad q, 1 + nq + p = point light p
     float pdf;
+    bool delta;      // a point light (LightAttributes::Delta)
 };
-LightSel selectLight(const Scene& s, float u) {
-    const uint32_t count = 1 + static_cast<uint32_t>(s.quads.size());
+LightSel selectLight(const Scene& s, float u) {  // uniformSelect, BasicLight/Kernel.cpp:91-98
+    const uint32_t nq = static_cast<uint32_t>(s.quads.size());
+    const uint32_t count = 1 + nq + static_cast<uint32_t>(s.points.size());
     const uint32_t sel = std::min(static_cast<uint32_t>(std::floor(u * static_cast<float>(count))), count - 1);
-    return {sel, 1.0f / static_cast<float>(count)};
+    return {sel, 1.0f / static_cast<float>(count), sel > nq};
 }
 
 struct LightSample {
@@ -857,6 +863,16 @@ LightSample sampleLight(const Scene& s, uint32_t light, V3 hit, float u1, float 
         ls.rad = s.env;
         ls.pdf = st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
         ls.dist = 1e5f;
+        return ls;
+    }
+    if (light > s.quads.size()) {  // pointSample, BasicLight/Kernel.cpp:35-45
+        const PointLight& p = s.points[light - 1 - s.quads.size()];
+        const V3 delta = p.pos - hit;
+        const float dis2 = dot(delta, delta);
+        ls.rad = V3{p.intensity.x / dis2, p.intensity.y / dis2, p.intensity.z / dis2};
+        ls.dist = std::sqrt(std::fmax(dis2, 0.0f));
+        ls.dir = V3{delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
+        ls.pdf = 1.0f;
         return ls;
     }
     const Quad& q = s.quads[light - 1];
@@ -930,34 +946,36 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
                     ++st.shadow;
                     if (!occluded(s, Ray{hit, ls.dir}, std::fmin(1e-3f, ls.dist), ls.dist)) {
                         float w = std::fabs(wi.z) / ls.pdf;
-                        w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
+                        if (!sel.delta) w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));  // :51-55
                         partA = mul(f, ls.rad) * w;
                     }
                 }
             }
-            smp.next2(d9, u1, u2);
-            const BsdfSample bs = bsdfSample(mat, wo, ng, u1, u2);
-            if (bs.pdf > 0.0f && valid(bs.f)) {
-                float weight = std::fabs(bs.wi.z) / bs.pdf;
-                const V3 dir = fr.toWorld(bs.wi);
-                const Hit lh = traceClosest(s, Ray{hit, dir}, 1e-3f, 1e5f, false, nullptr);
-                ++st.mis;
-                V3 rad{0, 0, 0};
-                float lpdf = 0.0f;
-                if (lh.prim == kMiss) {  // environment: SEEvaluate / SEPdf (BasicLight/Kernel.cpp:77-89)
-                    rad = s.env;
-                    const float theta = std::acos(std::fmin(1.0f, std::fmax(-1.0f, dir.z)));
-                    const float st2 = std::sin(theta);
-                    lpdf = st2 == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st2);
-                } else if ((lh.prim & kQuadBit) && sel.light == 1 + (lh.prim & ~kQuadBit)) {
-                    const Quad& q = s.quads[lh.prim & ~kQuadBit];
-                    const V3 qn = hitNormal(s, lh, dir);
-                    rad = dot(qn, dir) < 0.0f ? q.radiance : V3{0, 0, 0};  // diffuseEvaluate
-                    lpdf = lh.t * lh.t / (std::fabs(dot(qn, dir)) * q.area);  // calcGeometrySamplePdf
-                }
-                if (valid(rad) && lpdf > 0.0f) {
-                    weight = weight * powerHeuristic(bs.pdf, lpdf);
-                    partB = mul(rad, bs.f) * weight;
+            if (!sel.delta) {  // sampleBSDF returns before drawing for a delta light (:61-62)
+                smp.next2(d9, u1, u2);
+                const BsdfSample bs = bsdfSample(mat, wo, ng, u1, u2);
+                if (bs.pdf > 0.0f && valid(bs.f)) {
+                    float weight = std::fabs(bs.wi.z) / bs.pdf;
+                    const V3 dir = fr.toWorld(bs.wi);
+                    const Hit lh = traceClosest(s, Ray{hit, dir}, 1e-3f, 1e5f, false, nullptr);
+                    ++st.mis;
+                    V3 rad{0, 0, 0};
+                    float lpdf = 0.0f;
+                    if (lh.prim == kMiss) {  // environment: SEEvaluate / SEPdf (BasicLight/Kernel.cpp:77-89)
+                        rad = s.env;
+                        const float theta = std::acos(std::fmin(1.0f, std::fmax(-1.0f, dir.z)));
+                        const float st2 = std::sin(theta);
+                        lpdf = st2 == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st2);
+                    } else if ((lh.prim & kQuadBit) && sel.light == 1 + (lh.prim & ~kQuadBit)) {
+                        const Quad& q = s.quads[lh.prim & ~kQuadBit];
+                        const V3 qn = hitNormal(s, lh, dir);
+                        rad = dot(qn, dir) < 0.0f ? q.radiance : V3{0, 0, 0};  // diffuseEvaluate
+                        lpdf = lh.t * lh.t / (std::fabs(dot(qn, dir)) * q.area);  // calcGeometrySamplePdf
+                    }
+                    if (valid(rad) && lpdf > 0.0f) {
+                        weight = weight * powerHeuristic(bs.pdf, lpdf);
+                        partB = mul(rad, bs.f) * weight;
+                    }
                 }
             }
             L = L + mul(pf, (partA + partB) * (1.0f / sel.pdf));
@@ -1236,6 +1254,15 @@ void or_bsdf_sample(const float* mat, int front, const float* wo, float u1, floa
     wi[0] = s.wi.x, wi[1] = s.wi.y, wi[2] = s.wi.z;
     f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
     *pdf = s.pdf;
+}
+// points[6*n] = {position xyz, intensity rgb}; light order: environment, quads, points.
+void or_scene_set_points(void* scene, const float* points, uint32_t n) {
+    auto* s = static_cast<Scene*>(scene);
+    s->points.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = points + 6 * i;
+        s->points.push_back({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}});
+    }
 }
 float or_fresnel_dielectric(float cosI, float etaI, float etaT) { return fresnelDielectric(cosI, etaI, etaT); }
 float or_remap_roughness(float roughness) { return remapRoughness(roughness); }

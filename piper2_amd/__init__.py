@@ -62,6 +62,7 @@ class SceneDesc(ctypes.Structure):
         ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
         ("quads", ctypes.c_void_p), ("num_quads", ctypes.c_uint32),
         ("camera", Camera), ("env_radiance", F3),
+        ("point_lights", ctypes.c_void_p), ("num_point_lights", ctypes.c_uint32),
     ]
 
 
@@ -441,13 +442,18 @@ def scene_desc(sc):
     for i, q in enumerate(quads):
         quad_arr[i] = Quad(F3(*q[0:3]), F3(*q[3:6]), F3(*q[6:9]), F3(*q[9:12]))
     sc["_mat_arr"], sc["_quad_arr"] = mat_arr, quad_arr
+    pts = sc.get("point_lights")  # optional (n, 6) float32: position, intensity (pg_point_light)
+    if pts is not None and len(pts):
+        pts = sc["_points"] = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
+    else:
+        pts = None
     c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
     cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))
     return SceneDesc(
         _ptr(sc["vertices"]), sc["vertices"].shape[0], _ptr(sc["indices"]), sc["indices"].shape[0],
         _ptr(sc["tri_object"]), _ptr(sc["object_material"]), sc["object_material"].shape[0],
         ctypes.cast(mat_arr, ctypes.c_void_p), len(mats), ctypes.cast(quad_arr, ctypes.c_void_p), len(quads),
-        cam, F3(*sc["env"]))
+        cam, F3(*sc["env"]), _ptr(pts) if pts is not None else None, 0 if pts is None else pts.shape[0])
 
 
 def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0, film_filter=0,

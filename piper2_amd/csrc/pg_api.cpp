@@ -154,7 +154,7 @@ struct pg_scene {
     pg::BvhParams bvhParams;  // of the current build
     uint32_t builder = PG_BVH_HOST;
     pg::DevScene ds{};
-    DevBuf tris, triMatLeaf, nodes, mats, quads;
+    DevBuf tris, triMatLeaf, nodes, mats, quads, points;
     uint32_t spillPerLane = 1;
     float lo[3] = {0, 0, 0}, invExt[3] = {1, 1, 1};  // scene box (triangles and light quads) -> [0,1]^3
 };
@@ -994,6 +994,7 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
                 "missing mesh arrays");
         require(desc->num_materials > 0 && desc->materials, "at least one material is required");
         require(desc->num_quads <= 4096 && (desc->num_quads == 0 || desc->quads), "bad light quads");
+        require(desc->num_point_lights <= 4096 && (desc->num_point_lights == 0 || desc->point_lights), "bad point lights");
         auto s = std::make_unique<pg_scene>();
         s->dev = dev;
         s->triVerts.resize(static_cast<size_t>(nt) * 9);
@@ -1052,10 +1053,13 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         hipStream_t st = dev->stream;
         upload(s->mats, mats.data(), mats.size() * sizeof(pg::DevMaterial), st);
         upload(s->quads, quads.data(), quads.size() * sizeof(pg::DevQuad), st);
+        upload(s->points, desc->point_lights, desc->num_point_lights * sizeof(pg_point_light), st);
         pg::DevScene& ds = s->ds;
         ds.mats = s->mats.as<pg::DevMaterial>();
         ds.quads = s->quads.as<pg::DevQuad>();
         ds.numQuads = desc->num_quads;
+        ds.points = s->points.as<float>();
+        ds.numPoints = desc->num_point_lights;
         ds.sobol = dev->sobol.as<uint32_t>();
         ds.sobolT = dev->sobolT.as<uint32_t>();
         ds.sobolB = dev->sobolB.as<uint32_t>();

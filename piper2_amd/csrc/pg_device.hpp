@@ -683,6 +683,16 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
         ls.dist = 1e5f;
         return ls;
     }
+    if (light > s.numQuads) {  // pointSample (Light/BasicLight/Kernel.cpp:35-45): a delta light, pdf 1
+        const float* p = s.points + 6 * (light - 1u - s.numQuads);
+        const V3 delta = V3{p[0], p[1], p[2]} - hit;
+        const float dis2 = dot(delta, delta);
+        ls.rad = {p[3] / dis2, p[4] / dis2, p[5] / dis2};
+        ls.dist = sqrtf(fmaxf(dis2, 0.0f));
+        ls.dir = {delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
+        ls.pdf = 1.0f;
+        return ls;
+    }
     const DevQuad& q = s.quads[light - 1];
     const float uu = d1 ? (u1 - 1.0f) / 1.0f : u1;
     const V3 src = ld3(q.origin) + ld3(q.u) * uu + ld3(q.v) * u2;

@@ -287,18 +287,24 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 sobolWindow(s, ps.p[path].idx, meta.x, u);
                 uint32_t used = specV ? 2u : 7u;
 
+                // A point light is a delta light: its sample is not MIS-weighted and the BSDF branch
+                // draws nothing (select.delta, :48-54, :61-62), so the continuation follows the light
+                // sample's dimensions directly.
+                bool deltaSel = false;
                 if (!specV) {  // multipleImportanceSampling (:23-107) runs only if noSpecular
-                    // light selection: uniformSelect over {environment, quads}
-                    const uint32_t count = 1u + s.numQuads;
+                    // light selection: uniformSelect over {environment, quads, point lights}
+                    const uint32_t count = 1u + s.numQuads + s.numPoints;
                     const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
                     const float selInv = 1.0f / (1.0f / static_cast<float>(count));
+                    deltaSel = sel > s.numQuads;
+                    if (deltaSel) used = 5u;
                     const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
                     if (ls.pdf > 0.0f && valid(ls.rad)) {
                         const V3 wi = fr.toLocal(ls.dir);
                         const V3 f = bsdfEval(mat, wo, wi, ng);
                         if (valid(f)) {
                             float w = fabsf(wi.z) / ls.pdf;
-                            w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
+                            if (!deltaSel) w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
                             const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
                             // staged where it is made, so its registers are free for the rest of the vertex
                             lds.nee[tid] = ShadowRec{hp.x, hp.y, hp.z, fminf(1e-3f, ls.dist), ls.dir.x, ls.dir.y,
@@ -306,7 +312,8 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                             pushShadow = true;
                         }
                     }
-                    const BsdfSample bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
+                    BsdfSample bs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+                    if (!deltaSel) bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
                     if (bs.pdf > 0.0f && valid(bs.f)) {
                         const float weight = fabsf(bs.wi.z) / bs.pdf;
                         const V3 dir = fr.toWorld(bs.wi);
@@ -354,7 +361,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 // continuation (:164-190). With a guide: one-sample MIS that draws from the guide
                 // mixture with probability alpha, else from the BSDF, and weights by the combined
                 // density alpha p_guide + (1 - alpha) p_bsdf, so the estimator stays unbiased.
-                const float c0 = specV ? u[0] : u[5], c1 = specV ? u[1] : u[6];
+                const float c0 = specV ? u[0] : (deltaSel ? u[3] : u[5]), c1 = specV ? u[1] : (deltaSel ? u[4] : u[6]);
                 const float uc1 = d9 ? c1 : c0, uc2 = d9 ? c0 : c1;
                 BsdfSample ss;
                 float pdfMix = 0.0f, a = 0.0f;
@@ -390,7 +397,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     if (meta.y > rc.minRR) {
                         const float qrr = fmaxf(0.05f, 1.0f - maxc(thr));
                         used += 1u;
-                        if ((specV ? u[2] : u[7]) < qrr) alive = false;
+                        if ((specV ? u[2] : (deltaSel ? u[5] : u[7])) < qrr) alive = false;
                         else thr = thr * (1.0f / (1.0f - qrr));
                     }
                     if (alive) {

@@ -67,12 +67,14 @@ struct DevScene {
     const void* nodes;       // WideNode<width>[]
     const DevMaterial* mats;
     const DevQuad* quads;
+    const float* points;         // point lights: position xyz, intensity rgb
     const uint32_t* sobol;       // kSobolDims * kSobolCols
     const uint32_t* sobolT;      // the same matrices bit-major: kSobolCols * kSobolDims
     const uint32_t* sobolB;      // byte-sliced: kByteChunks * 256 * kByteDims
     const uint64_t* sobolInv;    // [kMaxLog2Res+1][32]
     const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
     uint32_t numTris, numNodes, numQuads, width;
+    uint32_t numPoints;
     DevCamera cam;
     float env[3];
     uint32_t hasBvh;

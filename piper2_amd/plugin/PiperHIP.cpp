@@ -130,6 +130,7 @@ struct FlatScene {
     std::vector<uint32_t> index, triObject, objectMaterial;
     std::vector<pg_material> materials;
     std::vector<pg_quad> quads;
+    std::vector<pg_point_light> points;
     float env[3] = {0, 0, 0};
     std::vector<std::pair<const Node*, Mat34>> sensors;  // leaf nodes of sensors and their transform
 };
@@ -171,6 +172,12 @@ void flatten(const std::shared_ptr<Node>& node, const Mat34& toWorld, FlatScene&
     } else if (auto* light = dynamic_cast<const Light*>(leaf->object.get())) {
         if (light->isEnvironment) {
             for (int k = 0; k < 3; ++k) f.env[k] = light->radiance[k];
+        } else if (light->isPoint) {  // pointInit: the node's origin in world space
+            const float zero[3] = {0.0f, 0.0f, 0.0f};
+            const auto o = applyPoint(toWorld, zero);
+            pg_point_light p{};
+            for (int k = 0; k < 3; ++k) p.position[k] = o[k], p.intensity[k] = light->radiance[k];
+            f.points.push_back(p);
         } else {
             pg_quad q{};
             const auto o = applyPoint(toWorld, light->origin.data());
@@ -332,6 +339,8 @@ public:
         d.num_materials = static_cast<uint32_t>(mFlat.materials.size());
         d.quads = mFlat.quads.data();
         d.num_quads = static_cast<uint32_t>(mFlat.quads.size());
+        d.point_lights = mFlat.points.empty() ? nullptr : mFlat.points.data();
+        d.num_point_lights = static_cast<uint32_t>(mFlat.points.size());
         for (int k = 0; k < 3; ++k) {
             d.camera.position[k] = (*toWorld)[4 * k + 3];
             d.camera.look_at[k] = cam.lookAt[k];
@@ -448,6 +457,11 @@ std::shared_ptr<Light> Light::areaPlane(std::array<float, 3> origin, std::array<
     l->origin = origin, l->u = u, l->v = v, l->radiance = radiance;
     return l;
 }
+std::shared_ptr<Light> Light::point(std::array<float, 3> intensity) {
+    auto l = std::make_shared<Light>();
+    l->isPoint = true, l->radiance = intensity;
+    return l;
+}
 std::shared_ptr<Light> Light::environment(std::array<float, 3> radiance) {
     auto l = std::make_shared<Light>();
     l->isEnvironment = true, l->radiance = radiance;
@@ -462,7 +476,7 @@ const std::vector<std::string>& hipBuiltinSymbols() {
         "mirrorInit",  "mirrorSample", "blackBodyEvaluate", "blackBodyPdf", "glassInit", "glassSample",
         "glassEvaluate", "glassPdf", "substrateInit", "substrateSample", "substrateEvaluate", "substratePdf",
         "diffuseSample", "diffuseEvaluate", "diffusePdf", "planeSample",
-        "uniformSelect", "SEInit",   "SESample",      "SEEvaluate",    "SEPdf",      "constantTexture"};
+        "uniformSelect", "pointInit", "pointSample", "deltaEvaluate", "deltaPdf", "SEInit",   "SESample",      "SEEvaluate",    "SEPdf",      "constantTexture"};
     return s;
 }
 

@@ -183,15 +183,17 @@ public:
     std::array<float, 3> specular{};     // plastic / substrate specular, glass transmission
 };
 class Medium : public Object {};
-// Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V) and the environment
-// (Light/BasicLight SampledEnvironment with constant radiance).
+// Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V), the environment
+// (Light/BasicLight SampledEnvironment with constant radiance) and the point light
+// (BasicLight.cpp:55-80: Intensity, placed at its node's origin).
 class Light : public Object {
 public:
     static std::shared_ptr<Light> areaPlane(std::array<float, 3> origin, std::array<float, 3> u,
                                             std::array<float, 3> v, std::array<float, 3> radiance);
     static std::shared_ptr<Light> environment(std::array<float, 3> radiance);
-    bool isEnvironment = false;
-    std::array<float, 3> origin{}, u{}, v{}, radiance{};
+    static std::shared_ptr<Light> point(std::array<float, 3> intensity);
+    bool isEnvironment = false, isPoint = false;
+    std::array<float, 3> origin{}, u{}, v{}, radiance{};  // radiance: the point light's intensity
 };
 // Sensor/PerspectiveCamera (PCData, Sensor/PerspectiveCamera/Shared.hpp:22-29; config
 // PerspectiveCamera.cpp:39-45). The camera sits at its node's translation.

@@ -150,7 +150,7 @@ class _Builder:
 
 
 def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
-            specular=False):
+            specular=False, point_lights=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
@@ -158,7 +158,8 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     seeded matte roughness (Oren-Nayar); specular=True makes the floor substrate (its albedo as the
     diffuse colour) and adds a smooth glass sphere (outward winding, so rays enter through front
     faces and leave through back faces) and a mirror panel before the back wall, tilted to show the
-    ceiling. The light and the rest of the geometry stay as in the plain scene."""
+    ceiling; point_lights=True adds two point lights (lights 2 and 3 after the environment and the
+    ceiling quad). The light and the rest of the geometry stay as in the plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -207,9 +208,12 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     aspect = width / height
     cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
-    return b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
-                    ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else "")
-                    + ("_specular" if specular else ""))
+    sc = b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
+                  ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else "")
+                  + ("_specular" if specular else "") + ("_points" if point_lights else ""))
+    if point_lights:
+        sc["point_lights"] = np.array([[0.45, 0.6, 0.2, 1.5, 1.4, 1.2], [-0.55, -0.3, 0.45, 0.6, 0.7, 1.0]], np.float32)
+    return sc
 
 
 def terrain_spheres(seed=2, width=1920, height=1080, spp=64, max_depth=8, min_rr=3, grid=(1000, 500),

@@ -26,6 +26,7 @@ def lib():
         L.or_scene_create.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint32, _VP,
                                       ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]
         L.or_scene_destroy.argtypes = [_VP]
+        L.or_scene_set_points.argtypes = [_VP, _VP, ctypes.c_uint32]
         L.or_trace.argtypes = [_VP, _VP, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int, _VP]
         L.or_occluded.argtypes = [_VP, _VP, ctypes.c_uint32, _VP]
         L.or_camera_rays.argtypes = [_VP, _VP, _VP, _VP, ctypes.c_uint32, _VP, _VP]
@@ -68,6 +69,10 @@ class OracleScene:
         v, i, to, om, m, q, c, e = self._arrs
         self.h = lib().or_scene_create(_p(v), v.shape[0], _p(i), i.shape[0], _p(to), _p(om), om.shape[0], _p(m),
                                        m.shape[0], _p(q), q.shape[0], _p(c), _p(e))
+        pts = sc.get("point_lights")
+        if pts is not None and len(pts):
+            self._pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
+            lib().or_scene_set_points(self.h, _p(self._pts), self._pts.shape[0])
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -24,7 +24,7 @@ namespace {
 
 struct SceneFile {
     uint32_t nv, nt, no, nm, nq, width, height, spp, maxDepth, minRR, compat, fit;
-    std::vector<float> verts, materials, quads, camera, env, srt;
+    std::vector<float> verts, materials, quads, camera, env, srt, points;
     std::vector<uint32_t> index, triObject, objectMaterial;
 };
 
@@ -55,6 +55,9 @@ SceneFile load(const char* path) {
     readArr(f, s.env, 3);
     readArr(f, s.srt, 16ull * s.no);
     if (!f) throw PiperException("truncated scene file");
+    uint32_t np = 0;  // optional tail: point lights {position, intensity}
+    if (f.read(reinterpret_cast<char*>(&np), 4)) readArr(f, s.points, 6ull * np);
+    if (np && !f) throw PiperException("truncated scene file");
     return s;
 }
 
@@ -124,6 +127,11 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
         const float* p = &s.quads[12 * q];
         auto light = Light::areaPlane({p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, {p[9], p[10], p[11]});
         children.push_back({TransformInfo{}, tracer.buildNode(light)});
+    }
+    for (size_t p = 0; p + 6 <= s.points.size(); p += 6) {  // a point light sits at its node's origin
+        TransformSRT at = identitySRT();
+        at.transX = s.points[p], at.transY = s.points[p + 1], at.transZ = s.points[p + 2];
+        children.push_back({staticTransform(at), tracer.buildNode(Light::point({s.points[p + 3], s.points[p + 4], s.points[p + 5]}))});
     }
     if (s.env[0] > 0.0f || s.env[1] > 0.0f || s.env[2] > 0.0f)
         children.push_back({TransformInfo{}, tracer.buildNode(Light::environment({s.env[0], s.env[1], s.env[2]}))});

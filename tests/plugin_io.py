@@ -25,6 +25,11 @@ def write_scene(sc, path, srt=None, compat=0, fit=0):
                       (sc["materials"], np.float32), (sc["quads"], np.float32), (sc["camera"][:14], np.float32),
                       (sc["env"], np.float32), (srt, np.float32)):
             f.write(np.ascontiguousarray(a, dt).tobytes())
+        pts = sc.get("point_lights")
+        if pts is not None and len(pts):  # optional tail: count + {position, intensity} per light
+            pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
+            f.write(np.uint32(pts.shape[0]).tobytes())
+            f.write(pts.tobytes())
 
 
 def run_driver(*args, timeout=120):

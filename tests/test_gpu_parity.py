@@ -193,6 +193,15 @@ def test_specular_surfaces_compat_flags_match_oracle(dev):
     _path_parity(sc, dev, flags=flags)
 
 
+def test_point_lights_paths_match_oracle(dev):
+    # delta lights: no MIS weight on their samples, no BSDF ray, and the continuation's dimensions
+    # shift by two when one is picked
+    sc = scenes.cornell(width=128, height=128, spp=16, point_lights=True, specular=True)
+    _path_parity(sc, dev)
+    flags = pg.PG_COMPAT_D1 | pg.PG_COMPAT_D9
+    _path_parity(sc, dev, flags=flags, spp=4)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

@@ -64,8 +64,8 @@ def test_plastic_and_rough_matte_surfaces(dev, tmp_path):
     assert np.array_equal(film, direct_film(dev, sc))
 
 
-def test_mirror_glass_substrate_surfaces(dev, tmp_path):
-    sc = scenes.cornell(width=64, height=48, spp=8, specular=True)
+def test_mirror_glass_substrate_surfaces_and_point_lights(dev, tmp_path):
+    sc = scenes.cornell(width=64, height=48, spp=8, specular=True, point_lights=True)
     film, _ = plugin_film(tmp_path, sc, "specular")
     assert np.array_equal(film, direct_film(dev, sc))
 

@@ -500,6 +500,37 @@ def test_direct_lighting_matches_polygon_irradiance():
     assert np.median(rel) < 0.02 and rel.mean() < 0.04, (np.median(rel), rel.mean())
 
 
+def test_point_light_inverse_square_closed_form():
+    """max_depth 1 with one point light over a Lambert plane: the light is picked with probability
+    1/2 (the black environment is the other light) and, as a delta light, its sample is not MIS
+    weighted and no BSDF ray is drawn; every path is 0 or exactly 2 rho/pi I cos(theta) / d^2, and the
+    mean over a pixel's samples approaches rho/pi I cos(theta) / d^2."""
+    rho, inten, lp = 0.6, np.array([3.0, 2.0, 1.0]), np.array([0.3, 1.2, -0.2])
+    b = scenes._Builder()
+    b.add([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], [[0, 1, 2], [0, 2, 3]], b.material(0, [rho] * 3))
+    cam = scenes.make_camera([0, 3, 0.01], [0, 0, 0], [0, 0, -1], 50, 1.0)
+    sc = b.finish(np.zeros((0, 12), np.float32), cam, width=16, height=16, spp=64, max_depth=1, min_rr=1,
+                  ndc=(0.0, 0.0, 1.0, 1.0))
+    sc["point_lights"] = np.array([[*lp, *inten]], np.float32)
+    o = OracleScene(sc)
+    res = o.render(path_l=True, film=False)
+    pl = res["path_l"].reshape(16 * 16, 64, 3)
+    pix = np.array([[x, y, s] for y in range(16) for x in range(16) for s in range(64)], np.uint32)
+    rays, _ = o.camera_rays(pix)
+    t, prim, _ = o.trace(rays)
+    hit = (rays[:, :3] + rays[:, 3:6] * t[:, None]).reshape(256, 64, 3)
+    d = lp[None, None] - hit
+    d2 = (d * d).sum(-1)
+    cos = d[..., 1] / np.sqrt(d2)
+    expect = rho / np.pi * inten[None, None] * (cos / d2)[..., None]
+    lit = np.abs(pl - 2 * expect) <= 2e-5 * 2 * expect + 1e-7
+    dark = pl == 0
+    assert np.all(lit.all(-1) | dark.all(-1))
+    frac = lit.all(-1).mean()
+    assert abs(frac - 0.5) < 0.02, frac  # Sobol dimension 5 splits the samples evenly
+    assert np.allclose(pl.mean(1), expect.mean(1), rtol=0.1)
+
+
 def test_russian_roulette_is_unbiased():
     sc = scenes.cornell(width=32, height=32, spp=256)
     o = OracleScene(sc)
