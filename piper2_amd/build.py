@@ -31,6 +31,8 @@ def _compile(src: str) -> str:
 
 PLUGIN_SRC = os.path.join(HERE, "plugin", "PiperHIP.cpp")
 PLUGIN_LIB = os.path.join(HERE, "libpiper_hip_plugin.so")
+FRONTEND_SRC = os.path.join(HERE, "frontend", "piper_render.cpp")
+FRONTEND_BIN = os.path.join(HERE, "piper_render")
 CXX = os.environ.get("CXX", "g++")
 
 
@@ -46,6 +48,11 @@ def build() -> str:
     subprocess.run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", PLUGIN_SRC, "-L" + HERE,
                     "-lpiper_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], check=True)
     os.replace(tmp, PLUGIN_LIB)
+    # scene front-end (the Render operator over the component pair): JSON scene -> EXR frames
+    tmp = FRONTEND_BIN + ".tmp"
+    subprocess.run([CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", FRONTEND_SRC, "-I" + os.path.join(HERE, "plugin"),
+                    "-L" + HERE, "-lpiper_hip_plugin", "-lpiper_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], check=True)
+    os.replace(tmp, FRONTEND_BIN)
     return LIB
 
 

@@ -482,10 +482,15 @@ const std::vector<std::string>& hipBuiltinSymbols() {
 
 // ---- HIPAccelerator ----
 HIPAccelerator::HIPAccelerator(int ordinal) { check(pg_device_create(ordinal, &mDevice), "pg_device_create"); }
+std::shared_ptr<HIPAccelerator> HIPAccelerator::hostOnly() {
+    return std::shared_ptr<HIPAccelerator>(new HIPAccelerator());
+}
 HIPAccelerator::~HIPAccelerator() {
     if (mDevice) pg_device_destroy(mDevice);
 }
-const char* HIPAccelerator::getNativePlatform() const noexcept { return pg_device_platform(mDevice); }
+const char* HIPAccelerator::getNativePlatform() const noexcept {
+    return mDevice ? pg_device_platform(mDevice) : "host (no device)";
+}
 std::vector<std::string> HIPAccelerator::getSupportedLinkableFormat() const { return {"HIP-gfx950-builtin"}; }
 std::shared_ptr<Resource> HIPAccelerator::createBuffer(size_t size, size_t alignment,
                                                        std::function<void(std::byte*)> data) {
@@ -496,6 +501,7 @@ std::shared_ptr<Resource> HIPAccelerator::createBuffer(size_t size, size_t align
     return b;
 }
 std::shared_ptr<TiledOutput> HIPAccelerator::createTiledOutput(size_t size, size_t alignment) {
+    if (!mDevice) raise("createTiledOutput: this accelerator has no device");
     if (size % 16 != 0 || alignment > 16) raise("createTiledOutput: expects an RGBW float buffer");
     return std::make_shared<HIPTiledOutput>(mDevice, size);
 }
@@ -582,6 +588,7 @@ std::unique_ptr<Pipeline> HIPTracer::buildPipeline(const std::shared_ptr<Node>& 
                                                    std::shared_ptr<Sampler> sampler) {
     (void)lightSampler;  // uniform light selection is built in
     if (!scene || !sampler) raise("buildPipeline: scene and sampler are required");
+    if (!mAccelerator->device()) raise("buildPipeline: the accelerator has no device");
     // the FixedSampler's filter becomes the film filter of every launch (its RDData payload)
     uint32_t filter = PG_FILTER_BOX;
     float alpha = 0.0f;
@@ -598,6 +605,23 @@ std::unique_ptr<Pipeline> HIPTracer::buildPipeline(const std::shared_ptr<Node>& 
     flatten(scene, identity34(), flat);
     return std::make_unique<HIPPipeline>(mAccelerator->device(), std::move(flat), integrator, std::move(sampler), filter,
                                          alpha);
+}
+
+SceneExport exportScene(const std::shared_ptr<Node>& scene) {
+    if (!scene) raise("exportScene: null scene");
+    FlatScene flat;
+    flatten(scene, identity34(), flat);
+    SceneExport e;
+    e.verts = std::move(flat.verts);
+    e.index = std::move(flat.index);
+    e.triObject = std::move(flat.triObject);
+    e.objectMaterial = std::move(flat.objectMaterial);
+    e.materials = std::move(flat.materials);
+    e.quads = std::move(flat.quads);
+    e.points = std::move(flat.points);
+    for (int k = 0; k < 3; ++k) e.env[k] = flat.env[k];
+    for (const auto& [n, m] : flat.sensors) e.sensors.push_back({n, m});
+    return e;
 }
 
 size_t HIPTracer::getAlignmentRequirement(AlignmentRequirement requirement) const noexcept {

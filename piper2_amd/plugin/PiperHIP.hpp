@@ -267,6 +267,9 @@ public:
 class HIPAccelerator final : public Accelerator {
 public:
     explicit HIPAccelerator(int ordinal);  // pg_device_create
+    // An accelerator without a device: scene graphs build (buffers are host memory) and export,
+    // but pipelines and tiled outputs raise.
+    static std::shared_ptr<HIPAccelerator> hostOnly();
     ~HIPAccelerator() override;
     HIPAccelerator(const HIPAccelerator&) = delete;
     HIPAccelerator& operator=(const HIPAccelerator&) = delete;
@@ -278,6 +281,7 @@ public:
     pg_device* device() const noexcept { return mDevice; }
 
 private:
+    HIPAccelerator() = default;
     pg_device* mDevice = nullptr;
 };
 
@@ -320,5 +324,19 @@ public:
 
 // The built-in kernel symbols a HIP pipeline provides (buildProgram accepts only these).
 const std::vector<std::string>& hipBuiltinSymbols();
+
+// What buildPipeline uploads for a scene graph: world-space triangles grouped by instance, one
+// material per instance, the lights (environment, quads, points) and every sensor with its node's
+// local-to-world transform (row-major 3x4). No device is needed.
+struct SceneExport {
+    std::vector<float> verts;
+    std::vector<uint32_t> index, triObject, objectMaterial;
+    std::vector<pg_material> materials;
+    std::vector<pg_quad> quads;
+    std::vector<pg_point_light> points;
+    std::array<float, 3> env{};
+    std::vector<std::pair<const Node*, std::array<float, 12>>> sensors;
+};
+SceneExport exportScene(const std::shared_ptr<Node>& scene);
 
 }  // namespace Piper

@@ -1,0 +1,186 @@
+"""Scene files in the reference's JSON schema (PiperRenderer.cpp) for the front-end tests, written
+from a generated scene dict, plus readers for what piper_render writes (.pgsc exports, EXR frames)."""
+import json
+import os
+import struct
+
+import numpy as np
+
+from plugin_io import IDENTITY_SRT
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FRONTEND = os.path.join(ROOT, "piper2_amd", "piper_render")
+
+SURF = "Piper.BuiltinComponent.ClassicSurface."
+CONST = "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture"
+
+
+def _tex(*v):
+    return {"ClassID": CONST, "Value": [float(x) for x in v]}
+
+
+def _rgba(c):
+    return _tex(c[0], c[1], c[2], 1.0)
+
+
+def surface_config(m):
+    """One material row [type, r, g, b, alpha, spec rgb] as a reference surface config."""
+    t, c, a, s = int(m[0]), m[1:4], float(m[4]), m[5:8]
+    if t == 0:
+        return {"ClassID": SURF + "Matte", "Diffuse": _rgba(c), "Roughness": _tex(a)}
+    if t == 2:
+        return {"ClassID": SURF + "Plastic", "Diffuse": _rgba(c), "Specular": _rgba(s), "Roughness": _tex(a)}
+    if t == 3:
+        return {"ClassID": SURF + "Mirror", "Reflection": _rgba(c)}
+    if t == 4:
+        return {"ClassID": SURF + "Glass", "Reflection": _rgba(c), "Transmission": _rgba(s), "RoughnessX": _tex(a),
+                "RoughnessY": _tex(a)}
+    if t == 5:
+        return {"ClassID": SURF + "Substrate", "Diffuse": _rgba(c), "Specular": _rgba(s), "RoughnessX": _tex(a),
+                "RoughnessY": _tex(a)}
+    raise ValueError(f"material type {t} has no reference class")
+
+
+def write_obj(path, verts, tris, negative=False):
+    with open(path, "w") as f:
+        f.write("# written by the front-end tests\n")
+        for v in verts:
+            f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in v))
+        n = len(verts)
+        for t in tris:
+            if negative:
+                f.write("f %d/1/1 %d//2 %d\n" % tuple(int(i) - n for i in t))
+            else:
+                f.write("f %d %d %d\n" % tuple(int(i) + 1 for i in t))
+
+
+def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transforms=None):
+    """Write sc as scene.json + one OBJ per object + options.json under folder; returns the options
+    path. transforms: optional {object index: Transform config} (the OBJ then holds the object's
+    vertices as given, the transform places them)."""
+    os.makedirs(folder, exist_ok=True)
+    children = []
+    for o, mat in enumerate(sc["object_material"]):
+        tri = sc["indices"][sc["tri_object"] == o]
+        vid, inv = np.unique(tri, return_inverse=True)
+        name = f"object{o}.obj"
+        write_obj(os.path.join(folder, name), sc["vertices"][vid], inv.reshape(-1, 3), negative=(o % 2 == 1))
+        inst = {"Type": "Instance", "Geometry": {"ClassID": "Piper.BuiltinComponent.TriangleMesh.TriangleMesh",
+                                                 "Path": name},
+                "Surface": surface_config(sc["materials"][mat])}
+        if transforms and o in transforms:
+            inst = {"Type": "Group", "Children": [dict(inst, Transform=transforms[o])]}
+        children.append(inst)
+    for q in sc["quads"]:
+        plane = {"ClassID": "Piper.BuiltinComponent.BasicShapes.Plane",
+                 "Primitives": [{"Origin": q[0:3].tolist(), "U": q[3:6].tolist(), "V": q[6:9].tolist()}]}
+        children.append({"Type": "Instance", "ClassID": "Piper.BuiltinComponent.AreaLight.Diffuse", "Geometry": plane,
+                         "Radiance": q[9:12].tolist()})
+    for p in sc.get("point_lights", np.zeros((0, 6))):
+        children.append({"Type": "Group", "Children": [{
+            "Type": "Instance", "ClassID": "Piper.BuiltinComponent.BasicLight.Point", "Intensity": p[3:6].tolist(),
+            "Transform": {"Translation": p[0:3].tolist()}}]})
+    children.append({"Type": "Instance", "ClassID": "Piper.BuiltinComponent.BasicLight.SampledEnvironment",
+                     "Radiance": sc["env"].tolist()})
+    c = sc["camera"].astype(np.float64)
+    fstop = 1e30 if c[12] == 0 else float(c[11] / (2 * c[12]))
+    sensor = {"Type": "Instance", "ID": "Camera", "ClassID": "Piper.BuiltinComponent.PerspectiveCamera.Sensor",
+              "LookAt": c[3:6].tolist(), "Up": c[6:9].tolist(), "SensorSize": [c[9] * 1000, c[10] * 1000],
+              "FocalLength": c[11] * 1000, "FStop": fstop, "Transform": {"Translation": c[0:3].tolist()}}
+    children.append({"Type": "Group", "Children": [sensor]})
+    scene = {
+        "RenderDriver": {"ClassID": "Piper.BuiltinComponent.FixedSampler.Driver",
+                         "Filter": filter_cfg or {"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter",
+                                                  "FilterType": "box"}},
+        "Actions": [{"ActiveSensor": "Camera", "TimeBegin": 0.0, "ShutterOpenTime": 0.0, "ShutterCloseTime": 0.0,
+                     "FrameCount": 1, "FitMode": fit}],
+        "Objects": [],
+        "Scene": {"Type": "Group", "Children": children},
+        "LightSampler": {"ClassID": "Piper.BuiltinComponent.BasicLight.UniformLightSampler"},
+        "Integrator": {"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator",
+                       "MaxTraceDepth": int(sc["max_depth"]), "MinRussianRouletteDepth": int(sc["min_rr"])},
+        "Sampler": {"ClassID": "Piper.BuiltinComponent.ScrambledSobolSampler.Sampler",
+                    "SamplesPerPixel": int(spp or sc["spp"]), "Scramble": False},
+    }
+    with open(os.path.join(folder, "scene.json"), "w") as f:
+        json.dump(scene, f, indent=1)
+    opts = {"ImageSize": [int(sc["width"]), int(sc["height"])], "SceneDesc": "scene.json",
+            "SceneParser": "Piper.Infrastructure.JsonSerializer.JsonSerializer",
+            "Tracer": {"ClassID": "Piper.BuiltinComponent.Embree.Tracer"}, "OutputDir": "out"}
+    path = os.path.join(folder, "options.json")
+    with open(path, "w") as f:
+        json.dump(opts, f, indent=1)
+    return path
+
+
+def read_pgsc(path):
+    """The scene-file layout of tests/plugin/render_scene.cpp, as a scene dict."""
+    with open(path, "rb") as f:
+        data = f.read()
+    assert data[:4] == b"PGSC" and struct.unpack("<I", data[4:8])[0] == 1
+    head = np.frombuffer(data, np.uint32, 12, 8)
+    nv, nt, no, nm, nq, w, h, spp, md, rr, compat, fit = (int(x) for x in head)
+    off = 56
+    out = {}
+
+    def take(name, dt, n, shape):
+        nonlocal off
+        a = np.frombuffer(data, dt, n, off).reshape(shape)
+        off += a.nbytes
+        out[name] = a.copy()
+
+    take("vertices", np.float32, 3 * nv, (nv, 3))
+    take("indices", np.uint32, 3 * nt, (nt, 3))
+    take("tri_object", np.uint32, nt, (nt,))
+    take("object_material", np.uint32, no, (no,))
+    take("materials", np.float32, 8 * nm, (nm, 8))
+    take("quads", np.float32, 12 * nq, (nq, 12))
+    take("camera", np.float32, 14, (14,))
+    take("env", np.float32, 3, (3,))
+    take("srt", np.float32, 16 * no, (no, 16))
+    assert np.all(out["srt"] == IDENTITY_SRT[None])
+    if off < len(data):
+        npt = struct.unpack("<I", data[off:off + 4])[0]
+        off += 4
+        take("point_lights", np.float32, 6 * npt, (npt, 6))
+    assert off == len(data)
+    out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
+    return out
+
+
+def read_exr(path):
+    """Minimal reader for the uncompressed HALF scanline EXR piper_render writes, following the
+    OpenEXR file layout; returns (h, w, 3) float16 RGB and the header attributes."""
+    with open(path, "rb") as f:
+        d = f.read()
+    assert struct.unpack("<ii", d[:8]) == (20000630, 2)
+    off, attrs = 8, {}
+    while d[off] != 0:
+        e = d.index(b"\0", off)
+        name = d[off:e].decode()
+        e2 = d.index(b"\0", e + 1)
+        typ = d[e + 1:e2].decode()
+        size = struct.unpack("<i", d[e2 + 1:e2 + 5])[0]
+        attrs[name] = (typ, d[e2 + 5:e2 + 5 + size])
+        off = e2 + 5 + size
+    off += 1
+    x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    chans, cl = [], attrs["channels"][1]
+    p = 0
+    while cl[p] != 0:
+        e = cl.index(b"\0", p)
+        chans.append((cl[p:e].decode(), struct.unpack("<i", cl[e + 1:e + 5])[0]))
+        p = e + 17
+    assert attrs["compression"][1] == b"\0" and all(t == 1 for _, t in chans)
+    table = np.frombuffer(d, np.uint64, h, off)
+    img = np.zeros((h, w, len(chans)), np.float16)
+    for y in range(h):
+        o = int(table[y])
+        ly, size = struct.unpack("<ii", d[o:o + 8])
+        planes = np.frombuffer(d, np.float16, w * len(chans), o + 8).reshape(len(chans), w)
+        assert size == planes.nbytes
+        img[ly - y0] = planes.T
+    names = [c for c, _ in chans]
+    rgb = np.stack([img[..., names.index(c)] for c in "RGB"], -1)
+    return rgb, attrs

@@ -1,0 +1,167 @@
+"""The scene front-end (piper2_amd/piper_render) on the CPU: reference-schema JSON scenes, mesh
+files and transforms flatten to exactly what the generators build (--export, no device), EXR
+output follows the OpenEXR layout with IEEE half conversion, and malformed input is reported."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from piper2_amd import scenes
+from scene_json import FRONTEND, read_exr, read_pgsc, write_obj, write_scene_json
+
+pytestmark = pytest.mark.skipif(not os.path.exists(FRONTEND), reason="piper_render is not built")
+
+
+def run(*args):
+    return subprocess.run([FRONTEND, *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def export(sc, folder, **kw):
+    opts = write_scene_json(sc, str(folder), **kw)
+    r = run(opts, "--export", folder / "exp")
+    assert r.returncode == 0, r.stdout + r.stderr
+    return read_pgsc(folder / "exp" / "Action0.pgsc")
+
+
+def test_export_reproduces_generated_scene(tmp_path):
+    # every surface class, negative OBJ indices on every other object, an area light, two point
+    # lights (Point under a translated group), the environment and a translated sensor
+    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True)
+    sc["env"] = np.array([0.1, 0.2, 0.3], np.float32)
+    e = export(sc, tmp_path)
+    for k in ("vertices", "indices", "tri_object", "object_material", "materials", "quads", "env", "point_lights"):
+        assert np.array_equal(e[k], np.asarray(sc[k]).astype(e[k].dtype)), k
+    cam = sc["camera"]
+    assert np.array_equal(e["camera"][:9], cam[:9])
+    assert np.allclose(e["camera"][9:12], cam[9:12], rtol=1e-6)  # millimetres in the file
+    assert e["camera"][12] < 1e-20  # FStop 1e30: a pinhole
+    assert (e["width"], e["height"], e["spp"], e["max_depth"], e["min_rr"]) == (64, 48, 8, 5, 5)
+
+
+def test_transform_srt_matches_scipy(tmp_path):
+    """Translation, per-axis Scale and Euler Rotation [pitch, yaw, roll]: the reference's quaternion
+    is scipy's intrinsic Y-X-Z rotation (yaw, then pitch, then roll); local -> world applies the
+    scale, then the rotation, then the translation."""
+    sc = scenes.cornell(width=32, height=32, spp=4)
+    pitch, yaw, roll = 0.3, -0.7, 1.1
+    s, t = np.array([1.5, 0.5, 2.0]), np.array([0.2, -0.1, 0.4])
+    tr = {"Translation": t.tolist(), "Scale": s.tolist(), "Rotation": [pitch, yaw, roll]}
+    e = export(sc, tmp_path, transforms={5: tr, 6: {"Scale": 0.5}})
+    rot = Rotation.from_euler("YXZ", [yaw, pitch, roll])
+    for o, f in ((5, lambda v: rot.apply(v * s) + t), (6, lambda v: v * 0.5)):
+        m = sc["tri_object"] == o
+        want = f(sc["vertices"][sc["indices"][m]].reshape(-1, 3).astype(np.float64))
+        got = e["vertices"][e["indices"][e["tri_object"] == o]].reshape(-1, 3)
+        assert np.allclose(got, want, atol=2e-6), o
+    # quaternion form: [w, x, y, z]
+    q = rot.as_quat()  # x, y, z, w
+    e2 = export(sc, tmp_path / "q", transforms={5: {"Rotation": [q[3], q[0], q[1], q[2]]}})
+    m = sc["tri_object"] == 5
+    assert np.allclose(e2["vertices"][e2["indices"][e2["tri_object"] == 5]].reshape(-1, 3),
+                       rot.apply(sc["vertices"][sc["indices"][m]].reshape(-1, 3).astype(np.float64)), atol=2e-6)
+
+
+def test_ply_and_obj_polygons(tmp_path):
+    """A quad and a pentagon as OBJ polygons (fanned from the first corner) and the same mesh as
+    binary and ascii PLY give the same triangles."""
+    verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [2, 0, 0], [3, 0.5, 0], [2.5, 1.5, 0]], np.float32)
+    polys = [[0, 1, 2, 3], [1, 4, 5, 6, 2]]
+    fan = [[p[0], p[i], p[i + 1]] for p in polys for i in range(1, len(p) - 1)]
+    with open(tmp_path / "m.obj", "w") as f:
+        for v in verts:
+            f.write("v %.9g %.9g %.9g\n" % tuple(v))
+        f.write("vt 0 0\nvn 0 0 1\n# comment\n")
+        for p in polys:
+            f.write("f " + " ".join(f"{i + 1}/1/1" for i in p) + "\n")
+    with open(tmp_path / "m.ply", "wb") as f:
+        f.write(b"ply\nformat binary_little_endian 1.0\ncomment test\nelement vertex 7\nproperty float x\n"
+                b"property float y\nproperty float z\nproperty uchar red\nelement face 2\n"
+                b"property list uchar int vertex_indices\nend_header\n")
+        for v in verts:
+            f.write(v.tobytes() + b"\x07")
+        for p in polys:
+            f.write(bytes([len(p)]) + np.array(p, np.int32).tobytes())
+    with open(tmp_path / "a.ply", "w") as f:
+        f.write("ply\nformat ascii 1.0\nelement vertex 7\nproperty double x\nproperty double y\nproperty double z\n"
+                "element face 2\nproperty list uchar uint vertex_indices\nend_header\n")
+        for v in verts:
+            f.write("%.9g %.9g %.9g\n" % tuple(v))
+        for p in polys:
+            f.write(f"{len(p)} " + " ".join(map(str, p)) + "\n")
+    for name in ("m.obj", "m.ply", "a.ply"):
+        sc = scenes.cornell(width=16, height=16, spp=1)
+        opts = write_scene_json(sc, str(tmp_path / name.replace(".", "_")))
+        scene = json.load(open(os.path.join(os.path.dirname(opts), "scene.json")))
+        scene["Scene"]["Children"][0]["Geometry"]["Path"] = str(tmp_path / name)
+        json.dump(scene, open(os.path.join(os.path.dirname(opts), "scene.json"), "w"))
+        r = run(opts, "--export", tmp_path / "x")
+        assert r.returncode == 0, r.stderr
+        e = read_pgsc(tmp_path / "x" / "Action0.pgsc")
+        m = e["tri_object"] == 0
+        assert np.array_equal(e["vertices"][e["indices"][m]], verts[np.array(fan)]), name
+
+
+def test_exr_layout_and_half_rounding(tmp_path):
+    rng = np.random.default_rng(3)
+    w, h = 37, 11
+    img = np.exp(rng.normal(0, 4, (h, w, 3))).astype(np.float32)
+    edge = np.array([0, 65504, 65519.99, 65520, 1e9, 6.1035156e-05, 6.0e-05, 2.98e-08, 2.9802322e-08, 5.96e-08,
+                     1.0009765625, 1.00048828125, 1.00146484375, 0.1, 1 / 3], np.float32)
+    img.reshape(-1)[:edge.size] = edge
+    img[5, 5] = [np.nan, 1, 1]
+    img[6, 6] = [1, -0.5, 1]
+    img[7, 7] = [np.inf, 1, 1]
+    img.tofile(tmp_path / "rgb.f32")
+    r = run("--write-exr", tmp_path / "rgb.f32", w, h, tmp_path / "o.exr")
+    assert r.returncode == 0 and "3 invalid pixels" in r.stdout, r.stdout + r.stderr
+    got, attrs = read_exr(tmp_path / "o.exr")
+    with np.errstate(over="ignore"):
+        want = img.astype(np.float16)  # numpy: IEEE round to nearest even, overflow to infinity
+    for yx in ((5, 5), (6, 6), (7, 7)):
+        want[yx] = [1, 0, 0]
+    assert np.array_equal(got.view(np.uint16), want.view(np.uint16))
+    assert attrs["lineOrder"][1] == b"\0" and attrs["displayWindow"][1] == attrs["dataWindow"][1]
+    assert set(attrs) == {"channels", "compression", "dataWindow", "displayWindow", "lineOrder", "pixelAspectRatio",
+                          "screenWindowCenter", "screenWindowWidth"}
+
+
+@pytest.mark.parametrize("mutate,message", [
+    (lambda s: s.update(Integrator={"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator"}), "MaxTraceDepth"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(Diffuse={"ClassID": "Piper.BuiltinComponent."
+                                                                    "ProceduralTexture.CheckBoard"}), "ConstantTexture"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(
+        Diffuse={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [1, 1, 1]}),
+     "unsupported channel count 3"),
+    (lambda s: s["Actions"][0].update(ActiveSensor="Nobody"), "Failed to find a node"),
+    (lambda s: s["Actions"][0].update(FitMode="Stretch"), "Invalid FitMode"),
+    (lambda s: s["Scene"]["Children"].append({"Type": "Group", "Children": [{"Type": "Instance", "ClassID": "X.Y",
+                                                                             "Transform": {"Offset": 0}}]}),
+     "motion blur"),
+    (lambda s: s["Scene"]["Children"][0].update(Type="Mesh"), "invalid scene node type"),
+])
+def test_errors_are_reported(tmp_path, mutate, message):
+    sc = scenes.cornell(width=16, height=16, spp=1)
+    opts = write_scene_json(sc, str(tmp_path))
+    path = os.path.join(os.path.dirname(opts), "scene.json")
+    scene = json.load(open(path))
+    mutate(scene)
+    json.dump(scene, open(path, "w"))
+    r = run(opts, "--export", tmp_path / "x")
+    assert r.returncode == 1 and message in r.stderr, r.stderr
+
+
+def test_json_syntax_errors_name_the_position(tmp_path):
+    sc = scenes.cornell(width=16, height=16, spp=1)
+    opts = write_scene_json(sc, str(tmp_path))
+    path = os.path.join(os.path.dirname(opts), "scene.json")
+    with open(path, "w") as f:
+        f.write('{\n  "RenderDriver": {\n    "ClassID": "a" "b"\n  }\n}\n')
+    r = run(opts, "--export", tmp_path / "x")
+    assert r.returncode == 1 and "line 3, column" in r.stderr, r.stderr
+    with open(path, "w") as f:
+        f.write('{"a": "\\ud83d\\ude00 \\u00e9", "b": [1e3, -0.5, true, false, null]}')
+    r = run(opts, "--export", tmp_path / "x")
+    assert r.returncode == 1 and "missing key \"RenderDriver\"" in r.stderr, r.stderr  # parsed, then validated

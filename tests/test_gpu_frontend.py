@@ -1,0 +1,59 @@
+"""The scene front-end on the device: a reference-schema JSON scene rendered by piper_render
+(HIPTracer pipeline, FixedSampler frames, EXR output) equals pg_render of the scene the front-end
+exports, pixel for pixel after the same half conversion."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import piper2_amd as pg
+from piper2_amd import scenes
+from scene_json import FRONTEND, read_exr, read_pgsc, write_scene_json
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = pg.Device(0)
+    yield d
+    d.close()
+
+
+def direct_rgb(dev, e, film_filter=0, alpha=0.0):
+    sc = dict(e)
+    cam = sc["camera"]
+    assert sc["fit"] == 0
+    sc["ndc"], _ = scenes.calc_render_rect(sc["width"], sc["height"], float(cam[9] / cam[10]))
+    s = pg.Scene(dev, sc)
+    f = pg.Film(dev, sc["width"], sc["height"])
+    s.render(pg.render_params(sc, film_filter=film_filter, filter_alpha=alpha), f)
+    film = f.download()
+    w = film[..., 3:]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rgb = np.where(w != 0, film[..., :3] / w, np.float32(0)).astype(np.float32)
+    return rgb.astype(np.float16)
+
+
+@pytest.mark.parametrize("filt", [None, {"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter",
+                                         "FilterType": "gaussian", "Alpha": 2.0}])
+def test_json_scene_renders_like_pg_render(dev, tmp_path, filt):
+    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True)
+    opts = write_scene_json(sc, str(tmp_path), filter_cfg=filt)
+    scene_path = tmp_path / "scene.json"
+    scene = json.load(open(scene_path))
+    scene["Actions"][0]["FrameCount"] = 2
+    json.dump(scene, open(scene_path, "w"))
+    r = subprocess.run([FRONTEND, opts, "--export", tmp_path / "exp"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    e = read_pgsc(tmp_path / "exp" / "Action0.pgsc")
+    print("piper_render", opts, flush=True)
+    r = subprocess.run([FRONTEND, opts], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    frames = [read_exr(tmp_path / "out" / "Action0" / f"Frame{k}.exr")[0] for k in range(2)]
+    want = direct_rgb(dev, e, *((pg.PG_FILTER_GAUSSIAN, 2.0) if filt else ()))
+    assert np.array_equal(frames[0].view(np.uint16), want.view(np.uint16))
+    assert np.array_equal(frames[1].view(np.uint16), frames[0].view(np.uint16))  # a static scene
+    assert "paths" in r.stdout
