@@ -75,6 +75,29 @@ typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22
     float radiance[3];
 } pg_quad;
 
+#define PG_TEX_NONE 0xFFFFFFFFu  /* a material input without a texture: its constant is used */
+#define PG_TEX_CHECKERBOARD 1u   /* CheckBoard (ProceduralTexture/Kernel.cpp:28-38): White where the
+                                    parities of (u, v) scaled by Scale agree, Black elsewhere */
+#define PG_TEX_IMAGE 2u          /* SampledTexture (SampledTexture/Kernel.cpp:37-82): 8-bit texels
+                                    scaled by 1/255, bilinear, Repeat or Mirror wrap */
+#define PG_WRAP_REPEAT 0u
+#define PG_WRAP_MIRROR 1u
+typedef struct pg_texture {
+    uint32_t type;         /* PG_TEX_CHECKERBOARD or PG_TEX_IMAGE */
+    uint32_t channels;     /* 1, 2 or 4 */
+    float black[4], white[4];
+    float scale;           /* checkerboard */
+    uint32_t width, height, wrap; /* image: texels hold height rows of width * channels bytes */
+    const uint8_t* texels;
+} pg_texture;
+/* Texture of each colour input of a material (PG_TEX_NONE or an index into pg_scene_desc.textures);
+ * a texture needs at least 3 channels and replaces the constant at every hit, sampled at the hit's
+ * texture coordinate. */
+typedef struct pg_material_textures {
+    uint32_t albedo;   /* replaces pg_material.albedo */
+    uint32_t reserved; /* replaces pg_material.reserved (plastic / substrate specular, glass transmission) */
+} pg_material_textures;
+
 typedef struct pg_point_light { /* PointLightData (Light/BasicLight/Shared.hpp:26-29): a delta light */
     float position[3];  /* the node's world position (pointInit, Light/BasicLight/Kernel.cpp:27-33) */
     float intensity[3]; /* radiant intensity; radiance at distance d is intensity / d^2 (:35-45) */
@@ -107,6 +130,13 @@ typedef struct pg_scene_desc {
      * order. Zero / NULL for none. */
     const pg_point_light* point_lights;
     uint32_t num_point_lights;
+    /* textures (optional): per-vertex texture coordinates (2 floats per vertex; NULL gives (0, 0)
+     * everywhere, calcTriangleMeshSurface, Tracer/Embree/Kernel.cpp:170-192), the textures, and one
+     * pg_material_textures per material (NULL: no textured material) */
+    const float* texcoords;
+    const pg_texture* textures;
+    uint32_t num_textures;
+    const pg_material_textures* material_textures;
 } pg_scene_desc;
 
 #define PG_BVH_HOST 0u   /* binned SAH on the host's cores (multithreaded) */

@@ -189,6 +189,15 @@ struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
     V3 origin, u, v, normal, radiance;
     float area;
 };
+// Textures: CheckBoard (ProceduralTexture/Kernel.cpp:28-38) and 8-bit sampled images
+// (SampledTexture/Kernel.cpp:22-82).
+struct Texture {
+    uint32_t type;  // 1 checkerboard, 2 image
+    uint32_t channels;
+    float black[4], white[4], scale;
+    uint32_t width, height, wrap;  // wrap 0 repeat, 1 mirror
+    std::vector<uint8_t> texels;
+};
 struct PointLight {  // PointLightData, BasicLight/Shared.hpp:26-29 (position from the node)
     V3 pos, intensity;
 };
@@ -210,6 +219,9 @@ struct Scene {
     std::vector<Material> mats;
     std::vector<Quad> quads;
     std::vector<PointLight> points;
+    std::vector<float> triUV;  // 6 per triangle: texture coordinates of corners 0, 1, 2
+    std::vector<Texture> textures;
+    std::vector<std::array<uint32_t, 2>> matTex;  // per material: albedo, spec texture (~0u: none)
     Camera cam;
     V3 env;
     std::vector<BNode> nodes;
@@ -781,6 +793,63 @@ BsdfSample bsdfSample(const Material& m, V3 wo, V3 ng, float u1, float u2) {
     return s;
 }
 
+// Texture lookups. The checkerboard folds x * scale into [0, 2) with an IEEE remainder; the image
+// takes the texels left and right of p - 1/2 with wrap, weighted by the distance of p to the texel
+// centre, and scales the 8-bit values by 1/255.
+std::array<float, 4> sampleTexture(const Texture& t, float u, float v) {
+    std::array<float, 4> c{0.0f, 0.0f, 0.0f, 0.0f};
+    if (t.type == 1) {
+        auto fold = [&](float x) {
+            const float r = std::remainder(x * t.scale, 2.0f);
+            return r < 0.0f ? r + 2.0f : r;
+        };
+        const float* src = (fold(u) > 1.0f) == (fold(v) > 1.0f) ? t.white : t.black;
+        for (uint32_t i = 0; i < t.channels; ++i) c[i] = src[i];
+        return c;
+    }
+    auto wrapIndex = [&](float p, uint32_t size) {
+        const float r = std::remainder(p, static_cast<float>(size));
+        const float q = t.wrap == 0 ? (r < 0.0f ? r + static_cast<float>(size) : r) : std::fabs(r);
+        return std::min(static_cast<uint32_t>(q), size - 1);
+    };
+    auto step = [&](uint32_t i, uint32_t size) { return i + 1 == size ? (t.wrap == 0 ? 0u : i) : i + 1; };
+    const float pu = u * static_cast<float>(t.width), pv = v * static_cast<float>(t.height);
+    const uint32_t x0 = wrapIndex(pu - 0.5f, t.width), x1 = step(x0, t.width);
+    const uint32_t y0 = wrapIndex(pv - 0.5f, t.height), y1 = step(y0, t.height);
+    const float ax = 0.5f + std::floor(pu + 0.5f) - pu, ay = 0.5f + std::floor(pv + 0.5f) - pv;
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    const uint32_t xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+    const float ws[4] = {ax * ay, bx * ay, ax * by, bx * by};
+    for (int k = 0; k < 4; ++k) {
+        const uint8_t* px = t.texels.data() + (static_cast<size_t>(ys[k]) * t.width + xs[k]) * t.channels;
+        for (uint32_t i = 0; i < t.channels; ++i) c[i] += ws[k] * static_cast<float>(px[i]);
+    }
+    for (uint32_t i = 0; i < t.channels; ++i) c[i] *= 0.003921568f;
+    return c;
+}
+// The material at a triangle hit with its textures applied at the hit's texture coordinate
+// (corner 1 weighted by u, corner 2 by v, corner 0 by 1 - u - v).
+void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
+    if (matId >= s.matTex.size()) return;
+    const auto& b = s.matTex[matId];
+    if (b[0] == ~0u && b[1] == ~0u) return;
+    float tu = 0.0f, tv = 0.0f;
+    if (!s.triUV.empty()) {
+        const float* c = &s.triUV[6 * static_cast<size_t>(h.prim)];
+        const float w = 1.0f - h.u - h.v;
+        tu = c[2] * h.u + c[4] * h.v + c[0] * w;
+        tv = c[3] * h.u + c[5] * h.v + c[1] * w;
+    }
+    if (b[0] != ~0u) {
+        const auto c = sampleTexture(s.textures[b[0]], tu, tv);
+        m.albedo = {c[0], c[1], c[2]};
+    }
+    if (b[1] != ~0u) {
+        const auto c = sampleTexture(s.textures[b[1]], tu, tv);
+        m.spec = {c[0], c[1], c[2]};
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Integrator (Integrator/PathTracing/Kernel.cpp) and per-sample driver (Tracer/Embree/Kernel.cpp)
 // ---------------------------------------------------------------------------------------------
@@ -924,7 +993,9 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
         }
         if (h.prim == kMiss || depth >= p.maxDepth) break;  // :148
 
-        Material mat = s.mats[s.objMat[s.triObj[h.prim]]];
+        const uint32_t matId = s.objMat[s.triObj[h.prim]];
+        Material mat = s.mats[matId];
+        applyTextures(s, matId, h, mat);
         const V3 n = hitNormal(s, h, ray.d);
         mat.front = dot(ray.d, triNg(s.tris[h.prim])) < 0.0f;  // Embree.cpp:336
         const Frame fr = shadingFrame(n, d3);
@@ -1263,6 +1334,30 @@ void or_scene_set_points(void* scene, const float* points, uint32_t n) {
         const float* p = points + 6 * i;
         s->points.push_back({{p[0], p[1], p[2]}, {p[3], p[4], p[5]}});
     }
+}
+// Textures: triuv[6*nt] texture coordinates of each triangle's corners (or NULL), one texture per
+// or_scene_add_texture call, mtex[2*nmat] albedo / spec texture per material (~0u: none).
+void or_scene_set_textures(void* scene, const float* triuv, const uint32_t* mtex) {
+    auto* s = static_cast<Scene*>(scene);
+    s->triUV.assign(triuv, triuv ? triuv + 6 * s->tris.size() : triuv);
+    s->matTex.resize(s->mats.size());
+    for (size_t i = 0; i < s->mats.size(); ++i) s->matTex[i] = {mtex[2 * i], mtex[2 * i + 1]};
+}
+// itype {type, channels, width, height, wrap}; ftype {black[4], white[4], scale}
+uint32_t or_scene_add_texture(void* scene, const uint32_t* itype, const float* ftype, const uint8_t* texels) {
+    auto* s = static_cast<Scene*>(scene);
+    Texture t;
+    t.type = itype[0], t.channels = itype[1], t.width = itype[2], t.height = itype[3], t.wrap = itype[4];
+    std::memcpy(t.black, ftype, 16), std::memcpy(t.white, ftype + 4, 16);
+    t.scale = ftype[8];
+    if (t.type == 2) t.texels.assign(texels, texels + static_cast<size_t>(t.width) * t.height * t.channels);
+    s->textures.push_back(std::move(t));
+    return static_cast<uint32_t>(s->textures.size() - 1);
+}
+// one texture lookup (for tests)
+void or_texture_sample(void* scene, uint32_t tex, float u, float v, float* out) {
+    const auto c = sampleTexture(static_cast<Scene*>(scene)->textures[tex], u, v);
+    std::memcpy(out, c.data(), 16);
 }
 float or_fresnel_dielectric(float cosI, float etaI, float etaT) { return fresnelDielectric(cosI, etaI, etaT); }
 float or_remap_roughness(float roughness) { return remapRoughness(roughness); }

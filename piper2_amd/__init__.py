@@ -35,6 +35,14 @@ PG_FILTER_GAUSSIAN = 2
 PG_MAT_LAMBERT = 0
 PG_MAT_CONDUCTOR = 1
 PG_MAT_PLASTIC = 2
+PG_MAT_MIRROR = 3
+PG_MAT_GLASS = 4
+PG_MAT_SUBSTRATE = 5
+PG_TEX_NONE = 0xFFFFFFFF
+PG_TEX_CHECKERBOARD = 1
+PG_TEX_IMAGE = 2
+PG_WRAP_REPEAT = 0
+PG_WRAP_MIRROR = 1
 PG_HIT_MISS = 0xFFFFFFFF
 PG_HIT_QUAD = 0x80000000
 
@@ -63,7 +71,15 @@ class SceneDesc(ctypes.Structure):
         ("quads", ctypes.c_void_p), ("num_quads", ctypes.c_uint32),
         ("camera", Camera), ("env_radiance", F3),
         ("point_lights", ctypes.c_void_p), ("num_point_lights", ctypes.c_uint32),
+        ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("num_textures", ctypes.c_uint32),
+        ("material_textures", ctypes.c_void_p),
     ]
+
+
+class Texture(ctypes.Structure):  # pg_texture
+    _fields_ = [("type", ctypes.c_uint32), ("channels", ctypes.c_uint32), ("black", ctypes.c_float * 4),
+                ("white", ctypes.c_float * 4), ("scale", ctypes.c_float), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("wrap", ctypes.c_uint32), ("texels", ctypes.c_void_p)]
 
 
 class BvhParams(ctypes.Structure):
@@ -447,13 +463,36 @@ def scene_desc(sc):
         pts = sc["_points"] = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
     else:
         pts = None
+    # optional textures: "texcoords" (V, 2) float32, "textures" (list of dicts with the pg_texture
+    # fields; "texels" a uint8 array), "material_textures" (M, 2) uint32
+    texcoords = sc.get("texcoords")
+    if texcoords is not None:
+        texcoords = sc["_texcoords"] = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+    texs = sc.get("textures") or []
+    tex_arr = (Texture * max(len(texs), 1))()
+    keep = []
+    for i, t in enumerate(texs):
+        texels = None
+        if t.get("texels") is not None:
+            texels = np.ascontiguousarray(t["texels"], np.uint8)
+            keep.append(texels)
+        tex_arr[i] = Texture(int(t["type"]), int(t["channels"]), (ctypes.c_float * 4)(*t.get("black", [0] * 4)),
+                             (ctypes.c_float * 4)(*t.get("white", [0] * 4)), float(t.get("scale", 0.0)),
+                             int(t.get("width", 0)), int(t.get("height", 0)), int(t.get("wrap", 0)),
+                             _ptr(texels) if texels is not None else None)
+    mtex = sc.get("material_textures")
+    if mtex is not None:
+        mtex = sc["_mtex"] = np.ascontiguousarray(mtex, np.uint32).reshape(-1, 2)
+    sc["_tex_arr"], sc["_texels"] = tex_arr, keep
     c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
     cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))
     return SceneDesc(
         _ptr(sc["vertices"]), sc["vertices"].shape[0], _ptr(sc["indices"]), sc["indices"].shape[0],
         _ptr(sc["tri_object"]), _ptr(sc["object_material"]), sc["object_material"].shape[0],
         ctypes.cast(mat_arr, ctypes.c_void_p), len(mats), ctypes.cast(quad_arr, ctypes.c_void_p), len(quads),
-        cam, F3(*sc["env"]), _ptr(pts) if pts is not None else None, 0 if pts is None else pts.shape[0])
+        cam, F3(*sc["env"]), _ptr(pts) if pts is not None else None, 0 if pts is None else pts.shape[0],
+        _ptr(texcoords) if texcoords is not None else None, ctypes.cast(tex_arr, ctypes.c_void_p) if texs else None,
+        len(texs), _ptr(mtex) if mtex is not None else None)
 
 
 def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0, film_filter=0,

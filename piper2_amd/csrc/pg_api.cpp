@@ -154,7 +154,7 @@ struct pg_scene {
     pg::BvhParams bvhParams;  // of the current build
     uint32_t builder = PG_BVH_HOST;
     pg::DevScene ds{};
-    DevBuf tris, triMatLeaf, nodes, mats, quads, points;
+    DevBuf tris, triMatLeaf, nodes, mats, quads, points, triUV, textures, texels, matTex;
     uint32_t spillPerLane = 1;
     float lo[3] = {0, 0, 0}, invExt[3] = {1, 1, 1};  // scene box (triangles and light quads) -> [0,1]^3
 };
@@ -537,6 +537,8 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
     }
     const pg::BatchTable* btp = opt.batch ? &bt : nullptr;
     const pg::DevScene& ds0 = scenes[0]->ds;  // the batch's BVH width; per-path scenes come from bt
+    bool anyTextured = false;
+    for (uint32_t i = 0; i < nScenes; ++i) anyTextured = anyTextured || scenes[i]->ds.textured != 0;
     const pg::RenderConsts& rc0 = opt.batch ? items[0].rc : opt.rc0;
 
     const pg::PathState ps = pathState(dev);
@@ -694,7 +696,7 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
                                               dev->mix.as<float>(), blocks);
             });
         timeStage(kShade, [&] {
-            pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp);
+            pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp, anyTextured);
         });
         timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
         if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
@@ -1035,6 +1037,59 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
             }
             mats[i] = pg::DevMaterial{type, {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha, {spec[0], spec[1], spec[2]}};
         }
+        // textures: per-triangle corner texcoords, the texture table with one texel pool, and the
+        // binding of each material's colour inputs
+        std::vector<pg::DevTexture> texTab(std::max<uint32_t>(desc->num_textures, 1));
+        std::vector<uint8_t> texelPool;
+        require(desc->num_textures == 0 || desc->textures, "textures missing");
+        for (uint32_t i = 0; i < desc->num_textures; ++i) {
+            const pg_texture& t = desc->textures[i];
+            require(t.channels == 1 || t.channels == 2 || t.channels == 4, "texture channels must be 1, 2 or 4");
+            pg::DevTexture& d = texTab[i];
+            d = pg::DevTexture{};
+            d.type = t.type, d.channels = t.channels;
+            std::memcpy(d.black, t.black, 16), std::memcpy(d.white, t.white, 16);
+            if (t.type == PG_TEX_CHECKERBOARD) {
+                require(std::isfinite(t.scale), "checkerboard scale must be finite");
+                d.scale = t.scale;
+            } else {
+                require(t.type == PG_TEX_IMAGE, "unknown texture type");
+                require(t.width > 0 && t.height > 0 && t.width <= 65536 && t.height <= 65536 && t.texels,
+                        "image texture needs width, height and texels");
+                require(t.wrap == PG_WRAP_REPEAT || t.wrap == PG_WRAP_MIRROR, "unknown texture wrap mode");
+                d.width = t.width, d.height = t.height, d.wrap = t.wrap;
+                d.offset = static_cast<uint32_t>(texelPool.size());
+                const size_t bytes = static_cast<size_t>(t.width) * t.height * t.channels;
+                require(texelPool.size() + bytes < (1ull << 32), "texel pool exceeds 4 GiB");
+                texelPool.insert(texelPool.end(), t.texels, t.texels + bytes);
+            }
+        }
+        std::vector<uint2> matTex(desc->num_materials, uint2{pg::kTexNone, pg::kTexNone});
+        bool textured = false;
+        if (desc->material_textures) {
+            for (uint32_t i = 0; i < desc->num_materials; ++i) {
+                const pg_material_textures& b = desc->material_textures[i];
+                for (uint32_t t : {b.albedo, b.reserved}) {
+                    require(t == PG_TEX_NONE || t < desc->num_textures, "material texture index out of range");
+                    require(t == PG_TEX_NONE || desc->textures[t].channels >= 3, "colour textures need 4 channels");
+                }
+                matTex[i] = uint2{b.albedo, b.reserved};
+                if (b.albedo != PG_TEX_NONE || b.reserved != PG_TEX_NONE) {
+                    mats[i].type |= pg::kMatTextured;
+                    textured = true;
+                }
+            }
+        }
+        std::vector<float> triUV;
+        if (textured && desc->texcoords) {
+            triUV.resize(static_cast<size_t>(nt) * 6);
+            for (uint32_t i = 0; i < nt; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t v = desc->indices[3 * i + k];
+                    triUV[6 * i + 2 * k] = desc->texcoords[2 * v];
+                    triUV[6 * i + 2 * k + 1] = desc->texcoords[2 * v + 1];
+                }
+        }
         std::vector<pg::DevQuad> quads(desc->num_quads);
         for (uint32_t i = 0; i < desc->num_quads; ++i) {
             const pg_quad& q = desc->quads[i];
@@ -1060,6 +1115,15 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         ds.numQuads = desc->num_quads;
         ds.points = s->points.as<float>();
         ds.numPoints = desc->num_point_lights;
+        upload(s->textures, texTab.data(), texTab.size() * sizeof(pg::DevTexture), st);
+        upload(s->texels, texelPool.data(), texelPool.size(), st);
+        upload(s->matTex, matTex.data(), matTex.size() * sizeof(uint2), st);
+        upload(s->triUV, triUV.data(), triUV.size() * sizeof(float), st);
+        ds.textures = s->textures.as<pg::DevTexture>();
+        ds.texels = s->texels.as<uint8_t>();
+        ds.matTex = s->matTex.as<uint2>();
+        ds.triUV = triUV.empty() ? nullptr : s->triUV.as<float>();
+        ds.textured = textured ? 1u : 0u;
         ds.sobol = dev->sobol.as<uint32_t>();
         ds.sobolT = dev->sobolT.as<uint32_t>();
         ds.sobolB = dev->sobolB.as<uint32_t>();

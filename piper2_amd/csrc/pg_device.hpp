@@ -473,23 +473,106 @@ __device__ __forceinline__ float fresnelDielectric(float cosI, float etaI, float
     return 0.5f * (par * par + perp * perp);
 }
 
-// A hit's material. The plastic and Oren-Nayar extras (spec) stay in the scene's table and are
-// read where a lobe needs them, so they hold no registers across the vertex.
+// Textures (Texture/ProceduralTexture/Kernel.cpp:28-38, Texture/SampledTexture/Kernel.cpp:22-82).
+__device__ __forceinline__ float checkerMod(float x, float scale) {
+    const float rem = remainderf(x * scale, 2.0f);
+    return rem < 0.0f ? rem + 2.0f : rem;
+}
+__device__ __forceinline__ uint32_t texLocate(float p, uint32_t size, uint32_t wrap) {
+    const float fs = static_cast<float>(size);
+    const float rem = remainderf(p, fs);
+    const float q = wrap == kWrapRepeat ? (rem < 0.0f ? rem + fs : rem) : fabsf(rem);
+    return min(static_cast<uint32_t>(q), size - 1u);
+}
+__device__ __forceinline__ uint32_t texNext(uint32_t p, uint32_t size, uint32_t wrap) {
+    const uint32_t q = p + 1u;
+    return q == size ? (wrap == kWrapRepeat ? 0u : p) : q;
+}
+__device__ __noinline__ float4 sampleTexture(const DevTexture* texTab, const uint8_t* texels, uint32_t t, float2 uv) {
+    const DevTexture& tx = texTab[t];
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (tx.type == kTexChecker) {
+        const bool white = (checkerMod(uv.x, tx.scale) > 1.0f) == (checkerMod(uv.y, tx.scale) > 1.0f);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) c[i] = i < tx.channels ? (white ? tx.white[i] : tx.black[i]) : 0.0f;
+    } else {  // bilinear over 8-bit texels, left tap at floor(p - 0.5)
+        const float pu = uv.x * static_cast<float>(tx.width), pv = uv.y * static_cast<float>(tx.height);
+        const uint32_t lu = texLocate(pu - 0.5f, tx.width, tx.wrap), ru = texNext(lu, tx.width, tx.wrap);
+        const uint32_t lv = texLocate(pv - 0.5f, tx.height, tx.wrap), rv = texNext(lv, tx.height, tx.wrap);
+        const float lwx = 0.5f + floorf(pu + 0.5f) - pu, lwy = 0.5f + floorf(pv + 0.5f) - pv;
+        const float rwx = 1.0f - lwx, rwy = 1.0f - lwy;
+        const uint8_t* base = texels + tx.offset;
+        const uint32_t stride = tx.width * tx.channels;
+        const uint32_t taps[4][2] = {{lu, lv}, {ru, lv}, {lu, rv}, {ru, rv}};
+        const float wts[4] = {lwx * lwy, rwx * lwy, lwx * rwy, rwx * rwy};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint8_t* px = base + taps[k][1] * stride + taps[k][0] * tx.channels;
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i)
+                if (i < tx.channels) c[i] += wts[k] * static_cast<float>(px[i]);
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) c[i] *= 0.003921568f;
+    }
+    return make_float4(c[0], c[1], c[2], c[3]);
+}
+// calcTriangleMeshSurface (Tracer/Embree/Kernel.cpp:175-188): corners 1 and 2 weighted by the
+// hit's barycentrics u and v, corner 0 by 1 - u - v.
+__device__ __forceinline__ float2 hitTexcoord(const DevScene& s, uint32_t pos, float bu, float bv) {
+    if (!s.triUV) return make_float2(0.0f, 0.0f);
+    const uint32_t prim = __float_as_uint(s.tris[3 * pos].w);
+    const float* t = s.triUV + 6 * static_cast<size_t>(prim);
+    const float w = 1.0f - bu - bv;
+    return make_float2(t[2] * bu + t[4] * bv + t[0] * w, t[3] * bu + t[5] * bv + t[1] * w);
+}
+
+// A hit's material. The plastic and Oren-Nayar extras (spec) stay in the scene's table (or its
+// texture) and are read where a lobe needs them, so they hold no registers across the vertex.
 struct Mat {
-    uint32_t bits;  // type (0 Lambert, 1 conductor, 2 plastic, 3 Oren-Nayar) | front face << 8
-    V3 albedo;      // Lambert / Oren-Nayar / plastic diffuse, conductor F0
+    uint32_t bits;  // type (0 Lambert, 1 conductor, 2 plastic, 3 mirror, 4 glass, 5 substrate,
+                    // 16 Oren-Nayar) | front face << 8 | kMatTextured
+    V3 albedo;      // diffuse colour, conductor F0, mirror / glass reflection (texture applied)
     float alpha;    // GGX alpha; Oren-Nayar: A
     uint32_t id;
+    float2 uv;      // the hit's texture coordinate (textured materials)
     const DevMaterial* tab;
+    const DevTexture* texTab;
+    const uint8_t* texels;
+    const uint2* matTex;
     __device__ __forceinline__ uint32_t type() const { return bits & 0xFFu; }
-    // plastic: the hit is on the triangle's front face (plasticInit's etaI, etaT)
-    __device__ __forceinline__ bool front() const { return (bits >> 8) != 0u; }
-    // plastic specular reflectance; Oren-Nayar: B in x
-    __device__ __forceinline__ V3 spec() const { return ld3(tab[id].spec); }
+    // plastic / glass: the hit is on the triangle's front face (etaI, etaT of their init)
+    __device__ __forceinline__ bool front() const { return ((bits >> 8) & 1u) != 0u; }
+    // plastic / substrate specular, glass transmission; Oren-Nayar: B in x
+    __device__ __forceinline__ V3 spec() const {
+        if (bits & kMatTextured) {
+            const uint32_t t = matTex[id].y;
+            if (t != kTexNone) {
+                const float4 c = sampleTexture(texTab, texels, t, uv);
+                return {c.x, c.y, c.z};
+            }
+        }
+        return ld3(tab[id].spec);
+    }
 };
-__device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true) {
+// TEX false: the scene has no textured material, so the flag is cleared and every texture lookup
+// (here and in Mat::spec) compiles away.
+template <bool TEX = true>
+__device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true, uint32_t pos = 0,
+                                       float bu = 0.0f, float bv = 0.0f) {
     const DevMaterial& m = s.mats[id];
-    return Mat{m.type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, id, s.mats};
+    const uint32_t type = TEX ? m.type : (m.type & ~kMatTextured);
+    Mat r{type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, id, make_float2(0.0f, 0.0f), s.mats, s.textures,
+          s.texels, s.matTex};
+    if (type & kMatTextured) {
+        r.uv = hitTexcoord(s, pos, bu, bv);
+        const uint32_t t = s.matTex[id].x;
+        if (t != kTexNone) {
+            const float4 c = sampleTexture(s.textures, s.texels, t, r.uv);
+            r.albedo = {c.x, c.y, c.z};
+        }
+    }
+    return r;
 }
 
 // Plastic's glossy lobe: MicrofacetReflection<FresnelDielectric> (Classic/Kernel.cpp:384-433,

@@ -218,7 +218,7 @@ static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging s
 // Waves per SIMD: 4 (<= 128 VGPRs) for the plain and batched kernels: at 5 (<= 96 VGPRs) the
 // plastic / Oren-Nayar lobes spill 14 VGPRs and the stage takes 18.9 instead of 17.0 ms on the
 // 1080p64 terrain frame. The guided ones keep 3 for their extra state.
-template <bool BATCH, bool GUIDED>
+template <bool BATCH, bool GUIDED, bool TEX>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
 kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
        GuideShade gs, BatchTable bt) {
@@ -274,7 +274,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             } else {
                 bool front;
                 const V3 nrm = triNormal(s, pos, ray.d, &front);
-                const Mat mat = loadMat(s, s.triMat[pos], front);
+                const Mat mat = loadMat<TEX>(s, s.triMat[pos], front, pos, hit4.z, hit4.w);
                 const Frame fr = shadingFrame(nrm, d3);
                 const V3 wo = fr.toLocal(-ray.d);
                 const V3 ng = fr.toLocal(nrm);
@@ -923,22 +923,29 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
     if (s.width == 8) extendW<8>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
     else extendW<4>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
 }
-void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
-                 const BatchTable* bt) {
+template <bool TEX>
+void shadeVariant(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
+                  const BatchTable* bt) {
     const dim3 grid(gridThreads / 256);
     if (bt && gs.enabled)  // guided batch: per-item guides in bt.guide
-        hipLaunchKernelGGL((kShade<true, true>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
-                           *bt);
+        hipLaunchKernelGGL((kShade<true, true, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+                           gs, *bt);
     else if (bt)
-        hipLaunchKernelGGL((kShade<true, false>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
-                           *bt);
+        hipLaunchKernelGGL((kShade<true, false, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+                           gs, *bt);
     else if (gs.enabled)
-        hipLaunchKernelGGL((kShade<false, true>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
-                           BatchTable{});
+        hipLaunchKernelGGL((kShade<false, true, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+                           gs, BatchTable{});
     else
-        hipLaunchKernelGGL((kShade<false, false>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs,
-                           BatchTable{});
+        hipLaunchKernelGGL((kShade<false, false, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+                           gs, BatchTable{});
+}
+void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                 uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
+                 const BatchTable* bt, bool textured) {
+    if (textured) shadeVariant<true>(st, s, ps, q, rc, curSlot, nextSlot, gridThreads, totals, gs, bt);
+    else shadeVariant<false>(st, s, ps, q, rc, curSlot, nextSlot, gridThreads, totals, gs, bt);
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {

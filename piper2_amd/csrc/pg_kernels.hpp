@@ -106,7 +106,7 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
                   const BatchTable* bt = nullptr);
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
-                 const BatchTable* bt = nullptr);
+                 const BatchTable* bt, bool textured);
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,

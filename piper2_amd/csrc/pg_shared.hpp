@@ -39,6 +39,17 @@ static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 B");
 // Material types: the public PG_MAT_* values (0 Lambert, 1 conductor, 2 plastic, 3 mirror,
 // 4 smooth glass, 5 substrate) plus this device-only one.
 constexpr uint32_t kDevOrenNayar = 16;  // matte with roughness > 0
+constexpr uint32_t kMatTextured = 0x80000000u;  // type flag: a colour input comes from a texture
+constexpr uint32_t kTexNone = 0xFFFFFFFFu;
+constexpr uint32_t kTexChecker = 1, kTexImage = 2;   // PG_TEX_*
+constexpr uint32_t kWrapRepeat = 0, kWrapMirror = 1;  // PG_WRAP_*
+struct DevTexture {  // 64 B
+    uint32_t type, channels, width, height;
+    uint32_t wrap;
+    uint32_t offset;  // first texel byte in the scene's texel pool
+    float scale, pad;
+    float black[4], white[4];
+};
 struct DevMaterial {  // 32 B
     uint32_t type;
     float albedo[3];   // diffuse reflectance (Lambert, Oren-Nayar, plastic, substrate), conductor F0,
@@ -68,6 +79,10 @@ struct DevScene {
     const DevMaterial* mats;
     const DevQuad* quads;
     const float* points;         // point lights: position xyz, intensity rgb
+    const float* triUV;          // texture coordinates of each original triangle's corners (6 floats)
+    const DevTexture* textures;
+    const uint8_t* texels;
+    const uint2* matTex;         // per material: albedo and spec texture (kTexNone: the constant)
     const uint32_t* sobol;       // kSobolDims * kSobolCols
     const uint32_t* sobolT;      // the same matrices bit-major: kSobolCols * kSobolDims
     const uint32_t* sobolB;      // byte-sliced: kByteChunks * 256 * kByteDims
@@ -75,6 +90,7 @@ struct DevScene {
     const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
     uint32_t numTris, numNodes, numQuads, width;
     uint32_t numPoints;
+    uint32_t textured;  // some material has a texture: shade takes its textured variant
     DevCamera cam;
     float env[3];
     uint32_t hasBvh;

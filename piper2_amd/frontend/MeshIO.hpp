@@ -1,12 +1,15 @@
 // Dependency-free triangle-mesh loaders, standing in for the Assimp import of
-// Geometry/TriangleMesh (TriangleMesh.cpp:47-60, aiProcess_Triangulate): Wavefront OBJ and PLY
-// (ascii and binary_little_endian). Positions only: the HIP path has no texture coordinates.
-// Polygons are fanned into triangles around their first vertex, which is what a triangulating
-// importer does for convex faces. Errors throw std::runtime_error naming the file and line.
+// Geometry/TriangleMesh (TriangleMesh.cpp:47-119, aiProcess_Triangulate): Wavefront OBJ and PLY
+// (ascii and binary_little_endian), positions and the first set of texture coordinates. An OBJ
+// whose faces name texture coordinates gets one vertex per distinct (position, texcoord) pair, as
+// an importer that joins identical vertices produces. Polygons are fanned into triangles around
+// their first vertex, which is what a triangulating importer does for convex faces. Errors throw
+// std::runtime_error naming the file and line.
 #pragma once
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -17,6 +20,7 @@ namespace PiperFrontend {
 struct TriangleMeshData {
     std::vector<float> verts;     // float3 per vertex
     std::vector<uint32_t> index;  // uint3 per triangle
+    std::vector<float> uv;        // float2 per vertex, or empty
 };
 
 namespace detail {
@@ -35,11 +39,21 @@ inline bool endsWith(const std::string& s, const std::string& suffix) {
 inline TriangleMeshData loadObj(const std::string& path) {
     std::ifstream f(path);
     if (!f) throw std::runtime_error("cannot open mesh " + path);
-    TriangleMeshData m;
+    std::vector<float> pos, tex;                 // as listed in the file
+    std::vector<std::pair<long, long>> corners;  // (position, texcoord or -1) per triangle corner
+    bool anyTex = false;
     std::string line;
     size_t lineNo = 0;
-    std::vector<uint32_t> poly;
+    std::vector<std::pair<long, long>> poly;
     auto fail = [&](const std::string& msg) { throw std::runtime_error(path + ":" + std::to_string(lineNo) + ": " + msg); };
+    auto resolve = [&](const std::string& s, long n) -> long {
+        char* end = nullptr;
+        const long k = std::strtol(s.c_str(), &end, 10);
+        if (s.empty() || *end != '\0' || k == 0) fail("bad face corner index \"" + s + "\"");
+        const long idx = k > 0 ? k - 1 : n + k;
+        if (idx < 0 || idx >= n) fail("face index out of range");
+        return idx;
+    };
     while (std::getline(f, line)) {
         ++lineNo;
         std::istringstream in(line);
@@ -48,24 +62,49 @@ inline TriangleMeshData loadObj(const std::string& path) {
         if (tag == "v") {
             float x, y, z;
             if (!(in >> x >> y >> z)) fail("bad vertex");
-            m.verts.insert(m.verts.end(), {x, y, z});
+            pos.insert(pos.end(), {x, y, z});
+        } else if (tag == "vt") {
+            float u, v = 0.0f;
+            if (!(in >> u)) fail("bad texture coordinate");
+            in >> v;
+            tex.insert(tex.end(), {u, v});
         } else if (tag == "f") {
             poly.clear();
             std::string corner;
-            const long nv = static_cast<long>(m.verts.size() / 3);
             while (in >> corner) {
-                const size_t slash = corner.find('/');
-                const std::string head = corner.substr(0, slash);
-                char* end = nullptr;
-                const long k = std::strtol(head.c_str(), &end, 10);
-                if (head.empty() || *end != '\0' || k == 0) fail("bad face corner \"" + corner + "\"");
-                const long idx = k > 0 ? k - 1 : nv + k;
-                if (idx < 0 || idx >= nv) fail("face index out of range");
-                poly.push_back(static_cast<uint32_t>(idx));
+                const size_t s1 = corner.find('/');
+                const long p = resolve(corner.substr(0, s1), static_cast<long>(pos.size() / 3));
+                long t = -1;
+                if (s1 != std::string::npos) {
+                    const size_t s2 = corner.find('/', s1 + 1);
+                    const std::string ts = corner.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
+                    if (!ts.empty()) {
+                        t = resolve(ts, static_cast<long>(tex.size() / 2));
+                        anyTex = true;
+                    }
+                }
+                poly.push_back({p, t});
             }
             if (poly.size() < 3) fail("face with fewer than 3 corners");
-            for (size_t t = 1; t + 1 < poly.size(); ++t) m.index.insert(m.index.end(), {poly[0], poly[t], poly[t + 1]});
+            for (size_t t = 1; t + 1 < poly.size(); ++t) corners.insert(corners.end(), {poly[0], poly[t], poly[t + 1]});
         }
+    }
+    TriangleMeshData m;
+    if (!anyTex) {  // positions as listed, faces index them directly
+        m.verts = std::move(pos);
+        for (const auto& c : corners) m.index.push_back(static_cast<uint32_t>(c.first));
+        return m;
+    }
+    std::map<std::pair<long, long>, uint32_t> lookup;  // one vertex per distinct (position, texcoord)
+    for (const auto& c : corners) {
+        auto it = lookup.find(c);
+        if (it == lookup.end()) {
+            it = lookup.emplace(c, static_cast<uint32_t>(m.verts.size() / 3)).first;
+            m.verts.insert(m.verts.end(), {pos[3 * c.first], pos[3 * c.first + 1], pos[3 * c.first + 2]});
+            if (c.second >= 0) m.uv.insert(m.uv.end(), {tex[2 * c.second], tex[2 * c.second + 1]});
+            else m.uv.insert(m.uv.end(), {0.0f, 0.0f});
+        }
+        m.index.push_back(it->second);
     }
     return m;
 }
@@ -156,12 +195,17 @@ inline TriangleMeshData loadPly(const std::string& path) {
     size_t nv = 0;
     for (const Element& e : elems) {
         if (e.name == "vertex") {
-            int ix = -1, iy = -1, iz = -1;
+            int ix = -1, iy = -1, iz = -1, iu = -1, iv = -1;
             for (size_t k = 0; k < e.props.size(); ++k) {
-                if (e.props[k].name == "x") ix = static_cast<int>(k);
-                if (e.props[k].name == "y") iy = static_cast<int>(k);
-                if (e.props[k].name == "z") iz = static_cast<int>(k);
+                const std::string& n = e.props[k].name;
+                if (n == "x") ix = static_cast<int>(k);
+                if (n == "y") iy = static_cast<int>(k);
+                if (n == "z") iz = static_cast<int>(k);
+                if (n == "u" || n == "s" || n == "texture_u" || n == "texture_s") iu = static_cast<int>(k);
+                if (n == "v" || n == "t" || n == "texture_v" || n == "texture_t") iv = static_cast<int>(k);
             }
+            const bool hasUV = iu >= 0 && iv >= 0;
+            if (hasUV) m.uv.resize(2 * e.count);
             if (ix < 0 || iy < 0 || iz < 0) fail("vertex element without x, y, z");
             nv = e.count;
             m.verts.resize(3 * nv);
@@ -174,6 +218,10 @@ inline TriangleMeshData loadPly(const std::string& path) {
                 m.verts[3 * v] = static_cast<float>(row[static_cast<size_t>(ix)]);
                 m.verts[3 * v + 1] = static_cast<float>(row[static_cast<size_t>(iy)]);
                 m.verts[3 * v + 2] = static_cast<float>(row[static_cast<size_t>(iz)]);
+                if (hasUV) {
+                    m.uv[2 * v] = static_cast<float>(row[static_cast<size_t>(iu)]);
+                    m.uv[2 * v + 1] = static_cast<float>(row[static_cast<size_t>(iv)]);
+                }
             }
         } else if (e.name == "face") {
             if (e.props.empty() || e.props[0].countType.empty()) fail("face element without an index list");

@@ -15,10 +15,14 @@
 // the active sensor (the scene-file layout of tests/plugin/render_scene.cpp).
 // --write-exr converts raw float32 RGB (row-major, top row first) into an EXR file.
 //
-// Mesh paths (TriangleMesh Path) are relative to the scene file. Components of the reference that
-// the HIP path does not implement (textures other than ConstantTexture, rough glass, anisotropic
-// substrate, multi-primitive area lights, time-sampled transforms) are rejected with a message.
+// Mesh and image paths are relative to the scene file. Colour inputs take ConstantTexture,
+// CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from binary PPM / PGM / PAM with
+// 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of the reference
+// that the HIP path does not implement (textured roughness, rough glass, anisotropic substrate,
+// multi-primitive area lights, time-sampled transforms, other image formats) are rejected with a
+// message.
 #include <array>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -102,9 +106,90 @@ std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const cha
     for (size_t k = 0; k < v.size(); ++k) r[k] = num(v[k]);
     return r;
 }
-std::array<float, 3> colour(const Json& cfg, const char* what) {
+// An 8-bit netpbm image: P5 (grey), P6 (RGB), P7 (PAM, DEPTH 1..4). Three channels are widened
+// to four with alpha 255 (SampledTexture.cpp:143-151 asks stb_image for RGBA then).
+std::shared_ptr<Texture> loadImageTexture(const fs::path& path, uint32_t wrap) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) fail("cannot open image " + path.string());
+    auto token = [&]() {
+        std::string t;
+        char c;
+        while (f.get(c)) {
+            if (c == '#') {
+                std::string skip;
+                std::getline(f, skip);
+                continue;
+            }
+            if (std::isspace(static_cast<unsigned char>(c))) {
+                if (!t.empty()) break;
+                continue;
+            }
+            t += c;
+        }
+        return t;
+    };
+    const std::string magic = token();
+    uint32_t w = 0, h = 0, depth = 0, maxval = 0;
+    if (magic == "P5" || magic == "P6") {
+        w = static_cast<uint32_t>(std::stoul(token())), h = static_cast<uint32_t>(std::stoul(token()));
+        maxval = static_cast<uint32_t>(std::stoul(token()));
+        depth = magic == "P5" ? 1 : 3;
+    } else if (magic == "P7") {
+        for (std::string t = token(); t != "ENDHDR"; t = token()) {
+            if (t.empty()) fail("truncated PAM header in " + path.string());
+            if (t == "WIDTH") w = static_cast<uint32_t>(std::stoul(token()));
+            else if (t == "HEIGHT") h = static_cast<uint32_t>(std::stoul(token()));
+            else if (t == "DEPTH") depth = static_cast<uint32_t>(std::stoul(token()));
+            else if (t == "MAXVAL") maxval = static_cast<uint32_t>(std::stoul(token()));
+            else if (t == "TUPLTYPE") (void)token();
+        }
+    } else {
+        fail("unsupported image format (binary PPM, PGM and PAM are read): " + path.string());
+    }
+    if (maxval != 255 || w == 0 || h == 0 || depth < 1 || depth > 4) fail("unsupported image layout in " + path.string());
+    std::vector<uint8_t> raw(static_cast<size_t>(w) * h * depth);
+    if (!f.read(reinterpret_cast<char*>(raw.data()), static_cast<std::streamsize>(raw.size())))
+        fail("truncated image data in " + path.string());
+    if (depth != 3) return Texture::image(w, h, depth, wrap, std::move(raw));
+    std::vector<uint8_t> rgba(static_cast<size_t>(w) * h * 4);
+    for (size_t i = 0; i < static_cast<size_t>(w) * h; ++i) {
+        std::memcpy(&rgba[4 * i], &raw[3 * i], 3);
+        rgba[4 * i + 3] = 255;
+    }
+    return Texture::image(w, h, 4, wrap, std::move(rgba));
+}
+
+// A colour input: a constant (ConstantTexture) or a texture (CheckBoard, sampled image) that the
+// surface samples at each hit.
+struct ColourInput {
+    std::array<float, 3> value{};
+    std::shared_ptr<Texture> texture;
+};
+ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base) {
+    if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
+    if (!cfg.find("ClassID")) {  // a sampled image (Tracer/Embree/Embree.cpp:1334-1349)
+        const std::string& wrap = cfg.at("WrapMode").string();
+        if (wrap != "Repeat" && wrap != "Mirror") fail("Unrecognized wrap mode " + wrap + cfg.where());
+        const Json& img = cfg.at("Image");
+        requireClass(img, "Piper.BuiltinComponent.SampledTexture", "Image");
+        fs::path p = img.at("Path").string();
+        if (p.is_relative()) p = base / p;
+        auto t = loadImageTexture(p, wrap == "Repeat" ? PG_WRAP_REPEAT : PG_WRAP_MIRROR);
+        if (t->desc.channels < 4) fail(std::string(what) + ": a colour image needs RGB or RGBA samples" + cfg.where());
+        return {{}, t};
+    }
+    const auto [m, c] = classOf(cfg);
+    if (m == "Piper.BuiltinComponent.ProceduralTexture" && c == "CheckBoard") {  // ProceduralTexture.cpp:55-80
+        const auto& b = cfg.at("Black").array();
+        const auto& w = cfg.at("White").array();
+        if (b.size() != w.size()) fail("Channel of black and white must be identical." + cfg.where());
+        if (b.size() != 4) fail(std::string(what) + ": a colour checkerboard needs 4 channels" + cfg.where());
+        std::array<float, 4> black{}, white{};
+        for (size_t k = 0; k < 4; ++k) black[k] = num(b[k]), white[k] = num(w[k]);
+        return {{}, Texture::checkerboard(black, white, num(cfg.at("Scale")), 4)};
+    }
     const auto v = constantTexture(cfg, 4, what);
-    return {v[0], v[1], v[2]};
+    return {{v[0], v[1], v[2]}, nullptr};
 }
 float scalar(const Json& cfg, const char* what) { return constantTexture(cfg, 1, what)[0]; }
 
@@ -190,16 +275,21 @@ private:
     HIPTracer& mTracer;
     fs::path mBase;
 
-    std::shared_ptr<Geometry> meshGeometry(const std::vector<float>& verts, const std::vector<uint32_t>& index) {
+    std::shared_ptr<Geometry> meshGeometry(const std::vector<float>& verts, const std::vector<uint32_t>& index,
+                                           const std::vector<float>& uv = {}) {
         if (index.empty()) fail("a mesh without triangles");
-        const size_t vBytes = verts.size() * 4, iOff = (vBytes + 15) / 16 * 16, size = iOff + index.size() * 4;
+        const size_t vBytes = verts.size() * 4, iOff = (vBytes + 15) / 16 * 16;
+        const size_t uvOff = (iOff + index.size() * 4 + 15) / 16 * 16;
+        const size_t size = uv.empty() ? iOff + index.size() * 4 : uvOff + uv.size() * 4;
         auto buffer = mTracer.getAccelerator().createBuffer(size, 16, [&](std::byte* p) {
             std::memcpy(p, verts.data(), vBytes);
             std::memcpy(p + iOff, index.data(), index.size() * 4);
+            if (!uv.empty()) std::memcpy(p + uvOff, uv.data(), uv.size() * 4);
         });
         GeometryDesc gd;
         gd.desc = TriangleIndexedGeometryDesc{static_cast<uint32_t>(verts.size() / 3), static_cast<uint32_t>(index.size() / 3),
                                               buffer, size, 0, iOff};
+        if (!uv.empty()) gd.desc.texCoords = uvOff;
         return std::make_shared<Geometry>(gd);
     }
     struct PlanePrim {
@@ -217,7 +307,7 @@ private:
             fs::path p = cfg.at("Path").string();
             if (p.is_relative()) p = mBase / p;
             const auto mesh = PiperFrontend::loadMesh(p.string());
-            return meshGeometry(mesh.verts, mesh.index);
+            return meshGeometry(mesh.verts, mesh.index, mesh.uv);
         }
         if (m == "Piper.BuiltinComponent.BasicShapes" && c == "Plane") {  // a plane as a surface: two triangles
             std::vector<float> verts;
@@ -234,23 +324,43 @@ private:
         }
         fail("unsupported geometry " + cfg.at("ClassID").string() + cfg.where());
     }
-    static std::shared_ptr<Surface> surface(const Json& cfg) {  // Surface/Classic (Classic.cpp:207-237)
+    std::shared_ptr<Surface> surface(const Json& cfg) const {  // Surface/Classic (Classic.cpp:207-237)
         const auto [m, c] = classOf(cfg);
         if (m != "Piper.BuiltinComponent.ClassicSurface") fail("unsupported surface " + cfg.at("ClassID").string() + cfg.where());
-        if (c == "Matte") return Surface::matte(colour(cfg.at("Diffuse"), "Diffuse"), scalar(cfg.at("Roughness"), "Roughness"));
-        if (c == "Plastic")
-            return Surface::plastic(colour(cfg.at("Diffuse"), "Diffuse"), colour(cfg.at("Specular"), "Specular"),
-                                    scalar(cfg.at("Roughness"), "Roughness"));
-        if (c == "Mirror") return Surface::mirror(colour(cfg.at("Reflection"), "Reflection"));
-        if (c == "Glass")
-            return Surface::glass(colour(cfg.at("Reflection"), "Reflection"), colour(cfg.at("Transmission"), "Transmission"),
-                                  scalar(cfg.at("RoughnessX"), "RoughnessX"), scalar(cfg.at("RoughnessY"), "RoughnessY"));
-        if (c == "Substrate")
-            return Surface::substrate(colour(cfg.at("Diffuse"), "Diffuse"), colour(cfg.at("Specular"), "Specular"),
-                                      scalar(cfg.at("RoughnessX"), "RoughnessX"), scalar(cfg.at("RoughnessY"), "RoughnessY"));
+        auto in = [&](const char* key) { return colourInput(cfg.at(key), key, mBase); };
+        auto bind = [](std::shared_ptr<Surface> s, const ColourInput& a, const ColourInput& b) {
+            s->colorTexture = a.texture;
+            s->specularTexture = b.texture;
+            return s;
+        };
+        const ColourInput none;
+        if (c == "Matte") {
+            const auto d = in("Diffuse");
+            return bind(Surface::matte(d.value, scalar(cfg.at("Roughness"), "Roughness")), d, none);
+        }
+        if (c == "Plastic") {
+            const auto d = in("Diffuse"), s = in("Specular");
+            return bind(Surface::plastic(d.value, s.value, scalar(cfg.at("Roughness"), "Roughness")), d, s);
+        }
+        if (c == "Mirror") {
+            const auto r = in("Reflection");
+            return bind(Surface::mirror(r.value), r, none);
+        }
+        if (c == "Glass") {
+            const auto r = in("Reflection"), t = in("Transmission");
+            return bind(Surface::glass(r.value, t.value, scalar(cfg.at("RoughnessX"), "RoughnessX"),
+                                       scalar(cfg.at("RoughnessY"), "RoughnessY")), r, t);
+        }
+        if (c == "Substrate") {
+            const auto d = in("Diffuse"), s = in("Specular");
+            return bind(Surface::substrate(d.value, s.value, scalar(cfg.at("RoughnessX"), "RoughnessX"),
+                                           scalar(cfg.at("RoughnessY"), "RoughnessY")), d, s);
+        }
         if (c == "BlackBody") return Surface::matte({0.0f, 0.0f, 0.0f});  // no lobe: the path ends here
-        if (c == "Conductor")  // the GGX conductor this path adds (not a reference class): F0, Roughness
-            return Surface::conductor(colour(cfg.at("F0"), "F0"), scalar(cfg.at("Roughness"), "Roughness"));
+        if (c == "Conductor") {  // the GGX conductor this path adds (not a reference class): F0, Roughness
+            const auto f = in("F0");
+            return bind(Surface::conductor(f.value, scalar(cfg.at("Roughness"), "Roughness")), f, none);
+        }
         fail("unsupported surface class " + c + cfg.where());
     }
     static std::shared_ptr<Object> object(const Json& cfg) {  // an Instance with a ClassID: a light or a sensor
@@ -367,9 +477,27 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     const TransformSRT id = identitySRT();
     for (uint32_t o = 0; o < no; ++o) put(f, reinterpret_cast<const float*>(&id), 16);
     const uint32_t np = static_cast<uint32_t>(e.points.size());
-    if (np) {
+    if (np || !e.textures.empty()) {  // optional tail: point lights, then textures
         put(f, &np, 1);
         for (const pg_point_light& p : e.points) put(f, p.position, 3), put(f, p.intensity, 3);
+    }
+    if (!e.textures.empty()) {
+        // texcoords (float2 per vertex, zeros when no mesh had any), the textures (type, channels,
+        // black, white, scale, width, height, wrap, then the texels), the bindings per material
+        std::vector<float> uv = e.texcoords;
+        uv.resize(2 * static_cast<size_t>(nv), 0.0f);
+        put(f, uv.data(), uv.size());
+        const uint32_t nx = static_cast<uint32_t>(e.textures.size());
+        put(f, &nx, 1);
+        for (const auto& t : e.textures) {
+            const pg_texture& d = t->desc;
+            const uint32_t head4[2] = {d.type, d.channels};
+            put(f, head4, 2), put(f, d.black, 4), put(f, d.white, 4), put(f, &d.scale, 1);
+            const uint32_t dims[3] = {d.width, d.height, d.wrap};
+            put(f, dims, 3);
+            put(f, t->texels.data(), t->texels.size());
+        }
+        for (const pg_material_textures& b : e.materialTextures) put(f, &b.albedo, 1), put(f, &b.reserved, 1);
     }
     if (!f) fail("cannot write " + out.string());
 }

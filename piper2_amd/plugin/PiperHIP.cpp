@@ -109,6 +109,7 @@ private:
 struct MeshAccel final : AccelerationStructure {
     std::vector<float> verts;     // float3 per vertex, with GeometryDesc::transform applied
     std::vector<uint32_t> index;  // uint3 per triangle
+    std::vector<float> uv;        // float2 per vertex, or empty
 };
 struct HIPGSMInstance final : GSMInstance {
     std::shared_ptr<MeshAccel> mesh;
@@ -131,6 +132,9 @@ struct FlatScene {
     std::vector<pg_material> materials;
     std::vector<pg_quad> quads;
     std::vector<pg_point_light> points;
+    std::vector<float> texcoords;  // float2 per vertex once any mesh has them (zeros elsewhere)
+    std::vector<std::shared_ptr<Texture>> textures;
+    std::vector<pg_material_textures> materialTextures;
     float env[3] = {0, 0, 0};
     std::vector<std::pair<const Node*, Mat34>> sensors;  // leaf nodes of sensors and their transform
 };
@@ -149,6 +153,14 @@ uint32_t materialOf(FlatScene& f, const Surface& s) {
     m.alpha = s.alpha;
     for (int k = 0; k < 3; ++k) m.reserved[k] = s.specular[k];
     f.materials.push_back(m);
+    auto texIndex = [&](const std::shared_ptr<Texture>& t) -> uint32_t {
+        if (!t) return PG_TEX_NONE;
+        for (size_t i = 0; i < f.textures.size(); ++i)
+            if (f.textures[i] == t) return static_cast<uint32_t>(i);
+        f.textures.push_back(t);
+        return static_cast<uint32_t>(f.textures.size() - 1);
+    };
+    f.materialTextures.push_back({texIndex(s.colorTexture), texIndex(s.specularTexture)});
     return static_cast<uint32_t>(f.materials.size() - 1);
 }
 
@@ -164,6 +176,11 @@ void flatten(const std::shared_ptr<Node>& node, const Mat34& toWorld, FlatScene&
         for (size_t v = 0; v < inst->mesh->verts.size(); v += 3) {
             const auto p = applyPoint(toWorld, &inst->mesh->verts[v]);
             f.verts.insert(f.verts.end(), p.begin(), p.end());
+        }
+        if (!inst->mesh->uv.empty() || !f.texcoords.empty()) {
+            f.texcoords.resize(2 * static_cast<size_t>(base), 0.0f);
+            if (inst->mesh->uv.empty()) f.texcoords.resize(f.verts.size() / 3 * 2, 0.0f);
+            else f.texcoords.insert(f.texcoords.end(), inst->mesh->uv.begin(), inst->mesh->uv.end());
         }
         const uint32_t obj = static_cast<uint32_t>(f.objectMaterial.size());
         f.objectMaterial.push_back(materialOf(f, *inst->surface));
@@ -341,6 +358,17 @@ public:
         d.num_quads = static_cast<uint32_t>(mFlat.quads.size());
         d.point_lights = mFlat.points.empty() ? nullptr : mFlat.points.data();
         d.num_point_lights = static_cast<uint32_t>(mFlat.points.size());
+        std::vector<pg_texture> textures;
+        for (const auto& t : mFlat.textures) {
+            textures.push_back(t->desc);
+            textures.back().texels = t->texels.empty() ? nullptr : t->texels.data();
+        }
+        if (!textures.empty()) {
+            d.textures = textures.data();
+            d.num_textures = static_cast<uint32_t>(textures.size());
+            d.material_textures = mFlat.materialTextures.data();
+            d.texcoords = mFlat.texcoords.empty() ? nullptr : mFlat.texcoords.data();
+        }
         for (int k = 0; k < 3; ++k) {
             d.camera.position[k] = (*toWorld)[4 * k + 3];
             d.camera.look_at[k] = cam.lookAt[k];
@@ -430,6 +458,26 @@ std::shared_ptr<Surface> Surface::plastic(std::array<float, 3> diffuse, std::arr
     auto s = std::make_shared<Surface>();
     s->kind = Kind::Plastic, s->color = diffuse, s->alpha = roughness, s->specular = specular;
     return s;
+}
+std::shared_ptr<Texture> Texture::checkerboard(std::array<float, 4> black, std::array<float, 4> white, float scale,
+                                               uint32_t channels) {
+    if (channels != 1 && channels != 2 && channels != 4) raise("Texture: channels must be 1, 2 or 4");
+    auto t = std::make_shared<Texture>();
+    t->desc.type = PG_TEX_CHECKERBOARD, t->desc.channels = channels, t->desc.scale = scale;
+    std::memcpy(t->desc.black, black.data(), 16), std::memcpy(t->desc.white, white.data(), 16);
+    return t;
+}
+std::shared_ptr<Texture> Texture::image(uint32_t width, uint32_t height, uint32_t channels, uint32_t wrap,
+                                        std::vector<uint8_t> texels) {
+    if (channels != 1 && channels != 2 && channels != 4) raise("Texture: channels must be 1, 2 or 4");
+    if (wrap != PG_WRAP_REPEAT && wrap != PG_WRAP_MIRROR) raise("Texture: unknown wrap mode");
+    if (width == 0 || height == 0 || texels.size() != static_cast<size_t>(width) * height * channels)
+        raise("Texture: texel count does not match width x height x channels");
+    auto t = std::make_shared<Texture>();
+    t->desc.type = PG_TEX_IMAGE, t->desc.channels = channels, t->desc.width = width, t->desc.height = height;
+    t->desc.wrap = wrap;
+    t->texels = std::move(texels);
+    return t;
 }
 std::shared_ptr<Surface> Surface::mirror(std::array<float, 3> reflection) {
     auto s = std::make_shared<Surface>();
@@ -526,6 +574,12 @@ std::shared_ptr<AccelerationStructure> HIPTracer::buildAcceleration(const Geomet
     std::memcpy(mesh->index.data(), buf->bytes.data() + g.index, static_cast<size_t>(g.triCount) * 12);
     for (uint32_t i : mesh->index)
         if (i >= g.vertCount) raise("buildAcceleration: triangle index out of range");
+    if (g.texCoords != SIZE_MAX) {
+        if (g.texCoords + static_cast<size_t>(g.vertCount) * 8 > g.bufferSize)
+            raise("buildAcceleration: texture coordinates outside the buffer");
+        mesh->uv.resize(static_cast<size_t>(g.vertCount) * 2);
+        std::memcpy(mesh->uv.data(), buf->bytes.data() + g.texCoords, static_cast<size_t>(g.vertCount) * 8);
+    }
     if (desc.transform) {
         for (size_t v = 0; v < mesh->verts.size(); v += 3) {
             const auto p = applyPoint(*desc.transform, &mesh->verts[v]);
@@ -621,6 +675,11 @@ SceneExport exportScene(const std::shared_ptr<Node>& scene) {
     e.points = std::move(flat.points);
     for (int k = 0; k < 3; ++k) e.env[k] = flat.env[k];
     for (const auto& [n, m] : flat.sensors) e.sensors.push_back({n, m});
+    if (!flat.textures.empty()) {
+        e.texcoords = std::move(flat.texcoords);
+        e.textures = std::move(flat.textures);
+        e.materialTextures = std::move(flat.materialTextures);
+    }
     return e;
 }
 

@@ -140,7 +140,8 @@ struct TriangleIndexedGeometryDesc final {  // Tracer.hpp:51-63
     size_t bufferSize;
     size_t vertices;  // byte offset of float3 vertices
     size_t index;     // byte offset of uint3 indices
-    size_t texCoords = SIZE_MAX, normal = SIZE_MAX, tangent = SIZE_MAX;  // optional, unused by the path
+    size_t texCoords = SIZE_MAX;  // optional byte offset of float2 texture coordinates (one per vertex)
+    size_t normal = SIZE_MAX, tangent = SIZE_MAX;  // optional, unused by the path
 };
 struct GeometryDesc final {  // Tracer.hpp:70-73; transform: row-major 3x4 local -> world
     std::optional<std::array<float, 12>> transform;
@@ -160,6 +161,17 @@ private:
 // Surface/Classic: Matte (MatteData, Classic/Shared.hpp:51-54 with a constant texture), Plastic
 // (PlasticData, Classic/Shared.hpp:56-60 and Classic.cpp:123-147: Diffuse, Specular, Roughness as
 // constant textures) and the GGX conductor this path adds (alpha = roughness^2, Schlick F0).
+// Texture/ProceduralTexture CheckBoard (Black, White, Scale) and Texture/SampledTexture (an 8-bit
+// image with Repeat or Mirror wrap): bound to a surface's colour inputs.
+class Texture : public Object {
+public:
+    static std::shared_ptr<Texture> checkerboard(std::array<float, 4> black, std::array<float, 4> white, float scale,
+                                                 uint32_t channels = 4);
+    static std::shared_ptr<Texture> image(uint32_t width, uint32_t height, uint32_t channels, uint32_t wrap,
+                                          std::vector<uint8_t> texels);
+    pg_texture desc{};             // texels points into `texels`
+    std::vector<uint8_t> texels;
+};
 class Surface : public Object {
 public:
     enum class Kind { Matte, Conductor, Plastic, Mirror, Glass, Substrate };
@@ -181,6 +193,7 @@ public:
                                          // mirror / glass reflection
     float alpha;                         // matte roughness, conductor GGX alpha, plastic / substrate roughness
     std::array<float, 3> specular{};     // plastic / substrate specular, glass transmission
+    std::shared_ptr<Texture> colorTexture, specularTexture;  // replace color / specular when set
 };
 class Medium : public Object {};
 // Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V), the environment
@@ -336,6 +349,9 @@ struct SceneExport {
     std::vector<pg_point_light> points;
     std::array<float, 3> env{};
     std::vector<std::pair<const Node*, std::array<float, 12>>> sensors;
+    std::vector<float> texcoords;                        // float2 per vertex, or empty
+    std::vector<std::shared_ptr<Texture>> textures;
+    std::vector<pg_material_textures> materialTextures;  // per material (empty without textures)
 };
 SceneExport exportScene(const std::shared_ptr<Node>& scene);
 

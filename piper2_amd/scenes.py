@@ -150,7 +150,7 @@ class _Builder:
 
 
 def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
-            specular=False, point_lights=False):
+            specular=False, point_lights=False, textured=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
@@ -159,7 +159,10 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     diffuse colour) and adds a smooth glass sphere (outward winding, so rays enter through front
     faces and leave through back faces) and a mirror panel before the back wall, tilted to show the
     ceiling; point_lights=True adds two point lights (lights 2 and 3 after the environment and the
-    ceiling quad). The light and the rest of the geometry stay as in the plain scene."""
+    ceiling quad); textured=True adds texture coordinates (a planar projection of every vertex),
+    a checkerboard on the back wall, 8-bit images on the floor (repeat) and the left wall (mirror)
+    and, with plastic=True, a checkerboard specular colour on the first box. The light and the rest
+    of the geometry stay as in the plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -213,6 +216,28 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
                   + ("_specular" if specular else "") + ("_points" if point_lights else ""))
     if point_lights:
         sc["point_lights"] = np.array([[0.45, 0.6, 0.2, 1.5, 1.4, 1.2], [-0.55, -0.3, 0.45, 0.6, 0.7, 1.0]], np.float32)
+    if textured:
+        v = sc["vertices"].astype(np.float64)
+        sc["texcoords"] = np.ascontiguousarray(np.stack([0.7 * v[:, 0] + 0.3 * v[:, 2] + 1.1,
+                                                         0.8 * v[:, 1] - 0.2 * v[:, 2] + 0.9], 1).astype(np.float32))
+        trng = np.random.default_rng(seed + 2000)
+        none = 0xFFFFFFFF
+        sc["textures"] = [
+            {"type": 1, "channels": 4, "black": [0.1, 0.1, 0.15, 1.0], "white": [0.8, 0.75, 0.6, 1.0], "scale": 3.0},
+            {"type": 2, "channels": 4, "width": 13, "height": 9, "wrap": 0,
+             "texels": trng.integers(0, 256, (9, 13, 4), dtype=np.uint8)},
+            {"type": 2, "channels": 4, "width": 7, "height": 5, "wrap": 1,
+             "texels": trng.integers(0, 256, (5, 7, 4), dtype=np.uint8)},
+            {"type": 1, "channels": 4, "black": [0.05, 0.05, 0.05, 1.0], "white": [0.9, 0.2, 0.2, 1.0], "scale": 5.0},
+        ]
+        mt = np.full((len(sc["materials"]), 2), none, np.uint32)
+        mt[sc["object_material"][2], 0] = 0  # back wall
+        mt[sc["object_material"][0], 0] = 1  # floor
+        mt[sc["object_material"][3], 0] = 2  # left wall
+        if plastic:
+            mt[sc["object_material"][5], 1] = 3  # the first box's specular colour
+        sc["material_textures"] = mt
+        sc["name"] += "_textured"
     return sc
 
 

@@ -27,6 +27,10 @@ def lib():
                                       ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]
         L.or_scene_destroy.argtypes = [_VP]
         L.or_scene_set_points.argtypes = [_VP, _VP, ctypes.c_uint32]
+        L.or_scene_set_textures.argtypes = [_VP, _VP, _VP]
+        L.or_scene_add_texture.restype = ctypes.c_uint32
+        L.or_scene_add_texture.argtypes = [_VP, _VP, _VP, _VP]
+        L.or_texture_sample.argtypes = [_VP, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, _VP]
         L.or_trace.argtypes = [_VP, _VP, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int, _VP]
         L.or_occluded.argtypes = [_VP, _VP, ctypes.c_uint32, _VP]
         L.or_camera_rays.argtypes = [_VP, _VP, _VP, _VP, ctypes.c_uint32, _VP, _VP]
@@ -73,6 +77,27 @@ class OracleScene:
         if pts is not None and len(pts):
             self._pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
             lib().or_scene_set_points(self.h, _p(self._pts), self._pts.shape[0])
+        if sc.get("textures"):
+            self._keep = []
+            for t in sc["textures"]:
+                it = np.array([t["type"], t["channels"], t.get("width", 0), t.get("height", 0), t.get("wrap", 0)],
+                              np.uint32)
+                ft = np.array([*t.get("black", [0] * 4), *t.get("white", [0] * 4), t.get("scale", 0.0)], np.float32)
+                tx = np.ascontiguousarray(t["texels"], np.uint8) if t.get("texels") is not None else None
+                self._keep += [it, ft, tx]
+                lib().or_scene_add_texture(self.h, _p(it), _p(ft), _p(tx))
+            triuv = None
+            if sc.get("texcoords") is not None:
+                uv = np.asarray(sc["texcoords"], np.float32).reshape(-1, 2)
+                triuv = np.ascontiguousarray(uv[sc["indices"].astype(np.int64)].reshape(-1, 6))
+            mt = np.ascontiguousarray(sc["material_textures"], np.uint32).reshape(-1, 2)
+            self._keep += [triuv, mt]
+            lib().or_scene_set_textures(self.h, _p(triuv), _p(mt))
+
+    def texture_sample(self, tex, u, v):
+        out = np.zeros(4, np.float32)
+        lib().or_texture_sample(self.h, int(tex), np.float32(u), np.float32(v), _p(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -41,17 +41,37 @@ def surface_config(m):
     raise ValueError(f"material type {t} has no reference class")
 
 
-def write_obj(path, verts, tris, negative=False):
+def write_obj(path, verts, tris, negative=False, uv=None):
     with open(path, "w") as f:
         f.write("# written by the front-end tests\n")
         for v in verts:
             f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in v))
+        for t in (uv if uv is not None else []):
+            f.write("vt %.9g %.9g\n" % tuple(float(x) for x in t))
         n = len(verts)
         for t in tris:
-            if negative:
-                f.write("f %d/1/1 %d//2 %d\n" % tuple(int(i) - n for i in t))
+            if uv is not None:  # the texcoord list parallels the vertex list
+                f.write("f %d/%d %d/%d %d/%d\n" % tuple(int(i) + 1 for i in t for _ in range(2)))
+            elif negative:
+                f.write("f %d//1 %d %d//2\n" % tuple(int(i) - n for i in t))  # normals named, not read
             else:
                 f.write("f %d %d %d\n" % tuple(int(i) + 1 for i in t))
+
+
+def write_pam(path, texels):
+    h, w, c = texels.shape
+    with open(path, "wb") as f:
+        f.write(f"P7\nWIDTH {w}\nHEIGHT {h}\nDEPTH {c}\nMAXVAL 255\nTUPLTYPE RGB_ALPHA\nENDHDR\n".encode())
+        f.write(np.ascontiguousarray(texels, np.uint8).tobytes())
+
+
+def texture_config(t, folder, name):
+    if t["type"] == 1:
+        return {"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": list(t["black"]),
+                "White": list(t["white"]), "Scale": t["scale"]}
+    write_pam(os.path.join(folder, name), t["texels"])
+    return {"WrapMode": "Repeat" if t["wrap"] == 0 else "Mirror",
+            "Image": {"ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": name}}
 
 
 def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transforms=None):
@@ -64,10 +84,19 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
         tri = sc["indices"][sc["tri_object"] == o]
         vid, inv = np.unique(tri, return_inverse=True)
         name = f"object{o}.obj"
-        write_obj(os.path.join(folder, name), sc["vertices"][vid], inv.reshape(-1, 3), negative=(o % 2 == 1))
+        uv = sc["texcoords"][vid] if sc.get("texcoords") is not None else None
+        write_obj(os.path.join(folder, name), sc["vertices"][vid], inv.reshape(-1, 3), negative=(o % 2 == 1), uv=uv)
+        surf = surface_config(sc["materials"][mat])
+        if sc.get("material_textures") is not None:
+            for slot, key in enumerate(("albedo", "spec")):
+                t = int(sc["material_textures"][mat][slot])
+                if t != 0xFFFFFFFF:
+                    keys = {"albedo": ("Diffuse", "Reflection"), "spec": ("Specular", "Transmission")}[key]
+                    k = next(k for k in keys if k in surf)
+                    surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam")
         inst = {"Type": "Instance", "Geometry": {"ClassID": "Piper.BuiltinComponent.TriangleMesh.TriangleMesh",
                                                  "Path": name},
-                "Surface": surface_config(sc["materials"][mat])}
+                "Surface": surf}
         if transforms and o in transforms:
             inst = {"Type": "Group", "Children": [dict(inst, Transform=transforms[o])]}
         children.append(inst)
@@ -143,6 +172,25 @@ def read_pgsc(path):
         npt = struct.unpack("<I", data[off:off + 4])[0]
         off += 4
         take("point_lights", np.float32, 6 * npt, (npt, 6))
+    if off < len(data):  # textures
+        take("texcoords", np.float32, 2 * nv, (nv, 2))
+        ntex = struct.unpack("<I", data[off:off + 4])[0]
+        off += 4
+        texs = []
+        for _ in range(ntex):
+            typ, ch = struct.unpack("<II", data[off:off + 8])
+            fl = np.frombuffer(data, np.float32, 9, off + 8)
+            wd, ht, wrap = struct.unpack("<III", data[off + 44:off + 56])
+            off += 56
+            t = {"type": typ, "channels": ch, "black": fl[0:4].tolist(), "white": fl[4:8].tolist(),
+                 "scale": float(fl[8]), "width": wd, "height": ht, "wrap": wrap}
+            if typ == 2:
+                n = wd * ht * ch
+                t["texels"] = np.frombuffer(data, np.uint8, n, off).reshape(ht, wd, ch).copy()
+                off += n
+            texs.append(t)
+        out["textures"] = texs
+        take("material_textures", np.uint32, 2 * nm, (nm, 2))
     assert off == len(data)
     out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
     return out

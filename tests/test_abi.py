@@ -25,6 +25,30 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(pg.SIGNATURES), set(names) ^ set(pg.SIGNATURES)
 
 
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors have the header's sizes and field offsets (a field the Python side lacks
+    would make the library read past the caller's struct)."""
+    import subprocess
+    pairs = [("pg_scene_desc", pg.SceneDesc), ("pg_texture", pg.Texture), ("pg_material", pg.Material),
+             ("pg_quad", pg.Quad), ("pg_camera", pg.Camera), ("pg_render_params", pg.RenderParams),
+             ("pg_stats", pg.Stats)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "piper_hip.h"', "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, cls in pairs:
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
+
+
 def test_sobol_matrices_from_library_match_golden():
     g = np.load(os.path.join(ROOT, "tests", "golden", "sobol_columns.npz"))
     assert np.array_equal(pg.sobol_matrices(), g["columns"])

@@ -41,6 +41,29 @@ def test_export_reproduces_generated_scene(tmp_path):
     assert (e["width"], e["height"], e["spp"], e["max_depth"], e["min_rr"]) == (64, 48, 8, 5, 5)
 
 
+def test_textured_scene_export(tmp_path):
+    """CheckBoard and sampled-image colour inputs (PAM files, Repeat and Mirror) and OBJ texture
+    coordinates reach the flattened scene: the same corner coordinates and, per material input, the
+    same texture (the plugin numbers textures in first-use order)."""
+    sc = scenes.cornell(width=32, height=32, spp=4, plastic=True, textured=True)
+    e = export(sc, tmp_path)
+    assert np.array_equal(e["vertices"][e["indices"]], sc["vertices"][sc["indices"]])
+    assert np.array_equal(e["texcoords"][e["indices"]], sc["texcoords"][sc["indices"]])
+    for m in range(len(sc["materials"])):
+        for slot in range(2):
+            a, b = int(sc["material_textures"][m, slot]), int(e["material_textures"][m, slot])
+            assert (a == 0xFFFFFFFF) == (b == 0xFFFFFFFF), (m, slot)
+            if a == 0xFFFFFFFF:
+                continue
+            ta, tb = sc["textures"][a], e["textures"][b]
+            assert ta["type"] == tb["type"] and ta["channels"] == tb["channels"]
+            if ta["type"] == 1:
+                assert np.allclose(ta["black"], tb["black"]) and np.allclose(ta["white"], tb["white"])
+                assert ta["scale"] == tb["scale"]
+            else:
+                assert np.array_equal(ta["texels"], tb["texels"]) and ta["wrap"] == tb["wrap"]
+
+
 def test_transform_srt_matches_scipy(tmp_path):
     """Translation, per-axis Scale and Euler Rotation [pitch, yaw, roll]: the reference's quaternion
     is scipy's intrinsic Y-X-Z rotation (yaw, then pitch, then roll); local -> world applies the
@@ -130,8 +153,10 @@ def test_exr_layout_and_half_rounding(tmp_path):
 
 @pytest.mark.parametrize("mutate,message", [
     (lambda s: s.update(Integrator={"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator"}), "MaxTraceDepth"),
-    (lambda s: s["Scene"]["Children"][0]["Surface"].update(Diffuse={"ClassID": "Piper.BuiltinComponent."
-                                                                    "ProceduralTexture.CheckBoard"}), "ConstantTexture"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(Roughness={"ClassID": "Piper.BuiltinComponent."
+                                                                      "ProceduralTexture.CheckBoard"}), "ConstantTexture"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(Diffuse={"WrapMode": "Repeat", "Image": {
+        "ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": "picture.png"}}), "cannot open image"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(
         Diffuse={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [1, 1, 1]}),
      "unsupported channel count 3"),

@@ -37,10 +37,12 @@ def direct_rgb(dev, e, film_filter=0, alpha=0.0):
     return rgb.astype(np.float16)
 
 
-@pytest.mark.parametrize("filt", [None, {"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter",
-                                         "FilterType": "gaussian", "Alpha": 2.0}])
-def test_json_scene_renders_like_pg_render(dev, tmp_path, filt):
-    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True)
+@pytest.mark.parametrize("filt,textured", [
+    (None, False), (None, True),
+    ({"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter", "FilterType": "gaussian", "Alpha": 2.0}, False)])
+def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured):
+    sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True,
+                        textured=textured)
     opts = write_scene_json(sc, str(tmp_path), filter_cfg=filt)
     scene_path = tmp_path / "scene.json"
     scene = json.load(open(scene_path))

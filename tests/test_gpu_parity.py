@@ -202,6 +202,12 @@ def test_point_lights_paths_match_oracle(dev):
     _path_parity(sc, dev, flags=flags, spp=4)
 
 
+def test_textured_paths_match_oracle(dev):
+    # checkerboard and 8-bit image textures (repeat and mirror) on walls and a box's specular colour
+    sc = scenes.cornell(width=128, height=128, spp=16, textured=True, plastic=True, specular=True)
+    _path_parity(sc, dev)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

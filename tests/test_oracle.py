@@ -440,6 +440,63 @@ def test_lossless_specular_object_vanishes_in_a_white_furnace(kind):
     assert np.all(pl[~escaped] == 0.0)
 
 
+def test_checkerboard_and_image_textures():
+    """Checkerboard: at cell centres white exactly where floor(u s) + floor(v s) is even. Image:
+    at texel centres the bilinear lookup returns that texel x 0.003921568 (the 1/255 of the
+    reference), halfway between two texels their mean, and Repeat is periodic in the texture
+    coordinate."""
+    sc = scenes.cornell(width=8, height=8, spp=1, textured=True)
+    o = OracleScene(sc)
+    chk = sc["textures"][0]
+    s = chk["scale"]
+    for i in range(-5, 6):
+        for j in range(-4, 5):
+            u, v = (i + 0.5) / s, (j + 0.5) / s
+            want = chk["white"] if (i + j) % 2 == 0 else chk["black"]
+            assert np.array_equal(o.texture_sample(0, u, v), np.float32(want)), (i, j)
+    for k in (1, 2):
+        img = sc["textures"][k]
+        tex, w, h = img["texels"].astype(np.float32), img["width"], img["height"]
+        # Mirror folds with |IEEE remainder| (SampledTexture/Kernel.cpp:28-32), which reflects about
+        # the image's middle: texel x is read as min(x, size - x)
+        fold = (lambda i, n: i) if img["wrap"] == 0 else (lambda i, n: min(i, n - i))
+        for y in range(h):
+            for x in range(w):
+                got = o.texture_sample(k, (x + 0.5) / w, (y + 0.5) / h)
+                want = tex[fold(y, h), fold(x, w)] * np.float32(0.003921568)
+                # (x + 1/2) / w rounds to float, so the weights are 1 and 0 only to rounding
+                assert np.allclose(got, want, rtol=1e-5, atol=1e-6), (k, x, y)
+        x, y = 1, 1
+        mid = o.texture_sample(k, (x + 1.0) / w, (y + 0.5) / h)
+        assert np.allclose(mid, 0.5 * (tex[y, x] + tex[y, x + 1]) * 0.003921568, rtol=1e-5, atol=1e-6)
+    rng = np.random.default_rng(1)
+    img = sc["textures"][1]
+    for u, v in rng.uniform(0.05, 0.95, (50, 2)):
+        assert np.allclose(o.texture_sample(1, u, v), o.texture_sample(1, u + 1.0, v - 2.0), atol=2e-5)
+
+
+def test_uniform_checkerboard_renders_as_its_constant():
+    """A checkerboard whose two colours equal a material's constant albedo renders the same film
+    bit for bit (texture coordinates interpolated at every hit, the lookup replacing the constant)."""
+    sc = scenes.cornell(width=32, height=32, spp=8, textured=True, plastic=True)
+    plain = dict(sc)
+    for k in ("textures", "material_textures", "texcoords"):
+        plain.pop(k)
+    mt = sc["material_textures"]
+    texs = []
+    for i, t in enumerate(sc["textures"]):
+        mats = np.nonzero(mt == i)
+        m, slot = int(mats[0][0]), int(mats[1][0])
+        col = sc["materials"][m, 1:4] if slot == 0 else sc["materials"][m, 5:8]
+        texs.append({"type": 1, "channels": 4, "black": [*col, 1.0], "white": [*col, 1.0], "scale": 3.0})
+    uni = dict(sc, textures=texs)
+    a = OracleScene(plain).render(threads=4)["film"]
+    b = OracleScene(uni).render(threads=4)["film"]
+    c = OracleScene(sc).render(threads=4)["film"]
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
 def test_plastic_without_specular_renders_as_lambert():
     """A scene whose Lambert materials are restated as plastic with a black specular colour renders
     the same film, bit for bit."""
