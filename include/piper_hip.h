@@ -49,8 +49,9 @@ extern "C" {
                                GGX with dielectric Fresnel (eta 1.46), specular = reserved[0..2],
                                roughness = alpha (remapRoughness applied, :378-381) */
 #define PG_MAT_MIRROR 3u    /* mirror (Classic/Kernel.cpp:663-692): perfect reflection scaled by albedo */
-#define PG_MAT_GLASS 4u     /* smooth glass (glassInit with roughness 0, FresnelSpecular :499-541, eta 1.5):
-                               reflection = albedo, transmission = reserved[0..2]; alpha must be 0 */
+#define PG_MAT_GLASS 4u     /* glass (glassInit :597-656, eta 1.5): reflection = albedo, transmission =
+                               reserved[0..2], roughness = alpha: 0 gives smooth glass (FresnelSpecular
+                               :499-541), > 0 microfacet reflection + transmission (remapRoughness) */
 #define PG_MAT_SUBSTRATE 5u /* substrate (FresnelBlend, Classic/Kernel.cpp:694-779): diffuse = albedo,
                                specular = reserved[0..2], roughness = alpha (remapRoughness applied) */
 

@@ -613,7 +613,8 @@ float fresnelDielectric(float cosI, float etaI, float etaT) {
 // One BxDF of a material. Kinds: the Lambert lobe (LambertianReflection, Kernel.cpp:81-92), the
 // GGX lobe with a Schlick conductor Fresnel, and the GGX lobe with the dielectric Fresnel of
 // plasticInit (Kernel.cpp:441-471: eta 1.46 outside-in on the front face, swapped on the back).
-enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric, kLobeOrenNayar, kLobeMirror, kLobeGlass, kLobeBlend };
+enum LobeKind : uint32_t { kLobeLambert, kLobeConductor, kLobeDielectric, kLobeOrenNayar, kLobeMirror, kLobeGlass, kLobeBlend,
+                           kLobeGlassR, kLobeGlassT };
 struct Lobe {
     LobeKind kind;
     V3 scale;
@@ -632,8 +633,11 @@ Lobes lobesOf(const Material& m) {
         r.l[r.n++] = {kLobeConductor, m.albedo};
     } else if (m.type == kMatMirror) {  // mirrorInit, Kernel.cpp:667-676
         r.l[r.n++] = {kLobeMirror, m.albedo};
-    } else if (m.type == kMatGlass) {  // glassInit with zero roughness: one FresnelSpecular lobe, Kernel.cpp:608-640
+    } else if (m.type == kMatGlass && m.alpha == 0.0f) {  // glassInit, zero roughness: FresnelSpecular, Kernel.cpp:608-640
         if (valid(m.albedo) || valid(m.spec)) r.l[r.n++] = {kLobeGlass, m.albedo};
+    } else if (m.type == kMatGlass) {  // rough: microfacet reflection and transmission, Kernel.cpp:641-655
+        if (valid(m.albedo)) r.l[r.n++] = {kLobeGlassR, m.albedo};
+        if (valid(m.spec)) r.l[r.n++] = {kLobeGlassT, m.spec};
     } else if (m.type == kMatSubstrate) {  // substrateInit, Kernel.cpp:757-777
         if (valid(m.albedo) || valid(m.spec)) r.l[r.n++] = {kLobeBlend, m.albedo};
     } else {
@@ -678,8 +682,47 @@ float blendPdf(const Material& m, V3 wo, V3 wi) {
     const V3 wh = normalize(wo + wi);
     return 0.5f * (std::fabs(wi.z) * kInvPi + ggxD(wh, m.alpha) / (4.0f * std::fabs(wo.z) * (1.0f + ggxLambda(wo, m.alpha))));
 }
+// Rough glass. Reflection: the GGX lobe with the dielectric Fresnel of the face (eta 1.5).
+// Transmission: Walter et al. 2007's microfacet BTDF with the generalized half vector
+// h = -(eta_i wo + eta_t wi) (made to face +z; zero unless wo.h > 0 > wi.h, their chi+ terms),
+// carrying radiance, so scaled by (eta_i / eta_t)^2:
+// f = |wi.h| |wo.h| (1 - F(wo.h)) D(h) G / (|wi.z| |wo.z| (wo.h + (eta_t/eta_i) wi.h)^2), and a pdf
+// of the visible normal h times the Jacobian (eta_t/eta_i)^2 |wi.h| / (wo.h + (eta_t/eta_i) wi.h)^2.
+float glassEta(const Material& m) { return m.front ? 1.5f : 1.0f / 1.5f; }  // eta_t / eta_i
+V3 glassHalf(const Material& m, V3 wo, V3 wi) {
+    V3 h = normalize(wo + wi * glassEta(m));
+    return h.z < 0.0f ? -h : h;
+}
+float glassTPdf(const Material& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+    const V3 h = glassHalf(m, wo, wi);
+    const float ow = dot(wo, h), iw = dot(wi, h);
+    if (!(ow > 0.0f && iw < 0.0f)) return 0.0f;
+    const float e = glassEta(m), den = ow + e * iw;
+    const float visible = ggxD(h, m.alpha) * std::fabs(ow) / (wo.z * (1.0f + ggxLambda(wo, m.alpha)));
+    return visible * std::fabs(e * e * iw / (den * den));
+}
 V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     if (b.kind == kLobeLambert) return b.scale * kInvPi;
+    if (b.kind == kLobeGlassT) {
+        if (wo.z <= 0.0f || wi.z >= 0.0f) return {0.0f, 0.0f, 0.0f};
+        const V3 h = glassHalf(m, wo, wi);
+        const float ow = dot(wo, h), iw = dot(wi, h);
+        if (!(ow > 0.0f && iw < 0.0f)) return {0.0f, 0.0f, 0.0f};  // Walter's chi+(i.h / i.n)
+        const float fr = m.front ? fresnelDielectric(ow, 1.0f, 1.5f) : fresnelDielectric(ow, 1.5f, 1.0f);
+        const float den = ow + glassEta(m) * iw;
+        const float g = 1.0f / (1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha));
+        return b.scale * ((1.0f - fr) * ggxD(h, m.alpha) * g * std::fabs(iw) * std::fabs(ow) /
+                          (std::fabs(wo.z * wi.z) * den * den));
+    }
+    if (b.kind == kLobeGlassR) {
+        if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+        const V3 wh = normalize(wo + wi);
+        const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+        const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+        const float fr = m.front ? fresnelDielectric(dot(wi, wh), 1.0f, 1.5f) : fresnelDielectric(dot(wi, wh), 1.5f, 1.0f);
+        return b.scale * (fr * k);
+    }
     if (b.kind == kLobeMirror || b.kind == kLobeGlass) return {0.0f, 0.0f, 0.0f};  // delta lobes evaluate to 0
     if (b.kind == kLobeBlend) return blendF(m, b.scale, wo, wi);
     if (b.kind == kLobeOrenNayar) return b.scale * orenNayarScale(m, wo, wi);
@@ -691,8 +734,10 @@ V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     const float fr = m.front ? fresnelDielectric(dot(wi, wh), 1.0f, 1.46f) : fresnelDielectric(dot(wi, wh), 1.46f, 1.0f);
     return b.scale * (fr * k);
 }
+bool lobeTransmits(const Lobe& b) { return b.kind == kLobeGlassT; }
 float lobePdf(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     if (b.kind == kLobeMirror || b.kind == kLobeGlass) return 0.0f;
+    if (b.kind == kLobeGlassT) return glassTPdf(m, wo, wi);
     if (b.kind == kLobeBlend) return blendPdf(m, wo, wi);
     if (b.kind == kLobeLambert || b.kind == kLobeOrenNayar) return wi.z > 0.0f ? std::fabs(wi.z) * kInvPi : 0.0f;  // Kernel.cpp:76-78
     return conductorPdf(m, wo, wi);  // the same visible-normal pdf for either Fresnel
@@ -725,6 +770,21 @@ BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u
         s.pdf = 1.0f - fr;
         s.f = m.spec * (eta * eta * s.pdf / std::fabs(s.wi.z));  // radiance scaled by (eI / eT)^2
         s.part = kPartTransmission | kPartSpecular;
+        return s;
+    }
+    if (b.kind == kLobeGlassT) {  // a visible microfacet normal h, then Snell's law about h
+        if (wo.z <= 0.0f) return s;
+        const V3 h = ggxSampleWh(wo, m.alpha, u1, u2);
+        const float c = dot(wo, h);
+        if (c < 0.0f) return s;
+        const float eta = 1.0f / glassEta(m);  // eta_i / eta_t
+        const float sin2T = eta * eta * std::fmax(0.0f, 1.0f - c * c);
+        if (sin2T >= 1.0f) return s;
+        const float cosT = std::sqrt(std::fmax(0.0f, 1.0f - sin2T));
+        s.wi = normalize(h * (eta * c - cosT) - wo * eta);
+        s.pdf = glassTPdf(m, wo, s.wi);
+        if (s.pdf > 0.0f) s.f = lobeF(m, b, wo, s.wi);
+        s.part = kPartTransmission;
         return s;
     }
     if (b.kind == kLobeBlend) {  // FresnelBlend::sample, Kernel.cpp:734-750
@@ -764,10 +824,11 @@ inline bool sameSide(V3 wo, V3 wi, V3 ng) { return dot(wo, ng) * dot(wi, ng) > 0
 // pdf is their mean, and sample draws lobe floor(u1 count), rescales u1, then adds the other
 // lobes' values (same side only) and pdfs and divides the pdf by the count.
 V3 bsdfEval(const Material& m, V3 wo, V3 wi, V3 ng) {
-    if (!sameSide(wo, wi, ng)) return {0.0f, 0.0f, 0.0f};
+    const bool same = sameSide(wo, wi, ng);  // addition: Reflection on the same side, else Transmission
     const Lobes ls = lobesOf(m);
     V3 f{0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 0; k < ls.n; ++k) f = f + lobeF(m, ls.l[k], wo, wi);
+    for (uint32_t k = 0; k < ls.n; ++k)
+        if (lobeTransmits(ls.l[k]) != same) f = f + lobeF(m, ls.l[k], wo, wi);
     return f;
 }
 float bsdfPdf(const Material& m, V3 wo, V3 wi) {
@@ -786,7 +847,7 @@ BsdfSample bsdfSample(const Material& m, V3 wo, V3 ng, float u1, float u2) {
     const bool same = sameSide(wo, s.wi, ng);
     for (uint32_t k = 0; k < ls.n && !(s.part & kPartSpecular); ++k) {
         if (k == pick) continue;
-        if (same) s.f = s.f + lobeF(m, ls.l[k], wo, s.wi);
+        if (lobeTransmits(ls.l[k]) != same) s.f = s.f + lobeF(m, ls.l[k], wo, s.wi);
         s.pdf += lobePdf(m, ls.l[k], wo, s.wi);
     }
     if (ls.n > 1) s.pdf = s.pdf / cnt;
@@ -1004,7 +1065,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
         const V3 hit = ray.o + ray.d * h.t;
 
         // multipleImportanceSampling, :23-107, only where init left noSpecular set (:158-163)
-        if (mat.type != kMatMirror && mat.type != kMatGlass) {
+        if (mat.type != kMatMirror && !(mat.type == kMatGlass && mat.alpha == 0.0f)) {
             const LightSel sel = selectLight(s, smp.next());
             V3 partA{0, 0, 0}, partB{0, 0, 0};
             float u1, u2;
@@ -1141,7 +1202,8 @@ void splat(F* rgbw, const Params& p, float fx, float fy, V3 L, std::vector<std::
 // One material row of or_scene_create's mats array.
 Material materialOf(const float* m) {
     Material r{static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]};
-    if (r.type == kMatPlastic || r.type == kMatSubstrate) r.alpha = remapRoughness(m[4]);
+    if (r.type == kMatPlastic || r.type == kMatSubstrate || (r.type == kMatGlass && m[4] > 0.0f))
+        r.alpha = remapRoughness(m[4]);
     if (r.type == kMatPlastic || r.type == kMatSubstrate || r.type == kMatGlass) r.spec = {m[5], m[6], m[7]};
     if (r.type == kMatLambert && m[4] > 0.0f) {  // OrenNayar(reflection, sigma), Kernel.cpp:99-106
         const float sigma = m[4] * kPi, s2 = sigma * sigma;

@@ -1017,13 +1017,15 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         for (uint32_t i = 0; i < desc->num_materials; ++i) {
             const pg_material& m = desc->materials[i];
             require(m.type <= PG_MAT_SUBSTRATE, "unknown material type");
-            require(m.type != PG_MAT_GLASS || m.alpha == 0.0f, "rough glass is not supported (alpha must be 0)");
+            require(m.type != PG_MAT_GLASS || m.alpha >= 0.0f, "glass roughness must not be negative");
             require(m.type != PG_MAT_CONDUCTOR || m.alpha > 0.0f, "conductor alpha must be positive");
             require(m.type != PG_MAT_LAMBERT || (m.alpha >= 0.0f && m.alpha <= 1.0f), "matte roughness must be in [0, 1]");
             uint32_t type = m.type;
             float alpha = m.alpha;
             float spec[3] = {m.reserved[0], m.reserved[1], m.reserved[2]};
-            if (m.type == PG_MAT_PLASTIC || m.type == PG_MAT_SUBSTRATE) {  // remapRoughness (Classic/Kernel.cpp:378-381)
+            if (m.type == PG_MAT_GLASS && m.alpha > 0.0f) type = pg::kDevRoughGlass;  // glassInit :641-655
+            if (m.type == PG_MAT_PLASTIC || m.type == PG_MAT_SUBSTRATE || type == pg::kDevRoughGlass) {
+                // remapRoughness (Classic/Kernel.cpp:378-381)
                 const float x = std::log(std::fmax(1e-3f, m.alpha));
                 alpha = std::fmax(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
             } else if (m.type == PG_MAT_LAMBERT && m.alpha > 0.0f) {

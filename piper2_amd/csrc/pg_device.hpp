@@ -587,6 +587,59 @@ __device__ __forceinline__ V3 plasticGlossyF(const Mat& m, V3 wo, V3 wi) {
 }
 __device__ __forceinline__ float lambertPdf(V3 wi) { return wi.z > 0.0f ? fabsf(wi.z) * kInvPi : 0.0f; }
 
+// Rough glass (glassInit with roughness > 0, Classic/Kernel.cpp:641-655): reflection =
+// MicrofacetReflection with the dielectric Fresnel (eta 1.5, face-swapped) scaled by the reflection
+// colour; transmission = microfacet refraction (Walter et al. 2007, in pbrt-v3's form with radiance
+// scaled by (eta_i / eta_t)^2, which cancels its eta^2) scaled by the transmission colour. The
+// reference's MicrofacetTransmission::sample refracts with eta_t / eta_i, so its samples do not
+// follow its own pdf; this is the consistent form.
+__device__ __forceinline__ float glassEtaI(const Mat& m) { return m.front() ? 1.0f : 1.5f; }
+__device__ __forceinline__ float glassEtaT(const Mat& m) { return m.front() ? 1.5f : 1.0f; }
+__device__ __forceinline__ V3 roughGlassR(const Mat& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
+    const V3 wh = normalize(wo + wi);
+    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+    const float k = ggxD(wh, m.alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    return m.albedo * (fresnelDielectric(dot(wi, wh), glassEtaI(m), glassEtaT(m)) * k);
+}
+__device__ __forceinline__ V3 roughGlassT(const Mat& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z >= 0.0f) return {0.0f, 0.0f, 0.0f};
+    const float eta = glassEtaT(m) / glassEtaI(m);
+    V3 wh = normalize(wo + wi * eta);
+    if (wh.z < 0.0f) wh = -wh;
+    const float ow = dot(wo, wh), iw = dot(wi, wh);
+    if (!(ow > 0.0f && iw < 0.0f)) return {0.0f, 0.0f, 0.0f};  // Walter's chi+: a microfacet wo sees
+    const float fr = fresnelDielectric(ow, glassEtaI(m), glassEtaT(m));
+    const float den = ow + eta * iw;
+    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
+    const float v = (1.0f - fr) * ggxD(wh, m.alpha) * fabsf(iw) * fabsf(ow) / (g * fabsf(wo.z * wi.z) * den * den);
+    return m.spec() * v;
+}
+__device__ __forceinline__ float roughGlassTPdf(const Mat& m, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+    const float eta = glassEtaT(m) / glassEtaI(m);
+    V3 wh = normalize(wo + wi * eta);
+    if (wh.z < 0.0f) wh = -wh;
+    const float ow = dot(wo, wh), iw = dot(wi, wh);
+    if (!(ow > 0.0f && iw < 0.0f)) return 0.0f;  // only normals the visible-normal sampler draws
+    const float den = ow + eta * iw;
+    const float dwh = fabsf(eta * eta * iw / (den * den));  // d omega_h / d omega_i
+    return ggxD(wh, m.alpha) * fabsf(ow) / (wo.z * (1.0f + ggxLambda(wo, m.alpha))) * dwh;
+}
+__device__ __forceinline__ uint32_t glassLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
+__device__ __forceinline__ V3 roughGlassEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
+    const uint32_t lobes = glassLobes(m);
+    if (dot(wo, ng) * dot(wi, ng) > 0.0f) return (lobes & 1u) ? roughGlassR(m, wo, wi) : V3{0.0f, 0.0f, 0.0f};
+    return (lobes & 2u) ? roughGlassT(m, wo, wi) : V3{0.0f, 0.0f, 0.0f};
+}
+__device__ __forceinline__ float roughGlassPdf(const Mat& m, V3 wo, V3 wi) {
+    const uint32_t lobes = glassLobes(m);
+    float p = 0.0f;
+    if (lobes & 1u) p += conductorPdf(m.alpha, wo, wi);
+    if (lobes & 2u) p += roughGlassTPdf(m, wo, wi);
+    return lobes == 3u ? p / 2.0f : p;
+}
+
 // OrenNayar::evaluate (Classic/Kernel.cpp:108-131): A and B are precomputed on upload; the value is
 // zero when either direction is within sin(theta) < 1e-4 of the normal, as the reference has it.
 __device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
@@ -633,6 +686,7 @@ __device__ __forceinline__ bool isSpecular(const Mat& m) { return m.type() == 3u
 __device__ __forceinline__ uint32_t plasticLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
 
 __device__ __forceinline__ V3 bsdfEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
+    if (m.type() == kDevRoughGlass) return roughGlassEval(m, wo, wi, ng);
     if (!(dot(wo, ng) * dot(wi, ng) > 0.0f)) return {0.0f, 0.0f, 0.0f};  // Classic/Kernel.cpp:220
     if (m.type() == 0) return m.albedo * kInvPi;
     if (m.type() == 1) return conductorF(m.albedo, m.alpha, wo, wi);
@@ -649,6 +703,7 @@ __device__ __forceinline__ float bsdfPdf(const Mat& m, V3 wo, V3 wi) {
     if (m.type() == 0 || m.type() == kDevOrenNayar) return lambertPdf(wi);
     if (m.type() == 1) return conductorPdf(m.alpha, wo, wi);
     if (m.type() == 5u) return fresnelBlendPdf(m, wo, wi);
+    if (m.type() == kDevRoughGlass) return roughGlassPdf(m, wo, wi);
     if (m.type() != 2u) return 0.0f;
     const uint32_t lobes = plasticLobes(m);
     float p = 0.0f;
@@ -708,6 +763,38 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
             s.wi = normalize(V3{0.0f, 0.0f, eta * cosI - cosT} - wo * eta);
             s.pdf = 1.0f - fr;
             s.f = rt * (eta * eta * s.pdf / fabsf(s.wi.z));
+        }
+        return s;
+    }
+    if (m.type() == kDevRoughGlass) {  // the generic mixture over the reflection and transmission lobes
+        const uint32_t lobes = glassLobes(m);
+        const uint32_t count = __popc(lobes);
+        if (count == 0) return s;
+        const uint32_t select = min(static_cast<uint32_t>(floorf(u1 * static_cast<float>(count))), count - 1u);
+        u1 = u1 * static_cast<float>(count) - static_cast<float>(select);
+        const bool reflect = (lobes & 1u) && select == 0;
+        if (reflect) {
+            s = ggxReflectSample(wo, m.alpha, u1, u2);
+            if (s.pdf > 0.0f) s.f = roughGlassR(m, wo, s.wi);
+        } else {  // a visible microfacet normal, then Snell's law about it with eta_i / eta_t
+            if (wo.z <= 0.0f) return s;
+            const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+            const float c = dot(wo, wh);
+            if (c < 0.0f) return s;
+            const float eta = glassEtaI(m) / glassEtaT(m);
+            const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - c * c);
+            if (sin2T >= 1.0f) return s;
+            const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sin2T));
+            s.wi = normalize(wh * (eta * c - cosT) - wo * eta);
+            s.pdf = roughGlassTPdf(m, wo, s.wi);
+            if (s.pdf > 0.0f) s.f = roughGlassT(m, wo, s.wi);
+        }
+        if (!(s.pdf > 0.0f)) return s;
+        if (count > 1) {  // the other lobe: its value where its side matches, its pdf always
+            const bool same = dot(wo, ng) * dot(s.wi, ng) > 0.0f;
+            if (reflect && !same) s.f = s.f + roughGlassT(m, wo, s.wi);
+            if (!reflect && same) s.f = s.f + roughGlassR(m, wo, s.wi);
+            s.pdf = (s.pdf + (reflect ? roughGlassTPdf(m, wo, s.wi) : conductorPdf(m.alpha, wo, s.wi))) / 2.0f;
         }
         return s;
     }

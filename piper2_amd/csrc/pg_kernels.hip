@@ -200,6 +200,7 @@ __device__ __forceinline__ bool isTrainPath(const ItemGuide& ig, uint32_t g) {
 // fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
 constexpr float kGuideMinAlpha = 0.04f;
 __device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
+    if (m.type() == kDevRoughGlass) return 0.0f;  // the guide proposes reflected directions only
     return (m.type() != 1u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
 }
 

@@ -39,6 +39,7 @@ static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 B");
 // Material types: the public PG_MAT_* values (0 Lambert, 1 conductor, 2 plastic, 3 mirror,
 // 4 smooth glass, 5 substrate) plus this device-only one.
 constexpr uint32_t kDevOrenNayar = 16;  // matte with roughness > 0
+constexpr uint32_t kDevRoughGlass = 17;  // glass with roughness > 0 (alpha: remapped roughness)
 constexpr uint32_t kMatTextured = 0x80000000u;  // type flag: a colour input comes from a texture
 constexpr uint32_t kTexNone = 0xFFFFFFFFu;
 constexpr uint32_t kTexChecker = 1, kTexImage = 2;   // PG_TEX_*

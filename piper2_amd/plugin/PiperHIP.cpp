@@ -486,9 +486,10 @@ std::shared_ptr<Surface> Surface::mirror(std::array<float, 3> reflection) {
 }
 std::shared_ptr<Surface> Surface::glass(std::array<float, 3> reflection, std::array<float, 3> transmission,
                                         float roughnessX, float roughnessY) {
-    if (roughnessX != 0.0f || roughnessY != 0.0f) raise("glass: only smooth glass (roughness 0) is supported");
+    if (roughnessX != roughnessY) raise("glass: anisotropic roughness is not supported");
+    if (!(roughnessX >= 0.0f) || !std::isfinite(roughnessX)) raise("glass roughness must be finite and not negative");
     auto s = std::make_shared<Surface>();
-    s->kind = Kind::Glass, s->color = reflection, s->alpha = 0.0f, s->specular = transmission;
+    s->kind = Kind::Glass, s->color = reflection, s->alpha = roughnessX, s->specular = transmission;
     return s;
 }
 std::shared_ptr<Surface> Surface::substrate(std::array<float, 3> diffuse, std::array<float, 3> specular,

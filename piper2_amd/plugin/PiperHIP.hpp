@@ -182,7 +182,8 @@ public:
     static std::shared_ptr<Surface> plastic(std::array<float, 3> diffuse, std::array<float, 3> specular, float roughness);
     // Mirror (Classic.cpp:148-170: Reflection)
     static std::shared_ptr<Surface> mirror(std::array<float, 3> reflection);
-    // Glass (Reflection, Transmission, RoughnessX, RoughnessY): the smooth form only (both roughnesses 0)
+    // Glass (Reflection, Transmission, RoughnessX, RoughnessY): smooth with roughness 0, rough
+    // (microfacet reflection + transmission) otherwise; isotropic only
     static std::shared_ptr<Surface> glass(std::array<float, 3> reflection, std::array<float, 3> transmission,
                                           float roughnessX = 0.0f, float roughnessY = 0.0f);
     // Substrate (Classic.cpp:172-195: Diffuse, Specular, RoughnessX, RoughnessY): isotropic only

@@ -150,7 +150,7 @@ class _Builder:
 
 
 def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
-            specular=False, point_lights=False, textured=False):
+            specular=False, point_lights=False, textured=False, glass_roughness=0.0):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
@@ -158,7 +158,7 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     seeded matte roughness (Oren-Nayar); specular=True makes the floor substrate (its albedo as the
     diffuse colour) and adds a smooth glass sphere (outward winding, so rays enter through front
     faces and leave through back faces) and a mirror panel before the back wall, tilted to show the
-    ceiling; point_lights=True adds two point lights (lights 2 and 3 after the environment and the
+    ceiling (glass_roughness > 0 makes the sphere rough glass); point_lights=True adds two point lights (lights 2 and 3 after the environment and the
     ceiling quad); textured=True adds texture coordinates (a planar projection of every vertex),
     a checkerboard on the back wall, 8-bit images on the floor (repeat) and the left wall (mirror)
     and, with plastic=True, a checkerboard specular colour on the first box. The light and the rest
@@ -203,7 +203,8 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
         b.add(v, t, b.material(PLASTIC, [0.5, 0.3, 0.2], 0.2, [0.8, 0.8, 0.8]))
     if specular:
         sv, st = icosphere(2)
-        b.add(sv * 0.28 + np.array([0.3, 0.05, 0.35]), st, b.material(GLASS, [1.0, 1.0, 1.0], 0.0, [0.95, 0.95, 0.95]))
+        b.add(sv * 0.28 + np.array([0.3, 0.05, 0.35]), st,
+              b.material(GLASS, [1.0, 1.0, 1.0], glass_roughness, [0.95, 0.95, 0.95]))
         mirror = [[-0.7, -0.3, -0.55], [-0.1, -0.3, -0.55], [-0.1, 0.393, -0.95], [-0.7, 0.393, -0.95]]  # tilted up 30 deg
         v, t = _quad_tris(*mirror)
         b.add(v, t, b.material(MIRROR, [0.9, 0.9, 0.9]))

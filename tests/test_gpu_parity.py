@@ -208,6 +208,12 @@ def test_textured_paths_match_oracle(dev):
     _path_parity(sc, dev)
 
 
+def test_rough_glass_paths_match_oracle(dev):
+    # a rough glass sphere: light sampled through the surface (transmission evaluated for NEE)
+    sc = scenes.cornell(width=128, height=128, spp=16, specular=True, glass_roughness=0.3)
+    _path_parity(sc, dev)
+
+
 def test_terrain_crop_paths_match_oracle(dev, terrain):
     # GGX conductors, 16 lights, Russian roulette after depth 3
     _path_parity(terrain, dev, rect=(800, 400, 96, 64), spp=4, frac=0.98)

@@ -420,6 +420,102 @@ def test_substrate_reciprocity_and_sampling(rough):
     assert np.allclose(est / n, albedo, rtol=0.03), (est / n, albedo)
 
 
+def _ggx_lambda(w, a):
+    c2 = w[2] ** 2
+    t2 = max(0.0, 1 - c2) / c2
+    return (-1 + math.sqrt(1 + a * a * t2)) / 2
+
+
+def _ggx_d(h, a):
+    c2 = h[2] ** 2
+    t2 = max(0.0, 1 - c2) / c2
+    return 1 / (math.pi * a * a * c2 * c2 * (1 + t2 / (a * a)) ** 2)
+
+
+def _fresnel_textbook(c, n1, n2):
+    c = min(1.0, abs(c))
+    s = n1 / n2 * math.sqrt(max(0.0, 1 - c * c))
+    if s >= 1:
+        return 1.0
+    ti, tt = math.acos(c), math.asin(s)
+    if ti == 0:
+        return ((n1 - n2) / (n1 + n2)) ** 2
+    return 0.5 * ((math.sin(ti - tt) / math.sin(ti + tt)) ** 2 + (math.tan(ti - tt) / math.tan(ti + tt)) ** 2)
+
+
+@pytest.mark.parametrize("front", [True, False])
+def test_rough_glass_transmission_matches_walter(front):
+    """Rough glass transmission against Walter et al. 2007's BTDF written in their notation: the
+    half vector h_t = -(eta_i i + eta_o o) normalised, f = |i.h| |o.h| / (|i.n| |o.n|) eta_o^2
+    (1 - F(i.h)) G D(h) / (eta_i i.h + eta_o o.h)^2 where i.h > 0 > o.h (their chi+ terms, zero
+    elsewhere), with i = wo on the eta_i side; radiance carries
+    the extra factor (eta_i / eta_o)^2. G = 1 / (1 + Lambda(i) + Lambda(o)) and the Fresnel term is
+    the sine/tangent textbook form. The reflection half must match the white conductor x Fresnel."""
+    rough = 0.3
+    alpha = lib().or_remap_roughness(np.float32(rough))
+    T = np.array([0.9, 0.8, 0.7])
+    mat = np.array([4, 0, 0, 0, rough, *T], np.float32)  # no reflection colour: transmission only
+    ni, no = (1.0, 1.5) if front else (1.5, 1.0)
+    rng = np.random.default_rng(12)
+    checked = 0
+    for _ in range(1200):
+        i = _upper(rng)
+        o = _upper(rng)
+        o[2] = -o[2]
+        h = -(ni * i + no * o)
+        h /= np.linalg.norm(h)
+        if h[2] < 0:
+            h = -h
+        ih, oh = float(np.dot(i, h)), float(np.dot(o, h))
+        f, _ = _bsdf(mat, i, o, front)
+        if not (ih > 0 > oh):  # Walter's chi+ terms: no microfacet visible from i refracts i into o
+            assert not f.any()
+            continue
+        g = 1 / (1 + _ggx_lambda(i, alpha) + _ggx_lambda(o, alpha))
+        fw = abs(ih) * abs(oh) / (abs(i[2]) * abs(o[2])) * no * no * (1 - _fresnel_textbook(ih, ni, no)) * g * \
+            _ggx_d(h, alpha) / (ni * ih + no * oh) ** 2
+        want = T * fw * (ni / no) ** 2
+        assert np.allclose(f, want, rtol=5e-3, atol=1e-7), (i, o, f, want)  # float32 near the critical angle
+        checked += 1
+    assert checked > 100
+    # reflection half: white conductor GGX times the face's Fresnel, times the reflection colour
+    matr = np.array([4, 0.6, 0.7, 0.8, rough, 0, 0, 0], np.float32)
+    cond = np.array([1, 1, 1, 1, alpha, 0, 0, 0], np.float32)
+    for _ in range(50):
+        wo, wi = _upper(rng), _upper(rng)
+        wh = (wo + wi) / np.linalg.norm(wo + wi)
+        fr = _fresnel_textbook(float(np.dot(wi, wh)), ni, no)
+        assert np.allclose(_bsdf(matr, wo, wi, front)[0], np.array([0.6, 0.7, 0.8]) * fr * _bsdf(cond, wo, wi)[0],
+                           rtol=1e-3, atol=1e-7)
+
+
+@pytest.mark.parametrize("front", [True, False])
+def test_rough_glass_mixture_sampling_matches_evaluate(front):
+    """Both lobes: a sample's value and pdf equal the evaluate path's (the other lobe's value only
+    on its side, its pdf always, halved) and E[f |cos| / pdf] equals quadrature of the evaluate path
+    over the whole sphere."""
+    mat = np.array([4, 0.9, 0.9, 0.9, 0.4, 0.8, 0.85, 0.9], np.float32)
+    wo = np.array([0.4, 0.3, 0.0], np.float32)
+    wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+    rng = np.random.default_rng(31)
+    n, est, below = 30000, np.zeros(3), 0
+    for u1, u2 in rng.uniform(0, 1, (n, 2)):
+        wi, f, pdf = _sample(mat, wo, u1, u2, front)
+        if pdf > 0:
+            f2, pdf2 = _bsdf(mat, wo, wi, front)
+            assert np.allclose(f, f2, rtol=1e-4, atol=1e-8) and abs(pdf - pdf2) <= 1e-4 * pdf
+            est += f * abs(wi[2]) / pdf
+            below += wi[2] < 0
+    assert below > n // 10  # from inside, total internal reflection removes many transmissions
+    dirs, dw = _hemisphere_grid(400, 200)
+    quad = np.zeros(3)
+    for d, w in zip(dirs, dw):
+        for z in (1, -1):
+            v = np.array([d[0], d[1], z * d[2]])
+            quad += _bsdf(mat, wo, v, front)[0] * d[2] * w
+    assert np.allclose(est / n, quad, rtol=0.04), (est / n, quad)
+
+
 @pytest.mark.parametrize("kind", ["glass", "mirror"])
 def test_lossless_specular_object_vanishes_in_a_white_furnace(kind):
     """A white environment (radiance 1) around a lossless glass sphere (R = T = 1) or mirror sphere:
