@@ -18,9 +18,8 @@
 // Mesh and image paths are relative to the scene file. Colour inputs take ConstantTexture,
 // CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from binary PPM / PGM / PAM with
 // 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of the reference
-// that the HIP path does not implement (textured roughness, rough glass, anisotropic substrate,
-// multi-primitive area lights, time-sampled transforms, other image formats) are rejected with a
-// message.
+// that the HIP path does not implement (textured roughness, anisotropic roughness, multi-primitive
+// area lights, time-sampled transforms, other image formats) are rejected with a message.
 #include <array>
 #include <cctype>
 #include <cmath>
