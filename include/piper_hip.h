@@ -91,12 +91,15 @@ typedef struct pg_texture {
     uint32_t width, height, wrap; /* image: texels hold height rows of width * channels bytes */
     const uint8_t* texels;
 } pg_texture;
-/* Texture of each colour input of a material (PG_TEX_NONE or an index into pg_scene_desc.textures);
- * a texture needs at least 3 channels and replaces the constant at every hit, sampled at the hit's
- * texture coordinate. */
+/* Texture of each input of a material (PG_TEX_NONE or an index into pg_scene_desc.textures),
+ * sampled at the hit's texture coordinate and replacing the constant there. Colour inputs need a
+ * texture of at least 3 channels; the roughness input reads channel 0 and is a roughness as the
+ * surface's config gives it (matte: 0 Lambert, > 0 Oren-Nayar; conductor: alpha = roughness^2;
+ * plastic, substrate: remapRoughness; glass: 0 smooth, > 0 rough). */
 typedef struct pg_material_textures {
     uint32_t albedo;   /* replaces pg_material.albedo */
     uint32_t reserved; /* replaces pg_material.reserved (plastic / substrate specular, glass transmission) */
+    uint32_t alpha;    /* the roughness input (not for mirrors) */
 } pg_material_textures;
 
 typedef struct pg_point_light { /* PointLightData (Light/BasicLight/Shared.hpp:26-29): a delta light */

@@ -221,7 +221,7 @@ struct Scene {
     std::vector<PointLight> points;
     std::vector<float> triUV;  // 6 per triangle: texture coordinates of corners 0, 1, 2
     std::vector<Texture> textures;
-    std::vector<std::array<uint32_t, 2>> matTex;  // per material: albedo, spec texture (~0u: none)
+    std::vector<std::array<uint32_t, 3>> matTex;  // per material: albedo, spec, roughness texture (~0u: none)
     Camera cam;
     V3 env;
     std::vector<BNode> nodes;
@@ -893,7 +893,7 @@ std::array<float, 4> sampleTexture(const Texture& t, float u, float v) {
 void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
     if (matId >= s.matTex.size()) return;
     const auto& b = s.matTex[matId];
-    if (b[0] == ~0u && b[1] == ~0u) return;
+    if (b[0] == ~0u && b[1] == ~0u && b[2] == ~0u) return;
     float tu = 0.0f, tv = 0.0f;
     if (!s.triUV.empty()) {
         const float* c = &s.triUV[6 * static_cast<size_t>(h.prim)];
@@ -908,6 +908,23 @@ void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
     if (b[1] != ~0u) {
         const auto c = sampleTexture(s.textures[b[1]], tu, tv);
         m.spec = {c[0], c[1], c[2]};
+    }
+    if (b[2] != ~0u) {  // the roughness at this hit, read as each surface's init reads its config
+        const float r = sampleTexture(s.textures[b[2]], tu, tv)[0];
+        if (m.type == kMatLambert) {
+            m.orenNayar = r > 0.0f;
+            if (m.orenNayar) {
+                const float sigma = r * kPi, s2 = sigma * sigma;
+                m.onA = 1.0f - s2 / (2.0f * (s2 + 0.33f));
+                m.onB = 0.45f * s2 / (s2 + 0.09f);
+            }
+        } else if (m.type == kMatConductor) {
+            m.alpha = std::fmax(r * r, 1e-7f);
+        } else if (m.type == kMatPlastic || m.type == kMatSubstrate) {
+            m.alpha = remapRoughness(r);
+        } else if (m.type == kMatGlass) {
+            m.alpha = r > 0.0f ? remapRoughness(r) : 0.0f;
+        }
     }
 }
 
@@ -1398,12 +1415,12 @@ void or_scene_set_points(void* scene, const float* points, uint32_t n) {
     }
 }
 // Textures: triuv[6*nt] texture coordinates of each triangle's corners (or NULL), one texture per
-// or_scene_add_texture call, mtex[2*nmat] albedo / spec texture per material (~0u: none).
+// or_scene_add_texture call, mtex[3*nmat] albedo / spec / roughness texture per material (~0u: none).
 void or_scene_set_textures(void* scene, const float* triuv, const uint32_t* mtex) {
     auto* s = static_cast<Scene*>(scene);
     s->triUV.assign(triuv, triuv ? triuv + 6 * s->tris.size() : triuv);
     s->matTex.resize(s->mats.size());
-    for (size_t i = 0; i < s->mats.size(); ++i) s->matTex[i] = {mtex[2 * i], mtex[2 * i + 1]};
+    for (size_t i = 0; i < s->mats.size(); ++i) s->matTex[i] = {mtex[3 * i], mtex[3 * i + 1], mtex[3 * i + 2]};
 }
 // itype {type, channels, width, height, wrap}; ftype {black[4], white[4], scale}
 uint32_t or_scene_add_texture(void* scene, const uint32_t* itype, const float* ftype, const uint8_t* texels) {

@@ -464,7 +464,7 @@ def scene_desc(sc):
     else:
         pts = None
     # optional textures: "texcoords" (V, 2) float32, "textures" (list of dicts with the pg_texture
-    # fields; "texels" a uint8 array), "material_textures" (M, 2) uint32
+    # fields; "texels" a uint8 array), "material_textures" (M, 3) uint32 (albedo, reserved, roughness)
     texcoords = sc.get("texcoords")
     if texcoords is not None:
         texcoords = sc["_texcoords"] = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
@@ -482,7 +482,7 @@ def scene_desc(sc):
                              _ptr(texels) if texels is not None else None)
     mtex = sc.get("material_textures")
     if mtex is not None:
-        mtex = sc["_mtex"] = np.ascontiguousarray(mtex, np.uint32).reshape(-1, 2)
+        mtex = sc["_mtex"] = np.ascontiguousarray(mtex, np.uint32).reshape(-1, 3)
     sc["_tex_arr"], sc["_texels"] = tex_arr, keep
     c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
     cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))

@@ -1066,17 +1066,21 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
                 texelPool.insert(texelPool.end(), t.texels, t.texels + bytes);
             }
         }
-        std::vector<uint2> matTex(desc->num_materials, uint2{pg::kTexNone, pg::kTexNone});
+        std::vector<uint4> matTex(desc->num_materials, uint4{pg::kTexNone, pg::kTexNone, pg::kTexNone, 0u});
         bool textured = false;
         if (desc->material_textures) {
             for (uint32_t i = 0; i < desc->num_materials; ++i) {
                 const pg_material_textures& b = desc->material_textures[i];
-                for (uint32_t t : {b.albedo, b.reserved}) {
+                for (uint32_t t : {b.albedo, b.reserved, b.alpha})
                     require(t == PG_TEX_NONE || t < desc->num_textures, "material texture index out of range");
+                for (uint32_t t : {b.albedo, b.reserved})
                     require(t == PG_TEX_NONE || desc->textures[t].channels >= 3, "colour textures need 4 channels");
-                }
-                matTex[i] = uint2{b.albedo, b.reserved};
-                if (b.albedo != PG_TEX_NONE || b.reserved != PG_TEX_NONE) {
+                require(b.alpha == PG_TEX_NONE || desc->materials[i].type != PG_MAT_MIRROR, "a mirror has no roughness");
+                const uint32_t mt = desc->materials[i].type;
+                require(b.reserved == PG_TEX_NONE || mt == PG_MAT_PLASTIC || mt == PG_MAT_SUBSTRATE || mt == PG_MAT_GLASS,
+                        "only plastic, substrate and glass have a second colour input");
+                matTex[i] = uint4{b.albedo, b.reserved, b.alpha, desc->materials[i].type};
+                if (b.albedo != PG_TEX_NONE || b.reserved != PG_TEX_NONE || b.alpha != PG_TEX_NONE) {
                     mats[i].type |= pg::kMatTextured;
                     textured = true;
                 }
@@ -1119,11 +1123,11 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         ds.numPoints = desc->num_point_lights;
         upload(s->textures, texTab.data(), texTab.size() * sizeof(pg::DevTexture), st);
         upload(s->texels, texelPool.data(), texelPool.size(), st);
-        upload(s->matTex, matTex.data(), matTex.size() * sizeof(uint2), st);
+        upload(s->matTex, matTex.data(), matTex.size() * sizeof(uint4), st);
         upload(s->triUV, triUV.data(), triUV.size() * sizeof(float), st);
         ds.textures = s->textures.as<pg::DevTexture>();
         ds.texels = s->texels.as<uint8_t>();
-        ds.matTex = s->matTex.as<uint2>();
+        ds.matTex = s->matTex.as<uint4>();
         ds.triUV = triUV.empty() ? nullptr : s->triUV.as<float>();
         ds.textured = textured ? 1u : 0u;
         ds.sobol = dev->sobol.as<uint32_t>();

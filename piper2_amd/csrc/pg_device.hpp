@@ -529,17 +529,19 @@ __device__ __forceinline__ float2 hitTexcoord(const DevScene& s, uint32_t pos, f
 
 // A hit's material. The plastic and Oren-Nayar extras (spec) stay in the scene's table (or its
 // texture) and are read where a lobe needs them, so they hold no registers across the vertex.
+constexpr uint32_t kRoughFromTexture = 0x200u;  // Mat::bits: alpha (and Oren-Nayar's B) come from a texture
 struct Mat {
     uint32_t bits;  // type (0 Lambert, 1 conductor, 2 plastic, 3 mirror, 4 glass, 5 substrate,
-                    // 16 Oren-Nayar) | front face << 8 | kMatTextured
+                    // 16 Oren-Nayar, 17 rough glass) | front face << 8 | kRoughFromTexture | kMatTextured
     V3 albedo;      // diffuse colour, conductor F0, mirror / glass reflection (texture applied)
     float alpha;    // GGX alpha; Oren-Nayar: A
+    float aux;      // Oren-Nayar B when the roughness comes from a texture
     uint32_t id;
     float2 uv;      // the hit's texture coordinate (textured materials)
     const DevMaterial* tab;
     const DevTexture* texTab;
     const uint8_t* texels;
-    const uint2* matTex;
+    const uint4* matTex;
     __device__ __forceinline__ uint32_t type() const { return bits & 0xFFu; }
     // plastic / glass: the hit is on the triangle's front face (etaI, etaT of their init)
     __device__ __forceinline__ bool front() const { return ((bits >> 8) & 1u) != 0u; }
@@ -561,15 +563,34 @@ template <bool TEX = true>
 __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool front = true, uint32_t pos = 0,
                                        float bu = 0.0f, float bv = 0.0f) {
     const DevMaterial& m = s.mats[id];
-    const uint32_t type = TEX ? m.type : (m.type & ~kMatTextured);
-    Mat r{type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, id, make_float2(0.0f, 0.0f), s.mats, s.textures,
-          s.texels, s.matTex};
+    const uint32_t type = TEX ? m.type : (m.type & 0xFFu);  // without textures: the plain type byte
+    Mat r{type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, 0.0f, id, make_float2(0.0f, 0.0f), s.mats,
+          s.textures, s.texels, s.matTex};
     if (type & kMatTextured) {
         r.uv = hitTexcoord(s, pos, bu, bv);
-        const uint32_t t = s.matTex[id].x;
-        if (t != kTexNone) {
-            const float4 c = sampleTexture(s.textures, s.texels, t, r.uv);
+        const uint4 b = s.matTex[id];
+        if (b.x != kTexNone) {
+            const float4 c = sampleTexture(s.textures, s.texels, b.x, r.uv);
             r.albedo = {c.x, c.y, c.z};
+        }
+        if (b.z != kTexNone) {  // the roughness at this hit picks the lobe as each surface's init does
+            const float rough = sampleTexture(s.textures, s.texels, b.z, r.uv).x;
+            uint32_t t = b.w;
+            if (b.w == 0u && rough > 0.0f) {  // matteInit: Oren-Nayar, sigma = roughness x pi
+                const float sigma = rough * kPi, s2 = sigma * sigma;
+                t = kDevOrenNayar;
+                r.alpha = 1.0f - s2 / (2.0f * (s2 + 0.33f));
+                r.aux = 0.45f * s2 / (s2 + 0.09f);
+            } else if (b.w == 1u) {
+                r.alpha = fmaxf(rough * rough, 1e-7f);
+            } else if (b.w == 2u || b.w == 5u || (b.w == 4u && rough > 0.0f)) {
+                if (b.w == 4u) t = kDevRoughGlass;
+                const float x = logf(fmaxf(1e-3f, rough));  // remapRoughness (Classic/Kernel.cpp:378-381)
+                r.alpha = fmaxf(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+            } else if (b.w == 4u) {
+                r.alpha = 0.0f;
+            }
+            r.bits = (r.bits & ~0xFFu) | t | kRoughFromTexture;
         }
     }
     return r;
@@ -652,7 +673,8 @@ __device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
     const bool iSteeper = fabsf(wi.z) > fabsf(wo.z);
     const float sinAlpha = iSteeper ? sinO : sinI;
     const float tanBeta = iSteeper ? sinI / fabsf(wi.z) : sinO / fabsf(wo.z);
-    return m.albedo * (kInvPi * (m.alpha + m.spec().x * maxCos * sinAlpha * tanBeta));
+    const float b = (m.bits & kRoughFromTexture) ? m.aux : m.spec().x;
+    return m.albedo * (kInvPi * (m.alpha + b * maxCos * sinAlpha * tanBeta));
 }
 
 // FresnelBlend::evaluate (Classic/Kernel.cpp:705-722): Ashikhmin-Shirley diffuse + Schlick-weighted

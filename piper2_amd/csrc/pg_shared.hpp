@@ -83,7 +83,7 @@ struct DevScene {
     const float* triUV;          // texture coordinates of each original triangle's corners (6 floats)
     const DevTexture* textures;
     const uint8_t* texels;
-    const uint2* matTex;         // per material: albedo and spec texture (kTexNone: the constant)
+    const uint4* matTex;         // per material: albedo, spec, roughness texture (kTexNone: the constant), base type
     const uint32_t* sobol;       // kSobolDims * kSobolCols
     const uint32_t* sobolT;      // the same matrices bit-major: kSobolCols * kSobolDims
     const uint32_t* sobolB;      // byte-sliced: kByteChunks * 256 * kByteDims

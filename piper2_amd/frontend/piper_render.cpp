@@ -15,11 +15,11 @@
 // the active sensor (the scene-file layout of tests/plugin/render_scene.cpp).
 // --write-exr converts raw float32 RGB (row-major, top row first) into an EXR file.
 //
-// Mesh and image paths are relative to the scene file. Colour inputs take ConstantTexture,
-// CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from binary PPM / PGM / PAM with
-// 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of the reference
-// that the HIP path does not implement (textured roughness, anisotropic roughness, multi-primitive
-// area lights, time-sampled transforms, other image formats) are rejected with a message.
+// Mesh and image paths are relative to the scene file. Colour and roughness inputs take
+// ConstantTexture, CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from binary PPM /
+// PGM / PAM with 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of
+// the reference that the HIP path does not implement (anisotropic roughness, multi-primitive area
+// lights, time-sampled transforms, other image formats) are rejected with a message.
 #include <array>
 #include <cctype>
 #include <cmath>
@@ -90,6 +90,30 @@ void requireClass(const Json& cfg, const char* module, const char* cls) {
 
 // ConstantTexture (ProceduralTexture.cpp:27-45): Value holds 1, 2 or 4 channels; the surface asks
 // for 4 (colours, the first three used) or 1 (roughness).
+bool sameJson(const Json& a, const Json& b) {
+    if (a.type() != b.type()) return false;
+    switch (a.type()) {
+        case Json::Type::Null: return true;
+        case Json::Type::Bool: return a.boolean() == b.boolean();
+        case Json::Type::Number: return a.number() == b.number();
+        case Json::Type::String: return a.string() == b.string();
+        case Json::Type::Array: {
+            if (a.array().size() != b.array().size()) return false;
+            for (size_t i = 0; i < a.array().size(); ++i)
+                if (!sameJson(a.array()[i], b.array()[i])) return false;
+            return true;
+        }
+        case Json::Type::Object: {
+            if (a.members().size() != b.members().size()) return false;
+            for (const auto& [k, v] : a.members()) {
+                const Json* w = b.find(k);
+                if (!w || !sameJson(v, *w)) return false;
+            }
+            return true;
+        }
+    }
+    return false;
+}
 std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const char* what) {
     if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
     if (!cfg.find("ClassID")) fail(std::string(what) + ": sampled image textures are not supported" + cfg.where());
@@ -158,13 +182,13 @@ std::shared_ptr<Texture> loadImageTexture(const fs::path& path, uint32_t wrap) {
     return Texture::image(w, h, 4, wrap, std::move(rgba));
 }
 
-// A colour input: a constant (ConstantTexture) or a texture (CheckBoard, sampled image) that the
-// surface samples at each hit.
+// A colour or roughness input: a constant (ConstantTexture) or a texture (CheckBoard, sampled
+// image) that the surface samples at each hit.
 struct ColourInput {
     std::array<float, 3> value{};
     std::shared_ptr<Texture> texture;
 };
-ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base) {
+ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base, size_t channels = 4) {
     if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
     if (!cfg.find("ClassID")) {  // a sampled image (Tracer/Embree/Embree.cpp:1334-1349)
         const std::string& wrap = cfg.at("WrapMode").string();
@@ -174,7 +198,7 @@ ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base)
         fs::path p = img.at("Path").string();
         if (p.is_relative()) p = base / p;
         auto t = loadImageTexture(p, wrap == "Repeat" ? PG_WRAP_REPEAT : PG_WRAP_MIRROR);
-        if (t->desc.channels < 4) fail(std::string(what) + ": a colour image needs RGB or RGBA samples" + cfg.where());
+        if (t->desc.channels < channels) fail(std::string(what) + ": a colour image needs RGB or RGBA samples" + cfg.where());
         return {{}, t};
     }
     const auto [m, c] = classOf(cfg);
@@ -182,15 +206,16 @@ ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base)
         const auto& b = cfg.at("Black").array();
         const auto& w = cfg.at("White").array();
         if (b.size() != w.size()) fail("Channel of black and white must be identical." + cfg.where());
-        if (b.size() != 4) fail(std::string(what) + ": a colour checkerboard needs 4 channels" + cfg.where());
+        if (b.size() != 1 && b.size() != 2 && b.size() != 4)
+            fail(std::string(what) + ": unsupported channel count " + std::to_string(b.size()) + cfg.where());
+        if (b.size() < channels) fail(std::string(what) + ": a colour checkerboard needs 4 channels" + cfg.where());
         std::array<float, 4> black{}, white{};
-        for (size_t k = 0; k < 4; ++k) black[k] = num(b[k]), white[k] = num(w[k]);
-        return {{}, Texture::checkerboard(black, white, num(cfg.at("Scale")), 4)};
+        for (size_t k = 0; k < b.size(); ++k) black[k] = num(b[k]), white[k] = num(w[k]);
+        return {{}, Texture::checkerboard(black, white, num(cfg.at("Scale")), static_cast<uint32_t>(b.size()))};
     }
-    const auto v = constantTexture(cfg, 4, what);
+    const auto v = constantTexture(cfg, channels, what);
     return {{v[0], v[1], v[2]}, nullptr};
 }
-float scalar(const Json& cfg, const char* what) { return constantTexture(cfg, 1, what)[0]; }
 
 // parseTransform (PiperRenderer.cpp:58-137): Translation, Scale (number, [x, y, z] or
 // {Scale, Skew, Shift}), Rotation (Euler [pitch, yaw, roll] in radians, or quaternion [w, x, y, z]).
@@ -327,38 +352,50 @@ private:
         const auto [m, c] = classOf(cfg);
         if (m != "Piper.BuiltinComponent.ClassicSurface") fail("unsupported surface " + cfg.at("ClassID").string() + cfg.where());
         auto in = [&](const char* key) { return colourInput(cfg.at(key), key, mBase); };
-        auto bind = [](std::shared_ptr<Surface> s, const ColourInput& a, const ColourInput& b) {
+        auto rough = [&](const char* key) { return colourInput(cfg.at(key), key, mBase, 1); };
+        auto roughXY = [&]() {  // one roughness for both axes
+            const auto x = rough("RoughnessX"), y = rough("RoughnessY");
+            const bool same = x.texture ? (y.texture && sameJson(cfg.at("RoughnessX"), cfg.at("RoughnessY")))
+                                        : (!y.texture && x.value[0] == y.value[0]);
+            if (!same) fail("anisotropic roughness is not supported" + cfg.where());
+            return x;
+        };
+        auto bind = [](std::shared_ptr<Surface> s, const ColourInput& a, const ColourInput& b, const ColourInput& r) {
             s->colorTexture = a.texture;
             s->specularTexture = b.texture;
+            s->roughnessTexture = r.texture;
             return s;
         };
+        // a textured roughness replaces the constant at every hit; the constant handed to the
+        // factory then only has to be valid
+        auto constant = [](const ColourInput& r, float fallback) { return r.texture ? fallback : r.value[0]; };
         const ColourInput none;
         if (c == "Matte") {
-            const auto d = in("Diffuse");
-            return bind(Surface::matte(d.value, scalar(cfg.at("Roughness"), "Roughness")), d, none);
+            const auto d = in("Diffuse"), r = rough("Roughness");
+            return bind(Surface::matte(d.value, constant(r, 0.0f)), d, none, r);
         }
         if (c == "Plastic") {
-            const auto d = in("Diffuse"), s = in("Specular");
-            return bind(Surface::plastic(d.value, s.value, scalar(cfg.at("Roughness"), "Roughness")), d, s);
+            const auto d = in("Diffuse"), s = in("Specular"), r = rough("Roughness");
+            return bind(Surface::plastic(d.value, s.value, constant(r, 0.5f)), d, s, r);
         }
         if (c == "Mirror") {
             const auto r = in("Reflection");
-            return bind(Surface::mirror(r.value), r, none);
+            return bind(Surface::mirror(r.value), r, none, none);
         }
         if (c == "Glass") {
-            const auto r = in("Reflection"), t = in("Transmission");
-            return bind(Surface::glass(r.value, t.value, scalar(cfg.at("RoughnessX"), "RoughnessX"),
-                                       scalar(cfg.at("RoughnessY"), "RoughnessY")), r, t);
+            const auto r = in("Reflection"), t = in("Transmission"), x = roughXY();
+            const float v = constant(x, 0.0f);
+            return bind(Surface::glass(r.value, t.value, v, v), r, t, x);
         }
         if (c == "Substrate") {
-            const auto d = in("Diffuse"), s = in("Specular");
-            return bind(Surface::substrate(d.value, s.value, scalar(cfg.at("RoughnessX"), "RoughnessX"),
-                                           scalar(cfg.at("RoughnessY"), "RoughnessY")), d, s);
+            const auto d = in("Diffuse"), s = in("Specular"), x = roughXY();
+            const float v = constant(x, 0.5f);
+            return bind(Surface::substrate(d.value, s.value, v, v), d, s, x);
         }
         if (c == "BlackBody") return Surface::matte({0.0f, 0.0f, 0.0f});  // no lobe: the path ends here
         if (c == "Conductor") {  // the GGX conductor this path adds (not a reference class): F0, Roughness
-            const auto f = in("F0");
-            return bind(Surface::conductor(f.value, scalar(cfg.at("Roughness"), "Roughness")), f, none);
+            const auto f = in("F0"), r = rough("Roughness");
+            return bind(Surface::conductor(f.value, constant(r, 0.5f)), f, none, r);
         }
         fail("unsupported surface class " + c + cfg.where());
     }
@@ -496,7 +533,7 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
             put(f, dims, 3);
             put(f, t->texels.data(), t->texels.size());
         }
-        for (const pg_material_textures& b : e.materialTextures) put(f, &b.albedo, 1), put(f, &b.reserved, 1);
+        for (const pg_material_textures& b : e.materialTextures) put(f, &b.albedo, 1), put(f, &b.reserved, 1), put(f, &b.alpha, 1);
     }
     if (!f) fail("cannot write " + out.string());
 }

@@ -160,7 +160,7 @@ uint32_t materialOf(FlatScene& f, const Surface& s) {
         f.textures.push_back(t);
         return static_cast<uint32_t>(f.textures.size() - 1);
     };
-    f.materialTextures.push_back({texIndex(s.colorTexture), texIndex(s.specularTexture)});
+    f.materialTextures.push_back({texIndex(s.colorTexture), texIndex(s.specularTexture), texIndex(s.roughnessTexture)});
     return static_cast<uint32_t>(f.materials.size() - 1);
 }
 

@@ -195,6 +195,7 @@ public:
     float alpha;                         // matte roughness, conductor GGX alpha, plastic / substrate roughness
     std::array<float, 3> specular{};     // plastic / substrate specular, glass transmission
     std::shared_ptr<Texture> colorTexture, specularTexture;  // replace color / specular when set
+    std::shared_ptr<Texture> roughnessTexture;               // replaces the roughness (channel 0) when set
 };
 class Medium : public Object {};
 // Light/AreaLight with a Geometry/BasicShapes plane (origin, U, V), the environment

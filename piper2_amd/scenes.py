@@ -160,8 +160,10 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     faces and leave through back faces) and a mirror panel before the back wall, tilted to show the
     ceiling (glass_roughness > 0 makes the sphere rough glass); point_lights=True adds two point lights (lights 2 and 3 after the environment and the
     ceiling quad); textured=True adds texture coordinates (a planar projection of every vertex),
-    a checkerboard on the back wall, 8-bit images on the floor (repeat) and the left wall (mirror)
-    and, with plastic=True, a checkerboard specular colour on the first box. The light and the rest
+    a checkerboard on the back wall, 8-bit images on the floor (repeat) and the left wall (mirror),
+    a checkerboard roughness (0 and 0.6: Lambert and Oren-Nayar cells) on the ceiling and, with
+    plastic=True, a checkerboard specular colour on the first box and an image roughness on the
+    second. The light and the rest
     of the geometry stay as in the plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
@@ -231,12 +233,18 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
              "texels": trng.integers(0, 256, (5, 7, 4), dtype=np.uint8)},
             {"type": 1, "channels": 4, "black": [0.05, 0.05, 0.05, 1.0], "white": [0.9, 0.2, 0.2, 1.0], "scale": 5.0},
         ]
-        mt = np.full((len(sc["materials"]), 2), none, np.uint32)
+        sc["textures"].append({"type": 1, "channels": 1, "black": [0.0, 0, 0, 0], "white": [0.6, 0, 0, 0],
+                               "scale": 4.0})  # roughness: Lambert / Oren-Nayar cells
+        sc["textures"].append({"type": 2, "channels": 1, "width": 6, "height": 4, "wrap": 0,
+                               "texels": trng.integers(20, 200, (4, 6, 1), dtype=np.uint8)})  # roughness image
+        mt = np.full((len(sc["materials"]), 3), none, np.uint32)
         mt[sc["object_material"][2], 0] = 0  # back wall
         mt[sc["object_material"][0], 0] = 1  # floor
         mt[sc["object_material"][3], 0] = 2  # left wall
+        mt[sc["object_material"][1], 2] = 4  # ceiling roughness: checkerboard of 0 and 0.6
         if plastic:
             mt[sc["object_material"][5], 1] = 3  # the first box's specular colour
+            mt[sc["object_material"][6], 2] = 5  # the second box's roughness
         sc["material_textures"] = mt
         sc["name"] += "_textured"
     return sc

@@ -90,7 +90,7 @@ class OracleScene:
             if sc.get("texcoords") is not None:
                 uv = np.asarray(sc["texcoords"], np.float32).reshape(-1, 2)
                 triuv = np.ascontiguousarray(uv[sc["indices"].astype(np.int64)].reshape(-1, 6))
-            mt = np.ascontiguousarray(sc["material_textures"], np.uint32).reshape(-1, 2)
+            mt = np.ascontiguousarray(sc["material_textures"], np.uint32).reshape(-1, 3)
             self._keep += [triuv, mt]
             lib().or_scene_set_textures(self.h, _p(triuv), _p(mt))
 

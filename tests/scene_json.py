@@ -60,15 +60,17 @@ def write_obj(path, verts, tris, negative=False, uv=None):
 
 def write_pam(path, texels):
     h, w, c = texels.shape
+    kind = {1: "GRAYSCALE", 2: "GRAYSCALE_ALPHA", 4: "RGB_ALPHA"}[c]
     with open(path, "wb") as f:
-        f.write(f"P7\nWIDTH {w}\nHEIGHT {h}\nDEPTH {c}\nMAXVAL 255\nTUPLTYPE RGB_ALPHA\nENDHDR\n".encode())
+        f.write(f"P7\nWIDTH {w}\nHEIGHT {h}\nDEPTH {c}\nMAXVAL 255\nTUPLTYPE {kind}\nENDHDR\n".encode())
         f.write(np.ascontiguousarray(texels, np.uint8).tobytes())
 
 
 def texture_config(t, folder, name):
     if t["type"] == 1:
-        return {"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": list(t["black"]),
-                "White": list(t["white"]), "Scale": t["scale"]}
+        ch = t["channels"]
+        return {"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": list(t["black"][:ch]),
+                "White": list(t["white"][:ch]), "Scale": t["scale"]}
     write_pam(os.path.join(folder, name), t["texels"])
     return {"WrapMode": "Repeat" if t["wrap"] == 0 else "Mirror",
             "Image": {"ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": name}}
@@ -88,12 +90,16 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
         write_obj(os.path.join(folder, name), sc["vertices"][vid], inv.reshape(-1, 3), negative=(o % 2 == 1), uv=uv)
         surf = surface_config(sc["materials"][mat])
         if sc.get("material_textures") is not None:
-            for slot, key in enumerate(("albedo", "spec")):
+            for slot, key in enumerate(("albedo", "spec", "rough")):
                 t = int(sc["material_textures"][mat][slot])
                 if t != 0xFFFFFFFF:
-                    keys = {"albedo": ("Diffuse", "Reflection"), "spec": ("Specular", "Transmission")}[key]
-                    k = next(k for k in keys if k in surf)
-                    surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam")
+                    keys = {"albedo": ("Diffuse", "Reflection"), "spec": ("Specular", "Transmission"),
+                            "rough": ("Roughness", "RoughnessX")}[key]
+                    for k in keys + (("RoughnessY",) if key == "rough" else ()):
+                        if k in surf:
+                            surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam")
+                            if k != "RoughnessX":
+                                break
         inst = {"Type": "Instance", "Geometry": {"ClassID": "Piper.BuiltinComponent.TriangleMesh.TriangleMesh",
                                                  "Path": name},
                 "Surface": surf}
@@ -190,7 +196,7 @@ def read_pgsc(path):
                 off += n
             texs.append(t)
         out["textures"] = texs
-        take("material_textures", np.uint32, 2 * nm, (nm, 2))
+        take("material_textures", np.uint32, 3 * nm, (nm, 3))
     assert off == len(data)
     out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
     return out

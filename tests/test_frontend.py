@@ -50,7 +50,7 @@ def test_textured_scene_export(tmp_path):
     assert np.array_equal(e["vertices"][e["indices"]], sc["vertices"][sc["indices"]])
     assert np.array_equal(e["texcoords"][e["indices"]], sc["texcoords"][sc["indices"]])
     for m in range(len(sc["materials"])):
-        for slot in range(2):
+        for slot in range(3):
             a, b = int(sc["material_textures"][m, slot]), int(e["material_textures"][m, slot])
             assert (a == 0xFFFFFFFF) == (b == 0xFFFFFFFF), (m, slot)
             if a == 0xFFFFFFFF:
@@ -154,7 +154,7 @@ def test_exr_layout_and_half_rounding(tmp_path):
 @pytest.mark.parametrize("mutate,message", [
     (lambda s: s.update(Integrator={"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator"}), "MaxTraceDepth"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(Roughness={"ClassID": "Piper.BuiltinComponent."
-                                                                      "ProceduralTexture.CheckBoard"}), "ConstantTexture"),
+                                                                      "ProceduralTexture.Marble"}), "ConstantTexture"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(Diffuse={"WrapMode": "Repeat", "Image": {
         "ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": "picture.png"}}), "cannot open image"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(

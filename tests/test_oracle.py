@@ -572,8 +572,9 @@ def test_checkerboard_and_image_textures():
 
 
 def test_uniform_checkerboard_renders_as_its_constant():
-    """A checkerboard whose two colours equal a material's constant albedo renders the same film
-    bit for bit (texture coordinates interpolated at every hit, the lookup replacing the constant)."""
+    """A checkerboard whose two values equal a material's constants (colours and roughness) renders
+    the same film bit for bit (texture coordinates interpolated at every hit, the lookup replacing
+    the constant, the roughness mapped per hit as at scene creation)."""
     sc = scenes.cornell(width=32, height=32, spp=8, textured=True, plastic=True)
     plain = dict(sc)
     for k in ("textures", "material_textures", "texcoords"):
@@ -583,7 +584,7 @@ def test_uniform_checkerboard_renders_as_its_constant():
     for i, t in enumerate(sc["textures"]):
         mats = np.nonzero(mt == i)
         m, slot = int(mats[0][0]), int(mats[1][0])
-        col = sc["materials"][m, 1:4] if slot == 0 else sc["materials"][m, 5:8]
+        col = [sc["materials"][m, 1:4], sc["materials"][m, 5:8], [sc["materials"][m, 4], 0, 0]][slot]
         texs.append({"type": 1, "channels": 4, "black": [*col, 1.0], "white": [*col, 1.0], "scale": 3.0})
     uni = dict(sc, textures=texs)
     a = OracleScene(plain).render(threads=4)["film"]
