@@ -37,6 +37,11 @@ def mixed_batch():
     items.append((sc, pg.render_params(sc)))
     sc = scenes.cornell(seed=9, width=33, height=17, spp=2, max_depth=3)
     items.append((sc, pg.render_params(sc, flags=pg.PG_COMPAT_D3 | pg.PG_COMPAT_D13)))
+    # textures, every surface kind, point lights: the batch takes the textured shade variant, which
+    # must leave the untextured scenes' films unchanged too
+    sc = scenes.cornell(seed=3, width=48, height=40, spp=8, textured=True, plastic=True, specular=True,
+                        point_lights=True, glass_roughness=0.2, rough_walls=True)
+    items.append((sc, pg.render_params(sc)))
     return items
 
 

@@ -267,6 +267,25 @@ def test_invalid_arguments_rejected(dev, cornell):
         s.render(p)
 
 
+@pytest.mark.parametrize("mutate,message", [
+    (lambda sc: sc["materials"].__setitem__((0, 0), 9), "unknown material type"),
+    (lambda sc: sc["materials"].__setitem__((0, 4), 1.5), "matte roughness"),
+    (lambda sc: sc["material_textures"].__setitem__((0, 0), 99), "texture index out of range"),
+    (lambda sc: sc["material_textures"].__setitem__((0, 0), 4), "colour textures need 4 channels"),
+    (lambda sc: sc["material_textures"].__setitem__((0, 1), 0), "second colour input"),
+    (lambda sc: sc["textures"][1].__setitem__("wrap", 7), "wrap mode"),
+    (lambda sc: sc["textures"][0].__setitem__("channels", 3), "channels must be 1, 2 or 4"),
+])
+def test_invalid_scene_descriptions_rejected(dev, mutate, message):
+    # scene creation validates materials, textures and bindings before anything reaches the device
+    sc = scenes.cornell(width=16, height=16, spp=1, textured=True, plastic=True)
+    sc["materials"] = sc["materials"].copy()
+    mutate(sc)
+    with pytest.raises(pg.PgError) as e:
+        pg.Scene(dev, sc)
+    assert e.value.code == pg.PG_ERR_INVALID and message in str(e.value), str(e.value)
+
+
 @pytest.mark.parametrize("film_filter,alpha,fp64", [(1, 0.0, False), (2, 2.0, False), (2, 1.5, True)])
 def test_filtered_film_matches_oracle(dev, film_filter, alpha, fp64):
     # FixedSampler's triangle and gaussian filters (FixedSampler/Kernel.cpp:55-70): the same per-sample
