@@ -16,8 +16,8 @@
 // --write-exr converts raw float32 RGB (row-major, top row first) into an EXR file.
 //
 // Mesh and image paths are relative to the scene file. Colour and roughness inputs take
-// ConstantTexture, CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from binary PPM /
-// PGM / PAM with 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of
+// ConstantTexture, CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from PNG or binary
+// PPM / PGM / PAM with 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of
 // the reference that the HIP path does not implement (anisotropic roughness, multi-primitive area
 // lights, time-sampled transforms, other image formats) are rejected with a message.
 #include <array>
@@ -38,6 +38,7 @@
 #include "Exr.hpp"
 #include "Json.hpp"
 #include "MeshIO.hpp"
+#include "Png.hpp"
 
 using namespace Piper;
 using PiperFrontend::Json;
@@ -129,9 +130,21 @@ std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const cha
     for (size_t k = 0; k < v.size(); ++k) r[k] = num(v[k]);
     return r;
 }
-// An 8-bit netpbm image: P5 (grey), P6 (RGB), P7 (PAM, DEPTH 1..4). Three channels are widened
-// to four with alpha 255 (SampledTexture.cpp:143-151 asks stb_image for RGBA then).
+// An 8-bit PNG or netpbm image (P5 grey, P6 RGB, P7 PAM with DEPTH 1..4). Three channels are
+// widened to four with alpha 255 (SampledTexture.cpp:143-151 asks stb_image for RGBA then).
 std::shared_ptr<Texture> loadImageTexture(const fs::path& path, uint32_t wrap) {
+    std::string ext = path.extension().string();
+    for (char& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    if (ext == ".png") {
+        auto img = PiperFrontend::loadPng(path.string());
+        if (img.channels != 3) return Texture::image(img.width, img.height, img.channels, wrap, std::move(img.texels));
+        std::vector<uint8_t> rgba(static_cast<size_t>(img.width) * img.height * 4);
+        for (size_t i = 0; i < static_cast<size_t>(img.width) * img.height; ++i) {
+            std::memcpy(&rgba[4 * i], &img.texels[3 * i], 3);
+            rgba[4 * i + 3] = 255;
+        }
+        return Texture::image(img.width, img.height, 4, wrap, std::move(rgba));
+    }
     std::ifstream f(path, std::ios::binary);
     if (!f) fail("cannot open image " + path.string());
     auto token = [&]() {
@@ -167,7 +180,7 @@ std::shared_ptr<Texture> loadImageTexture(const fs::path& path, uint32_t wrap) {
             else if (t == "TUPLTYPE") (void)token();
         }
     } else {
-        fail("unsupported image format (binary PPM, PGM and PAM are read): " + path.string());
+        fail("unsupported image format (PNG, binary PPM, PGM and PAM are read): " + path.string());
     }
     if (maxval != 255 || w == 0 || h == 0 || depth < 1 || depth > 4) fail("unsupported image layout in " + path.string());
     std::vector<uint8_t> raw(static_cast<size_t>(w) * h * depth);

@@ -3,6 +3,7 @@ from a generated scene dict, plus readers for what piper_render writes (.pgsc ex
 import json
 import os
 import struct
+import zlib
 
 import numpy as np
 
@@ -66,17 +67,59 @@ def write_pam(path, texels):
         f.write(np.ascontiguousarray(texels, np.uint8).tobytes())
 
 
-def texture_config(t, folder, name):
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = np.abs(p - a), np.abs(p - b), np.abs(p - c)
+    return np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, b, c))
+
+
+def write_png(path, texels, level=6, rgb=False):
+    """8-bit PNG (PNG spec 1.2): row y uses scanline filter y % 5 (None, Sub, Up, Average, Paeth)
+    so every filter is decoded; level 0 makes stored deflate blocks, higher levels Huffman-coded ones.
+    rgb=True drops a 4-channel image's alpha (colour type 2)."""
+    img = np.ascontiguousarray(texels[..., :3] if rgb else texels, np.uint8).astype(np.int32)
+    h, w, c = img.shape
+    raw = bytearray()
+    for y in range(h):
+        cur = img[y].reshape(-1)
+        up = img[y - 1].reshape(-1) if y else np.zeros_like(cur)
+        left = np.concatenate([np.zeros(c, np.int32), cur[:-c]])
+        upleft = np.concatenate([np.zeros(c, np.int32), up[:-c]])
+        f = y % 5
+        pred = [np.zeros_like(cur), left, up, (left + up) // 2, _paeth(left, up, upleft)][f]
+        raw.append(f)
+        raw += ((cur - pred) & 0xFF).astype(np.uint8).tobytes()
+
+    def chunk(kind, body):
+        return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xFFFFFFFF)
+
+    colour = {1: 0, 2: 4, 3: 2, 4: 6}[c]
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, colour, 0, 0, 0)))
+        data = zlib.compress(bytes(raw), level)
+        half = len(data) // 2  # two IDAT chunks: the stream continues across them
+        f.write(chunk(b"IDAT", data[:half]) + chunk(b"IDAT", data[half:]) + chunk(b"IEND", b""))
+
+
+def texture_config(t, folder, name, image_format="pam"):
+    """A texture as a reference texture config; image_format "pam" or "png" (level 9 for colour,
+    0 for single-channel images, so both stored and Huffman-coded deflate blocks are read)."""
     if t["type"] == 1:
         ch = t["channels"]
         return {"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": list(t["black"][:ch]),
                 "White": list(t["white"][:ch]), "Scale": t["scale"]}
-    write_pam(os.path.join(folder, name), t["texels"])
+    if image_format == "png":
+        name = name.rsplit(".", 1)[0] + ".png"
+        texels = t["texels"]
+        rgb = texels.shape[2] == 4 and bool((texels[..., 3] == 255).all())
+        write_png(os.path.join(folder, name), texels, level=9 if texels.shape[2] > 1 else 0, rgb=rgb)
+    else:
+        write_pam(os.path.join(folder, name), t["texels"])
     return {"WrapMode": "Repeat" if t["wrap"] == 0 else "Mirror",
             "Image": {"ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": name}}
 
 
-def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transforms=None):
+def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transforms=None, image_format="pam"):
     """Write sc as scene.json + one OBJ per object + options.json under folder; returns the options
     path. transforms: optional {object index: Transform config} (the OBJ then holds the object's
     vertices as given, the transform places them)."""
@@ -97,7 +140,7 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
                             "rough": ("Roughness", "RoughnessX")}[key]
                     for k in keys + (("RoughnessY",) if key == "rough" else ()):
                         if k in surf:
-                            surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam")
+                            surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam", image_format)
                             if k != "RoughnessX":
                                 break
         inst = {"Type": "Instance", "Geometry": {"ClassID": "Piper.BuiltinComponent.TriangleMesh.TriangleMesh",

@@ -41,12 +41,26 @@ def test_export_reproduces_generated_scene(tmp_path):
     assert (e["width"], e["height"], e["spp"], e["max_depth"], e["min_rr"]) == (64, 48, 8, 5, 5)
 
 
-def test_textured_scene_export(tmp_path):
-    """CheckBoard and sampled-image colour inputs (PAM files, Repeat and Mirror) and OBJ texture
-    coordinates reach the flattened scene: the same corner coordinates and, per material input, the
-    same texture (the plugin numbers textures in first-use order)."""
+@pytest.mark.parametrize("image_format,opaque", [("pam", False), ("png", False), ("png", True)])
+def test_textured_scene_export(tmp_path, image_format, opaque):
+    """CheckBoard and sampled-image colour inputs (PAM or PNG files, Repeat and Mirror) and OBJ
+    texture coordinates reach the flattened scene: the same corner coordinates and, per material
+    input, the same texture (the plugin numbers textures in first-use order). PNG files cycle through
+    all five scanline filters and both stored and Huffman-coded deflate blocks; opaque colour images
+    go out as RGB PNGs and come back as RGBA with alpha 255, as the reference's loader asks for."""
     sc = scenes.cornell(width=32, height=32, spp=4, plastic=True, textured=True)
-    e = export(sc, tmp_path)
+    if image_format == "png":  # a larger smooth image (dynamic Huffman codes), a flat one (fixed codes)
+        y, x = np.mgrid[0:48, 0:64]
+        sc["textures"][1].update(width=64, height=48, texels=np.stack(
+            [(x * 4) % 256, (y * 5) % 256, ((x + y) * 3) % 256, (x * y) % 256], -1).astype(np.uint8))
+        sc["textures"][2]["texels"] = np.full((5, 7, 4), 77, np.uint8)
+    if opaque:
+        for t in sc["textures"]:
+            if t["type"] == 2 and t["channels"] == 4:
+                t["texels"][..., 3] = 255
+    e = export(sc, tmp_path, image_format=image_format)
+    if image_format == "png":
+        assert {_first_deflate_block(p) for p in tmp_path.glob("*.png")} == {0, 1, 2}
     assert np.array_equal(e["vertices"][e["indices"]], sc["vertices"][sc["indices"]])
     assert np.array_equal(e["texcoords"][e["indices"]], sc["texcoords"][sc["indices"]])
     for m in range(len(sc["materials"])):
@@ -62,6 +76,17 @@ def test_textured_scene_export(tmp_path):
                 assert ta["scale"] == tb["scale"]
             else:
                 assert np.array_equal(ta["texels"], tb["texels"]) and ta["wrap"] == tb["wrap"]
+
+
+def _first_deflate_block(path):
+    """BTYPE of the first deflate block of a PNG's zlib stream (0 stored, 1 fixed, 2 dynamic)."""
+    d, p, z = path.read_bytes(), 8, b""
+    while p < len(d):
+        n = int.from_bytes(d[p:p + 4], "big")
+        if d[p + 4:p + 8] == b"IDAT":
+            z += d[p + 8:p + 8 + n]
+        p += 12 + n
+    return (z[2] >> 1) & 3
 
 
 def test_transform_srt_matches_scipy(tmp_path):
