@@ -381,7 +381,9 @@ pg::Queues queues(pg_device* dev) {
 }
 
 uint32_t traversalLanes(pg_device* dev, uint64_t work) {
-    const uint64_t full = static_cast<uint64_t>(dev->numCUs) * 16 * 64;  // 16 waves per CU
+    // the most lanes a traversal launch can take; each launch trims its grid to what the device
+    // holds of that kernel at once
+    const uint64_t full = static_cast<uint64_t>(dev->numCUs) * pg::kTravWavesPerCuMax * 64;
     const uint64_t need = (work + 63) / 64 * 64;
     return static_cast<uint32_t>(std::max<uint64_t>(64, std::min(full, need)));
 }

@@ -7,6 +7,10 @@
 // atomic per wave (ballot + popcount prefix). The per-bounce structure restates trace() and
 // multipleImportanceSampling() of Integrator/PathTracing/Kernel.cpp:23-193; the per-sample driver
 // restates piperMain (Tracer/Embree/Kernel.cpp:47-100).
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "pg_device.hpp"
 #include "pg_guide.hpp"
 #include "pg_kernels.hpp"
@@ -897,32 +901,60 @@ void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Q
     else hipLaunchKernelGGL(kRegen<false>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, BatchTable{});
 }
 
+// Persistent traversal grids hold at most the waves the device keeps resident at once for the
+// kernel (its occupancy, set by VGPRs or the LDS stack): a larger grid leaves a tail of waves that
+// start only as others end, which on a 1920x1080 64-spp frame cost 7-12 ms of ~91. gridLanes
+// (the lanes the spill region is sized for) stays the upper bound.
+static dim3 residentGrid(const void* kernel, uint32_t gridLanes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, uint32_t> cache;
+    uint32_t waves = gridLanes / 64;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return dim3(waves);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, kernel});
+    if (it == cache.end()) {
+        int cus = 0, blocks = 0;
+        uint32_t resident = 0;  // 0: unknown, keep the caller's grid
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 64, 0) == hipSuccess && cus > 0 && blocks > 0)
+            resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
+        it = cache.emplace(std::make_pair(dev, kernel), resident).first;
+    }
+    if (it->second) waves = std::min(waves, it->second);
+    return dim3(std::max(waves, 1u));
+}
+template <typename F>
+static dim3 fit(F* kernel, uint32_t gridLanes) {
+    return residentGrid(reinterpret_cast<const void*>(kernel), gridLanes);
+}
+
 template <int W>
 static void extendW(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
-                    const uint32_t* count, dim3 grid, uint2* spill, uint32_t spillPerLane, unsigned long long* tc,
-                    const BatchTable* bt) {
+                    const uint32_t* count, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane,
+                    unsigned long long* tc, const BatchTable* bt) {
+    const dim3 b(64);
     if (bt) {
-        if (tc) hipLaunchKernelGGL((kExtend<W, true, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
-                                   spillPerLane, tc, *bt);
-        else hipLaunchKernelGGL((kExtend<W, false, true>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
-                                spillPerLane, tc, *bt);
+        if (tc) hipLaunchKernelGGL((kExtend<W, true, true>), fit(kExtend<W, true, true>, gridLanes), b, 0, st, s, ps,
+                                   queue, count, spill, spillPerLane, tc, *bt);
+        else hipLaunchKernelGGL((kExtend<W, false, true>), fit(kExtend<W, false, true>, gridLanes), b, 0, st, s, ps,
+                                queue, count, spill, spillPerLane, tc, *bt);
     } else {
-        if (tc) hipLaunchKernelGGL((kExtend<W, true, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
-                                   spillPerLane, tc, BatchTable{});
-        else hipLaunchKernelGGL((kExtend<W, false, false>), grid, dim3(64), 0, st, s, ps, queue, count, spill,
-                                spillPerLane, tc, BatchTable{});
+        if (tc) hipLaunchKernelGGL((kExtend<W, true, false>), fit(kExtend<W, true, false>, gridLanes), b, 0, st, s, ps,
+                                   queue, count, spill, spillPerLane, tc, BatchTable{});
+        else hipLaunchKernelGGL((kExtend<W, false, false>), fit(kExtend<W, false, false>, gridLanes), b, 0, st, s, ps,
+                                queue, count, spill, spillPerLane, tc, BatchTable{});
     }
 }
 
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
                   const BatchTable* bt) {
-    const dim3 grid(gridLanes / 64);
     auto* tc = reinterpret_cast<unsigned long long*>(travCounts);
     const uint32_t* queue = q.ext[curSlot];
     const uint32_t* count = q.counts + curSlot;
-    if (s.width == 8) extendW<8>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
-    else extendW<4>(st, s, ps, queue, count, grid, spill, spillPerLane, tc, bt);
+    if (s.width == 8) extendW<8>(st, s, ps, queue, count, gridLanes, spill, spillPerLane, tc, bt);
+    else extendW<4>(st, s, ps, queue, count, gridLanes, spill, spillPerLane, tc, bt);
 }
 template <bool TEX>
 void shadeVariant(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
@@ -950,26 +982,32 @@ void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Q
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
-    const dim3 grid(gridLanes / 64);
     const BatchTable b = bt ? *bt : BatchTable{};
     if (s.width == 8) {
-        if (bt) hipLaunchKernelGGL((kShadow<8, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kShadow<8, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        if (bt) hipLaunchKernelGGL((kShadow<8, true>), fit(kShadow<8, true>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kShadow<8, false>), fit(kShadow<8, false>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
     } else {
-        if (bt) hipLaunchKernelGGL((kShadow<4, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kShadow<4, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        if (bt) hipLaunchKernelGGL((kShadow<4, true>), fit(kShadow<4, true>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kShadow<4, false>), fit(kShadow<4, false>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
     }
 }
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
-    const dim3 grid(gridLanes / 64);
     const BatchTable b = bt ? *bt : BatchTable{};
     if (s.width == 8) {
-        if (bt) hipLaunchKernelGGL((kMis<8, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kMis<8, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        if (bt) hipLaunchKernelGGL((kMis<8, true>), fit(kMis<8, true>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kMis<8, false>), fit(kMis<8, false>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
     } else {
-        if (bt) hipLaunchKernelGGL((kMis<4, true>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kMis<4, false>), grid, dim3(64), 0, st, s, ps, q, spill, spillPerLane, b);
+        if (bt) hipLaunchKernelGGL((kMis<4, true>), fit(kMis<4, true>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
+        else hipLaunchKernelGGL((kMis<4, false>), fit(kMis<4, false>, gridLanes), dim3(64), 0, st,
+                s, ps, q, spill, spillPerLane, b);
     }
 }
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
@@ -1014,11 +1052,12 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
 }
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane) {
-    const dim3 grid(gridLanes / 64);
     if (s.width == 8)
-        hipLaunchKernelGGL(kTrace<8>, grid, dim3(64), 0, st, s, rays, n, t, prim, uv, occ, spill, spillPerLane);
+        hipLaunchKernelGGL(kTrace<8>, fit(kTrace<8>, gridLanes), dim3(64), 0, st,
+                s, rays, n, t, prim, uv, occ, spill, spillPerLane);
     else
-        hipLaunchKernelGGL(kTrace<4>, grid, dim3(64), 0, st, s, rays, n, t, prim, uv, occ, spill, spillPerLane);
+        hipLaunchKernelGGL(kTrace<4>, fit(kTrace<4>, gridLanes), dim3(64), 0, st,
+                s, rays, n, t, prim, uv, occ, spill, spillPerLane);
 }
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film) {

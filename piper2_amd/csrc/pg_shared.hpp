@@ -20,7 +20,12 @@ constexpr uint32_t kLeafCountShift = 27;
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 constexpr uint32_t kMaxLeafSize = 8;
 // Traversal stack entries per lane kept in LDS; deeper entries spill to a per-lane HBM region.
-constexpr int kLdsStack = 16;
+// 10 entries (5 KB per 64-lane wave) let 32 waves share a CU's 160 KB, so registers, not LDS,
+// bound the traversal kernels' occupancy (16 entries held them to 20 waves per CU).
+constexpr int kLdsStack = 10;
+// Upper bound on the resident waves per CU of a traversal kernel (the LDS limit above); the
+// per-lane spill region is sized for this many lanes.
+constexpr uint32_t kTravWavesPerCuMax = 32;
 
 constexpr uint32_t kHitMiss = 0xFFFFFFFFu;
 constexpr uint32_t kHitQuad = 0x80000000u;
