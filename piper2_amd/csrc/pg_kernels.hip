@@ -888,27 +888,14 @@ static inline uint32_t blocksFor(uint64_t n, uint32_t bs, uint32_t cap) {
     return static_cast<uint32_t>(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity) {
-    hipLaunchKernelGGL(kPoolInit, dim3(blocksFor(capacity, 256, 1u << 30)), dim3(256), 0, st, q, capacity);
-}
-void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot) {
-    hipLaunchKernelGGL(kIterBegin, dim3(1), dim3(1), 0, st, q, curSlot);
-}
-void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                 uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt) {
-    const dim3 grid(gridThreads / 256);
-    if (bt) hipLaunchKernelGGL(kRegen<true>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, *bt);
-    else hipLaunchKernelGGL(kRegen<false>, grid, dim3(256), 0, st, s, ps, q, rc, curSlot, BatchTable{});
-}
-
-// Persistent traversal grids hold at most the waves the device keeps resident at once for the
-// kernel (its occupancy, set by VGPRs or the LDS stack): a larger grid leaves a tail of waves that
-// start only as others end, which on a 1920x1080 64-spp frame cost 7-12 ms of ~91. gridLanes
-// (the lanes the spill region is sized for) stays the upper bound.
-static dim3 residentGrid(const void* kernel, uint32_t gridLanes) {
+// Persistent (grid-stride) kernels launch at most the blocks the device keeps resident at once
+// for the kernel (its occupancy, set by VGPRs or LDS): a larger grid leaves a tail of blocks that
+// start only as others end, which for traversal on a 1920x1080 64-spp frame cost 7-12 ms of ~91.
+// gridLanes (for traversal: the lanes the spill region is sized for) stays the upper bound.
+static dim3 residentGrid(const void* kernel, uint32_t gridLanes, uint32_t blockSize) {
     static std::mutex mu;
     static std::map<std::pair<int, const void*>, uint32_t> cache;
-    uint32_t waves = gridLanes / 64;
+    uint32_t waves = gridLanes / blockSize;  // blocks
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return dim3(waves);
     std::lock_guard<std::mutex> lock(mu);
@@ -917,7 +904,8 @@ static dim3 residentGrid(const void* kernel, uint32_t gridLanes) {
         int cus = 0, blocks = 0;
         uint32_t resident = 0;  // 0: unknown, keep the caller's grid
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 64, 0) == hipSuccess && cus > 0 && blocks > 0)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, static_cast<int>(blockSize), 0) == hipSuccess &&
+            cus > 0 && blocks > 0)
             resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
         it = cache.emplace(std::make_pair(dev, kernel), resident).first;
     }
@@ -925,8 +913,22 @@ static dim3 residentGrid(const void* kernel, uint32_t gridLanes) {
     return dim3(std::max(waves, 1u));
 }
 template <typename F>
-static dim3 fit(F* kernel, uint32_t gridLanes) {
-    return residentGrid(reinterpret_cast<const void*>(kernel), gridLanes);
+static dim3 fit(F* kernel, uint32_t gridLanes, uint32_t blockSize = 64) {
+    return residentGrid(reinterpret_cast<const void*>(kernel), gridLanes, blockSize);
+}
+
+void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity) {
+    hipLaunchKernelGGL(kPoolInit, dim3(blocksFor(capacity, 256, 1u << 30)), dim3(256), 0, st, q, capacity);
+}
+void launchIterBegin(hipStream_t st, const Queues& q, uint32_t curSlot) {
+    hipLaunchKernelGGL(kIterBegin, dim3(1), dim3(1), 0, st, q, curSlot);
+}
+void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
+                 uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt) {
+    if (bt) hipLaunchKernelGGL(kRegen<true>, fit(kRegen<true>, gridThreads, 256), dim3(256), 0, st,
+            s, ps, q, rc, curSlot, *bt);
+    else hipLaunchKernelGGL(kRegen<false>, fit(kRegen<false>, gridThreads, 256), dim3(256), 0, st,
+            s, ps, q, rc, curSlot, BatchTable{});
 }
 
 template <int W>
@@ -960,18 +962,21 @@ template <bool TEX>
 void shadeVariant(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                   uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
                   const BatchTable* bt) {
-    const dim3 grid(gridThreads / 256);
     if (bt && gs.enabled)  // guided batch: per-item guides in bt.guide
-        hipLaunchKernelGGL((kShade<true, true, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+        hipLaunchKernelGGL((kShade<true, true, TEX>), fit(kShade<true, true, TEX>, gridThreads, 256), dim3(256), 0, st,
+                s, ps, q, rc, curSlot, nextSlot, totals,
                            gs, *bt);
     else if (bt)
-        hipLaunchKernelGGL((kShade<true, false, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+        hipLaunchKernelGGL((kShade<true, false, TEX>), fit(kShade<true, false, TEX>, gridThreads, 256), dim3(256), 0, st,
+                s, ps, q, rc, curSlot, nextSlot, totals,
                            gs, *bt);
     else if (gs.enabled)
-        hipLaunchKernelGGL((kShade<false, true, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+        hipLaunchKernelGGL((kShade<false, true, TEX>), fit(kShade<false, true, TEX>, gridThreads, 256), dim3(256), 0, st,
+                s, ps, q, rc, curSlot, nextSlot, totals,
                            gs, BatchTable{});
     else
-        hipLaunchKernelGGL((kShade<false, false, TEX>), grid, dim3(256), 0, st, s, ps, q, rc, curSlot, nextSlot, totals,
+        hipLaunchKernelGGL((kShade<false, false, TEX>), fit(kShade<false, false, TEX>, gridThreads, 256), dim3(256), 0, st,
+                s, ps, q, rc, curSlot, nextSlot, totals,
                            gs, BatchTable{});
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
@@ -1012,9 +1017,10 @@ void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Que
 }
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
                   const RecordRing& ring, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt) {
-    const dim3 grid(gridThreads / 256);
-    if (bt) hipLaunchKernelGGL(kRecord<true>, grid, dim3(256), 0, st, ps, q, rc, ring, gs, tr, *bt);
-    else hipLaunchKernelGGL(kRecord<false>, grid, dim3(256), 0, st, ps, q, rc, ring, gs, tr, BatchTable{});
+    if (bt) hipLaunchKernelGGL(kRecord<true>, fit(kRecord<true>, gridThreads, 256), dim3(256), 0, st,
+            ps, q, rc, ring, gs, tr, *bt);
+    else hipLaunchKernelGGL(kRecord<false>, fit(kRecord<false>, gridThreads, 256), dim3(256), 0, st,
+            ps, q, rc, ring, gs, tr, BatchTable{});
 }
 void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& ring, uint64_t first, void* quad,
                    float* film32, double* film64) {
