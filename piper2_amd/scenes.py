@@ -436,6 +436,87 @@ def hidden_cornell(width=64, height=64, spp=64, max_depth=6):
                     min_rr=max_depth, ndc=sc["ndc"], name="hidden_cornell")
 
 
+def sphere_furnace(rho=0.7, intensity=(2.0, 2.0, 2.0), radius=1.0, subdiv=4, width=32, height=32, spp=64,
+                   max_depth=3):
+    """Closed diffuse sphere (an icosphere mesh, albedo rho) with one point light at its centre and
+    a black environment; the camera sits inside, off centre. Every surface point receives the
+    direct irradiance E0 = I / R^2, and inside a sphere any uniform radiosity B gives every point
+    the irradiance B again (the form factor to an area element is dA / (4 pi R^2) from anywhere on
+    the sphere), so each bounce multiplies by rho: with max_depth D (D vertices take a light sample,
+    PathTracing/Kernel.cpp:148) every camera ray carries (E0 / pi) sum_{k=1..D} rho^k. Russian
+    roulette is off (min_rr = max_depth)."""
+    v, f = icosphere(subdiv)
+    b = _Builder()
+    b.add(v * radius, f, b.material(LAMBERT, [rho] * 3))
+    cam = make_camera([0.15 * radius, -0.1 * radius, 0.3 * radius], [-0.2 * radius, 0.1 * radius, -radius],
+                      [0, 1, 0], 70.0, width / height)
+    ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
+    sc = b.finish(np.zeros((0, 12), np.float32), cam, width=width, height=height, spp=spp, max_depth=max_depth,
+                  min_rr=max_depth, ndc=ndc, name="sphere_furnace")
+    sc["point_lights"] = np.array([[0.0, 0.0, 0.0, *intensity]], np.float32)
+    return sc
+
+
+def sphere_furnace_radiance(rho, intensity, radius, max_depth):
+    """Closed-form radiance of every camera ray in sphere_furnace (per channel)."""
+    e0 = np.asarray(intensity, np.float64) / (radius * radius)
+    return e0 / math.pi * sum(rho ** k for k in range(1, max_depth + 1))
+
+
+def triangle_targets(seed=0, n_tris=16, rays_per_tri=64):
+    """Isolated random triangles (one every 12 units along x) and rays aimed at points of known
+    barycentrics: returns (scene, rays (N, 8), expected t (N,), expected (u, v) (N, 2), triangle
+    index (N,)). The target point is (1 - u - v) v0 + u v1 + v v2 (Embree's convention: u weighs
+    v1 and v weighs v2, Tracer/Embree/Kernel.cpp:176-178); the ray starts a distance t0 before it
+    along a random direction scaled by s in {0.5, 1, 2}, so the expected hit distance in ray
+    parameter units is t0 / s (the expected values are solved again in FP64 for the rays as rounded
+    to float32). Directions of every sign on every dominant axis occur."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    tris = []
+    for i in range(n_tris):
+        c = np.array([12.0 * i, 0.0, 0.0])
+        tv = c + rng.uniform(-1.5, 1.5, (3, 3))
+        while np.linalg.norm(np.cross(tv[1] - tv[0], tv[2] - tv[0])) < 0.5:
+            tv = c + rng.uniform(-1.5, 1.5, (3, 3))
+        b.add(tv, [[0, 1, 2]], b.material(LAMBERT, [0.5, 0.5, 0.5]))
+        tris.append(tv.astype(np.float32).astype(np.float64))
+    rays, ts, uvs, idx = [], [], [], []
+    for i, tv in enumerate(tris):
+        nrm = np.cross(tv[1] - tv[0], tv[2] - tv[0])
+        nrm /= np.linalg.norm(nrm)
+        for _ in range(rays_per_tri):
+            while True:
+                u, v = rng.uniform(0.05, 0.9, 2)
+                if u + v < 0.95:
+                    break
+            p = (1 - u - v) * tv[0] + u * tv[1] + v * tv[2]
+            while True:
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                if abs(np.dot(d, nrm)) > 0.2:
+                    break
+            t0 = rng.uniform(0.5, 5.0)
+            s = [0.5, 1.0, 2.0][int(rng.integers(3))]
+            o = p - d * t0
+            rays.append([*o, *(d * s), 1e-3, 1e5])
+            ts.append(t0 / s)
+            uvs.append([u, v])
+            idx.append(i)
+    rays = np.array(rays, np.float32)
+    # the float32 rays hit slightly off the designed points: solve o + t d = (1-u-v) v0 + u v1 + v v2
+    # exactly (FP64) for the stored rays
+    for k, i in enumerate(idx):
+        tv = tris[i]
+        o, d = rays[k, :3].astype(np.float64), rays[k, 3:6].astype(np.float64)
+        a = np.stack([d, tv[0] - tv[1], tv[0] - tv[2]], 1)
+        ts[k], uvs[k][0], uvs[k][1] = np.linalg.solve(a, tv[0] - o)
+    cam = make_camera([0, 0, 20], [0, 0, 0], [0, 1, 0], 40.0, 1.0)
+    sc = b.finish(np.zeros((0, 12), np.float32), cam, width=8, height=8, spp=1, max_depth=1, min_rr=1,
+                  ndc=(0.0, 0.0, 1.0, 1.0), name="triangle_targets")
+    return sc, rays, np.array(ts), np.array(uvs), np.array(idx, np.uint32)
+
+
 def world_triangles(sc):
     """(T, 3, 3) float32 world-space triangle vertices."""
     return sc["vertices"][sc["indices"]]

@@ -123,6 +123,26 @@ def test_adam_step_matches_oracle(dev):
     assert np.allclose(th1, ref, atol=2e-6, rtol=1e-5)
 
 
+def test_adam_steps_carry_moments_and_bias_correction(dev):
+    """Five Adam steps on different batches: the GPU parameters follow the oracle's Adam applied to
+    the GPU's own gradients with the moments carried over and the bias correction of each step
+    (a missing correction is invisible at step 1 but moves step 2 by 35%)."""
+    rng = np.random.default_rng(14)
+    g = pg.Guide(dev, seed=14, lr=1e-2, train_batch=1024)
+    th = g.get_params().astype(np.float64)
+    m, v = np.zeros_like(th), np.zeros_like(th)
+    n = 1024
+    for t in range(1, 6):
+        p01, nrm, wo = _inputs(rng, n)
+        rec = pg.Guide.records(p01, nrm, wo, _unit(rng, n), rng.uniform(0.1, 1.0, n).astype(np.float32))
+        _, grad = g.gradients(rec)
+        g.train(rec)
+        th, m, v = go.adam(th, grad.astype(np.float64), m, v, t, lr=1e-2, b1=0.9, b2=0.99, eps=1e-8)
+        got = g.get_params().astype(np.float64)
+        assert np.allclose(got, th, atol=1e-5, rtol=1e-5), (t, np.abs(got - th).max())
+        th = got  # continue from the GPU's FP32 parameters
+
+
 def test_training_learns_a_directional_target(dev):
     """Radiance arriving from one position-dependent direction: the fitted mixture must put far
     more density there than a uniform sphere (1 / 4 pi) after training."""

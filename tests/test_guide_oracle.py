@@ -95,3 +95,23 @@ def test_loss_gradient_matches_finite_differences():
         tm[k] -= h
         fd = (go.loss(tp, feats, w, c, quant=False) - go.loss(tm, feats, w, c, quant=False)) / (2 * h)
         assert abs(fd - grad[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, grad[k])
+
+
+def test_adam_constant_gradient_closed_form():
+    """With a constant gradient g, bias-corrected Adam moves every parameter by lr g / (|g| + eps)
+    per step (m_t / (1 - b1^t) = g and v_t / (1 - b2^t) = g^2 exactly); without the correction the
+    step t would be lr (1 - b1^t) / sqrt(1 - b2^t) sign(g), which equals lr at t = 1 (b1 = 0.9,
+    b2 = 0.99) but 1.35 lr at t = 2 and 1.62 lr at t = 3, so five steps pin the correction."""
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=257) * np.logspace(-3, 1, 257)
+    theta = rng.normal(size=257)
+    m, v = np.zeros_like(g), np.zeros_like(g)
+    th = theta.copy()
+    lr, eps = 1e-2, 1e-8
+    for t in range(1, 6):
+        th, m, v = go.adam(th, g, m, v, t, lr=lr, b1=0.9, b2=0.99, eps=eps)
+        expect = theta - t * lr * g / (np.abs(g) + eps)
+        assert np.allclose(th, expect, rtol=0, atol=1e-12), t
+    # the uncorrected sequence differs from step 2 on
+    unc = sum((1 - 0.9 ** t) / np.sqrt(1 - 0.99 ** t) for t in range(1, 6))
+    assert abs(unc - 5.0) > 1.0

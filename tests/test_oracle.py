@@ -717,3 +717,35 @@ def test_film_filters_partition_and_gaussian_profile():
     g1 = np.maximum(np.exp(-2.0 * d * d) - np.exp(-2.0), 0.0)
     t1 = np.maximum(0.0, 1.0 - np.abs(d))
     assert abs(ratio - (g1.mean() / t1.mean()) ** 2) < 0.05 * (g1.mean() / t1.mean()) ** 2
+
+
+def test_triangle_hit_distance_and_barycentrics_closed_form():
+    """Rays aimed at points of known barycentrics on isolated triangles: the hit distance is the
+    FP64 solution of o + t d = (1 - u - v) v0 + u v1 + v v2 in units of the (non-unit) direction,
+    and the returned (u, v) weigh v1 and v2 as Embree reports them (Tracer/Embree/Kernel.cpp:176-178).
+    A t scaled by |d|, or u and v swapped, fails this."""
+    sc, rays, t_exp, uv_exp, idx = scenes.triangle_targets(seed=11)
+    o = OracleScene(sc)
+    for brute in (False, True):
+        t, prim, uv = o.trace(rays, brute=brute)
+        assert np.array_equal(prim, idx)
+        assert np.abs(t / t_exp - 1.0).max() < 1e-5
+        assert np.abs(uv - uv_exp).max() < 1e-5
+    # the swapped reading is far off (designed u, v differ by more than 1e-3 on almost every ray)
+    assert np.mean(np.abs(uv[:, ::-1] - uv_exp).max(1) > 1e-3) > 0.95
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+def test_sphere_furnace_multibounce_closed_form(depth):
+    """Multi-bounce diffuse transport against a closed form: inside a closed diffuse sphere with a
+    point light at its centre every camera ray carries (E0 / pi) sum_{k=1..D} rho^k (the vertices at
+    depths 0..D-1 take a light sample, PathTracing/Kernel.cpp:148). With rho = 0.7 a depth off by
+    one moves the mean by 9% or more; a missed throughput update by 30% or more."""
+    rho, inten = 0.7, (2.0, 1.5, 1.0)
+    sc = scenes.sphere_furnace(rho=rho, intensity=inten, max_depth=depth, width=64, height=64, spp=64)
+    pl = OracleScene(sc).render(path_l=True, film=False)["path_l"].astype(np.float64)
+    expect = scenes.sphere_furnace_radiance(rho, inten, 1.0, depth)
+    assert np.all(np.abs(pl.mean(0) / expect - 1.0) < 0.01), (pl.mean(0), expect)
+    # every vertex picks the point light (probability 1/2, weight 2) or the black environment, so
+    # a path's radiance is a sum of D terms that are 0 or close to 2 rho^(k+1) E0 / pi
+    assert pl.max() < 2.01 * expect.max() and pl.min() >= 0.0
