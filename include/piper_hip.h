@@ -237,6 +237,17 @@ int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample
 /* rays: float[8*n] = {o xyz, d xyz, tmin, tmax}; outputs t[n], prim[n], uv[2n] (host arrays). */
 int pg_trace(pg_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv);
 int pg_occluded(pg_scene* scene, const float* rays, uint32_t n, uint8_t* occluded);
+/* pg_trace without any allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). rays, t, prim and uv are DEVICE pointers on the scene's GPU, laid out as in
+ * pg_trace; the results equal pg_trace's. The traversal stack entries that overflow on-chip
+ * memory spill to a fixed 16 MiB array in device memory (one per GPU, held until the process
+ * ends) instead of a per-call buffer; a BVH so deep that one wave's spill exceeds it is rejected
+ * with PG_ERR_INVALID. stream: a hipStream_t of the scene's GPU, the call then only enqueues the
+ * work; NULL runs it on the device handle's stream and waits for it. All calls on one GPU share
+ * that array: calls that overlap in time on one GPU race. */
+int pg_trace_noalloc(pg_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
+                     void* stream);
 /* Camera rays for (px, py, sample) triples as piperMain makes them; rays as in pg_trace, film[2n]. */
 int pg_camera_rays(pg_scene* scene, const pg_render_params* params, const uint32_t* pix, uint32_t n,
                    float* rays, float* film);

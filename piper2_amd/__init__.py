@@ -141,6 +141,7 @@ SIGNATURES = {
     "pg_render_paths": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _VP, ctypes.POINTER(Stats)]),
     "pg_trace": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
     "pg_occluded": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "pg_trace_noalloc": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP, _VP, _VP]),
     "pg_camera_rays": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _U32, _VP, _VP]),
     "pg_sobol": (ctypes.c_int, [_VP, _U32, _U32, _VP, _U32, _VP, _VP]),
     "pg_sobol_matrices": (ctypes.c_int, [_VP]),

@@ -103,6 +103,33 @@ struct GraphExec {
     }
 };
 
+// Traversal scratch of pg_trace_noalloc: the stack entries that overflow a lane's LDS slice spill
+// here instead of into a per-call allocation. Each GPU the library runs on holds its own copy
+// from module load until the process ends. Host code never uses the array's name as an address:
+// it takes the device's copy once per device (scratchAddress, at pg_device_create) and passes
+// that pointer on. Every pg_trace_noalloc call on one GPU shares that GPU's copy, so calls that
+// overlap in time on one GPU race.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint2 scratch[kScratchBytes / sizeof(uint2)];
+
+constexpr int kMaxOrdinals = 64;
+std::mutex g_scratchMu;
+void* g_scratchPtr[kMaxOrdinals] = {};
+
+// The device's copy of `scratch`, looked up once per device; `ordinal` must be current.
+void* scratchAddress(int ordinal) {
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if (ordinal < 0 || ordinal >= kMaxOrdinals) throw PgError(PG_ERR_INVALID, "device ordinal beyond 63");
+    if (!g_scratchPtr[ordinal]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            throw PgError(PG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchPtr[ordinal] = ptr;
+    }
+    return g_scratchPtr[ordinal];
+}
+
 }  // namespace
 
 struct pg_device {
@@ -116,6 +143,7 @@ struct pg_device {
     std::mutex mu;
     bool injectFault = false;
     std::string platform;
+    void* scratch = nullptr;  // this GPU's copy of the fixed traversal scratch (pg_trace_noalloc)
     DevBuf sobol, sobolT, sobolB, sobolInv, sobolDelta;
     // render workspace
     uint32_t capacity = 0;
@@ -952,6 +980,7 @@ int pg_device_create(int ordinal, pg_device** out) {
         dev->numCUs = prop.multiProcessorCount;
         dev->platform = std::string("AMD HIP ") + prop.gcnArchName;
         PG_HIP(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+        dev->scratch = scratchAddress(ordinal);
         PG_HIP(hipHostMalloc(&dev->status, kPinnedBytes, hipHostMallocDefault));
         std::vector<uint32_t> m;
         std::vector<uint64_t> inv, delta;
@@ -1321,6 +1350,29 @@ int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* pri
         PG_HIP(hipMemcpy(t, dt.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
         PG_HIP(hipMemcpy(prim, dp.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
         PG_HIP(hipMemcpy(uv, duv.p, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int pg_trace_noalloc(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv, void* stream) {
+    return guarded([&] {
+        require(s && (n == 0 || (rays && t && prim && uv)), "null argument");
+        if (n == 0) return;
+        pg_device* dev = s->dev;
+        std::lock_guard<std::mutex> lock(dev->mu);
+        dev->checkFault();
+        PG_HIP(hipSetDevice(dev->ordinal));
+        const size_t perLane = static_cast<size_t>(s->spillPerLane) * sizeof(uint2);
+        const size_t fitLanes = kScratchBytes / perLane / 64 * 64;
+        if (fitLanes < 64)
+            throw PgError(PG_ERR_INVALID, "BVH too deep for the fixed traversal scratch: one wave needs " +
+                                              std::to_string(perLane * 64) + " bytes, the scratch holds " +
+                                              std::to_string(kScratchBytes));
+        const uint32_t lanes = static_cast<uint32_t>(std::min<size_t>(traversalLanes(dev, n), fitLanes));
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : dev->stream;
+        pg::launchTrace(st, s->ds, rays, n, t, prim, uv, nullptr, lanes, static_cast<uint2*>(dev->scratch),
+                        s->spillPerLane);
+        PG_HIP(hipGetLastError());
+        if (!stream) PG_HIP(hipStreamSynchronize(st));
     });
 }
 
