@@ -18,13 +18,25 @@ struct BitReader {
     const uint8_t* p;
     size_t n, pos = 0;
     uint32_t bit = 0;
-    uint32_t bits(uint32_t count) {  // LSB-first
+    // the next `count` stream bits (LSB-first) without consuming them; zeros past the end
+    uint32_t peek(uint32_t count) const {
         uint32_t v = 0;
-        for (uint32_t i = 0; i < count; ++i) {
-            if (pos >= n) throw std::runtime_error("PNG: truncated compressed data");
-            v |= static_cast<uint32_t>((p[pos] >> bit) & 1u) << i;
-            if (++bit == 8) bit = 0, ++pos;
+        size_t q = pos;
+        uint32_t b = bit;
+        for (uint32_t i = 0; i < count && q < n; ++i) {
+            v |= static_cast<uint32_t>((p[q] >> b) & 1u) << i;
+            if (++b == 8) b = 0, ++q;
         }
+        return v;
+    }
+    void skip(uint32_t count) {
+        const size_t at = pos * 8 + bit + count;
+        if (at > n * 8) throw std::runtime_error("PNG: truncated compressed data");
+        pos = at / 8, bit = static_cast<uint32_t>(at % 8);
+    }
+    uint32_t bits(uint32_t count) {
+        const uint32_t v = peek(count);
+        skip(count);
         return v;
     }
     void alignByte() {
@@ -32,36 +44,43 @@ struct BitReader {
     }
 };
 
-// canonical Huffman decoding table: counts per length and symbols in code order
+// Prefix code of RFC 1951 section 3.2.2, decoded through a table indexed by the next 15 stream
+// bits: codes of each length are consecutive integers in symbol order, starting where the
+// shorter lengths' codes end; a code is sent most significant bit first, so in the LSB-first
+// stream a code of length l occupies every table index whose low l bits are the code reversed.
 struct Huffman {
-    uint16_t count[16] = {};
-    std::vector<uint16_t> symbol;
+    static constexpr uint32_t kMaxBits = 15;
+    std::vector<uint32_t> table;  // symbol << 4 | code length; 0 where no code starts
     void build(const uint8_t* lengths, size_t n) {
-        symbol.assign(n, 0);
-        for (auto& c : count) c = 0;
-        for (size_t i = 0; i < n; ++i) ++count[lengths[i]];
-        count[0] = 0;
-        uint16_t offs[16] = {};
-        for (int l = 1; l < 16; ++l) offs[l] = static_cast<uint16_t>(offs[l - 1] + count[l - 1]);
-        for (size_t i = 0; i < n; ++i)
-            if (lengths[i]) symbol[offs[lengths[i]]++] = static_cast<uint16_t>(i);
+        uint32_t lengthCount[kMaxBits + 1] = {};
+        for (size_t i = 0; i < n; ++i) ++lengthCount[lengths[i]];
+        lengthCount[0] = 0;
+        uint32_t nextCode[kMaxBits + 1] = {};
+        for (uint32_t l = 1, code = 0; l <= kMaxBits; ++l) {
+            code = (code + lengthCount[l - 1]) << 1;
+            nextCode[l] = code;
+        }
+        table.assign(size_t(1) << kMaxBits, 0);
+        for (size_t sym = 0; sym < n; ++sym) {
+            const uint32_t l = lengths[sym];
+            if (!l) continue;
+            const uint32_t code = nextCode[l]++;
+            if (code >> l) throw std::runtime_error("PNG: over-subscribed code lengths");
+            uint32_t rev = 0;
+            for (uint32_t b = 0; b < l; ++b) rev |= ((code >> b) & 1u) << (l - 1 - b);
+            for (uint32_t i = rev; i < (1u << kMaxBits); i += 1u << l) table[i] = static_cast<uint32_t>(sym) << 4 | l;
+        }
     }
     uint32_t decode(BitReader& br) const {
-        int code = 0, first = 0, index = 0;
-        for (int l = 1; l < 16; ++l) {
-            code |= static_cast<int>(br.bits(1));
-            const int c = count[l];
-            if (code - c < first) return symbol[static_cast<size_t>(index + (code - first))];
-            index += c;
-            first += c;
-            first <<= 1;
-            code <<= 1;
-        }
-        throw std::runtime_error("PNG: bad Huffman code");
+        const uint32_t e = table[br.peek(kMaxBits)];
+        if (!e) throw std::runtime_error("PNG: bad Huffman code");
+        br.skip(e & 15u);
+        return e >> 4;
     }
 };
 
-inline std::vector<uint8_t> inflate(const std::vector<uint8_t>& z) {
+// maxOut: the decoded size the image header allows (anything larger is an error)
+inline std::vector<uint8_t> inflate(const std::vector<uint8_t>& z, size_t maxOut) {
     if (z.size() < 2 || (z[0] & 0x0F) != 8 || ((z[0] << 8) | z[1]) % 31 != 0 || (z[1] & 0x20))
         throw std::runtime_error("PNG: bad zlib header");
     BitReader br{z.data() + 2, z.size() - 2};
@@ -80,10 +99,13 @@ inline std::vector<uint8_t> inflate(const std::vector<uint8_t>& z) {
             br.alignByte();
             if (br.pos + 4 > br.n) throw std::runtime_error("PNG: truncated stored block");
             const uint32_t len = br.p[br.pos] | (br.p[br.pos + 1] << 8);
+            const uint32_t nlen = br.p[br.pos + 2] | (br.p[br.pos + 3] << 8);
+            if (nlen != (~len & 0xFFFFu)) throw std::runtime_error("PNG: stored block length check failed");
             br.pos += 4;
             if (br.pos + len > br.n) throw std::runtime_error("PNG: truncated stored block");
             out.insert(out.end(), br.p + br.pos, br.p + br.pos + len);
             br.pos += len;
+            if (out.size() > maxOut) throw std::runtime_error("PNG: more image data than the header allows");
             continue;
         }
         if (type == 3) throw std::runtime_error("PNG: bad block type");
@@ -135,8 +157,17 @@ inline std::vector<uint8_t> inflate(const std::vector<uint8_t>& z) {
                 const size_t from = out.size() - d;
                 for (uint32_t k = 0; k < len; ++k) out.push_back(out[from + k]);
             }
+            if (out.size() > maxOut) throw std::runtime_error("PNG: more image data than the header allows");
         }
     }
+    // RFC 1950: the Adler-32 checksum of the uncompressed data follows the last block
+    br.alignByte();
+    if (br.pos + 4 > br.n) throw std::runtime_error("PNG: missing zlib checksum");
+    uint32_t a = 1, b = 0;
+    for (const uint8_t v : out) a = (a + v) % 65521u, b = (b + a) % 65521u;
+    const uint32_t want = (uint32_t(br.p[br.pos]) << 24) | (uint32_t(br.p[br.pos + 1]) << 16) |
+                          (uint32_t(br.p[br.pos + 2]) << 8) | br.p[br.pos + 3];
+    if (want != (b << 16 | a)) throw std::runtime_error("PNG: zlib checksum mismatch");
     return out;
 }
 
@@ -157,15 +188,19 @@ inline DecodedImage loadPng(const std::string& path) {
     if (d.size() < 8 || !std::equal(sig, sig + 8, d.begin())) throw std::runtime_error("not a PNG file: " + path);
     DecodedImage img;
     uint32_t depth = 0, colour = 0, interlace = 0;
+    bool header = false;
     std::vector<uint8_t> idat;
     for (size_t p = 8; p + 12 <= d.size();) {
         const uint32_t len = png_detail::be32(&d[p]);
         if (p + 12 + static_cast<size_t>(len) > d.size()) throw std::runtime_error("truncated PNG chunk in " + path);
         const std::string type(reinterpret_cast<const char*>(&d[p + 4]), 4);
         const uint8_t* body = &d[p + 8];
+        if ((type == "IHDR") == header) throw std::runtime_error("PNG: IHDR must be the first chunk and appear once: " + path);
         if (type == "IHDR") {
+            if (len != 13) throw std::runtime_error("PNG: IHDR of " + std::to_string(len) + " bytes in " + path);
             img.width = png_detail::be32(body), img.height = png_detail::be32(body + 4);
             depth = body[8], colour = body[9], interlace = body[12];
+            header = true;
         } else if (type == "IDAT") {
             idat.insert(idat.end(), body, body + len);
         } else if (type == "IEND") {
@@ -174,10 +209,17 @@ inline DecodedImage loadPng(const std::string& path) {
         p += 12 + len;
     }
     const uint32_t ch = colour == 0 ? 1 : colour == 4 ? 2 : colour == 2 ? 3 : colour == 6 ? 4 : 0;
+    if (!header) throw std::runtime_error("PNG without IHDR: " + path);
     if (depth != 8 || ch == 0 || interlace != 0 || img.width == 0 || img.height == 0)
         throw std::runtime_error("unsupported PNG (8-bit grey / grey-alpha / RGB / RGBA, not interlaced): " + path);
-    const std::vector<uint8_t> raw = png_detail::inflate(idat);
+    // the PNG limit is 2^31 - 1 per side; the texel count is held to 2^28 so no size below wraps
+    constexpr uint64_t kMaxTexelBytes = uint64_t(1) << 30;
+    if (img.width > 0x7FFFFFFFu || img.height > 0x7FFFFFFFu ||
+        static_cast<uint64_t>(img.width) * img.height * ch > kMaxTexelBytes)
+        throw std::runtime_error("PNG image too large (" + std::to_string(img.width) + " x " +
+                                 std::to_string(img.height) + "): " + path);
     const size_t stride = static_cast<size_t>(img.width) * ch;
+    const std::vector<uint8_t> raw = png_detail::inflate(idat, (stride + 1) * img.height);
     if (raw.size() < (stride + 1) * img.height) throw std::runtime_error("PNG: too little image data in " + path);
     img.channels = ch;
     img.texels.assign(stride * img.height, 0);

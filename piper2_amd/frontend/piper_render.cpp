@@ -115,6 +115,13 @@ bool sameJson(const Json& a, const Json& b) {
     }
     return false;
 }
+// Tracer::generateTexture (Tracer/Embree/Embree.cpp:1423-1432): a texture must have exactly the
+// channel count its input asks for (colours 4, roughness 1; Classic.cpp:89-195).
+void requireChannels(size_t got, size_t want, const char* what, const Json& cfg) {
+    if (got != want)
+        fail(std::string(what) + ": Mismatched channel. Expect " + std::to_string(want) + ", but get " +
+             std::to_string(got) + cfg.where());
+}
 std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const char* what) {
     if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
     if (!cfg.find("ClassID")) fail(std::string(what) + ": sampled image textures are not supported" + cfg.where());
@@ -124,8 +131,7 @@ std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const cha
     const auto& v = cfg.at("Value").array();
     if (v.size() != 1 && v.size() != 2 && v.size() != 4)
         fail(std::string(what) + ": unsupported channel count " + std::to_string(v.size()) + cfg.where());
-    if (v.size() < channels)
-        fail(std::string(what) + ": needs " + std::to_string(channels) + " channels" + cfg.where());
+    requireChannels(v.size(), channels, what, cfg);
     std::array<float, 4> r{};
     for (size_t k = 0; k < v.size(); ++k) r[k] = num(v[k]);
     return r;
@@ -211,7 +217,7 @@ ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base,
         fs::path p = img.at("Path").string();
         if (p.is_relative()) p = base / p;
         auto t = loadImageTexture(p, wrap == "Repeat" ? PG_WRAP_REPEAT : PG_WRAP_MIRROR);
-        if (t->desc.channels < channels) fail(std::string(what) + ": a colour image needs RGB or RGBA samples" + cfg.where());
+        requireChannels(t->desc.channels, channels, what, cfg);  // RGB was widened to RGBA
         return {{}, t};
     }
     const auto [m, c] = classOf(cfg);
@@ -221,7 +227,7 @@ ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base,
         if (b.size() != w.size()) fail("Channel of black and white must be identical." + cfg.where());
         if (b.size() != 1 && b.size() != 2 && b.size() != 4)
             fail(std::string(what) + ": unsupported channel count " + std::to_string(b.size()) + cfg.where());
-        if (b.size() < channels) fail(std::string(what) + ": a colour checkerboard needs 4 channels" + cfg.where());
+        requireChannels(b.size(), channels, what, cfg);
         std::array<float, 4> black{}, white{};
         for (size_t k = 0; k < b.size(); ++k) black[k] = num(b[k]), white[k] = num(w[k]);
         return {{}, Texture::checkerboard(black, white, num(cfg.at("Scale")), static_cast<uint32_t>(b.size()))};
@@ -346,18 +352,22 @@ private:
             const auto mesh = PiperFrontend::loadMesh(p.string());
             return meshGeometry(mesh.verts, mesh.index, mesh.uv);
         }
-        if (m == "Piper.BuiltinComponent.BasicShapes" && c == "Plane") {  // a plane as a surface: two triangles
-            std::vector<float> verts;
+        if (m == "Piper.BuiltinComponent.BasicShapes" && c == "Plane") {
+            // a plane as a surface: two triangles whose corners carry the parallelogram coordinates
+            // (0,0), (1,0), (1,1), (0,1), so the interpolated texture coordinate of a hit is its
+            // (u, v) in the plane (calcTexCoord, BasicShapes/Kernel.cpp:30-56 and :109)
+            std::vector<float> verts, uv;
             std::vector<uint32_t> index;
             for (const auto& p : planes(cfg)) {
                 const uint32_t b = static_cast<uint32_t>(verts.size() / 3);
                 for (int corner = 0; corner < 4; ++corner) {
                     const float a = (corner == 1 || corner == 2) ? 1.0f : 0.0f, d = corner >= 2 ? 1.0f : 0.0f;
                     for (int k = 0; k < 3; ++k) verts.push_back(p.origin[k] + a * p.u[k] + d * p.v[k]);
+                    uv.push_back(a), uv.push_back(d);
                 }
                 index.insert(index.end(), {b, b + 1, b + 2, b, b + 2, b + 3});
             }
-            return meshGeometry(verts, index);
+            return meshGeometry(verts, index, uv);
         }
         fail("unsupported geometry " + cfg.at("ClassID").string() + cfg.where());
     }

@@ -7,7 +7,7 @@ import zlib
 
 import numpy as np
 
-from plugin_io import IDENTITY_SRT
+from plugin_io import IDENTITY_SRT  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FRONTEND = os.path.join(ROOT, "piper2_amd", "piper_render")
@@ -189,6 +189,19 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
     with open(path, "w") as f:
         json.dump(opts, f, indent=1)
     return path
+
+
+def make_plane_object(options_path, child, origin, u, v):
+    """Replace the geometry of scene child `child` (an Instance with a Surface) by a one-primitive
+    BasicShapes Plane: the surface is then shaded at the plane's own (u, v) coordinates."""
+    path = os.path.join(os.path.dirname(options_path), "scene.json")
+    scene = json.load(open(path))
+    node = scene["Scene"]["Children"][child]
+    assert node["Type"] == "Instance" and "Surface" in node
+    node["Geometry"] = {"ClassID": "Piper.BuiltinComponent.BasicShapes.Plane",
+                        "Primitives": [{"Origin": list(map(float, origin)), "U": list(map(float, u)),
+                                        "V": list(map(float, v))}]}
+    json.dump(scene, open(path, "w"), indent=1)
 
 
 def read_pgsc(path):

@@ -4,13 +4,14 @@ output follows the OpenEXR layout with IEEE half conversion, and malformed input
 import json
 import os
 import subprocess
+import zlib
 
 import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
 
 from piper2_amd import scenes
-from scene_json import FRONTEND, read_exr, read_pgsc, write_obj, write_scene_json
+from scene_json import FRONTEND, make_plane_object, read_exr, read_pgsc, write_obj, write_png, write_scene_json
 
 pytestmark = pytest.mark.skipif(not os.path.exists(FRONTEND), reason="piper_render is not built")
 
@@ -76,6 +77,34 @@ def test_textured_scene_export(tmp_path, image_format, opaque):
                 assert ta["scale"] == tb["scale"]
             else:
                 assert np.array_equal(ta["texels"], tb["texels"]) and ta["wrap"] == tb["wrap"]
+
+
+# the Cornell box's back wall as a parallelogram: origin, U, V
+BACK_WALL = ([-1.0, -1.0, -1.0], [0.0, 2.0, 0.0], [2.0, 0.0, 0.0])
+
+
+def test_textured_plane_geometry_export(tmp_path):
+    """A BasicShapes Plane used as surface geometry is shaded at each hit's plane coordinates
+    (calcTexCoord, BasicShapes/Kernel.cpp:30-56, :109): its two triangles carry per-corner texture
+    coordinates with vertex = origin + s U + t V exactly, so the interpolated coordinate of any hit
+    is the hit's (s, t); the checkerboard on it stays attached."""
+    sc = scenes.cornell(width=32, height=32, spp=4, textured=True)
+    opts = write_scene_json(sc, str(tmp_path))
+    make_plane_object(opts, 2, *BACK_WALL)
+    r = run(opts, "--export", tmp_path / "exp")
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = read_pgsc(tmp_path / "exp" / "Action0.pgsc")
+    tri = e["indices"][e["tri_object"] == 2]
+    assert tri.shape == (2, 3)
+    o, u, v = (np.array(a) for a in BACK_WALL)
+    vid = np.unique(tri)
+    uv = e["texcoords"][vid].astype(np.float64)
+    assert sorted(map(tuple, uv.tolist())) == [(0, 0), (0, 1), (1, 0), (1, 1)]
+    assert np.array_equal(e["vertices"][vid], (o + uv[:, :1] * u + uv[:, 1:] * v).astype(np.float32))
+    back = int(sc["object_material"][2])
+    t = int(e["material_textures"][int(e["object_material"][2]), 0])
+    assert t != 0xFFFFFFFF and e["textures"][t]["type"] == 1
+    assert int(sc["material_textures"][back, 0]) != 0xFFFFFFFF
 
 
 def _first_deflate_block(path):
@@ -185,6 +214,17 @@ def test_exr_layout_and_half_rounding(tmp_path):
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(
         Diffuse={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [1, 1, 1]}),
      "unsupported channel count 3"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(
+        Roughness={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [0.1, 0.2]}),
+     "Mismatched channel. Expect 1, but get 2"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(
+        Diffuse={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": [0.1], "White": [0.9],
+                 "Scale": 2.0}),
+     "Mismatched channel. Expect 4, but get 1"),
+    (lambda s: s["Scene"]["Children"][0]["Surface"].update(
+        Roughness={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.CheckBoard", "Black": [0.1, 0, 0, 1],
+                   "White": [0.9, 0, 0, 1], "Scale": 2.0}),
+     "Mismatched channel. Expect 1, but get 4"),
     (lambda s: s["Actions"][0].update(ActiveSensor="Nobody"), "Failed to find a node"),
     (lambda s: s["Actions"][0].update(FitMode="Stretch"), "Invalid FitMode"),
     (lambda s: s["Scene"]["Children"].append({"Type": "Group", "Children": [{"Type": "Instance", "ClassID": "X.Y",
@@ -215,3 +255,57 @@ def test_json_syntax_errors_name_the_position(tmp_path):
         f.write('{"a": "\\ud83d\\ude00 \\u00e9", "b": [1e3, -0.5, true, false, null]}')
     r = run(opts, "--export", tmp_path / "x")
     assert r.returncode == 1 and "missing key \"RenderDriver\"" in r.stderr, r.stderr  # parsed, then validated
+
+
+def _png_chunks(path):
+    d = open(path, "rb").read()
+    assert d[:8] == b"\x89PNG\r\n\x1a\n"
+    out, p = [], 8
+    while p < len(d):
+        n = int.from_bytes(d[p:p + 4], "big")
+        out.append([d[p + 4:p + 8], d[p + 8:p + 8 + n]])
+        p += 12 + n
+    return out
+
+
+def _png_write(path, chunks):
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        for kind, body in chunks:
+            f.write(len(body).to_bytes(4, "big") + kind + body + (zlib.crc32(kind + body) & 0xFFFFFFFF).to_bytes(4, "big"))
+
+
+def _idat(chunks):
+    return b"".join(b for k, b in chunks if k == b"IDAT")
+
+
+def _set_idat(chunks, data):
+    return [c for c in chunks if c[0] != b"IDAT" and c[0] != b"IEND"] + [[b"IDAT", data], [b"IEND", b""]]
+
+
+@pytest.mark.parametrize("corrupt,message", [
+    (lambda c: [[b"IHDR", c[0][1][:12]]] + c[1:], "IHDR of 12 bytes"),
+    (lambda c: [[b"tEXt", b"a\0b"]] + c, "IHDR must be the first chunk"),
+    (lambda c: c[:1] + [[b"IHDR", c[0][1]]] + c[1:], "IHDR must be the first chunk"),
+    (lambda c: [[b"IHDR", (70000).to_bytes(4, "big") * 2 + c[0][1][8:]]] + c[1:], "too large"),
+    (lambda c: _set_idat(c, _idat(c)[:-4] + bytes(b ^ 0x5A for b in _idat(c)[-4:])), "checksum mismatch"),
+    (lambda c: _set_idat(c, _idat(c)[:-4]), "missing zlib checksum"),
+    (lambda c: _set_idat(c, _idat(c)[:5] + bytes([_idat(c)[5] ^ 1]) + _idat(c)[6:]), "length check failed"),
+    (lambda c: _set_idat(c, _idat(c)[:len(_idat(c)) // 2]), "truncated"),
+])
+def test_malformed_png_is_reported(tmp_path, corrupt, message):
+    """The PNG reader rejects a short or misplaced IHDR, sizes whose texel count would overflow, a
+    wrong stored-block NLEN (RFC 1951 3.2.4), a wrong or missing Adler-32 (RFC 1950) and truncated
+    data, each with a message naming the problem."""
+    sc = scenes.cornell(width=16, height=16, spp=1)
+    opts = write_scene_json(sc, str(tmp_path))
+    png = tmp_path / "picture.png"
+    write_png(str(png), (np.arange(6 * 5 * 4) % 251).astype(np.uint8).reshape(6, 5, 4), level=0)
+    _png_write(str(png), corrupt(_png_chunks(str(png))))
+    path = os.path.join(os.path.dirname(opts), "scene.json")
+    scene = json.load(open(path))
+    scene["Scene"]["Children"][0]["Surface"]["Diffuse"] = {
+        "WrapMode": "Repeat", "Image": {"ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": "picture.png"}}
+    json.dump(scene, open(path, "w"))
+    r = run(opts, "--export", tmp_path / "x")
+    assert r.returncode == 1 and message in r.stderr, r.stderr

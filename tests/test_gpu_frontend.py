@@ -9,8 +9,9 @@ import numpy as np
 import pytest
 
 import piper2_amd as pg
+from oracle_lib import OracleScene
 from piper2_amd import scenes
-from scene_json import FRONTEND, read_exr, read_pgsc, write_scene_json
+from scene_json import FRONTEND, make_plane_object, read_exr, read_pgsc, write_scene_json
 
 pytestmark = pytest.mark.gpu
 
@@ -37,13 +38,17 @@ def direct_rgb(dev, e, film_filter=0, alpha=0.0):
     return rgb.astype(np.float16)
 
 
-@pytest.mark.parametrize("filt,textured", [
-    (None, False), (None, True),
-    ({"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter", "FilterType": "gaussian", "Alpha": 2.0}, False)])
-def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured):
+@pytest.mark.parametrize("filt,textured,plane", [
+    (None, False, False), (None, True, False), (None, True, True),
+    ({"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter", "FilterType": "gaussian", "Alpha": 2.0}, False,
+     False)])
+def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured, plane):
+    """plane=True makes the checkerboard back wall a BasicShapes Plane, shaded at its own (u, v)."""
     sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True,
                         textured=textured)
     opts = write_scene_json(sc, str(tmp_path), filter_cfg=filt)
+    if plane:
+        make_plane_object(opts, 2, [-1.0, -1.0, -1.0], [0.0, 2.0, 0.0], [2.0, 0.0, 0.0])
     scene_path = tmp_path / "scene.json"
     scene = json.load(open(scene_path))
     scene["Actions"][0]["FrameCount"] = 2
@@ -59,3 +64,11 @@ def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured):
     assert np.array_equal(frames[0].view(np.uint16), want.view(np.uint16))
     assert np.array_equal(frames[1].view(np.uint16), frames[0].view(np.uint16))  # a static scene
     assert "paths" in r.stdout
+    if plane:  # the plane's checkerboard through the whole chain: GPU paths equal the oracle's
+        ex = dict(e)
+        ex["ndc"], _ = scenes.calc_render_rect(ex["width"], ex["height"], float(ex["camera"][9] / ex["camera"][10]))
+        s = pg.Scene(dev, ex)
+        _, pl = s.render(pg.render_params(ex), None, path_l=True)
+        ref = OracleScene(ex).render(path_l=True, film=False, threads=16)["path_l"]
+        close = np.all(np.abs(pl - ref) <= 1e-3 * np.abs(ref) + 1e-5, axis=1)
+        assert close.mean() >= 0.99, close.mean()
