@@ -25,34 +25,71 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, threads, budget_s=12.0):
-    """CPU oracle (the C++ restatement of the reference CPU path) on a crop of the same frame."""
+def _cpu_host():
+    """This process's CPUs (as nproc counts them: its affinity set), the machine's logical CPUs and
+    the CPU model string."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nproc, os.cpu_count() or nproc, model
+
+
+def cpu_baseline(sc, wide, c1, c1_wide, threads, budget_s=10.0):
+    """CPU oracle (the C++ restatement of the reference CPU path, 8x8 tiles over std::thread
+    workers with samples looped per pixel, as FixedSampler.cpp:73-94 and Parallel.cpp:475-491 run
+    it) traversing the same wide BVH the GPU renders with (downloaded from the library), on one
+    centred crop of the same frame with `threads` workers and with one, and the whole Cornell-box
+    frame (256x256, 16 spp)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleScene
 
+    nproc, ncpu, model = _cpu_host()
     t0 = time.perf_counter()
     o = OracleScene(sc)
-    log(f"cpu oracle: BVH built in {time.perf_counter() - t0:.1f}s")
+    o.use_wide_bvh(*wide)
+    log(f"cpu oracle: scene loaded in {time.perf_counter() - t0:.1f}s")
     w, h = sc["width"], sc["height"]
 
-    def run(cw, ch, spp):
+    def run(cw, ch, spp, nthreads):
         rect = ((w - cw) // 2, (h - ch) // 2, cw, ch)
         t = time.perf_counter()
-        o.render(rect=rect, spp_range=(0, spp), threads=threads, film=True)
-        return time.perf_counter() - t, rect
+        o.render(rect=rect, spp_range=(0, spp), threads=nthreads, film=True)
+        return time.perf_counter() - t
 
-    dt, _ = run(192, 108, 1)  # calibration
-    target = 192 * 108 / max(dt, 1e-6) * budget_s  # paths that fit the budget
-    if target >= w * h:
-        cw, ch, spp = w, h, max(1, min(64, int(target // (w * h))))
-    else:
-        scale = (target / (w * h)) ** 0.5
-        cw, ch, spp = max(64, int(w * scale) // 8 * 8), max(36, int(h * scale) // 8 * 8), 1
-    dt, rect = run(cw, ch, spp)
-    paths = cw * ch * spp
-    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement, own binned-SAH BVH2) on a {cw}x{ch} crop at {spp} spp "
-                      f"({paths} paths, {dt:.1f} s) of the same 1920x1080 frame"}
+    # one centred crop for both figures, sized so one thread renders it at 1 spp in budget / 3
+    dt = run(96, 54, 1, 1)
+    scale = min(1.0, (96 * 54 / max(dt, 1e-6) * budget_s / 3 / (w * h)) ** 0.5)
+    cw, ch = min(w, max(64, int(w * scale) // 8 * 8)), min(h, max(36, int(h * scale) // 8 * 8))
+    dt1 = run(cw, ch, 1, 1)
+    dtn = run(cw, ch, 1, threads)
+    spp = max(1, min(64, int(budget_s / max(dtn, 1e-6))))  # the multithreaded figure over about budget_s
+    if spp > 1:
+        dtn = run(cw, ch, spp, threads)
+    mp1, mp = cw * ch / dt1 / 1e6, cw * ch * spp / dtn / 1e6
+    sample1 = f"{cw}x{ch} crop at 1 spp ({cw * ch} paths, {dt1:.1f} s)"
+    sample = f"{cw}x{ch} crop at {spp} spp ({cw * ch * spp} paths, {dtn:.1f} s)"
+    oc = OracleScene(c1)
+    oc.use_wide_bvh(*c1_wide)
+    t = time.perf_counter()
+    oc.render(threads=threads, film=True)
+    dt_c1 = time.perf_counter() - t
+    c1_paths = c1["width"] * c1["height"] * c1["spp"]
+    width = wide[0]
+    return {"value": round(mp, 3), "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, traversing the GPU's own BVH{width}) on a {sample} of the same "
+                      f"1920x1080 frame, {threads} threads",
+            "nproc": nproc, "logical_cpus": ncpu, "cpu_model": model,
+            "threads_note": "worker threads capped at 16, the host-CPU share of one GPU on this machine",
+            "one_thread": {"value": round(mp1, 3), "unit": "Mpaths/s", "sample": sample1},
+            "cornell_256x256_16spp": {"value": round(c1_paths / dt_c1 / 1e6, 3), "unit": "Mpaths/s",
+                                      "cores": threads, "seconds": round(dt_c1, 2)}}
 
 
 def measured_traffic(bvh_width, wavefront_paths):
@@ -464,6 +501,21 @@ def main():
             scene.render(gparams, film)
         graph_ms = round((time.perf_counter() - t_graph) * 1e3 / args.steps, 2)
         log(f"graph replay: {graph_ms} ms per frame")
+    # the Cornell-box frame (256x256, 16 spp, depth 5): whole-frame throughput on the GPU
+    c1 = scenes.cornell()
+    c1_scene = pg.Scene(dev, c1, bvh_width=info["width"])
+    c1_film = pg.Film(dev, c1["width"], c1["height"])
+    c1_params = pg.render_params(c1, flags=pg.PG_NO_STAGE_TIMING)
+    c1_scene.render(c1_params, c1_film)
+    t_c1 = time.perf_counter()
+    for _ in range(10):
+        c1_film.clear()
+        c1_scene.render(c1_params, c1_film)
+    c1_ms = (time.perf_counter() - t_c1) * 1e3 / 10
+    c1_gpu = {"mpaths_per_s": round(c1["width"] * c1["height"] * c1["spp"] / c1_ms / 1e3, 1), "ms": round(c1_ms, 3)}
+    log(f"cornell 256x256x16: {c1_gpu}")
+    c1_film.close()
+    c1_scene.close()
     ms_per_step = elapsed * 1e3 / args.steps
     paths = sc["width"] * sc["height"] * args.spp * args.steps
     mpaths = paths / elapsed / 1e6
@@ -493,6 +545,7 @@ def main():
                    "wavefront_paths": args.wavefront_paths},
         "autotune": tune,
         "graph_ms_per_step": graph_ms,
+        "cornell_256x256_16spp": c1_gpu,
         "mrays_per_s": round(total_rays / elapsed / 1e6, 2),
         "copy_kernel_gbps": round(copy_gbps, 1),
         "rays_per_path": round(total_rays / paths, 3),
@@ -519,8 +572,11 @@ def main():
     if args.guided_batch > 0:
         result["guided_batch_hard_rooms"] = guided_batch(dev, args.guided_batch, info["width"])
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(sc, threads)
+        threads = min(_cpu_host()[0], 16)
+        c1_scene = pg.Scene(dev, c1, bvh_width=info["width"])
+        result["cpu_baseline"] = cpu_baseline(sc, (info["width"], *scene.bvh_download()), c1,
+                                              (info["width"], *c1_scene.bvh_download()), threads)
+        c1_scene.close()
         log(f"cpu baseline: {result['cpu_baseline']}")
     print(json.dumps(result), flush=True)
 

@@ -226,6 +226,13 @@ struct Scene {
     V3 env;
     std::vector<BNode> nodes;
     std::vector<uint32_t> order;  // BVH leaf order -> triangle index
+    // Optional wide BVH as the GPU library builds and lays it out (pg_scene_bvh_download): nodes of
+    // `wideWidth` children, each six planes of W floats (lo x, y, z, hi x, y, z) then W child
+    // references (leaf: top bit, (count - 1) << 27, first triangle); triangles in leaf order as
+    // 12 floats (v0, original index bits, v1, -, v2, -). When set, rays traverse it instead.
+    uint32_t wideWidth = 0;
+    std::vector<uint32_t> wideNodes;  // raw 32-bit words
+    std::vector<float> wideTris;
 };
 
 // Binned-SAH BVH2 over triangle boxes (the oracle's own acceleration structure; only the hit
@@ -450,6 +457,8 @@ bool boxHit(const BNode& n, const Ray& r, const RayPre& p, float tmin, float tma
     return t0 <= t1;
 }
 
+bool traverseWide(const Scene& s, const Ray& r, float tmin, float tmax, bool any, Hit& h, uint64_t* counts);
+
 // rtcIntersect1 with mask gsm|areaLight (Embree.cpp:267-355): closest triangle or light quad.
 Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brute, uint64_t* counts) {
     Hit best{tmax, kMiss, 0.0f, 0.0f};
@@ -458,6 +467,8 @@ Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brut
     if (brute) {
         for (uint32_t i = 0; i < s.tris.size(); ++i)
             if (intersectTri(r, p, s.tris[i], t, u, v)) consider(best, t, i, u, v, tmin);
+    } else if (s.wideWidth) {
+        if (!s.wideTris.empty()) traverseWide(s, r, tmin, tmax, false, best, counts);
     } else if (!s.nodes.empty()) {
         uint32_t stack[128];
         int sp = 0;
@@ -485,8 +496,92 @@ Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brut
     return best;
 }
 
+// Traversal of the wide BVH in the order the GPU kernel uses (pg_device.hpp traverse()): slab
+// distances as fma(plane, 1/d, -o/d), the nearest hit child entered first, the other hit
+// children pushed in child order, popped last-in first-out and skipped when their entry distance
+// exceeds the best hit by more than 1e-5 relative; leaves test their triangles in order. So the
+// nodes visited and triangles tested are the GPU's own counts. any = true: return at the first
+// triangle hit in (tmin, tmax).
+constexpr uint32_t kWideLeaf = 0x80000000u, kWideCountShift = 27, kWideFirstMask = (1u << 27) - 1u;
+constexpr uint32_t kWideInvalid = 0xFFFFFFFFu;
+bool traverseWide(const Scene& s, const Ray& r, float tmin, float tmax, bool any, Hit& h, uint64_t* counts) {
+    const uint32_t W = s.wideWidth;
+    const RayPre p = precompute(r);
+    const float ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
+    const float oix = r.o.x * ix, oiy = r.o.y * iy, oiz = r.o.z * iz;
+    uint32_t hprim = kMiss;  // original index of the current best (tie rule)
+    std::vector<std::pair<uint32_t, float>> stack;
+    uint32_t ref = 0;
+    while (true) {
+        if (!(ref & kWideLeaf)) {
+            if (counts) ++counts[0];
+            const uint32_t* n = &s.wideNodes[static_cast<size_t>(ref) * 7 * W];
+            auto plane = [&](int k, uint32_t c) {
+                float f;
+                std::memcpy(&f, &n[k * W + c], 4);
+                return f;
+            };
+            const float tcull = any ? tmax : h.t * 1.00001f;
+            float tn[8];
+            uint32_t mask = 0;
+            for (uint32_t c = 0; c < W; ++c) {
+                const float ax = std::fma(plane(0, c), ix, -oix), bx = std::fma(plane(3, c), ix, -oix);
+                const float ay = std::fma(plane(1, c), iy, -oiy), by = std::fma(plane(4, c), iy, -oiy);
+                const float az = std::fma(plane(2, c), iz, -oiz), bz = std::fma(plane(5, c), iz, -oiz);
+                const float t0 = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)), std::fmax(std::fmin(az, bz), tmin));
+                const float t1 = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
+                                           std::fmin(std::fmax(az, bz), tcull)) * 1.0000004f;
+                tn[c] = t0;
+                if (n[6 * W + c] != kWideInvalid && t0 <= t1) mask |= 1u << c;
+            }
+            if (mask) {
+                uint32_t best = 0;
+                float bt = INFINITY;
+                for (uint32_t c = 0; c < W; ++c)
+                    if (((mask >> c) & 1) && tn[c] < bt) bt = tn[c], best = c;
+                for (uint32_t c = 0; c < W; ++c)
+                    if (((mask >> c) & 1) && c != best) stack.push_back({n[6 * W + c], tn[c]});
+                ref = n[6 * W + best];
+                continue;
+            }
+        } else {
+            const uint32_t first = ref & kWideFirstMask, count = ((ref >> kWideCountShift) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + count; ++k) {
+                if (counts) ++counts[1];
+                const float* q = &s.wideTris[static_cast<size_t>(k) * 12];
+                const Tri tri{{q[0], q[1], q[2]}, {q[4], q[5], q[6]}, {q[8], q[9], q[10]}};
+                float t, u, v;
+                if (!intersectTri(r, p, tri, t, u, v)) continue;
+                if (any) {
+                    if (t > tmin && t < tmax) return true;
+                } else {
+                    uint32_t prim;
+                    std::memcpy(&prim, &q[3], 4);
+                    if (t > tmin && (t < h.t || (t == h.t && prim < hprim))) h = {t, prim, u, v}, hprim = prim;
+                }
+            }
+        }
+        bool found = false;
+        while (!stack.empty()) {
+            const auto e = stack.back();
+            stack.pop_back();
+            if (any || e.second <= h.t * 1.00001f) {
+                ref = e.first;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return !any && h.prim != kMiss;
+}
+
 // rtcOccluded1 with mask gsm only (Embree.cpp:357-391): lights never occlude.
 bool occluded(const Scene& s, const Ray& r, float tmin, float tmax) {
+    if (s.wideWidth) {
+        Hit h{tmax, kMiss, 0.0f, 0.0f};
+        return !s.wideTris.empty() && traverseWide(s, r, tmin, tmax, true, h, nullptr);
+    }
     if (s.nodes.empty()) return false;
     const RayPre p = precompute(r);
     float t, u, v;
@@ -1047,6 +1142,7 @@ inline float powerHeuristic(float a, float b) {  // PathTracing/Kernel.cpp:27-31
 
 struct PathStats {
     uint64_t ext = 0, shadow = 0, mis = 0;
+    uint64_t trav[2] = {0, 0};  // extension rays: nodes visited, triangles tested
 };
 
 // trace() + multipleImportanceSampling(), PathTracing/Kernel.cpp:23-193
@@ -1057,7 +1153,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
     bool specular = true;
     uint32_t depth = 0;
     while (true) {
-        const Hit h = traceClosest(s, ray, 1e-3f, 1e5f, false, nullptr);
+        const Hit h = traceClosest(s, ray, 1e-3f, 1e5f, false, st.trav);
         ++st.ext;
         const bool isLight = h.prim != kMiss && (h.prim & kQuadBit);
         if (specular && (isLight || h.prim == kMiss)) {  // :119-140
@@ -1285,6 +1381,18 @@ void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint
 }
 void or_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
 
+// Traverse a wide BVH laid out as the GPU library builds it (pg_scene_bvh_download: numNodes nodes
+// of 28 * width bytes, numTris leaf-order triangles of 48 bytes) instead of the oracle's own BVH2.
+void or_scene_set_wide_bvh(void* sp, uint32_t width, const void* nodes, uint64_t numNodes, const void* tris,
+                           uint64_t numTris) {
+    Scene& s = *static_cast<Scene*>(sp);
+    s.wideWidth = width;
+    s.wideNodes.resize(static_cast<size_t>(numNodes) * 7 * width);
+    std::memcpy(s.wideNodes.data(), nodes, s.wideNodes.size() * 4);
+    s.wideTris.resize(static_cast<size_t>(numTris) * 12);
+    std::memcpy(s.wideTris.data(), tris, s.wideTris.size() * 4);
+}
+
 // rays[8*n] = {o xyz, d xyz, tmin, tmax}; outputs t, prim (kMiss / kQuadBit|q / tri), u, v.
 // brute != 0 tests every triangle instead of using the BVH. counts (optional) gets
 // {nodes visited, triangles tested} summed over the rays.
@@ -1380,9 +1488,11 @@ void or_render(void* sp, const uint32_t* iparams, const float* ndc, double* film
     for (int i = 1; i < threads; ++i) pool.emplace_back(worker, i);
     worker(0);
     for (auto& t : pool) t.join();
-    if (rayCounts) {
-        rayCounts[0] = rayCounts[1] = rayCounts[2] = 0;
-        for (auto& st : stats) rayCounts[0] += st.ext, rayCounts[1] += st.shadow, rayCounts[2] += st.mis;
+    if (rayCounts) {  // extension, shadow and MIS rays; nodes visited and triangles tested by extension rays
+        for (int k = 0; k < 5; ++k) rayCounts[k] = 0;
+        for (auto& st : stats)
+            rayCounts[0] += st.ext, rayCounts[1] += st.shadow, rayCounts[2] += st.mis, rayCounts[3] += st.trav[0],
+                rayCounts[4] += st.trav[1];
     }
 }
 

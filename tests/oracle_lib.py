@@ -26,6 +26,7 @@ def lib():
         L.or_scene_create.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint32, _VP,
                                       ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]
         L.or_scene_destroy.argtypes = [_VP]
+        L.or_scene_set_wide_bvh.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]
         L.or_scene_set_points.argtypes = [_VP, _VP, ctypes.c_uint32]
         L.or_scene_set_textures.argtypes = [_VP, _VP, _VP]
         L.or_scene_add_texture.restype = ctypes.c_uint32
@@ -94,6 +95,14 @@ class OracleScene:
             self._keep += [triuv, mt]
             lib().or_scene_set_textures(self.h, _p(triuv), _p(mt))
 
+    def use_wide_bvh(self, width, nodes, tris):
+        """Traverse the GPU library's own wide BVH (Scene.bvh_download()) from now on, in the GPU's
+        order, so trace counts and render counts are the GPU kernel's."""
+        self._wide = (np.ascontiguousarray(nodes), np.ascontiguousarray(tris))
+        n_nodes = self._wide[0].nbytes // (28 * width)
+        lib().or_scene_set_wide_bvh(self.h, width, _p(self._wide[0]), n_nodes, _p(self._wide[1]),
+                                    self._wide[1].nbytes // 48)
+
     def texture_sample(self, tex, u, v):
         out = np.zeros(4, np.float32)
         lib().or_texture_sample(self.h, int(tex), np.float32(u), np.float32(v), _p(out))
@@ -141,6 +150,6 @@ class OracleScene:
         ndc = np.array([*sc["ndc"], filter_alpha], np.float32)
         f = np.zeros((h, w, 4), np.float64) if film else None
         pl = np.zeros((rect[2] * rect[3] * (s1 - s0), 3), np.float32) if path_l else None
-        rc = np.zeros(3, np.uint64)
+        rc = np.zeros(5, np.uint64)
         lib().or_render(self.h, _p(ip), _p(ndc), _p(f), _p(pl), int(threads or os.cpu_count() or 1), _p(rc))
-        return {"film": f, "path_l": pl, "rays": rc}
+        return {"film": f, "path_l": pl, "rays": rc[:3], "nodes_visited": int(rc[3]), "tris_tested": int(rc[4])}
