@@ -62,6 +62,40 @@ public:
 
     static Json parse(const std::string& text);
 
+    // values built in code (component configs made by factories)
+    static Json makeNumber(double v) {
+        Json j;
+        j.mType = Type::Number, j.mNumber = v;
+        return j;
+    }
+    static Json makeString(std::string v) {
+        Json j;
+        j.mType = Type::String, j.mString = std::move(v);
+        return j;
+    }
+    static Json makeArray(std::vector<Json> v) {
+        Json j;
+        j.mType = Type::Array, j.mArray = std::move(v);
+        return j;
+    }
+    static Json makeObject(std::vector<std::pair<std::string, Json>> members) {
+        Json j;
+        j.mType = Type::Object, j.mMembers = std::move(members);
+        return j;
+    }
+    // the same object with member `key` replaced (or added)
+    Json with(const std::string& key, Json v) const {
+        expect(Type::Object, "an object");
+        Json j = *this;
+        for (auto& [k, old] : j.mMembers)
+            if (k == key) {
+                old = std::move(v);
+                return j;
+            }
+        j.mMembers.emplace_back(key, std::move(v));
+        return j;
+    }
+
 private:
     friend class JsonParser;
     void expect(Type t, const char* what) const {

@@ -38,7 +38,6 @@
 #include "Exr.hpp"
 #include "Json.hpp"
 #include "MeshIO.hpp"
-#include "Png.hpp"
 
 using namespace Piper;
 using PiperFrontend::Json;
@@ -87,153 +86,6 @@ bool isClass(const Json& cfg, const char* module, const char* cls) {
 void requireClass(const Json& cfg, const char* module, const char* cls) {
     if (!isClass(cfg, module, cls))
         fail(std::string("expected ClassID ") + module + "." + cls + ", got " + cfg.at("ClassID").string() + cfg.where());
-}
-
-// ConstantTexture (ProceduralTexture.cpp:27-45): Value holds 1, 2 or 4 channels; the surface asks
-// for 4 (colours, the first three used) or 1 (roughness).
-bool sameJson(const Json& a, const Json& b) {
-    if (a.type() != b.type()) return false;
-    switch (a.type()) {
-        case Json::Type::Null: return true;
-        case Json::Type::Bool: return a.boolean() == b.boolean();
-        case Json::Type::Number: return a.number() == b.number();
-        case Json::Type::String: return a.string() == b.string();
-        case Json::Type::Array: {
-            if (a.array().size() != b.array().size()) return false;
-            for (size_t i = 0; i < a.array().size(); ++i)
-                if (!sameJson(a.array()[i], b.array()[i])) return false;
-            return true;
-        }
-        case Json::Type::Object: {
-            if (a.members().size() != b.members().size()) return false;
-            for (const auto& [k, v] : a.members()) {
-                const Json* w = b.find(k);
-                if (!w || !sameJson(v, *w)) return false;
-            }
-            return true;
-        }
-    }
-    return false;
-}
-// Tracer::generateTexture (Tracer/Embree/Embree.cpp:1423-1432): a texture must have exactly the
-// channel count its input asks for (colours 4, roughness 1; Classic.cpp:89-195).
-void requireChannels(size_t got, size_t want, const char* what, const Json& cfg) {
-    if (got != want)
-        fail(std::string(what) + ": Mismatched channel. Expect " + std::to_string(want) + ", but get " +
-             std::to_string(got) + cfg.where());
-}
-std::array<float, 4> constantTexture(const Json& cfg, size_t channels, const char* what) {
-    if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
-    if (!cfg.find("ClassID")) fail(std::string(what) + ": sampled image textures are not supported" + cfg.where());
-    const auto [m, c] = classOf(cfg);
-    if (m != "Piper.BuiltinComponent.ProceduralTexture" || c != "ConstantTexture")
-        fail(std::string(what) + ": only ConstantTexture is supported by the HIP path (got " + c + ")" + cfg.where());
-    const auto& v = cfg.at("Value").array();
-    if (v.size() != 1 && v.size() != 2 && v.size() != 4)
-        fail(std::string(what) + ": unsupported channel count " + std::to_string(v.size()) + cfg.where());
-    requireChannels(v.size(), channels, what, cfg);
-    std::array<float, 4> r{};
-    for (size_t k = 0; k < v.size(); ++k) r[k] = num(v[k]);
-    return r;
-}
-// An 8-bit PNG or netpbm image (P5 grey, P6 RGB, P7 PAM with DEPTH 1..4). Three channels are
-// widened to four with alpha 255 (SampledTexture.cpp:143-151 asks stb_image for RGBA then).
-std::shared_ptr<Texture> loadImageTexture(const fs::path& path, uint32_t wrap) {
-    std::string ext = path.extension().string();
-    for (char& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
-    if (ext == ".png") {
-        auto img = PiperFrontend::loadPng(path.string());
-        if (img.channels != 3) return Texture::image(img.width, img.height, img.channels, wrap, std::move(img.texels));
-        std::vector<uint8_t> rgba(static_cast<size_t>(img.width) * img.height * 4);
-        for (size_t i = 0; i < static_cast<size_t>(img.width) * img.height; ++i) {
-            std::memcpy(&rgba[4 * i], &img.texels[3 * i], 3);
-            rgba[4 * i + 3] = 255;
-        }
-        return Texture::image(img.width, img.height, 4, wrap, std::move(rgba));
-    }
-    std::ifstream f(path, std::ios::binary);
-    if (!f) fail("cannot open image " + path.string());
-    auto token = [&]() {
-        std::string t;
-        char c;
-        while (f.get(c)) {
-            if (c == '#') {
-                std::string skip;
-                std::getline(f, skip);
-                continue;
-            }
-            if (std::isspace(static_cast<unsigned char>(c))) {
-                if (!t.empty()) break;
-                continue;
-            }
-            t += c;
-        }
-        return t;
-    };
-    const std::string magic = token();
-    uint32_t w = 0, h = 0, depth = 0, maxval = 0;
-    if (magic == "P5" || magic == "P6") {
-        w = static_cast<uint32_t>(std::stoul(token())), h = static_cast<uint32_t>(std::stoul(token()));
-        maxval = static_cast<uint32_t>(std::stoul(token()));
-        depth = magic == "P5" ? 1 : 3;
-    } else if (magic == "P7") {
-        for (std::string t = token(); t != "ENDHDR"; t = token()) {
-            if (t.empty()) fail("truncated PAM header in " + path.string());
-            if (t == "WIDTH") w = static_cast<uint32_t>(std::stoul(token()));
-            else if (t == "HEIGHT") h = static_cast<uint32_t>(std::stoul(token()));
-            else if (t == "DEPTH") depth = static_cast<uint32_t>(std::stoul(token()));
-            else if (t == "MAXVAL") maxval = static_cast<uint32_t>(std::stoul(token()));
-            else if (t == "TUPLTYPE") (void)token();
-        }
-    } else {
-        fail("unsupported image format (PNG, binary PPM, PGM and PAM are read): " + path.string());
-    }
-    if (maxval != 255 || w == 0 || h == 0 || depth < 1 || depth > 4) fail("unsupported image layout in " + path.string());
-    std::vector<uint8_t> raw(static_cast<size_t>(w) * h * depth);
-    if (!f.read(reinterpret_cast<char*>(raw.data()), static_cast<std::streamsize>(raw.size())))
-        fail("truncated image data in " + path.string());
-    if (depth != 3) return Texture::image(w, h, depth, wrap, std::move(raw));
-    std::vector<uint8_t> rgba(static_cast<size_t>(w) * h * 4);
-    for (size_t i = 0; i < static_cast<size_t>(w) * h; ++i) {
-        std::memcpy(&rgba[4 * i], &raw[3 * i], 3);
-        rgba[4 * i + 3] = 255;
-    }
-    return Texture::image(w, h, 4, wrap, std::move(rgba));
-}
-
-// A colour or roughness input: a constant (ConstantTexture) or a texture (CheckBoard, sampled
-// image) that the surface samples at each hit.
-struct ColourInput {
-    std::array<float, 3> value{};
-    std::shared_ptr<Texture> texture;
-};
-ColourInput colourInput(const Json& cfg, const char* what, const fs::path& base, size_t channels = 4) {
-    if (!cfg.isObject()) fail(std::string(what) + ": expected a texture object" + cfg.where());
-    if (!cfg.find("ClassID")) {  // a sampled image (Tracer/Embree/Embree.cpp:1334-1349)
-        const std::string& wrap = cfg.at("WrapMode").string();
-        if (wrap != "Repeat" && wrap != "Mirror") fail("Unrecognized wrap mode " + wrap + cfg.where());
-        const Json& img = cfg.at("Image");
-        requireClass(img, "Piper.BuiltinComponent.SampledTexture", "Image");
-        fs::path p = img.at("Path").string();
-        if (p.is_relative()) p = base / p;
-        auto t = loadImageTexture(p, wrap == "Repeat" ? PG_WRAP_REPEAT : PG_WRAP_MIRROR);
-        requireChannels(t->desc.channels, channels, what, cfg);  // RGB was widened to RGBA
-        return {{}, t};
-    }
-    const auto [m, c] = classOf(cfg);
-    if (m == "Piper.BuiltinComponent.ProceduralTexture" && c == "CheckBoard") {  // ProceduralTexture.cpp:55-80
-        const auto& b = cfg.at("Black").array();
-        const auto& w = cfg.at("White").array();
-        if (b.size() != w.size()) fail("Channel of black and white must be identical." + cfg.where());
-        if (b.size() != 1 && b.size() != 2 && b.size() != 4)
-            fail(std::string(what) + ": unsupported channel count " + std::to_string(b.size()) + cfg.where());
-        requireChannels(b.size(), channels, what, cfg);
-        std::array<float, 4> black{}, white{};
-        for (size_t k = 0; k < b.size(); ++k) black[k] = num(b[k]), white[k] = num(w[k]);
-        return {{}, Texture::checkerboard(black, white, num(cfg.at("Scale")), static_cast<uint32_t>(b.size()))};
-    }
-    const auto v = constantTexture(cfg, channels, what);
-    return {{v[0], v[1], v[2]}, nullptr};
 }
 
 // parseTransform (PiperRenderer.cpp:58-137): Translation, Scale (number, [x, y, z] or
@@ -371,55 +223,27 @@ private:
         }
         fail("unsupported geometry " + cfg.at("ClassID").string() + cfg.where());
     }
-    std::shared_ptr<Surface> surface(const Json& cfg) const {  // Surface/Classic (Classic.cpp:207-237)
+    // Surface/Classic (Classic.cpp:207-237): each input is kept as its texture config; the tracer
+    // loads it when the pipeline materialises the surface (generateTexture checks the channels).
+    std::shared_ptr<Surface> surface(const Json& cfg) const {
         const auto [m, c] = classOf(cfg);
         if (m != "Piper.BuiltinComponent.ClassicSurface") fail("unsupported surface " + cfg.at("ClassID").string() + cfg.where());
-        auto in = [&](const char* key) { return colourInput(cfg.at(key), key, mBase); };
-        auto rough = [&](const char* key) { return colourInput(cfg.at(key), key, mBase, 1); };
-        auto roughXY = [&]() {  // one roughness for both axes
-            const auto x = rough("RoughnessX"), y = rough("RoughnessY");
-            const bool same = x.texture ? (y.texture && sameJson(cfg.at("RoughnessX"), cfg.at("RoughnessY")))
-                                        : (!y.texture && x.value[0] == y.value[0]);
-            if (!same) fail("anisotropic roughness is not supported" + cfg.where());
-            return x;
+        auto in = [&](std::initializer_list<const char*> keys) {
+            std::vector<Json> v;
+            for (const char* k : keys) v.push_back(cfg.at(k));
+            return v;
         };
-        auto bind = [](std::shared_ptr<Surface> s, const ColourInput& a, const ColourInput& b, const ColourInput& r) {
-            s->colorTexture = a.texture;
-            s->specularTexture = b.texture;
-            s->roughnessTexture = r.texture;
-            return s;
-        };
-        // a textured roughness replaces the constant at every hit; the constant handed to the
-        // factory then only has to be valid
-        auto constant = [](const ColourInput& r, float fallback) { return r.texture ? fallback : r.value[0]; };
-        const ColourInput none;
-        if (c == "Matte") {
-            const auto d = in("Diffuse"), r = rough("Roughness");
-            return bind(Surface::matte(d.value, constant(r, 0.0f)), d, none, r);
-        }
-        if (c == "Plastic") {
-            const auto d = in("Diffuse"), s = in("Specular"), r = rough("Roughness");
-            return bind(Surface::plastic(d.value, s.value, constant(r, 0.5f)), d, s, r);
-        }
-        if (c == "Mirror") {
-            const auto r = in("Reflection");
-            return bind(Surface::mirror(r.value), r, none, none);
-        }
-        if (c == "Glass") {
-            const auto r = in("Reflection"), t = in("Transmission"), x = roughXY();
-            const float v = constant(x, 0.0f);
-            return bind(Surface::glass(r.value, t.value, v, v), r, t, x);
-        }
-        if (c == "Substrate") {
-            const auto d = in("Diffuse"), s = in("Specular"), x = roughXY();
-            const float v = constant(x, 0.5f);
-            return bind(Surface::substrate(d.value, s.value, v, v), d, s, x);
-        }
-        if (c == "BlackBody") return Surface::matte({0.0f, 0.0f, 0.0f});  // no lobe: the path ends here
-        if (c == "Conductor") {  // the GGX conductor this path adds (not a reference class): F0, Roughness
-            const auto f = in("F0"), r = rough("Roughness");
-            return bind(Surface::conductor(f.value, constant(r, 0.5f)), f, none, r);
-        }
+        using K = Surface::Kind;
+        if (c == "Matte") return std::make_shared<Surface>(K::Matte, in({"Diffuse", "Roughness"}));
+        if (c == "Plastic") return std::make_shared<Surface>(K::Plastic, in({"Diffuse", "Specular", "Roughness"}));
+        if (c == "Mirror") return std::make_shared<Surface>(K::Mirror, in({"Reflection"}));
+        if (c == "Glass")
+            return std::make_shared<Surface>(K::Glass, in({"Reflection", "Transmission", "RoughnessX", "RoughnessY"}));
+        if (c == "Substrate")
+            return std::make_shared<Surface>(K::Substrate, in({"Diffuse", "Specular", "RoughnessX", "RoughnessY"}));
+        if (c == "BlackBody") return std::make_shared<Surface>(K::BlackBody, std::vector<Json>{});
+        // the GGX conductor this path adds (not a reference class): F0, Roughness (alpha = roughness^2)
+        if (c == "Conductor") return std::make_shared<Surface>(K::Conductor, in({"F0", "Roughness"}));
         fail("unsupported surface class " + c + cfg.where());
     }
     static std::shared_ptr<Object> object(const Json& cfg) {  // an Instance with a ClassID: a light or a sensor
@@ -427,9 +251,9 @@ private:
         if (m == "Piper.BuiltinComponent.AreaLight" && c == "Diffuse") {  // AreaLight.cpp:17-20
             const Json& g = cfg.at("Geometry");
             requireClass(g, "Piper.BuiltinComponent.BasicShapes", "Plane");
-            const auto prims = planes(g);
-            if (prims.size() != 1) fail("area lights with more than one plane primitive are not supported" + g.where());
-            return Light::areaPlane(prims[0].origin, prims[0].u, prims[0].v, vec3(cfg.at("Radiance")));
+            std::vector<Light::PlanePrimitive> prims;
+            for (const auto& p : planes(g)) prims.push_back({p.origin, p.u, p.v});
+            return Light::area(std::move(prims), vec3(cfg.at("Radiance")));
         }
         if (m == "Piper.BuiltinComponent.BasicLight" && c == "Point") return Light::point(vec3(cfg.at("Intensity")));
         if (m == "Piper.BuiltinComponent.BasicLight" && c == "SampledEnvironment")
@@ -499,12 +323,12 @@ void put(std::ofstream& f, const T* p, size_t n) {
     f.write(reinterpret_cast<const char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
 }
 
-void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorNode, const Sensor& cam, const Globals& g,
-                  uint32_t width, uint32_t height, FitMode fit) {
-    const std::array<float, 12>* toWorld = nullptr;
-    for (const auto& [n, m] : e.sensors)
-        if (n == sensorNode) toWorld = &m;
-    if (!toWorld) fail("the active sensor is not in the scene");
+void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorNode, const Globals& g, uint32_t width,
+                  uint32_t height, FitMode fit) {
+    const pg_camera* cam = nullptr;
+    for (size_t i = 0; i < e.sensors.size(); ++i)
+        if (e.sensors[i].first == sensorNode) cam = &e.cameras[i];
+    if (!cam) fail("the active sensor is not in the scene");
     const uint32_t nv = static_cast<uint32_t>(e.verts.size() / 3), nt = static_cast<uint32_t>(e.index.size() / 3);
     const uint32_t no = static_cast<uint32_t>(e.objectMaterial.size()), nm = static_cast<uint32_t>(e.materials.size());
     const uint32_t nq = static_cast<uint32_t>(e.quads.size());
@@ -528,9 +352,9 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     for (const pg_quad& q : e.quads) {
         put(f, q.origin, 3), put(f, q.u, 3), put(f, q.v, 3), put(f, q.radiance, 3);
     }
-    const float camera[14] = {(*toWorld)[3], (*toWorld)[7], (*toWorld)[11], cam.lookAt[0], cam.lookAt[1], cam.lookAt[2],
-                              cam.upRef[0], cam.upRef[1], cam.upRef[2], cam.size[0], cam.size[1], cam.focalLength,
-                              cam.apertureRadius, 0.0f};
+    const float camera[14] = {cam->position[0], cam->position[1], cam->position[2], cam->look_at[0], cam->look_at[1],
+                              cam->look_at[2], cam->up[0], cam->up[1], cam->up[2], cam->sensor_size[0],
+                              cam->sensor_size[1], cam->focal_length, cam->aperture_radius, 0.0f};
     put(f, camera, 14);
     put(f, e.env.data(), 3);
     const TransformSRT id = identitySRT();
@@ -548,13 +372,13 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
         put(f, uv.data(), uv.size());
         const uint32_t nx = static_cast<uint32_t>(e.textures.size());
         put(f, &nx, 1);
-        for (const auto& t : e.textures) {
-            const pg_texture& d = t->desc;
+        for (size_t i = 0; i < e.textures.size(); ++i) {
+            const pg_texture& d = e.textures[i];
             const uint32_t head4[2] = {d.type, d.channels};
             put(f, head4, 2), put(f, d.black, 4), put(f, d.white, 4), put(f, &d.scale, 1);
             const uint32_t dims[3] = {d.width, d.height, d.wrap};
             put(f, dims, 3);
-            put(f, t->texels.data(), t->texels.size());
+            put(f, e.textureTexels[i].data(), e.textureTexels[i].size());
         }
         for (const pg_material_textures& b : e.materialTextures) put(f, &b.albedo, 1), put(f, &b.reserved, 1), put(f, &b.alpha, 1);
     }
@@ -576,6 +400,7 @@ int render(const fs::path& optionsPath, const std::string& exportDir) {
     const bool exporting = !exportDir.empty();
     auto accelerator = exporting ? HIPAccelerator::hostOnly() : std::make_shared<HIPAccelerator>(0);
     HIPTracer tracer(accelerator);
+    tracer.setResourceRoot(scenePath.parent_path().string());  // image paths are relative to the scene file
     SceneBuilder builder(tracer, scenePath.parent_path());
     const auto root = builder.build(scene.at("Scene"));
     auto sensorOf = [&](const Action& a) {
@@ -586,14 +411,13 @@ int render(const fs::path& optionsPath, const std::string& exportDir) {
     };
     if (exporting) {
         fs::create_directories(exportDir);
-        const SceneExport e = exportScene(root);
+        const SceneExport e = exportScene(tracer, root);
         for (size_t i = 0; i < actions.size(); ++i) {
             const auto node = sensorOf(actions[i]);
-            const auto it = builder.sensors.find(node.get());
-            if (it == builder.sensors.end()) fail("ActiveSensor \"" + actions[i].activeSensor + "\" is not a sensor");
-            const Sensor* cam = it->second.get();
-            exportAction(fs::path(exportDir) / ("Action" + std::to_string(i) + ".pgsc"), e, node.get(), *cam, g, width,
-                         height, actions[i].fitMode);
+            if (!builder.sensors.count(node.get()))
+                fail("ActiveSensor \"" + actions[i].activeSensor + "\" is not a sensor");
+            exportAction(fs::path(exportDir) / ("Action" + std::to_string(i) + ".pgsc"), e, node.get(), g, width, height,
+                         actions[i].fitMode);
         }
         std::printf("exported %zu action(s) to %s\n", actions.size(), exportDir.c_str());
         return 0;

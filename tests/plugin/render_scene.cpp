@@ -11,10 +11,15 @@
 // u32 objectMaterial[no], f32 materials[8 nm] (type, rgb, alpha, specular rgb), f32 quads[12 nq]
 // (origin, U, V, radiance), f32 camera[14] (position, lookAt, up, sensor w h, focal, aperture, 0),
 // f32 env[3], f32 srt[16 no] (TransformSRT per object, applied to its triangles).
+#include <dlfcn.h>
+
+#include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <map>
+#include <sstream>
 
 #include "PiperHIP.hpp"
 
@@ -112,11 +117,10 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
             surface = Surface::glass({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4], m[4]);
         } else if (m[0] == 5.0f) {
             surface = Surface::substrate({m[1], m[2], m[3]}, {m[5], m[6], m[7]}, m[4], m[4]);
+        } else if (m[0] == 0.0f) {
+            surface = Surface::matte({m[1], m[2], m[3]}, m[4]);
         } else {
-            surface = std::make_shared<Surface>();  // alpha as stored (Surface::conductor squares a roughness)
-            surface->kind = m[0] == 0.0f ? Surface::Kind::Matte : Surface::Kind::Conductor;
-            surface->color = {m[1], m[2], m[3]};
-            surface->alpha = m[4];
+            surface = Surface::conductorAlpha({m[1], m[2], m[3]}, m[4]);  // the file holds GGX alpha
         }
         auto inst = tracer.buildGSMInstance(geometry, surface, nullptr);
         TransformSRT srt;
@@ -150,11 +154,53 @@ Built buildScene(HIPTracer& tracer, const SceneFile& s) {
     return b;
 }
 
-int render(const char* scenePath, const char* outPath, const std::string& filterType, float alpha) {
+// The host side of module loading (PiperCore.cpp:785-796): dlopen the plugin library, check its
+// protocol, build the module with the host's allocator and ask it for a class.
+struct HostAllocator final : Allocator {
+    void* alloc(size_t size, size_t align) override { return ::operator new(size, std::align_val_t(align)); }
+    void free(void* p) override { ::operator delete(p, std::align_val_t(alignof(std::max_align_t))); }
+};
+struct LoadedModule {
+    void* handle = nullptr;
+    HostAllocator allocator;
+    PiperContext context;
+    Module* module = nullptr;
+    explicit LoadedModule(const char* path) {
+        handle = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+        if (!handle) throw PiperException(std::string("dlopen: ") + dlerror());
+        auto protocol = reinterpret_cast<const char* (*)()>(dlsym(handle, "piperGetProtocol"));
+        auto init = reinterpret_cast<Module* (*)(PiperContext&, Allocator&, const char*)>(dlsym(handle, "piperInitModule"));
+        if (!protocol || !init) throw PiperException("the library exports no Piper module");
+        if (std::string(protocol()) != "GCC@STD@0") throw PiperException(std::string("protocol mismatch: ") + protocol());
+        module = init(context, allocator, path);
+    }
+    ~LoadedModule() {
+        module->~Module();
+        allocator.free(module);
+    }
+};
+std::string pluginPath() {
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<const void*>(&piperInitModule), &info) || !info.dli_fname)
+        throw PiperException("cannot locate the plugin library");
+    return info.dli_fname;
+}
+
+int render(const char* scenePath, const char* outPath, const std::string& filterType, float alpha, bool viaModule) {
     const SceneFile s = load(scenePath);
-    auto acc = std::make_shared<HIPAccelerator>(0);
-    HIPTracer tracer(acc);
-    std::cout << "platform: " << acc->getNativePlatform() << "\n";
+    std::unique_ptr<LoadedModule> mod;
+    std::shared_ptr<HIPTracer> tracerRef;
+    if (viaModule) {  // the pair by class name, as a Piper host loads a Tracer plugin
+        mod = std::make_unique<LoadedModule>(pluginPath().c_str());
+        tracerRef = std::dynamic_pointer_cast<HIPTracer>(
+            mod->module->newInstance("Tracer", Config::parse(R"({"Accelerator": {"Device": 0}})")));
+        if (!tracerRef) throw PiperException("the module's Tracer is not a HIPTracer");
+        std::cout << "module: " << pluginPath() << " protocol " << piperGetProtocol() << "\n";
+    } else {
+        tracerRef = std::make_shared<HIPTracer>(std::make_shared<HIPAccelerator>(0));
+    }
+    HIPTracer& tracer = *tracerRef;
+    std::cout << "platform: " << tracer.getAccelerator().getNativePlatform() << "\n";
     const Built b = buildScene(tracer, s);
     Integrator integrator;
     integrator.maxDepth = s.maxDepth, integrator.minDepthForRussianRoulette = s.minRR, integrator.compatFlags = s.compat;
@@ -177,6 +223,189 @@ int render(const char* scenePath, const char* outPath, const std::string& filter
 }
 
 // Each case must raise PiperException (the reference would hand it to raiseException).
+// Payloads of every built-in component, recorded through a MaterializeContext, one line each:
+// "<component> <program symbol> <payload hex>"; calls made by a payload are listed before it.
+int payloads() {
+    auto acc = HIPAccelerator::hostOnly();
+    HIPTracer tracer(acc);
+    std::vector<std::pair<std::string, SBTPayload>> calls;
+    std::vector<std::shared_ptr<Resource>> resources;
+    auto hex = [](const SBTPayload& p) {
+        std::ostringstream o;
+        for (std::byte b : p) o << std::hex << std::setw(2) << std::setfill('0') << static_cast<int>(b);
+        return o.str();
+    };
+    std::map<const RTProgram*, std::string> programSymbol;  // symbol of each program handed out
+    auto registerCall = [&](const std::shared_ptr<RTProgram>& p, const SBTPayload& payload) {
+        calls.push_back({programSymbol.count(p.get()) ? programSymbol[p.get()] : "?", payload});
+        std::cout << "call " << calls.size() << " " << calls.back().first << " " << hex(payload) << "\n";
+        return reinterpret_cast<CallHandle>(static_cast<uintptr_t>(calls.size()));
+    };
+    struct RecordingTracer final : Tracer {
+        HIPTracer& inner;
+        std::map<const RTProgram*, std::string>& names;
+        RecordingTracer(HIPTracer& t, std::map<const RTProgram*, std::string>& n) : inner(t), names(n) {}
+        std::shared_ptr<AccelerationStructure> buildAcceleration(const GeometryDesc& d) override { return inner.buildAcceleration(d); }
+        std::shared_ptr<GSMInstance> buildGSMInstance(std::shared_ptr<Geometry> g, std::shared_ptr<Surface> s,
+                                                      std::shared_ptr<Medium> m) override {
+            return inner.buildGSMInstance(g, s, m);
+        }
+        std::shared_ptr<Node> buildNode(const std::shared_ptr<Object>& o) override { return inner.buildNode(o); }
+        std::shared_ptr<Node> buildNode(const std::vector<std::pair<TransformInfo, std::shared_ptr<Node>>>& c) override {
+            return inner.buildNode(c);
+        }
+        std::shared_ptr<RTProgram> buildProgram(const LinkableProgram& l, std::string symbol) override {
+            auto p = inner.buildProgram(l, symbol);
+            names[p.get()] = symbol;
+            return p;
+        }
+        std::unique_ptr<Pipeline> buildPipeline(const std::shared_ptr<Node>& s, Integrator& i, RenderDriver& r,
+                                                LightSampler& l, std::shared_ptr<Sampler> sm) override {
+            return inner.buildPipeline(s, i, r, l, sm);
+        }
+        std::shared_ptr<Texture> generateTexture(const Config& d, uint32_t c) const override { return inner.generateTexture(d, c); }
+        size_t getAlignmentRequirement(AlignmentRequirement r) const noexcept override { return inner.getAlignmentRequirement(r); }
+        Accelerator& getAccelerator() const noexcept override { return inner.getAccelerator(); }
+    } rec(tracer, programSymbol);
+    MaterializeContext* ctxPtr = nullptr;
+    const MaterializeContext ctx{rec, *acc,
+                                 [&](std::shared_ptr<Resource> r) {
+                                     resources.push_back(r);
+                                     return static_cast<uint32_t>(resources.size() - 1);
+                                 },
+                                 registerCall,
+                                 [&](const Config& desc, uint32_t channel) {
+                                     const auto t = rec.generateTexture(desc, channel);
+                                     const auto prog = t->materialize(*ctxPtr);
+                                     return registerCall(prog.sample, prog.payload);
+                                 }};
+    ctxPtr = const_cast<MaterializeContext*>(&ctx);
+    auto show = [&](const char* what, const std::shared_ptr<RTProgram>& p, const SBTPayload& payload) {
+        std::cout << what << " " << programSymbol[p.get()] << " " << hex(payload) << "\n";
+    };
+    const auto traversal = reinterpret_cast<TraversalHandle>(uintptr_t(0x1122334455667788ull));
+    {
+        auto m = Surface::matte({0.25f, 0.5f, 0.75f}, 0.125f);
+        const auto p = m->materialize(ctx);
+        show("matte", p.init, p.payload);
+    }
+    {
+        auto g = Surface::glass({0.9f, 0.8f, 0.7f}, {0.6f, 0.5f, 0.4f}, 0.3f, 0.3f);
+        const auto p = g->materialize(ctx);
+        show("glass", p.init, p.payload);
+    }
+    {
+        auto cb = std::make_shared<Surface>(Surface::Kind::Matte,
+                                            std::vector<Config>{checkerboardConfig({0.1f, 0.2f, 0.3f, 1.0f}, {0.9f, 0.8f, 0.7f, 1.0f}, 4.0f),
+                                                                scalarConfig(0.0f)});
+        const auto p = cb->materialize(ctx);
+        show("checkered", p.init, p.payload);
+    }
+    {
+        auto l = Light::area({{{1, 2, 3}, {0.5f, 0, 0}, {0, 0, 2}}, {{4, 5, 6}, {0, 1.5f, 0}, {1, 0, 0}}}, {10, 20, 30});
+        const auto p = l->materialize(traversal, ctx);
+        show("area", p.init, p.payload);
+    }
+    {
+        const auto p = Light::environment({0.1f, 0.2f, 0.3f})->materialize(traversal, ctx);
+        show("environment", p.init, p.payload);
+        const auto q = Light::point({4, 5, 6})->materialize(traversal, ctx);
+        show("point", q.init, q.payload);
+    }
+    {
+        Sensor cam;
+        cam.lookAt = {1, 2, 3}, cam.upRef = {0, 1, 0}, cam.size = {0.036f, 0.024f}, cam.focalLength = 0.05f;
+        cam.apertureRadius = 0.0125f;
+        const auto p = cam.materialize(traversal, ctx);
+        show("sensor", p.rayGen, p.payload);
+    }
+    {
+        Integrator in;
+        in.maxDepth = 12, in.minDepthForRussianRoulette = 3;
+        const auto p = in.materialize(ctx);
+        show("integrator", p.trace, p.payload);
+    }
+    {
+        Sampler smp;
+        smp.samplesPerPixel = 48;
+        const auto a = smp.generatePayload(1920, 1080);
+        std::cout << "sampler " << a.samplesPerPixel << " " << a.maxDimension << " " << hex(a.payload) << "\n";
+    }
+    {
+        LightSampler ls;
+        ls.preprocess({Light::environment({0, 0, 0}), Light::areaPlane({0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 1}),
+                       Light::point({1, 1, 1})});
+        const auto p = ls.materialize(ctx);
+        show("lightsampler", p.select, p.payload);
+    }
+    {
+        FixedSampler fs;
+        fs.filter.filterType = "gaussian", fs.filter.alpha = 2.0f;
+        const auto p = fs.materialize(ctx);
+        show("driver", p.accumulate, p.payload);
+    }
+    std::cout << "resources " << resources.size() << "\n";
+    return 0;
+}
+
+// A native kernel through the accelerator's kernel half (Accelerator.hpp:141-160): compiled from a
+// "Native" table with a static redirect and a dynamic symbol, launched with trivially copyable
+// arguments and a resource look-up table.
+struct KernelProbe {
+    Dim3 size{};
+    uint32_t a = 0;
+    double b = 0.0;
+    size_t resources = 0;
+    const void* dynamicSymbol = nullptr;
+    bool context = false;
+};
+KernelProbe g_probe;
+int g_marker = 0;
+void probeKernel(const NativeLaunch& l) {
+    g_probe.size = l.size;
+    g_probe.a = l.args.get<uint32_t>(0);
+    g_probe.b = l.args.get<double>(1);
+    g_probe.resources = l.resources.size();
+    g_probe.dynamicSymbol = l.dynamicSymbols.at(0);
+    auto lock = l.context.makeCurrent();
+    g_probe.context = true;
+}
+int kernel() {
+    auto acc = HIPAccelerator::hostOnly();
+    const LinkableProgram table = makeNativeLinkable({{"probe", reinterpret_cast<const void*>(&probeKernel)},
+                                                      {"marker", &g_marker}},
+                                                     7);
+    auto k = acc->compileKernel({table}, {{"main", "probe"}}, {"marker"}, "main");
+    auto lut = acc->createResourceLUT({acc->createBuffer(64, 16, nullptr), acc->createBuffer(32, 16, nullptr)});
+    auto f = acc->launchKernel(Dim3{3, 4, 5}, k, lut, uint32_t{77}, 2.5);
+    if (g_probe.a != 0) return 1;  // nothing runs before the future is polled
+    f.getSync();
+    const bool ok = g_probe.size.x == 3 && g_probe.size.y == 4 && g_probe.size.z == 5 && g_probe.a == 77 &&
+                    g_probe.b == 2.5 && g_probe.resources == 2 && g_probe.dynamicSymbol == &g_marker && g_probe.context;
+    std::cout << (ok ? "kernel ok" : "kernel mismatch") << " contexts " << acc->enumerateContexts().size() << "\n";
+    return ok ? 0 : 1;
+}
+
+// Module entry points on the host: protocol, classes, an unknown class.
+int module() {
+    LoadedModule mod(pluginPath().c_str());
+    auto acc = std::dynamic_pointer_cast<HIPAccelerator>(
+        mod.module->newInstance("Accelerator", Config::parse(R"({"HostOnly": true})")));
+    auto tracer = std::dynamic_pointer_cast<HIPTracer>(
+        mod.module->newInstance("Tracer", Config::parse(R"({"Accelerator": {"HostOnly": true}})")));
+    std::cout << "protocol " << piperGetProtocol() << "\n";
+    std::cout << "accelerator " << (acc ? acc->getNativePlatform() : "none") << " formats "
+              << (acc ? acc->getSupportedLinkableFormat().at(0) : "") << "\n";
+    std::cout << "tracer " << (tracer ? "HIPTracer" : "none") << "\n";
+    try {
+        (void)mod.module->newInstance("Denoiser", Config::makeObject({}));
+        std::cout << "unknown class: no error\n";
+    } catch (const PiperException& e) {
+        std::cout << "unknown class: raised: " << e.what() << "\n";
+    }
+    return acc && tracer ? 0 : 1;
+}
+
 int errors(const char* scenePath) {
     const SceneFile s = load(scenePath);
     auto acc = std::make_shared<HIPAccelerator>(0);
@@ -192,8 +421,44 @@ int errors(const char* scenePath) {
             std::cout << name << ": raised: " << e.what() << "\n";
         }
     };
-    expect("unknown symbol", [&] { (void)tracer.buildProgram({"HIP-gfx950-builtin", {"orenNayarInit"}}, "orenNayarInit"); });
-    expect("foreign format", [&] { (void)tracer.buildProgram({"LLVM IR", {"trace"}}, "trace"); });
+    expect("unknown symbol", [&] {
+        int marker = 0;
+        (void)tracer.buildProgram(makeNativeLinkable({{"orenNayarInit", &marker}}, 1), "orenNayarInit");
+    });
+    expect("missing symbol", [&] { (void)tracer.buildProgram(hipBuiltinProgram(), "disneyInit"); });
+    expect("foreign format", [&] {
+        LinkableProgram ir;
+        ir.format = "LLVM IR";
+        (void)tracer.buildProgram(ir, "trace");
+    });
+    expect("corrupt native table", [&] {
+        LinkableProgram p = makeNativeLinkable({{"trace", nullptr}}, 2);
+        p.exchange.resize(p.exchange.size() - 3);
+        (void)tracer.buildProgram(p, "trace");
+    });
+    expect("kernel of a foreign format", [&] {
+        LinkableProgram ir;
+        ir.format = "LLVM IR";
+        (void)acc->compileKernel({ir}, {}, {}, "piperMain");
+    });
+    expect("undefined kernel entry", [&] { (void)acc->compileKernel({hipBuiltinProgram()}, {}, {}, "piperMain"); });
+    expect("built-in as kernel entry", [&] { (void)acc->compileKernel({hipBuiltinProgram()}, {}, {}, "trace"); });
+    expect("texture channels", [&] { (void)tracer.generateTexture(colourConfig({1, 1, 1}), 1); });
+    expect("anisotropic glass", [&] {
+        auto glass = std::make_shared<Surface>(Surface::Kind::Glass,
+                                               std::vector<Config>{colourConfig({1, 1, 1}), colourConfig({1, 1, 1}),
+                                                                   scalarConfig(0.1f), scalarConfig(0.2f)});
+        auto buf = acc->createBuffer(48, 16, [](std::byte* p) {
+            const float v[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+            const uint32_t idx[3] = {0, 1, 2};
+            std::memcpy(p, v, 36);
+            std::memcpy(p + 36, idx, 12);
+        });
+        GeometryDesc gd;
+        gd.desc = TriangleIndexedGeometryDesc{3, 1, buf, 48, 0, 36};
+        (void)exportScene(tracer, tracer.buildNode({{TransformInfo{}, tracer.buildNode(tracer.buildGSMInstance(
+                                                                          std::make_shared<Geometry>(gd), glass, nullptr))}}));
+    });
     expect("motion blur", [&] {
         TransformInfo info;
         info.transforms = {{0u, identitySRT()}, {1u, identitySRT()}};
@@ -244,9 +509,16 @@ int errors(const char* scenePath) {
 int main(int argc, char** argv) {
     try {
         if (argc == 3 && std::string(argv[1]) == "--errors") return errors(argv[2]);
-        if (argc >= 3 && argc <= 5 && std::string(argv[1]) != "--errors")
-            return render(argv[1], argv[2], argc >= 4 ? argv[3] : "box", argc == 5 ? std::stof(argv[4]) : 0.0f);
-        std::cerr << "usage: render_scene SCENE OUT [FILTER [ALPHA]] | render_scene --errors SCENE\n";
+        if (argc == 2 && std::string(argv[1]) == "--payloads") return payloads();
+        if (argc == 2 && std::string(argv[1]) == "--kernel") return kernel();
+        if (argc == 2 && std::string(argv[1]) == "--module") return module();
+        const bool viaModule = argc >= 2 && std::string(argv[1]) == "--via-module";
+        const int a0 = viaModule ? 2 : 1;
+        if (argc - a0 >= 2 && argc - a0 <= 4 && std::string(argv[1]) != "--errors")
+            return render(argv[a0], argv[a0 + 1], argc - a0 >= 3 ? argv[a0 + 2] : "box",
+                          argc - a0 == 4 ? std::stof(argv[a0 + 3]) : 0.0f, viaModule);
+        std::cerr << "usage: render_scene [--via-module] SCENE OUT [FILTER [ALPHA]] | render_scene --errors SCENE | "
+                     "render_scene --payloads | --kernel | --module\n";
         return 2;
     } catch (const PiperException& e) {
         std::cerr << "error: " << e.what() << "\n";

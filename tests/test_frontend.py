@@ -208,12 +208,13 @@ def test_exr_layout_and_half_rounding(tmp_path):
 @pytest.mark.parametrize("mutate,message", [
     (lambda s: s.update(Integrator={"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator"}), "MaxTraceDepth"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(Roughness={"ClassID": "Piper.BuiltinComponent."
-                                                                      "ProceduralTexture.Marble"}), "ConstantTexture"),
+                                                                      "ProceduralTexture.Marble"}),
+     "unsupported texture class"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(Diffuse={"WrapMode": "Repeat", "Image": {
         "ClassID": "Piper.BuiltinComponent.SampledTexture.Image", "Path": "picture.png"}}), "cannot open image"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(
         Diffuse={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [1, 1, 1]}),
-     "unsupported channel count 3"),
+     "Unsupported channel 3"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(
         Roughness={"ClassID": "Piper.BuiltinComponent.ProceduralTexture.ConstantTexture", "Value": [0.1, 0.2]}),
      "Mismatched channel. Expect 1, but get 2"),

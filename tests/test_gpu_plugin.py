@@ -27,10 +27,10 @@ def direct_film(dev, sc, flags=0, film_filter=0, alpha=0.0):
     return f.download()
 
 
-def plugin_film(tmp_path, sc, name, filter_args=(), **kw):
+def plugin_film(tmp_path, sc, name, filter_args=(), via_module=False, **kw):
     path, out = tmp_path / f"{name}.pgsc", tmp_path / f"{name}.rgbw"
     write_scene(sc, path, **kw)
-    r = run_driver(str(path), str(out), *filter_args)
+    r = run_driver(*(["--via-module"] if via_module else []), str(path), str(out), *filter_args)
     assert r.returncode == 0, r.stdout + r.stderr
     film = np.fromfile(out, np.float32).reshape(sc["height"], sc["width"], 4)
     return film, r.stdout
@@ -43,6 +43,18 @@ def test_fixed_sampler_frame_matches_pg_render(dev, tmp_path):
     # the 8x8 tiles of the frame were coalesced into one render call over the whole rect
     assert "1 render calls" in log and f"{100 * 68 * 8} paths" in log, log
     assert re.search(r"rect: 0 0 100 68", log)
+
+
+def test_module_loaded_pair_matches_pg_render(dev, tmp_path):
+    """The pair as a Piper host loads it: dlopen the plugin library, check piperGetProtocol, build
+    the module with the host's allocator and ask it for the "Tracer" class (with its "Accelerator"
+    config); every component then reaches the device through its reference-layout payload
+    (MatteData, DiffuseAreaLightData + CDFData, PCData, SobolData, Data, RDData + GaussianFilterData)
+    and the pipeline's render kernel (compileKernel / launchKernel). The film equals pg_render's."""
+    sc = scenes.cornell(width=80, height=64, spp=8, plastic=True, specular=True, point_lights=True)
+    film, log = plugin_film(tmp_path, sc, "module", filter_args=("gaussian", "1.5"), via_module=True)
+    assert "protocol GCC@STD@0" in log, log
+    assert np.array_equal(film, direct_film(dev, sc, film_filter=pg.PG_FILTER_GAUSSIAN, alpha=1.5))
 
 
 @pytest.mark.parametrize("name,code,alpha", [("triangle", pg.PG_FILTER_TRIANGLE, 0.0),
@@ -121,4 +133,10 @@ def test_error_paths_raise(tmp_path):
     write_scene(sc, path)
     r = run_driver("--errors", str(path))
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "raised 7 of 7" in r.stdout, r.stdout
+    assert "raised 14 of 14" in r.stdout, r.stdout
+    for case in ("foreign format: raised: buildProgram: unsupported linkable format LLVM IR",
+                 "kernel of a foreign format: raised: Unsupported linkable format LLVM IR",
+                 "undefined kernel entry: raised: compileKernel: undefined symbol piperMain",
+                 "texture channels: raised: Mismatched channel. Expect 1, but get 4",
+                 "anisotropic glass: raised: glass: anisotropic roughness is not supported"):
+        assert case in r.stdout, case
