@@ -9,35 +9,79 @@ namespace pg {
 // ------------------------------------------------------------------------------------------
 // Inputs of one sample
 // ------------------------------------------------------------------------------------------
+// A source hands out each lane's sample in stages so that its dependent loads can be in flight
+// while the wave computes the previous samples: begin(i) issues the first loads of entry i (~0u:
+// none), stage2 .. stage4 the loads that depend on them, finish() turns them into the inputs.
+__device__ __forceinline__ GuideInput noInput() {
+    return guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
+}
+// Inputs at a hit: position in the scene box mapped to [0,1]^3, the geometric normal facing the
+// ray (as triNormal) and the outgoing direction.
+__device__ __forceinline__ GuideInput hitInput(float4 oo, float4 dd, float t, float4 a, float4 b, float4 c, float3 lo,
+                                               float3 invExt) {
+    const V3 dir{dd.x, dd.y, dd.z};
+    const V3 p = V3{oo.x, oo.y, oo.z} + dir * t;
+    const V3 v0 = xyz(a), v1 = xyz(b), v2 = xyz(c);
+    const V3 ng = normalize(cross(v1 - v0, v2 - v0));
+    const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;
+    const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
+                                   fminf(fmaxf((p.y - lo.y) * invExt.y, 0.0f), 1.0f),
+                                   fminf(fmaxf((p.z - lo.z) * invExt.z, 0.0f), 1.0f));
+    return guideInput(p01, make_float3(n.x, n.y, n.z), make_float3(-dir.x, -dir.y, -dir.z));
+}
+
 // From an explicit array: 3 float4 per sample = (pos01, -) (normal, -) (wo, -).
 struct ArraySource {
     const float4* in;
-    __device__ __forceinline__ GuideInput get(uint32_t i) const {
-        const float4 p = in[3 * i], n = in[3 * i + 1], w = in[3 * i + 2];
-        return guideInput(make_float3(p.x, p.y, p.z), make_float3(n.x, n.y, n.z), make_float3(w.x, w.y, w.z));
+    struct Pend {
+        float4 p, n, w;
+        bool valid;
+    };
+    __device__ __forceinline__ Pend begin(uint32_t i) const {
+        Pend q{};
+        q.valid = i != ~0u;
+        if (q.valid) q.p = in[3 * i], q.n = in[3 * i + 1], q.w = in[3 * i + 2];
+        return q;
+    }
+    __device__ __forceinline__ void stage2(Pend&) const {}
+    __device__ __forceinline__ void stage3(Pend&) const {}
+    __device__ __forceinline__ void stage4(Pend&) const {}
+    __device__ __forceinline__ GuideInput finish(const Pend& q) const {
+        if (!q.valid) return noInput();
+        return guideInput(make_float3(q.p.x, q.p.y, q.p.z), make_float3(q.n.x, q.n.y, q.n.z),
+                          make_float3(q.w.x, q.w.y, q.w.z));
     }
 };
-// From the path state of an extension-queue entry (hit point, shading normal, -ray direction).
+// From the path state of an extension-queue entry (hit point, shading normal, -ray direction):
+// queue entry -> path record -> hit triangle.
 struct PathSource {
     const float4* tris;  // leaf-order triangles of the scene (only the hit triangle's normal is needed)
     const PathRec* paths;
     const uint32_t* queue;
     float3 lo, invExt;
-    __device__ __forceinline__ GuideInput get(uint32_t i) const {
-        const uint32_t path = queue[i];
-        const float4 h = paths[path].hit;
-        const uint32_t pos = __float_as_uint(h.y);
-        if (pos == kHitMiss || (pos & kHitQuad)) return guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
-        const float4 oo = paths[path].o, dd = paths[path].d;
-        const V3 dir{dd.x, dd.y, dd.z};
-        const V3 p = V3{oo.x, oo.y, oo.z} + dir * h.x;
-        const V3 v0 = xyz(tris[3 * pos]), v1 = xyz(tris[3 * pos + 1]), v2 = xyz(tris[3 * pos + 2]);
-        const V3 ng = normalize(cross(v1 - v0, v2 - v0));
-        const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;  // as triNormal
-        const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
-                                       fminf(fmaxf((p.y - lo.y) * invExt.y, 0.0f), 1.0f),
-                                       fminf(fmaxf((p.z - lo.z) * invExt.z, 0.0f), 1.0f));
-        return guideInput(p01, make_float3(n.x, n.y, n.z), make_float3(-dir.x, -dir.y, -dir.z));
+    struct Pend {
+        uint32_t path;
+        float4 h, o, d, a, b, c;
+        bool valid, tri;
+    };
+    __device__ __forceinline__ Pend begin(uint32_t i) const {
+        Pend q{};
+        q.valid = i != ~0u;
+        if (q.valid) q.path = queue[i];
+        return q;
+    }
+    __device__ __forceinline__ void stage2(Pend& q) const {
+        if (q.valid) q.h = paths[q.path].hit, q.o = paths[q.path].o, q.d = paths[q.path].d;
+    }
+    __device__ __forceinline__ void stage3(Pend& q) const {
+        const uint32_t pos = __float_as_uint(q.h.y);
+        q.tri = q.valid && pos != kHitMiss && !(pos & kHitQuad);
+        if (q.tri) q.a = tris[3 * pos], q.b = tris[3 * pos + 1], q.c = tris[3 * pos + 2];
+    }
+    __device__ __forceinline__ void stage4(Pend&) const {}
+    __device__ __forceinline__ GuideInput finish(const Pend& q) const {
+        if (!q.tri) return noInput();
+        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, lo, invExt);
     }
 };
 
@@ -88,46 +132,55 @@ __device__ __forceinline__ void waveLdsSync() {
 // Each wave takes 64 samples at a time: every lane gathers one sample's inputs (one dependent load
 // chain per 64 samples) into the wave's LDS stage, then the wave runs four 16-sample MFMA tiles.
 constexpr int kInferStageBytes = 64 * 3 * 16;  // per wave: 64 samples x 3 float4
-// 8 waves share one LDS copy of the weights: 37 KB + 8 x 3 KB, two blocks per CU (4 waves per SIMD)
+// 8 waves share one LDS copy of the weights: 54 KB + 8 x 3 KB, two blocks per CU (4 waves per SIMD)
 constexpr int kInferBlock = 512;
 static_assert(kInferBlock == static_cast<int>(kGroupUnit), "a grouped unit is one block's 8 x 64 samples");
 
-// Mixtures of one wave's 64 samples: lane l's sample is queue/array entry `entry` (~0u: none).
-// Wave-uniform; the stage is the wave's own LDS slice.
-template <typename Src>
-__device__ __forceinline__ void inferWave64(const Src& src, uint32_t entry, const GuideLds& L, float4* stage,
-                                            float* __restrict__ mix) {
+// Mixtures of one 16-sample tile of the wave's stage (tile t: stage entries 16t..16t+15).
+__device__ __forceinline__ void inferTile(int t, const GuideLdsX& L, const float4* stage, float* __restrict__ mix) {
     const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
-    const bool valid = entry != ~0u;
-    {
-        const GuideInput in = valid ? src.get(entry) : guideInput(make_float3(0, 0, 0), make_float3(0, 0, 1), make_float3(0, 0, 1));
-        stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
-        stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
-        stage[3 * lane + 2] = make_float4(in.wz, __uint_as_float(entry), 0.0f, 0.0f);
+    const int k = 16 * t + s;
+    const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
+    const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
+    const uint32_t i = __float_as_uint(c.y);
+    bf16x4 x0[9];
+    guideFeatureTiles(in, grp, x0);
+    f32x4 out[3];
+    guideForwardTileX32(L, x0, out);
+    const LobePair lp = lobesFromTile(out);
+    if (i != ~0u) {
+        float* o = mix + static_cast<size_t>(i) * kGMixFloats;
+        reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
+        reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
+        reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
+        reinterpret_cast<float2*>(o + 40)[grp] =
+            make_float2(lp.pi[0] * vmfNormSampled(lp.kappa[0]), lp.pi[1] * vmfNormSampled(lp.kappa[1]));
     }
-    const uint64_t live = __ballot(valid);
+}
+
+// Mixtures of one wave's 64 samples: lane l's sample is queue/array entry `entry` (~0u: none) with
+// inputs `in`. While the four tiles run, the lane's next entry `next` goes through the source's
+// load stages; returns its inputs. Wave-uniform; the stage is the wave's own LDS slice.
+template <typename Src>
+__device__ __forceinline__ GuideInput inferWave64(const Src& src, uint32_t entry, const GuideInput& in, uint32_t next,
+                                                  const GuideLdsX& L, float4* stage, float* __restrict__ mix) {
+    const int lane = __lane_id();
+    stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
+    stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
+    stage[3 * lane + 2] = make_float4(in.wz, __uint_as_float(entry), 0.0f, 0.0f);
+    const uint64_t live = __ballot(entry != ~0u);
     waveLdsSync();  // the wave's own stage: written and read by its lanes only
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        if (((live >> (16 * t)) & 0xFFFFull) == 0) continue;  // wave-uniform
-        const int k = 16 * t + s;
-        const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
-        const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
-        const uint32_t i = __float_as_uint(c.y);
-        bf16x4 x0[9];
-        guideFeatureTiles(in, grp, x0);
-        f32x4 out[3];
-        guideForwardTile(L, x0, out);
-        const LobePair lp = lobesFromTile(out);
-        if (i != ~0u) {
-            float* o = mix + static_cast<size_t>(i) * kGMixFloats;
-            reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
-            reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
-            reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
-            reinterpret_cast<float2*>(o + 40)[grp] = make_float2(lp.pi[0] * vmfNorm(lp.kappa[0]), lp.pi[1] * vmfNorm(lp.kappa[1]));
-        }
-    }
+    typename Src::Pend p = src.begin(next);
+    if (live & 0xFFFFull) inferTile(0, L, stage, mix);  // wave-uniform tests
+    src.stage2(p);
+    if ((live >> 16) & 0xFFFFull) inferTile(1, L, stage, mix);
+    src.stage3(p);
+    if ((live >> 32) & 0xFFFFull) inferTile(2, L, stage, mix);
+    src.stage4(p);
+    if (live >> 48) inferTile(3, L, stage, mix);
+    const GuideInput nin = src.finish(p);
     waveLdsSync();  // stage reads done before the next samples overwrite it
+    return nin;
 }
 
 template <typename Src>
@@ -136,14 +189,22 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHo
                                                            float* __restrict__ mix) {
     extern __shared__ unsigned char smem[];
     const uint32_t n = nDev ? *nDev : nHost;
-    const GuideLds L = guideLdsLayout(smem);
-    float4* stage = reinterpret_cast<float4*>(smem + kGLdsWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
-    guideLoadLds(L, wb, theta);
+    const GuideLdsX L = guideLdsXLayout(smem);
+    float4* stage = reinterpret_cast<float4*>(smem + kGLdsXWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
+    guideLoadLdsX(L, wb, theta);
     __syncthreads();  // the weights are shared by the block's waves; after this each wave runs alone
     const uint32_t lane = __lane_id();
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, waves = gridDim.x * (blockDim.x / 64);
-    for (uint32_t base = wave * 64; base < n; base += waves * 64)  // wave-uniform
-        inferWave64(src, base + lane < n ? base + lane : ~0u, L, stage, mix);
+    auto entryAt = [&](uint32_t base) { return base < n && base + lane < n ? base + lane : ~0u; };
+    uint32_t base = wave * 64, entry = entryAt(base);
+    typename Src::Pend p = src.begin(entry);
+    src.stage2(p), src.stage3(p), src.stage4(p);
+    GuideInput in = src.finish(p);
+    for (; base < n; base += waves * 64) {  // wave-uniform
+        const uint32_t next = entryAt(base + waves * 64);
+        in = inferWave64(src, entry, in, next, L, stage, mix);
+        entry = next;
+    }
 }
 
 // ---- guided batches: inference grouped by guide --------------------------------------------------
@@ -154,11 +215,35 @@ struct BatchPathSource {
     const PathRec* paths;
     const uint32_t* queue;
     __device__ __forceinline__ uint32_t key(uint32_t i) const { return guide[paths[queue[i]].meta.z >> 16].key; }
-    __device__ __forceinline__ GuideInput get(uint32_t i) const {
-        const uint32_t path = queue[i];
-        const uint32_t item = paths[path].meta.z >> 16;
-        const PathSource one{scenes[item].tris, paths, queue, guide[item].lo, guide[item].invExt};
-        return one.get(i);
+    // queue entry -> path record (with its item) -> the item's scene and box -> hit triangle
+    struct Pend {
+        uint32_t path, item;
+        float4 h, o, d, a, b, c;
+        const float4* tris;
+        float3 lo, invExt;
+        bool valid, tri;
+    };
+    __device__ __forceinline__ Pend begin(uint32_t i) const {
+        Pend q{};
+        q.valid = i != ~0u;
+        if (q.valid) q.path = queue[i];
+        return q;
+    }
+    __device__ __forceinline__ void stage2(Pend& q) const {
+        if (q.valid)
+            q.h = paths[q.path].hit, q.o = paths[q.path].o, q.d = paths[q.path].d, q.item = paths[q.path].meta.z >> 16;
+    }
+    __device__ __forceinline__ void stage3(Pend& q) const {
+        if (q.valid) q.tris = scenes[q.item].tris, q.lo = guide[q.item].lo, q.invExt = guide[q.item].invExt;
+    }
+    __device__ __forceinline__ void stage4(Pend& q) const {
+        const uint32_t pos = __float_as_uint(q.h.y);
+        q.tri = q.valid && pos != kHitMiss && !(pos & kHitQuad);
+        if (q.tri) q.a = q.tris[3 * pos], q.b = q.tris[3 * pos + 1], q.c = q.tris[3 * pos + 2];
+    }
+    __device__ __forceinline__ GuideInput finish(const Pend& q) const {
+        if (!q.tri) return noInput();
+        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, q.lo, q.invExt);
     }
 };
 
@@ -246,24 +331,38 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSourc
                                                                   const float* const* __restrict__ keyTheta,
                                                                   float* __restrict__ mix) {
     extern __shared__ unsigned char smem[];
-    const GuideLds L = guideLdsLayout(smem);
-    float4* stage = reinterpret_cast<float4*>(smem + kGLdsWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
+    const GuideLdsX L = guideLdsXLayout(smem);
+    float4* stage = reinterpret_cast<float4*>(smem + kGLdsXWeightBytes + (threadIdx.x / 64) * kInferStageBytes);
     const uint32_t units = gb.keyStart[nKeys] / kGroupUnit;
     const uint32_t per = (units + gridDim.x - 1) / gridDim.x;
     const uint32_t u0 = blockIdx.x * per, u1 = min(units, u0 + per);
     const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
-    uint32_t key = 0, loaded = ~0u;
+    // the guide of unit u (keys in unit order, so one forward scan per block)
+    uint32_t scanKey = 0;
+    auto keyOf = [&](uint32_t u) {
+        while (scanKey + 1 < nKeys && gb.keyStart[scanKey + 1] <= u * kGroupUnit) ++scanKey;
+        return scanKey;
+    };
+    auto entryOf = [&](uint32_t u, uint32_t k) {
+        const uint32_t pos = u * kGroupUnit + wave * 64 + lane;
+        return pos < gb.keyEnd[k] ? gb.perm[pos] : ~0u;
+    };
+    uint32_t loaded = ~0u;
+    uint32_t key = u0 < u1 ? keyOf(u0) : 0, entry = u0 < u1 ? entryOf(u0, key) : ~0u;
+    BatchPathSource::Pend p = src.begin(entry);
+    src.stage2(p), src.stage3(p), src.stage4(p);
+    GuideInput in = src.finish(p);
     for (uint32_t u = u0; u < u1; ++u) {  // block-uniform
-        const uint32_t pos0 = u * kGroupUnit;
-        while (key + 1 < nKeys && gb.keyStart[key + 1] <= pos0) ++key;
         if (key != loaded) {
             __syncthreads();  // every wave is done with the previous weights
-            guideLoadLds(L, keyWb[key], keyTheta[key]);
+            guideLoadLdsX(L, keyWb[key], keyTheta[key]);
             __syncthreads();
             loaded = key;
         }
-        const uint32_t pos = pos0 + wave * 64 + lane;
-        inferWave64(src, pos < gb.keyEnd[key] ? gb.perm[pos] : ~0u, L, stage, mix);
+        const uint32_t nextKey = u + 1 < u1 ? keyOf(u + 1) : key;
+        const uint32_t next = u + 1 < u1 ? entryOf(u + 1, nextKey) : ~0u;
+        in = inferWave64(src, entry, in, next, L, stage, mix);
+        key = nextKey, entry = next;
     }
 }
 
@@ -483,7 +582,8 @@ __global__ void __launch_bounds__(256) kGuideSumC(const float4* rec, uint32_t n,
 // ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
-constexpr size_t kInferLds = kGLdsWeightBytes + (kInferBlock / 64) * kInferStageBytes;
+constexpr size_t kInferLds = kGLdsXWeightBytes + (kInferBlock / 64) * kInferStageBytes;
+static_assert(2 * kInferLds <= 160 * 1024, "two inference blocks (16 waves) fit one CU's LDS");
 constexpr size_t kTrainLds = kGLdsWeightBytes + kGLdsTransBytes;
 
 static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {

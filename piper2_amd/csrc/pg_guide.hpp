@@ -2,8 +2,9 @@
 // and the 8-lobe von Mises-Fisher mixture used as a one-sample-MIS proposal next to the BSDF.
 //
 // Network: 144 input features -> 4 hidden layers of 64 (ReLU) -> 40 outputs in 48 rows (8 zero
-// rows). Weights are row-major [out][in] FP32 masters with a BF16 copy; matrix products run on
-// v_mfma_f32_16x16x16_bf16 with FP32 accumulation. Inputs and outputs are ordered so that each lane
+// rows). Weights are row-major [out][in] FP32 masters with a BF16 copy; matrix products run with
+// FP32 accumulation on v_mfma_f32_16x16x32_bf16 (inference) and v_mfma_f32_16x16x16_bf16
+// (training), one instruction shape per accumulator chain. Inputs and outputs are ordered so that each lane
 // computes the features it feeds and decodes the lobes it holds (guideFeatureTiles, output layout).
 //
 // MFMA operand layout used throughout (lane l, g = l / 16, s = l % 16, j = 0..3):
@@ -44,7 +45,12 @@ constexpr int kGMixFloats = 48;
 constexpr int kGLd0 = kGIn + 4, kGLdH = kGHid + 4, kGLdOT = kGOut + 4;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 // Round to nearest even: gfx950's v_cvt_pk_bf16_f32 (what a __bf16 cast lowers to).
 __device__ __forceinline__ uint16_t toBf16(float f) {
@@ -58,22 +64,31 @@ __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float(s
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// gfx950's K = 32 BF16 product (twice the K of mfma16 per instruction). Operand layout, lane l
+// (s = l % 16, g = l / 16): A holds A[s][8g + j], B holds B[8g + j][s] (j = 0..7); D as mfma16.
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c, 0,
+                                                  0, 0);
+}
+__device__ __forceinline__ bf16x8 cat(bf16x4 lo, bf16x4 hi) {
+    return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// Two floats to two BF16 (round to nearest even) in one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ i16x2 toBf16x2(float a, float b) {
+    return __builtin_bit_cast(i16x2, __builtin_convertvector(f32x2{a, b}, bf16v2));
+}
+// ReLU on BF16 bit patterns: as signed 16-bit integers every negative value (and -0) is below 0,
+// positive values keep their order, so max(x, 0) in v_pk_max_i16 is the float ReLU.
+__device__ __forceinline__ i16x2 reluBf16x2(i16x2 v) { return __builtin_elementwise_max(v, i16x2{0, 0}); }
 
 __device__ __forceinline__ bf16x4 packRelu(f32x4 v) {
-    bf16x4 r;
-    r[0] = static_cast<short>(toBf16(fmaxf(v[0], 0.0f)));
-    r[1] = static_cast<short>(toBf16(fmaxf(v[1], 0.0f)));
-    r[2] = static_cast<short>(toBf16(fmaxf(v[2], 0.0f)));
-    r[3] = static_cast<short>(toBf16(fmaxf(v[3], 0.0f)));
-    return r;
+    const i16x2 lo = reluBf16x2(toBf16x2(v[0], v[1])), hi = reluBf16x2(toBf16x2(v[2], v[3]));
+    return bf16x4{lo[0], lo[1], hi[0], hi[1]};
 }
 __device__ __forceinline__ bf16x4 pack(f32x4 v) {
-    bf16x4 r;
-    r[0] = static_cast<short>(toBf16(v[0]));
-    r[1] = static_cast<short>(toBf16(v[1]));
-    r[2] = static_cast<short>(toBf16(v[2]));
-    r[3] = static_cast<short>(toBf16(v[3]));
-    return r;
+    const i16x2 lo = toBf16x2(v[0], v[1]), hi = toBf16x2(v[2], v[3]);
+    return bf16x4{lo[0], lo[1], hi[0], hi[1]};
 }
 
 // Weights of one block in LDS (BF16, padded rows) and biases (FP32).
@@ -182,9 +197,7 @@ __device__ __forceinline__ void guideFeatureTiles(const GuideInput& g, int grp, 
     v[34] = grp == 0 ? g.nx : (grp == 1 ? g.nz : 0.0f);
     v[35] = grp == 0 ? g.ny : 0.0f;
 #pragma unroll
-    for (int kt = 0; kt < 9; ++kt)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) x0[kt][j] = static_cast<short>(toBf16(v[4 * kt + j]));
+    for (int kt = 0; kt < 9; ++kt) x0[kt] = pack(f32x4{v[4 * kt], v[4 * kt + 1], v[4 * kt + 2], v[4 * kt + 3]});
 }
 
 // Forward pass of one 16-sample tile. x0: layer-0 B operands (9 K tiles). out: 3 D tiles of the
@@ -234,6 +247,126 @@ __device__ __forceinline__ void guideForwardTile(const GuideLds& L, const bf16x4
     }
 }
 
+// ---- inference on K = 32 products --------------------------------------------------------------
+// A K = 32 tile's B operand is two K = 16 tiles of the layout above side by side: lane group g holds
+// rows 4g..4g+3 of the first and of the second, which are K positions 8g..8g+7 of the K = 32 tile.
+// So a layer's D tiles mt = 2kt, 2kt + 1 are the next layer's K = 32 B operand kt as they stand, and
+// the layer-0 feature tiles x0[2kt], x0[2kt + 1] are layer 0's K = 32 operand kt, provided the A
+// operand (the weights) holds, at K position 32kt + 8g + j, the input 32kt + 4g + j (j < 4) or
+// 32kt + 16 + 4g + (j - 4) (j >= 4). The inference LDS copy stores every 32-column block of every
+// weight row in that order; layer 0's last 16 columns (features 128..143) stay in a K = 16 product.
+// Parameters, features and outputs are unchanged.
+constexpr int kGLd0X = kGIn + 8, kGLdHX = kGHid + 8;  // 16-byte rows; 16 rows of a read hit distinct banks
+struct GuideLdsX {
+    uint16_t* w0;  // [64][kGLd0X]
+    uint16_t* wh;  // 3 x [64][kGLdHX]
+    uint16_t* w4;  // [48][kGLdHX]
+    float* b;      // 64 * 4 + 48
+};
+constexpr int kGLdsXWeightBytes = (kGHid * kGLd0X + 3 * kGHid * kGLdHX + kGOut * kGLdHX) * 2 + (4 * kGHid + kGOut) * 4;
+static_assert(((kGHid * kGLd0X + 3 * kGHid * kGLdHX + kGOut * kGLdHX) * 2) % 16 == 0, "16-byte aligned biases");
+
+__device__ __forceinline__ GuideLdsX guideLdsXLayout(unsigned char* base) {
+    GuideLdsX L;
+    L.w0 = reinterpret_cast<uint16_t*>(base);
+    L.wh = L.w0 + kGHid * kGLd0X;
+    L.w4 = L.wh + 3 * kGHid * kGLdHX;
+    L.b = reinterpret_cast<float*>(L.w4 + kGOut * kGLdHX);
+    return L;
+}
+
+// Rows of COLS BF16 weights into LDS rows of stride LD, each 32-column block below PERM in the K = 32
+// operand order (8 consecutive LDS elements 32b + 8q.. = inputs 32b + 4q..+3 and 32b + 16 + 4q..+3),
+// the columns from PERM on copied as they are.
+template <int ROWS, int COLS, int LD, int PERM>
+__device__ __forceinline__ void guideCopyRowsX(uint16_t* dst, const uint16_t* __restrict__ src) {
+    static_assert(COLS % 8 == 0 && PERM % 32 == 0 && LD % 8 == 0, "16-byte groups");
+    constexpr int kVec = COLS / 8;
+    for (int i = threadIdx.x; i < ROWS * kVec; i += blockDim.x) {
+        const int r = i / kVec, q8 = i - r * kVec, c = 8 * q8;
+        const uint16_t* row = src + r * COLS;
+        uint4 w;
+        if (c < PERM) {
+            const int b = c & ~31, q = (c & 31) >> 3;
+            const uint2 lo = *reinterpret_cast<const uint2*>(row + b + 4 * q);
+            const uint2 hi = *reinterpret_cast<const uint2*>(row + b + 16 + 4 * q);
+            w = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        } else {
+            w = *reinterpret_cast<const uint4*>(row + c);
+        }
+        *reinterpret_cast<uint4*>(dst + r * LD + c) = w;
+    }
+}
+
+__device__ __forceinline__ void guideLoadLdsX(const GuideLdsX& L, const uint16_t* __restrict__ wb,
+                                              const float* __restrict__ theta) {
+    guideCopyRowsX<kGHid, kGIn, kGLd0X, 128>(L.w0, wb + kGW0);
+    guideCopyRowsX<kGHid, kGHid, kGLdHX, kGHid>(L.wh, wb + kGW1);
+    guideCopyRowsX<kGHid, kGHid, kGLdHX, kGHid>(L.wh + kGHid * kGLdHX, wb + kGW2);
+    guideCopyRowsX<kGHid, kGHid, kGLdHX, kGHid>(L.wh + 2 * kGHid * kGLdHX, wb + kGW3);
+    guideCopyRowsX<kGOut, kGHid, kGLdHX, kGHid>(L.w4, wb + kGW4);
+    for (int i = threadIdx.x; i < kGHid; i += blockDim.x) {
+        L.b[i] = theta[kGB0 + i];
+        L.b[kGHid + i] = theta[kGB1 + i];
+        L.b[2 * kGHid + i] = theta[kGB2 + i];
+        L.b[3 * kGHid + i] = theta[kGB3 + i];
+    }
+    for (int i = threadIdx.x; i < kGOut; i += blockDim.x) L.b[4 * kGHid + i] = theta[kGB4 + i];
+}
+
+__device__ __forceinline__ bf16x8 ldsA8(const uint16_t* w, int ld, int row, int col) {
+    return *reinterpret_cast<const bf16x8*>(w + row * ld + col);
+}
+__device__ __forceinline__ f32x4 biasTile(const float* b, int mt, int grp) {
+    return *reinterpret_cast<const f32x4*>(b + mt * 16 + 4 * grp);
+}
+
+// guideForwardTile on K = 32 products: layer 0 in 4 K = 32 steps and one K = 16 step per output
+// tile, the other layers in 2 K = 32 steps (50 matrix instructions per 16 samples instead of 96).
+__device__ __forceinline__ void guideForwardTileX32(const GuideLdsX& L, const bf16x4 (&x0)[9], f32x4 (&out)[3]) {
+    const int lane = __lane_id();
+    const int s = lane & 15, grp = lane >> 4;
+    f32x4 acc[4];
+    bf16x4 h[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        acc[mt] = biasTile(L.b, mt, grp);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+            acc[mt] = mfma32(ldsA8(L.w0, kGLd0X, mt * 16 + s, 32 * kt + 8 * grp), cat(x0[2 * kt], x0[2 * kt + 1]), acc[mt]);
+        // K tail 128..143 as a K = 32 product whose upper 16 columns are zero in both operands:
+        // an accumulator chain keeps one MFMA shape. A 16x16x16 product that reads as C the result
+        // of a 16x16x32 product issued one or two instructions earlier has read it unfinished on
+        // gfx950 (results then depend on timing), and the compiler pads only same-shape chains.
+        const bf16x4 z = {};
+        acc[mt] = mfma32(cat(ldsA(L.w0, kGLd0X, mt * 16 + s, 128 + 4 * grp), z), cat(x0[8], z), acc[mt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) h[mt] = packRelu(acc[mt]);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        const uint16_t* w = L.wh + l * kGHid * kGLdHX;
+        const float* b = L.b + (l + 1) * kGHid;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            acc[mt] = biasTile(b, mt, grp);
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+                acc[mt] = mfma32(ldsA8(w, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp), cat(h[2 * kt], h[2 * kt + 1]), acc[mt]);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) h[mt] = packRelu(acc[mt]);
+    }
+    const float* b4 = L.b + 4 * kGHid;
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt) {
+        out[mt] = biasTile(b4, mt, grp);
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+            out[mt] = mfma32(ldsA8(L.w4, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp), cat(h[2 * kt], h[2 * kt + 1]), out[mt]);
+    }
+}
+
 // ---- von Mises-Fisher mixture ---------------------------------------------------------------
 // vMF density on the sphere: kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
 __device__ __forceinline__ float vmfNorm(float kappa) {
@@ -241,6 +374,12 @@ __device__ __forceinline__ float vmfNorm(float kappa) {
 }
 __device__ __forceinline__ float vmfPdf(float kappa, float cosang) {
     return vmfNorm(kappa) * __expf(kappa * (cosang - 1.0f));
+}
+// The normalisation of the density mixtureSample draws from: it inverts the CDF
+// (exp(kappa (t - 1)) - E) / (1 - E) with E = __expf(-2 kappa), so its density is
+// kappa exp(kappa (cos - 1)) / (2 pi (1 - E)) with that same E.
+__device__ __forceinline__ float vmfNormSampled(float kappa) {
+    return kappa * __builtin_amdgcn_rcpf(2.0f * 3.14159265358979f * (1.0f - __expf(-2.0f * kappa)));
 }
 
 // Mixture record of one sample: lobe[i] = (mu xyz, kappa) for i < 8 at float4 i, weights pi at
