@@ -188,6 +188,33 @@ const char* pg_last_error(void);
  * PG_BVH_DEVICE); pg_scene_build_bvh rebuilds it with other parameters or on the host. */
 int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out);
 int pg_scene_build_bvh(pg_scene* scene, const pg_bvh_params* params);
+
+/* ---- moving instances (Group children with keyframed transforms: EmbreeBranchNode and
+ *      updateTransformSRT, Embree.cpp:706-835; ray time = sample dimension 2, Embree/Kernel.cpp:77) ---- */
+#define PG_MAX_MOTION_KEYS 8
+/* A scene whose triangles [instance_first[k], instance_first[k + 1]) (the last range ends at
+ * desc->num_triangles) are moving instance k's mesh in the instance's own space; triangles before
+ * instance_first[0] are static and in world space. instance_first is strictly increasing, at most
+ * 256 instances. Each instance gets a BVH of its own; the ray meets it in its space at the ray's
+ * time. Until pg_scene_set_motion is called an instance is drawn with its own coordinates. */
+int pg_scene_create_instanced(pg_device* dev, const pg_scene_desc* desc, const uint32_t* instance_first,
+                              uint32_t num_instances, pg_scene** out);
+/* Transform of one instance for the next renders, as the reference's updateTransformSRT hands it
+ * to Embree (rtcSetGeometryTimeRange / TimeStepCount / TransformQuaternion, Embree.cpp:756-772):
+ * num_keys transforms at equal time steps from t0 to t1 (times normalised to the shutter, so a
+ * sample's time lies in [0, 1]); the instance is visible for t0 <= time <= t1 (num_keys = 1: a
+ * fixed transform, always visible). Between two keys scale, skew, shift and translation
+ * interpolate linearly and the rotation spherically. keys: TransformSRT order (Kernel/Transform.hpp
+ * :390-393): scale xyz, skew xy xz yz, shift xyz, quaternion w x y z, translation xyz; the matrix
+ * is T R S. parent: row-major 3x4 static transform of the instance's parent nodes, applied after. */
+typedef struct pg_motion {
+    uint32_t num_keys;
+    uint32_t visible; /* 0: absent from the frame (rtcDisableGeometry, Embree.cpp:718-724) */
+    float t0, t1;
+    float keys[PG_MAX_MOTION_KEYS][16];
+    float parent[12];
+} pg_motion;
+int pg_scene_set_motion(pg_scene* scene, uint32_t instance, const pg_motion* motion);
 void pg_scene_destroy(pg_scene* scene);
 /* BVH shape after a build: {nodes, leaves, max depth, width, node bytes, triangle bytes}. */
 int pg_scene_bvh_info(pg_scene* scene, uint64_t info[6]);
@@ -236,6 +263,10 @@ int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample
  *      sobolStart / sobolGenerate, ScrambledSobolSampler/Kernel.cpp:52-67) ---- */
 /* rays: float[8*n] = {o xyz, d xyz, tmin, tmax}; outputs t[n], prim[n], uv[2n] (host arrays). */
 int pg_trace(pg_scene* scene, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv);
+/* pg_trace at ray times times[n] (RTCRay::time, Embree.cpp:290), for scenes with moving instances;
+ * pg_trace traces at time 0. */
+int pg_trace_at(pg_scene* scene, const float* rays, const float* times, uint32_t n, float* t, uint32_t* prim,
+                float* uv);
 int pg_occluded(pg_scene* scene, const float* rays, uint32_t n, uint8_t* occluded);
 /* pg_trace without any allocation: no counterpart in the reference. For callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -233,6 +233,17 @@ struct Scene {
     uint32_t wideWidth = 0;
     std::vector<uint32_t> wideNodes;  // raw 32-bit words
     std::vector<float> wideTris;
+    // Moving instances: triangles [numStatic, tris.size()) are instance meshes in their own space.
+    struct Instance {
+        uint32_t first = 0, count = 0;
+        uint32_t keys = 1;
+        bool visible = true;
+        double t0 = -INFINITY, t1 = INFINITY;
+        double srt[8][16] = {};  // TransformSRT keys, unit quaternions
+        double parent[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    };
+    uint32_t numStatic = 0;
+    std::vector<Instance> inst;
 };
 
 // Binned-SAH BVH2 over triangle boxes (the oracle's own acceleration structure; only the hit
@@ -254,7 +265,7 @@ struct Box {
 };
 
 void buildBVH(Scene& s) {
-    const uint32_t n = static_cast<uint32_t>(s.tris.size());
+    const uint32_t n = s.numStatic;  // moving instances are tested apart (instanceHits)
     s.nodes.clear();
     s.order.resize(n);
     if (n == 0) return;
@@ -459,16 +470,142 @@ bool boxHit(const BNode& n, const Ray& r, const RayPre& p, float tmin, float tma
 
 bool traverseWide(const Scene& s, const Ray& r, float tmin, float tmax, bool any, Hit& h, uint64_t* counts);
 
+// ---------------------------------------------------------------------------------------------
+// Moving instances: a Group child with a keyframed TransformSRT (EmbreeBranchNode, Embree.cpp:
+// 777-835) is an Embree instance whose transform updateTransformSRT (:709-773) sets as time steps
+// with rtcSetGeometryTimeRange / TimeStepCount / TransformQuaternion. Embree 3.12 (the version
+// Tracer/Embree/CMakeLists.txt:3 asks for) then draws it at the ray's time (RTCRay::time,
+// Embree.cpp:290, the sample's dimension 2, Embree/Kernel.cpp:77) as its documentation states for
+// quaternion motion: over n = keys - 1 equal segments of [t0, t1], segment i = clamp(floor(x), 0,
+// n - 1) of x = n (time - t0) / (t1 - t0) at fraction f = x - i; scale, skew, shift and
+// translation linear in f; the rotation spherical-linear on the shorter arc (Shoemake 1985,
+// q(f) = (sin((1 - f) W) q0 + sin(f W) q1) / sin W with cos W = q0 . q1 >= 0; normalised
+// linear interpolation when cos W > 0.9995); matrix T R S with
+// S = [[sx, kxy, kxz, hx], [0, sy, kyz, hy], [0, 0, sz, hz]]; the geometry is present only for
+// t0 <= time <= t1 (a time range inside the shutter lets it appear and disappear).
+// Computed here in double.
+// ---------------------------------------------------------------------------------------------
+void instanceMatrix(const Scene::Instance& I, double time, double m[12]) {
+    uint32_t i = 0;
+    double f = 0.0;
+    if (I.keys > 1) {
+        const double n = I.keys - 1.0;
+        const double x = n * (time - I.t0) / (I.t1 - I.t0);
+        i = static_cast<uint32_t>(std::min(std::max(std::floor(x), 0.0), n - 1.0));
+        f = x - i;
+    }
+    const double* a = I.srt[i];
+    const double* b = I.srt[I.keys > 1 ? i + 1 : i];
+    double k[16];
+    for (int j = 0; j < 16; ++j) k[j] = (1.0 - f) * a[j] + f * b[j];
+    double q1[4] = {b[9], b[10], b[11], b[12]};
+    double cw = a[9] * q1[0] + a[10] * q1[1] + a[11] * q1[2] + a[12] * q1[3];
+    if (cw < 0.0) {
+        cw = -cw;
+        for (double& c : q1) c = -c;
+    }
+    double q[4];
+    if (cw > 0.9995) {
+        double nn = 0.0;
+        for (int j = 0; j < 4; ++j) q[j] = (1.0 - f) * a[9 + j] + f * q1[j], nn += q[j] * q[j];
+        for (double& c : q) c /= std::sqrt(nn);
+    } else {
+        const double W = std::acos(cw), sw = std::sin(W);
+        const double ca = std::sin((1.0 - f) * W) / sw, cb = std::sin(f * W) / sw;
+        for (int j = 0; j < 4; ++j) q[j] = ca * a[9 + j] + cb * q1[j];
+    }
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    // rotation matrix of the unit quaternion, rows
+    const double R[3][3] = {{w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)},
+                            {2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)},
+                            {2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z}};
+    const double S[3][4] = {{k[0], k[3], k[4], k[6]}, {0, k[1], k[5], k[7]}, {0, 0, k[2], k[8]}};
+    double L[3][4];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) {
+            L[r][c] = R[r][0] * S[0][c] + R[r][1] * S[1][c] + R[r][2] * S[2][c];
+            if (c == 3) L[r][c] += k[13 + r];
+        }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) {
+            m[4 * r + c] = I.parent[4 * r] * L[0][c] + I.parent[4 * r + 1] * L[1][c] + I.parent[4 * r + 2] * L[2][c];
+            if (c == 3) m[4 * r + c] += I.parent[4 * r + 3];
+        }
+}
+
+bool instancePresent(const Scene::Instance& I, double time) { return I.visible && time >= I.t0 && time <= I.t1; }
+
+// The ray in the instance's space at `time`: inverse of the affine map (cofactors / determinant).
+Ray instanceRay(const Scene::Instance& I, const Ray& r, double time) {
+    double m[12];
+    instanceMatrix(I, time, m);
+    const double det = m[0] * (m[5] * m[10] - m[6] * m[9]) - m[1] * (m[4] * m[10] - m[6] * m[8]) +
+                       m[2] * (m[4] * m[9] - m[5] * m[8]);
+    const double inv[3][3] = {{(m[5] * m[10] - m[6] * m[9]) / det, (m[2] * m[9] - m[1] * m[10]) / det,
+                               (m[1] * m[6] - m[2] * m[5]) / det},
+                              {(m[6] * m[8] - m[4] * m[10]) / det, (m[0] * m[10] - m[2] * m[8]) / det,
+                               (m[2] * m[4] - m[0] * m[6]) / det},
+                              {(m[4] * m[9] - m[5] * m[8]) / det, (m[1] * m[8] - m[0] * m[9]) / det,
+                               (m[0] * m[5] - m[1] * m[4]) / det}};
+    const double o[3] = {r.o.x - m[3], r.o.y - m[7], r.o.z - m[11]}, d[3] = {r.d.x, r.d.y, r.d.z};
+    float lo[3], ld[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = static_cast<float>(inv[k][0] * o[0] + inv[k][1] * o[1] + inv[k][2] * o[2]);
+        ld[k] = static_cast<float>(inv[k][0] * d[0] + inv[k][1] * d[1] + inv[k][2] * d[2]);
+    }
+    return Ray{{lo[0], lo[1], lo[2]}, {ld[0], ld[1], ld[2]}};
+}
+
+// Triangle `prim` in world space at `time` (static triangles are stored there).
+Tri worldTri(const Scene& s, uint32_t prim, double time) {
+    const Tri& t = s.tris[prim];
+    if (prim < s.numStatic) return t;
+    for (const Scene::Instance& I : s.inst)
+        if (prim >= I.first && prim < I.first + I.count) {
+            double m[12];
+            instanceMatrix(I, time, m);
+            auto ap = [&](V3 v) {
+                return V3{static_cast<float>(m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3]),
+                          static_cast<float>(m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7]),
+                          static_cast<float>(m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11])};
+            };
+            return Tri{ap(t.v0), ap(t.v1), ap(t.v2)};
+        }
+    return t;
+}
+
+// Every moving instance present at `time`: its triangles against the ray in its own space (the
+// ray parameter t is the world one: the direction is mapped, not normalised). any: stop at the
+// first t in (tmin, tmax).
+bool instanceHits(const Scene& s, const Ray& r, double time, float tmin, float tmax, bool any, Hit& best) {
+    for (const Scene::Instance& I : s.inst) {
+        if (!instancePresent(I, time)) continue;
+        const Ray lr = instanceRay(I, r, time);
+        const RayPre lp = precompute(lr);
+        float t, u, v;
+        for (uint32_t i = I.first; i < I.first + I.count; ++i) {
+            if (!intersectTri(lr, lp, s.tris[i], t, u, v)) continue;
+            if (any) {
+                if (t > tmin && t < tmax) return true;
+            } else {
+                consider(best, t, i, u, v, tmin);
+            }
+        }
+    }
+    return false;
+}
+
 // rtcIntersect1 with mask gsm|areaLight (Embree.cpp:267-355): closest triangle or light quad.
-Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brute, uint64_t* counts) {
+Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brute, uint64_t* counts,
+                 double time = 0.0) {
     Hit best{tmax, kMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     float t, u, v;
     if (brute) {
-        for (uint32_t i = 0; i < s.tris.size(); ++i)
+        for (uint32_t i = 0; i < s.numStatic; ++i)
             if (intersectTri(r, p, s.tris[i], t, u, v)) consider(best, t, i, u, v, tmin);
     } else if (s.wideWidth) {
-        if (!s.wideTris.empty()) traverseWide(s, r, tmin, tmax, false, best, counts);
+        if (s.numStatic && !s.wideTris.empty()) traverseWide(s, r, tmin, tmax, false, best, counts);
     } else if (!s.nodes.empty()) {
         uint32_t stack[128];
         int sp = 0;
@@ -489,6 +626,7 @@ Hit traceClosest(const Scene& s, const Ray& r, float tmin, float tmax, bool brut
             }
         }
     }
+    instanceHits(s, r, time, tmin, tmax, false, best);
     // light quads after triangles; a quad must be strictly closer to win
     for (uint32_t q = 0; q < s.quads.size(); ++q)
         if (intersectQuad(r, s.quads[q], tmin, best.t, t)) best = {t, kQuadBit | q, 0.0f, 0.0f};
@@ -577,10 +715,12 @@ bool traverseWide(const Scene& s, const Ray& r, float tmin, float tmax, bool any
 }
 
 // rtcOccluded1 with mask gsm only (Embree.cpp:357-391): lights never occlude.
-bool occluded(const Scene& s, const Ray& r, float tmin, float tmax) {
+bool occluded(const Scene& s, const Ray& r, float tmin, float tmax, double time = 0.0) {
+    Hit ih{tmax, kMiss, 0.0f, 0.0f};
+    if (instanceHits(s, r, time, tmin, tmax, true, ih)) return true;
     if (s.wideWidth) {
         Hit h{tmax, kMiss, 0.0f, 0.0f};
-        return !s.wideTris.empty() && traverseWide(s, r, tmin, tmax, true, h, nullptr);
+        return s.numStatic && !s.wideTris.empty() && traverseWide(s, r, tmin, tmax, true, h, nullptr);
     }
     if (s.nodes.empty()) return false;
     const RayPre p = precompute(r);
@@ -1040,6 +1180,7 @@ struct Params {
 struct Sampler {  // PerSampleContext dimension counter, Embree/Kernel.cpp:37-45
     uint64_t index;
     uint32_t dim;
+    float time = 0.0f;  // PerSampleContext::time: the sample's dimension 2 (Embree/Kernel.cpp:77)
     float next() { return sobolSample(index, dim++); }
     void next2(bool d9, float& u1, float& u2) {
         const float a = next(), b = next();
@@ -1059,12 +1200,12 @@ struct Frame {  // SurfaceIntersectionInfo, Protocol.hpp:47-66
 // Hit post-process: calcTriangleMeshSurface (Embree/Kernel.cpp:170-192) and planeSurface
 // (BasicShapes/Kernel.cpp:99-111). Returns the world normal facing the incoming ray.
 V3 triNg(const Tri& tri) { return normalize(cross(tri.v1 - tri.v0, tri.v2 - tri.v0)); }
-V3 hitNormal(const Scene& s, const Hit& h, V3 dir) {
+V3 hitNormal(const Scene& s, const Hit& h, V3 dir, double time = 0.0) {
     if (h.prim & kQuadBit) {
         const Quad& q = s.quads[h.prim & ~kQuadBit];
         return dot(dir, q.normal) > 0.0f ? -q.normal : q.normal;  // face Front -> -normal
     }
-    const V3 ng = triNg(s.tris[h.prim]);
+    const V3 ng = triNg(worldTri(s, h.prim, time));
     return dot(dir, ng) < 0.0f ? ng : -ng;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
 }
 Frame shadingFrame(V3 n, bool d3) {
@@ -1153,7 +1294,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
     bool specular = true;
     uint32_t depth = 0;
     while (true) {
-        const Hit h = traceClosest(s, ray, 1e-3f, 1e5f, false, st.trav);
+        const Hit h = traceClosest(s, ray, 1e-3f, 1e5f, false, st.trav, smp.time);
         ++st.ext;
         const bool isLight = h.prim != kMiss && (h.prim & kQuadBit);
         if (specular && (isLight || h.prim == kMiss)) {  // :119-140
@@ -1170,8 +1311,8 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
         const uint32_t matId = s.objMat[s.triObj[h.prim]];
         Material mat = s.mats[matId];
         applyTextures(s, matId, h, mat);
-        const V3 n = hitNormal(s, h, ray.d);
-        mat.front = dot(ray.d, triNg(s.tris[h.prim])) < 0.0f;  // Embree.cpp:336
+        const V3 n = hitNormal(s, h, ray.d, smp.time);
+        mat.front = dot(ray.d, triNg(worldTri(s, h.prim, smp.time))) < 0.0f;  // Embree.cpp:336
         const Frame fr = shadingFrame(n, d3);
         const V3 wo = fr.toLocal(-ray.d);
         const V3 ng = fr.toLocal(n);
@@ -1189,7 +1330,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
                 const V3 f = bsdfEval(mat, wo, wi, ng);
                 if (valid(f)) {
                     ++st.shadow;
-                    if (!occluded(s, Ray{hit, ls.dir}, std::fmin(1e-3f, ls.dist), ls.dist)) {
+                    if (!occluded(s, Ray{hit, ls.dir}, std::fmin(1e-3f, ls.dist), ls.dist, smp.time)) {
                         float w = std::fabs(wi.z) / ls.pdf;
                         if (!sel.delta) w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));  // :51-55
                         partA = mul(f, ls.rad) * w;
@@ -1202,7 +1343,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
                 if (bs.pdf > 0.0f && valid(bs.f)) {
                     float weight = std::fabs(bs.wi.z) / bs.pdf;
                     const V3 dir = fr.toWorld(bs.wi);
-                    const Hit lh = traceClosest(s, Ray{hit, dir}, 1e-3f, 1e5f, false, nullptr);
+                    const Hit lh = traceClosest(s, Ray{hit, dir}, 1e-3f, 1e5f, false, nullptr, smp.time);
                     ++st.mis;
                     V3 rad{0, 0, 0};
                     float lpdf = 0.0f;
@@ -1266,6 +1407,7 @@ Ray cameraRay(const Camera& c, float ndcX, float ndcY, float u1, float u2) {
 V3 renderSample(const Scene& s, const Params& p, uint32_t m, uint32_t px, uint32_t py, uint32_t si, float& fx,
                 float& fy, PathStats& st) {
     Sampler smp{intervalToIndex(m, si, px, py), 5};
+    smp.time = sobolSample(smp.index, 2);
     const float res = static_cast<float>(1u << m);
     // sobolStart, Sampler/ScrambledSobolSampler/Kernel.cpp:52-62
     fx = std::fmin(std::fmax(sobolSample(smp.index, 0) * res - static_cast<float>(px), 0.0f), 1.0f);
@@ -1376,10 +1518,46 @@ void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint
     s->cam = {{cam[0], cam[1], cam[2]}, {cam[3], cam[4], cam[5]}, {cam[6], cam[7], cam[8]},
               cam[9], cam[10], cam[11], cam[12]};
     s->env = {env[0], env[1], env[2]};
+    s->numStatic = nt;
     buildBVH(*s);
     return s;
 }
 void or_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
+
+// Moving instances: triangles [first[k], first[k + 1]) (the last up to the scene's end) are
+// instance k's mesh in its own space; triangles before first[0] are static. Each instance starts
+// fixed at the identity until or_scene_set_motion.
+void or_scene_set_instances(void* sp, const uint32_t* first, uint32_t n) {
+    Scene& s = *static_cast<Scene*>(sp);
+    const uint32_t nt = static_cast<uint32_t>(s.tris.size());
+    s.inst.assign(n, Scene::Instance{});
+    for (uint32_t k = 0; k < n; ++k) {
+        Scene::Instance& I = s.inst[k];
+        I.first = first[k];
+        I.count = (k + 1 < n ? first[k + 1] : nt) - first[k];
+        const double id[16] = {1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0};
+        std::memcpy(I.srt[0], id, sizeof(id));
+    }
+    s.numStatic = n ? first[0] : nt;
+    buildBVH(s);
+}
+// keys[16 * numKeys]: TransformSRT (scale xyz, skew xy xz yz, shift xyz, quaternion w x y z,
+// translation xyz); parent[12]: row-major 3x4. numKeys 1: fixed and always present.
+void or_scene_set_motion(void* sp, uint32_t k, uint32_t numKeys, int visible, float t0, float t1, const float* keys,
+                         const float* parent) {
+    Scene::Instance& I = static_cast<Scene*>(sp)->inst[k];
+    I.keys = numKeys;
+    I.visible = visible != 0;
+    I.t0 = numKeys > 1 ? t0 : -INFINITY;
+    I.t1 = numKeys > 1 ? t1 : INFINITY;
+    for (uint32_t j = 0; j < numKeys; ++j) {
+        for (int c = 0; c < 16; ++c) I.srt[j][c] = keys[16 * j + c];
+        double n = 0.0;
+        for (int c = 9; c < 13; ++c) n += I.srt[j][c] * I.srt[j][c];
+        for (int c = 9; c < 13; ++c) I.srt[j][c] /= std::sqrt(n);
+    }
+    for (int c = 0; c < 12; ++c) I.parent[c] = parent[c];
+}
 
 // Traverse a wide BVH laid out as the GPU library builds it (pg_scene_bvh_download: numNodes nodes
 // of 28 * width bytes, numTris leaf-order triangles of 48 bytes) instead of the oracle's own BVH2.
@@ -1402,6 +1580,15 @@ void or_trace(void* sp, const float* rays, uint32_t n, float* t, uint32_t* prim,
     for (uint32_t i = 0; i < n; ++i) {
         const float* r = rays + 8 * i;
         const Hit h = traceClosest(s, Ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}}, r[6], r[7], brute != 0, counts);
+        t[i] = h.t, prim[i] = h.prim, uv[2 * i] = h.u, uv[2 * i + 1] = h.v;
+    }
+}
+// or_trace at per-ray times (moving instances).
+void or_trace_at(void* sp, const float* rays, const float* times, uint32_t n, float* t, uint32_t* prim, float* uv) {
+    const Scene& s = *static_cast<Scene*>(sp);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        const Hit h = traceClosest(s, Ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}}, r[6], r[7], false, nullptr, times[i]);
         t[i] = h.t, prim[i] = h.prim, uv[2 * i] = h.u, uv[2 * i + 1] = h.v;
     }
 }

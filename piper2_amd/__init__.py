@@ -120,6 +120,34 @@ class TuneResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+MAX_MOTION_KEYS = 8
+
+
+class Motion(ctypes.Structure):
+    """pg_motion: transform keys of one moving instance for the next renders."""
+    _fields_ = [("num_keys", ctypes.c_uint32), ("visible", ctypes.c_uint32), ("t0", ctypes.c_float),
+                ("t1", ctypes.c_float), ("keys", (ctypes.c_float * 16) * MAX_MOTION_KEYS),
+                ("parent", ctypes.c_float * 12)]
+
+
+IDENTITY_SRT = (1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+IDENTITY_3X4 = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+def motion_struct(keys, t0=0.0, t1=1.0, visible=True, parent=None):
+    """pg_motion from a list of 16-float TransformSRT keys (scale xyz, skew xy xz yz, shift xyz,
+    quaternion w x y z, translation xyz)."""
+    keys = [list(map(float, k)) for k in keys]
+    if not 1 <= len(keys) <= MAX_MOTION_KEYS or any(len(k) != 16 for k in keys):
+        raise ValueError("a motion has 1 to 8 keys of 16 floats")
+    m = Motion()
+    m.num_keys, m.visible, m.t0, m.t1 = len(keys), 1 if visible else 0, t0, t1
+    for i, k in enumerate(keys):
+        m.keys[i][:] = k
+    m.parent[:] = list(map(float, parent if parent is not None else IDENTITY_3X4))
+    return m
+
+
 # Every symbol the C ABI declares, with (restype, argtypes).
 _VP = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -130,6 +158,8 @@ SIGNATURES = {
     "pg_last_error": (ctypes.c_char_p, []),
     "pg_scene_create": (ctypes.c_int, [_VP, ctypes.POINTER(SceneDesc), ctypes.POINTER(_VP)]),
     "pg_scene_build_bvh": (ctypes.c_int, [_VP, ctypes.POINTER(BvhParams)]),
+    "pg_scene_create_instanced": (ctypes.c_int, [_VP, ctypes.POINTER(SceneDesc), _VP, _U32, ctypes.POINTER(_VP)]),
+    "pg_scene_set_motion": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(Motion)]),
     "pg_scene_destroy": (None, [_VP]),
     "pg_scene_bvh_info": (ctypes.c_int, [_VP, _VP]),
     "pg_scene_bvh_download": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]),
@@ -140,6 +170,7 @@ SIGNATURES = {
     "pg_render": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, ctypes.POINTER(Stats)]),
     "pg_render_paths": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _VP, ctypes.POINTER(Stats)]),
     "pg_trace": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
+    "pg_trace_at": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
     "pg_occluded": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
     "pg_trace_noalloc": (ctypes.c_int, [_VP, _VP, _U32, _VP, _VP, _VP, _VP]),
     "pg_camera_rays": (ctypes.c_int, [_VP, ctypes.POINTER(RenderParams), _VP, _U32, _VP, _VP]),
@@ -249,9 +280,22 @@ class Scene:
         self._keep = sc
         desc = scene_desc(sc)
         self.h = ctypes.c_void_p()
-        check(lib().pg_scene_create(device.h, ctypes.byref(desc), ctypes.byref(self.h)))
+        first = sc.get("instance_first")
+        if first is not None and len(first):  # moving instances (scenes.py: instance_first, motions)
+            self._first = np.ascontiguousarray(first, dtype=np.uint32)
+            check(lib().pg_scene_create_instanced(device.h, ctypes.byref(desc), _ptr(self._first), len(self._first),
+                                                  ctypes.byref(self.h)))
+        else:
+            check(lib().pg_scene_create(device.h, ctypes.byref(desc), ctypes.byref(self.h)))
         if (bvh_width, leaf_size, bins) != (8, 4, 32) or builder is not None:
             self.build_bvh(bvh_width, leaf_size, bins, builder)
+        for k, m in enumerate(sc.get("motions") or []):
+            self.set_motion(k, **m)
+
+    def set_motion(self, instance, keys, t0=0.0, t1=1.0, visible=True, parent=None):
+        """Keys of moving instance `instance` (pg_scene_set_motion)."""
+        m = motion_struct(keys, t0, t1, visible, parent)
+        check(lib().pg_scene_set_motion(self.h, instance, ctypes.byref(m)))
 
     def build_bvh(self, width=8, leaf_size=4, bins=32, builder=None):
         """Rebuild the BVH (builder: PG_BVH_DEVICE by default, or PG_BVH_HOST)."""
@@ -289,6 +333,19 @@ class Scene:
         prim = np.zeros(n, np.uint32)
         uv = np.zeros((n, 2), np.float32)
         check(lib().pg_trace(self.h, _ptr(rays), n, _ptr(t), _ptr(prim), _ptr(uv)))
+        return t, prim, uv
+
+    def trace_at(self, rays, times):
+        """Closest hits at per-ray times (pg_trace_at)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        times = np.ascontiguousarray(times, dtype=np.float32).reshape(-1)
+        n = rays.shape[0]
+        if times.shape[0] != n:
+            raise ValueError("one time per ray")
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.uint32)
+        uv = np.zeros((n, 2), np.float32)
+        check(lib().pg_trace_at(self.h, _ptr(rays), _ptr(times), n, _ptr(t), _ptr(prim), _ptr(uv)))
         return t, prim, uv
 
     def occluded(self, rays):

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -176,8 +177,14 @@ struct pg_device {
 
 struct pg_scene {
     pg_device* dev = nullptr;
-    std::vector<float> triVerts;   // 9 floats per triangle, world space
+    std::vector<float> triVerts;   // 9 floats per triangle: world space, or a moving instance's local space
     std::vector<uint32_t> triMat;  // material per original triangle
+    uint32_t numStatic = 0;        // triangles [0, numStatic) are static, in world space
+    std::vector<uint32_t> instFirst;    // moving instance k: triangles [instFirst[k], instFirst[k + 1])
+    std::vector<pg::DevInstance> inst;  // host copy of the instance table (root, keys, swept box)
+    std::vector<float> instBox;         // 6 floats per instance: its mesh's local bounds
+    DevBuf instBuf;
+    float staticLo[3] = {0, 0, 0}, staticHi[3] = {0, 0, 0};  // static triangles and light quads
     pg::WideBvh bvh;
     pg::BvhParams bvhParams;  // of the current build
     uint32_t builder = PG_BVH_HOST;
@@ -288,6 +295,220 @@ void guideStep(pg_guide* g, const float* rec, uint32_t n, bool update) {
 
 void guideSetLds() { pg::guideConfigureLds(); }
 
+// ---- moving instances on the host ----------------------------------------------------------
+// The device's instanceAt in double: keys seg and seg + 1 at fraction f (linear scale, skew,
+// shift and translation; slerp of the unit quaternions, normalised lerp above cosine 0.9995),
+// M = parent * T R S.
+void motionMatrix(const pg::DevInstance& I, uint32_t seg, double f, double m[12]) {
+    const float* a = I.srt[seg];
+    const float* b = I.srt[I.keys > 1 ? seg + 1 : seg];
+    double k[16];
+    for (int i = 0; i < 16; ++i) k[i] = a[i] * (1.0 - f) + b[i] * f;
+    double q0[4], q1[4], q[4], c = 0.0;
+    for (int i = 0; i < 4; ++i) q0[i] = a[9 + i], q1[i] = b[9 + i], c += q0[i] * q1[i];
+    if (c < 0.0) {
+        c = -c;
+        for (double& v : q1) v = -v;
+    }
+    if (c > 0.9995) {
+        double n = 0.0;
+        for (int i = 0; i < 4; ++i) q[i] = q0[i] * (1.0 - f) + q1[i] * f, n += q[i] * q[i];
+        for (double& v : q) v /= std::sqrt(n);
+    } else {
+        double pp[4], n = 0.0;
+        for (int i = 0; i < 4; ++i) pp[i] = q1[i] - c * q0[i], n += pp[i] * pp[i];
+        const double phi = f * std::acos(c), cs = std::cos(phi), sn = std::sin(phi) / std::sqrt(n);
+        for (int i = 0; i < 4; ++i) q[i] = cs * q0[i] + sn * pp[i];
+    }
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double r[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z),     2 * (x * z + w * y),
+                         2 * (x * y + w * z),     1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                         2 * (x * z - w * y),     2 * (y * z + w * x),     1 - 2 * (x * x + y * y)};
+    double l[12];
+    for (int i = 0; i < 3; ++i) {
+        l[4 * i] = r[3 * i] * k[0];
+        l[4 * i + 1] = r[3 * i] * k[3] + r[3 * i + 1] * k[1];
+        l[4 * i + 2] = r[3 * i] * k[4] + r[3 * i + 1] * k[5] + r[3 * i + 2] * k[2];
+        l[4 * i + 3] = r[3 * i] * k[6] + r[3 * i + 1] * k[7] + r[3 * i + 2] * k[8] + k[13 + i];
+    }
+    const float* P = I.parent;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j)
+            m[4 * i + j] = P[4 * i] * l[j] + P[4 * i + 1] * l[4 + j] + P[4 * i + 2] * l[8 + j] + (j == 3 ? P[4 * i + 3] : 0.0);
+}
+
+// World bounds of instance I over the times in [0, 1] (the range of a sample's time) at which it is
+// visible: per key segment, the local box's corners at 65 equally spaced fractions, grown by half
+// a step times a bound on the corners' speed, |d(world)/df| <= |P|_F (|dT| + 2 theta max|S p| +
+// |dS p|) with theta the quaternions' angle (the rotation turns at 2 theta per unit fraction).
+// Returns false if no such time exists.
+bool sweptBox(const pg::DevInstance& I, const float* box, float lo[3], float hi[3]) {
+    constexpr int kSteps = 64;
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double pn = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) pn += static_cast<double>(I.parent[4 * i + j]) * I.parent[4 * i + j];
+    pn = std::sqrt(pn);
+    bool any = false;
+    const uint32_t segs = I.keys > 1 ? I.keys - 1 : 1;
+    for (uint32_t seg = 0; seg < segs; ++seg) {
+        double fa = 0.0, fb = 0.0;
+        if (I.keys > 1) {
+            const double span = (static_cast<double>(I.t1) - I.t0) / (I.keys - 1);
+            const double ta = I.t0 + seg * span, tb = ta + span;
+            const double wa = std::max({ta, 0.0, static_cast<double>(I.t0)});
+            const double wb = std::min({tb, 1.0, static_cast<double>(I.t1)});
+            if (wa > wb) continue;
+            fa = (wa - ta) / span, fb = (wb - ta) / span;
+        }
+        any = true;
+        const float* a = I.srt[seg];
+        const float* b = I.srt[I.keys > 1 ? seg + 1 : seg];
+        double dq = 0.0;
+        for (int i = 0; i < 4; ++i) dq += static_cast<double>(a[9 + i]) * b[9 + i];
+        const double theta = std::acos(std::min(1.0, std::fabs(dq))) * 1.05;
+        const double dt = std::sqrt((b[13] - a[13]) * (b[13] - a[13]) + (b[14] - a[14]) * (b[14] - a[14]) +
+                                    (b[15] - a[15]) * (b[15] - a[15]));
+        double speed = 0.0, reach = 0.0;
+        for (int c = 0; c < 8; ++c) {
+            const double p[3] = {box[(c & 1) ? 3 : 0], box[(c & 2) ? 4 : 1], box[(c & 4) ? 5 : 2]};
+            auto sp = [&](const float* k, double out[3]) {
+                out[0] = k[0] * p[0] + k[3] * p[1] + k[4] * p[2] + k[6];
+                out[1] = k[1] * p[1] + k[5] * p[2] + k[7];
+                out[2] = k[2] * p[2] + k[8];
+            };
+            double sa[3], sb[3];
+            sp(a, sa), sp(b, sb);
+            const double na = std::sqrt(sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2]);
+            const double nb = std::sqrt(sb[0] * sb[0] + sb[1] * sb[1] + sb[2] * sb[2]);
+            const double nd = std::sqrt((sb[0] - sa[0]) * (sb[0] - sa[0]) + (sb[1] - sa[1]) * (sb[1] - sa[1]) +
+                                        (sb[2] - sa[2]) * (sb[2] - sa[2]));
+            speed = std::max(speed, pn * (dt + 2.0 * theta * std::max(na, nb) + nd));
+        }
+        const double step = (fb - fa) / kSteps;
+        for (int i = 0; i <= kSteps; ++i) {
+            double m[12];
+            motionMatrix(I, seg, fa + step * i, m);
+            for (int c = 0; c < 8; ++c) {
+                const double p[3] = {box[(c & 1) ? 3 : 0], box[(c & 2) ? 4 : 1], box[(c & 4) ? 5 : 2]};
+                for (int r = 0; r < 3; ++r) {
+                    const double v = m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
+                    blo[r] = std::min(blo[r], v), bhi[r] = std::max(bhi[r], v);
+                    reach = std::max(reach, std::fabs(v));
+                }
+            }
+        }
+        const double pad = speed * step * 0.5 + 1e-5 * reach;
+        for (int r = 0; r < 3; ++r) blo[r] -= pad, bhi[r] += pad;
+    }
+    for (int r = 0; r < 3; ++r) lo[r] = static_cast<float>(blo[r]), hi[r] = static_cast<float>(bhi[r]);
+    return any;
+}
+
+// Scene box for the guide's position inputs: static triangles and quads, and every visible
+// instance's swept bounds.
+void updateSceneBox(pg_scene* s) {
+    float blo[3], bhi[3];
+    for (int k = 0; k < 3; ++k) blo[k] = s->staticLo[k], bhi[k] = s->staticHi[k];
+    for (const pg::DevInstance& I : s->inst)
+        if (I.visible)
+            for (int k = 0; k < 3; ++k) blo[k] = std::min(blo[k], I.lo[k]), bhi[k] = std::max(bhi[k], I.hi[k]);
+    for (int k = 0; k < 3; ++k) {
+        if (!(bhi[k] > blo[k])) blo[k] = 0.0f, bhi[k] = 1.0f;
+        s->lo[k] = blo[k];
+        s->invExt[k] = 1.0f / (bhi[k] - blo[k]);
+    }
+}
+
+// An instance drawn with its local coordinates as world coordinates until its motion is set.
+pg::DevInstance fixedInstance(const float* box) {
+    pg::DevInstance I{};
+    I.keys = 1, I.visible = 1;
+    I.t0 = -FLT_MAX, I.t1 = FLT_MAX, I.rate = 0.0f;
+    const float id[16] = {1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0};
+    std::memcpy(I.srt[0], id, sizeof(id));
+    const float par[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    std::memcpy(I.parent, par, sizeof(par));
+    for (int k = 0; k < 3; ++k) I.lo[k] = box[k], I.hi[k] = box[3 + k];
+    return I;
+}
+
+// Moving instances: one BLAS per instance (host builder, the static tree's width and leaf size)
+// over its local-space triangles, appended to the static tree's node and triangle arrays with
+// node and triangle references rebased; the instance's leaf triangles keep their original
+// triangle number (tie rule, texture coordinates) and carry the instance index + 1 in v1.w.
+void appendInstances(pg_scene* s, const pg::BvhParams& bp, hipStream_t st) {
+    const uint32_t W = bp.width;
+    const size_t nodeFloats = 7 * static_cast<size_t>(W), nodeBytes = nodeFloats * 4;
+    const uint32_t staticNodes = s->numStatic ? s->bvh.numNodes : 0;
+    std::vector<float> nodes, tris;
+    std::vector<uint32_t> mats;
+    uint32_t nodeBase = staticNodes, triBase = s->numStatic;
+    uint32_t maxDepth = s->numStatic ? s->bvh.maxDepth : 0, leaves = s->numStatic ? s->bvh.numLeaves : 0;
+    for (size_t k = 0; k < s->inst.size(); ++k) {
+        const uint32_t first = s->instFirst[k], cnt = s->instFirst[k + 1] - first;
+        pg::WideBvh b;
+        pg::buildWideBvh(&s->triVerts[9 * static_cast<size_t>(first)], cnt, bp, b);
+        require(b.width == W, "instance BVH width differs from the scene's");
+        for (uint32_t n = 0; n < b.numNodes; ++n) {
+            uint32_t* child = reinterpret_cast<uint32_t*>(&b.nodes[n * nodeFloats + 6 * W]);
+            for (uint32_t c = 0; c < W; ++c) {
+                const uint32_t ref = child[c];
+                if (ref == pg::kInvalidChild) continue;
+                if (ref & pg::kLeafBit) {
+                    const uint32_t f = (ref & pg::kLeafFirstMask) + triBase;
+                    require(f <= pg::kLeafFirstMask, "too many triangles for the leaf references");
+                    child[c] = (ref & ~pg::kLeafFirstMask) | f;
+                } else {
+                    child[c] = ref + nodeBase;
+                }
+            }
+        }
+        const uint32_t tag = static_cast<uint32_t>(k) + 1u;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            float* o = &b.tris[12 * static_cast<size_t>(i)];
+            uint32_t prim;
+            std::memcpy(&prim, &o[3], 4);
+            prim += first;
+            std::memcpy(&o[3], &prim, 4);
+            std::memcpy(&o[7], &tag, 4);
+            mats.push_back(s->triMat[first + b.order[i]]);
+        }
+        nodes.insert(nodes.end(), b.nodes.begin(), b.nodes.end());
+        tris.insert(tris.end(), b.tris.begin(), b.tris.end());
+        s->inst[k].root = nodeBase;
+        nodeBase += b.numNodes;
+        triBase += cnt;
+        maxDepth = std::max(maxDepth, b.maxDepth);
+        leaves += b.numLeaves;
+    }
+    DevBuf nN, nT, nM;
+    nN.reserve(static_cast<size_t>(nodeBase) * nodeBytes);
+    nT.reserve(static_cast<size_t>(triBase) * 48);
+    nM.reserve(static_cast<size_t>(triBase) * 4);
+    if (s->numStatic) {
+        PG_HIP(hipMemcpyAsync(nN.p, s->nodes.p, static_cast<size_t>(staticNodes) * nodeBytes, hipMemcpyDeviceToDevice, st));
+        PG_HIP(hipMemcpyAsync(nT.p, s->tris.p, static_cast<size_t>(s->numStatic) * 48, hipMemcpyDeviceToDevice, st));
+        PG_HIP(hipMemcpyAsync(nM.p, s->triMatLeaf.p, static_cast<size_t>(s->numStatic) * 4, hipMemcpyDeviceToDevice, st));
+    }
+    PG_HIP(hipMemcpyAsync(static_cast<char*>(nN.p) + static_cast<size_t>(staticNodes) * nodeBytes, nodes.data(),
+                          nodes.size() * 4, hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(static_cast<char*>(nT.p) + static_cast<size_t>(s->numStatic) * 48, tris.data(), tris.size() * 4,
+                          hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(static_cast<char*>(nM.p) + static_cast<size_t>(s->numStatic) * 4, mats.data(), mats.size() * 4,
+                          hipMemcpyHostToDevice, st));
+    PG_HIP(hipStreamSynchronize(st));
+    std::swap(s->nodes.p, nN.p), std::swap(s->nodes.bytes, nN.bytes);
+    std::swap(s->tris.p, nT.p), std::swap(s->tris.bytes, nT.bytes);
+    std::swap(s->triMatLeaf.p, nM.p), std::swap(s->triMatLeaf.bytes, nM.bytes);
+    s->bvh.width = W;
+    s->bvh.numNodes = nodeBase;
+    s->bvh.numLeaves = leaves;
+    s->bvh.maxDepth = maxDepth;
+    upload(s->instBuf, s->inst.data(), s->inst.size() * sizeof(pg::DevInstance), st);
+    PG_HIP(hipStreamSynchronize(st));
+}
+
 void buildBvh(pg_scene* s, const pg_bvh_params* params) {
     pg::BvhParams bp;
     if (params) {
@@ -295,7 +516,7 @@ void buildBvh(pg_scene* s, const pg_bvh_params* params) {
         bp.leafSize = params->leaf_size;
         bp.bins = params->sah_bins;
     }
-    const uint32_t nt = static_cast<uint32_t>(s->triMat.size());
+    const uint32_t nt = s->numStatic;  // the static BVH; moving instances get BLASes of their own
     hipStream_t st = s->dev->stream;
     const bool onDevice = params && params->builder == PG_BVH_DEVICE && nt > 0;
     require(!params || params->builder == PG_BVH_HOST || params->builder == PG_BVH_DEVICE, "unknown BVH builder");
@@ -333,15 +554,19 @@ void buildBvh(pg_scene* s, const pg_bvh_params* params) {
         s->bvh.nodes.clear(), s->bvh.nodes.shrink_to_fit();  // the device copy is the one in use
         s->bvh.tris.clear(), s->bvh.tris.shrink_to_fit();
     }
+    if (nt == 0) s->bvh.numNodes = s->bvh.numLeaves = s->bvh.maxDepth = 0;
+    if (!s->inst.empty()) appendInstances(s, bp, st);
     s->bvhParams = bp;
     s->builder = onDevice ? PG_BVH_DEVICE : PG_BVH_HOST;
     s->ds.tris = s->tris.as<float4>();
     s->ds.triMat = s->triMatLeaf.as<uint32_t>();
     s->ds.nodes = s->nodes.p;
-    s->ds.numTris = nt;
+    s->ds.numTris = static_cast<uint32_t>(s->triMat.size());
     s->ds.numNodes = s->bvh.numNodes;
     s->ds.width = s->bvh.width;
     s->ds.hasBvh = nt > 0 ? 1u : 0u;
+    s->ds.inst = s->inst.empty() ? nullptr : s->instBuf.as<pg::DevInstance>();
+    s->ds.numInst = static_cast<uint32_t>(s->inst.size());
     const int need = static_cast<int>((s->bvh.width - 1) * s->bvh.maxDepth + 2) - pg::kLdsStack;
     s->spillPerLane = static_cast<uint32_t>(std::max(need, 1));
 }
@@ -1017,7 +1242,9 @@ int pg_debug_inject_fault(pg_device* dev) {
     });
 }
 
-int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
+namespace {
+int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* instanceFirst, uint32_t numInstances,
+                pg_scene** out) {
     return guarded([&] {
         require(dev && desc && out, "null argument");
         std::lock_guard<std::mutex> lock(dev->mu);
@@ -1043,6 +1270,24 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
             const uint32_t mat = desc->object_material[obj];
             require(mat < desc->num_materials, "material id out of range");
             s->triMat[i] = mat;
+        }
+        s->numStatic = nt;
+        if (numInstances) {
+            require(instanceFirst && numInstances <= pg::kMaxInstances, "bad moving instances");
+            s->instFirst.assign(instanceFirst, instanceFirst + numInstances);
+            s->instFirst.push_back(nt);
+            for (uint32_t k = 0; k < numInstances; ++k)
+                require(s->instFirst[k] < s->instFirst[k + 1], "every moving instance needs its own non-empty triangle range");
+            s->numStatic = s->instFirst[0];
+            s->instBox.resize(6 * static_cast<size_t>(numInstances));
+            for (uint32_t k = 0; k < numInstances; ++k) {
+                float* b = &s->instBox[6 * static_cast<size_t>(k)];
+                for (int a = 0; a < 3; ++a) b[a] = INFINITY, b[3 + a] = -INFINITY;
+                for (size_t v = 3 * static_cast<size_t>(s->instFirst[k]); v < 3 * static_cast<size_t>(s->instFirst[k + 1]); ++v)
+                    for (int a = 0; a < 3; ++a)
+                        b[a] = std::min(b[a], s->triVerts[3 * v + a]), b[3 + a] = std::max(b[3 + a], s->triVerts[3 * v + a]);
+                s->inst.push_back(fixedInstance(b));
+            }
         }
         std::vector<pg::DevMaterial> mats(desc->num_materials);
         for (uint32_t i = 0; i < desc->num_materials; ++i) {
@@ -1175,7 +1420,7 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
         auto grow = [&](const float* p) {
             for (int k = 0; k < 3; ++k) blo[k] = std::min(blo[k], p[k]), bhi[k] = std::max(bhi[k], p[k]);
         };
-        for (size_t i = 0; i < s->triVerts.size(); i += 3) grow(&s->triVerts[i]);
+        for (size_t i = 0; i < 9 * static_cast<size_t>(s->numStatic); i += 3) grow(&s->triVerts[i]);
         for (const auto& d : quads) {
             float c4[3];
             for (int a = 0; a < 2; ++a)
@@ -1184,16 +1429,59 @@ int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
                     grow(c4);
                 }
         }
-        for (int k = 0; k < 3; ++k) {
-            if (!(bhi[k] > blo[k])) blo[k] = 0.0f, bhi[k] = 1.0f;
-            s->lo[k] = blo[k];
-            s->invExt[k] = 1.0f / (bhi[k] - blo[k]);
-        }
+        for (int k = 0; k < 3; ++k) s->staticLo[k] = blo[k], s->staticHi[k] = bhi[k];
+        updateSceneBox(s.get());
         std::memcpy(ds.env, desc->env_radiance, 12);
         const pg_bvh_params defaults{8, 4, 32, PG_BVH_DEVICE};  // BVH8, leaves of <= 4, 32 bins, built on the GPU
         buildBvh(s.get(), &defaults);
         addChild(dev);
         *out = s.release();
+    });
+}
+}  // namespace
+
+int pg_scene_create(pg_device* dev, const pg_scene_desc* desc, pg_scene** out) {
+    return sceneCreate(dev, desc, nullptr, 0, out);
+}
+
+int pg_scene_create_instanced(pg_device* dev, const pg_scene_desc* desc, const uint32_t* instance_first,
+                              uint32_t num_instances, pg_scene** out) {
+    return sceneCreate(dev, desc, instance_first, num_instances, out);
+}
+
+int pg_scene_set_motion(pg_scene* s, uint32_t instance, const pg_motion* m) {
+    return guarded([&] {
+        require(s && m, "null argument");
+        require(instance < s->inst.size(), "instance index out of range");
+        require(m->num_keys >= 1 && m->num_keys <= PG_MAX_MOTION_KEYS, "a motion has 1 to 8 keys");
+        std::lock_guard<std::mutex> lock(s->dev->mu);
+        s->dev->checkFault();
+        pg::DevInstance I = s->inst[instance];
+        I.keys = m->num_keys;
+        I.visible = m->visible ? 1u : 0u;
+        if (I.keys > 1) {
+            require(std::isfinite(m->t0) && std::isfinite(m->t1) && m->t1 > m->t0, "key times need t0 < t1");
+            I.t0 = m->t0, I.t1 = m->t1;
+            I.rate = static_cast<float>(I.keys - 1) / (m->t1 - m->t0);
+        } else {
+            I.t0 = -FLT_MAX, I.t1 = FLT_MAX, I.rate = 0.0f;
+        }
+        for (uint32_t k = 0; k < I.keys; ++k) {
+            const float* src = m->keys[k];
+            for (int i = 0; i < 16; ++i) require(std::isfinite(src[i]), "motion keys must be finite");
+            const float n = std::sqrt(src[9] * src[9] + src[10] * src[10] + src[11] * src[11] + src[12] * src[12]);
+            require(n > 0.0f, "a motion key has a zero quaternion");
+            std::memcpy(I.srt[k], src, 64);
+            for (int i = 9; i < 13; ++i) I.srt[k][i] = src[i] / n;
+        }
+        for (int i = 0; i < 12; ++i) require(std::isfinite(m->parent[i]), "parent transform must be finite");
+        std::memcpy(I.parent, m->parent, sizeof(I.parent));
+        if (I.visible && !sweptBox(I, &s->instBox[6 * static_cast<size_t>(instance)], I.lo, I.hi)) I.visible = 0;
+        s->inst[instance] = I;
+        updateSceneBox(s);
+        hipStream_t st = s->dev->stream;
+        upload(s->instBuf, s->inst.data(), s->inst.size() * sizeof(pg::DevInstance), st);
+        PG_HIP(hipStreamSynchronize(st));
     });
 }
 
@@ -1331,6 +1619,10 @@ int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample
 }
 
 int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv) {
+    return pg_trace_at(s, rays, nullptr, n, t, prim, uv);
+}
+
+int pg_trace_at(pg_scene* s, const float* rays, const float* times, uint32_t n, float* t, uint32_t* prim, float* uv) {
     return guarded([&] {
         require(s && (n == 0 || (rays && t && prim && uv)), "null argument");
         if (n == 0) return;
@@ -1338,13 +1630,14 @@ int pg_trace(pg_scene* s, const float* rays, uint32_t n, float* t, uint32_t* pri
         std::lock_guard<std::mutex> lock(dev->mu);
         dev->checkFault();
         hipStream_t st = dev->stream;
-        DevBuf dr, dt, dp, duv, spill;
+        DevBuf dr, dtimes, dt, dp, duv, spill;
         upload(dr, rays, static_cast<size_t>(n) * 32, st);
+        if (times) upload(dtimes, times, static_cast<size_t>(n) * 4, st);
         dt.reserve(static_cast<size_t>(n) * 4), dp.reserve(static_cast<size_t>(n) * 4), duv.reserve(static_cast<size_t>(n) * 8);
         const uint32_t lanes = traversalLanes(dev, n);
         spill.reserve(static_cast<size_t>(lanes) * s->spillPerLane * 8);
         pg::launchTrace(st, s->ds, dr.as<float>(), n, dt.as<float>(), dp.as<uint32_t>(), duv.as<float>(), nullptr, lanes,
-                        spill.as<uint2>(), s->spillPerLane);
+                        spill.as<uint2>(), s->spillPerLane, times ? dtimes.as<float>() : nullptr);
         PG_HIP(hipGetLastError());
         PG_HIP(hipStreamSynchronize(st));
         PG_HIP(hipMemcpy(t, dt.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));

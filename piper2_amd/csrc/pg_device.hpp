@@ -269,11 +269,11 @@ struct TravCount {
 // COUNT = true also counts wide nodes visited and triangles tested (roofline bookkeeping).
 template <int W, bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const RayPre& p, float tmin, float tmax,
-                                         HitRec& h, uint2* ls, uint2* gs, TravCount& cnt) {
+                                         HitRec& h, uint2* ls, uint2* gs, TravCount& cnt, uint32_t root = 0) {
     const WideNode<W>* __restrict__ nodes = static_cast<const WideNode<W>*>(s.nodes);
     const float4* __restrict__ tris = s.tris;
     int sp = 0;
-    uint32_t ref = 0;  // root is node 0
+    uint32_t ref = root;  // node 0: the static triangles' root; an instance's BLAS root otherwise
     while (true) {
         if (!(ref & kLeafBit)) {
             if (COUNT) ++cnt.nodes;
@@ -337,13 +337,143 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
     return ANY ? false : h.pos != kHitMiss;
 }
 
-// Closest hit of triangles and light quads (rtcIntersect1 with mask gsm|areaLight, Embree.cpp:292).
+// ------------------------------------------------------------------------------------------
+// Moving instances. Local-to-world transform at a time as Embree 3.12 forms it from the keys the
+// reference sets with rtcSetGeometryTransformQuaternion (Embree.cpp:699-703, 756-772): scale,
+// skew, shift and translation interpolate linearly between the two keys around the time, the
+// rotation by spherical linear interpolation along the shorter arc (Shoemake 1985), normalised
+// linear interpolation once the two quaternions' cosine exceeds 0.9995; the matrix is
+// T R S with S = [[sx, kxy, kxz, hx], [0, sy, kyz, hy], [0, 0, sz, hz]] (Embree's quaternion
+// decomposition), then the parents' static transform.
+// ------------------------------------------------------------------------------------------
+struct Affine {
+    float m[12];  // row-major 3x4
+};
+__device__ __forceinline__ V3 affPoint(const Affine& a, V3 p) {
+    return {a.m[0] * p.x + a.m[1] * p.y + a.m[2] * p.z + a.m[3], a.m[4] * p.x + a.m[5] * p.y + a.m[6] * p.z + a.m[7],
+            a.m[8] * p.x + a.m[9] * p.y + a.m[10] * p.z + a.m[11]};
+}
+__device__ __forceinline__ V3 affVector(const Affine& a, V3 v) {
+    return {a.m[0] * v.x + a.m[1] * v.y + a.m[2] * v.z, a.m[4] * v.x + a.m[5] * v.y + a.m[6] * v.z,
+            a.m[8] * v.x + a.m[9] * v.y + a.m[10] * v.z};
+}
+__device__ __forceinline__ Affine affCompose(const float* a, const Affine& b) {  // a * b
+    Affine r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            r.m[4 * i + j] = a[4 * i] * b.m[j] + a[4 * i + 1] * b.m[4 + j] + a[4 * i + 2] * b.m[8 + j] + (j == 3 ? a[4 * i + 3] : 0.0f);
+    return r;
+}
+__device__ __forceinline__ Affine affInverse(const Affine& a) {
+    const float* m = a.m;
+    const float c00 = m[5] * m[10] - m[6] * m[9], c01 = m[6] * m[8] - m[4] * m[10], c02 = m[4] * m[9] - m[5] * m[8];
+    const float inv = 1.0f / (m[0] * c00 + m[1] * c01 + m[2] * c02);
+    Affine r;
+    r.m[0] = c00 * inv, r.m[1] = (m[2] * m[9] - m[1] * m[10]) * inv, r.m[2] = (m[1] * m[6] - m[2] * m[5]) * inv;
+    r.m[4] = c01 * inv, r.m[5] = (m[0] * m[10] - m[2] * m[8]) * inv, r.m[6] = (m[2] * m[4] - m[0] * m[6]) * inv;
+    r.m[8] = c02 * inv, r.m[9] = (m[1] * m[8] - m[0] * m[9]) * inv, r.m[10] = (m[0] * m[5] - m[1] * m[4]) * inv;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r.m[4 * i + 3] = -(r.m[4 * i] * m[3] + r.m[4 * i + 1] * m[7] + r.m[4 * i + 2] * m[11]);
+    return r;
+}
+// Unit quaternions (w, x, y, z) q0, q1 at fraction f.
+__device__ __forceinline__ void quatSlerp(const float* q0, const float* q1in, float f, float q[4]) {
+    float q1[4] = {q1in[0], q1in[1], q1in[2], q1in[3]};
+    float c = q0[0] * q1[0] + q0[1] * q1[1] + q0[2] * q1[2] + q0[3] * q1[3];
+    if (c < 0.0f) {
+        c = -c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q1[i] = -q1[i];
+    }
+    if (c > 0.9995f) {
+        float n = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = q0[i] * (1.0f - f) + q1[i] * f, n += q[i] * q[i];
+        const float inv = 1.0f / sqrtf(n);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] *= inv;
+        return;
+    }
+    float p[4], n = 0.0f;  // the part of q1 orthogonal to q0
+#pragma unroll
+    for (int i = 0; i < 4; ++i) p[i] = q1[i] - c * q0[i], n += p[i] * p[i];
+    const float inv = 1.0f / sqrtf(n), phi = f * acosf(c);
+    const float cs = cosf(phi), sn = sinf(phi) * inv;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = cs * q0[i] + sn * p[i];
+}
+__device__ __forceinline__ Affine instanceAt(const DevInstance& I, float time) {
+    uint32_t seg = 0;
+    float f = 0.0f;
+    if (I.keys > 1) {
+        const float x = (time - I.t0) * I.rate;
+        const float fl = fminf(fmaxf(floorf(x), 0.0f), static_cast<float>(I.keys - 2));
+        seg = static_cast<uint32_t>(fl);
+        f = x - fl;
+    }
+    const float* a = I.srt[seg];
+    const float* b = I.srt[I.keys > 1 ? seg + 1 : seg];
+    float k[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) k[i] = a[i] * (1.0f - f) + b[i] * f;
+    float q[4];
+    quatSlerp(a + 9, b + 9, f, q);
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    const float r[9] = {1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y),
+                        2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x),
+                        2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y)};
+    Affine m;  // R S, then the translation
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        m.m[4 * i] = r0 * k[0];
+        m.m[4 * i + 1] = r0 * k[3] + r1 * k[1];
+        m.m[4 * i + 2] = r0 * k[4] + r1 * k[5] + r2 * k[2];
+        m.m[4 * i + 3] = r0 * k[6] + r1 * k[7] + r2 * k[8] + k[13 + i];
+    }
+    return affCompose(I.parent, m);
+}
+__device__ __forceinline__ bool instanceVisible(const DevInstance& I, float time) {
+    return I.visible && time >= I.t0 && time <= I.t1;
+}
+// Slab test of a world box (the instance's swept bounds) against (tmin, tmax].
+__device__ __forceinline__ bool boxHit(const float* lo, const float* hi, const RayPre& p, float tmin, float tmax) {
+    const float ax = lo[0] * p.ix - p.oix, bx = hi[0] * p.ix - p.oix;
+    const float ay = lo[1] * p.iy - p.oiy, by = hi[1] * p.iy - p.oiy;
+    const float az = lo[2] * p.iz - p.oiz, bz = hi[2] * p.iz - p.oiz;
+    const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax)) * 1.0000004f;
+    return t0 <= t1;
+}
+// The instances after the static BVH (an Embree instance node over each moving mesh): the ray goes
+// into the instance's local space at its time (same parameter t, the direction is not
+// normalised) and through the instance's BLAS.
+template <int W, bool ANY, bool COUNT = false>
+__device__ __forceinline__ bool traverseInstances(const DevScene& s, const Ray& r, const RayPre& p, float time,
+                                                  float tmin, float tmax, HitRec& h, uint2* ls, uint2* gs,
+                                                  TravCount& cnt) {
+    for (uint32_t k = 0; k < s.numInst; ++k) {
+        const DevInstance& I = s.inst[k];
+        if (!instanceVisible(I, time) || !boxHit(I.lo, I.hi, p, tmin, ANY ? tmax : h.t * 1.00001f)) continue;
+        const Affine inv = affInverse(instanceAt(I, time));
+        const Ray lr{affPoint(inv, r.o), affVector(inv, r.d)};
+        const RayPre lp = precompute(lr);
+        if (traverse<W, ANY, COUNT>(s, lr, lp, tmin, tmax, h, ls, gs, cnt, I.root) && ANY) return true;
+    }
+    return false;
+}
+
+// Closest hit of triangles and light quads (rtcIntersect1 with mask gsm|areaLight, Embree.cpp:292)
+// at ray time `time` (the sample's dimension 2, Embree/Kernel.cpp:77; only moving instances use it).
 template <int W, bool COUNT = false>
 __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls,
-                                             uint2* gs, TravCount& cnt) {
+                                             uint2* gs, TravCount& cnt, float time = 0.0f) {
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     if (s.hasBvh) traverse<W, false, COUNT>(s, r, p, tmin, tmax, h, ls, gs, cnt);
+    if (s.numInst) traverseInstances<W, false, COUNT>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
     for (uint32_t q = 0; q < s.numQuads; ++q) {
         float t;
         if (intersectQuad(r, s.quads[q], tmin, h.t, t)) h = HitRec{t, kHitQuad | q, kHitQuad | q, 0.0f, 0.0f};
@@ -354,12 +484,14 @@ __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, fl
 
 // rtcOccluded1 with mask gsm (Embree.cpp:357-391): light quads never occlude.
 template <int W>
-__device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls, uint2* gs) {
-    if (!s.hasBvh) return false;
+__device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls, uint2* gs,
+                                         float time = 0.0f) {
+    if (!s.hasBvh && !s.numInst) return false;
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     TravCount cnt;
-    return traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, cnt);
+    if (s.hasBvh && traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, cnt)) return true;
+    return s.numInst && traverseInstances<W, true>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -382,8 +514,20 @@ __device__ __forceinline__ Frame shadingFrame(V3 n, bool d3) {
     f.b = cross(n, f.t);
     return f;
 }
-__device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir, bool* front = nullptr) {
-    const V3 v0 = xyz(s.tris[3 * pos]), v1 = xyz(s.tris[3 * pos + 1]), v2 = xyz(s.tris[3 * pos + 2]);
+// A moving instance's triangle in world space at `time` (static triangles are stored there).
+__device__ __forceinline__ void worldTriangle(const DevInstance* inst, float4 a, float4 b, float4 c, float time, V3& v0,
+                                              V3& v1, V3& v2) {
+    v0 = xyz(a), v1 = xyz(b), v2 = xyz(c);
+    const uint32_t k = __float_as_uint(b.w);
+    if (k) {
+        const Affine m = instanceAt(inst[k - 1], time);
+        v0 = affPoint(m, v0), v1 = affPoint(m, v1), v2 = affPoint(m, v2);
+    }
+}
+__device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir, bool* front = nullptr,
+                                        float time = 0.0f) {
+    V3 v0, v1, v2;
+    worldTriangle(s.inst, s.tris[3 * pos], s.tris[3 * pos + 1], s.tris[3 * pos + 2], time, v0, v1, v2);
     const V3 ng = normalize(cross(v1 - v0, v2 - v0));
     const bool f = dot(dir, ng) < 0.0f;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
     if (front) *front = f;

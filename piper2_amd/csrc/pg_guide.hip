@@ -18,10 +18,11 @@ __device__ __forceinline__ GuideInput noInput() {
 // Inputs at a hit: position in the scene box mapped to [0,1]^3, the geometric normal facing the
 // ray (as triNormal) and the outgoing direction.
 __device__ __forceinline__ GuideInput hitInput(float4 oo, float4 dd, float t, float4 a, float4 b, float4 c, float3 lo,
-                                               float3 invExt) {
+                                               float3 invExt, const DevInstance* inst) {
     const V3 dir{dd.x, dd.y, dd.z};
     const V3 p = V3{oo.x, oo.y, oo.z} + dir * t;
-    const V3 v0 = xyz(a), v1 = xyz(b), v2 = xyz(c);
+    V3 v0, v1, v2;  // a moving instance's triangle at the path's time (d.w)
+    worldTriangle(inst, a, b, c, dd.w, v0, v1, v2);
     const V3 ng = normalize(cross(v1 - v0, v2 - v0));
     const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;
     const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
@@ -59,6 +60,7 @@ struct PathSource {
     const PathRec* paths;
     const uint32_t* queue;
     float3 lo, invExt;
+    const DevInstance* inst;  // the scene's moving instances
     struct Pend {
         uint32_t path;
         float4 h, o, d, a, b, c;
@@ -81,7 +83,7 @@ struct PathSource {
     __device__ __forceinline__ void stage4(Pend&) const {}
     __device__ __forceinline__ GuideInput finish(const Pend& q) const {
         if (!q.tri) return noInput();
-        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, lo, invExt);
+        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, lo, invExt, inst);
     }
 };
 
@@ -220,6 +222,7 @@ struct BatchPathSource {
         uint32_t path, item;
         float4 h, o, d, a, b, c;
         const float4* tris;
+        const DevInstance* inst;
         float3 lo, invExt;
         bool valid, tri;
     };
@@ -234,7 +237,9 @@ struct BatchPathSource {
             q.h = paths[q.path].hit, q.o = paths[q.path].o, q.d = paths[q.path].d, q.item = paths[q.path].meta.z >> 16;
     }
     __device__ __forceinline__ void stage3(Pend& q) const {
-        if (q.valid) q.tris = scenes[q.item].tris, q.lo = guide[q.item].lo, q.invExt = guide[q.item].invExt;
+        if (q.valid)
+            q.tris = scenes[q.item].tris, q.inst = scenes[q.item].inst, q.lo = guide[q.item].lo,
+            q.invExt = guide[q.item].invExt;
     }
     __device__ __forceinline__ void stage4(Pend& q) const {
         const uint32_t pos = __float_as_uint(q.h.y);
@@ -243,7 +248,7 @@ struct BatchPathSource {
     }
     __device__ __forceinline__ GuideInput finish(const Pend& q) const {
         if (!q.tri) return noInput();
-        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, q.lo, q.invExt);
+        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, q.lo, q.invExt, q.inst);
     }
 };
 
@@ -600,7 +605,7 @@ void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const ui
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks) {
-    PathSource src{s.tris, ps.p, queue, lo, invExt};
+    PathSource src{s.tris, ps.p, queue, lo, invExt, s.inst};
     hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds, st, src,
                        capacity, count, wb, theta, mix);
 }

@@ -43,7 +43,7 @@ __device__ __forceinline__ uint64_t segmentPaths(const RenderConsts& rc) {
 }
 
 __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts& rc, uint32_t px, uint32_t py,
-                                          uint32_t si, uint64_t& idx, FilmPoint& fp) {
+                                          uint32_t si, uint64_t& idx, FilmPoint& fp, float& time) {
     idx = intervalToIndex(s, rc.log2Res, si, px, py);
     const float res = static_cast<float>(1u << rc.log2Res);
     float u[8];
@@ -56,7 +56,8 @@ __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts&
     fp = FilmPoint{fx, fy};
     const float ndcX = rc.ndc[0] + rc.ndc[2] * fx / static_cast<float>(rc.width);
     const float ndcY = rc.ndc[1] + rc.ndc[3] * fy / static_cast<float>(rc.height);
-    // dim 2 is the time sample (unused by static scenes); dims 3, 4 drive the lens
+    // dim 2 is the path's time (Embree/Kernel.cpp:77), read by moving instances; dims 3, 4 drive the lens
+    time = u[2];
     return cameraRay(s.cam, ndcX, ndcY, u[3], u[4]);
 }
 
@@ -140,9 +141,10 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
             pathPixel(rc, sppBegin, sppCount, g, px, py, si);
             uint64_t idx;
             FilmPoint fp;
-            const Ray r = primaryRay(s, rc, px, py, si, idx, fp);
+            float time;
+            const Ray r = primaryRay(s, rc, px, py, si, idx, fp, time);
             ps.p[slot].o = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-            ps.p[slot].d = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+            ps.p[slot].d = make_float4(r.d.x, r.d.y, r.d.z, time);  // d.w: the path's time, kept by every bounce
             ps.p[slot].thr = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             ps.p[slot].L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             ps.p[slot].Lmis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const u
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[path].meta.z)] : s0;
         const float4 o = ps.p[path].o, d = ps.p[path].d;
         const Ray r{xyz(o), xyz(d)};
-        const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt);
+        const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt, d.w);
         ps.p[path].hit = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
     }
     if (COUNT) {
@@ -249,7 +251,9 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             const bool d1 = rc.flags & kCompatD1, d3 = rc.flags & kCompatD3, d9 = rc.flags & kCompatD9,
                        d13 = rc.flags & kCompatD13;
             const bool envBlack = !valid(ld3(s.env));
-            Ray ray{xyz(ps.p[path].o), xyz(ps.p[path].d)};
+            const float4 d4 = ps.p[path].d;
+            const float time = d4.w;
+            Ray ray{xyz(ps.p[path].o), xyz(d4)};
             V3 thr = xyz(ps.p[path].thr);
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them);
             // L and Lmis are written back only when they change (adding a zero leaves L's bits)
@@ -278,7 +282,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 pushDone = true;
             } else {
                 bool front;
-                const V3 nrm = triNormal(s, pos, ray.d, &front);
+                const V3 nrm = triNormal(s, pos, ray.d, &front, time);
                 const Mat mat = loadMat<TEX>(s, s.triMat[pos], front, pos, hit4.z, hit4.w);
                 const Frame fr = shadingFrame(nrm, d3);
                 const V3 wo = fr.toLocal(-ray.d);
@@ -410,7 +414,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         // :184-187: offset along +Ng for reflection, -Ng for transmission
                         const V3 no = hp + nrm * (ss.wi.z < 0.0f ? -0.001f : 0.001f);
                         ps.p[path].o = make_float4(no.x, no.y, no.z, 0.0f);
-                        ps.p[path].d = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                        ps.p[path].d = make_float4(nd.x, nd.y, nd.z, time);
                         ps.p[path].thr = make_float4(thr.x, thr.y, thr.z, 0.0f);
                         meta.y += 1u;
                         meta.z = specV ? (meta.z | 1u) : (meta.z & ~1u);  // :189 specular = part & Specular
@@ -504,7 +508,8 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
         const ShadowRec r = q.shadow[i];
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path & ~kMisFlag].meta.z)] : s0;
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
-        if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs)) {
+        const float time = s.numInst ? ps.p[r.path & ~kMisFlag].d.w : 0.0f;
+        if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs, time)) {
             if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
                 ps.p[r.path & ~kMisFlag].Lmis = make_float4(r.cr, r.cg, r.cb, 0.0f);
             } else {
@@ -533,6 +538,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path].meta.z)] : s0;
         const bool envBlack = !valid(ld3(s.env));
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
+        const float time = s.numInst ? ps.p[r.path].d.w : 0.0f;
         V3 rad{0.0f, 0.0f, 0.0f};
         float lpdf = 0.0f;
         if (envBlack && r.light != 0u) {
@@ -551,7 +557,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
                 }
                 // tq > 0: the next float up makes the occlusion range (tmin, tq] inclusive
                 const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                if (first && !occluded<W>(s, ray, 1e-3f, tqUp, lstack + lane, gs)) {
+                if (first && !occluded<W>(s, ray, 1e-3f, tqUp, lstack + lane, gs, time)) {
                     const V3 qn = quadNormal(qd, ray.d);
                     rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
                     lpdf = tq * tq / (fabsf(dot(qn, ray.d)) * qd.area);
@@ -559,7 +565,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
             }
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc);
+            const HitRec h = closestHit<W>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc, time);
             if (h.pos == kHitMiss) {
                 rad = ld3(s.env);
                 lpdf = envPdf(ray.d);
@@ -839,7 +845,8 @@ __global__ void __launch_bounds__(256) kCopy(const uint4* __restrict__ src, uint
 // ------------------------------------------------------------------------------------------
 template <int W>
 __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint32_t n, float* t, uint32_t* prim,
-                                             float* uv, uint8_t* occ, uint2* spill, uint32_t spillPerLane) {
+                                             float* uv, uint8_t* occ, uint2* spill, uint32_t spillPerLane,
+                                             const float* times) {
     __shared__ uint2 lstack[kLdsStack * 64];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
@@ -847,11 +854,12 @@ __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint
     for (uint32_t i = gtid; i < n; i += gridDim.x * 64) {
         const float* r = rays + 8 * static_cast<size_t>(i);
         const Ray ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+        const float time = times ? times[i] : 0.0f;
         if (occ) {
-            occ[i] = occluded<W>(s, ray, r[6], r[7], lstack + lane, gs) ? 1 : 0;
+            occ[i] = occluded<W>(s, ray, r[6], r[7], lstack + lane, gs, time) ? 1 : 0;
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W>(s, ray, r[6], r[7], lstack + lane, gs, tc);
+            const HitRec h = closestHit<W>(s, ray, r[6], r[7], lstack + lane, gs, tc, time);
             t[i] = h.t;
             prim[i] = h.prim;
             uv[2 * i] = h.u;
@@ -865,7 +873,8 @@ __global__ void kCameraRays(DevScene s, RenderConsts rc, const uint32_t* pix, ui
     if (i >= n) return;
     uint64_t idx;
     FilmPoint fp;
-    const Ray r = primaryRay(s, rc, pix[3 * i], pix[3 * i + 1], pix[3 * i + 2], idx, fp);
+    float time;
+    const Ray r = primaryRay(s, rc, pix[3 * i], pix[3 * i + 1], pix[3 * i + 2], idx, fp, time);
     float* o = rays + 8 * static_cast<size_t>(i);
     o[0] = r.o.x, o[1] = r.o.y, o[2] = r.o.z, o[3] = r.d.x, o[4] = r.d.y, o[5] = r.d.z, o[6] = 1e-3f, o[7] = 1e5f;
     film[2 * i] = fp.x, film[2 * i + 1] = fp.y;
@@ -1057,13 +1066,13 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
     }
 }
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
-                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane) {
+                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const float* times) {
     if (s.width == 8)
         hipLaunchKernelGGL(kTrace<8>, fit(kTrace<8>, gridLanes), dim3(64), 0, st,
-                s, rays, n, t, prim, uv, occ, spill, spillPerLane);
+                s, rays, n, t, prim, uv, occ, spill, spillPerLane, times);
     else
         hipLaunchKernelGGL(kTrace<4>, fit(kTrace<4>, gridLanes), dim3(64), 0, st,
-                s, rays, n, t, prim, uv, occ, spill, spillPerLane);
+                s, rays, n, t, prim, uv, occ, spill, spillPerLane, times);
 }
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film) {

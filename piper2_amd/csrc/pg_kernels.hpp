@@ -119,8 +119,9 @@ void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const Re
 // film. quad: 16 floats (or doubles with film64) per rect pixel.
 void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& ring, uint64_t first, void* quad,
                    float* film32, double* film64);
+// times: the ray times (moving instances), or nullptr for time 0.
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
-                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane);
+                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const float* times = nullptr);
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film);
 void launchCopy(hipStream_t st, const void* src, void* dst, size_t bytes);

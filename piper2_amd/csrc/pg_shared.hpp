@@ -78,6 +78,23 @@ struct DevCamera {
     float sizeX, sizeY, focal, aperture;
 };
 
+// Moving instance (a Group child with a keyframed TransformSRT, Embree.cpp:709-773): its mesh is a
+// BLAS of its own in the scene's node and triangle arrays, in the instance's local space, drawn at
+// each ray's time. Its leaf triangles carry the instance index + 1 in the w of their second vertex
+// (0 on static triangles).
+constexpr uint32_t kMaxMotionKeys = 8;
+constexpr uint32_t kMaxInstances = 256;
+struct DevInstance {
+    float lo[3], hi[3];  // world box over every time at which the instance is visible
+    float t0, t1;        // visible for t0 <= time <= t1 (shutter-normalised time of the first and last key)
+    float rate;          // (keys - 1) / (t1 - t0): key segments per unit time
+    uint32_t keys;       // transforms at equal time steps from t0 to t1 (1: fixed)
+    uint32_t root;       // BLAS root node
+    uint32_t visible;    // 0: absent from the frame
+    float parent[12];    // static row-major 3x4 transform of the instance's parent nodes
+    float srt[kMaxMotionKeys][16];  // TransformSRT: scale xyz, skew xy xz yz, shift xyz, unit quaternion w x y z, translation xyz
+};
+
 struct DevScene {
     const float4* tris;      // 3 float4 per triangle in leaf order: (v0, prim bits) (v1, 0) (v2, 0)
     const uint32_t* triMat;  // material per leaf-order triangle
@@ -99,7 +116,9 @@ struct DevScene {
     uint32_t textured;  // some material has a texture: shade takes its textured variant
     DevCamera cam;
     float env[3];
-    uint32_t hasBvh;
+    uint32_t hasBvh;              // static triangles exist (their BVH's root is node 0)
+    const DevInstance* inst;      // moving instances
+    uint32_t numInst;
 };
 
 // Queue records written by the shade stage.

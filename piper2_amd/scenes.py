@@ -517,6 +517,97 @@ def triangle_targets(seed=0, n_tris=16, rays_per_tri=64):
     return sc, rays, np.array(ts), np.array(uvs), np.array(idx, np.uint32)
 
 
+def srt(scale=(1.0, 1.0, 1.0), quat=(1.0, 0.0, 0.0, 0.0), trans=(0.0, 0.0, 0.0), skew=(0.0, 0.0, 0.0),
+        shift=(0.0, 0.0, 0.0)):
+    """A TransformSRT key as 16 floats (Kernel/Transform.hpp:390-393): scale xyz, skew xy xz yz,
+    shift xyz, quaternion w x y z, translation xyz."""
+    return (*map(float, scale), *map(float, skew), *map(float, shift), *map(float, quat), *map(float, trans))
+
+
+def motion_occluder(kind="translate", width=48, height=32, spp=16, t_range=(0.0, 1.0), half=0.6, travel=3.0,
+                    depth=5.0):
+    """A moving black occluder in front of a white environment, seen by a pinhole camera at the
+    origin looking down -z: every path's radiance is 1 or 0, depending only on whether its camera
+    ray meets the occluder at the path's time (closed form: occluder_blocked). The occluder is the
+    scene's one moving instance (no static geometry), two keys at times t_range:
+      "translate": a square of half-width `half` at z = -depth whose centre moves along x from
+                   -travel/2 to +travel/2;
+      "rotate":    a 2 x 0.3 plank at z = -depth turning a quarter turn about the view axis."""
+    b = _Builder()
+    blk = b.material(LAMBERT, [0.0, 0.0, 0.0])
+    if kind == "translate":
+        verts, tris = _quad_tris([-half, -half, 0], [half, -half, 0], [half, half, 0], [-half, half, 0])
+        keys = [srt(trans=(-travel / 2, 0.0, -depth)), srt(trans=(travel / 2, 0.0, -depth))]
+    elif kind == "rotate":
+        verts, tris = _quad_tris([-1.0, -0.15, 0], [1.0, -0.15, 0], [1.0, 0.15, 0], [-1.0, 0.15, 0])
+        h = math.sqrt(0.5)
+        keys = [srt(trans=(0.0, 0.0, -depth)), srt(quat=(h, 0.0, 0.0, h), trans=(0.0, 0.0, -depth))]
+    else:
+        raise ValueError(kind)
+    b.add(verts, tris, blk)
+    cam = make_camera([0.0, 0.0, 0.0], [0.0, 0.0, -depth], [0.0, 1.0, 0.0], 40.0, width / height)
+    ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
+    sc = b.finish(np.zeros((0, 12), np.float32), cam, width=width, height=height, spp=spp, max_depth=2, min_rr=2,
+                  ndc=ndc, name="motion_occluder_" + kind)
+    sc["env"] = np.ones(3, np.float32)
+    sc["instance_first"] = np.array([0], np.uint32)
+    sc["motions"] = [dict(keys=keys, t0=t_range[0], t1=t_range[1])]
+    sc["occluder"] = dict(kind=kind, half=half, travel=travel, depth=depth, t_range=t_range)
+    return sc
+
+
+def occluder_blocked(occ, rays, times):
+    """Closed form of motion_occluder, in float64: for camera rays (N, 8) and times
+    (N,), whether the ray meets the occluder at its time, and the distance of its plane point from
+    the occluder's outline (to leave out rays within rounding of an edge)."""
+    rays = np.asarray(rays, np.float64)
+    t = np.asarray(times, np.float64)
+    o, d = rays[:, 0:3], rays[:, 3:6]
+    s = (-occ["depth"] - o[:, 2]) / d[:, 2]
+    X, Y = o[:, 0] + d[:, 0] * s, o[:, 1] + d[:, 1] * s
+    t0, t1 = occ["t_range"]
+    present = (t >= t0) & (t <= t1)
+    f = np.clip((t - t0) / (t1 - t0), 0.0, 1.0)
+    if occ["kind"] == "translate":
+        cx = -occ["travel"] / 2 + occ["travel"] * f
+        a, b = np.abs(X - cx), np.abs(Y)
+        hx = hy = occ["half"]
+    else:
+        th = 0.5 * math.pi * f  # slerp: the rotation angle grows linearly in the fraction
+        a = np.abs(X * np.cos(th) + Y * np.sin(th))
+        b = np.abs(-X * np.sin(th) + Y * np.cos(th))
+        hx, hy = 1.0, 0.15
+    inside = (a <= hx) & (b <= hy)
+    margin = np.minimum(np.abs(a - hx), np.abs(b - hy))
+    margin = np.where(np.abs(t - t0) < 1e-6, 0.0, margin)
+    margin = np.where(np.abs(t - t1) < 1e-6, 0.0, margin)
+    return present & inside, margin
+
+
+def moving_cornell(width=64, height=64, spp=8, max_depth=5):
+    """The Cornell box (static) with a moving instance: a tall box that slides, turns about +y and
+    grows while the shutter is open (three keys), lit by the ceiling light: shadows, normals and
+    MIS rays all see the box at the path's time."""
+    sc = cornell(width=width, height=height, spp=spp, max_depth=max_depth)
+    b = _Builder()
+    b.v, b.t, b.obj = [sc["vertices"].astype(np.float64)], [sc["indices"].astype(np.int64)], [sc["tri_object"]]
+    b.objmat, b.mats, b.nv = list(sc["object_material"]), [list(m) for m in sc["materials"]], sc["vertices"].shape[0]
+    first = int(sc["indices"].shape[0])
+    verts, tris = _box((0.0, 0.0), 0.0, (0.3, 0.5, 0.3), 0.0)
+    b.add(verts, tris, b.material(LAMBERT, [0.7, 0.6, 0.3]))
+    out = b.finish(sc["quads"], sc["camera"], width=width, height=height, spp=spp, max_depth=max_depth,
+                   min_rr=max_depth, ndc=sc["ndc"], name="moving_cornell")
+    c, s_ = math.cos(math.radians(30)), math.sin(math.radians(30))
+    c2, s2 = math.cos(math.radians(60)), math.sin(math.radians(60))
+    y0 = float(np.min(sc["vertices"][:, 1])) + 0.01  # just above the floor: no coplanar faces
+    keys = [srt(trans=(-0.25, y0, 0.1)),
+            srt(quat=(c, 0.0, s_, 0.0), trans=(0.0, y0, 0.0)),
+            srt(scale=(1.0, 1.3, 1.0), quat=(c2, 0.0, s2, 0.0), trans=(0.25, y0, -0.1))]
+    out["instance_first"] = np.array([first], np.uint32)
+    out["motions"] = [dict(keys=keys, t0=0.0, t1=1.0)]
+    return out
+
+
 def world_triangles(sc):
     """(T, 3, 3) float32 world-space triangle vertices."""
     return sc["vertices"][sc["indices"]]

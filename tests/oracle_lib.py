@@ -27,6 +27,10 @@ def lib():
                                       ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]
         L.or_scene_destroy.argtypes = [_VP]
         L.or_scene_set_wide_bvh.argtypes = [_VP, ctypes.c_uint32, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]
+        L.or_scene_set_instances.argtypes = [_VP, _VP, ctypes.c_uint32]
+        L.or_scene_set_motion.argtypes = [_VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_float,
+                                          ctypes.c_float, _VP, _VP]
+        L.or_trace_at.argtypes = [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, _VP]
         L.or_scene_set_points.argtypes = [_VP, _VP, ctypes.c_uint32]
         L.or_scene_set_textures.argtypes = [_VP, _VP, _VP]
         L.or_scene_add_texture.restype = ctypes.c_uint32
@@ -74,6 +78,12 @@ class OracleScene:
         v, i, to, om, m, q, c, e = self._arrs
         self.h = lib().or_scene_create(_p(v), v.shape[0], _p(i), i.shape[0], _p(to), _p(om), om.shape[0], _p(m),
                                        m.shape[0], _p(q), q.shape[0], _p(c), _p(e))
+        first = sc.get("instance_first")
+        if first is not None and len(first):
+            self._first = np.ascontiguousarray(first, np.uint32)
+            lib().or_scene_set_instances(self.h, _p(self._first), len(self._first))
+            for k, m in enumerate(sc.get("motions") or []):
+                self.set_motion(k, **m)
         pts = sc.get("point_lights")
         if pts is not None and len(pts):
             self._pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
@@ -94,6 +104,21 @@ class OracleScene:
             mt = np.ascontiguousarray(sc["material_textures"], np.uint32).reshape(-1, 3)
             self._keep += [triuv, mt]
             lib().or_scene_set_textures(self.h, _p(triuv), _p(mt))
+
+    def set_motion(self, instance, keys, t0=0.0, t1=1.0, visible=True, parent=None):
+        """Keys of moving instance `instance` (as Scene.set_motion)."""
+        keys = np.ascontiguousarray(keys, np.float32).reshape(-1, 16)
+        par = np.ascontiguousarray(parent if parent is not None else [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+        lib().or_scene_set_motion(self.h, instance, keys.shape[0], int(bool(visible)), float(t0), float(t1), _p(keys),
+                                  _p(par))
+
+    def trace_at(self, rays, times):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        times = np.ascontiguousarray(times, np.float32).reshape(-1)
+        n = rays.shape[0]
+        t, prim, uv = np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros((n, 2), np.float32)
+        lib().or_trace_at(self.h, _p(rays), _p(times), n, _p(t), _p(prim), _p(uv))
+        return t, prim, uv
 
     def use_wide_bvh(self, width, nodes, tris):
         """Traverse the GPU library's own wide BVH (Scene.bvh_download()) from now on, in the GPU's
