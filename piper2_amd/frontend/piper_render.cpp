@@ -152,9 +152,18 @@ public:
             for (const Json& child : cfg.at("Children").array()) {
                 TransformInfo info;
                 if (const Json* tr = child.find("Transform")) {
-                    if (tr->find("Offset"))
-                        fail("time-sampled transforms (motion blur) are not supported by the HIP tracer" + tr->where());
-                    info.transforms.push_back({0u, parseTransform(*tr)});
+                    if (const Json* offset = tr->find("Offset")) {  // keyframes (PiperRenderer.cpp:170-187)
+                        info.offset = num(*offset);
+                        info.step = num(tr->at("Step"));
+                        for (const Json& key : tr->at("KeyFrames").array())
+                            info.transforms.push_back({count(key.at("Key")), parseTransform(key.at("Transform"))});
+                        if (info.transforms.size() < 2) fail("Invalid dynamic transform" + tr->where());
+                        for (size_t k = 1; k < info.transforms.size(); ++k)
+                            if (info.transforms[k].first <= info.transforms[k - 1].first)
+                                fail("Invalid dynamic transform" + tr->where());
+                    } else {
+                        info.transforms.push_back({0u, parseTransform(*tr)});
+                    }
                 }
                 children.push_back({info, build(child)});
             }
@@ -323,8 +332,11 @@ void put(std::ofstream& f, const T* p, size_t n) {
     f.write(reinterpret_cast<const char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
 }
 
+// Version 1: the layout below; version 2 (scenes with moving instances) always writes the point
+// light count and a texture flag, then the moving instances with their keys for the action's
+// first frame.
 void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorNode, const Globals& g, uint32_t width,
-                  uint32_t height, FitMode fit) {
+                  uint32_t height, FitMode fit, float frameBegin, float frameEnd) {
     const pg_camera* cam = nullptr;
     for (size_t i = 0; i < e.sensors.size(); ++i)
         if (e.sensors[i].first == sensorNode) cam = &e.cameras[i];
@@ -337,7 +349,8 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     std::ofstream f(out, std::ios::binary);
     if (!f) fail("cannot write " + out.string());
     f.write("PGSC", 4);
-    const uint32_t version = 1;
+    const bool moving = !e.instanceFirst.empty();
+    const uint32_t version = moving ? 2 : 1;
     put(f, &version, 1);
     put(f, head, 12);
     put(f, e.verts.data(), e.verts.size());
@@ -360,9 +373,13 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     const TransformSRT id = identitySRT();
     for (uint32_t o = 0; o < no; ++o) put(f, reinterpret_cast<const float*>(&id), 16);
     const uint32_t np = static_cast<uint32_t>(e.points.size());
-    if (np || !e.textures.empty()) {  // optional tail: point lights, then textures
+    if (np || !e.textures.empty() || moving) {  // optional tail: point lights, then textures
         put(f, &np, 1);
         for (const pg_point_light& p : e.points) put(f, p.position, 3), put(f, p.intensity, 3);
+    }
+    if (moving) {
+        const uint32_t hasTextures = e.textures.empty() ? 0u : 1u;
+        put(f, &hasTextures, 1);
     }
     if (!e.textures.empty()) {
         // texcoords (float2 per vertex, zeros when no mesh had any), the textures (type, channels,
@@ -381,6 +398,18 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
             put(f, e.textureTexels[i].data(), e.textureTexels[i].size());
         }
         for (const pg_material_textures& b : e.materialTextures) put(f, &b.albedo, 1), put(f, &b.reserved, 1), put(f, &b.alpha, 1);
+    }
+    if (moving) {  // moving instances: first triangles, then per instance its frame keys
+        const uint32_t ni = static_cast<uint32_t>(e.instanceFirst.size());
+        put(f, &ni, 1);
+        put(f, e.instanceFirst.data(), ni);
+        for (const auto& m : e.motions) {
+            const pg_motion pm = motionForInterval(*m.keys, m.parent, frameBegin, frameEnd);
+            const uint32_t head2[2] = {pm.num_keys, pm.visible};
+            put(f, head2, 2), put(f, &pm.t0, 1), put(f, &pm.t1, 1);
+            for (uint32_t k = 0; k < pm.num_keys; ++k) put(f, pm.keys[k], 16);
+            put(f, pm.parent, 12);
+        }
     }
     if (!f) fail("cannot write " + out.string());
 }
@@ -417,7 +446,7 @@ int render(const fs::path& optionsPath, const std::string& exportDir) {
             if (!builder.sensors.count(node.get()))
                 fail("ActiveSensor \"" + actions[i].activeSensor + "\" is not a sensor");
             exportAction(fs::path(exportDir) / ("Action" + std::to_string(i) + ".pgsc"), e, node.get(), g, width, height,
-                         actions[i].fitMode);
+                         actions[i].fitMode, actions[i].timeBegin, actions[i].timeBegin + actions[i].shutterOpen);
         }
         std::printf("exported %zu action(s) to %s\n", actions.size(), exportDir.c_str());
         return 0;

@@ -227,7 +227,12 @@ struct LeafNode final : Node {
     std::shared_ptr<Object> object;  // HIPGSMInstance, Light or Sensor
 };
 struct GroupNode final : Node {
-    std::vector<std::pair<Mat34, std::shared_ptr<Node>>> children;
+    struct Child {
+        Mat34 m;                                    // static transform (identity for a keyframed child)
+        std::shared_ptr<const TransformInfo> keys;  // keyframed transform (EmbreeBranchNode), or null
+        std::shared_ptr<Node> node;
+    };
+    std::vector<Child> children;
 };
 
 // The scene graph collected for materialisation (Embree.cpp:1096-1106 collect()): instances,
@@ -236,8 +241,14 @@ struct GroupNode final : Node {
 struct Collected {
     struct Inst {
         const HIPGSMInstance* inst;
-        Mat34 toWorld;
+        Mat34 toWorld;  // into world space, or into its moving instance's space
+        int mover;      // index into movers, -1 for static geometry
     };
+    struct Mover {  // a keyframed Group child: its subtree moves as one instance
+        std::shared_ptr<const TransformInfo> keys;
+        Mat34 parent;  // static transform of the Group it sits in
+    };
+    std::vector<Mover> movers;
     struct Lit {
         std::shared_ptr<Light> light;
         TraversalHandle traversal;
@@ -261,15 +272,28 @@ struct Collected {
     }
 };
 
-void collect(const std::shared_ptr<Node>& node, const Mat34& toWorld, Collected& c) {
+// A keyframed child starts a moving instance (the reference's instance node with a time-sampled
+// transform, Embree.cpp:777-835): its subtree is flattened into the child's own space under the
+// static transform of the Group it sits in.
+void collect(const std::shared_ptr<Node>& node, const Mat34& toWorld, Collected& c, int mover = -1) {
     if (auto* g = dynamic_cast<const GroupNode*>(node.get())) {
-        for (const auto& [m, child] : g->children) collect(child, compose(toWorld, m), c);
+        for (const auto& ch : g->children) {
+            if (ch.keys) {
+                if (mover >= 0) raise("buildPipeline: keyframed transforms nested in a keyframed transform are not supported");
+                c.movers.push_back({ch.keys, toWorld});
+                collect(ch.node, identity34(), c, static_cast<int>(c.movers.size() - 1));
+            } else {
+                collect(ch.node, compose(toWorld, ch.m), c, mover);
+            }
+        }
         return;
     }
     auto* leaf = dynamic_cast<const LeafNode*>(node.get());
     if (!leaf) raise("buildPipeline: node was not built by this tracer");
     if (auto* inst = dynamic_cast<const HIPGSMInstance*>(leaf->object.get())) {
-        c.instances.push_back({inst, toWorld});
+        c.instances.push_back({inst, toWorld, mover});
+    } else if (mover >= 0) {
+        raise("buildPipeline: lights and sensors under a keyframed transform are not supported");
     } else if (auto light = std::dynamic_pointer_cast<Light>(leaf->object)) {
         c.lights.push_back({light, c.handle(toWorld)});
     } else if (auto* sensor = dynamic_cast<const Sensor*>(leaf->object.get())) {
@@ -540,7 +564,16 @@ DecodedScene decodeScene(Tracer& tracer, Registry& reg, const std::shared_ptr<No
     collect(scene, identity34(), c);
     SceneDecoder dec(tracer, reg);
     SceneExport& f = dec.out;
-    for (const auto& [inst, toWorld] : c.instances) {
+    // static triangles first, then each moving instance's (pg_scene_create_instanced's layout)
+    std::stable_sort(c.instances.begin(), c.instances.end(),
+                     [](const Collected::Inst& a, const Collected::Inst& b) { return a.mover < b.mover; });
+    int lastMover = -1;
+    for (const auto& [inst, toWorld, mover] : c.instances) {
+        if (mover >= 0 && mover != lastMover) {  // a moving instance's triangles start here
+            f.instanceFirst.push_back(static_cast<uint32_t>(f.index.size() / 3));
+            f.motions.push_back({c.movers[mover].keys, c.movers[mover].parent});
+            lastMover = mover;
+        }
         const uint32_t base = static_cast<uint32_t>(f.verts.size() / 3);
         for (size_t v = 0; v < inst->mesh->verts.size(); v += 3) {
             const auto p = applyPoint(toWorld, &inst->mesh->verts[v]);
@@ -591,6 +624,7 @@ public:
         pg_render_params base;
         std::vector<Tile> pending;
         std::vector<pg_stats> stats;
+        std::string error;  // of the last updateTimeInterval
     };
 
     HIPTraceLauncher(HIPAccelerator& acc, std::shared_ptr<Kernel> kernel, std::shared_ptr<pg_scene> scene, uint32_t w,
@@ -603,11 +637,25 @@ public:
     }
     [[nodiscard]] std::pair<uint32_t, uint32_t> getFilmResolution() const noexcept override { return {mW, mH}; }
     [[nodiscard]] RenderRECT getRenderRECT() const noexcept override { return mRect; }
-    // Static scenes: the time interval selects nothing (ray time is still Sobol dimension 2).
-    void updateTimeInterval(float begin, float end) noexcept override { mTime = {begin, end}; }
+    // Sets every moving instance's keys for the frame (the ray time is Sobol dimension 2 within
+    // it); a frame the reference cannot render is reported by the next launch.
+    void updateTimeInterval(float begin, float end) noexcept override {
+        mTime = {begin, end};
+        mShared->error.clear();
+        try {
+            for (size_t k = 0; k < mMotions.size(); ++k) {
+                const pg_motion m = motionForInterval(*mMotions[k].keys, mMotions[k].parent, begin, end);
+                check(pg_scene_set_motion(mScene.get(), static_cast<uint32_t>(k), &m), "pg_scene_set_motion");
+            }
+        } catch (const std::exception& e) {
+            mShared->error = e.what();
+        }
+    }
+    void setMotions(std::vector<SceneExport::Motion> motions) { mMotions = std::move(motions); }
 
     [[nodiscard]] FutureVoid launch(const RenderRECT& rect, const std::function<SBTPayload(uint32_t)>& launchData,
                                     const std::vector<std::shared_ptr<Resource>>& resources) override {
+        if (!mShared->error.empty()) raise(mShared->error);
         const auto ld = unpack<LaunchData>(launchData(0), "launch: LaunchData");
         if (ld.w != mW || ld.h != mH) raise("launch: LaunchData resolution differs from the film");
         if (ld.rgbw >= resources.size()) raise("launch: LaunchData names a missing resource");
@@ -673,6 +721,7 @@ private:
     uint32_t mW, mH;
     RenderRECT mRect;
     std::pair<float, float> mTime{0.0f, 0.0f};
+    std::vector<SceneExport::Motion> mMotions;
     std::shared_ptr<Shared> mShared;
 };
 
@@ -760,7 +809,12 @@ public:
             d.num_materials = 1;
         }
         pg_scene* raw = nullptr;
-        check(pg_scene_create(mAcc.device(), &d, &raw), "pg_scene_create");
+        if (f.instanceFirst.empty())
+            check(pg_scene_create(mAcc.device(), &d, &raw), "pg_scene_create");
+        else
+            check(pg_scene_create_instanced(mAcc.device(), &d, f.instanceFirst.data(),
+                                            static_cast<uint32_t>(f.instanceFirst.size()), &raw),
+                  "pg_scene_create_instanced");
         std::shared_ptr<pg_scene> scene(raw, pg_scene_destroy);
         const auto [ndc, rect] = calcRenderRECT(width, height, cam->sensor_size[0] / cam->sensor_size[1], fitMode);
         pg_render_params p = mBase;
@@ -768,6 +822,7 @@ public:
         p.spp_begin = 0, p.spp_end = sa.samplesPerPixel;
         p.ndc[0] = ndc.ox, p.ndc[1] = ndc.oy, p.ndc[2] = ndc.sx, p.ndc[3] = ndc.sy;
         auto launcher = std::make_shared<HIPTraceLauncher>(mAcc, mKernel, scene, width, height, rect, p);
+        launcher->setMotions(f.motions);
         mLaunchers.push_back(launcher);
         return launcher;
     }
@@ -1236,18 +1291,81 @@ std::shared_ptr<Node> HIPTracer::buildNode(const std::shared_ptr<Object>& object
     return n;
 }
 
+// EmbreeBranchNode (Embree.cpp:784-808): a static child transform is one key of index 0 with
+// step <= 0; a keyframed one (step > 0) moves its subtree as one instance at each ray's time.
 std::shared_ptr<Node> HIPTracer::buildNode(const std::vector<std::pair<TransformInfo, std::shared_ptr<Node>>>& children) {
     auto g = std::make_shared<GroupNode>();
     for (const auto& [info, child] : children) {
         if (!child) raise("buildNode: null child");
-        if (info.transforms.size() > 1)
-            raise("buildNode: time-sampled transforms (motion blur) are not supported by the HIP tracer");
+        if (info.step > 0.0f) {
+            if (info.transforms.empty()) raise("buildNode: a keyframed transform without keys");
+            for (size_t k = 1; k < info.transforms.size(); ++k)
+                if (info.transforms[k].first <= info.transforms[k - 1].first)
+                    raise("buildNode: keyframe indices must increase");
+            for (const auto& key : info.transforms) (void)fromSRT(key.second);  // rejects a zero quaternion
+            g->children.push_back({identity34(), std::make_shared<const TransformInfo>(info), child});
+            continue;
+        }
         Mat34 m = identity34();
-        if (!info.transforms.empty() && !isIdentity(info.transforms.front().second))
-            m = fromSRT(info.transforms.front().second);
-        g->children.push_back({m, child});
+        if (!info.transforms.empty()) {
+            if (info.transforms.size() != 1 || info.transforms.front().first != 0u) raise("Unrecognized transform.");
+            if (!isIdentity(info.transforms.front().second)) m = fromSRT(info.transforms.front().second);
+        }
+        g->children.push_back({m, nullptr, child});
     }
     return g;
+}
+
+pg_motion motionForInterval(const TransformInfo& info, const std::array<float, 12>& parent, float begin, float end) {
+    pg_motion out{};
+    std::memcpy(out.parent, parent.data(), sizeof(out.parent));
+    const auto& keys = info.transforms;
+    if (keys.empty()) raise("updateTimeInterval: a keyframed transform without keys");
+    auto timeOf = [&](const std::pair<uint32_t, TransformSRT>& k) { return info.offset + info.step * static_cast<float>(k.first); };
+    auto put = [&](uint32_t slot, const TransformSRT& t) { std::memcpy(out.keys[slot], &t, sizeof(t)); };
+    out.num_keys = 1;
+    put(0, keys.front().second);
+    if (timeOf(keys.back()) < begin || timeOf(keys.front()) > end) {  // rtcDisableGeometry (:718-724)
+        out.visible = 0;
+        return out;
+    }
+    out.visible = 1;
+    // both ends looked up with `begin` (:726-734): the last key at or before it and the first after it
+    auto after = std::upper_bound(keys.begin(), keys.end(), begin,
+                                  [&](float t, const std::pair<uint32_t, TransformSRT>& k) { return t < timeOf(k); });
+    auto first = after, last = after;
+    if (first != keys.begin()) --first;
+    if (last == keys.end()) --last;
+    uint32_t step = 0;  // gcd of the index steps from first to last (:742-749)
+    for (auto it = first; it != last; ++it) {
+        uint32_t a = step, b = std::next(it)->first - it->first;
+        while (b) a %= b, std::swap(a, b);
+        step = a;
+    }
+    if (step == 0 || !(end > begin))
+        raise("updateTimeInterval: the frame [" + std::to_string(begin) + ", " + std::to_string(end) +
+              "] opens outside the keyframes' time span (before the first key or at or after the last) or is empty; "
+              "the reference divides by zero there");
+    const uint32_t count = (last->first - first->first) / step + 1;
+    if (count > PG_MAX_MOTION_KEYS) raise("updateTimeInterval: more than 8 time steps in one frame");
+    out.num_keys = count;
+    out.t0 = (timeOf(*first) - begin) / (end - begin);
+    out.t1 = (timeOf(*last) - begin) / (end - begin);
+    put(0, first->second);
+    auto cur = first, nxt = std::next(first);
+    for (uint32_t i = 1; i < count; ++i) {  // in-between steps: lerp of the neighbouring keys (:762-771)
+        const uint32_t key = first->first + i * step;
+        if (key == nxt->first) {
+            put(i, nxt->second);
+            cur = nxt, ++nxt;
+        } else {
+            const float u = static_cast<float>(key - cur->first) / static_cast<float>(nxt->first - cur->first);
+            const float* x = reinterpret_cast<const float*>(&cur->second);
+            const float* y = reinterpret_cast<const float*>(&nxt->second);
+            for (int c = 0; c < 16; ++c) out.keys[i][c] = x[c] * (1.0f - u) + y[c] * u;
+        }
+    }
+    return out;
 }
 
 std::shared_ptr<RTProgram> HIPTracer::buildProgram(const LinkableProgram& linkable, std::string symbol) {

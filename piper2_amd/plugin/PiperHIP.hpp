@@ -64,10 +64,19 @@ struct alignas(16) TransformSRT final {
         transY, transZ;
 };
 TransformSRT identitySRT();
-struct TransformInfo final {  // Tracer.hpp:80-84 (keyframes; only static transforms are supported)
+// Tracer.hpp:80-84. Static: step <= 0 and at most one key, index 0 (EmbreeBranchNode,
+// Embree.cpp:790-801). Keyframed (step > 0): key k sits at time offset + step * index.
+struct TransformInfo final {
     float offset = 0.0f, step = 0.0f;
     std::vector<std::pair<uint32_t, TransformSRT>> transforms;  // local to world
 };
+// The transform of a keyframed instance for the frame [begin, end], as updateTransformSRT
+// (Embree.cpp:709-773) hands it to Embree: the keys around `begin` (the reference looks up both
+// ends of the key range with `begin`), their times mapped to the shutter as (t - begin) / (end -
+// begin); invisible when the keys' span misses the frame. parent: the static transform of the
+// Group the instance sits in. Raises where the reference divides by zero (a shutter that opens
+// before the first key or at or after the last, or end == begin).
+pg_motion motionForInterval(const TransformInfo& info, const std::array<float, 12>& parent, float begin, float end);
 
 using SBTPayload = std::vector<std::byte>;  // Tracer.hpp:86-91
 template <typename T>
@@ -661,6 +670,13 @@ struct SceneExport {
     std::vector<pg_texture> textures;                    // texels point into textureTexels
     std::vector<std::vector<uint8_t>> textureTexels;
     std::vector<pg_material_textures> materialTextures;  // per material (empty without textures)
+    // moving instances: instance k's triangles start at instanceFirst[k] (static ones come first)
+    struct Motion {
+        std::shared_ptr<const TransformInfo> keys;
+        std::array<float, 12> parent;  // static transform of the Group it sits in
+    };
+    std::vector<uint32_t> instanceFirst;
+    std::vector<Motion> motions;
 };
 SceneExport exportScene(Tracer& tracer, const std::shared_ptr<Node>& scene);
 

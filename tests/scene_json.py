@@ -208,7 +208,8 @@ def read_pgsc(path):
     """The scene-file layout of tests/plugin/render_scene.cpp, as a scene dict."""
     with open(path, "rb") as f:
         data = f.read()
-    assert data[:4] == b"PGSC" and struct.unpack("<I", data[4:8])[0] == 1
+    version = struct.unpack("<I", data[4:8])[0]
+    assert data[:4] == b"PGSC" and version in (1, 2)
     head = np.frombuffer(data, np.uint32, 12, 8)
     nv, nt, no, nm, nq, w, h, spp, md, rr, compat, fit = (int(x) for x in head)
     off = 56
@@ -230,11 +231,16 @@ def read_pgsc(path):
     take("env", np.float32, 3, (3,))
     take("srt", np.float32, 16 * no, (no, 16))
     assert np.all(out["srt"] == IDENTITY_SRT[None])
+    has_textures = False
     if off < len(data):
         npt = struct.unpack("<I", data[off:off + 4])[0]
         off += 4
         take("point_lights", np.float32, 6 * npt, (npt, 6))
-    if off < len(data):  # textures
+        has_textures = off < len(data)
+    if version == 2:  # moving instances: an explicit texture flag, the instances after the textures
+        has_textures = struct.unpack("<I", data[off:off + 4])[0] == 1
+        off += 4
+    if has_textures:
         take("texcoords", np.float32, 2 * nv, (nv, 2))
         ntex = struct.unpack("<I", data[off:off + 4])[0]
         off += 4
@@ -253,6 +259,21 @@ def read_pgsc(path):
             texs.append(t)
         out["textures"] = texs
         take("material_textures", np.uint32, 3 * nm, (nm, 3))
+    if version == 2:
+        ni = struct.unpack("<I", data[off:off + 4])[0]
+        off += 4
+        take("instance_first", np.uint32, ni, (ni,))
+        motions = []
+        for _ in range(ni):
+            nk, vis = struct.unpack("<II", data[off:off + 8])
+            t0, t1 = struct.unpack("<ff", data[off + 8:off + 16])
+            off += 16
+            keys = np.frombuffer(data, np.float32, 16 * nk, off).reshape(nk, 16).copy()
+            off += keys.nbytes
+            parent = np.frombuffer(data, np.float32, 12, off).copy()
+            off += parent.nbytes
+            motions.append(dict(keys=keys, t0=t0, t1=t1, visible=bool(vis), parent=parent))
+        out["motions"] = motions
     assert off == len(data)
     out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
     return out

@@ -205,6 +205,14 @@ def test_exr_layout_and_half_rounding(tmp_path):
                           "screenWindowCenter", "screenWindowWidth"}
 
 
+def _keyframe(scene, keys, step, child=0):
+    """Wrap scene child `child` in a Group with a keyframed transform of the given key indices."""
+    node = scene["Scene"]["Children"][child]
+    kf = [{"Key": k, "Transform": {"Translation": [0.1 * k, 0.0, 0.0]}} for k in keys]
+    scene["Scene"]["Children"][child] = {"Type": "Group", "Children": [
+        dict(node, Transform={"Offset": 0.0, "Step": step, "KeyFrames": kf})]}
+
+
 @pytest.mark.parametrize("mutate,message", [
     (lambda s: s.update(Integrator={"ClassID": "Piper.BuiltinComponent.PathTracing.Integrator"}), "MaxTraceDepth"),
     (lambda s: s["Scene"]["Children"][0]["Surface"].update(Roughness={"ClassID": "Piper.BuiltinComponent."
@@ -230,10 +238,19 @@ def test_exr_layout_and_half_rounding(tmp_path):
     (lambda s: s["Actions"][0].update(FitMode="Stretch"), "Invalid FitMode"),
     (lambda s: s["Scene"]["Children"].append({"Type": "Group", "Children": [{"Type": "Instance", "ClassID": "X.Y",
                                                                              "Transform": {"Offset": 0}}]}),
-     "motion blur"),
+     'missing key "Step"'),
+    (lambda s: _keyframe(s, [0], step=1.0), "Invalid dynamic transform"),
+    (lambda s: _keyframe(s, [2, 1], step=1.0), "Invalid dynamic transform"),
+    (lambda s: _keyframe(s, [0, 1], step=0.0), "Unrecognized transform."),
+    (lambda s: (_keyframe(s, [1, 2], step=1.0), s["Actions"][0].update(TimeBegin=0.5, ShutterOpenTime=1.0)),
+     "opens outside the keyframes' time span"),
+    (lambda s: _keyframe(s, [0, 1], step=1.0, child=-2), "lights and sensors under a keyframed transform"),
     (lambda s: s["Scene"]["Children"][0].update(Type="Mesh"), "invalid scene node type"),
 ])
 def test_errors_are_reported(tmp_path, mutate, message):
+    """Besides malformed input: a keyframed transform needs two or more increasing keys and Step > 0
+    (PiperRenderer.cpp:181-187, Embree.cpp:795-800), a frame opening outside its keys' span (where
+    the reference's updateTransformSRT divides by zero) and lights or sensors under it are refused."""
     sc = scenes.cornell(width=16, height=16, spp=1)
     opts = write_scene_json(sc, str(tmp_path))
     path = os.path.join(os.path.dirname(opts), "scene.json")
@@ -310,3 +327,60 @@ def test_malformed_png_is_reported(tmp_path, corrupt, message):
     json.dump(scene, open(path, "w"))
     r = run(opts, "--export", tmp_path / "x")
     assert r.returncode == 1 and message in r.stderr, r.stderr
+
+
+def _motion_scene(tmp_path, begin, shutter_open, frames=1):
+    """The Cornell box with its first box (object 5) on a keyframed transform: keys 0, 1 and 3 at
+    time Offset 0 + Step 0.5 x key, translated and turned; one action from `begin`."""
+    sc = scenes.cornell(width=32, height=24, spp=4)
+    kf = [{"Key": 0, "Transform": {"Translation": [0.0, 0.0, 0.0]}},
+          {"Key": 1, "Transform": {"Translation": [0.2, 0.0, -0.1], "Rotation": [1.0, 0.0, 0.0, 0.1]}},
+          {"Key": 3, "Transform": {"Translation": [0.4, 0.1, 0.0], "Scale": [1.0, 1.2, 1.0]}}]
+    opts = write_scene_json(sc, str(tmp_path), transforms={5: {"Offset": 0.0, "Step": 0.5, "KeyFrames": kf}})
+    path = os.path.join(os.path.dirname(opts), "scene.json")
+    scene = json.load(open(path))
+    scene["Actions"][0].update(TimeBegin=begin, ShutterOpenTime=shutter_open, ShutterCloseTime=0.1,
+                               FrameCount=frames)
+    json.dump(scene, open(path, "w"))
+    return sc, opts
+
+
+@pytest.mark.parametrize("begin,shutter,keys,t0,t1", [
+    (0.6, 0.2, (1, 3), -0.5, 4.5),    # keys 1 (time 0.5) and 3 (time 1.5) around the frame
+    (0.6, 1.4, (1, 3), -1 / 14, 9 / 14),  # a frame past key 3: still keys 1 and 3 (looked up with begin)
+    (0.0, 0.25, (0, 1), 0.0, 2.0),    # opening exactly at key 0
+])
+def test_keyframed_transform_export(tmp_path, begin, shutter, keys, t0, t1):
+    """The moving object's triangles follow the static ones in its own space; its frame keys are the
+    two around the frame's begin with times (t - begin) / (end - begin) (updateTransformSRT,
+    Embree.cpp:709-773)."""
+    sc, opts = _motion_scene(tmp_path, begin, shutter)
+    r = run(opts, "--export", tmp_path / "exp")
+    assert r.returncode == 0, r.stdout + r.stderr
+    e = read_pgsc(tmp_path / "exp" / "Action0.pgsc")
+    moving = sc["tri_object"] == 5
+    assert list(e["instance_first"]) == [int((~moving).sum())]
+    obj = e["tri_object"]
+    mats = e["materials"][e["object_material"][obj[e["instance_first"][0]:]]]
+    assert np.all(mats == sc["materials"][sc["object_material"][5]][None])
+    tri = sc["indices"][moving]
+    vid = np.unique(tri)
+    assert np.array_equal(np.unique(e["vertices"][e["indices"][e["instance_first"][0]:]].reshape(-1, 3), axis=0),
+                          np.unique(sc["vertices"][vid], axis=0))
+    (m,) = e["motions"]
+    assert m["visible"] and m["keys"].shape == (2, 16)
+    assert m["t0"] == pytest.approx(t0, rel=1e-5, abs=1e-6) and m["t1"] == pytest.approx(t1, rel=1e-5)
+    want = {0: scenes.srt(), 1: scenes.srt(trans=(0.2, 0.0, -0.1), quat=(1.0, 0.0, 0.0, 0.1)),
+            3: scenes.srt(trans=(0.4, 0.1, 0.0), scale=(1.0, 1.2, 1.0))}
+    assert np.array_equal(m["keys"], np.array([want[k] for k in keys], np.float32))
+    assert np.array_equal(m["parent"], np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32))
+
+
+def test_keyframed_object_outside_its_keys_is_absent(tmp_path):
+    """A frame after the last key's time leaves the object out of the frame (rtcDisableGeometry,
+    Embree.cpp:718-724)."""
+    sc, opts = _motion_scene(tmp_path, 2.0, 0.5)
+    r = run(opts, "--export", tmp_path / "exp")
+    assert r.returncode == 0, r.stdout + r.stderr
+    (m,) = read_pgsc(tmp_path / "exp" / "Action0.pgsc")["motions"]
+    assert not m["visible"]

@@ -72,3 +72,36 @@ def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured, plane)
         ref = OracleScene(ex).render(path_l=True, film=False, threads=16)["path_l"]
         close = np.all(np.abs(pl - ref) <= 1e-3 * np.abs(ref) + 1e-5, axis=1)
         assert close.mean() >= 0.99, close.mean()
+
+
+def test_keyframed_scene_frames_render_like_pg_render(dev, tmp_path):
+    """A box on a keyframed transform: piper_render's frames (the launcher sets each frame's keys,
+    updateTransformSRT's selection) equal pg_render of the exported scene with that frame's keys,
+    whose paths match the oracle's."""
+    sc = scenes.cornell(width=48, height=32, spp=8)
+    kf = [{"Key": 0, "Transform": {"Translation": [0.0, 0.0, 0.0]}},
+          {"Key": 1, "Transform": {"Translation": [0.3, 0.0, -0.1], "Rotation": [1.0, 0.0, 0.3, 0.0]}},
+          {"Key": 3, "Transform": {"Translation": [0.5, 0.1, 0.0], "Scale": [1.0, 1.3, 1.0]}}]
+    opts = write_scene_json(sc, str(tmp_path), transforms={5: {"Offset": 0.0, "Step": 0.5, "KeyFrames": kf}})
+    scene_path = tmp_path / "scene.json"
+    scene = json.load(open(scene_path))
+    a0 = dict(scene["Actions"][0], TimeBegin=0.6, ShutterOpenTime=0.2, ShutterCloseTime=0.1, FrameCount=2)
+    scene["Actions"] = [a0, dict(a0, TimeBegin=0.9, FrameCount=1)]  # action 1 = action 0's second frame
+    json.dump(scene, open(scene_path, "w"))
+    r = subprocess.run([FRONTEND, opts, "--export", tmp_path / "exp"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    exports = [read_pgsc(tmp_path / "exp" / f"Action{i}.pgsc") for i in range(2)]
+    print("piper_render", opts, flush=True)
+    r = subprocess.run([FRONTEND, opts], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    frames = [read_exr(tmp_path / "out" / "Action0" / f"Frame{k}.exr")[0] for k in range(2)]
+    for frame, e in zip(frames, exports):
+        assert np.array_equal(frame.view(np.uint16), direct_rgb(dev, e).view(np.uint16))
+    assert not np.array_equal(frames[0].view(np.uint16), frames[1].view(np.uint16))
+    ex = dict(exports[0])
+    ex["ndc"], _ = scenes.calc_render_rect(ex["width"], ex["height"], float(ex["camera"][9] / ex["camera"][10]))
+    s = pg.Scene(dev, ex)
+    _, pl = s.render(pg.render_params(ex), None, path_l=True)
+    ref = OracleScene(ex).render(path_l=True, film=False, threads=16)["path_l"]
+    close = np.all(np.abs(pl - ref) <= 1e-3 * np.abs(ref) + 1e-5, axis=1)
+    assert close.mean() >= 0.98, close.mean()
