@@ -789,6 +789,7 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
         upload(dev->batchOffsets, tabOffsets.data(), tabOffsets.size() * 8, st);
         bt = pg::BatchTable{dev->batchScenes.as<pg::DevScene>(), dev->batchRc.as<pg::RenderConsts>(),
                             dev->batchOffsets.as<uint64_t>(), static_cast<uint32_t>(items.size())};
+        for (uint32_t i = 0; i < nScenes; ++i) bt.motion |= scenes[i]->ds.numInst != 0 ? 1u : 0u;
     }
     const pg::BatchTable* btp = opt.batch ? &bt : nullptr;
     const pg::DevScene& ds0 = scenes[0]->ds;  // the batch's BVH width; per-path scenes come from bt

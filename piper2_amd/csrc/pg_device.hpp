@@ -467,13 +467,15 @@ __device__ __forceinline__ bool traverseInstances(const DevScene& s, const Ray& 
 
 // Closest hit of triangles and light quads (rtcIntersect1 with mask gsm|areaLight, Embree.cpp:292)
 // at ray time `time` (the sample's dimension 2, Embree/Kernel.cpp:77; only moving instances use it).
-template <int W, bool COUNT = false>
+// MOTION = false: the kernel variant for scenes without moving instances, which carries none of
+// their code (and so none of its registers).
+template <int W, bool COUNT = false, bool MOTION = false>
 __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls,
                                              uint2* gs, TravCount& cnt, float time = 0.0f) {
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     if (s.hasBvh) traverse<W, false, COUNT>(s, r, p, tmin, tmax, h, ls, gs, cnt);
-    if (s.numInst) traverseInstances<W, false, COUNT>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
+    if (MOTION && s.numInst) traverseInstances<W, false, COUNT>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
     for (uint32_t q = 0; q < s.numQuads; ++q) {
         float t;
         if (intersectQuad(r, s.quads[q], tmin, h.t, t)) h = HitRec{t, kHitQuad | q, kHitQuad | q, 0.0f, 0.0f};
@@ -483,15 +485,15 @@ __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, fl
 }
 
 // rtcOccluded1 with mask gsm (Embree.cpp:357-391): light quads never occlude.
-template <int W>
+template <int W, bool MOTION = false>
 __device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls, uint2* gs,
                                          float time = 0.0f) {
-    if (!s.hasBvh && !s.numInst) return false;
+    if (!s.hasBvh && !(MOTION && s.numInst)) return false;
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     TravCount cnt;
     if (s.hasBvh && traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, cnt)) return true;
-    return s.numInst && traverseInstances<W, true>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
+    return MOTION && s.numInst && traverseInstances<W, true>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -515,19 +517,21 @@ __device__ __forceinline__ Frame shadingFrame(V3 n, bool d3) {
     return f;
 }
 // A moving instance's triangle in world space at `time` (static triangles are stored there).
+template <bool MOTION>
 __device__ __forceinline__ void worldTriangle(const DevInstance* inst, float4 a, float4 b, float4 c, float time, V3& v0,
                                               V3& v1, V3& v2) {
     v0 = xyz(a), v1 = xyz(b), v2 = xyz(c);
     const uint32_t k = __float_as_uint(b.w);
-    if (k) {
+    if (MOTION && k) {
         const Affine m = instanceAt(inst[k - 1], time);
         v0 = affPoint(m, v0), v1 = affPoint(m, v1), v2 = affPoint(m, v2);
     }
 }
+template <bool MOTION = false>
 __device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir, bool* front = nullptr,
                                         float time = 0.0f) {
     V3 v0, v1, v2;
-    worldTriangle(s.inst, s.tris[3 * pos], s.tris[3 * pos + 1], s.tris[3 * pos + 2], time, v0, v1, v2);
+    worldTriangle<MOTION>(s.inst, s.tris[3 * pos], s.tris[3 * pos + 1], s.tris[3 * pos + 2], time, v0, v1, v2);
     const V3 ng = normalize(cross(v1 - v0, v2 - v0));
     const bool f = dot(dir, ng) < 0.0f;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
     if (front) *front = f;

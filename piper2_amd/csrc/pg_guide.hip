@@ -17,12 +17,13 @@ __device__ __forceinline__ GuideInput noInput() {
 }
 // Inputs at a hit: position in the scene box mapped to [0,1]^3, the geometric normal facing the
 // ray (as triNormal) and the outgoing direction.
+template <bool MOTION>
 __device__ __forceinline__ GuideInput hitInput(float4 oo, float4 dd, float t, float4 a, float4 b, float4 c, float3 lo,
                                                float3 invExt, const DevInstance* inst) {
     const V3 dir{dd.x, dd.y, dd.z};
     const V3 p = V3{oo.x, oo.y, oo.z} + dir * t;
     V3 v0, v1, v2;  // a moving instance's triangle at the path's time (d.w)
-    worldTriangle(inst, a, b, c, dd.w, v0, v1, v2);
+    worldTriangle<MOTION>(inst, a, b, c, dd.w, v0, v1, v2);
     const V3 ng = normalize(cross(v1 - v0, v2 - v0));
     const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;
     const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
@@ -54,7 +55,8 @@ struct ArraySource {
     }
 };
 // From the path state of an extension-queue entry (hit point, shading normal, -ray direction):
-// queue entry -> path record -> hit triangle.
+// queue entry -> path record -> hit triangle. MOTION: the scene has moving instances.
+template <bool MOTION>
 struct PathSource {
     const float4* tris;  // leaf-order triangles of the scene (only the hit triangle's normal is needed)
     const PathRec* paths;
@@ -83,7 +85,7 @@ struct PathSource {
     __device__ __forceinline__ void stage4(Pend&) const {}
     __device__ __forceinline__ GuideInput finish(const Pend& q) const {
         if (!q.tri) return noInput();
-        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, lo, invExt, inst);
+        return hitInput<MOTION>(q.o, q.d, q.h.x, q.a, q.b, q.c, lo, invExt, inst);
     }
 };
 
@@ -211,6 +213,7 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHo
 
 // ---- guided batches: inference grouped by guide --------------------------------------------------
 // Queue entries of a guided batch: path slot -> item -> its scene's triangles and box.
+template <bool MOTION>
 struct BatchPathSource {
     const DevScene* scenes;
     const ItemGuide* guide;
@@ -248,7 +251,7 @@ struct BatchPathSource {
     }
     __device__ __forceinline__ GuideInput finish(const Pend& q) const {
         if (!q.tri) return noInput();
-        return hitInput(q.o, q.d, q.h.x, q.a, q.b, q.c, q.lo, q.invExt, q.inst);
+        return hitInput<MOTION>(q.o, q.d, q.h.x, q.a, q.b, q.c, q.lo, q.invExt, q.inst);
     }
 };
 
@@ -259,7 +262,7 @@ __device__ __forceinline__ void sortShare(uint32_t n, uint32_t& lo, uint32_t& hi
     hi = min(n, lo + chunk);
 }
 
-__global__ void __launch_bounds__(256) kGroupCount(BatchPathSource src, const uint32_t* count, uint32_t nKeys,
+__global__ void __launch_bounds__(256) kGroupCount(BatchPathSource<false> src, const uint32_t* count, uint32_t nKeys,
                                                    GroupBuffers gb) {
     __shared__ uint32_t hist[kMaxBatchGuides];
     for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) hist[k] = 0;
@@ -318,7 +321,7 @@ __global__ void __launch_bounds__(kSortBlocks) kGroupScan(uint32_t nKeys, GroupB
 
 // The same shares as kGroupCount; inside a block the entries of one key take consecutive slots
 // from the block's base (LDS atomics: the order inside a group does not matter).
-__global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource src, const uint32_t* count, uint32_t nKeys,
+__global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource<false> src, const uint32_t* count, uint32_t nKeys,
                                                      GroupBuffers gb) {
     __shared__ uint32_t next[kMaxBatchGuides];
     for (uint32_t k = threadIdx.x; k < nKeys; k += blockDim.x) next[k] = gb.blockBase[k * kSortBlocks + blockIdx.x];
@@ -331,7 +334,8 @@ __global__ void __launch_bounds__(256) kGroupScatter(BatchPathSource src, const 
 // Each block takes a contiguous range of units (kGroupUnit entries of one guide each) and reloads
 // the weights only when the guide changes; the order of entries inside a group does not matter
 // (every mixture depends on its own entry only).
-__global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSource src, uint32_t nKeys, GroupBuffers gb,
+template <bool MOTION>
+__global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSource<MOTION> src, uint32_t nKeys, GroupBuffers gb,
                                                                   const uint16_t* const* __restrict__ keyWb,
                                                                   const float* const* __restrict__ keyTheta,
                                                                   float* __restrict__ mix) {
@@ -354,7 +358,7 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSourc
     };
     uint32_t loaded = ~0u;
     uint32_t key = u0 < u1 ? keyOf(u0) : 0, entry = u0 < u1 ? entryOf(u0, key) : ~0u;
-    BatchPathSource::Pend p = src.begin(entry);
+    typename BatchPathSource<MOTION>::Pend p = src.begin(entry);
     src.stage2(p), src.stage3(p), src.stage4(p);
     GuideInput in = src.finish(p);
     for (uint32_t u = u0; u < u1; ++u) {  // block-uniform
@@ -605,9 +609,16 @@ void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const ui
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
                            const float* theta, float* mix, uint32_t maxBlocks) {
-    PathSource src{s.tris, ps.p, queue, lo, invExt, s.inst};
-    hipLaunchKernelGGL(kGuideInfer<PathSource>, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds, st, src,
-                       capacity, count, wb, theta, mix);
+    const dim3 grid(gridFor(capacity, kInferBlock, maxBlocks));
+    if (s.numInst) {
+        const PathSource<true> src{s.tris, ps.p, queue, lo, invExt, s.inst};
+        hipLaunchKernelGGL(kGuideInfer<PathSource<true>>, grid, dim3(kInferBlock), kInferLds, st, src, capacity, count,
+                           wb, theta, mix);
+    } else {
+        const PathSource<false> src{s.tris, ps.p, queue, lo, invExt, s.inst};
+        hipLaunchKernelGGL(kGuideInfer<PathSource<false>>, grid, dim3(kInferBlock), kInferLds, st, src, capacity, count,
+                           wb, theta, mix);
+    }
 }
 
 size_t groupBufferBytes(uint32_t capacity) {
@@ -627,12 +638,19 @@ void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathSta
                              const uint32_t* count, uint32_t capacity, uint32_t nKeys, const GroupBuffers& gb,
                              const uint16_t* const* keyWb, const float* const* keyTheta, float* mix,
                              uint32_t maxBlocks) {
-    const BatchPathSource src{bt.scenes, bt.guide, ps.p, queue};
+    const BatchPathSource<false> src{bt.scenes, bt.guide, ps.p, queue};
     hipLaunchKernelGGL(kGroupCount, dim3(kSortBlocks), dim3(256), 0, st, src, count, nKeys, gb);
     hipLaunchKernelGGL(kGroupScan, dim3(1), dim3(kSortBlocks), 0, st, nKeys, gb);
     hipLaunchKernelGGL(kGroupScatter, dim3(kSortBlocks), dim3(256), 0, st, src, count, nKeys, gb);
-    hipLaunchKernelGGL(kGuideInferGrouped, dim3(gridFor(capacity, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds,
-                       st, src, nKeys, gb, keyWb, keyTheta, mix);
+    const dim3 grid(gridFor(capacity, kInferBlock, maxBlocks));
+    if (bt.motion) {
+        const BatchPathSource<true> msrc{bt.scenes, bt.guide, ps.p, queue};
+        hipLaunchKernelGGL(kGuideInferGrouped<true>, grid, dim3(kInferBlock), kInferLds, st, msrc, nKeys, gb, keyWb,
+                           keyTheta, mix);
+    } else {
+        hipLaunchKernelGGL(kGuideInferGrouped<false>, grid, dim3(kInferBlock), kInferLds, st, src, nKeys, gb, keyWb,
+                           keyTheta, mix);
+    }
 }
 
 GuideBatchLayout guideBatchLayout(uint32_t n) {
@@ -693,10 +711,11 @@ void guideConfigureLds() {
     if (done) return;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<ArraySource>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<PathSource>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInferGrouped),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
+    for (const void* k : {reinterpret_cast<const void*>(&kGuideInfer<PathSource<false>>),
+                          reinterpret_cast<const void*>(&kGuideInfer<PathSource<true>>),
+                          reinterpret_cast<const void*>(&kGuideInferGrouped<false>),
+                          reinterpret_cast<const void*>(&kGuideInferGrouped<true>)})
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideTrainFB), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(kTrainLds));
     done = true;

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "pg_device.hpp"
 #include "pg_guide.hpp"
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
 // ------------------------------------------------------------------------------------------
 // extend: closest hit for every queued path
 // ------------------------------------------------------------------------------------------
-template <int W, bool COUNT, bool BATCH>
+template <int W, bool COUNT, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const uint32_t* __restrict__ queue,
                                               const uint32_t* __restrict__ count, uint2* spill, uint32_t spillPerLane,
                                               unsigned long long* travCounts, BatchTable bt) {
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const u
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[path].meta.z)] : s0;
         const float4 o = ps.p[path].o, d = ps.p[path].d;
         const Ray r{xyz(o), xyz(d)};
-        const HitRec h = closestHit<W, COUNT>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt, d.w);
+        const HitRec h = closestHit<W, COUNT, MOTION>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt, d.w);
         ps.p[path].hit = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
     }
     if (COUNT) {
@@ -225,7 +226,7 @@ static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging s
 // Waves per SIMD: 4 (<= 128 VGPRs) for the plain and batched kernels: at 5 (<= 96 VGPRs) the
 // plastic / Oren-Nayar lobes spill 14 VGPRs and the stage takes 18.9 instead of 17.0 ms on the
 // 1080p64 terrain frame. The guided ones keep 3 for their extra state.
-template <bool BATCH, bool GUIDED, bool TEX>
+template <bool BATCH, bool GUIDED, bool TEX, bool MOTION>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
 kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
        GuideShade gs, BatchTable bt) {
@@ -252,7 +253,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                        d13 = rc.flags & kCompatD13;
             const bool envBlack = !valid(ld3(s.env));
             const float4 d4 = ps.p[path].d;
-            const float time = d4.w;
+            const float time = MOTION ? d4.w : 0.0f;  // the path's time matters to moving instances only
             Ray ray{xyz(ps.p[path].o), xyz(d4)};
             V3 thr = xyz(ps.p[path].thr);
             // fold in last bounce's MIS-BSDF light hit (after its NEE term, as the reference adds them);
@@ -282,7 +283,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 pushDone = true;
             } else {
                 bool front;
-                const V3 nrm = triNormal(s, pos, ray.d, &front, time);
+                const V3 nrm = triNormal<MOTION>(s, pos, ray.d, &front, time);
                 const Mat mat = loadMat<TEX>(s, s.triMat[pos], front, pos, hit4.z, hit4.w);
                 const Frame fr = shadingFrame(nrm, d3);
                 const V3 wo = fr.toLocal(-ray.d);
@@ -496,7 +497,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
 // ------------------------------------------------------------------------------------------
 // shadow: occlusion test of NEE rays (sampleLightSource, PathTracing/Kernel.cpp:40-62)
 // ------------------------------------------------------------------------------------------
-template <int W, bool BATCH>
+template <int W, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
                                               BatchTable bt) {
     __shared__ uint2 lstack[kLdsStack * 64];
@@ -509,7 +510,7 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path & ~kMisFlag].meta.z)] : s0;
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         const float time = s.numInst ? ps.p[r.path & ~kMisFlag].d.w : 0.0f;
-        if (!occluded<W>(s, ray, r.tmin, r.tmax, lstack + lane, gs, time)) {
+        if (!occluded<W, MOTION>(s, ray, r.tmin, r.tmax, lstack + lane, gs, time)) {
             if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
                 ps.p[r.path & ~kMisFlag].Lmis = make_float4(r.cr, r.cg, r.cb, 0.0f);
             } else {
@@ -525,7 +526,7 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
 // mis: the separate BSDF-sampled ray of sampleBSDF (PathTracing/Kernel.cpp:63-102); counts only
 // if it hits the selected light (or escapes to the environment)
 // ------------------------------------------------------------------------------------------
-template <int W, bool BATCH>
+template <int W, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
                                            BatchTable bt) {
     __shared__ uint2 lstack[kLdsStack * 64];
@@ -557,7 +558,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
                 }
                 // tq > 0: the next float up makes the occlusion range (tmin, tq] inclusive
                 const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                if (first && !occluded<W>(s, ray, 1e-3f, tqUp, lstack + lane, gs, time)) {
+                if (first && !occluded<W, MOTION>(s, ray, 1e-3f, tqUp, lstack + lane, gs, time)) {
                     const V3 qn = quadNormal(qd, ray.d);
                     rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
                     lpdf = tq * tq / (fabsf(dot(qn, ray.d)) * qd.area);
@@ -565,7 +566,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
             }
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc, time);
+            const HitRec h = closestHit<W, false, MOTION>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc, time);
             if (h.pos == kHitMiss) {
                 rad = ld3(s.env);
                 lpdf = envPdf(ray.d);
@@ -843,7 +844,7 @@ __global__ void __launch_bounds__(256) kCopy(const uint4* __restrict__ src, uint
 // ------------------------------------------------------------------------------------------
 // parity kernels
 // ------------------------------------------------------------------------------------------
-template <int W>
+template <int W, bool MOTION>
 __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint32_t n, float* t, uint32_t* prim,
                                              float* uv, uint8_t* occ, uint2* spill, uint32_t spillPerLane,
                                              const float* times) {
@@ -856,10 +857,10 @@ __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint
         const Ray ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
         const float time = times ? times[i] : 0.0f;
         if (occ) {
-            occ[i] = occluded<W>(s, ray, r[6], r[7], lstack + lane, gs, time) ? 1 : 0;
+            occ[i] = occluded<W, MOTION>(s, ray, r[6], r[7], lstack + lane, gs, time) ? 1 : 0;
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W>(s, ray, r[6], r[7], lstack + lane, gs, tc, time);
+            const HitRec h = closestHit<W, false, MOTION>(s, ray, r[6], r[7], lstack + lane, gs, tc, time);
             t[i] = h.t;
             prim[i] = h.prim;
             uv[2 * i] = h.u;
@@ -940,23 +941,19 @@ void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Q
             s, ps, q, rc, curSlot, BatchTable{});
 }
 
-template <int W>
-static void extendW(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
-                    const uint32_t* count, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane,
-                    unsigned long long* tc, const BatchTable* bt) {
-    const dim3 b(64);
-    if (bt) {
-        if (tc) hipLaunchKernelGGL((kExtend<W, true, true>), fit(kExtend<W, true, true>, gridLanes), b, 0, st, s, ps,
-                                   queue, count, spill, spillPerLane, tc, *bt);
-        else hipLaunchKernelGGL((kExtend<W, false, true>), fit(kExtend<W, false, true>, gridLanes), b, 0, st, s, ps,
-                                queue, count, spill, spillPerLane, tc, *bt);
-    } else {
-        if (tc) hipLaunchKernelGGL((kExtend<W, true, false>), fit(kExtend<W, true, false>, gridLanes), b, 0, st, s, ps,
-                                   queue, count, spill, spillPerLane, tc, BatchTable{});
-        else hipLaunchKernelGGL((kExtend<W, false, false>), fit(kExtend<W, false, false>, gridLanes), b, 0, st, s, ps,
-                                queue, count, spill, spillPerLane, tc, BatchTable{});
-    }
+// Runtime flags to template arguments: f(std::integral_constant<bool, b>).
+template <typename F>
+static void withFlag(bool b, F&& f) {
+    if (b) f(std::true_type{});
+    else f(std::false_type{});
 }
+template <typename F>
+static void withWidth(uint32_t w, F&& f) {
+    if (w == 8) f(std::integral_constant<int, 8>{});
+    else f(std::integral_constant<int, 4>{});
+}
+// Kernels with the moving-instance code run only for scenes (or batches) that have moving instances.
+static bool anyMotion(const DevScene& s, const BatchTable* bt) { return bt ? bt->motion != 0 : s.numInst != 0; }
 
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
@@ -964,65 +961,64 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
     auto* tc = reinterpret_cast<unsigned long long*>(travCounts);
     const uint32_t* queue = q.ext[curSlot];
     const uint32_t* count = q.counts + curSlot;
-    if (s.width == 8) extendW<8>(st, s, ps, queue, count, gridLanes, spill, spillPerLane, tc, bt);
-    else extendW<4>(st, s, ps, queue, count, gridLanes, spill, spillPerLane, tc, bt);
-}
-template <bool TEX>
-void shadeVariant(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
-                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
-                  const BatchTable* bt) {
-    if (bt && gs.enabled)  // guided batch: per-item guides in bt.guide
-        hipLaunchKernelGGL((kShade<true, true, TEX>), fit(kShade<true, true, TEX>, gridThreads, 256), dim3(256), 0, st,
-                s, ps, q, rc, curSlot, nextSlot, totals,
-                           gs, *bt);
-    else if (bt)
-        hipLaunchKernelGGL((kShade<true, false, TEX>), fit(kShade<true, false, TEX>, gridThreads, 256), dim3(256), 0, st,
-                s, ps, q, rc, curSlot, nextSlot, totals,
-                           gs, *bt);
-    else if (gs.enabled)
-        hipLaunchKernelGGL((kShade<false, true, TEX>), fit(kShade<false, true, TEX>, gridThreads, 256), dim3(256), 0, st,
-                s, ps, q, rc, curSlot, nextSlot, totals,
-                           gs, BatchTable{});
-    else
-        hipLaunchKernelGGL((kShade<false, false, TEX>), fit(kShade<false, false, TEX>, gridThreads, 256), dim3(256), 0, st,
-                s, ps, q, rc, curSlot, nextSlot, totals,
-                           gs, BatchTable{});
+    const BatchTable b = bt ? *bt : BatchTable{};
+    withWidth(s.width, [&](auto w) {
+        withFlag(tc != nullptr, [&](auto cnt) {
+            withFlag(bt != nullptr, [&](auto batch) {
+                withFlag(anyMotion(s, bt), [&](auto mot) {
+                    constexpr int W = decltype(w)::value;
+                    constexpr bool C = decltype(cnt)::value, B = decltype(batch)::value, M = decltype(mot)::value;
+                    hipLaunchKernelGGL((kExtend<W, C, B, M>), fit(kExtend<W, C, B, M>, gridLanes), dim3(64), 0, st, s,
+                                       ps, queue, count, spill, spillPerLane, tc, b);
+                });
+            });
+        });
+    });
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
                  const BatchTable* bt, bool textured) {
-    if (textured) shadeVariant<true>(st, s, ps, q, rc, curSlot, nextSlot, gridThreads, totals, gs, bt);
-    else shadeVariant<false>(st, s, ps, q, rc, curSlot, nextSlot, gridThreads, totals, gs, bt);
+    const BatchTable b = bt ? *bt : BatchTable{};
+    withFlag(bt != nullptr, [&](auto batch) {  // a guided batch reads its per-item guides in bt.guide
+        withFlag(gs.enabled != 0, [&](auto guided) {
+            withFlag(textured, [&](auto tex) {
+                withFlag(anyMotion(s, bt), [&](auto mot) {
+                    constexpr bool B = decltype(batch)::value, G = decltype(guided)::value, T = decltype(tex)::value,
+                                   M = decltype(mot)::value;
+                    hipLaunchKernelGGL((kShade<B, G, T, M>), fit(kShade<B, G, T, M>, gridThreads, 256), dim3(256), 0,
+                                       st, s, ps, q, rc, curSlot, nextSlot, totals, gs, b);
+                });
+            });
+        });
+    });
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                   uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
     const BatchTable b = bt ? *bt : BatchTable{};
-    if (s.width == 8) {
-        if (bt) hipLaunchKernelGGL((kShadow<8, true>), fit(kShadow<8, true>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kShadow<8, false>), fit(kShadow<8, false>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-    } else {
-        if (bt) hipLaunchKernelGGL((kShadow<4, true>), fit(kShadow<4, true>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kShadow<4, false>), fit(kShadow<4, false>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-    }
+    withWidth(s.width, [&](auto w) {
+        withFlag(bt != nullptr, [&](auto batch) {
+            withFlag(anyMotion(s, bt), [&](auto mot) {
+                constexpr int W = decltype(w)::value;
+                constexpr bool B = decltype(batch)::value, M = decltype(mot)::value;
+                hipLaunchKernelGGL((kShadow<W, B, M>), fit(kShadow<W, B, M>, gridLanes), dim3(64), 0, st, s, ps, q,
+                                   spill, spillPerLane, b);
+            });
+        });
+    });
 }
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
                uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
     const BatchTable b = bt ? *bt : BatchTable{};
-    if (s.width == 8) {
-        if (bt) hipLaunchKernelGGL((kMis<8, true>), fit(kMis<8, true>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kMis<8, false>), fit(kMis<8, false>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-    } else {
-        if (bt) hipLaunchKernelGGL((kMis<4, true>), fit(kMis<4, true>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-        else hipLaunchKernelGGL((kMis<4, false>), fit(kMis<4, false>, gridLanes), dim3(64), 0, st,
-                s, ps, q, spill, spillPerLane, b);
-    }
+    withWidth(s.width, [&](auto w) {
+        withFlag(bt != nullptr, [&](auto batch) {
+            withFlag(anyMotion(s, bt), [&](auto mot) {
+                constexpr int W = decltype(w)::value;
+                constexpr bool B = decltype(batch)::value, M = decltype(mot)::value;
+                hipLaunchKernelGGL((kMis<W, B, M>), fit(kMis<W, B, M>, gridLanes), dim3(64), 0, st, s, ps, q, spill,
+                                   spillPerLane, b);
+            });
+        });
+    });
 }
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
                   const RecordRing& ring, const GuideShade& gs, const GuideRecordOut& tr, const BatchTable* bt) {
@@ -1067,12 +1063,14 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
 }
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
                  uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const float* times) {
-    if (s.width == 8)
-        hipLaunchKernelGGL(kTrace<8>, fit(kTrace<8>, gridLanes), dim3(64), 0, st,
-                s, rays, n, t, prim, uv, occ, spill, spillPerLane, times);
-    else
-        hipLaunchKernelGGL(kTrace<4>, fit(kTrace<4>, gridLanes), dim3(64), 0, st,
-                s, rays, n, t, prim, uv, occ, spill, spillPerLane, times);
+    withWidth(s.width, [&](auto w) {
+        withFlag(s.numInst != 0, [&](auto mot) {
+            constexpr int W = decltype(w)::value;
+            constexpr bool M = decltype(mot)::value;
+            hipLaunchKernelGGL((kTrace<W, M>), fit(kTrace<W, M>, gridLanes), dim3(64), 0, st, s, rays, n, t, prim, uv,
+                               occ, spill, spillPerLane, times);
+        });
+    });
 }
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film) {

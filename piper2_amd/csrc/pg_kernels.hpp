@@ -81,6 +81,7 @@ struct BatchTable {
     const uint64_t* pathOffsets;  // n + 1 entries
     uint32_t n;
     const ItemGuide* guide;       // per item, in guided batches (nullptr otherwise)
+    uint32_t motion;              // some scene has moving instances (selects the kernel variants)
 };
 
 // Film records live in a ring of `ring` slots: path number G writes slot G mod ring.
