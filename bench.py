@@ -266,30 +266,40 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
     return out
 
 
-def guided_batch(dev, n, bvh_width, scale=8, ref_spp=1 << 15, train_passes=32, train_spp=16, max_level=14,
-                 pass_spp=16):
-    """Hard-indirect batch: n seeded hard rooms (512x512 to 1920x1080, depth 12), every batched call
-    one persistent launch with one guide per room.
-    Throughput: one pass_spp pass of all rooms at their own resolutions, plain batch vs guided batch
-    (guides trained beforehand, frozen). Convergence: the same rooms at 1/scale resolution (same
-    geometry and camera); references of ref_spp samples (FP64 films) by the frozen guides;
-    progressive batched renders (one batch call per doubling level) by the plain tracer and by
-    fresh guides trained online; per room, the batch time at which its film first reaches 1% relMSE."""
+def guided_batch(dev, n, bvh_width, crop=64, ref_spp=1 << 16, train_scale=8, train_passes=32, train_spp=16,
+                 budget_s=12.0, max_level=16, pass_spp=16, which=("plain", "guided", "guided_shared", "guided_4x_training")):
+    """Hard-indirect batch at its stated size: n seeded hard rooms at their own resolutions (512x512
+    to 1920x1080, 499k triangles each, depth 12), each batched call one persistent launch.
+    Throughput: one pass_spp pass of all rooms, plain batch vs guided batch (trained guides, frozen).
+    Convergence: progressive renders of the whole images (one batch call per doubling level, all
+    samples accumulated) by each estimator; after every level the relMSE of a fixed crop x crop
+    window of each image (the window at the 90th percentile of window brightness in a 256-spp
+    preview at 1/train_scale resolution: lit, but not the light) against a ref_spp-sample FP64
+    reference of that window, rendered
+    by frozen guides trained beforehand on 1/train_scale-resolution renders of the same rooms
+    (disjoint samples; any guide leaves the estimator unbiased). Estimators: the plain tracer; fresh
+    per-room guides trained online during the render; one fresh guide shared by all rooms (the rooms
+    are seeded variants of one family, so it trains on all of them); per-room guides taking 4x the
+    Adam steps per call. Levels stop once every room is below 1% or after budget_s seconds of render
+    time (budget_s = 0: no limit). A room's time to 1% is measured (log-log between levels) or, if its
+    last level is above 1%, extrapolated as t x relMSE / 0.01 (relMSE falling as 1/time) and marked.
+    The error breakdown at the last level: median relMSE, and the share of squared error in the worst
+    1% of crop pixels."""
     import math
     import numpy as np
     import piper2_amd as pg
     from piper2_amd import scenes
     t0 = time.perf_counter()
     full = [scenes.hard_room_variant(k) for k in range(n)]
-    small = [scenes.hard_room_variant(k, scale=scale) for k in range(n)]
+    small = [scenes.hard_room_variant(k, scale=train_scale) for k in range(n)]
     objs = [pg.Scene(dev, sc, bvh_width=bvh_width) for sc in full]  # small differs only in resolution
-    films = [pg.Film(dev, sc["width"], sc["height"]) for sc in small]
     log(f"guided batch: {n} rooms, {sum(sc['indices'].shape[0] for sc in full)} triangles, built in "
         f"{time.perf_counter() - t0:.1f}s")
     R = 1 << 16
 
-    def params(scs, rng, flags=0):
-        return [pg.render_params(sc, spp=1 << 18, spp_range=rng, flags=flags) for sc in scs]
+    def params(scs, rng, flags=0, rects=None):
+        return [pg.render_params(sc, spp=1 << 18, spp_range=rng, flags=flags, rect=None if rects is None else rects[i])
+                for i, sc in enumerate(scs)]
 
     guides = [pg.Guide(dev, seed=100 + k) for k in range(n)]
     t0 = time.perf_counter()
@@ -302,51 +312,104 @@ def guided_batch(dev, n, bvh_width, scale=8, ref_spp=1 << 15, train_passes=32, t
         f.set_params(g.get_params())
         frozen.append(f)
         g.close()
-    for s0 in range(R, R + ref_spp, 4096):
-        st, _ = pg.render_batch(dev, objs, params(small, (s0, min(s0 + 4096, R + ref_spp)), pg.PG_FILM_FP64),
-                                films, guides=frozen)
-        log(f"guided batch: reference samples {s0 - R}..{min(s0 + 4096, R + ref_spp) - R} in {st['ms_total']:.0f} ms")
-    refs = [_img(f.download(fp64=True)) for f in films]
-    log(f"guided batch: guides trained and {ref_spp}-spp references in {time.perf_counter() - t0:.1f}s")
+    fsmall = [pg.Film(dev, sc["width"], sc["height"]) for sc in small]
+    pg.render_batch(dev, objs, params(small, (3 * R, 3 * R + 256)), fsmall, guides=frozen)
+    rects, b = [], crop // train_scale
+    for f, sc in zip(fsmall, full):
+        img = _img(f.download()).mean(-1)
+        h, w = img.shape
+        sums = img[:h - h % b, :w - w % b].reshape(h // b, b, w // b, b).mean((1, 3))
+        flat = sums.reshape(-1)
+        k = int(np.argsort(flat)[int(0.9 * (flat.size - 1))])
+        y, x = divmod(k, sums.shape[1])
+        rects.append((min(x * crop, sc["width"] - crop), min(y * crop, sc["height"] - crop), crop, crop))
+        f.close()
+    ffull = [pg.Film(dev, sc["width"], sc["height"]) for sc in full]
+    for s0 in range(R, R + ref_spp, 8192):
+        pg.render_batch(dev, objs, params(full, (s0, min(s0 + 8192, R + ref_spp)), pg.PG_FILM_FP64, rects), ffull,
+                        guides=frozen)
+    refs = [_img(f.download(fp64=True))[y:y + crop, x:x + crop] for f, (x, y, _, _) in zip(ffull, rects)]
+    log(f"guided batch: guides trained and {ref_spp}-spp {crop}x{crop} crop references in "
+        f"{time.perf_counter() - t0:.1f}s")
 
-    def progressive(gs):
-        for f in films:
+    def progressive(gs, label):
+        for f in ffull:
             f.clear()
-        t, prev, hit = 0.0, [None] * n, [None] * n
+        t, traces = 0.0, [[] for _ in range(n)]
         for k in range(max_level + 1):
             rng = (0, 1) if k == 0 else (1 << (k - 1), 1 << k)
-            st, _ = pg.render_batch(dev, objs, params(small, rng), films, guides=gs)
+            st, _ = pg.render_batch(dev, objs, params(full, rng), ffull, guides=gs)
             t += st["ms_total"] / 1e3
-            log(f"guided batch: {'plain' if gs is None else 'guided'} level {k} ({rng[1]} spp) at {t:.3f} s")
-            for i, (f, ref) in enumerate(zip(films, refs)):
-                e = _rel_mse(_img(f.download()), ref)
-                if hit[i] is None and e <= 0.01:
-                    if prev[i] is None:
-                        hit[i] = t
-                    else:
-                        (tp, ep) = prev[i]
-                        fr = (math.log(0.01) - math.log(ep)) / (math.log(e) - math.log(ep))
-                        hit[i] = math.exp(math.log(tp) + fr * (math.log(t) - math.log(tp)))
-                prev[i] = (t, e)
-            if all(h is not None for h in hit):
+            errs = []
+            for i, (f, ref, (x, y, _, _)) in enumerate(zip(ffull, refs, rects)):
+                img = _img(f.download())[y:y + crop, x:x + crop]
+                errs.append(_rel_mse(img, ref))
+                traces[i].append((rng[1], t, errs[-1], img))
+            log(f"guided batch: {label} level {k} ({rng[1]} spp) at {t:.3f} s, median crop relMSE "
+                f"{float(np.median(errs)):.4f}")
+            if max(errs) <= 0.01 or (budget_s and t >= budget_s):
                 break
-        return hit, t
+        hit, marked = [], []
+        for tr in traces:
+            h = None
+            for (s_a, t_a, e_a, _), (s_b, t_b, e_b, _) in zip([(0, 0.0, None, None)] + tr, tr):
+                if e_b <= 0.01:
+                    if e_a is None or e_a <= 0.01:
+                        h = t_b
+                    else:
+                        fr = (math.log(0.01) - math.log(e_a)) / (math.log(e_b) - math.log(e_a))
+                        h = math.exp(math.log(t_a) + fr * (math.log(t_b) - math.log(t_a)))
+                    break
+            if h is None:  # extrapolated: relMSE ~ 1 / time beyond the last level
+                h = tr[-1][1] * tr[-1][2] / 0.01
+            hit.append(h)
+            marked.append(tr[-1][2] > 0.01)
+        last = []
+        for tr, ref in zip(traces, refs):
+            img = tr[-1][3]
+            sq = ((img - ref) ** 2 / (ref * ref + 0.01)).sum(-1).reshape(-1)
+            top = np.sort(sq)[::-1][:max(1, sq.size // 100)]
+            last.append((tr[-1][2], float(top.sum() / max(sq.sum(), 1e-30))))
+        out = {"s_to_1pct": [round(v, 3) for v in hit], "extrapolated": marked, "seconds": round(t, 3),
+               "spp": traces[0][-1][0],
+               "median_relmse_last": round(float(np.median([e for e, _ in last])), 5),
+               "worst1pct_error_share_median": round(float(np.median([w for _, w in last])), 3),
+               "trace_median": [[tr0[0], round(tr0[1], 3), round(float(np.median([tr[j][2] for tr in traces])), 5)]
+                                for j, tr0 in enumerate(traces[0])]}
+        return out, traces
 
-    t_plain, clock_plain = progressive(None)
-    online = [pg.Guide(dev, seed=200 + k) for k in range(n)]
-    t_guided, clock_guided = progressive(online)
-    log(f"guided batch: time to 1% per room, plain {t_plain}; guided {t_guided}")
-    both = [(a, b) for a, b in zip(t_plain, t_guided) if a is not None and b is not None]
+    results, all_traces = {}, {}
+    estimators = [("plain", lambda: None),
+                  ("guided", lambda: [pg.Guide(dev, seed=200 + k) for k in range(n)]),
+                  ("guided_shared", lambda: [pg.Guide(dev, seed=300)] * n),
+                  ("guided_4x_training", lambda: [pg.Guide(dev, seed=400 + k, train_records=1 << 20) for k in range(n)])]
+    for name, make in estimators:
+        if name not in which:
+            continue
+        gs = make()
+        results[name], all_traces[name] = progressive(gs, name)
+        for g in set(gs or []):
+            g.close()
+    plain = results["plain"]["s_to_1pct"] if "plain" in results else None
+    for name in results:
+        if name != "plain" and plain is not None:
+            sp = [a / b for a, b in zip(plain, results[name]["s_to_1pct"])]
+            results[name]["speedup_per_room"] = [round(v, 3) for v in sp]
+            results[name]["median_speedup"] = round(float(np.median(sp)), 3)
+            # relMSE at equal render time: the plain trace's last level against this estimator's trace
+            eq = []
+            for tp, tg in zip(all_traces["plain"], all_traces[name]):
+                a = equal_time([(s_, t_, e_) for s_, t_, e_, _ in tp], [(s_, t_, e_) for s_, t_, e_, _ in tg])
+                if a:
+                    eq.append(a[-1][1] / a[-1][2])
+            if eq:
+                results[name]["equal_time_plain_over_guided_median"] = round(float(np.median(eq)), 3)
     out = {"rooms": n, "resolutions": [[sc["width"], sc["height"]] for sc in full], "depth": 12,
-           "convergence_scale": f"1/{scale} resolution", "reference": f"{ref_spp}-spp FP64, frozen trained guides",
-           "plain_s_to_1pct": [None if v is None else round(v, 4) for v in t_plain],
-           "guided_s_to_1pct": [None if v is None else round(v, 4) for v in t_guided],
-           "plain_all_rooms_s": round(max(v for v in t_plain if v is not None), 4) if all(v is not None for v in t_plain) else None,
-           "guided_all_rooms_s": round(max(v for v in t_guided if v is not None), 4) if all(v is not None for v in t_guided) else None}
-    if both:
-        out["median_speedup"] = round(float(np.median([a / b for a, b in both])), 3)
+           "crop": f"{crop}x{crop}, the window at the 90th percentile of brightness of a 1/{train_scale}-resolution preview",
+           "crops": [list(r[:2]) for r in rects],
+           "reference": f"{ref_spp}-spp FP64 of the crop, frozen guides trained at 1/{train_scale} resolution",
+           "budget_s": budget_s, "estimators": results}
     # throughput at the rooms' own resolutions
-    ffull = [pg.Film(dev, sc["width"], sc["height"]) for sc in full]
     tp = {}
     for name, gs in (("plain", None), ("guided", frozen)):
         st, lat = pg.render_batch(dev, objs, params(full, (0, pass_spp)), ffull, guides=gs)  # warm
@@ -356,8 +419,8 @@ def guided_batch(dev, n, bvh_width, scale=8, ref_spp=1 << 15, train_passes=32, t
                     "latency_ms_p99": round(float(np.percentile(lat, 99)), 2)}
     out["pass_spp"] = pass_spp
     out["throughput"] = tp
-    log(f"guided batch: {out}")
-    for o in ffull + films + objs + frozen + online:
+    log(f"guided batch: { {k: {kk: v[kk] for kk in ('median_speedup', 'seconds', 'spp', 'median_relmse_last') if kk in v} for k, v in results.items()} }")
+    for o in ffull + objs + frozen:
         o.close()
     return out
 
@@ -430,6 +493,11 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--guided-batch", type=int, default=16, help="hard rooms of the guided batch leg (0: skip it)")
+    ap.add_argument("--guided-batch-budget", type=float, default=12.0,
+                    help="render seconds per estimator in the guided batch convergence (0: until 1%%)")
+    ap.add_argument("--guided-batch-estimators", default="plain,guided,guided_shared,guided_4x_training",
+                    help="estimators of the guided batch convergence (comma-separated)")
+    ap.add_argument("--only-guided-batch", action="store_true", help="run the guided batch leg alone")
     ap.add_argument("--guided-profile", action="store_true", help="only the guided profiling workload (for rocprofv3)")
     ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
@@ -444,6 +512,12 @@ def main():
     import piper2_amd as pg
     from piper2_amd import scenes
 
+    if args.only_guided_batch:  # the hard-room batch alone (long convergence runs)
+        dev = pg.Device(0)
+        res = guided_batch(dev, args.guided_batch, args.bvh_width or 4, budget_s=args.guided_batch_budget,
+                           which=tuple(args.guided_batch_estimators.split(",")))
+        print(json.dumps({"guided_batch_hard_rooms": res}), flush=True)
+        return
     t0 = time.perf_counter()
     sc = scenes.terrain_spheres(spp=args.spp)
     log(f"scene generated in {time.perf_counter() - t0:.1f}s: {sc['indices'].shape[0]} triangles")
@@ -570,7 +644,9 @@ def main():
                                                              "terrain slit", max_level=9),
         }
     if args.guided_batch > 0:
-        result["guided_batch_hard_rooms"] = guided_batch(dev, args.guided_batch, info["width"])
+        result["guided_batch_hard_rooms"] = guided_batch(dev, args.guided_batch, info["width"],
+                                                         budget_s=args.guided_batch_budget,
+                                                         which=tuple(args.guided_batch_estimators.split(",")))
     if not args.no_cpu:
         threads = min(_cpu_host()[0], 16)
         c1_scene = pg.Scene(dev, c1, bvh_width=info["width"])
