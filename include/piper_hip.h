@@ -245,10 +245,28 @@ int pg_render_paths(pg_scene* scene, const pg_render_params* params, pg_film* fi
 int pg_render_batch(pg_device* dev, pg_scene* const* scenes, const pg_render_params* params, uint32_t n,
                     pg_film* const* films, double* latency_ms, pg_stats* stats);
 
+/* Launch settings of a scene's renders (no choice changes any result): the traversal stack
+ * entries each lane keeps in LDS (4..16, default 10; deeper entries spill to device memory), the
+ * traversal waves per CU at most (0, the default: as many as the device keeps resident), the shade
+ * block (128 or 256 threads, default 256) and the waves per guide-inference block (4, 8 or 16,
+ * default 8; guided batches always use 8). pg_autotune sets them; pg_scene_set_launch sets them
+ * directly. */
+typedef struct pg_launch_config {
+    uint32_t lds_stack;
+    uint32_t trav_waves;
+    uint32_t shade_block;
+    uint32_t infer_waves;
+} pg_launch_config;
+int pg_scene_set_launch(pg_scene* scene, const pg_launch_config* config);
+int pg_scene_get_launch(pg_scene* scene, pg_launch_config* config);
+
 /* Autotuner (no reference counterpart: the reference has no tunable launch shape). Times a slice
  * of `params` (the whole rect, the first samples of its range, about sample_paths paths) at BVH
- * widths 4 and 8 and path-pool capacities 2^21..2^24, leaves the scene built at the fastest width
- * and returns the fastest capacity; pass it as params->wavefront_paths. The films are untouched. */
+ * widths 4 and 8 and path-pool capacities 2^21..2^24, then, one after another, the launch settings:
+ * traversal LDS stack depth {6, 8, 10, 12, 16} x traversal waves per CU {resident, 16, 20, 24}, the
+ * shade block {128, 256} and the waves per inference block {4, 8, 16} (on guided renders with a
+ * fresh guide). Leaves the scene built at the fastest width with the fastest launch settings and
+ * returns the fastest capacity; pass it as params->wavefront_paths. The films are untouched. */
 typedef struct pg_tune_result {
     uint32_t bvh_width;        /* the scene's BVH width from now on */
     uint32_t wavefront_paths;  /* fastest path-pool capacity */
@@ -256,6 +274,7 @@ typedef struct pg_tune_result {
     uint32_t trials;
     uint32_t reserved;
     double mpaths_per_s;       /* throughput of the best trial */
+    pg_launch_config launch;   /* the launch settings the scene now has */
 } pg_tune_result;
 int pg_autotune(pg_scene* scene, const pg_render_params* params, uint64_t sample_paths, pg_tune_result* out);
 

@@ -111,13 +111,24 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class LaunchConfig(ctypes.Structure):
+    """pg_launch_config: launch settings of a scene's renders."""
+    _fields_ = [("lds_stack", ctypes.c_uint32), ("trav_waves", ctypes.c_uint32), ("shade_block", ctypes.c_uint32),
+                ("infer_waves", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("bvh_width", ctypes.c_uint32), ("wavefront_paths", ctypes.c_uint32),
                 ("sample_paths", ctypes.c_uint64), ("trials", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("mpaths_per_s", ctypes.c_double)]
+                ("mpaths_per_s", ctypes.c_double), ("launch", LaunchConfig)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("reserved", "launch")}
+        d["launch"] = self.launch.as_dict()
+        return d
 
 
 MAX_MOTION_KEYS = 8
@@ -160,6 +171,8 @@ SIGNATURES = {
     "pg_scene_build_bvh": (ctypes.c_int, [_VP, ctypes.POINTER(BvhParams)]),
     "pg_scene_create_instanced": (ctypes.c_int, [_VP, ctypes.POINTER(SceneDesc), _VP, _U32, ctypes.POINTER(_VP)]),
     "pg_scene_set_motion": (ctypes.c_int, [_VP, _U32, ctypes.POINTER(Motion)]),
+    "pg_scene_set_launch": (ctypes.c_int, [_VP, ctypes.POINTER(LaunchConfig)]),
+    "pg_scene_get_launch": (ctypes.c_int, [_VP, ctypes.POINTER(LaunchConfig)]),
     "pg_scene_destroy": (None, [_VP]),
     "pg_scene_bvh_info": (ctypes.c_int, [_VP, _VP]),
     "pg_scene_bvh_download": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]),
@@ -291,6 +304,16 @@ class Scene:
             self.build_bvh(bvh_width, leaf_size, bins, builder)
         for k, m in enumerate(sc.get("motions") or []):
             self.set_motion(k, **m)
+
+    def set_launch(self, lds_stack=10, trav_waves=0, shade_block=256, infer_waves=8):
+        """Launch settings of this scene's renders (pg_scene_set_launch)."""
+        c = LaunchConfig(lds_stack, trav_waves, shade_block, infer_waves)
+        check(lib().pg_scene_set_launch(self.h, ctypes.byref(c)))
+
+    def get_launch(self):
+        c = LaunchConfig()
+        check(lib().pg_scene_get_launch(self.h, ctypes.byref(c)))
+        return c.as_dict()
 
     def set_motion(self, instance, keys, t0=0.0, t1=1.0, visible=True, parent=None):
         """Keys of moving instance `instance` (pg_scene_set_motion)."""

@@ -190,7 +190,9 @@ struct pg_scene {
     uint32_t builder = PG_BVH_HOST;
     pg::DevScene ds{};
     DevBuf tris, triMatLeaf, nodes, mats, quads, points, triUV, textures, texels, matTex;
-    uint32_t spillPerLane = 1;
+    pg::LaunchTune tune;            // launch settings of this scene's renders (pg_autotune)
+    uint32_t stackNeed = 1;         // deepest traversal stack a lane can need
+    uint32_t spillPerLane = 1;      // of them beyond the LDS part (tune.ldsStack)
     float lo[3] = {0, 0, 0}, invExt[3] = {1, 1, 1};  // scene box (triangles and light quads) -> [0,1]^3
 };
 
@@ -567,8 +569,23 @@ void buildBvh(pg_scene* s, const pg_bvh_params* params) {
     s->ds.hasBvh = nt > 0 ? 1u : 0u;
     s->ds.inst = s->inst.empty() ? nullptr : s->instBuf.as<pg::DevInstance>();
     s->ds.numInst = static_cast<uint32_t>(s->inst.size());
-    const int need = static_cast<int>((s->bvh.width - 1) * s->bvh.maxDepth + 2) - pg::kLdsStack;
-    s->spillPerLane = static_cast<uint32_t>(std::max(need, 1));
+    s->stackNeed = (s->bvh.width - 1) * s->bvh.maxDepth + 2;
+    s->spillPerLane = std::max<uint32_t>(s->stackNeed > s->tune.ldsStack ? s->stackNeed - s->tune.ldsStack : 0u, 1u);
+}
+
+pg::LaunchTune tuneFrom(const pg_launch_config& c) {
+    require(c.lds_stack >= 4 && c.lds_stack <= pg::kLdsStackMax, "lds_stack must be 4..16");
+    require(c.trav_waves <= 64, "trav_waves must be 0..64");
+    require(c.shade_block == 128 || c.shade_block == 256, "shade_block must be 128 or 256");
+    require(c.infer_waves == 4 || c.infer_waves == 8 || c.infer_waves == 16, "infer_waves must be 4, 8 or 16");
+    return pg::LaunchTune{c.lds_stack, c.trav_waves, c.shade_block, c.infer_waves};
+}
+pg_launch_config configOf(const pg::LaunchTune& t) {
+    return pg_launch_config{t.ldsStack, t.travWaves, t.shadeBlock, t.inferWaves};
+}
+void setTune(pg_scene* s, const pg::LaunchTune& t) {
+    s->tune = t;
+    s->spillPerLane = std::max<uint32_t>(s->stackNeed > t.ldsStack ? s->stackNeed - t.ldsStack : 0u, 1u);
 }
 
 uint32_t log2Ceil(uint32_t v) {
@@ -753,8 +770,10 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
                                                     (it.rc.filter == pg::kFilterBox ? 16 : 36) *
                                                     ((it.rc.flags & pg::kFilmFp64) ? 8 : 4));
     }
+    const pg::LaunchTune lt = scenes[0]->tune;  // a batch launches with its first scene's settings
     for (uint32_t i = 0; i < nScenes; ++i) {
-        spillPerLane = std::max(spillPerLane, scenes[i]->spillPerLane);
+        const uint32_t need = scenes[i]->stackNeed;
+        spillPerLane = std::max(spillPerLane, need > lt.ldsStack ? need - lt.ldsStack : 1u);
         const float* env = scenes[i]->ds.env;
         envBlack = envBlack && !(env[0] > 0.0f || env[1] > 0.0f || env[2] > 0.0f);
     }
@@ -938,8 +957,8 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
             pg::launchIterBegin(st, q, cur);
             pg::launchRegen(st, ds0, ps, q, rc0, cur, threads, btp);
         });
-        timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, trav, btp); });
-        if (!keys.empty())  // 2 inference blocks of 8 waves per CU fit (LDS)
+        timeStage(kExtend, [&] { pg::launchExtend(st, ds0, ps, q, cur, lanes, spill, spillPerLane, trav, btp, lt); });
+        if (!keys.empty())  // grouped: 2 inference blocks of 8 waves per CU fit (LDS)
             timeStage(kGuide, [&] {
                 const uint32_t blocks = static_cast<uint32_t>(dev->numCUs) * 2;
                 if (opt.batch)
@@ -949,13 +968,13 @@ void runPaths(pg_device* dev, pg_scene* const* scenes, uint32_t nScenes, const s
                 else
                     pg::launchGuideInferPaths(st, ds0, ps, q.ext[cur], q.counts + cur, cap, gs.lo, gs.invExt,
                                               keys[0]->wb.as<uint16_t>(), keys[0]->theta.as<float>(),
-                                              dev->mix.as<float>(), blocks);
+                                              dev->mix.as<float>(), static_cast<uint32_t>(dev->numCUs), lt.inferWaves);
             });
         timeStage(kShade, [&] {
-            pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp, anyTextured);
+            pg::launchShade(st, ds0, ps, q, rc0, cur, nxt, threads, dev->totals.as<uint64_t>(), gs, btp, anyTextured, lt);
         });
-        timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
-        if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, btp); });
+        timeStage(kShadow, [&] { pg::launchShadow(st, ds0, ps, q, lanes, spill, spillPerLane, btp, lt); });
+        if (!envBlack) timeStage(kMis, [&] { pg::launchMis(st, ds0, ps, q, lanes, spill, spillPerLane, btp, lt); });
         timeStage(kFilm, [&] { pg::launchRecord(st, ps, q, rc0, threads, ring, gs, tr, btp); });
     };
     // Iterations run in groups of kCheckEvery (even, so every group starts on extension queue 0).
@@ -1179,11 +1198,61 @@ void autotune(pg_scene* s, const pg_render_params* params, uint64_t samplePaths,
         pg_bvh_params bp{bestWidth, s->bvhParams.leafSize, s->bvhParams.bins, s->builder};
         buildBvh(s, &bp);
     }
+    p.wavefront_paths = bestCap;
+    // Launch settings, one after another (coordinate descent) at the chosen width and capacity:
+    // traversal LDS stack depth x traversal waves per CU, then the shade block, then the waves per
+    // guide-inference block (timed on guided renders with a fresh, untrained guide: the network's
+    // cost does not depend on its weights). The best of two runs per trial.
+    auto timeTrial = [&](const pg::LaunchTune& t, pg_guide* g) {
+        setTune(s, t);
+        double best = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            render(s, &p, nullptr, nullptr, nullptr, g);
+            best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        ++trials;
+        return best;
+    };
+    pg::LaunchTune tune = s->tune;
+    double tuneMs = timeTrial(tune, nullptr);
+    for (uint32_t depth : {6u, 8u, 10u, 12u, 16u})
+        for (uint32_t waves : {0u, 16u, 20u, 24u}) {
+            pg::LaunchTune t = tune;
+            t.ldsStack = depth, t.travWaves = waves;
+            if (t.ldsStack == tune.ldsStack && t.travWaves == tune.travWaves) continue;
+            const double ms = timeTrial(t, nullptr);
+            if (ms < tuneMs) tuneMs = ms, tune = t;
+        }
+    for (uint32_t blk : {128u, 256u}) {
+        pg::LaunchTune t = tune;
+        t.shadeBlock = blk;
+        if (blk == tune.shadeBlock) continue;
+        const double ms = timeTrial(t, nullptr);
+        if (ms < tuneMs) tuneMs = ms, tune = t;
+    }
+    bestMs = std::min(bestMs, tuneMs);
+    pg_guide_params gp{7u, 1e-2f, 0.9f, 0.99f, 1e-8f, 0.5f, 1u << 14, 0u};
+    pg_guide* g = nullptr;
+    if (pg_guide_create(s->dev, &gp, &g) == PG_OK) {
+        double inferMs = 1e300;
+        uint32_t inferBest = tune.inferWaves;
+        for (uint32_t w : {4u, 8u, 16u}) {
+            pg::LaunchTune t = tune;
+            t.inferWaves = w;
+            const double ms = timeTrial(t, g);
+            if (ms < inferMs) inferMs = ms, inferBest = w;
+        }
+        tune.inferWaves = inferBest;
+        pg_guide_destroy(g);
+    }
+    setTune(s, tune);
     out->bvh_width = bestWidth;
     out->wavefront_paths = bestCap;
     out->sample_paths = paths;
     out->trials = trials;
     out->mpaths_per_s = static_cast<double>(paths) / bestMs / 1e3;
+    out->launch = configOf(tune);
 }
 
 }  // namespace
@@ -1486,6 +1555,22 @@ int pg_scene_set_motion(pg_scene* s, uint32_t instance, const pg_motion* m) {
     });
 }
 
+int pg_scene_set_launch(pg_scene* s, const pg_launch_config* c) {
+    return guarded([&] {
+        require(s && c, "null argument");
+        const pg::LaunchTune t = tuneFrom(*c);
+        std::lock_guard<std::mutex> lock(s->dev->mu);
+        setTune(s, t);
+    });
+}
+
+int pg_scene_get_launch(pg_scene* s, pg_launch_config* c) {
+    return guarded([&] {
+        require(s && c, "null argument");
+        *c = configOf(s->tune);
+    });
+}
+
 int pg_scene_build_bvh(pg_scene* s, const pg_bvh_params* params) {
     return guarded([&] {
         require(s && params, "null argument");
@@ -1638,7 +1723,7 @@ int pg_trace_at(pg_scene* s, const float* rays, const float* times, uint32_t n, 
         const uint32_t lanes = traversalLanes(dev, n);
         spill.reserve(static_cast<size_t>(lanes) * s->spillPerLane * 8);
         pg::launchTrace(st, s->ds, dr.as<float>(), n, dt.as<float>(), dp.as<uint32_t>(), duv.as<float>(), nullptr, lanes,
-                        spill.as<uint2>(), s->spillPerLane, times ? dtimes.as<float>() : nullptr);
+                        spill.as<uint2>(), s->spillPerLane, s->tune, times ? dtimes.as<float>() : nullptr);
         PG_HIP(hipGetLastError());
         PG_HIP(hipStreamSynchronize(st));
         PG_HIP(hipMemcpy(t, dt.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
@@ -1664,7 +1749,7 @@ int pg_trace_noalloc(pg_scene* s, const float* rays, uint32_t n, float* t, uint3
         const uint32_t lanes = static_cast<uint32_t>(std::min<size_t>(traversalLanes(dev, n), fitLanes));
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : dev->stream;
         pg::launchTrace(st, s->ds, rays, n, t, prim, uv, nullptr, lanes, static_cast<uint2*>(dev->scratch),
-                        s->spillPerLane);
+                        s->spillPerLane, s->tune);
         PG_HIP(hipGetLastError());
         if (!stream) PG_HIP(hipStreamSynchronize(st));
     });
@@ -1684,7 +1769,7 @@ int pg_occluded(pg_scene* s, const float* rays, uint32_t n, uint8_t* occ) {
         const uint32_t lanes = traversalLanes(dev, n);
         spill.reserve(static_cast<size_t>(lanes) * s->spillPerLane * 8);
         pg::launchTrace(st, s->ds, dr.as<float>(), n, nullptr, nullptr, nullptr, docc.as<uint8_t>(), lanes,
-                        spill.as<uint2>(), s->spillPerLane);
+                        spill.as<uint2>(), s->spillPerLane, s->tune);
         PG_HIP(hipGetLastError());
         PG_HIP(hipStreamSynchronize(st));
         PG_HIP(hipMemcpy(occ, docc.p, n, hipMemcpyDeviceToHost));

@@ -236,8 +236,9 @@ __device__ __forceinline__ bool intersectQuad(const Ray& r, const DevQuad& q, fl
 
 // ------------------------------------------------------------------------------------------
 // Wide-BVH traversal, one ray per lane. Each lane keeps a stack of (child reference, entry
-// distance) pairs: the first kLdsStack entries live in the wave's LDS slice (lane-minor, so the
-// 64 lanes of a wave hit distinct banks), deeper entries spill to a per-lane HBM region.
+// distance) pairs: the first entries (a launch setting, kLdsStack by default) live in the wave's
+// LDS slice (lane-minor, so the 64 lanes of a wave hit distinct banks), deeper entries spill to a
+// per-lane HBM region.
 // Closest hit: nearest child first, the rest pushed and culled on pop against the best t.
 // ------------------------------------------------------------------------------------------
 
@@ -249,15 +250,22 @@ struct HitRec {
     float u, v;
 };
 
-__device__ __forceinline__ void stackPush(uint2* ls, uint2* gs, int& sp, uint32_t ref, float t) {
+// A lane's traversal stack: `depth` entries in the wave's LDS slice (entry k at ls[64 k]), the
+// deeper ones in the lane's spill region gs.
+struct TravStack {
+    uint2* ls;
+    uint2* gs;
+    int depth;
+};
+__device__ __forceinline__ void stackPush(const TravStack& st, int& sp, uint32_t ref, float t) {
     const uint2 e = make_uint2(ref, __float_as_uint(t));
-    if (sp < kLdsStack) ls[sp * 64] = e;
-    else gs[sp - kLdsStack] = e;
+    if (sp < st.depth) st.ls[sp * 64] = e;
+    else st.gs[sp - st.depth] = e;
     ++sp;
 }
-__device__ __forceinline__ uint2 stackPop(const uint2* ls, const uint2* gs, int& sp) {
+__device__ __forceinline__ uint2 stackPop(const TravStack& st, int& sp) {
     --sp;
-    return sp < kLdsStack ? ls[sp * 64] : gs[sp - kLdsStack];
+    return sp < st.depth ? st.ls[sp * 64] : st.gs[sp - st.depth];
 }
 
 // ANY = false: closest hit of triangles in (tmin, h.t] (tie: smaller original prim).
@@ -269,7 +277,7 @@ struct TravCount {
 // COUNT = true also counts wide nodes visited and triangles tested (roofline bookkeeping).
 template <int W, bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const RayPre& p, float tmin, float tmax,
-                                         HitRec& h, uint2* ls, uint2* gs, TravCount& cnt, uint32_t root = 0) {
+                                         HitRec& h, const TravStack& stk, TravCount& cnt, uint32_t root = 0) {
     const WideNode<W>* __restrict__ nodes = static_cast<const WideNode<W>*>(s.nodes);
     const float4* __restrict__ tris = s.tris;
     int sp = 0;
@@ -302,7 +310,7 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
                     if (((mask >> c) & 1) && tn[c] < bt) bt = tn[c], best = c, bestRef = n.child[c];
 #pragma unroll
                 for (int c = 0; c < W; ++c)
-                    if (((mask >> c) & 1) && c != best) stackPush(ls, gs, sp, n.child[c], tn[c]);
+                    if (((mask >> c) & 1) && c != best) stackPush(stk, sp, n.child[c], tn[c]);
                 ref = bestRef;
                 continue;
             }
@@ -325,7 +333,7 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
         // pop the next entry that can still hold a closer hit
         bool found = false;
         while (sp > 0) {
-            const uint2 e = stackPop(ls, gs, sp);
+            const uint2 e = stackPop(stk, sp);
             if (ANY || __uint_as_float(e.y) <= h.t * 1.00001f) {
                 ref = e.x;
                 found = true;
@@ -452,7 +460,7 @@ __device__ __forceinline__ bool boxHit(const float* lo, const float* hi, const R
 // normalised) and through the instance's BLAS.
 template <int W, bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool traverseInstances(const DevScene& s, const Ray& r, const RayPre& p, float time,
-                                                  float tmin, float tmax, HitRec& h, uint2* ls, uint2* gs,
+                                                  float tmin, float tmax, HitRec& h, const TravStack& stk,
                                                   TravCount& cnt) {
     for (uint32_t k = 0; k < s.numInst; ++k) {
         const DevInstance& I = s.inst[k];
@@ -460,7 +468,7 @@ __device__ __forceinline__ bool traverseInstances(const DevScene& s, const Ray& 
         const Affine inv = affInverse(instanceAt(I, time));
         const Ray lr{affPoint(inv, r.o), affVector(inv, r.d)};
         const RayPre lp = precompute(lr);
-        if (traverse<W, ANY, COUNT>(s, lr, lp, tmin, tmax, h, ls, gs, cnt, I.root) && ANY) return true;
+        if (traverse<W, ANY, COUNT>(s, lr, lp, tmin, tmax, h, stk, cnt, I.root) && ANY) return true;
     }
     return false;
 }
@@ -470,12 +478,12 @@ __device__ __forceinline__ bool traverseInstances(const DevScene& s, const Ray& 
 // MOTION = false: the kernel variant for scenes without moving instances, which carries none of
 // their code (and so none of its registers).
 template <int W, bool COUNT = false, bool MOTION = false>
-__device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls,
-                                             uint2* gs, TravCount& cnt, float time = 0.0f) {
+__device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, float tmin, float tmax,
+                                             const TravStack& stk, TravCount& cnt, float time = 0.0f) {
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
-    if (s.hasBvh) traverse<W, false, COUNT>(s, r, p, tmin, tmax, h, ls, gs, cnt);
-    if (MOTION && s.numInst) traverseInstances<W, false, COUNT>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
+    if (s.hasBvh) traverse<W, false, COUNT>(s, r, p, tmin, tmax, h, stk, cnt);
+    if (MOTION && s.numInst) traverseInstances<W, false, COUNT>(s, r, p, time, tmin, tmax, h, stk, cnt);
     for (uint32_t q = 0; q < s.numQuads; ++q) {
         float t;
         if (intersectQuad(r, s.quads[q], tmin, h.t, t)) h = HitRec{t, kHitQuad | q, kHitQuad | q, 0.0f, 0.0f};
@@ -486,14 +494,14 @@ __device__ __forceinline__ HitRec closestHit(const DevScene& s, const Ray& r, fl
 
 // rtcOccluded1 with mask gsm (Embree.cpp:357-391): light quads never occlude.
 template <int W, bool MOTION = false>
-__device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, uint2* ls, uint2* gs,
+__device__ __forceinline__ bool occluded(const DevScene& s, const Ray& r, float tmin, float tmax, const TravStack& stk,
                                          float time = 0.0f) {
     if (!s.hasBvh && !(MOTION && s.numInst)) return false;
     HitRec h{tmax, kHitMiss, kHitMiss, 0.0f, 0.0f};
     const RayPre p = precompute(r);
     TravCount cnt;
-    if (s.hasBvh && traverse<W, true>(s, r, p, tmin, tmax, h, ls, gs, cnt)) return true;
-    return MOTION && s.numInst && traverseInstances<W, true>(s, r, p, time, tmin, tmax, h, ls, gs, cnt);
+    if (s.hasBvh && traverse<W, true>(s, r, p, tmin, tmax, h, stk, cnt)) return true;
+    return MOTION && s.numInst && traverseInstances<W, true>(s, r, p, time, tmin, tmax, h, stk, cnt);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -187,8 +187,9 @@ __device__ __forceinline__ GuideInput inferWave64(const Src& src, uint32_t entry
     return nin;
 }
 
-template <typename Src>
-__global__ void __launch_bounds__(kInferBlock) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
+// WAVES: waves per block, sharing one LDS copy of the weights (a launch setting).
+template <typename Src, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) kGuideInfer(Src src, uint32_t nHost, const uint32_t* nDev,
                                                            const uint16_t* __restrict__ wb, const float* __restrict__ theta,
                                                            float* __restrict__ mix) {
     extern __shared__ unsigned char smem[];
@@ -591,8 +592,12 @@ __global__ void __launch_bounds__(256) kGuideSumC(const float4* rec, uint32_t n,
 // ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
-constexpr size_t kInferLds = kGLdsXWeightBytes + (kInferBlock / 64) * kInferStageBytes;
+constexpr size_t inferLds(int waves) { return kGLdsXWeightBytes + static_cast<size_t>(waves) * kInferStageBytes; }
+constexpr size_t kInferLds = inferLds(kInferBlock / 64);
 static_assert(2 * kInferLds <= 160 * 1024, "two inference blocks (16 waves) fit one CU's LDS");
+static_assert(inferLds(16) <= 160 * 1024, "a 16-wave inference block fits one CU's LDS");
+// blocks of `waves` waves one CU's 160 KB of LDS holds
+constexpr uint32_t inferBlocksPerCu(int waves) { return static_cast<uint32_t>((160 * 1024) / inferLds(waves)); }
 constexpr size_t kTrainLds = kGLdsWeightBytes + kGLdsTransBytes;
 
 static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {
@@ -602,23 +607,35 @@ static uint32_t gridFor(uint32_t n, uint32_t per, uint32_t cap) {
 
 void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const uint16_t* wb, const float* theta,
                            float* mix, uint32_t maxBlocks) {
-    hipLaunchKernelGGL(kGuideInfer<ArraySource>, dim3(gridFor(n, kInferBlock, maxBlocks)), dim3(kInferBlock), kInferLds, st,
-                       ArraySource{reinterpret_cast<const float4*>(in)}, n, nullptr, wb, theta, mix);
+    hipLaunchKernelGGL((kGuideInfer<ArraySource, kInferBlock / 64>), dim3(gridFor(n, kInferBlock, maxBlocks)),
+                       dim3(kInferBlock), kInferLds, st, ArraySource{reinterpret_cast<const float4*>(in)}, n, nullptr,
+                       wb, theta, mix);
 }
 
+template <int WAVES, typename Src>
+static void inferPaths(hipStream_t st, const Src& src, const uint32_t* count, uint32_t capacity, const uint16_t* wb,
+                       const float* theta, float* mix, uint32_t cus) {
+    const dim3 grid(gridFor(capacity, WAVES * 64, cus * inferBlocksPerCu(WAVES)));
+    hipLaunchKernelGGL((kGuideInfer<Src, WAVES>), grid, dim3(WAVES * 64), inferLds(WAVES), st, src, capacity, count, wb,
+                       theta, mix);
+}
+template <int WAVES>
+static void inferPathsW(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
+                        const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
+                        const float* theta, float* mix, uint32_t cus) {
+    if (s.numInst)
+        inferPaths<WAVES>(st, PathSource<true>{s.tris, ps.p, queue, lo, invExt, s.inst}, count, capacity, wb, theta, mix,
+                          cus);
+    else
+        inferPaths<WAVES>(st, PathSource<false>{s.tris, ps.p, queue, lo, invExt, s.inst}, count, capacity, wb, theta,
+                          mix, cus);
+}
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
-                           const float* theta, float* mix, uint32_t maxBlocks) {
-    const dim3 grid(gridFor(capacity, kInferBlock, maxBlocks));
-    if (s.numInst) {
-        const PathSource<true> src{s.tris, ps.p, queue, lo, invExt, s.inst};
-        hipLaunchKernelGGL(kGuideInfer<PathSource<true>>, grid, dim3(kInferBlock), kInferLds, st, src, capacity, count,
-                           wb, theta, mix);
-    } else {
-        const PathSource<false> src{s.tris, ps.p, queue, lo, invExt, s.inst};
-        hipLaunchKernelGGL(kGuideInfer<PathSource<false>>, grid, dim3(kInferBlock), kInferLds, st, src, capacity, count,
-                           wb, theta, mix);
-    }
+                           const float* theta, float* mix, uint32_t cus, uint32_t waves) {
+    if (waves == 4) inferPathsW<4>(st, s, ps, queue, count, capacity, lo, invExt, wb, theta, mix, cus);
+    else if (waves == 16) inferPathsW<16>(st, s, ps, queue, count, capacity, lo, invExt, wb, theta, mix, cus);
+    else inferPathsW<8>(st, s, ps, queue, count, capacity, lo, invExt, wb, theta, mix, cus);
 }
 
 size_t groupBufferBytes(uint32_t capacity) {
@@ -709,13 +726,18 @@ void launchGuideAdam(hipStream_t st, const float* partial, uint32_t chunks, floa
 void guideConfigureLds() {
     static bool done = false;
     if (done) return;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer<ArraySource>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
-    for (const void* k : {reinterpret_cast<const void*>(&kGuideInfer<PathSource<false>>),
-                          reinterpret_cast<const void*>(&kGuideInfer<PathSource<true>>),
-                          reinterpret_cast<const void*>(&kGuideInferGrouped<false>),
-                          reinterpret_cast<const void*>(&kGuideInferGrouped<true>)})
-        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kInferLds));
+    auto setLds = [](const void* k, size_t bytes) {
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+    };
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<ArraySource, kInferBlock / 64>), kInferLds);
+    setLds(reinterpret_cast<const void*>(&kGuideInferGrouped<false>), kInferLds);
+    setLds(reinterpret_cast<const void*>(&kGuideInferGrouped<true>), kInferLds);
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<false>, 4>), inferLds(4));
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<true>, 4>), inferLds(4));
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<false>, 8>), inferLds(8));
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<true>, 8>), inferLds(8));
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<false>, 16>), inferLds(16));
+    setLds(reinterpret_cast<const void*>(&kGuideInfer<PathSource<true>, 16>), inferLds(16));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideTrainFB), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(kTrainLds));
     done = true;

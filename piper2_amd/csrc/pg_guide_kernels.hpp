@@ -40,10 +40,10 @@ void buildGuideTiles(void* scratch, const GuideBatchLayout& b, WGradTile* host);
 void launchGuideInferArray(hipStream_t st, const float* in, uint32_t n, const uint16_t* wb, const float* theta,
                            float* mix, uint32_t maxBlocks);
 // Mixture records for the surface hits of an extension queue (count read on the device); record i
-// belongs to queue entry i.
+// belongs to queue entry i. cus: the device's CUs; waves: waves per block (4, 8 or 16).
 void launchGuideInferPaths(hipStream_t st, const DevScene& s, const PathState& ps, const uint32_t* queue,
                            const uint32_t* count, uint32_t capacity, float3 lo, float3 invExt, const uint16_t* wb,
-                           const float* theta, float* mix, uint32_t maxBlocks);
+                           const float* theta, float* mix, uint32_t cus, uint32_t waves);
 // Guided batches: the extension queue mixes scenes with different guides. Each iteration groups
 // the queue entries by guide key without global atomics: kSortBlocks blocks count their
 // contiguous share of the queue per key, one block scans those counts (every group padded to

@@ -9,6 +9,7 @@
 // restates piperMain (Tracer/Embree/Kernel.cpp:47-100).
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <type_traits>
 
@@ -164,8 +165,8 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
 template <int W, bool COUNT, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const uint32_t* __restrict__ queue,
                                               const uint32_t* __restrict__ count, uint2* spill, uint32_t spillPerLane,
-                                              unsigned long long* travCounts, BatchTable bt) {
-    __shared__ uint2 lstack[kLdsStack * 64];
+                                              unsigned long long* travCounts, BatchTable bt, int ldsDepth) {
+    extern __shared__ uint2 lstack[];  // ldsDepth x 64 entries
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(64) kExtend(DevScene s0, PathState ps, const u
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[path].meta.z)] : s0;
         const float4 o = ps.p[path].o, d = ps.p[path].d;
         const Ray r{xyz(o), xyz(d)};
-        const HitRec h = closestHit<W, COUNT, MOTION>(s, r, 1e-3f, 1e5f, lstack + lane, gs, cnt, d.w);
+        const HitRec h = closestHit<W, COUNT, MOTION>(s, r, 1e-3f, 1e5f, TravStack{lstack + lane, gs, ldsDepth}, cnt, d.w);
         ps.p[path].hit = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
     }
     if (COUNT) {
@@ -214,11 +215,11 @@ __device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
 // Device-scope atomics on one counter serialise across the whole chip, so the shade stage takes
 // one per block and queue (not one per wave); the staged records then leave as coalesced stores.
-constexpr uint32_t kShadeBlock = 256;
+template <uint32_t BLOCK>
 struct ShadeLds {
-    ShadowRec nee[kShadeBlock];   // NEE shadow ray
-    ShadowRec rec2[kShadeBlock];  // MIS-BSDF occlusion query (ShadowRec) or MIS ray (MisRec)
-    uint32_t cnt[5][kShadeBlock / 64];  // per wave: ext, done, nee, MIS query, MIS ray
+    ShadowRec nee[BLOCK];   // NEE shadow ray
+    ShadowRec rec2[BLOCK];  // MIS-BSDF occlusion query (ShadowRec) or MIS ray (MisRec)
+    uint32_t cnt[5][BLOCK / 64];  // per wave: ext, done, nee, MIS query, MIS ray
     uint32_t base[4];                   // block's first entry: ext, done, shadow, mis queue
 };
 static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging slots");
@@ -226,19 +227,19 @@ static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging s
 // Waves per SIMD: 4 (<= 128 VGPRs) for the plain and batched kernels: at 5 (<= 96 VGPRs) the
 // plastic / Oren-Nayar lobes spill 14 VGPRs and the stage takes 18.9 instead of 17.0 ms on the
 // 1080p64 terrain frame. The guided ones keep 3 for their extra state.
-template <bool BATCH, bool GUIDED, bool TEX, bool MOTION>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
+template <bool BATCH, bool GUIDED, bool TEX, bool MOTION, uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
 kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
        GuideShade gs, BatchTable bt) {
-    __shared__ ShadeLds lds;
+    __shared__ ShadeLds<BLOCK> lds;
     const uint32_t n = q.counts[curSlot];
     const uint32_t* qcur = q.ext[curSlot];
     uint32_t* qnext = q.ext[nextSlot];
     const uint32_t tid = threadIdx.x, wave = tid / 64, lane = __lane_id();
     const uint64_t lowerLanes = (1ull << lane) - 1ull;
     uint64_t totExt = 0, totNee = 0, totMis = 0;  // thread 0: this block's ray counts
-    const uint32_t stride = gridDim.x * kShadeBlock;
-    for (uint32_t i0 = blockIdx.x * kShadeBlock; i0 < n; i0 += stride) {  // block-uniform loop (barriers)
+    const uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += stride) {  // block-uniform loop (barriers)
         const uint32_t i = i0 + tid;
         bool pushExt = false, pushShadow = false, pushMis = false, pushMisQuery = false, pushDone = false;
         uint32_t path = 0;
@@ -458,7 +459,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
         __syncthreads();
         if (tid < 4) {  // queue tid: ext, done, shadow (NEE + MIS queries), mis
             uint32_t t = 0;
-            for (uint32_t w = 0; w < kShadeBlock / 64; ++w)
+            for (uint32_t w = 0; w < BLOCK / 64; ++w)
                 t += tid == 2 ? lds.cnt[2][w] + lds.cnt[3][w] : lds.cnt[tid == 3 ? 4 : tid][w];
             uint32_t* ctr = tid == 0 ? &q.counts[nextSlot]
                             : tid == 1 ? &q.counts[kCntDone]
@@ -467,7 +468,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
             lds.base[tid] = t ? atomicAdd(ctr, t) : 0u;
         }
         if (tid == 0)
-            for (uint32_t w = 0; w < kShadeBlock / 64; ++w) {
+            for (uint32_t w = 0; w < BLOCK / 64; ++w) {
                 totExt += lds.cnt[0][w];
                 totNee += lds.cnt[2][w];
                 totMis += lds.cnt[3][w] + lds.cnt[4][w];
@@ -499,8 +500,8 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
 // ------------------------------------------------------------------------------------------
 template <int W, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
-                                              BatchTable bt) {
-    __shared__ uint2 lstack[kLdsStack * 64];
+                                              BatchTable bt, int ldsDepth) {
+    extern __shared__ uint2 lstack[];  // ldsDepth x 64 entries
     const uint32_t n = q.counts[kCntShadow];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
@@ -510,7 +511,7 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
         const DevScene& s = BATCH ? bt.scenes[sceneOf(ps.p[r.path & ~kMisFlag].meta.z)] : s0;
         const Ray ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}};
         const float time = s.numInst ? ps.p[r.path & ~kMisFlag].d.w : 0.0f;
-        if (!occluded<W, MOTION>(s, ray, r.tmin, r.tmax, lstack + lane, gs, time)) {
+        if (!occluded<W, MOTION>(s, ray, r.tmin, r.tmax, TravStack{lstack + lane, gs, ldsDepth}, time)) {
             if (r.path & kMisFlag) {  // MIS-BSDF light hit: kept apart, folded in after the NEE term
                 ps.p[r.path & ~kMisFlag].Lmis = make_float4(r.cr, r.cg, r.cb, 0.0f);
             } else {
@@ -528,8 +529,8 @@ __global__ void __launch_bounds__(64) kShadow(DevScene s0, PathState ps, Queues 
 // ------------------------------------------------------------------------------------------
 template <int W, bool BATCH, bool MOTION>
 __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, uint2* spill, uint32_t spillPerLane,
-                                           BatchTable bt) {
-    __shared__ uint2 lstack[kLdsStack * 64];
+                                           BatchTable bt, int ldsDepth) {
+    extern __shared__ uint2 lstack[];  // ldsDepth x 64 entries
     const uint32_t n = q.counts[kCntMis];
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
@@ -558,7 +559,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
                 }
                 // tq > 0: the next float up makes the occlusion range (tmin, tq] inclusive
                 const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
-                if (first && !occluded<W, MOTION>(s, ray, 1e-3f, tqUp, lstack + lane, gs, time)) {
+                if (first && !occluded<W, MOTION>(s, ray, 1e-3f, tqUp, TravStack{lstack + lane, gs, ldsDepth}, time)) {
                     const V3 qn = quadNormal(qd, ray.d);
                     rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
                     lpdf = tq * tq / (fabsf(dot(qn, ray.d)) * qd.area);
@@ -566,7 +567,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
             }
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W, false, MOTION>(s, ray, 1e-3f, 1e5f, lstack + lane, gs, tc, time);
+            const HitRec h = closestHit<W, false, MOTION>(s, ray, 1e-3f, 1e5f, TravStack{lstack + lane, gs, ldsDepth}, tc, time);
             if (h.pos == kHitMiss) {
                 rad = ld3(s.env);
                 lpdf = envPdf(ray.d);
@@ -847,8 +848,8 @@ __global__ void __launch_bounds__(256) kCopy(const uint4* __restrict__ src, uint
 template <int W, bool MOTION>
 __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint32_t n, float* t, uint32_t* prim,
                                              float* uv, uint8_t* occ, uint2* spill, uint32_t spillPerLane,
-                                             const float* times) {
-    __shared__ uint2 lstack[kLdsStack * 64];
+                                             const float* times, int ldsDepth) {
+    extern __shared__ uint2 lstack[];  // ldsDepth x 64 entries
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * 64 + lane;
     uint2* gs = spill + static_cast<size_t>(gtid) * spillPerLane;
@@ -857,10 +858,10 @@ __global__ void __launch_bounds__(64) kTrace(DevScene s, const float* rays, uint
         const Ray ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
         const float time = times ? times[i] : 0.0f;
         if (occ) {
-            occ[i] = occluded<W, MOTION>(s, ray, r[6], r[7], lstack + lane, gs, time) ? 1 : 0;
+            occ[i] = occluded<W, MOTION>(s, ray, r[6], r[7], TravStack{lstack + lane, gs, ldsDepth}, time) ? 1 : 0;
         } else {
             TravCount tc;
-            const HitRec h = closestHit<W, false, MOTION>(s, ray, r[6], r[7], lstack + lane, gs, tc, time);
+            const HitRec h = closestHit<W, false, MOTION>(s, ray, r[6], r[7], TravStack{lstack + lane, gs, ldsDepth}, tc, time);
             t[i] = h.t;
             prim[i] = h.prim;
             uv[2 * i] = h.u;
@@ -902,29 +903,34 @@ static inline uint32_t blocksFor(uint64_t n, uint32_t bs, uint32_t cap) {
 // for the kernel (its occupancy, set by VGPRs or LDS): a larger grid leaves a tail of blocks that
 // start only as others end, which for traversal on a 1920x1080 64-spp frame cost 7-12 ms of ~91.
 // gridLanes (for traversal: the lanes the spill region is sized for) stays the upper bound.
-static dim3 residentGrid(const void* kernel, uint32_t gridLanes, uint32_t blockSize) {
+// dynLds: the launch's dynamic LDS bytes per block; perCuCap > 0 caps the blocks per CU.
+static dim3 residentGrid(const void* kernel, uint32_t gridLanes, uint32_t blockSize, size_t dynLds, uint32_t perCuCap) {
+    struct Fit {
+        uint32_t cus, perCu;  // 0: unknown, keep the caller's grid
+    };
     static std::mutex mu;
-    static std::map<std::pair<int, const void*>, uint32_t> cache;
-    uint32_t waves = gridLanes / blockSize;  // blocks
+    static std::map<std::tuple<int, const void*, size_t>, Fit> cache;
+    uint32_t blocks = gridLanes / blockSize;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return dim3(waves);
+    if (hipGetDevice(&dev) != hipSuccess) return dim3(std::max(blocks, 1u));
     std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find({dev, kernel});
+    auto it = cache.find({dev, kernel, dynLds});
     if (it == cache.end()) {
-        int cus = 0, blocks = 0;
-        uint32_t resident = 0;  // 0: unknown, keep the caller's grid
+        int cus = 0, per = 0;
+        Fit f{0, 0};
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, static_cast<int>(blockSize), 0) == hipSuccess &&
-            cus > 0 && blocks > 0)
-            resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
-        it = cache.emplace(std::make_pair(dev, kernel), resident).first;
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, static_cast<int>(blockSize), dynLds) == hipSuccess &&
+            cus > 0 && per > 0)
+            f = Fit{static_cast<uint32_t>(cus), static_cast<uint32_t>(per)};
+        it = cache.emplace(std::make_tuple(dev, kernel, dynLds), f).first;
     }
-    if (it->second) waves = std::min(waves, it->second);
-    return dim3(std::max(waves, 1u));
+    const Fit& f = it->second;
+    if (f.perCu) blocks = std::min(blocks, f.cus * (perCuCap ? std::min(perCuCap, f.perCu) : f.perCu));
+    return dim3(std::max(blocks, 1u));
 }
 template <typename F>
-static dim3 fit(F* kernel, uint32_t gridLanes, uint32_t blockSize = 64) {
-    return residentGrid(reinterpret_cast<const void*>(kernel), gridLanes, blockSize);
+static dim3 fit(F* kernel, uint32_t gridLanes, uint32_t blockSize = 64, size_t dynLds = 0, uint32_t perCuCap = 0) {
+    return residentGrid(reinterpret_cast<const void*>(kernel), gridLanes, blockSize, dynLds, perCuCap);
 }
 
 void launchPoolInit(hipStream_t st, const Queues& q, uint32_t capacity) {
@@ -955,21 +961,25 @@ static void withWidth(uint32_t w, F&& f) {
 // Kernels with the moving-instance code run only for scenes (or batches) that have moving instances.
 static bool anyMotion(const DevScene& s, const BatchTable* bt) { return bt ? bt->motion != 0 : s.numInst != 0; }
 
+static size_t stackLds(const LaunchTune& lt) { return static_cast<size_t>(lt.ldsStack) * 64 * sizeof(uint2); }
+
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
-                  const BatchTable* bt) {
+                  const BatchTable* bt, const LaunchTune& lt) {
     auto* tc = reinterpret_cast<unsigned long long*>(travCounts);
     const uint32_t* queue = q.ext[curSlot];
     const uint32_t* count = q.counts + curSlot;
     const BatchTable b = bt ? *bt : BatchTable{};
+    const size_t lds = stackLds(lt);
     withWidth(s.width, [&](auto w) {
         withFlag(tc != nullptr, [&](auto cnt) {
             withFlag(bt != nullptr, [&](auto batch) {
                 withFlag(anyMotion(s, bt), [&](auto mot) {
                     constexpr int W = decltype(w)::value;
                     constexpr bool C = decltype(cnt)::value, B = decltype(batch)::value, M = decltype(mot)::value;
-                    hipLaunchKernelGGL((kExtend<W, C, B, M>), fit(kExtend<W, C, B, M>, gridLanes), dim3(64), 0, st, s,
-                                       ps, queue, count, spill, spillPerLane, tc, b);
+                    hipLaunchKernelGGL((kExtend<W, C, B, M>), fit(kExtend<W, C, B, M>, gridLanes, 64, lds, lt.travWaves),
+                                       dim3(64), lds, st, s, ps, queue, count, spill, spillPerLane, tc, b,
+                                       static_cast<int>(lt.ldsStack));
                 });
             });
         });
@@ -977,45 +987,52 @@ void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const 
 }
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
-                 const BatchTable* bt, bool textured) {
+                 const BatchTable* bt, bool textured, const LaunchTune& lt) {
     const BatchTable b = bt ? *bt : BatchTable{};
-    withFlag(bt != nullptr, [&](auto batch) {  // a guided batch reads its per-item guides in bt.guide
-        withFlag(gs.enabled != 0, [&](auto guided) {
-            withFlag(textured, [&](auto tex) {
-                withFlag(anyMotion(s, bt), [&](auto mot) {
-                    constexpr bool B = decltype(batch)::value, G = decltype(guided)::value, T = decltype(tex)::value,
-                                   M = decltype(mot)::value;
-                    hipLaunchKernelGGL((kShade<B, G, T, M>), fit(kShade<B, G, T, M>, gridThreads, 256), dim3(256), 0,
-                                       st, s, ps, q, rc, curSlot, nextSlot, totals, gs, b);
+    auto go = [&](auto blk) {
+        withFlag(bt != nullptr, [&](auto batch) {  // a guided batch reads its per-item guides in bt.guide
+            withFlag(gs.enabled != 0, [&](auto guided) {
+                withFlag(textured, [&](auto tex) {
+                    withFlag(anyMotion(s, bt), [&](auto mot) {
+                        constexpr bool B = decltype(batch)::value, G = decltype(guided)::value,
+                                       T = decltype(tex)::value, M = decltype(mot)::value;
+                        constexpr uint32_t K = decltype(blk)::value;
+                        hipLaunchKernelGGL((kShade<B, G, T, M, K>), fit(kShade<B, G, T, M, K>, gridThreads, K), dim3(K),
+                                           0, st, s, ps, q, rc, curSlot, nextSlot, totals, gs, b);
+                    });
                 });
             });
         });
-    });
+    };
+    if (lt.shadeBlock == 128) go(std::integral_constant<uint32_t, 128>{});
+    else go(std::integral_constant<uint32_t, 256>{});
 }
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
+                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt, const LaunchTune& lt) {
     const BatchTable b = bt ? *bt : BatchTable{};
+    const size_t lds = stackLds(lt);
     withWidth(s.width, [&](auto w) {
         withFlag(bt != nullptr, [&](auto batch) {
             withFlag(anyMotion(s, bt), [&](auto mot) {
                 constexpr int W = decltype(w)::value;
                 constexpr bool B = decltype(batch)::value, M = decltype(mot)::value;
-                hipLaunchKernelGGL((kShadow<W, B, M>), fit(kShadow<W, B, M>, gridLanes), dim3(64), 0, st, s, ps, q,
-                                   spill, spillPerLane, b);
+                hipLaunchKernelGGL((kShadow<W, B, M>), fit(kShadow<W, B, M>, gridLanes, 64, lds, lt.travWaves), dim3(64),
+                                   lds, st, s, ps, q, spill, spillPerLane, b, static_cast<int>(lt.ldsStack));
             });
         });
     });
 }
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-               uint2* spill, uint32_t spillPerLane, const BatchTable* bt) {
+               uint2* spill, uint32_t spillPerLane, const BatchTable* bt, const LaunchTune& lt) {
     const BatchTable b = bt ? *bt : BatchTable{};
+    const size_t lds = stackLds(lt);
     withWidth(s.width, [&](auto w) {
         withFlag(bt != nullptr, [&](auto batch) {
             withFlag(anyMotion(s, bt), [&](auto mot) {
                 constexpr int W = decltype(w)::value;
                 constexpr bool B = decltype(batch)::value, M = decltype(mot)::value;
-                hipLaunchKernelGGL((kMis<W, B, M>), fit(kMis<W, B, M>, gridLanes), dim3(64), 0, st, s, ps, q, spill,
-                                   spillPerLane, b);
+                hipLaunchKernelGGL((kMis<W, B, M>), fit(kMis<W, B, M>, gridLanes, 64, lds, lt.travWaves), dim3(64), lds,
+                                   st, s, ps, q, spill, spillPerLane, b, static_cast<int>(lt.ldsStack));
             });
         });
     });
@@ -1062,13 +1079,15 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
     }
 }
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
-                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const float* times) {
+                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const LaunchTune& lt,
+                 const float* times) {
+    const size_t lds = stackLds(lt);
     withWidth(s.width, [&](auto w) {
         withFlag(s.numInst != 0, [&](auto mot) {
             constexpr int W = decltype(w)::value;
             constexpr bool M = decltype(mot)::value;
-            hipLaunchKernelGGL((kTrace<W, M>), fit(kTrace<W, M>, gridLanes), dim3(64), 0, st, s, rays, n, t, prim, uv,
-                               occ, spill, spillPerLane, times);
+            hipLaunchKernelGGL((kTrace<W, M>), fit(kTrace<W, M>, gridLanes, 64, lds, lt.travWaves), dim3(64), lds, st, s,
+                               rays, n, t, prim, uv, occ, spill, spillPerLane, times, static_cast<int>(lt.ldsStack));
         });
     });
 }

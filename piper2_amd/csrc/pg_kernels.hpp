@@ -92,6 +92,17 @@ struct RecordRing {
     uint32_t* itemDone;  // paths recorded per item
 };
 
+// Launch settings of a scene (pg_autotune picks them; no choice changes any result): the
+// traversal stack entries each lane keeps in LDS (deeper ones spill to HBM), the traversal waves
+// per CU (0: as many as stay resident), the shade block and the waves per guide-inference block.
+constexpr uint32_t kLdsStackMax = 16;
+struct LaunchTune {
+    uint32_t ldsStack = kLdsStack;  // 4 .. kLdsStackMax
+    uint32_t travWaves = 0;
+    uint32_t shadeBlock = 256;      // 128 or 256
+    uint32_t inferWaves = 8;        // 4, 8 or 16 (guided batches always use 8)
+};
+
 // `bt` selects the batched kernels (nullptr: one scene, `s` and `rc` in kernel arguments).
 // Batched launches still take the first scene as `s`: all scenes share its BVH width.
 //
@@ -104,14 +115,14 @@ void launchRegen(hipStream_t st, const DevScene& s, const PathState& ps, const Q
                  uint32_t curSlot, uint32_t gridThreads, const BatchTable* bt = nullptr);
 void launchExtend(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t curSlot,
                   uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, uint32_t* travCounts,
-                  const BatchTable* bt = nullptr);
+                  const BatchTable* bt, const LaunchTune& lt);
 void launchShade(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, const RenderConsts& rc,
                  uint32_t curSlot, uint32_t nextSlot, uint32_t gridThreads, uint64_t* totals, const GuideShade& gs,
-                 const BatchTable* bt, bool textured);
+                 const BatchTable* bt, bool textured, const LaunchTune& lt);
 void launchShadow(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
+                  uint2* spill, uint32_t spillPerLane, const BatchTable* bt, const LaunchTune& lt);
 void launchMis(hipStream_t st, const DevScene& s, const PathState& ps, const Queues& q, uint32_t gridLanes,
-               uint2* spill, uint32_t spillPerLane, const BatchTable* bt = nullptr);
+               uint2* spill, uint32_t spillPerLane, const BatchTable* bt, const LaunchTune& lt);
 // Film records of the paths that finished this iteration (done entries above kCntRecorded).
 void launchRecord(hipStream_t st, const PathState& ps, const Queues& q, const RenderConsts& rc, uint32_t gridThreads,
                   const RecordRing& ring, const GuideShade& gs, const GuideRecordOut& tr,
@@ -122,7 +133,8 @@ void launchResolve(hipStream_t st, const RenderConsts& rc, const RecordRing& rin
                    float* film32, double* film64);
 // times: the ray times (moving instances), or nullptr for time 0.
 void launchTrace(hipStream_t st, const DevScene& s, const float* rays, uint32_t n, float* t, uint32_t* prim, float* uv,
-                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const float* times = nullptr);
+                 uint8_t* occ, uint32_t gridLanes, uint2* spill, uint32_t spillPerLane, const LaunchTune& lt,
+                 const float* times = nullptr);
 void launchCameraRays(hipStream_t st, const DevScene& s, const RenderConsts& rc, const uint32_t* pix, uint32_t n,
                       float* rays, float* film);
 void launchCopy(hipStream_t st, const void* src, void* dst, size_t bytes);
