@@ -71,7 +71,7 @@ typedef struct pg_material {
     float reserved[3];  /* plastic, substrate: specular rgb; glass: transmission rgb */
 } pg_material;
 
-typedef struct pg_quad { /* one-primitive Plane light (BasicShapes/Shared.hpp:22-28, AreaLight/Shared.hpp:22-26) */
+typedef struct pg_quad { /* Plane primitive of an area light (BasicShapes/Shared.hpp:22-28, AreaLight/Shared.hpp:22-26) */
     float origin[3], u[3], v[3];
     float radiance[3];
 } pg_quad;
@@ -126,12 +126,12 @@ typedef struct pg_scene_desc {
     uint32_t num_objects;
     const pg_material* materials;
     uint32_t num_materials;
-    const pg_quad* quads; /* area lights; light 0 is the environment (Embree.cpp:1175-1188) */
+    const pg_quad* quads; /* primitives of the area lights; light 0 is the environment (Embree.cpp:1175-1188) */
     uint32_t num_quads;
     pg_camera camera;
     float env_radiance[3]; /* constant environment (BasicLight/Kernel.cpp:59-89) */
-    /* point lights; lights are selected uniformly over {environment, quads, point lights} in that
-     * order. Zero / NULL for none. */
+    /* point lights; lights are selected uniformly over {environment, area lights, point lights} in
+     * that order. Zero / NULL for none. */
     const pg_point_light* point_lights;
     uint32_t num_point_lights;
     /* textures (optional): per-vertex texture coordinates (2 floats per vertex; NULL gives (0, 0)
@@ -141,6 +141,12 @@ typedef struct pg_scene_desc {
     const pg_texture* textures;
     uint32_t num_textures;
     const pg_material_textures* material_textures;
+    /* area lights of several primitives (DiffuseAreaLight over a Plane with several Primitives,
+     * BasicShapes.cpp:59-94): light l is the next area_light_quads[l] quads, which share one radiance;
+     * the counts cover every quad. A light picks its primitive by area (select, Kernel/Sampling.hpp:52-63)
+     * and its density is over the whole light's area. NULL: every quad is a light of its own. */
+    const uint32_t* area_light_quads;
+    uint32_t num_area_lights;
 } pg_scene_desc;
 
 #define PG_BVH_HOST 0u   /* binned SAH on the host's cores (multithreaded) */

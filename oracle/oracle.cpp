@@ -187,7 +187,13 @@ struct Material {
 };
 struct Quad {  // BasicShapes/Shared.hpp:22-28, BasicShapes.cpp:70-90
     V3 origin, u, v, normal, radiance;
-    float area;
+    float area;           // area of the owning light: every primitive of its Plane (BasicShapes.cpp:86)
+    uint32_t light = 0;   // owning area light
+    float cdf = 1.0f, pdf = 1.0f;  // area CDF and probability within the light (BasicShapes.cpp:87-94)
+};
+// Area light: DiffuseAreaLight over a Plane with `count` primitives, quads [first, first + count).
+struct AreaLight {
+    uint32_t first, count;
 };
 // Textures: CheckBoard (ProceduralTexture/Kernel.cpp:28-38) and 8-bit sampled images
 // (SampledTexture/Kernel.cpp:22-82).
@@ -218,6 +224,7 @@ struct Scene {
     std::vector<uint32_t> triObj, objMat;
     std::vector<Material> mats;
     std::vector<Quad> quads;
+    std::vector<AreaLight> lights;
     std::vector<PointLight> points;
     std::vector<float> triUV;  // 6 per triangle: texture coordinates of corners 0, 1, 2
     std::vector<Texture> textures;
@@ -1218,25 +1225,44 @@ Frame shadingFrame(V3 n, bool d3) {
     return f;
 }
 
-// uniformSelect (Light/BasicLight/Kernel.cpp:91-98) over {environment, quad lights...}
+// uniformSelect (Light/BasicLight/Kernel.cpp:91-98) over {environment, area lights, point lights}
 struct LightSel {
-    uint32_t light;  // 0 = environment, 1 + q = quad q, 1 + nq + p = point light p
+    uint32_t light;  // 0 = environment, 1 + l = area light l, 1 + nl + p = point light p
     float pdf;
     bool delta;      // a point light (LightAttributes::Delta)
 };
 LightSel selectLight(const Scene& s, float u) {  // uniformSelect, BasicLight/Kernel.cpp:91-98
-    const uint32_t nq = static_cast<uint32_t>(s.quads.size());
-    const uint32_t count = 1 + nq + static_cast<uint32_t>(s.points.size());
+    const uint32_t nl = static_cast<uint32_t>(s.lights.size());
+    const uint32_t count = 1 + nl + static_cast<uint32_t>(s.points.size());
     const uint32_t sel = std::min(static_cast<uint32_t>(std::floor(u * static_cast<float>(count))), count - 1);
-    return {sel, 1.0f / static_cast<float>(count), sel > nq};
+    return {sel, 1.0f / static_cast<float>(count), sel > nl};
 }
 
 struct LightSample {
     V3 dir, rad;
     float dist, pdf;
 };
-// diffuseSample (AreaLight/Kernel.cpp:24-41) over planeSample (BasicShapes/Kernel.cpp:113-134)
-// for a one-primitive plane; SESample (BasicLight/Kernel.cpp:63-76) for the environment.
+// select (Kernel/Sampling.hpp:52-63): the first primitive whose CDF entry exceeds u, found by the
+// same bisection. u is then rescaled into that primitive's own [0, 1): spec mode subtracts the CDF's
+// lower bound (the previous entry), D1 subtracts its upper bound cdf[r] as the reference does.
+// u = 1 (possible: the Sobol conversion has no clamp) would index one past the CDF in the
+// reference; here it picks the last primitive.
+uint32_t selectPrimitive(const Scene& s, const AreaLight& L, float& u, bool d1) {
+    uint32_t l = 0, r = L.count;
+    while (l < r) {
+        const uint32_t mid = (l + r) >> 1;
+        if (u >= s.quads[L.first + mid].cdf) l = mid + 1;
+        else r = mid;
+    }
+    r = std::min(r, L.count - 1);
+    const Quad& q = s.quads[L.first + r];
+    const float lower = d1 ? q.cdf : (r ? s.quads[L.first + r - 1].cdf : 0.0f);
+    u = (u - lower) / q.pdf;
+    return L.first + r;
+}
+
+// diffuseSample (AreaLight/Kernel.cpp:24-41) over planeSample (BasicShapes/Kernel.cpp:113-134);
+// SESample (BasicLight/Kernel.cpp:63-76) for the environment.
 LightSample sampleLight(const Scene& s, uint32_t light, V3 hit, float u1, float u2, bool d1) {
     LightSample ls{{0, 0, 0}, {0, 0, 0}, 0.0f, 0.0f};
     if (light == 0) {
@@ -1248,8 +1274,8 @@ LightSample sampleLight(const Scene& s, uint32_t light, V3 hit, float u1, float 
         ls.dist = 1e5f;
         return ls;
     }
-    if (light > s.quads.size()) {  // pointSample, BasicLight/Kernel.cpp:35-45
-        const PointLight& p = s.points[light - 1 - s.quads.size()];
+    if (light > s.lights.size()) {  // pointSample, BasicLight/Kernel.cpp:35-45
+        const PointLight& p = s.points[light - 1 - s.lights.size()];
         const V3 delta = p.pos - hit;
         const float dis2 = dot(delta, delta);
         ls.rad = V3{p.intensity.x / dis2, p.intensity.y / dis2, p.intensity.z / dis2};
@@ -1258,10 +1284,8 @@ LightSample sampleLight(const Scene& s, uint32_t light, V3 hit, float u1, float 
         ls.pdf = 1.0f;
         return ls;
     }
-    const Quad& q = s.quads[light - 1];
-    // select() over a one-entry CDF {1}: spec remaps with the lower bound (u unchanged),
-    // D1 remaps with the upper bound: u - 1.
-    const float uu = d1 ? (u1 - 1.0f) / 1.0f : u1;
+    float uu = u1;
+    const Quad& q = s.quads[selectPrimitive(s, s.lights[light - 1], uu, d1)];
     const V3 src = q.origin + q.u * uu + q.v * u2;
     V3 n = q.normal;
     if (dot(hit - src, n) <= 0.0f) n = -n;
@@ -1352,7 +1376,7 @@ V3 tracePath(const Scene& s, const Params& p, Sampler& smp, Ray ray, PathStats& 
                         const float theta = std::acos(std::fmin(1.0f, std::fmax(-1.0f, dir.z)));
                         const float st2 = std::sin(theta);
                         lpdf = st2 == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st2);
-                    } else if ((lh.prim & kQuadBit) && sel.light == 1 + (lh.prim & ~kQuadBit)) {
+                    } else if ((lh.prim & kQuadBit) && sel.light == 1 + s.quads[lh.prim & ~kQuadBit].light) {
                         const Quad& q = s.quads[lh.prim & ~kQuadBit];
                         const V3 qn = hitNormal(s, lh, dir);
                         rad = dot(qn, dir) < 0.0f ? q.radiance : V3{0, 0, 0};  // diffuseEvaluate
@@ -1513,7 +1537,9 @@ void* or_scene_create(const float* verts, uint32_t nv, const uint32_t* idx, uint
         const V3 det = cross(Q.u, Q.v);  // BasicShapes.cpp:76-90
         Q.area = length(det);
         Q.normal = {det.x / Q.area, det.y / Q.area, det.z / Q.area};
+        Q.light = i;
         s->quads.push_back(Q);
+        s->lights.push_back({i, 1});
     }
     s->cam = {{cam[0], cam[1], cam[2]}, {cam[3], cam[4], cam[5]}, {cam[6], cam[7], cam[8]},
               cam[9], cam[10], cam[11], cam[12]};
@@ -1702,7 +1728,40 @@ void or_bsdf_sample(const float* mat, int front, const float* wo, float u1, floa
     f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
     *pdf = s.pdf;
 }
-// points[6*n] = {position xyz, intensity rgb}; light order: environment, quads, points.
+// Area lights of several primitives: light l is the next counts[l] quads (sum = number of quads);
+// without this call every quad is a light of its own. As the Plane constructor
+// (BasicShapes.cpp:59-94): the light's area is the float sum of its primitives' |U x V|, the CDF
+// entry of primitive i the running sum divided by that area, its probability its area divided by it.
+int or_scene_set_area_lights(void* scene, const uint32_t* counts, uint32_t n) {
+    auto* s = static_cast<Scene*>(scene);
+    uint32_t first = 0;
+    for (uint32_t l = 0; l < n; ++l) first += counts[l];
+    if (first != s->quads.size()) return -1;
+    s->lights.clear();
+    first = 0;
+    for (uint32_t l = 0; l < n; ++l) {
+        float total = 0.0f;
+        std::vector<float> cum, area;
+        for (uint32_t k = 0; k < counts[l]; ++k) {
+            const Quad& q = s->quads[first + k];
+            const float a = length(cross(q.u, q.v));
+            total = total + a;
+            cum.push_back(total);
+            area.push_back(a);
+        }
+        for (uint32_t k = 0; k < counts[l]; ++k) {
+            Quad& q = s->quads[first + k];
+            q.light = l;
+            q.cdf = cum[k] / total;
+            q.pdf = area[k] / total;
+            q.area = total;
+        }
+        s->lights.push_back({first, counts[l]});
+        first += counts[l];
+    }
+    return 0;
+}
+// points[6*n] = {position xyz, intensity rgb}; light order: environment, area lights, points.
 void or_scene_set_points(void* scene, const float* points, uint32_t n) {
     auto* s = static_cast<Scene*>(scene);
     s->points.clear();

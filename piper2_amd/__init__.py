@@ -73,6 +73,7 @@ class SceneDesc(ctypes.Structure):
         ("point_lights", ctypes.c_void_p), ("num_point_lights", ctypes.c_uint32),
         ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("num_textures", ctypes.c_uint32),
         ("material_textures", ctypes.c_void_p),
+        ("area_light_quads", ctypes.c_void_p), ("num_area_lights", ctypes.c_uint32),
     ]
 
 
@@ -565,6 +566,10 @@ def scene_desc(sc):
     if mtex is not None:
         mtex = sc["_mtex"] = np.ascontiguousarray(mtex, np.uint32).reshape(-1, 3)
     sc["_tex_arr"], sc["_texels"] = tex_arr, keep
+    # optional "area_lights": primitives per area light, in quad order (pg_scene_desc.area_light_quads)
+    al = sc.get("area_lights")
+    if al is not None:
+        al = sc["_area_lights"] = np.ascontiguousarray(al, np.uint32).reshape(-1)
     c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
     cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))
     return SceneDesc(
@@ -573,7 +578,8 @@ def scene_desc(sc):
         ctypes.cast(mat_arr, ctypes.c_void_p), len(mats), ctypes.cast(quad_arr, ctypes.c_void_p), len(quads),
         cam, F3(*sc["env"]), _ptr(pts) if pts is not None else None, 0 if pts is None else pts.shape[0],
         _ptr(texcoords) if texcoords is not None else None, ctypes.cast(tex_arr, ctypes.c_void_p) if texs else None,
-        len(texs), _ptr(mtex) if mtex is not None else None)
+        len(texs), _ptr(mtex) if mtex is not None else None, _ptr(al) if al is not None else None,
+        0 if al is None else al.shape[0])
 
 
 def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0, film_filter=0,

@@ -189,7 +189,7 @@ struct pg_scene {
     pg::BvhParams bvhParams;  // of the current build
     uint32_t builder = PG_BVH_HOST;
     pg::DevScene ds{};
-    DevBuf tris, triMatLeaf, nodes, mats, quads, points, triUV, textures, texels, matTex;
+    DevBuf tris, triMatLeaf, nodes, mats, quads, lights, points, triUV, textures, texels, matTex;
     pg::LaunchTune tune;            // launch settings of this scene's renders (pg_autotune)
     uint32_t stackNeed = 1;         // deepest traversal stack a lane can need
     uint32_t spillPerLane = 1;      // of them beyond the LDS part (tune.ldsStack)
@@ -1442,6 +1442,21 @@ int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* insta
                     triUV[6 * i + 2 * k + 1] = desc->texcoords[2 * v + 1];
                 }
         }
+        // area lights: light l is the next area_light_quads[l] quads, or every quad a light of its own
+        std::vector<pg::DevAreaLight> lights;
+        if (desc->area_light_quads) {
+            require(desc->num_area_lights <= 4096, "bad area lights");
+            uint32_t first = 0;
+            for (uint32_t l = 0; l < desc->num_area_lights; ++l) {
+                const uint32_t c = desc->area_light_quads[l];
+                require(c > 0 && c <= desc->num_quads - first, "area light primitive counts must cover the quads");
+                lights.push_back({first, c});
+                first += c;
+            }
+            require(first == desc->num_quads, "area light primitive counts must cover the quads");
+        } else {
+            for (uint32_t i = 0; i < desc->num_quads; ++i) lights.push_back({i, 1});
+        }
         std::vector<pg::DevQuad> quads(desc->num_quads);
         for (uint32_t i = 0; i < desc->num_quads; ++i) {
             const pg_quad& q = desc->quads[i];
@@ -1457,14 +1472,38 @@ int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* insta
             const float a0 = std::fabs(det[0]), a1 = std::fabs(det[1]), a2 = std::fabs(det[2]);
             d.maxDetComp = (a0 >= a1 && a0 >= a2) ? 0u : ((a1 >= a0 && a1 >= a2) ? 1u : 2u);
         }
+        // Plane (BasicShapes.cpp:59-94): the light's area is the float sum of its primitives' areas in
+        // order, a primitive's CDF entry the running sum over that total, its probability its area over it;
+        // DiffuseAreaLight has one radiance, so the primitives of one light must share it
+        for (uint32_t l = 0; l < lights.size(); ++l) {
+            const pg::DevAreaLight L = lights[l];
+            float total = 0.0f;
+            for (uint32_t k = 0; k < L.count; ++k) {
+                pg::DevQuad& d = quads[L.first + k];
+                require(std::memcmp(d.radiance, quads[L.first].radiance, 12) == 0,
+                        "the primitives of one area light must share its radiance");
+                total = total + d.area;
+                d.cdf = total;
+            }
+            for (uint32_t k = 0; k < L.count; ++k) {
+                pg::DevQuad& d = quads[L.first + k];
+                d.light = l;
+                d.cdf = d.cdf / total;
+                d.pdf = d.area / total;
+                d.area = total;
+            }
+        }
         hipStream_t st = dev->stream;
         upload(s->mats, mats.data(), mats.size() * sizeof(pg::DevMaterial), st);
         upload(s->quads, quads.data(), quads.size() * sizeof(pg::DevQuad), st);
+        upload(s->lights, lights.data(), lights.size() * sizeof(pg::DevAreaLight), st);
         upload(s->points, desc->point_lights, desc->num_point_lights * sizeof(pg_point_light), st);
         pg::DevScene& ds = s->ds;
         ds.mats = s->mats.as<pg::DevMaterial>();
         ds.quads = s->quads.as<pg::DevQuad>();
         ds.numQuads = desc->num_quads;
+        ds.lights = s->lights.as<pg::DevAreaLight>();
+        ds.numLights = static_cast<uint32_t>(lights.size());
         ds.points = s->points.as<float>();
         ds.numPoints = desc->num_point_lights;
         upload(s->textures, texTab.data(), texTab.size() * sizeof(pg::DevTexture), st);

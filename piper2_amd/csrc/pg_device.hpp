@@ -1019,6 +1019,52 @@ struct LightSample {
     V3 dir, rad;
     float dist, pdf;
 };
+// select (Kernel/Sampling.hpp:52-63) over an area light's primitive CDF: the first primitive whose
+// CDF entry exceeds u (same bisection), then u rescaled into that primitive: by the CDF's lower
+// bound in spec mode, by its upper bound cdf[r] with D1 as the reference does. u = 1 (the Sobol
+// conversion has no clamp) picks the last primitive where the reference reads past its CDF.
+__device__ __forceinline__ uint32_t selectPrimitive(const DevScene& s, const DevAreaLight L, float& u, bool d1) {
+    uint32_t l = 0, r = L.count;
+    while (l < r) {
+        const uint32_t mid = (l + r) >> 1;
+        if (u >= s.quads[L.first + mid].cdf) l = mid + 1;
+        else r = mid;
+    }
+    r = min(r, L.count - 1u);
+    const float hiCdf = s.quads[L.first + r].cdf;
+    const float lower = d1 ? hiCdf : (r ? s.quads[L.first + r - 1u].cdf : 0.0f);
+    u = (u - lower) / s.quads[L.first + r].pdf;
+    return L.first + r;
+}
+
+// The primitive of area light `light` (0-based) that the ray meets first, by closestHit's quad
+// rule (smaller t, ties to the smaller index), or false.
+__device__ __forceinline__ bool firstLightPrimitive(const DevScene& s, uint32_t light, const Ray& r, float& tq,
+                                                    uint32_t& qi) {
+    const DevAreaLight L = s.lights[light];
+    bool hit = false;
+    tq = 1e5f;
+    for (uint32_t k = L.first; k < L.first + L.count; ++k) {
+        float t;
+        if (intersectQuad(r, s.quads[k], 1e-3f, tq, t)) tq = t, qi = k, hit = true;
+    }
+    return hit;
+}
+// No quad outside area light `light` meets the ray before quad qi at tq (closestHit's quad order).
+__device__ __forceinline__ bool noOtherLightFirst(const DevScene& s, uint32_t light, const Ray& r, float tq,
+                                                  uint32_t qi) {
+    const DevAreaLight L = s.lights[light];
+    for (uint32_t k = 0; k < s.numQuads; ++k) {
+        if (k == L.first) {
+            k += L.count - 1u;
+            continue;
+        }
+        float t2;
+        if (intersectQuad(r, s.quads[k], 1e-3f, 1e5f, t2) && (t2 < tq || (t2 == tq && k < qi))) return false;
+    }
+    return true;
+}
+
 __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t light, V3 hit, float u1, float u2,
                                                    bool d1) {
     LightSample ls{{0, 0, 0}, {0, 0, 0}, 0.0f, 0.0f};
@@ -1031,8 +1077,8 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
         ls.dist = 1e5f;
         return ls;
     }
-    if (light > s.numQuads) {  // pointSample (Light/BasicLight/Kernel.cpp:35-45): a delta light, pdf 1
-        const float* p = s.points + 6 * (light - 1u - s.numQuads);
+    if (light > s.numLights) {  // pointSample (Light/BasicLight/Kernel.cpp:35-45): a delta light, pdf 1
+        const float* p = s.points + 6 * (light - 1u - s.numLights);
         const V3 delta = V3{p[0], p[1], p[2]} - hit;
         const float dis2 = dot(delta, delta);
         ls.rad = {p[3] / dis2, p[4] / dis2, p[5] / dis2};
@@ -1041,8 +1087,8 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
         ls.pdf = 1.0f;
         return ls;
     }
-    const DevQuad& q = s.quads[light - 1];
-    const float uu = d1 ? (u1 - 1.0f) / 1.0f : u1;
+    float uu = u1;
+    const DevQuad& q = s.quads[selectPrimitive(s, s.lights[light - 1u], uu, d1)];
     const V3 src = ld3(q.origin) + ld3(q.u) * uu + ld3(q.v) * u2;
     V3 n = ld3(q.normal);
     if (dot(hit - src, n) <= 0.0f) n = -n;

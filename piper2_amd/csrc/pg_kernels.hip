@@ -303,11 +303,11 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 // sample's dimensions directly.
                 bool deltaSel = false;
                 if (!specV) {  // multipleImportanceSampling (:23-107) runs only if noSpecular
-                    // light selection: uniformSelect over {environment, quads, point lights}
-                    const uint32_t count = 1u + s.numQuads + s.numPoints;
+                    // light selection: uniformSelect over {environment, area lights, point lights}
+                    const uint32_t count = 1u + s.numLights + s.numPoints;
                     const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
                     const float selInv = 1.0f / (1.0f / static_cast<float>(count));
-                    deltaSel = sel > s.numQuads;
+                    deltaSel = sel > s.numLights;
                     if (deltaSel) used = 5u;
                     const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
                     if (ls.pdf > 0.0f && valid(ls.rad)) {
@@ -334,24 +334,18 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                                 MisRec{hp.x, hp.y, hp.z, bs.pdf, dir.x, dir.y, dir.z, sel, c.x, c.y, c.z, path};
                             pushMis = true;
                         } else if (sel != 0u) {
-                            // With a black environment the BSDF ray contributes only if the selected
-                            // quad is its closest hit: the quad must be hit, beat every other quad in
-                            // (t, index) order (closestHit's quad loop), and no triangle may lie in
-                            // (tmin, tq] (triangles win ties). The first two are decided here; the
-                            // triangle test becomes an occlusion query on the shadow queue.
+                            // With a black environment the BSDF ray contributes only if a primitive
+                            // of the selected light is its closest hit: that primitive must be the
+                            // light's first along the ray, beat every other light's quads in (t, index)
+                            // order (closestHit's quad loop), and no triangle may lie in (tmin, tq]
+                            // (triangles win ties). The first two are decided here; the triangle test
+                            // becomes an occlusion query on the shadow queue.
                             const Ray mray{hp, dir};
-                            const uint32_t qi = sel - 1u;
-                            const DevQuad& qd = s.quads[qi];
                             float tq;
-                            if (intersectQuad(mray, qd, 1e-3f, 1e5f, tq)) {
-                                bool first = true;
-                                for (uint32_t k = 0; k < s.numQuads && first; ++k) {
-                                    float t2;
-                                    if (k != qi && intersectQuad(mray, s.quads[k], 1e-3f, 1e5f, t2) &&
-                                        (t2 < tq || (t2 == tq && k < qi)))
-                                        first = false;
-                                }
-                                if (first) {
+                            uint32_t qi = 0;
+                            if (firstLightPrimitive(s, sel - 1u, mray, tq, qi)) {
+                                const DevQuad& qd = s.quads[qi];
+                                if (noOtherLightFirst(s, sel - 1u, mray, tq, qi)) {
                                     const V3 qn = quadNormal(qd, dir);
                                     const V3 rad = dot(qn, dir) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
                                     const float lpdf = tq * tq / (fabsf(dot(qn, dir)) * qd.area);
@@ -544,19 +538,15 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
         V3 rad{0.0f, 0.0f, 0.0f};
         float lpdf = 0.0f;
         if (envBlack && r.light != 0u) {
-            // With a black environment the ray contributes only if the selected quad is its
-            // closest hit. Same decision as closestHit(): the quad must be hit, beat every other
-            // quad in (t, index) order, and no triangle may lie in (tmin, t_q] (triangles win ties).
-            const uint32_t sel = r.light - 1u;
-            const DevQuad& qd = s.quads[sel];
+            // With a black environment the ray contributes only if a primitive of the selected
+            // light is its closest hit. Same decision as closestHit(): the light's first primitive
+            // along the ray must beat every other light's quads in (t, index) order, and no triangle
+            // may lie in (tmin, t_q] (triangles win ties).
             float tq;
-            if (intersectQuad(ray, qd, 1e-3f, 1e5f, tq)) {
-                bool first = true;
-                for (uint32_t k = 0; k < s.numQuads && first; ++k) {
-                    float t2;
-                    if (k != sel && intersectQuad(ray, s.quads[k], 1e-3f, 1e5f, t2) && (t2 < tq || (t2 == tq && k < sel)))
-                        first = false;
-                }
+            uint32_t qi = 0;
+            if (firstLightPrimitive(s, r.light - 1u, ray, tq, qi)) {
+                const DevQuad& qd = s.quads[qi];
+                const bool first = noOtherLightFirst(s, r.light - 1u, ray, tq, qi);
                 // tq > 0: the next float up makes the occlusion range (tmin, tq] inclusive
                 const float tqUp = __uint_as_float(__float_as_uint(tq) + 1u);
                 if (first && !occluded<W, MOTION>(s, ray, 1e-3f, tqUp, TravStack{lstack + lane, gs, ldsDepth}, time)) {
@@ -571,7 +561,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
             if (h.pos == kHitMiss) {
                 rad = ld3(s.env);
                 lpdf = envPdf(ray.d);
-            } else if ((h.pos & kHitQuad) && r.light == 1u + (h.pos & ~kHitQuad)) {
+            } else if ((h.pos & kHitQuad) && r.light == 1u + s.quads[h.pos & ~kHitQuad].light) {
                 const DevQuad& qd = s.quads[h.pos & ~kHitQuad];
                 const V3 qn = quadNormal(qd, ray.d);
                 rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};

@@ -67,10 +67,15 @@ struct DevMaterial {  // 32 B
 struct DevQuad {  // Plane primitive (PerPlaneData) + DiffuseAreaLightData, 80 B
     float origin[3], u[3], v[3];
     float normal[3];
-    float radiance[3];
-    float area;
+    float radiance[3];    // the owning light's radiance
+    float area;           // the owning light's area: every primitive of its Plane (DiffuseAreaLightData::area)
     uint32_t maxDetComp;  // PerPlaneData::maxDetComp, BasicShapes.cpp:70-75
-    uint32_t reserved[3];
+    uint32_t light;       // owning area light: quads of one light are consecutive
+    float cdf, pdf;       // this primitive's area CDF and probability within its light (BasicShapes.cpp:76-94)
+};
+// Area light: a Plane of `count` primitives starting at quad `first` (DiffuseAreaLight over Plane).
+struct DevAreaLight {
+    uint32_t first, count;
 };
 
 struct DevCamera {
@@ -113,6 +118,8 @@ struct DevScene {
     const uint64_t* sobolDelta;  // [kMaxLog2Res+1][64]
     uint32_t numTris, numNodes, numQuads, width;
     uint32_t numPoints;
+    const DevAreaLight* lights;  // area lights: lights are {environment, area lights, point lights}
+    uint32_t numLights;
     uint32_t textured;  // some material has a texture: shade takes its textured variant
     DevCamera cam;
     float env[3];
@@ -131,7 +138,7 @@ struct alignas(16) ShadowRec {  // 48 B
 struct alignas(16) MisRec {  // 48 B
     float ox, oy, oz, bsdfPdf;
     float dx, dy, dz;
-    uint32_t light;  // selected light (0 = environment, 1 + q = quad q)
+    uint32_t light;  // selected light (0 = environment, 1 + l = area light l)
     float cr, cg, cb;
     uint32_t path;
 };

@@ -18,8 +18,8 @@
 // Mesh and image paths are relative to the scene file. Colour and roughness inputs take
 // ConstantTexture, CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from PNG or binary
 // PPM / PGM / PAM with 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of
-// the reference that the HIP path does not implement (anisotropic roughness, multi-primitive area
-// lights, time-sampled transforms, other image formats) are rejected with a message.
+// the reference that the HIP path does not implement (anisotropic roughness, keyframes nested in
+// keyframes, other image formats) are rejected with a message.
 #include <array>
 #include <cctype>
 #include <cmath>
@@ -350,7 +350,9 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     if (!f) fail("cannot write " + out.string());
     f.write("PGSC", 4);
     const bool moving = !e.instanceFirst.empty();
-    const uint32_t version = moving ? 2 : 1;
+    bool multiPrimitive = false;  // some area light has several primitives: their counts end the file
+    for (uint32_t c : e.lightQuads) multiPrimitive = multiPrimitive || c != 1;
+    const uint32_t version = (moving ? 2u : 1u) | (multiPrimitive ? 4u : 0u);
     put(f, &version, 1);
     put(f, head, 12);
     put(f, e.verts.data(), e.verts.size());
@@ -373,11 +375,14 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     const TransformSRT id = identitySRT();
     for (uint32_t o = 0; o < no; ++o) put(f, reinterpret_cast<const float*>(&id), 16);
     const uint32_t np = static_cast<uint32_t>(e.points.size());
-    if (np || !e.textures.empty() || moving) {  // optional tail: point lights, then textures
+    // with moving instances or several-primitive lights the tail is explicit: point lights, a
+    // texture flag, the textures, the instances, the light counts
+    const bool explicitTail = moving || multiPrimitive;
+    if (np || !e.textures.empty() || explicitTail) {  // optional tail: point lights, then textures
         put(f, &np, 1);
         for (const pg_point_light& p : e.points) put(f, p.position, 3), put(f, p.intensity, 3);
     }
-    if (moving) {
+    if (explicitTail) {
         const uint32_t hasTextures = e.textures.empty() ? 0u : 1u;
         put(f, &hasTextures, 1);
     }
@@ -410,6 +415,11 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
             for (uint32_t k = 0; k < pm.num_keys; ++k) put(f, pm.keys[k], 16);
             put(f, pm.parent, 12);
         }
+    }
+    if (multiPrimitive) {  // primitives (consecutive quads) per area light
+        const uint32_t nl = static_cast<uint32_t>(e.lightQuads.size());
+        put(f, &nl, 1);
+        put(f, e.lightQuads.data(), nl);
     }
     if (!f) fail("cannot write " + out.string());
 }

@@ -500,17 +500,22 @@ public:
             const auto& shape = mReg.call(d.sample.handle, "AreaLight sample");
             if (shape.symbol != "planeSample") raise("AreaLight: only BasicShapes Plane geometry is supported");
             const auto cdf = unpack<CDFData>(shape.payload, "CDFData");
-            if (cdf.size != 1) raise("area lights with more than one plane primitive are not supported");
+            if (cdf.size == 0) raise("CDFData: a Plane without primitives");
             const auto& bytes = mReg.buffer(cdf.primitives, "CDFData primitives");
-            if (bytes.size() < sizeof(PerPlaneData)) raise("CDFData: primitive buffer too small");
-            PerPlaneData p;
-            std::memcpy(&p, bytes.data(), sizeof(p));
+            if (bytes.size() < cdf.size * sizeof(PerPlaneData)) raise("CDFData: primitive buffer too small");
+            // every primitive of the Plane, in order; the library rebuilds the area CDF from them as
+            // the Plane constructor does (BasicShapes.cpp:59-94)
             const Mat34& m = Collected::transformOf(cdf.traversal);
-            const auto o = applyPoint(m, &p.origin.x), u = applyVector(m, &p.u.x), v = applyVector(m, &p.v.x);
-            pg_quad q{};
-            for (int k = 0; k < 3; ++k) q.origin[k] = o[k], q.u[k] = u[k], q.v[k] = v[k];
-            q.radiance[0] = d.radiance.x, q.radiance[1] = d.radiance.y, q.radiance[2] = d.radiance.z;
-            out.quads.push_back(q);
+            for (uint32_t i = 0; i < cdf.size; ++i) {
+                PerPlaneData p;
+                std::memcpy(&p, bytes.data() + i * sizeof(PerPlaneData), sizeof(p));
+                const auto o = applyPoint(m, &p.origin.x), u = applyVector(m, &p.u.x), v = applyVector(m, &p.v.x);
+                pg_quad q{};
+                for (int k = 0; k < 3; ++k) q.origin[k] = o[k], q.u[k] = u[k], q.v[k] = v[k];
+                q.radiance[0] = d.radiance.x, q.radiance[1] = d.radiance.y, q.radiance[2] = d.radiance.z;
+                out.quads.push_back(q);
+            }
+            out.lightQuads.push_back(cdf.size);
         } else {
             raise("light: " + init + " is not a light program");
         }
@@ -793,6 +798,8 @@ public:
         d.num_materials = static_cast<uint32_t>(f.materials.size());
         d.quads = f.quads.data();
         d.num_quads = static_cast<uint32_t>(f.quads.size());
+        d.area_light_quads = f.lightQuads.empty() ? nullptr : f.lightQuads.data();
+        d.num_area_lights = static_cast<uint32_t>(f.lightQuads.size());
         d.point_lights = f.points.empty() ? nullptr : f.points.data();
         d.num_point_lights = static_cast<uint32_t>(f.points.size());
         if (!f.textures.empty()) {

@@ -662,6 +662,7 @@ struct SceneExport {
     std::vector<uint32_t> index, triObject, objectMaterial;
     std::vector<pg_material> materials;
     std::vector<pg_quad> quads;
+    std::vector<uint32_t> lightQuads;  // primitives (consecutive quads) of each area light
     std::vector<pg_point_light> points;
     std::array<float, 3> env{};
     std::vector<std::pair<const Node*, std::array<float, 12>>> sensors;

@@ -150,7 +150,7 @@ class _Builder:
 
 
 def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
-            specular=False, point_lights=False, textured=False, glass_roughness=0.0):
+            specular=False, point_lights=False, textured=False, glass_roughness=0.0, multi_light=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
@@ -163,8 +163,10 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     a checkerboard on the back wall, 8-bit images on the floor (repeat) and the left wall (mirror),
     a checkerboard roughness (0 and 0.6: Lambert and Oren-Nayar cells) on the ceiling and, with
     plastic=True, a checkerboard specular colour on the first box and an image roughness on the
-    second. The light and the rest
-    of the geometry stay as in the plain scene."""
+    second; multi_light=True replaces the ceiling light by one Plane light of three primitives of
+    unequal area (two halves of a 0.5 x 0.5 quad cut at 0.6 of its width and a 0.2 x 0.2 piece
+    beside them) and adds a second light of two strips high on the back wall ("area_lights" [3, 2]).
+    The light and the rest of the geometry stay as in the plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -211,12 +213,21 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
         v, t = _quad_tris(*mirror)
         b.add(v, t, b.material(MIRROR, [0.9, 0.9, 0.9]))
     quads = [[-0.25, 0.999, -0.25, 0.5, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0]]
+    if multi_light:
+        quads = [[-0.25, 0.999, -0.25, 0.3, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0],
+                 [0.05, 0.999, -0.25, 0.2, 0.0, 0.0, 0.0, 0.0, 0.5, 15.0, 15.0, 15.0],
+                 [-0.6, 0.999, 0.3, 0.2, 0.0, 0.0, 0.0, 0.0, 0.2, 15.0, 15.0, 15.0],
+                 [-0.8, 0.6, -0.999, 0.3, 0.0, 0.0, 0.0, 0.2, 0.0, 4.0, 5.0, 6.0],
+                 [0.5, 0.55, -0.999, 0.25, 0.0, 0.0, 0.0, 0.3, 0.0, 4.0, 5.0, 6.0]]
     aspect = width / height
     cam = make_camera([0, 0, 3.8], [0, 0, 0], [0, 1, 0], 40.0, aspect)
     ndc, _ = calc_render_rect(width, height, float(cam[9] / cam[10]))
     sc = b.finish(quads, cam, width=width, height=height, spp=spp, max_depth=max_depth, min_rr=max_depth,
                   ndc=ndc, name="cornell" + ("_plastic" if plastic else "") + ("_rough" if rough_walls else "")
-                  + ("_specular" if specular else "") + ("_points" if point_lights else ""))
+                  + ("_specular" if specular else "") + ("_points" if point_lights else "")
+                  + ("_multilight" if multi_light else ""))
+    if multi_light:
+        sc["area_lights"] = np.array([3, 2], np.uint32)
     if point_lights:
         sc["point_lights"] = np.array([[0.45, 0.6, 0.2, 1.5, 1.4, 1.2], [-0.55, -0.3, 0.45, 0.6, 0.7, 1.0]], np.float32)
     if textured:

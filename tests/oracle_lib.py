@@ -32,6 +32,8 @@ def lib():
                                           ctypes.c_float, _VP, _VP]
         L.or_trace_at.argtypes = [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, _VP]
         L.or_scene_set_points.argtypes = [_VP, _VP, ctypes.c_uint32]
+        L.or_scene_set_area_lights.argtypes = [_VP, _VP, ctypes.c_uint32]
+        L.or_scene_set_area_lights.restype = ctypes.c_int
         L.or_scene_set_textures.argtypes = [_VP, _VP, _VP]
         L.or_scene_add_texture.restype = ctypes.c_uint32
         L.or_scene_add_texture.argtypes = [_VP, _VP, _VP, _VP]
@@ -84,6 +86,10 @@ class OracleScene:
             lib().or_scene_set_instances(self.h, _p(self._first), len(self._first))
             for k, m in enumerate(sc.get("motions") or []):
                 self.set_motion(k, **m)
+        al = sc.get("area_lights")  # optional: primitives per area light (pg_scene_desc.area_light_quads)
+        if al is not None:
+            self._al = np.ascontiguousarray(al, np.uint32)
+            assert lib().or_scene_set_area_lights(self.h, _p(self._al), len(self._al)) == 0
         pts = sc.get("point_lights")
         if pts is not None and len(pts):
             self._pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)

@@ -149,11 +149,16 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
         if transforms and o in transforms:
             inst = {"Type": "Group", "Children": [dict(inst, Transform=transforms[o])]}
         children.append(inst)
-    for q in sc["quads"]:
+    counts = sc.get("area_lights")  # primitives per area light (one each by default)
+    counts = [1] * len(sc["quads"]) if counts is None else [int(c) for c in counts]
+    first = 0
+    for c in counts:
+        qs = sc["quads"][first:first + c]
+        first += c
         plane = {"ClassID": "Piper.BuiltinComponent.BasicShapes.Plane",
-                 "Primitives": [{"Origin": q[0:3].tolist(), "U": q[3:6].tolist(), "V": q[6:9].tolist()}]}
+                 "Primitives": [{"Origin": q[0:3].tolist(), "U": q[3:6].tolist(), "V": q[6:9].tolist()} for q in qs]}
         children.append({"Type": "Instance", "ClassID": "Piper.BuiltinComponent.AreaLight.Diffuse", "Geometry": plane,
-                         "Radiance": q[9:12].tolist()})
+                         "Radiance": qs[0][9:12].tolist()})
     for p in sc.get("point_lights", np.zeros((0, 6))):
         children.append({"Type": "Group", "Children": [{
             "Type": "Instance", "ClassID": "Piper.BuiltinComponent.BasicLight.Point", "Intensity": p[3:6].tolist(),
@@ -209,7 +214,9 @@ def read_pgsc(path):
     with open(path, "rb") as f:
         data = f.read()
     version = struct.unpack("<I", data[4:8])[0]
-    assert data[:4] == b"PGSC" and version in (1, 2)
+    assert data[:4] == b"PGSC" and version & 3 in (1, 2) and version < 8
+    multi_primitive = bool(version & 4)  # area lights of several primitives: their counts end the file
+    version &= 3
     head = np.frombuffer(data, np.uint32, 12, 8)
     nv, nt, no, nm, nq, w, h, spp, md, rr, compat, fit = (int(x) for x in head)
     off = 56
@@ -237,7 +244,7 @@ def read_pgsc(path):
         off += 4
         take("point_lights", np.float32, 6 * npt, (npt, 6))
         has_textures = off < len(data)
-    if version == 2:  # moving instances: an explicit texture flag, the instances after the textures
+    if version == 2 or multi_primitive:  # an explicit texture flag; instances, light counts after the textures
         has_textures = struct.unpack("<I", data[off:off + 4])[0] == 1
         off += 4
     if has_textures:
@@ -274,6 +281,10 @@ def read_pgsc(path):
             off += parent.nbytes
             motions.append(dict(keys=keys, t0=t0, t1=t1, visible=bool(vis), parent=parent))
         out["motions"] = motions
+    if multi_primitive:
+        nl = struct.unpack("<I", data[off:off + 4])[0]
+        off += 4
+        take("area_lights", np.uint32, nl, (nl,))
     assert off == len(data)
     out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
     return out

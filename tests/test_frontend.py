@@ -42,6 +42,16 @@ def test_export_reproduces_generated_scene(tmp_path):
     assert (e["width"], e["height"], e["spp"], e["max_depth"], e["min_rr"]) == (64, 48, 8, 5, 5)
 
 
+def test_multi_primitive_light_export(tmp_path):
+    # Plane lights with several Primitives keep their primitives in order and their grouping
+    sc = scenes.cornell(width=32, height=32, spp=4, multi_light=True)
+    e = export(sc, tmp_path)
+    assert np.array_equal(e["quads"], sc["quads"])
+    assert np.array_equal(e["area_lights"], [3, 2])
+    one = scenes.cornell(width=32, height=32, spp=4)
+    assert "area_lights" not in export(one, tmp_path / "one")  # one primitive per light: no counts
+
+
 @pytest.mark.parametrize("image_format,opaque", [("pam", False), ("png", False), ("png", True)])
 def test_textured_scene_export(tmp_path, image_format, opaque):
     """CheckBoard and sampled-image colour inputs (PAM or PNG files, Repeat and Mirror) and OBJ

@@ -234,6 +234,37 @@ def test_point_lights_paths_match_oracle(dev):
     _path_parity(sc, dev, flags=flags, spp=4)
 
 
+def test_multi_primitive_lights_paths_match_oracle(dev):
+    # two Plane lights of 3 and 2 primitives: the area pick of a primitive, the density over the
+    # light's whole area, and a BSDF ray that counts on any primitive of the selected light (in
+    # shade's black-environment query), with spec and D1 rescaling
+    sc = scenes.cornell(width=128, height=128, spp=16, multi_light=True, plastic=True)
+    _path_parity(sc, dev)
+    _path_parity(sc, dev, flags=pg.PG_COMPAT_D1 | pg.PG_COMPAT_D9, spp=4)
+
+
+def test_multi_primitive_lights_lit_environment_match_oracle(dev):
+    # a non-black environment sends the BSDF rays through the closest-hit MIS kernel instead
+    sc = scenes.cornell(width=64, height=64, spp=16, multi_light=True, point_lights=True)
+    sc["env"] = np.array([0.3, 0.4, 0.5], np.float32)
+    _path_parity(sc, dev)
+
+
+def test_area_light_counts_rejected(dev):
+    sc = scenes.cornell(width=16, height=16, spp=1, multi_light=True)
+    for bad in ([3, 3], [5, 0], [2, 2]):
+        sc2 = dict(sc)
+        sc2["area_lights"] = np.array(bad, np.uint32)
+        with pytest.raises(pg.PgError, match="area light primitive counts"):
+            pg.Scene(dev, sc2)
+    sc2 = dict(sc)
+    q = sc["quads"].copy()
+    q[1, 9] = 14.0
+    sc2["quads"] = q
+    with pytest.raises(pg.PgError, match="share its radiance"):
+        pg.Scene(dev, sc2)
+
+
 def test_textured_paths_match_oracle(dev):
     # checkerboard and 8-bit image textures (repeat and mirror) on walls and a box's specular colour
     sc = scenes.cornell(width=128, height=128, spp=16, textured=True, plastic=True, specular=True)

@@ -654,6 +654,91 @@ def test_direct_lighting_matches_polygon_irradiance():
     assert np.median(rel) < 0.02 and rel.mean() < 0.04, (np.median(rel), rel.mean())
 
 
+def _floor_under_light(quads, area_lights, rho=0.6, spp=64, flags=0):
+    b = scenes._Builder()
+    b.add([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], [[0, 1, 2], [0, 2, 3]], b.material(0, [rho] * 3))
+    cam = scenes.make_camera([0, 3, 0.01], [0, 0, 0], [0, 0, -1], 50, 1.0)
+    sc = b.finish(np.asarray(quads, np.float32), cam, width=32, height=32, spp=spp, max_depth=1, min_rr=1,
+                  ndc=(0.0, 0.0, 1.0, 1.0))
+    if area_lights is not None:
+        sc["area_lights"] = np.asarray(area_lights, np.uint32)
+    return sc
+
+
+def test_multi_primitive_light_matches_polygon_irradiance():
+    """One area light of three primitives of unequal area (a Plane with three Primitives): the light
+    picks a primitive by area and its density is over the whole light's area, so direct light at
+    depth 1 is rho/pi times the sum of the three polygons' irradiances. A density over one
+    primitive's own area, or a uniform primitive pick, fails by far more than the tolerance."""
+    rho, le = 0.6, 10.0
+    quads = [[-1.2, 1.0, -0.6, 0.9, 0.0, 0.0, 0.0, 0.0, 0.5, le, le, le],
+             [0.4, 1.3, 0.2, 0.3, 0.0, 0.0, 0.0, 0.1, 0.3, le, le, le],
+             [-0.3, 0.8, 0.5, 0.5, 0.2, 0.0, 0.0, 0.0, 0.6, le, le, le]]
+    sc = _floor_under_light(quads, [3], rho)
+    o = OracleScene(sc)
+    pl = o.render(path_l=True, film=False)["path_l"].reshape(32, 32, 64, 3)
+    pix = np.array([[x, y, s] for y in range(32) for x in range(32) for s in range(64)], np.uint32)
+    rays, _ = o.camera_rays(pix)
+    t, prim, _ = o.trace(rays)
+    hit = rays[:, :3] + rays[:, 3:6] * t[:, None]
+    floor = prim < 0x80000000
+    polys = [[np.array(q[0:3]) + np.array(q[3:6]) * a + np.array(q[6:9]) * c for a, c in ((0, 0), (1, 0), (1, 1), (0, 1))]
+             for q in quads]
+    n = np.array([0, 1.0, 0])
+    expect = np.array([rho / math.pi * sum(_polygon_irradiance(h.astype(np.float64), n, p, le) for p in polys)
+                       if f else 0.0 for h, f in zip(hit, floor)])
+    m = floor.reshape(32, 32, 64).all(-1)
+    e_pix = expect.reshape(32, 32, 64).mean(-1)[m]
+    g_pix = pl.reshape(-1, 3)[:, 0].reshape(32, 32, 64).mean(-1)[m]
+    assert m.sum() > 500
+    rel = np.abs(g_pix - e_pix) / e_pix
+    assert np.median(rel) < 0.03 and rel.mean() < 0.05, (np.median(rel), rel.mean())
+    # the same primitives as three lights of their own: the same expectation, a different estimator
+    sep = OracleScene(_floor_under_light(quads, None, rho)).render(path_l=True, film=False)["path_l"]
+    s_pix = sep.reshape(32, 32, 64, 3)[..., 0].mean(-1)[m]
+    rel2 = np.abs(s_pix - e_pix) / e_pix
+    assert np.median(rel2) < 0.03, np.median(rel2)
+
+
+@pytest.mark.parametrize("split", [(0.3, 0.7), (0.5, 0.25, 0.25)])
+def test_split_light_renders_as_the_whole_light(split):
+    """A light quad cut along U into primitives of fractions f_i of one Plane light: the area pick
+    with the lower-bound rescale maps the light sample u1 to the same point of the whole quad, the
+    density is over the same total area, and a BSDF ray that meets any piece counts for the selected
+    light, so every path's radiance equals the uncut light's up to rounding (Cornell box, 5 bounces).
+    A rescale by the upper bound (defect D1), a density over one piece, or a BSDF ray that counts only
+    on the sampled piece fails it."""
+    sc = scenes.cornell(width=24, height=24, spp=16)
+    q = sc["quads"][0].astype(np.float64)
+    pieces, acc = [], 0.0
+    for f in split:
+        piece = q.copy()
+        piece[0:3] = q[0:3] + q[3:6] * acc
+        piece[3:6] = q[3:6] * f
+        pieces.append(piece)
+        acc += f
+    cut = dict(sc)
+    cut["quads"] = np.asarray(pieces, np.float32)
+    cut["area_lights"] = np.array([len(split)], np.uint32)
+    a = OracleScene(sc).render(path_l=True, film=False, threads=4)["path_l"]
+    b = OracleScene(cut).render(path_l=True, film=False, threads=4)["path_l"]
+    close = np.all(np.abs(a - b) <= 2e-4 * np.abs(a) + 1e-6, axis=1)
+    assert close.mean() > 0.995, close.mean()
+    assert abs(b.mean() - a.mean()) <= 1e-3 * a.mean()
+    # D1 rescales by the upper CDF bound: the uncut light's points move by -U, the pieces' by -U_i
+    d1 = 1
+    a1 = OracleScene(sc).render(path_l=True, film=False, threads=4, flags=d1)["path_l"]
+    b1 = OracleScene(cut).render(path_l=True, film=False, threads=4, flags=d1)["path_l"]
+    assert np.mean(np.all(np.abs(a1 - b1) <= 2e-4 * np.abs(a1) + 1e-6, axis=1)) < 0.9
+
+
+def test_area_light_counts_must_cover_the_quads():
+    sc = _floor_under_light([[-0.5, 1.0, -0.25, 1.0, 0, 0, 0, 0, 0.5, 1, 1, 1]] * 2, None)
+    o = OracleScene(sc)
+    bad = np.array([3], np.uint32)
+    assert lib().or_scene_set_area_lights(o.h, bad.ctypes.data, 1) == -1
+
+
 def test_point_light_inverse_square_closed_form():
     """max_depth 1 with one point light over a Lambert plane: the light is picked with probability
     1/2 (the black environment is the other light) and, as a delta light, its sample is not MIS
