@@ -147,6 +147,14 @@ typedef struct pg_scene_desc {
      * and its density is over the whole light's area. NULL: every quad is a light of its own. */
     const uint32_t* area_light_quads;
     uint32_t num_area_lights;
+    /* anisotropic roughness (Glass and Substrate take RoughnessX and RoughnessY, Classic.cpp:98-117,
+     * 177-195, Classic/Kernel.cpp:630-655, 769-773; pg_material.alpha and the alpha texture are
+     * RoughnessX): per material its RoughnessY constant (NULL: pg_material.alpha) and a texture for it
+     * (PG_TEX_NONE: the constant; NULL: none). Both NULL: every RoughnessY is its RoughnessX, texture
+     * included. Glass is smooth only where both roughnesses are zero. Other surfaces have one
+     * roughness: their RoughnessY must equal pg_material.alpha, with no texture. */
+    const float* roughness_y;
+    const uint32_t* roughness_y_textures;
 } pg_scene_desc;
 
 #define PG_BVH_HOST 0u   /* binned SAH on the host's cores (multithreaded) */

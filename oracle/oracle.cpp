@@ -17,6 +17,7 @@
 // GGX sampling uses exact visible-normal sampling with its exact pdf in both modes (the reference's
 // pdf omits D(wh), Surface/Classic/Kernel.cpp:405-412, so its plain estimator is biased).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -179,7 +180,13 @@ enum : uint32_t { kMatLambert = 0, kMatConductor = 1, kMatPlastic = 2, kMatMirro
 struct Material {
     uint32_t type;
     V3 albedo;          // Lambert rho, conductor F0, plastic diffuse
-    float alpha;        // GGX alpha (plastic: remapRoughness of the scene's roughness)
+    float alpha;        // GGX alpha (plastic: remapRoughness of the scene's roughness; glass, substrate:
+                        // of RoughnessX)
+    float alphaY = 0.0f;  // GGX alpha along the shading frame's y axis (glass, substrate: of
+                          // RoughnessY; every other surface: alpha)
+    float roughX = 0.0f, roughY = 0.0f;  // glass, substrate: RoughnessX / RoughnessY as configured
+    uint32_t texY = ~0u;  // glass, substrate: texture of RoughnessY (~0u: the constant)
+    bool ownY = false;    // RoughnessY given on its own (or_scene_set_roughness_y), not RoughnessX's
     V3 spec{0, 0, 0};   // plastic / substrate specular, glass transmission
     bool front = true;  // plastic: the hit is on the triangle's front face
     float onA = 1.0f, onB = 0.0f;  // matte with roughness > 0: Oren-Nayar A and B
@@ -782,25 +789,43 @@ struct BsdfSample {
     uint32_t part = kPartReflection;
 };
 
-// GGX (Trowbridge-Reitz) isotropic distribution, Classic/Kernel.cpp:289-382, alpha = roughness^2.
-float ggxLambda(V3 w, float alpha) {  // Kernel.cpp:326-341
+// GGX (Trowbridge-Reitz) distribution, Classic/Kernel.cpp:289-382 (MicrofacetDistribution with
+// alphaX, alphaY; the conductor and plastic pass one alpha twice). The azimuth's cosine and sine come
+// from w over sin(theta), clamped to [-1, 1], as the reference computes them.
+struct Dist {
+    float ax, ay;
+};
+Dist dist(const Material& m) { return {m.alpha, m.alphaY}; }
+void azimuth(V3 w, float sin2, float& cosPhi, float& sinPhi) {
+    const float st = std::sqrt(sin2);
+    cosPhi = st == 0.0f ? 1.0f : std::fmin(1.0f, std::fmax(-1.0f, w.x / st));
+    sinPhi = st == 0.0f ? 0.0f : std::fmin(1.0f, std::fmax(-1.0f, w.y / st));
+}
+float ggxLambda(V3 w, Dist d) {  // Kernel.cpp:326-341
     const float cos2 = w.z * w.z;
     const float sin2 = std::fmax(0.0f, 1.0f - cos2);
     const float tan2 = sin2 / cos2;
     if (std::isinf(tan2)) return 0.0f;
-    return (-1.0f + std::sqrt(1.0f + alpha * alpha * tan2)) * 0.5f;
+    float cp, sp;
+    azimuth(w, sin2, cp, sp);
+    const float p1 = cp * d.ax, p2 = sp * d.ay;
+    return (-1.0f + std::sqrt(1.0f + (p1 * p1 + p2 * p2) * tan2)) * 0.5f;
 }
-float ggxD(V3 wh, float alpha) {  // Kernel.cpp:342-357
+float ggxD(V3 wh, Dist d) {  // Kernel.cpp:342-357
     const float cos2 = wh.z * wh.z;
     const float sin2 = std::fmax(0.0f, 1.0f - cos2);
     const float tan2 = sin2 / cos2;
     if (std::isinf(tan2)) return 0.0f;
-    const float e = cos2 * (1.0f + tan2 / (alpha * alpha));
-    return kInvPi / (alpha * alpha * e * e);
+    float cp, sp;
+    azimuth(wh, sin2, cp, sp);
+    const float p1 = cp / d.ax, p2 = sp / d.ay;
+    const float e = cos2 * (1.0f + (p1 * p1 + p2 * p2) * tan2);
+    return kInvPi / (d.ax * d.ay * e * e);
 }
-// Visible-normal sampling (Heitz, "Sampling the GGX Distribution of Visible Normals", JCGT 2018).
-V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2) {
-    const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
+// Visible-normal sampling (Heitz, "Sampling the GGX Distribution of Visible Normals", JCGT 2018):
+// the view direction stretched by (ax, ay), a point of the projected hemisphere, unstretched.
+V3 ggxSampleWh(V3 wo, Dist d, float u1, float u2) {
+    const V3 vh = normalize(V3{d.ax * wo.x, d.ay * wo.y, wo.z});
     const float lensq = vh.x * vh.x + vh.y * vh.y;
     V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / std::sqrt(lensq)) : V3{1.0f, 0.0f, 0.0f};
     const V3 t2 = cross(vh, t1);
@@ -811,7 +836,7 @@ V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2) {
     const float sm = 0.5f * (1.0f + vh.z);
     p2 = (1.0f - sm) * std::sqrt(std::fmax(0.0f, 1.0f - p1 * p1)) + sm * p2;
     const V3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::fmax(0.0f, 1.0f - p1 * p1 - p2 * p2));
-    return normalize(V3{alpha * nh.x, alpha * nh.y, std::fmax(0.0f, nh.z)});
+    return normalize(V3{d.ax * nh.x, d.ay * nh.y, std::fmax(0.0f, nh.z)});
 }
 V3 schlick(V3 f0, float c) {
     const float m = 1.0f - std::fmin(1.0f, std::fmax(0.0f, c));
@@ -822,14 +847,14 @@ V3 schlick(V3 f0, float c) {
 V3 conductorF(const Material& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
     const V3 wh = normalize(wo + wi);
-    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-    const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+    const float g = 1.0f + ggxLambda(wo, dist(m)) + ggxLambda(wi, dist(m));
+    const float k = ggxD(wh, dist(m)) / (4.0f * g * std::fabs(wo.z * wi.z));
     return schlick(m.albedo, dot(wi, wh)) * k;
 }
 float conductorPdf(const Material& m, V3 wo, V3 wi) {  // D(wh) G1(wo) / (4 cos(theta_o))
     if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
     const V3 wh = normalize(wo + wi);
-    return ggxD(wh, m.alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, m.alpha)));
+    return ggxD(wh, dist(m)) / (4.0f * wo.z * (1.0f + ggxLambda(wo, dist(m))));
 }
 
 // MicrofacetDistribution::remapRoughness, Classic/Kernel.cpp:378-381 (a quartic in log roughness).
@@ -914,7 +939,7 @@ V3 blendF(const Material& m, const V3& rd, V3 wo, V3 wi) {
     const float c = dot(wi, wh);
     const float tail = fifth(1.0f - c);  // Schlick: rs + (1 - rs) (1 - cos)^5
     const V3 fres{rs.x + oneMinus.x * tail, rs.y + oneMinus.y * tail, rs.z + oneMinus.z * tail};
-    const float spec = ggxD(wh, m.alpha) / (4.0f * std::fmax(std::fabs(wi.z), std::fabs(wo.z)) * std::fabs(c));
+    const float spec = ggxD(wh, dist(m)) / (4.0f * std::fmax(std::fabs(wi.z), std::fabs(wo.z)) * std::fabs(c));
     return mul(rd, oneMinus) * shape + fres * spec;
 }
 // Half cosine, half visible-normal GGX (the reference's pdf divides by 1 + Lambda(wo) + Lambda(wh),
@@ -922,7 +947,7 @@ V3 blendF(const Material& m, const V3& rd, V3 wo, V3 wi) {
 float blendPdf(const Material& m, V3 wo, V3 wi) {
     if (wi.z <= 0.0f) return 0.0f;
     const V3 wh = normalize(wo + wi);
-    return 0.5f * (std::fabs(wi.z) * kInvPi + ggxD(wh, m.alpha) / (4.0f * std::fabs(wo.z) * (1.0f + ggxLambda(wo, m.alpha))));
+    return 0.5f * (std::fabs(wi.z) * kInvPi + ggxD(wh, dist(m)) / (4.0f * std::fabs(wo.z) * (1.0f + ggxLambda(wo, dist(m)))));
 }
 // Rough glass. Reflection: the GGX lobe with the dielectric Fresnel of the face (eta 1.5).
 // Transmission: Walter et al. 2007's microfacet BTDF with the generalized half vector
@@ -941,7 +966,7 @@ float glassTPdf(const Material& m, V3 wo, V3 wi) {
     const float ow = dot(wo, h), iw = dot(wi, h);
     if (!(ow > 0.0f && iw < 0.0f)) return 0.0f;
     const float e = glassEta(m), den = ow + e * iw;
-    const float visible = ggxD(h, m.alpha) * std::fabs(ow) / (wo.z * (1.0f + ggxLambda(wo, m.alpha)));
+    const float visible = ggxD(h, dist(m)) * std::fabs(ow) / (wo.z * (1.0f + ggxLambda(wo, dist(m))));
     return visible * std::fabs(e * e * iw / (den * den));
 }
 V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
@@ -953,15 +978,15 @@ V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
         if (!(ow > 0.0f && iw < 0.0f)) return {0.0f, 0.0f, 0.0f};  // Walter's chi+(i.h / i.n)
         const float fr = m.front ? fresnelDielectric(ow, 1.0f, 1.5f) : fresnelDielectric(ow, 1.5f, 1.0f);
         const float den = ow + glassEta(m) * iw;
-        const float g = 1.0f / (1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha));
-        return b.scale * ((1.0f - fr) * ggxD(h, m.alpha) * g * std::fabs(iw) * std::fabs(ow) /
+        const float g = 1.0f / (1.0f + ggxLambda(wo, dist(m)) + ggxLambda(wi, dist(m)));
+        return b.scale * ((1.0f - fr) * ggxD(h, dist(m)) * g * std::fabs(iw) * std::fabs(ow) /
                           (std::fabs(wo.z * wi.z) * den * den));
     }
     if (b.kind == kLobeGlassR) {
         if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
         const V3 wh = normalize(wo + wi);
-        const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-        const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+        const float g = 1.0f + ggxLambda(wo, dist(m)) + ggxLambda(wi, dist(m));
+        const float k = ggxD(wh, dist(m)) / (4.0f * g * std::fabs(wo.z * wi.z));
         const float fr = m.front ? fresnelDielectric(dot(wi, wh), 1.0f, 1.5f) : fresnelDielectric(dot(wi, wh), 1.5f, 1.0f);
         return b.scale * (fr * k);
     }
@@ -971,8 +996,8 @@ V3 lobeF(const Material& m, const Lobe& b, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
     if (b.kind == kLobeConductor) return conductorF(m, wo, wi);
     const V3 wh = normalize(wo + wi);  // MicrofacetReflection::evaluate, Kernel.cpp:398-404
-    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-    const float k = ggxD(wh, m.alpha) / (4.0f * g * std::fabs(wo.z * wi.z));
+    const float g = 1.0f + ggxLambda(wo, dist(m)) + ggxLambda(wi, dist(m));
+    const float k = ggxD(wh, dist(m)) / (4.0f * g * std::fabs(wo.z * wi.z));
     const float fr = m.front ? fresnelDielectric(dot(wi, wh), 1.0f, 1.46f) : fresnelDielectric(dot(wi, wh), 1.46f, 1.0f);
     return b.scale * (fr * k);
 }
@@ -1016,7 +1041,7 @@ BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u
     }
     if (b.kind == kLobeGlassT) {  // a visible microfacet normal h, then Snell's law about h
         if (wo.z <= 0.0f) return s;
-        const V3 h = ggxSampleWh(wo, m.alpha, u1, u2);
+        const V3 h = ggxSampleWh(wo, dist(m), u1, u2);
         const float c = dot(wo, h);
         if (c < 0.0f) return s;
         const float eta = 1.0f / glassEta(m);  // eta_i / eta_t
@@ -1033,7 +1058,7 @@ BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u
         if (u1 < 0.5f) {
             s.wi = cosineHemisphere(std::fmin(2.0f * u1, 1.0f), u2);
         } else {
-            const V3 wh = ggxSampleWh(wo, m.alpha, std::fmin(2.0f * (u1 - 0.5f), 1.0f), u2);
+            const V3 wh = ggxSampleWh(wo, dist(m), std::fmin(2.0f * (u1 - 0.5f), 1.0f), u2);
             s.wi = wh * (2.0f * dot(wo, wh)) - wo;
             if (s.wi.z <= 0.0f) return s;
         }
@@ -1048,7 +1073,7 @@ BsdfSample lobeSample(const Material& m, const Lobe& b, V3 wo, float u1, float u
         return s;
     }
     if (wo.z <= 0.0f) return s;  // MicrofacetReflection::sample, Kernel.cpp:413-433
-    const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+    const V3 wh = ggxSampleWh(wo, dist(m), u1, u2);
     const float c = dot(wo, wh);
     if (c < 0.0f) return s;
     s.wi = wh * (2.0f * c) - wo;  // reflect, Transform.hpp:405-408
@@ -1133,9 +1158,9 @@ std::array<float, 4> sampleTexture(const Texture& t, float u, float v) {
 // The material at a triangle hit with its textures applied at the hit's texture coordinate
 // (corner 1 weighted by u, corner 2 by v, corner 0 by 1 - u - v).
 void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
-    if (matId >= s.matTex.size()) return;
-    const auto& b = s.matTex[matId];
-    if (b[0] == ~0u && b[1] == ~0u && b[2] == ~0u) return;
+    static const std::array<uint32_t, 3> kNone{~0u, ~0u, ~0u};
+    const auto& b = matId < s.matTex.size() ? s.matTex[matId] : kNone;
+    if (b[0] == ~0u && b[1] == ~0u && b[2] == ~0u && m.texY == ~0u) return;
     float tu = 0.0f, tv = 0.0f;
     if (!s.triUV.empty()) {
         const float* c = &s.triUV[6 * static_cast<size_t>(h.prim)];
@@ -1151,8 +1176,8 @@ void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
         const auto c = sampleTexture(s.textures[b[1]], tu, tv);
         m.spec = {c[0], c[1], c[2]};
     }
-    if (b[2] != ~0u) {  // the roughness at this hit, read as each surface's init reads its config
-        const float r = sampleTexture(s.textures[b[2]], tu, tv)[0];
+    if (b[2] != ~0u || m.texY != ~0u) {  // the roughness at this hit, read as each surface's init reads its config
+        const float r = b[2] != ~0u ? sampleTexture(s.textures[b[2]], tu, tv)[0] : m.roughX;
         if (m.type == kMatLambert) {
             m.orenNayar = r > 0.0f;
             if (m.orenNayar) {
@@ -1161,11 +1186,16 @@ void applyTextures(const Scene& s, uint32_t matId, const Hit& h, Material& m) {
                 m.onB = 0.45f * s2 / (s2 + 0.09f);
             }
         } else if (m.type == kMatConductor) {
-            m.alpha = std::fmax(r * r, 1e-7f);
-        } else if (m.type == kMatPlastic || m.type == kMatSubstrate) {
-            m.alpha = remapRoughness(r);
-        } else if (m.type == kMatGlass) {
-            m.alpha = r > 0.0f ? remapRoughness(r) : 0.0f;
+            m.alpha = m.alphaY = std::fmax(r * r, 1e-7f);
+        } else if (m.type == kMatPlastic) {
+            m.alpha = m.alphaY = remapRoughness(r);
+        } else if (m.type == kMatGlass || m.type == kMatSubstrate) {
+            // RoughnessY from its own texture or constant; glassInit (Kernel.cpp:630-655) is smooth only
+            // when both are zero, substrateInit (:769-773) always takes the remapped pair
+            const float ry = m.texY != ~0u ? sampleTexture(s.textures[m.texY], tu, tv)[0] : m.roughY;
+            const bool rough = m.type == kMatSubstrate || r > 0.0f || ry > 0.0f;
+            m.alpha = rough ? remapRoughness(r) : 0.0f;
+            m.alphaY = rough ? remapRoughness(ry) : 0.0f;
         }
     }
 }
@@ -1483,6 +1513,8 @@ Material materialOf(const float* m) {
     Material r{static_cast<uint32_t>(m[0]), {m[1], m[2], m[3]}, m[4]};
     if (r.type == kMatPlastic || r.type == kMatSubstrate || (r.type == kMatGlass && m[4] > 0.0f))
         r.alpha = remapRoughness(m[4]);
+    r.alphaY = r.alpha;  // isotropic until or_scene_set_roughness_y gives glass or substrate a RoughnessY
+    r.roughX = r.roughY = m[4];
     if (r.type == kMatPlastic || r.type == kMatSubstrate || r.type == kMatGlass) r.spec = {m[5], m[6], m[7]};
     if (r.type == kMatLambert && m[4] > 0.0f) {  // OrenNayar(reflection, sigma), Kernel.cpp:99-106
         const float sigma = m[4] * kPi, s2 = sigma * sigma;
@@ -1728,6 +1760,33 @@ void or_bsdf_sample(const float* mat, int front, const float* wo, float u1, floa
     f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
     *pdf = s.pdf;
 }
+// The same with RoughnessY = roughY for glass and substrate (their anisotropic distribution).
+Material anisotropicOf(const float* mat, float roughY) {
+    Material m = materialOf(mat);
+    if (m.type != kMatGlass && m.type != kMatSubstrate) return m;
+    m.roughY = roughY;
+    const bool rough = m.type == kMatSubstrate || m.roughX > 0.0f || roughY > 0.0f;
+    m.alpha = rough ? remapRoughness(m.roughX) : 0.0f;
+    m.alphaY = rough ? remapRoughness(roughY) : 0.0f;
+    return m;
+}
+void or_bsdf_eval_y(const float* mat, float roughY, int front, const float* wo, const float* wi, float* f, float* pdf) {
+    Material m = anisotropicOf(mat, roughY);
+    m.front = front != 0;
+    const V3 o{wo[0], wo[1], wo[2]}, i{wi[0], wi[1], wi[2]};
+    const V3 r = bsdfEval(m, o, i, V3{0, 0, 1});
+    f[0] = r.x, f[1] = r.y, f[2] = r.z;
+    *pdf = bsdfPdf(m, o, i);
+}
+void or_bsdf_sample_y(const float* mat, float roughY, int front, const float* wo, float u1, float u2, float* wi, float* f,
+                      float* pdf) {
+    Material m = anisotropicOf(mat, roughY);
+    m.front = front != 0;
+    const BsdfSample s = bsdfSample(m, V3{wo[0], wo[1], wo[2]}, V3{0, 0, 1}, u1, u2);
+    wi[0] = s.wi.x, wi[1] = s.wi.y, wi[2] = s.wi.z;
+    f[0] = s.f.x, f[1] = s.f.y, f[2] = s.f.z;
+    *pdf = s.pdf;
+}
 // Area lights of several primitives: light l is the next counts[l] quads (sum = number of quads);
 // without this call every quad is a light of its own. As the Plane constructor
 // (BasicShapes.cpp:59-94): the light's area is the float sum of its primitives' |U x V|, the CDF
@@ -1761,6 +1820,26 @@ int or_scene_set_area_lights(void* scene, const uint32_t* counts, uint32_t n) {
     }
     return 0;
 }
+// Anisotropic roughness of glass and substrate (RoughnessX is the mats row's roughness): ry[nmat]
+// RoughnessY per material (or NULL: equal to RoughnessX), texY[nmat] its texture (~0u: the constant;
+// or NULL). Returns -1 if a surface other than glass or substrate is given a second roughness.
+int or_scene_set_roughness_y(void* scene, const float* ry, const uint32_t* texY) {
+    auto* s = static_cast<Scene*>(scene);
+    for (size_t i = 0; i < s->mats.size(); ++i) {
+        Material& m = s->mats[i];
+        const float y = ry ? ry[i] : m.roughX;
+        const uint32_t t = texY ? texY[i] : ~0u;
+        const bool two = m.type == kMatGlass || m.type == kMatSubstrate;
+        if (!two && (y != m.roughX || t != ~0u)) return -1;
+        if (!two) continue;
+        m.roughY = y, m.texY = t, m.ownY = true;
+        // glassInit: smooth (alpha 0) only when both roughnesses are zero
+        const bool rough = m.type == kMatSubstrate || m.roughX > 0.0f || y > 0.0f;
+        m.alpha = rough ? remapRoughness(m.roughX) : 0.0f;
+        m.alphaY = rough ? remapRoughness(y) : 0.0f;
+    }
+    return 0;
+}
 // points[6*n] = {position xyz, intensity rgb}; light order: environment, area lights, points.
 void or_scene_set_points(void* scene, const float* points, uint32_t n) {
     auto* s = static_cast<Scene*>(scene);
@@ -1776,7 +1855,11 @@ void or_scene_set_textures(void* scene, const float* triuv, const uint32_t* mtex
     auto* s = static_cast<Scene*>(scene);
     s->triUV.assign(triuv, triuv ? triuv + 6 * s->tris.size() : triuv);
     s->matTex.resize(s->mats.size());
-    for (size_t i = 0; i < s->mats.size(); ++i) s->matTex[i] = {mtex[3 * i], mtex[3 * i + 1], mtex[3 * i + 2]};
+    for (size_t i = 0; i < s->mats.size(); ++i) {
+        s->matTex[i] = {mtex[3 * i], mtex[3 * i + 1], mtex[3 * i + 2]};
+        Material& m = s->mats[i];  // an isotropic glass or substrate: RoughnessY follows RoughnessX's texture
+        if (!m.ownY && (m.type == kMatGlass || m.type == kMatSubstrate)) m.texY = mtex[3 * i + 2];
+    }
 }
 // itype {type, channels, width, height, wrap}; ftype {black[4], white[4], scale}
 uint32_t or_scene_add_texture(void* scene, const uint32_t* itype, const float* ftype, const uint8_t* texels) {
@@ -1796,8 +1879,15 @@ void or_texture_sample(void* scene, uint32_t tex, float u, float v, float* out) 
 }
 float or_fresnel_dielectric(float cosI, float etaI, float etaT) { return fresnelDielectric(cosI, etaI, etaT); }
 float or_remap_roughness(float roughness) { return remapRoughness(roughness); }
+// the anisotropic D at a unit half vector (x, y, z)
+float or_ggx_d_aniso(float x, float y, float z, float ax, float ay) { return ggxD(V3{x, y, z}, Dist{ax, ay}); }
+float or_ggx_lambda(float x, float y, float z, float ax, float ay) { return ggxLambda(V3{x, y, z}, Dist{ax, ay}); }
+void or_ggx_sample_wh(const float* wo, float ax, float ay, float u1, float u2, float* wh) {
+    const V3 h = ggxSampleWh(V3{wo[0], wo[1], wo[2]}, Dist{ax, ay}, u1, u2);
+    wh[0] = h.x, wh[1] = h.y, wh[2] = h.z;
+}
 float or_ggx_d(float cosTheta, float alpha) {
     const float s = std::sqrt(std::fmax(0.0f, 1.0f - cosTheta * cosTheta));
-    return ggxD(V3{s, 0.0f, cosTheta}, alpha);
+    return ggxD(V3{s, 0.0f, cosTheta}, Dist{alpha, alpha});
 }
 }

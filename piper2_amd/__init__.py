@@ -74,6 +74,7 @@ class SceneDesc(ctypes.Structure):
         ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("num_textures", ctypes.c_uint32),
         ("material_textures", ctypes.c_void_p),
         ("area_light_quads", ctypes.c_void_p), ("num_area_lights", ctypes.c_uint32),
+        ("roughness_y", ctypes.c_void_p), ("roughness_y_textures", ctypes.c_void_p),
     ]
 
 
@@ -570,6 +571,14 @@ def scene_desc(sc):
     al = sc.get("area_lights")
     if al is not None:
         al = sc["_area_lights"] = np.ascontiguousarray(al, np.uint32).reshape(-1)
+    # optional anisotropic roughness: "roughness_y" (M,) float32 RoughnessY per material (glass and
+    # substrate; materials[:, 4] is RoughnessX) and "roughness_y_textures" (M,) uint32
+    ry = sc.get("roughness_y")
+    if ry is not None:
+        ry = sc["_roughness_y"] = np.ascontiguousarray(ry, np.float32).reshape(-1)
+    ryt = sc.get("roughness_y_textures")
+    if ryt is not None:
+        ryt = sc["_roughness_y_textures"] = np.ascontiguousarray(ryt, np.uint32).reshape(-1)
     c = sc["camera"]  # 14 floats: pos, lookAt, up, sizeX, sizeY, focal, aperture, 0
     cam = Camera(F3(*c[0:3]), F3(*c[3:6]), F3(*c[6:9]), (ctypes.c_float * 2)(c[9], c[10]), float(c[11]), float(c[12]))
     return SceneDesc(
@@ -579,7 +588,8 @@ def scene_desc(sc):
         cam, F3(*sc["env"]), _ptr(pts) if pts is not None else None, 0 if pts is None else pts.shape[0],
         _ptr(texcoords) if texcoords is not None else None, ctypes.cast(tex_arr, ctypes.c_void_p) if texs else None,
         len(texs), _ptr(mtex) if mtex is not None else None, _ptr(al) if al is not None else None,
-        0 if al is None else al.shape[0])
+        0 if al is None else al.shape[0], _ptr(ry) if ry is not None else None,
+        _ptr(ryt) if ryt is not None else None)
 
 
 def render_params(sc, spp=None, rect=None, spp_range=None, flags=0, wavefront_paths=0, film_filter=0,

@@ -1313,6 +1313,11 @@ int pg_debug_inject_fault(pg_device* dev) {
 }
 
 namespace {
+float remapRoughness(float roughness) {  // MicrofacetDistribution::remapRoughness, Classic/Kernel.cpp:378-381
+    const float x = std::log(std::fmax(1e-3f, roughness));
+    return std::fmax(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+}
+
 int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* instanceFirst, uint32_t numInstances,
                 pg_scene** out) {
     return guarded([&] {
@@ -1369,11 +1374,19 @@ int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* insta
             uint32_t type = m.type;
             float alpha = m.alpha;
             float spec[3] = {m.reserved[0], m.reserved[1], m.reserved[2]};
-            if (m.type == PG_MAT_GLASS && m.alpha > 0.0f) type = pg::kDevRoughGlass;  // glassInit :641-655
+            const bool twoRough = m.type == PG_MAT_GLASS || m.type == PG_MAT_SUBSTRATE;  // RoughnessX, RoughnessY
+            const float roughY = desc->roughness_y ? desc->roughness_y[i] : m.alpha;
+            const uint32_t texY = desc->roughness_y_textures ? desc->roughness_y_textures[i] : PG_TEX_NONE;
+            require(twoRough || (roughY == m.alpha && texY == PG_TEX_NONE),
+                    "only glass and substrate have a second roughness");
+            require(m.type != PG_MAT_GLASS || roughY >= 0.0f, "glass roughness must not be negative");
+            require(texY == PG_TEX_NONE || texY < desc->num_textures, "material texture index out of range");
+            float alphaY = alpha;
+            // glassInit (Classic/Kernel.cpp:630-655): smooth only when both roughnesses are zero
+            if (m.type == PG_MAT_GLASS && (m.alpha > 0.0f || roughY > 0.0f)) type = pg::kDevRoughGlass;
             if (m.type == PG_MAT_PLASTIC || m.type == PG_MAT_SUBSTRATE || type == pg::kDevRoughGlass) {
-                // remapRoughness (Classic/Kernel.cpp:378-381)
-                const float x = std::log(std::fmax(1e-3f, m.alpha));
-                alpha = std::fmax(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+                alpha = remapRoughness(m.alpha);  // MicrofacetDistribution::remapRoughness
+                alphaY = twoRough ? remapRoughness(roughY) : alpha;
             } else if (m.type == PG_MAT_LAMBERT && m.alpha > 0.0f) {
                 // matteInit (Classic/Kernel.cpp:260-266) picks OrenNayar; its constructor (:99-106)
                 // takes sigma = roughness x pi radians
@@ -1383,7 +1396,8 @@ int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* insta
                 alpha = 1.0f - s2 / (2.0f * (s2 + 0.33f));
                 spec[0] = 0.45f * s2 / (s2 + 0.09f), spec[1] = 0.0f, spec[2] = 0.0f;
             }
-            mats[i] = pg::DevMaterial{type, {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha, {spec[0], spec[1], spec[2]}};
+            mats[i] = pg::DevMaterial{type,   {m.albedo[0], m.albedo[1], m.albedo[2]}, alpha, {spec[0], spec[1], spec[2]},
+                                      alphaY, m.alpha, roughY, texY};
         }
         // textures: per-triangle corner texcoords, the texture table with one texel pool, and the
         // binding of each material's colour inputs
@@ -1426,10 +1440,20 @@ int sceneCreate(pg_device* dev, const pg_scene_desc* desc, const uint32_t* insta
                 require(b.reserved == PG_TEX_NONE || mt == PG_MAT_PLASTIC || mt == PG_MAT_SUBSTRATE || mt == PG_MAT_GLASS,
                         "only plastic, substrate and glass have a second colour input");
                 matTex[i] = uint4{b.albedo, b.reserved, b.alpha, desc->materials[i].type};
-                if (b.albedo != PG_TEX_NONE || b.reserved != PG_TEX_NONE || b.alpha != PG_TEX_NONE) {
-                    mats[i].type |= pg::kMatTextured;
-                    textured = true;
-                }
+            }
+        }
+        // without its own RoughnessY a glass or substrate is isotropic: RoughnessY is RoughnessX, texture too
+        const bool ownY = desc->roughness_y || desc->roughness_y_textures;
+        for (uint32_t i = 0; i < desc->num_materials; ++i) {
+            const uint32_t mt = desc->materials[i].type;
+            if (!ownY && (mt == PG_MAT_GLASS || mt == PG_MAT_SUBSTRATE)) mats[i].texY = matTex[i].z;
+        }
+        for (uint32_t i = 0; i < desc->num_materials; ++i) {
+            const uint4 b = matTex[i];
+            if (b.x != PG_TEX_NONE || b.y != PG_TEX_NONE || b.z != PG_TEX_NONE || mats[i].texY != PG_TEX_NONE) {
+                matTex[i].w = desc->materials[i].type;
+                mats[i].type |= pg::kMatTextured;
+                textured = true;
             }
         }
         std::vector<float> triUV;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "pg_shared.hpp"
 
@@ -597,6 +598,61 @@ __device__ __forceinline__ V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2
     const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
     return normalize(V3{alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)});
 }
+// Anisotropic Trowbridge-Reitz for glass and substrate, whose RoughnessX and RoughnessY give the
+// alphas along the shading frame's x and y axes (MicrofacetDistribution, Classic/Kernel.cpp:289-357,
+// in its form: the azimuth's cosine and sine from w over sin(theta), clamped to [-1, 1]).
+struct Ggx {
+    float ax, ay;
+};
+__device__ __forceinline__ float ggxLambda(V3 w, Ggx g) {  // Classic/Kernel.cpp:326-341
+    const float cos2 = w.z * w.z;
+    const float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (isinf(tan2)) return 0.0f;
+    const float sinT = sqrtf(sin2);
+    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, w.x / sinT));
+    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, w.y / sinT));
+    const float p1 = cosPhi * g.ax, p2 = sinPhi * g.ay;
+    return (-1.0f + sqrtf(1.0f + (p1 * p1 + p2 * p2) * tan2)) * 0.5f;
+}
+__device__ __forceinline__ float ggxD(V3 wh, Ggx g) {  // Classic/Kernel.cpp:342-357
+    const float cos2 = wh.z * wh.z;
+    const float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    const float tan2 = sin2 / cos2;
+    if (isinf(tan2)) return 0.0f;
+    const float sinT = sqrtf(sin2);
+    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, wh.x / sinT));
+    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, wh.y / sinT));
+    const float p1 = cosPhi / g.ax, p2 = sinPhi / g.ay;
+    const float e = cos2 * (1.0f + (p1 * p1 + p2 * p2) * tan2);
+    return kInvPi / (g.ax * g.ay * e * e);
+}
+// Visible-normal sampling (Heitz, JCGT 2018) with the view direction stretched by (ax, ay).
+__device__ __forceinline__ V3 ggxSampleWh(V3 wo, Ggx g, float u1, float u2) {
+    const V3 vh = normalize(V3{g.ax * wo.x, g.ay * wo.y, wo.z});
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / sqrtf(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    const V3 t2 = cross(vh, t1);
+    const float r = sqrtf(u1);
+    const float phi = 2.0f * kPi * u2;
+    const float p1 = r * cosf(phi);
+    float p2 = r * sinf(phi);
+    const float sm = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - sm) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return normalize(V3{g.ax * nh.x, g.ay * nh.y, fmaxf(0.0f, nh.z)});
+}
+// Density of a reflection drawn from the visible normals: D(wh) G1(wo) / (4 cos theta_o).
+__device__ __forceinline__ float ggxReflectPdf(Ggx g, V3 wo, V3 wi) {
+    if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
+    const V3 wh = normalize(wo + wi);
+    return ggxD(wh, g) / (4.0f * wo.z * (1.0f + ggxLambda(wo, g)));
+}
+__device__ __forceinline__ float remapRoughness(float roughness) {  // Classic/Kernel.cpp:378-381
+    const float x = logf(fmaxf(1e-3f, roughness));
+    return fmaxf(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
+}
+
 __device__ __forceinline__ V3 schlick(V3 f0, float c) {
     const float m = 1.0f - fminf(1.0f, fmaxf(0.0f, c));
     const float m5 = (m * m) * (m * m) * m;
@@ -690,8 +746,8 @@ struct Mat {
     uint32_t bits;  // type (0 Lambert, 1 conductor, 2 plastic, 3 mirror, 4 glass, 5 substrate,
                     // 16 Oren-Nayar, 17 rough glass) | front face << 8 | kRoughFromTexture | kMatTextured
     V3 albedo;      // diffuse colour, conductor F0, mirror / glass reflection (texture applied)
-    float alpha;    // GGX alpha; Oren-Nayar: A
-    float aux;      // Oren-Nayar B when the roughness comes from a texture
+    float alpha;    // GGX alpha (glass, substrate: along x); Oren-Nayar: A
+    float aux;      // glass, substrate: GGX alpha along y; Oren-Nayar: B when the roughness comes from a texture
     uint32_t id;
     float2 uv;      // the hit's texture coordinate (textured materials)
     const DevMaterial* tab;
@@ -701,6 +757,8 @@ struct Mat {
     __device__ __forceinline__ uint32_t type() const { return bits & 0xFFu; }
     // plastic / glass: the hit is on the triangle's front face (etaI, etaT of their init)
     __device__ __forceinline__ bool front() const { return ((bits >> 8) & 1u) != 0u; }
+    // glass, substrate: the anisotropic distribution of their RoughnessX / RoughnessY
+    __device__ __forceinline__ Ggx ggx() const { return Ggx{alpha, aux}; }
     // plastic / substrate specular, glass transmission; Oren-Nayar: B in x
     __device__ __forceinline__ V3 spec() const {
         if (bits & kMatTextured) {
@@ -720,7 +778,7 @@ __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool fron
                                        float bu = 0.0f, float bv = 0.0f) {
     const DevMaterial& m = s.mats[id];
     const uint32_t type = TEX ? m.type : (m.type & 0xFFu);  // without textures: the plain type byte
-    Mat r{type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, 0.0f, id, make_float2(0.0f, 0.0f), s.mats,
+    Mat r{type | (front ? 0x100u : 0u), ld3(m.albedo), m.alpha, m.alphaY, id, make_float2(0.0f, 0.0f), s.mats,
           s.textures, s.texels, s.matTex};
     if (type & kMatTextured) {
         r.uv = hitTexcoord(s, pos, bu, bv);
@@ -729,8 +787,9 @@ __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool fron
             const float4 c = sampleTexture(s.textures, s.texels, b.x, r.uv);
             r.albedo = {c.x, c.y, c.z};
         }
-        if (b.z != kTexNone) {  // the roughness at this hit picks the lobe as each surface's init does
-            const float rough = sampleTexture(s.textures, s.texels, b.z, r.uv).x;
+        const uint32_t texY = m.texY;
+        if (b.z != kTexNone || texY != kTexNone) {  // the roughness at this hit picks the lobe as each surface's init does
+            const float rough = b.z != kTexNone ? sampleTexture(s.textures, s.texels, b.z, r.uv).x : m.roughX;
             uint32_t t = b.w;
             if (b.w == 0u && rough > 0.0f) {  // matteInit: Oren-Nayar, sigma = roughness x pi
                 const float sigma = rough * kPi, s2 = sigma * sigma;
@@ -739,12 +798,19 @@ __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool fron
                 r.aux = 0.45f * s2 / (s2 + 0.09f);
             } else if (b.w == 1u) {
                 r.alpha = fmaxf(rough * rough, 1e-7f);
-            } else if (b.w == 2u || b.w == 5u || (b.w == 4u && rough > 0.0f)) {
-                if (b.w == 4u) t = kDevRoughGlass;
-                const float x = logf(fmaxf(1e-3f, rough));  // remapRoughness (Classic/Kernel.cpp:378-381)
-                r.alpha = fmaxf(1e-3f, 1.62142f + (0.819955f + (0.1734f + (0.0171201f + 0.000640711f * x) * x) * x) * x);
-            } else if (b.w == 4u) {
-                r.alpha = 0.0f;
+            } else if (b.w == 2u) {
+                r.alpha = remapRoughness(rough);
+            } else if (b.w == 4u || b.w == 5u) {
+                // glassInit (Classic/Kernel.cpp:630-655): smooth only when both roughnesses are zero;
+                // substrateInit (:769-773): always the remapped pair
+                const float roughY = texY != kTexNone ? sampleTexture(s.textures, s.texels, texY, r.uv).x : m.roughY;
+                if (b.w == 5u || rough > 0.0f || roughY > 0.0f) {
+                    if (b.w == 4u) t = kDevRoughGlass;
+                    r.alpha = remapRoughness(rough);
+                    r.aux = remapRoughness(roughY);
+                } else {
+                    r.alpha = 0.0f;
+                }
             }
             r.bits = (r.bits & ~0xFFu) | t | kRoughFromTexture;
         }
@@ -775,8 +841,8 @@ __device__ __forceinline__ float glassEtaT(const Mat& m) { return m.front() ? 1.
 __device__ __forceinline__ V3 roughGlassR(const Mat& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
     const V3 wh = normalize(wo + wi);
-    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-    const float k = ggxD(wh, m.alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    const float g = 1.0f + ggxLambda(wo, m.ggx()) + ggxLambda(wi, m.ggx());
+    const float k = ggxD(wh, m.ggx()) / (4.0f * g * fabsf(wo.z * wi.z));
     return m.albedo * (fresnelDielectric(dot(wi, wh), glassEtaI(m), glassEtaT(m)) * k);
 }
 __device__ __forceinline__ V3 roughGlassT(const Mat& m, V3 wo, V3 wi) {
@@ -788,8 +854,8 @@ __device__ __forceinline__ V3 roughGlassT(const Mat& m, V3 wo, V3 wi) {
     if (!(ow > 0.0f && iw < 0.0f)) return {0.0f, 0.0f, 0.0f};  // Walter's chi+: a microfacet wo sees
     const float fr = fresnelDielectric(ow, glassEtaI(m), glassEtaT(m));
     const float den = ow + eta * iw;
-    const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-    const float v = (1.0f - fr) * ggxD(wh, m.alpha) * fabsf(iw) * fabsf(ow) / (g * fabsf(wo.z * wi.z) * den * den);
+    const float g = 1.0f + ggxLambda(wo, m.ggx()) + ggxLambda(wi, m.ggx());
+    const float v = (1.0f - fr) * ggxD(wh, m.ggx()) * fabsf(iw) * fabsf(ow) / (g * fabsf(wo.z * wi.z) * den * den);
     return m.spec() * v;
 }
 __device__ __forceinline__ float roughGlassTPdf(const Mat& m, V3 wo, V3 wi) {
@@ -801,7 +867,7 @@ __device__ __forceinline__ float roughGlassTPdf(const Mat& m, V3 wo, V3 wi) {
     if (!(ow > 0.0f && iw < 0.0f)) return 0.0f;  // only normals the visible-normal sampler draws
     const float den = ow + eta * iw;
     const float dwh = fabsf(eta * eta * iw / (den * den));  // d omega_h / d omega_i
-    return ggxD(wh, m.alpha) * fabsf(ow) / (wo.z * (1.0f + ggxLambda(wo, m.alpha))) * dwh;
+    return ggxD(wh, m.ggx()) * fabsf(ow) / (wo.z * (1.0f + ggxLambda(wo, m.ggx()))) * dwh;
 }
 __device__ __forceinline__ uint32_t glassLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
 __device__ __forceinline__ V3 roughGlassEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
@@ -812,7 +878,7 @@ __device__ __forceinline__ V3 roughGlassEval(const Mat& m, V3 wo, V3 wi, V3 ng) 
 __device__ __forceinline__ float roughGlassPdf(const Mat& m, V3 wo, V3 wi) {
     const uint32_t lobes = glassLobes(m);
     float p = 0.0f;
-    if (lobes & 1u) p += conductorPdf(m.alpha, wo, wi);
+    if (lobes & 1u) p += ggxReflectPdf(m.ggx(), wo, wi);
     if (lobes & 2u) p += roughGlassTPdf(m, wo, wi);
     return lobes == 3u ? p / 2.0f : p;
 }
@@ -844,7 +910,7 @@ __device__ __forceinline__ V3 fresnelBlendF(const Mat& m, V3 wo, V3 wi) {
     const V3 wh = normalize(wo + wi);
     const float c = dot(wi, wh);
     const V3 schlick = rs + negRem * pow5(c - 1.0f);
-    const float ks = ggxD(wh, m.alpha) / (4.0f * fmaxf(fabsf(wi.z), fabsf(wo.z)) * fabsf(c));
+    const float ks = ggxD(wh, m.ggx()) / (4.0f * fmaxf(fabsf(wi.z), fabsf(wo.z)) * fabsf(c));
     return mul(m.albedo, negRem) * kd + schlick * ks;
 }
 // FresnelBlend::pdf with the visible-normal density of its GGX half (the reference's pdf divides by
@@ -852,7 +918,7 @@ __device__ __forceinline__ V3 fresnelBlendF(const Mat& m, V3 wo, V3 wi) {
 __device__ __forceinline__ float fresnelBlendPdf(const Mat& m, V3 wo, V3 wi) {
     if (wi.z <= 0.0f) return 0.0f;
     const V3 wh = normalize(wo + wi);
-    return 0.5f * (fabsf(wi.z) * kInvPi + ggxD(wh, m.alpha) / (4.0f * fabsf(wo.z) * (1.0f + ggxLambda(wo, m.alpha))));
+    return 0.5f * (fabsf(wi.z) * kInvPi + ggxD(wh, m.ggx()) / (4.0f * fabsf(wo.z) * (1.0f + ggxLambda(wo, m.ggx()))));
 }
 
 // Mirror and smooth glass are specular: their init clears noSpecular, so the vertex does no light
@@ -893,7 +959,8 @@ struct BsdfSample {
     V3 wi, f;
     float pdf;
 };
-__device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, float alpha, float u1, float u2) {
+template <typename A>  // float: isotropic alpha; Ggx: the anisotropic pair
+__device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, A alpha, float u1, float u2) {
     BsdfSample s{{0, 0, 0}, {0, 0, 0}, 0.0f};
     if (wo.z <= 0.0f) return s;
     const V3 wh = ggxSampleWh(wo, alpha, u1, u2);
@@ -901,7 +968,8 @@ __device__ __forceinline__ BsdfSample ggxReflectSample(V3 wo, float alpha, float
     if (c < 0.0f) return s;
     s.wi = wh * (2.0f * c) - wo;
     if (s.wi.z <= 0.0f) return s;
-    s.pdf = conductorPdf(alpha, wo, s.wi);
+    if constexpr (std::is_same<A, Ggx>::value) s.pdf = ggxReflectPdf(alpha, wo, s.wi);
+    else s.pdf = conductorPdf(alpha, wo, s.wi);
     return s;
 }
 __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, float u1, float u2) {
@@ -952,11 +1020,11 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
         u1 = u1 * static_cast<float>(count) - static_cast<float>(select);
         const bool reflect = (lobes & 1u) && select == 0;
         if (reflect) {
-            s = ggxReflectSample(wo, m.alpha, u1, u2);
+            s = ggxReflectSample(wo, m.ggx(), u1, u2);
             if (s.pdf > 0.0f) s.f = roughGlassR(m, wo, s.wi);
         } else {  // a visible microfacet normal, then Snell's law about it with eta_i / eta_t
             if (wo.z <= 0.0f) return s;
-            const V3 wh = ggxSampleWh(wo, m.alpha, u1, u2);
+            const V3 wh = ggxSampleWh(wo, m.ggx(), u1, u2);
             const float c = dot(wo, wh);
             if (c < 0.0f) return s;
             const float eta = glassEtaI(m) / glassEtaT(m);
@@ -972,7 +1040,7 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
             const bool same = dot(wo, ng) * dot(s.wi, ng) > 0.0f;
             if (reflect && !same) s.f = s.f + roughGlassT(m, wo, s.wi);
             if (!reflect && same) s.f = s.f + roughGlassR(m, wo, s.wi);
-            s.pdf = (s.pdf + (reflect ? roughGlassTPdf(m, wo, s.wi) : conductorPdf(m.alpha, wo, s.wi))) / 2.0f;
+            s.pdf = (s.pdf + (reflect ? roughGlassTPdf(m, wo, s.wi) : ggxReflectPdf(m.ggx(), wo, s.wi))) / 2.0f;
         }
         return s;
     }
@@ -980,7 +1048,7 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
         if (u1 < 0.5f) {
             s.wi = cosineHemisphere(fminf(2.0f * u1, 1.0f), u2);
         } else {
-            const V3 wh = ggxSampleWh(wo, m.alpha, fminf(2.0f * (u1 - 0.5f), 1.0f), u2);
+            const V3 wh = ggxSampleWh(wo, m.ggx(), fminf(2.0f * (u1 - 0.5f), 1.0f), u2);
             s.wi = wh * (2.0f * dot(wo, wh)) - wo;
             if (s.wi.z <= 0.0f) return s;
         }

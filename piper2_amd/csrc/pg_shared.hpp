@@ -56,13 +56,18 @@ struct DevTexture {  // 64 B
     float scale, pad;
     float black[4], white[4];
 };
-struct DevMaterial {  // 32 B
+struct DevMaterial {  // 48 B
     uint32_t type;
     float albedo[3];   // diffuse reflectance (Lambert, Oren-Nayar, plastic, substrate), conductor F0,
                        // mirror / glass reflection
-    float alpha;       // GGX alpha (plastic, substrate: remapped roughness); Oren-Nayar: A
+    float alpha;       // GGX alpha (plastic, substrate, glass: remapped RoughnessX); Oren-Nayar: A
     float spec[3];     // plastic / substrate specular; glass transmission; Oren-Nayar: {B, 0, 0}
+    float alphaY;      // glass, substrate: remapped RoughnessY (= alpha when isotropic)
+    float roughX, roughY;  // glass, substrate: the RoughnessX / RoughnessY constants as configured,
+                           // for a hit whose other roughness input comes from a texture
+    uint32_t texY;     // glass, substrate: RoughnessY texture (kTexNone: the constant)
 };
+static_assert(sizeof(DevMaterial) == 48, "DevMaterial is 48 B");
 
 struct DevQuad {  // Plane primitive (PerPlaneData) + DiffuseAreaLightData, 80 B
     float origin[3], u[3], v[3];

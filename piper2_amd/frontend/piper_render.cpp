@@ -18,8 +18,8 @@
 // Mesh and image paths are relative to the scene file. Colour and roughness inputs take
 // ConstantTexture, CheckBoard or a sampled image ({WrapMode, Image: {Path}}, read from PNG or binary
 // PPM / PGM / PAM with 8-bit samples, RGB widened to RGBA as the reference's loader does). Components of
-// the reference that the HIP path does not implement (anisotropic roughness, keyframes nested in
-// keyframes, other image formats) are rejected with a message.
+// the reference that the HIP path does not implement (keyframes nested in keyframes, other image
+// formats) are rejected with a message.
 #include <array>
 #include <cctype>
 #include <cmath>
@@ -352,7 +352,8 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     const bool moving = !e.instanceFirst.empty();
     bool multiPrimitive = false;  // some area light has several primitives: their counts end the file
     for (uint32_t c : e.lightQuads) multiPrimitive = multiPrimitive || c != 1;
-    const uint32_t version = (moving ? 2u : 1u) | (multiPrimitive ? 4u : 0u);
+    const bool anisotropic = e.anisotropic();  // RoughnessY per material and its texture end the file
+    const uint32_t version = (moving ? 2u : 1u) | (multiPrimitive ? 4u : 0u) | (anisotropic ? 8u : 0u);
     put(f, &version, 1);
     put(f, head, 12);
     put(f, e.verts.data(), e.verts.size());
@@ -377,7 +378,7 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
     const uint32_t np = static_cast<uint32_t>(e.points.size());
     // with moving instances or several-primitive lights the tail is explicit: point lights, a
     // texture flag, the textures, the instances, the light counts
-    const bool explicitTail = moving || multiPrimitive;
+    const bool explicitTail = moving || multiPrimitive || anisotropic;
     if (np || !e.textures.empty() || explicitTail) {  // optional tail: point lights, then textures
         put(f, &np, 1);
         for (const pg_point_light& p : e.points) put(f, p.position, 3), put(f, p.intensity, 3);
@@ -420,6 +421,10 @@ void exportAction(const fs::path& out, const SceneExport& e, const Node* sensorN
         const uint32_t nl = static_cast<uint32_t>(e.lightQuads.size());
         put(f, &nl, 1);
         put(f, e.lightQuads.data(), nl);
+    }
+    if (anisotropic) {  // RoughnessY per material, then its texture (PG_TEX_NONE: the constant)
+        put(f, e.roughnessY.data(), nm);
+        put(f, e.roughnessYTextures.data(), nm);
     }
     if (!f) fail("cannot write " + out.string());
 }

@@ -393,11 +393,6 @@ Input decodeTexture(const Registry& reg, Call<TextureSampleFunc> call, uint32_t 
     if (in.tex.channels != channels) raise(std::string(what) + ": texture of " + std::to_string(in.tex.channels) + " channels");
     return in;
 }
-bool sameInput(const Input& a, const Input& b) {
-    if (a.textured != b.textured) return false;
-    if (!a.textured) return a.value == b.value;
-    return std::memcmp(&a.tex, &b.tex, sizeof(pg_texture)) == 0 && a.texels == b.texels;
-}
 
 // Builds the flat scene the C ABI takes from the decoded payloads of every component.
 class SceneDecoder {
@@ -409,7 +404,7 @@ public:
             if (mSurfaces[i] == &s) return static_cast<uint32_t>(i);
         const SurfaceProgram prog = s.materialize(mReg.context(mTracer));
         pg_material m{};
-        Input a, b, r, ry;  // colour, second colour, roughness
+        Input a, b, r, ry;  // colour, second colour, roughness (glass, substrate: RoughnessX), RoughnessY
         const char* what = "surface";
         switch (s.kind) {
             case Surface::Kind::Matte: {
@@ -460,10 +455,11 @@ public:
                 b = decodeTexture(mReg, d.transmission, 4, glass ? "Glass Transmission" : "Substrate Specular");
                 r = decodeTexture(mReg, d.roughnessX, 1, "RoughnessX");
                 ry = decodeTexture(mReg, d.roughnessY, 1, "RoughnessY");
-                if (!sameInput(r, ry))
-                    raise(std::string(glass ? "glass" : "substrate") + ": anisotropic roughness is not supported");
-                if (!r.textured && !std::isfinite(r.value[0])) raise("roughness must be finite");
-                if (glass && !r.textured && r.value[0] < 0.0f) raise("glass roughness must be finite and not negative");
+                for (const Input* in : {&r, &ry}) {
+                    if (!in->textured && !std::isfinite(in->value[0])) raise("roughness must be finite");
+                    if (glass && !in->textured && in->value[0] < 0.0f)
+                        raise("glass roughness must be finite and not negative");
+                }
                 m.type = glass ? PG_MAT_GLASS : PG_MAT_SUBSTRATE;
                 break;
             }
@@ -477,6 +473,10 @@ public:
         m.alpha = r.textured ? 0.0f : r.value[0];
         out.materials.push_back(m);
         out.materialTextures.push_back({textureIndex(a), textureIndex(b), textureIndex(r)});
+        // RoughnessY: glass and substrate only; the others repeat their one roughness
+        const bool two = s.kind == Surface::Kind::Glass || s.kind == Surface::Kind::Substrate;
+        out.roughnessY.push_back(two && !ry.textured ? ry.value[0] : m.alpha);
+        out.roughnessYTextures.push_back(two ? textureIndex(ry) : PG_TEX_NONE);
         mSurfaces.push_back(&s);
         return static_cast<uint32_t>(out.materials.size() - 1);
     }
@@ -800,6 +800,10 @@ public:
         d.num_quads = static_cast<uint32_t>(f.quads.size());
         d.area_light_quads = f.lightQuads.empty() ? nullptr : f.lightQuads.data();
         d.num_area_lights = static_cast<uint32_t>(f.lightQuads.size());
+        if (f.anisotropic()) {
+            d.roughness_y = f.roughnessY.data();
+            d.roughness_y_textures = f.roughnessYTextures.data();
+        }
         d.point_lights = f.points.empty() ? nullptr : f.points.data();
         d.num_point_lights = static_cast<uint32_t>(f.points.size());
         if (!f.textures.empty()) {
@@ -984,15 +988,14 @@ std::shared_ptr<Surface> Surface::mirror(std::array<float, 3> reflection) {
 }
 std::shared_ptr<Surface> Surface::glass(std::array<float, 3> reflection, std::array<float, 3> transmission,
                                         float roughnessX, float roughnessY) {
-    if (roughnessX != roughnessY) raise("glass: anisotropic roughness is not supported");
-    if (!(roughnessX >= 0.0f) || !std::isfinite(roughnessX)) raise("glass roughness must be finite and not negative");
+    for (float v : {roughnessX, roughnessY})
+        if (!(v >= 0.0f) || !std::isfinite(v)) raise("glass roughness must be finite and not negative");
     return std::make_shared<Surface>(Kind::Glass, std::vector<Config>{colourConfig(reflection), colourConfig(transmission),
                                                                       scalarConfig(roughnessX), scalarConfig(roughnessY)});
 }
 std::shared_ptr<Surface> Surface::substrate(std::array<float, 3> diffuse, std::array<float, 3> specular,
                                             float roughnessX, float roughnessY) {
-    if (roughnessX != roughnessY) raise("substrate: anisotropic roughness is not supported");
-    if (!std::isfinite(roughnessX)) raise("substrate roughness must be finite");
+    if (!std::isfinite(roughnessX) || !std::isfinite(roughnessY)) raise("substrate roughness must be finite");
     return std::make_shared<Surface>(Kind::Substrate, std::vector<Config>{colourConfig(diffuse), colourConfig(specular),
                                                                           scalarConfig(roughnessX), scalarConfig(roughnessY)});
 }

@@ -671,6 +671,14 @@ struct SceneExport {
     std::vector<pg_texture> textures;                    // texels point into textureTexels
     std::vector<std::vector<uint8_t>> textureTexels;
     std::vector<pg_material_textures> materialTextures;  // per material (empty without textures)
+    // per material: RoughnessY and its texture (glass, substrate; the others repeat their roughness)
+    std::vector<float> roughnessY;
+    std::vector<uint32_t> roughnessYTextures;
+    [[nodiscard]] bool anisotropic() const {
+        for (size_t i = 0; i < materials.size(); ++i)
+            if (roughnessY[i] != materials[i].alpha || roughnessYTextures[i] != PG_TEX_NONE) return true;
+        return false;
+    }
     // moving instances: instance k's triangles start at instanceFirst[k] (static ones come first)
     struct Motion {
         std::shared_ptr<const TransformInfo> keys;

@@ -150,7 +150,8 @@ class _Builder:
 
 
 def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, rough_walls=False,
-            specular=False, point_lights=False, textured=False, glass_roughness=0.0, multi_light=False):
+            specular=False, point_lights=False, textured=False, glass_roughness=0.0, multi_light=False,
+            anisotropic=False):
     """Config 1: closed box of 5 quads (open toward the camera), 2 seeded boxes, one ceiling light.
 
     plastic=True makes the two boxes plastic (seeded specular colour and roughness) and adds a
@@ -165,8 +166,12 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
     plastic=True, a checkerboard specular colour on the first box and an image roughness on the
     second; multi_light=True replaces the ceiling light by one Plane light of three primitives of
     unequal area (two halves of a 0.5 x 0.5 quad cut at 0.6 of its width and a 0.2 x 0.2 piece
-    beside them) and adds a second light of two strips high on the back wall ("area_lights" [3, 2]).
-    The light and the rest of the geometry stay as in the plain scene."""
+    beside them) and adds a second light of two strips high on the back wall ("area_lights" [3, 2]);
+    anisotropic=True (with specular=True) gives the glass sphere RoughnessY 0.25 and the substrate
+    floor RoughnessY 0.8 ("roughness_y"; RoughnessX stays glass_roughness and 0.3, so the sphere is
+    rough even at glass_roughness 0) and, with textured=True too, the sphere's RoughnessY from the
+    roughness image ("roughness_y_textures"). The light and the rest of the geometry stay as in the
+    plain scene."""
     rng = np.random.default_rng(seed)
     prng = np.random.default_rng(seed + 1000)
     b = _Builder()
@@ -258,6 +263,18 @@ def cornell(seed=1, width=256, height=256, spp=16, max_depth=5, plastic=False, r
             mt[sc["object_material"][6], 2] = 5  # the second box's roughness
         sc["material_textures"] = mt
         sc["name"] += "_textured"
+    if anisotropic:
+        assert specular
+        ry = sc["materials"][:, 4].copy()
+        glass = sc["object_material"][-2]  # the sphere (the mirror is the last object)
+        floor = sc["object_material"][0]
+        ry[glass], ry[floor] = 0.25, 0.8
+        sc["roughness_y"] = ry
+        if textured:
+            ryt = np.full(len(ry), 0xFFFFFFFF, np.uint32)
+            ryt[glass] = 5
+            sc["roughness_y_textures"] = ryt
+        sc["name"] += "_anisotropic"
     return sc
 
 

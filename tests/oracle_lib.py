@@ -44,6 +44,15 @@ def lib():
         L.or_render.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_int, _VP]
         L.or_bsdf_eval.argtypes = [_VP, ctypes.c_int, _VP, _VP, _VP, _VP]
         L.or_bsdf_sample.argtypes = [_VP, ctypes.c_int, _VP, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP]
+        L.or_bsdf_eval_y.argtypes = [_VP, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP, _VP]
+        L.or_bsdf_sample_y.argtypes = [_VP, ctypes.c_float, ctypes.c_int, _VP, ctypes.c_float, ctypes.c_float, _VP, _VP,
+                                       _VP]
+        L.or_scene_set_roughness_y.argtypes = [_VP, _VP, _VP]
+        L.or_scene_set_roughness_y.restype = ctypes.c_int
+        for name in ("or_ggx_d_aniso", "or_ggx_lambda"):
+            getattr(L, name).argtypes = [ctypes.c_float] * 5
+            getattr(L, name).restype = ctypes.c_float
+        L.or_ggx_sample_wh.argtypes = [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _VP]
         L.or_fresnel_dielectric.restype = ctypes.c_float
         L.or_fresnel_dielectric.argtypes = [ctypes.c_float] * 3
         L.or_remap_roughness.restype = ctypes.c_float
@@ -90,6 +99,7 @@ class OracleScene:
         if al is not None:
             self._al = np.ascontiguousarray(al, np.uint32)
             assert lib().or_scene_set_area_lights(self.h, _p(self._al), len(self._al)) == 0
+        ry, ryt = sc.get("roughness_y"), sc.get("roughness_y_textures")
         pts = sc.get("point_lights")
         if pts is not None and len(pts):
             self._pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
@@ -110,6 +120,10 @@ class OracleScene:
             mt = np.ascontiguousarray(sc["material_textures"], np.uint32).reshape(-1, 3)
             self._keep += [triuv, mt]
             lib().or_scene_set_textures(self.h, _p(triuv), _p(mt))
+        if ry is not None or ryt is not None:  # anisotropic roughness of glass and substrate
+            self._ry = None if ry is None else np.ascontiguousarray(ry, np.float32)
+            self._ryt = None if ryt is None else np.ascontiguousarray(ryt, np.uint32)
+            assert lib().or_scene_set_roughness_y(self.h, _p(self._ry), _p(self._ryt)) == 0
 
     def set_motion(self, instance, keys, t0=0.0, t1=1.0, visible=True, parent=None):
         """Keys of moving instance `instance` (as Scene.set_motion)."""

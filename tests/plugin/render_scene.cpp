@@ -444,10 +444,10 @@ int errors(const char* scenePath) {
     expect("undefined kernel entry", [&] { (void)acc->compileKernel({hipBuiltinProgram()}, {}, {}, "piperMain"); });
     expect("built-in as kernel entry", [&] { (void)acc->compileKernel({hipBuiltinProgram()}, {}, {}, "trace"); });
     expect("texture channels", [&] { (void)tracer.generateTexture(colourConfig({1, 1, 1}), 1); });
-    expect("anisotropic glass", [&] {
+    expect("negative glass RoughnessY", [&] {
         auto glass = std::make_shared<Surface>(Surface::Kind::Glass,
                                                std::vector<Config>{colourConfig({1, 1, 1}), colourConfig({1, 1, 1}),
-                                                                   scalarConfig(0.1f), scalarConfig(0.2f)});
+                                                                   scalarConfig(0.1f), scalarConfig(-0.2f)});
         auto buf = acc->createBuffer(48, 16, [](std::byte* p) {
             const float v[9] = {0, 0, 0, 1, 0, 0, 0, 1, 0};
             const uint32_t idx[3] = {0, 1, 2};

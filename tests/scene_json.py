@@ -143,6 +143,13 @@ def write_scene_json(sc, folder, spp=None, filter_cfg=None, fit="Fill", transfor
                             surf[k] = texture_config(sc["textures"][t], folder, f"texture{t}.pam", image_format)
                             if k != "RoughnessX":
                                 break
+        if "RoughnessY" in surf:  # anisotropic glass / substrate: RoughnessY of its own
+            ry, ryt = sc.get("roughness_y"), sc.get("roughness_y_textures")
+            if ryt is not None and int(ryt[mat]) != 0xFFFFFFFF:
+                t = int(ryt[mat])
+                surf["RoughnessY"] = texture_config(sc["textures"][t], folder, f"texture{t}.pam", image_format)
+            elif ry is not None:
+                surf["RoughnessY"] = _tex(float(ry[mat]))
         inst = {"Type": "Instance", "Geometry": {"ClassID": "Piper.BuiltinComponent.TriangleMesh.TriangleMesh",
                                                  "Path": name},
                 "Surface": surf}
@@ -214,8 +221,9 @@ def read_pgsc(path):
     with open(path, "rb") as f:
         data = f.read()
     version = struct.unpack("<I", data[4:8])[0]
-    assert data[:4] == b"PGSC" and version & 3 in (1, 2) and version < 8
-    multi_primitive = bool(version & 4)  # area lights of several primitives: their counts end the file
+    assert data[:4] == b"PGSC" and version & 3 in (1, 2) and version < 16
+    multi_primitive = bool(version & 4)  # area lights of several primitives: their counts follow the instances
+    anisotropic = bool(version & 8)  # RoughnessY per material and its texture end the file
     version &= 3
     head = np.frombuffer(data, np.uint32, 12, 8)
     nv, nt, no, nm, nq, w, h, spp, md, rr, compat, fit = (int(x) for x in head)
@@ -244,7 +252,7 @@ def read_pgsc(path):
         off += 4
         take("point_lights", np.float32, 6 * npt, (npt, 6))
         has_textures = off < len(data)
-    if version == 2 or multi_primitive:  # an explicit texture flag; instances, light counts after the textures
+    if version == 2 or multi_primitive or anisotropic:  # an explicit texture flag; the rest after the textures
         has_textures = struct.unpack("<I", data[off:off + 4])[0] == 1
         off += 4
     if has_textures:
@@ -285,6 +293,9 @@ def read_pgsc(path):
         nl = struct.unpack("<I", data[off:off + 4])[0]
         off += 4
         take("area_lights", np.uint32, nl, (nl,))
+    if anisotropic:
+        take("roughness_y", np.float32, nm, (nm,))
+        take("roughness_y_textures", np.uint32, nm, (nm,))
     assert off == len(data)
     out.update(width=w, height=h, spp=spp, max_depth=md, min_rr=rr, fit=fit)
     return out

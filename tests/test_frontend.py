@@ -42,6 +42,23 @@ def test_export_reproduces_generated_scene(tmp_path):
     assert (e["width"], e["height"], e["spp"], e["max_depth"], e["min_rr"]) == (64, 48, 8, 5, 5)
 
 
+def test_anisotropic_roughness_export(tmp_path):
+    # RoughnessY of its own on glass and substrate (a constant, and an image on the sphere)
+    sc = scenes.cornell(width=32, height=32, spp=4, specular=True, glass_roughness=0.2, anisotropic=True,
+                        textured=True)
+    e = export(sc, tmp_path)
+    m = sc["materials"]
+    glass, floor = sc["object_material"][-2], sc["object_material"][0]
+    assert e["roughness_y"][floor] == np.float32(0.8)
+    assert e["roughness_y_textures"][floor] == 0xFFFFFFFF
+    t = e["roughness_y_textures"][glass]
+    assert t != 0xFFFFFFFF and np.array_equal(e["textures"][t]["texels"], sc["textures"][5]["texels"])
+    others = [i for i in range(len(m)) if m[i, 0] not in (4, 5)]
+    assert np.array_equal(e["roughness_y"][others], e["materials"][others, 4])
+    iso = export(scenes.cornell(width=32, height=32, spp=4, specular=True, glass_roughness=0.2), tmp_path / "iso")
+    assert "roughness_y" not in iso
+
+
 def test_multi_primitive_light_export(tmp_path):
     # Plane lights with several Primitives keep their primitives in order and their grouping
     sc = scenes.cornell(width=32, height=32, spp=4, multi_light=True)

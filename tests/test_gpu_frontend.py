@@ -40,13 +40,15 @@ def direct_rgb(dev, e, film_filter=0, alpha=0.0):
 
 @pytest.mark.parametrize("filt,textured,plane,multi", [
     (None, False, False, False), (None, True, False, False), (None, True, True, False), (None, False, False, True),
+    (None, True, False, True),
     ({"ClassID": "Piper.BuiltinComponent.FixedSampler.SimpleFilter", "FilterType": "gaussian", "Alpha": 2.0}, False,
      False, False)])
 def test_json_scene_renders_like_pg_render(dev, tmp_path, filt, textured, plane, multi):
     """plane=True makes the checkerboard back wall a BasicShapes Plane, shaded at its own (u, v);
-    multi=True has two area lights of several Plane primitives each."""
+    multi=True has two area lights of several Plane primitives each and anisotropic roughness on the
+    glass sphere and the substrate floor (the sphere's RoughnessY from an image when textured)."""
     sc = scenes.cornell(width=64, height=48, spp=8, plastic=True, rough_walls=True, specular=True, point_lights=True,
-                        textured=textured, multi_light=multi)
+                        textured=textured, multi_light=multi, glass_roughness=0.2 if multi else 0.0, anisotropic=multi)
     opts = write_scene_json(sc, str(tmp_path), filter_cfg=filt)
     if plane:
         make_plane_object(opts, 2, [-1.0, -1.0, -1.0], [0.0, 2.0, 0.0], [2.0, 0.0, 0.0])

@@ -265,6 +265,22 @@ def test_area_light_counts_rejected(dev):
         pg.Scene(dev, sc2)
 
 
+@pytest.mark.parametrize("glass_roughness", [0.3, 0.0])
+def test_anisotropic_surfaces_paths_match_oracle(dev, glass_roughness):
+    # RoughnessY differs from RoughnessX on a rough glass sphere and the substrate floor; with
+    # RoughnessX 0 the sphere is rough through RoughnessY alone (glassInit's both-zero rule)
+    sc = scenes.cornell(width=128, height=128, spp=16, specular=True, glass_roughness=glass_roughness, anisotropic=True)
+    _path_parity(sc, dev)
+
+
+def test_anisotropic_textured_roughness_paths_match_oracle(dev):
+    # the sphere's RoughnessY from an image while RoughnessX is a constant, next to textured
+    # isotropic roughness on the ceiling and a plastic box
+    sc = scenes.cornell(width=128, height=128, spp=16, specular=True, glass_roughness=0.2, anisotropic=True,
+                        textured=True, plastic=True)
+    _path_parity(sc, dev)
+
+
 def test_textured_paths_match_oracle(dev):
     # checkerboard and 8-bit image textures (repeat and mirror) on walls and a box's specular colour
     sc = scenes.cornell(width=128, height=128, spp=16, textured=True, plastic=True, specular=True)

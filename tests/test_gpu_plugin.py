@@ -138,5 +138,5 @@ def test_error_paths_raise(tmp_path):
                  "kernel of a foreign format: raised: Unsupported linkable format LLVM IR",
                  "undefined kernel entry: raised: compileKernel: undefined symbol piperMain",
                  "texture channels: raised: Mismatched channel. Expect 1, but get 4",
-                 "anisotropic glass: raised: glass: anisotropic roughness is not supported"):
+                 "negative glass RoughnessY: raised: glass roughness must be finite and not negative"):
         assert case in r.stdout, case

@@ -516,6 +516,151 @@ def test_rough_glass_mixture_sampling_matches_evaluate(front):
     assert np.allclose(est / n, quad, rtol=0.04), (est / n, quad)
 
 
+def _bsdf_y(mat, ry, wo, wi, front=True):
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    wo, wi = np.ascontiguousarray(wo, np.float32), np.ascontiguousarray(wi, np.float32)
+    lib().or_bsdf_eval_y(mat.ctypes.data, np.float32(ry), int(front), wo.ctypes.data, wi.ctypes.data, f.ctypes.data,
+                         pdf.ctypes.data)
+    return f, float(pdf[0])
+
+
+def _sample_y(mat, ry, wo, u1, u2, front=True):
+    wi, f, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    lib().or_bsdf_sample_y(mat.ctypes.data, np.float32(ry), int(front), wo.ctypes.data, np.float32(u1),
+                           np.float32(u2), wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return wi, f, float(pdf[0])
+
+
+def _sphere_grid(n_theta, n_phi):
+    th = (np.arange(n_theta) + 0.5) / n_theta * np.pi
+    ph = (np.arange(n_phi) + 0.5) / n_phi * (2 * np.pi)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    return dirs, (np.sin(T) * (np.pi / n_theta) * (2 * np.pi / n_phi)).reshape(-1)
+
+
+@pytest.mark.parametrize("ax,ay", [(0.1, 0.4), (0.6, 0.15)])
+def test_anisotropic_ggx_closed_forms(ax, ay):
+    """The anisotropic Trowbridge-Reitz D against the ellipsoid form 1 / (pi ax ay (x^2/ax^2 +
+    y^2/ay^2 + z^2)^2) and Lambda against the stretched form (sqrt(1 + (ax^2 x^2 + ay^2 y^2) / z^2)
+    - 1) / 2 (a swapped axis fails both); D integrates to 1 over projected area; the projected area
+    of the visible microfacets is cos(theta_o) (1 + Lambda(wo)) (Smith's masking identity, which ties
+    Lambda to D); and sampled visible normals have the moments of D_wo(h) = G1(wo) max(0, wo.h) D(h)
+    / cos(theta_o) (an axis mix-up in the sampler fails them)."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        h = _upper(rng)
+        x, y, z = (float(c) for c in h)
+        d = lib().or_ggx_d_aniso(x, y, z, ax, ay)
+        want = 1.0 / (math.pi * ax * ay * (x * x / (ax * ax) + y * y / (ay * ay) + z * z) ** 2)
+        assert abs(d - want) <= 1e-4 * want, (h, d, want)
+        lam = lib().or_ggx_lambda(x, y, z, ax, ay)
+        lw = (math.sqrt(1 + (ax * ax * x * x + ay * ay * y * y) / (z * z)) - 1) / 2
+        assert abs(lam - lw) <= 1e-4 * lw + 1e-7, (h, lam, lw)
+    dirs, dw = _sphere_grid(1600, 800)
+    up = dirs[:, 2] > 0
+    hd, hw = dirs[up], dw[up]
+    D = np.array([lib().or_ggx_d_aniso(float(a), float(b), float(c), ax, ay) for a, b, c in hd[::7]])
+    # every 7th direction of a fine grid: the coarse quadrature of D cos(theta) is within a percent
+    assert abs(np.sum(D * hd[::7, 2] * hw[::7]) * 7 - 1) < 0.03
+    Dfull = np.array([lib().or_ggx_d_aniso(float(a), float(b), float(c), ax, ay) for a, b, c in hd])
+    assert abs(np.sum(Dfull * hd[:, 2] * hw) - 1) < 3e-3
+    for wo in (np.array([0.5, 0.2, 0.0]), np.array([-0.3, 0.6, 0.0])):
+        wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+        lam = lib().or_ggx_lambda(*(float(c) for c in wo), ax, ay)
+        proj = np.sum(np.maximum(0.0, hd @ wo) * Dfull * hw)
+        assert abs(proj - wo[2] * (1 + lam)) < 5e-3 * wo[2] * (1 + lam), (proj, wo[2] * (1 + lam))
+        dens = np.maximum(0.0, hd @ wo) * Dfull / (wo[2] * (1 + lam))  # D_wo over h
+        want = [np.sum(dens * hw * hd[:, k] ** 2) for k in range(2)]
+        wo32 = np.ascontiguousarray(wo, np.float32)
+        out = np.zeros(3, np.float32)
+        got = np.zeros(2)
+        n = 40000
+        for u1, u2 in rng.uniform(0, 1, (n, 2)):
+            lib().or_ggx_sample_wh(wo32.ctypes.data, ax, ay, np.float32(u1), np.float32(u2), out.ctypes.data)
+            got += out[:2].astype(np.float64) ** 2
+        assert np.allclose(got / n, want, rtol=0.05), (got / n, want)
+
+
+def _rot90(v):
+    return np.array([-v[1], v[0], v[2]], np.float32)
+
+
+@pytest.mark.parametrize("kind", ["glass", "substrate"])
+def test_anisotropic_surfaces_rotate_with_their_axes(kind):
+    """Exchanging RoughnessX and RoughnessY equals turning both directions a quarter turn about the
+    normal, for the value and the pdf (glass on both faces, reflection and transmission), and the
+    sampler draws from the pdf it reports: a sample's value and pdf equal the evaluate path's, and
+    E[f |cos| / pdf] equals quadrature of the evaluate path."""
+    rx, ry = 0.2, 0.7
+    code = 4 if kind == "glass" else 5
+    spec = [0.8, 0.85, 0.9] if kind == "glass" else [0.2, 0.3, 0.4]
+    mat = np.array([code, 0.7, 0.6, 0.5, rx, *spec], np.float32)
+    swapped = np.array([code, 0.7, 0.6, 0.5, ry, *spec], np.float32)
+    rng = np.random.default_rng(17)
+    fronts = (True, False) if kind == "glass" else (True,)
+    for front in fronts:
+        for _ in range(300):
+            wo, wi = _upper(rng), _upper(rng)
+            if kind == "glass" and rng.uniform() < 0.5:
+                wi[2] = -wi[2]
+            f1, p1 = _bsdf_y(mat, ry, wo, wi, front)
+            f2, p2 = _bsdf_y(swapped, rx, _rot90(wo), _rot90(wi), front)
+            assert np.allclose(f1, f2, rtol=2e-4, atol=1e-8) and abs(p1 - p2) <= 2e-4 * p1 + 1e-9
+        iso, _ = _bsdf_y(mat, rx, wo, wi, front)
+        assert np.array_equal(iso, _bsdf(mat, wo, wi, front)[0])  # RoughnessY = RoughnessX: isotropic
+    # anisotropy matters: seen from the normal, a reflection 40 degrees off it (half vector tilted by
+    # 20 degrees) is far brighter in the wide direction (y) than in the narrow one (x)
+    refl = 0.7 if kind == "glass" else 0.0  # substrate: no diffuse part, the glossy lobe alone
+    narrow = np.array([code, refl, refl, refl, 0.02, *spec], np.float32)
+    wo, s40, c40 = np.array([0, 0, 1], np.float32), math.sin(math.radians(40)), math.cos(math.radians(40))
+    along_x = _bsdf_y(narrow, 0.5, wo, np.array([s40, 0, c40], np.float32))[0][0]
+    along_y = _bsdf_y(narrow, 0.5, wo, np.array([0, s40, c40], np.float32))[0][0]
+    assert along_y > 3 * along_x, (along_x, along_y)
+    wo = np.array([0.4, 0.3, 0.0], np.float32)
+    wo[2] = math.sqrt(1 - wo[0] ** 2 - wo[1] ** 2)
+    n, est = 30000, np.zeros(3)
+    for u1, u2 in rng.uniform(0, 1, (n, 2)):
+        wi, f, pdf = _sample_y(mat, ry, wo, u1, u2)
+        if pdf > 0:
+            f2, pdf2 = _bsdf_y(mat, ry, wo, wi)
+            assert np.allclose(f, f2, rtol=1e-4, atol=1e-8) and abs(pdf - pdf2) <= 1e-4 * pdf
+            est += f * abs(wi[2]) / pdf
+    dirs, dw = _hemisphere_grid(400, 200)
+    quad = np.zeros(3)
+    for d, w in zip(dirs, dw):
+        for z in ((1, -1) if kind == "glass" else (1,)):
+            quad += _bsdf_y(mat, ry, wo, np.array([d[0], d[1], z * d[2]], np.float32))[0] * d[2] * w
+    assert np.allclose(est / n, quad, rtol=0.04), (est / n, quad)
+
+
+def test_glass_rough_along_one_axis_only():
+    """glassInit is smooth only when both roughnesses are zero: RoughnessX 0 with RoughnessY 0.3 is a
+    rough glass (a non-delta lobe that evaluates to non-zero), RoughnessX = RoughnessY = 0 the
+    specular one (evaluates to zero)."""
+    mat = np.array([4, 0.9, 0.9, 0.9, 0.0, 0.9, 0.9, 0.9], np.float32)
+    wo = np.array([0.3, 0.1, math.sqrt(0.9)], np.float32)
+    wi = np.array([-0.25, -0.1, math.sqrt(1 - 0.0725)], np.float32)
+    assert not _bsdf_y(mat, 0.0, wo, wi)[0].any()
+    f, pdf = _bsdf_y(mat, 0.3, wo, wi)
+    assert f.all() and pdf > 0
+
+
+def test_anisotropic_render_and_isotropic_default():
+    """A Cornell box with an anisotropic rough glass sphere and substrate floor: RoughnessY equal to
+    RoughnessX renders bit for bit as no RoughnessY at all, and a different RoughnessY changes it."""
+    sc = scenes.cornell(width=24, height=24, spp=8, specular=True, glass_roughness=0.3)
+    base = OracleScene(sc).render(path_l=True, film=False, threads=4)["path_l"]
+    same = dict(sc)
+    same["roughness_y"] = sc["materials"][:, 4].copy()
+    assert np.array_equal(OracleScene(same).render(path_l=True, film=False, threads=4)["path_l"], base)
+    aniso = scenes.cornell(width=24, height=24, spp=8, specular=True, glass_roughness=0.3, anisotropic=True)
+    other = OracleScene(aniso).render(path_l=True, film=False, threads=4)["path_l"]
+    assert not np.array_equal(other, base)
+
+
 @pytest.mark.parametrize("kind", ["glass", "mirror"])
 def test_lossless_specular_object_vanishes_in_a_white_furnace(kind):
     """A white environment (radiance 1) around a lossless glass sphere (R = T = 1) or mirror sphere:
