@@ -102,7 +102,7 @@ __device__ __forceinline__ LobePair lobesFromTile(const f32x4 (&out)[3]) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const float ox = outSlot(out, 5 * q), oy = outSlot(out, 5 * q + 1), oz = outSlot(out, 5 * q + 2);
-        const float n = fmaxf(sqrtf(ox * ox + oy * oy + oz * oz), 1e-6f);
+        const float n = fmaxf(__builtin_amdgcn_sqrtf(ox * ox + oy * oy + oz * oz), 1e-6f);
         const float inv = __builtin_amdgcn_rcpf(n);  // 1 ulp: the stored lobe is what sampling and pdf use
         L.mn[q] = n;
         L.mx[q] = ox * inv, L.my[q] = oy * inv, L.mz[q] = oz * inv;
@@ -140,52 +140,72 @@ constexpr int kInferStageBytes = 64 * 3 * 16;  // per wave: 64 samples x 3 float
 constexpr int kInferBlock = 512;
 static_assert(kInferBlock == static_cast<int>(kGroupUnit), "a grouped unit is one block's 8 x 64 samples");
 
-// Mixtures of one 16-sample tile of the wave's stage (tile t: stage entries 16t..16t+15).
-__device__ __forceinline__ void inferTile(int t, const GuideLdsX& L, const float4* stage, float* __restrict__ mix) {
-    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
-    const int k = 16 * t + s;
-    const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
-    const GuideInput in{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
-    const uint32_t i = __float_as_uint(c.y);
-    bf16x4 x0[9];
-    guideFeatureTiles(in, grp, x0);
-    f32x4 out[3];
-    guideForwardTileX32(L, x0, out);
-    const LobePair lp = lobesFromTile(out);
-    if (i != ~0u) {
-        float* o = mix + static_cast<size_t>(i) * kGMixFloats;
-        reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
-        reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
-        reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
-        reinterpret_cast<float2*>(o + 40)[grp] =
-            make_float2(lp.pi[0] * vmfNormSampled(lp.kappa[0]), lp.pi[1] * vmfNormSampled(lp.kappa[1]));
-    }
+// Stores the mixture of this lane's lobes (2g, 2g + 1) of sample `i` (~0u: none).
+__device__ __forceinline__ void storeLobes(uint32_t i, const LobePair& lp, int grp, float* __restrict__ mix) {
+    if (i == ~0u) return;
+    float* o = mix + static_cast<size_t>(i) * kGMixFloats;
+    reinterpret_cast<float4*>(o)[2 * grp] = make_float4(lp.mx[0], lp.my[0], lp.mz[0], lp.kappa[0]);
+    reinterpret_cast<float4*>(o)[2 * grp + 1] = make_float4(lp.mx[1], lp.my[1], lp.mz[1], lp.kappa[1]);
+    reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
+    reinterpret_cast<float2*>(o + 40)[grp] =
+        make_float2(lp.pi[0] * vmfNormSampled(lp.kappa[0]), lp.pi[1] * vmfNormSampled(lp.kappa[1]));
 }
 
 // Mixtures of one wave's 64 samples: lane l's sample is queue/array entry `entry` (~0u: none) with
-// inputs `in`. While the four tiles run, the lane's next entry `next` goes through the source's
-// load stages; returns its inputs. Wave-uniform; the stage is the wave's own LDS slice.
-template <typename Src>
+// inputs `in`. The samples go through the wave's LDS stage so that tile t (samples 16t..16t+15) is
+// spread over all 64 lanes (lane group g computes features 4g.. of sample 16t + l % 16); per pass
+// of T tiles (4 / T passes) each tile's features feed its layer 0 at once, then layers 1..4 run the
+// T tiles side by side. While they run, the lane's next entry
+// `next` goes through the source's load stages; returns its inputs. Wave-uniform; the stage is the
+// wave's own LDS slice.
+template <int T, typename Src>
 __device__ __forceinline__ GuideInput inferWave64(const Src& src, uint32_t entry, const GuideInput& in, uint32_t next,
                                                   const GuideLdsX& L, float4* stage, float* __restrict__ mix) {
-    const int lane = __lane_id();
+    static_assert(T == 1 || T == 2 || T == 4, "tiles per pass");
+    const int lane = __lane_id(), s = lane & 15, grp = lane >> 4;
     stage[3 * lane] = make_float4(in.px, in.py, in.pz, in.nx);
     stage[3 * lane + 1] = make_float4(in.ny, in.nz, in.wx, in.wy);
     stage[3 * lane + 2] = make_float4(in.wz, __uint_as_float(entry), 0.0f, 0.0f);
     const uint64_t live = __ballot(entry != ~0u);
     waveLdsSync();  // the wave's own stage: written and read by its lanes only
     typename Src::Pend p = src.begin(next);
-    if (live & 0xFFFFull) inferTile(0, L, stage, mix);  // wave-uniform tests
-    src.stage2(p);
-    if ((live >> 16) & 0xFFFFull) inferTile(1, L, stage, mix);
-    src.stage3(p);
-    if ((live >> 32) & 0xFFFFull) inferTile(2, L, stage, mix);
-    src.stage4(p);
-    if (live >> 48) inferTile(3, L, stage, mix);
+    // The next samples' dependent loads are spread over the work: stage 2 after the first tile's
+    // layer 0, stage 3 after the third's, stage 4 after the last hidden layers.
+    constexpr uint64_t kPassLanes = T == 4 ? ~0ull : (1ull << (16 * T)) - 1ull;  // a pass's samples in `live`
+#pragma unroll
+    for (int pass = 0; pass < 4 / T; ++pass) {
+        const bool run = (live >> (16 * T * pass)) & kPassLanes;  // wave-uniform
+        f32x4 acc[T][4];
+        uint32_t idx[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {  // features and layer 0 tile by tile (one tile's features live)
+            const int tile = T * pass + t;
+            if (run) {
+                const int k = 16 * tile + s;
+                const float4 a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
+                idx[t] = __float_as_uint(c.y);
+                bf16x4 x0[9];
+                guideFeatureTiles(GuideInput{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x}, grp, x0);
+                guideLayer0X32(L, x0, acc[t]);
+            }
+            if (tile == 0) src.stage2(p);
+            if (tile == 2) src.stage3(p);
+        }
+        f32x4 out[T][3];
+        if (run) guideHiddenX32<T>(L, acc, out);
+        if (pass == 4 / T - 1) src.stage4(p);
+        if (run)
+#pragma unroll
+            for (int t = 0; t < T; ++t) storeLobes(idx[t], lobesFromTile(out[t]), grp, mix);
+    }
     const GuideInput nin = src.finish(p);
     waveLdsSync();  // stage reads done before the next samples overwrite it
     return nin;
 }
+
+// Tiles per pass of a block of `waves` waves: 4 (about 150 registers) where the block size allows;
+// a 16-wave block (1024 threads) caps the registers at 128, which one tile at a time fits.
+constexpr int inferTiles(int waves) { return waves >= 16 ? 1 : 4; }
 
 // WAVES: waves per block, sharing one LDS copy of the weights (a launch setting).
 template <typename Src, int WAVES>
@@ -207,7 +227,7 @@ __global__ void __launch_bounds__(WAVES * 64) kGuideInfer(Src src, uint32_t nHos
     GuideInput in = src.finish(p);
     for (; base < n; base += waves * 64) {  // wave-uniform
         const uint32_t next = entryAt(base + waves * 64);
-        in = inferWave64(src, entry, in, next, L, stage, mix);
+        in = inferWave64<inferTiles(WAVES)>(src, entry, in, next, L, stage, mix);
         entry = next;
     }
 }
@@ -371,7 +391,7 @@ __global__ void __launch_bounds__(kInferBlock) kGuideInferGrouped(BatchPathSourc
         }
         const uint32_t nextKey = u + 1 < u1 ? keyOf(u + 1) : key;
         const uint32_t next = u + 1 < u1 ? entryOf(u + 1, nextKey) : ~0u;
-        in = inferWave64(src, entry, in, next, L, stage, mix);
+        in = inferWave64<inferTiles(kInferBlock / 64)>(src, entry, in, next, L, stage, mix);
         key = nextKey, entry = next;
     }
 }

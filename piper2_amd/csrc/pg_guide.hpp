@@ -154,17 +154,47 @@ __device__ __forceinline__ GuideInput guideInput(float3 p01, float3 n, float3 wo
     return GuideInput{p01.x, p01.y, p01.z, n.x, n.y, n.z, wo.x, wo.y, wo.z};
 }
 
+// sin(pi x) and cos(pi x): x reduced to r = x - n / 2 with n = rint(2x) (exact for |x| < 2^22),
+// Taylor polynomials of sin and cos at t = pi r in [-pi/4, pi/4] (truncation below 3e-8), then the
+// quarter turn n applied. Cheaper than a library sincospi, accurate far beyond the BF16 features.
+__device__ __forceinline__ void sinCosPi(float x, float& s, float& c) {
+    const float n = rintf(2.0f * x);
+    const float r = fmaf(n, -0.5f, x);
+    const float t = r * 3.14159265358979323846f, t2 = t * t;
+    const float sp = t * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
+                                       -1.66666667e-1f), 1.0f);
+    const float cp = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.48015873e-5f, -1.38888889e-3f), 4.16666667e-2f), -0.5f), 1.0f);
+    const int q = static_cast<int>(n);
+    const float a = (q & 1) ? cp : sp, b = (q & 1) ? sp : cp;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+// atan2(y, x) in [-pi, pi]: atan of min / max on [0, 1] from an odd polynomial of degree 11 fitted
+// (reweighted least squares toward minimax) to atan, maximum error 1.8e-6 rad, then the octant.
+__device__ __forceinline__ float atan2Guide(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f, a2 = a * a;
+    float p = a * fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, -0.011719105f, 0.052647319f), -0.11642652f), 0.19354044f),
+                                -0.33262286f), 0.99997723f);
+    if (ay > ax) p = 1.57079632679489662f - p;
+    if (x < 0.0f) p = 3.14159265358979323846f - p;
+    return copysignf(p, y);
+}
+
 // Layer-0 input layout. Lane group g (lanes 16g..16g+15, one sample each) feeds the B-operand
 // slots k = 16 kt + 4 g + j (kt = 0..8, j = 0..3); slot t = 4 kt + j of group g holds
 //   t = 0..17  : position item P = 9 g + t / 2, coordinate P / 12 at frequency f = P % 12:
 //                sin(2^f pi p) for even t, cos(2^f pi p) for odd t (72 in all);
 //   t = 18..33 : one-blob bin b = 16 (g % 2) + t - 18 of theta (g < 2) or phi (g >= 2):
-//                exp(-(32 a - b - 0.5)^2 / 2), a = theta / pi or phi / 2 pi in [0, 1) (64 in all);
+//                exp(-(32 a - b - 0.5)^2 / 2), a = theta / pi or phi / 2 pi in [0, 1) (64 in all),
+//                from two anchors (bins 0 and 8 of the lane's run) and the product recurrence
+//                g(d - 1) = g(d) exp(d - 1/2); where an anchor underflows its 8 bins are below 2e-12;
 //   t = 34, 35 : n.x, n.y (g = 0); n.z, 0 (g = 1); 0, 0 (g = 2, 3).
 // Every lane thus computes exactly the 36 features it feeds, with the same instructions as every
 // other lane (no divergence, nothing computed twice). Zero slots: features 135, 138, 139, 142, 143.
-// Frequencies come from one sincospi at the start of each coordinate run and the double-angle
-// recurrence (sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)(cos x + sin x)), at most 8 steps.
+// Frequencies come from one sinCosPi at the start of each coordinate run and the double-angle
+// recurrence (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x), at most 8 steps.
 __device__ __forceinline__ void guideFeatureTiles(const GuideInput& g, int grp, bf16x4 (&x0)[9]) {
     float v[36];
     // position items: a run of coordinate c0 from frequency f0, then coordinate c0 + 1 from 0
@@ -172,27 +202,36 @@ __device__ __forceinline__ void guideFeatureTiles(const GuideInput& g, int grp, 
     const float va = c0 == 0 ? g.px : (c0 == 1 ? g.py : g.pz);
     const float vb = c0 == 0 ? g.py : g.pz;
     float s, c, s1, c1;
-    sincospif(ldexpf(va, f0), &s, &c);  // 2^f0 va is exact
-    sincospif(vb, &s1, &c1);
+    sinCosPi(ldexpf(va, f0), s, c);  // 2^f0 va is exact
+    sinCosPi(vb, s1, c1);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if (i == brk) s = s1, c = c1;
         v[2 * i] = s;
         v[2 * i + 1] = c;
-        const float ns = 2.0f * s * c;
-        c = (c - s) * (c + s);
+        const float t = s + s;  // sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x
+        const float ns = t * c;
+        c = fmaf(-t, s, 1.0f);
         s = ns;
     }
     // one-blob of theta (groups 0, 1) or phi (groups 2, 3): both from one atan2
     const bool th = grp < 2;
-    const float ay = th ? sqrtf(g.wx * g.wx + g.wy * g.wy) : g.wy, ax = th ? g.wz : g.wx;
-    float a = atan2f(ay, ax) * (th ? 0.31830988618379067154f : 0.15915494309189533577f);
+    const float ay = th ? __builtin_amdgcn_sqrtf(g.wx * g.wx + g.wy * g.wy) : g.wy, ax = th ? g.wz : g.wx;
+    float a = atan2Guide(ay, ax) * (th ? 0.31830988618379067154f : 0.15915494309189533577f);
     if (!th && a < 0.0f) a += 1.0f;
     const float u = a * kGBlob - static_cast<float>(16 * (grp & 1)) - 0.5f;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const float d = u - static_cast<float>(i);
-        v[18 + i] = __builtin_amdgcn_exp2f(d * d * -0.72134752044448170368f);  // exp(-d^2 / 2)
+    for (int h = 0; h < 2; ++h) {
+        const float d = u - static_cast<float>(8 * h);  // distance of the anchor bin from the blob's centre
+        float gk = __builtin_amdgcn_exp2f(d * d * -0.72134752044448170368f);      // exp(-d^2 / 2)
+        float rk = __builtin_amdgcn_exp2f((d - 0.5f) * 1.44269504088896340736f);  // exp(d - 1/2)
+        v[18 + 8 * h] = gk;
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            gk = gk * rk;
+            rk = rk * 0.36787944117144233f;  // exp(-1)
+            v[18 + 8 * h + k] = gk;
+        }
     }
     v[34] = grp == 0 ? g.nx : (grp == 1 ? g.nz : 0.0f);
     v[35] = grp == 0 ? g.ny : 0.0f;
@@ -321,13 +360,13 @@ __device__ __forceinline__ f32x4 biasTile(const float* b, int mt, int grp) {
     return *reinterpret_cast<const f32x4*>(b + mt * 16 + 4 * grp);
 }
 
-// guideForwardTile on K = 32 products: layer 0 in 4 K = 32 steps and one K = 16 step per output
-// tile, the other layers in 2 K = 32 steps (50 matrix instructions per 16 samples instead of 96).
-__device__ __forceinline__ void guideForwardTileX32(const GuideLdsX& L, const bf16x4 (&x0)[9], f32x4 (&out)[3]) {
+// Forward pass on K = 32 products: layer 0 in 4 K = 32 steps and one K = 16 tail step per output
+// tile, the other layers in 2 K = 32 steps (50 matrix instructions per 16-sample tile).
+// Layer 0 of one tile: the pre-activations acc (4 output tiles) from its feature tiles x0.
+__device__ __forceinline__ void guideLayer0X32(const GuideLdsX& L, const bf16x4 (&x0)[9], f32x4 (&acc)[4]) {
     const int lane = __lane_id();
     const int s = lane & 15, grp = lane >> 4;
-    f32x4 acc[4];
-    bf16x4 h[4];
+    const bf16x4 z = {};
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         acc[mt] = biasTile(L.b, mt, grp);
@@ -338,32 +377,54 @@ __device__ __forceinline__ void guideForwardTileX32(const GuideLdsX& L, const bf
         // an accumulator chain keeps one MFMA shape. A 16x16x16 product that reads as C the result
         // of a 16x16x32 product issued one or two instructions earlier has read it unfinished on
         // gfx950 (results then depend on timing), and the compiler pads only same-shape chains.
-        const bf16x4 z = {};
         acc[mt] = mfma32(cat(ldsA(L.w0, kGLd0X, mt * 16 + s, 128 + 4 * grp), z), cat(x0[8], z), acc[mt]);
     }
+}
+// Layers 1..4 of T tiles side by side from their layer-0 pre-activations: every weight operand read
+// from LDS feeds the T tiles' products, and the T independent accumulator chains of an output tile
+// keep the matrix pipe busy within one wave.
+template <int T>
+__device__ __forceinline__ void guideHiddenX32(const GuideLdsX& L, f32x4 (&acc)[T][4], f32x4 (&out)[T][3]) {
+    const int lane = __lane_id();
+    const int s = lane & 15, grp = lane >> 4;
+    bf16x4 h[T][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) h[mt] = packRelu(acc[mt]);
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) h[t][mt] = packRelu(acc[t][mt]);
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
         const uint16_t* w = L.wh + l * kGHid * kGLdHX;
         const float* b = L.b + (l + 1) * kGHid;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
-            acc[mt] = biasTile(b, mt, grp);
+            const f32x4 bias = biasTile(b, mt, grp);
 #pragma unroll
-            for (int kt = 0; kt < 2; ++kt)
-                acc[mt] = mfma32(ldsA8(w, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp), cat(h[2 * kt], h[2 * kt + 1]), acc[mt]);
+            for (int t = 0; t < T; ++t) acc[t][mt] = bias;
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt) {
+                const bf16x8 a = ldsA8(w, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp);
+#pragma unroll
+                for (int t = 0; t < T; ++t) acc[t][mt] = mfma32(a, cat(h[t][2 * kt], h[t][2 * kt + 1]), acc[t][mt]);
+            }
         }
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) h[mt] = packRelu(acc[mt]);
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) h[t][mt] = packRelu(acc[t][mt]);
     }
     const float* b4 = L.b + 4 * kGHid;
 #pragma unroll
     for (int mt = 0; mt < 3; ++mt) {
-        out[mt] = biasTile(b4, mt, grp);
+        const f32x4 bias = biasTile(b4, mt, grp);
 #pragma unroll
-        for (int kt = 0; kt < 2; ++kt)
-            out[mt] = mfma32(ldsA8(L.w4, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp), cat(h[2 * kt], h[2 * kt + 1]), out[mt]);
+        for (int t = 0; t < T; ++t) out[t][mt] = bias;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) {
+            const bf16x8 a = ldsA8(L.w4, kGLdHX, mt * 16 + s, 32 * kt + 8 * grp);
+#pragma unroll
+            for (int t = 0; t < T; ++t) out[t][mt] = mfma32(a, cat(h[t][2 * kt], h[t][2 * kt + 1]), out[t][mt]);
+        }
     }
 }
 

@@ -31,11 +31,11 @@ def test_guided_estimator_is_unbiased(dev):
     sc = scenes.hidden_cornell(width=64, height=64, spp=4096)
     s = pg.Scene(dev, sc)
     film = pg.Film(dev, 64, 64)
-    s.render(pg.render_params(sc, spp=4096, spp_range=(0, 2048), flags=pg.PG_FILM_FP64), film)
+    s.render(pg.render_params(sc, spp=4096, flags=pg.PG_FILM_FP64), film)
     plain = img(film.download(fp64=True))
     g = pg.Guide(dev, seed=11, train_records=0)  # random network: a poor but valid proposal
     film.clear()
-    s.render_guided(g, pg.render_params(sc, spp=4096, spp_range=(0, 1024), flags=pg.PG_FILM_FP64), film)
+    s.render_guided(g, pg.render_params(sc, spp=4096, flags=pg.PG_FILM_FP64), film)
     guided = img(film.download(fp64=True))
     assert abs(guided.mean() - plain.mean()) / plain.mean() < 0.01
     blocks = lambda a: a.reshape(8, 8, 8, 8, 3).mean(axis=(1, 3))  # noqa: E731
