@@ -479,6 +479,50 @@ def batched(dev, n_scenes, steps, bvh_width, wavefront_paths):
     return out
 
 
+def batched_small(dev, n_scenes, steps):
+    """Small-scene batch: n seeded Cornell boxes (scenes.small_batch_scene: 64x64 to 256x256, 1 to
+    16 spp, depth 5), each far below a path pool's capacity, in one pg_render_batch call against
+    one pg_render call per scene (each call waits for its own film). Aggregate Mpaths/s and
+    per-scene latency (device time from the batch's start until the scene's film is complete)."""
+    import numpy as np
+
+    import piper2_amd as pg
+    from piper2_amd import scenes
+    scs = [scenes.small_batch_scene(i) for i in range(n_scenes)]
+    objs = [pg.Scene(dev, sc, bvh_width=4) for sc in scs]
+    params = [pg.render_params(sc, flags=pg.PG_NO_STAGE_TIMING) for sc in scs]
+    films = [pg.Film(dev, sc["width"], sc["height"]) for sc in scs]
+    pg.render_batch(dev, objs, params, films)  # warmup
+    paths = rays = 0
+    lats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, lat = pg.render_batch(dev, objs, params, films)
+        paths += st["paths"]
+        rays += st["extension_rays"] + st["shadow_rays"] + st["mis_rays"]
+        lats.append(lat)
+    elapsed = (time.perf_counter() - t0) / steps
+    for s, p, f in zip(objs, params, films):  # warm the per-scene path
+        s.render(p, f)
+    t0 = time.perf_counter()
+    for s, p, f in zip(objs, params, films):
+        s.render(p, f)
+    t_seq = time.perf_counter() - t0
+    lats = np.concatenate(lats)
+    out = {"scenes": n_scenes, "paths_per_batch": int(paths // steps), "steps": steps,
+           "ms_per_batch": round(elapsed * 1e3, 2), "mpaths_per_s": round(paths / steps / elapsed / 1e6, 1),
+           "mrays_per_s": round(rays / steps / elapsed / 1e6, 1),
+           "latency_ms_p50": round(float(np.percentile(lats, 50)), 2),
+           "latency_ms_p99": round(float(np.percentile(lats, 99)), 2),
+           "sequential_ms": round(t_seq * 1e3, 2), "sequential_mpaths_per_s": round(paths / steps / t_seq / 1e6, 1),
+           "speedup_vs_sequential": round(t_seq / elapsed, 3)}
+    for f in films:
+        f.close()
+    for s in objs:
+        s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,6 +536,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--small-batch", type=int, default=256, help="scenes of the small-scene batch leg (0: skip it)")
     ap.add_argument("--guided-batch", type=int, default=16, help="hard rooms of the guided batch leg (0: skip it)")
     ap.add_argument("--guided-batch-budget", type=float, default=12.0,
                     help="render seconds per estimator in the guided batch convergence (0: until 1%%)")
@@ -635,6 +680,9 @@ def main():
     if args.batch_scenes > 0:
         result["batched"] = batched(dev, args.batch_scenes, args.batch_steps, info["width"], args.wavefront_paths)
         log(f"batched: {result['batched']}")
+    if args.small_batch > 0:
+        result["batched_small"] = batched_small(dev, args.small_batch, 3)
+        log(f"batched small scenes: {result['batched_small']}")
     if not args.no_guided:
         # time to 1% relMSE, guided (online training, one-sample MIS at 0.5) vs plain
         result["time_to_1pct_relmse"] = {

@@ -367,6 +367,25 @@ def batch_scene(index, max_depth=8, min_rr=3):
     return sc
 
 
+SMALL_RESOLUTIONS = [(64, 64), (128, 96), (128, 128), (256, 256)]
+SMALL_SPP = [1, 4, 16]
+
+
+def small_batch_scene(index):
+    """Scene `index` of the small-scene batch (seed 100 + index): a Cornell box (config 1's layout,
+    its boxes and wall colours drawn from the seed) at a resolution from SMALL_RESOLUTIONS and a
+    sample count from SMALL_SPP, depth 5; every fourth has plastic boxes, every fourth (offset two)
+    two several-primitive lights. Each holds a few thousand to a million paths: far fewer than a
+    path pool's capacity, the regime a shared pool is for."""
+    rng = np.random.default_rng(100 + index)
+    width, height = SMALL_RESOLUTIONS[int(rng.integers(len(SMALL_RESOLUTIONS)))]
+    spp = SMALL_SPP[int(rng.integers(len(SMALL_SPP)))]
+    sc = cornell(seed=100 + index, width=width, height=height, spp=spp, plastic=index % 4 == 0,
+                 multi_light=index % 4 == 2)
+    sc["name"] = f"small_{index}"
+    return sc
+
+
 def _grid_quad(origin, u, v, n):
     """A parallelogram tessellated into n x n cells (2 n^2 triangles)."""
     origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))

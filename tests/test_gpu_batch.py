@@ -75,6 +75,31 @@ def test_batch_matches_individual_renders(dev, wavefront):
     assert lat.max() <= st["ms_total"] * 1.05 + 1.0
 
 
+def test_small_scene_batch_matches_individual_renders(dev):
+    # 96 small Cornell boxes (64x64 to 256x256, 1 to 16 spp; plastic and several-primitive lights
+    # among them) through one path pool: every film equals its own pg_render call, and the batch's
+    # latencies are ordered within its total
+    scs = [scenes.small_batch_scene(i) for i in range(96)]
+    objs = [pg.Scene(dev, sc) for sc in scs]
+    params = [pg.render_params(sc) for sc in scs]
+    singles = []
+    for s, p, sc in zip(objs, params, scs):
+        f = pg.Film(dev, sc["width"], sc["height"])
+        s.render(p, f)
+        singles.append(f.download())
+        f.close()
+    films = [pg.Film(dev, sc["width"], sc["height"]) for sc in scs]
+    st, lat = pg.render_batch(dev, objs, params, films)
+    assert st["paths"] == sum(sc["width"] * sc["height"] * sc["spp"] for sc in scs)
+    for f, ref, sc in zip(films, singles, scs):
+        assert np.array_equal(f.download(), ref), sc["name"]
+    assert np.all(lat > 0) and np.all(lat <= st["ms_total"] * 1.01)
+    for f in films:
+        f.close()
+    for s in objs:
+        s.close()
+
+
 def test_batch_ring_reuse(dev):
     """Many scenes through small waves: the film record ring wraps and slots are reused."""
     objs, params, singles, films = [], [], [], []
