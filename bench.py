@@ -178,6 +178,34 @@ def guided_profile():
                       "plain": {k: plain[k] for k in keys}, "inference_roofline": guide_roofline(guided)}), flush=True)
 
 
+def inference_profile(n=1 << 22, calls=4):
+    """Profiling workload of the guide network alone (for rocprofv3): n samples of seeded random
+    inputs (position in [0,1]^3, unit normal and direction) through pg_guide_infer, `calls` times,
+    with a perturbed random network; prints the host time per call (with its copies)."""
+    import numpy as np
+
+    import piper2_amd as pg
+    rng = np.random.default_rng(0)
+    pos = rng.random((n, 3), dtype=np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    dev = pg.Device(0)
+    g = pg.Guide(dev, seed=3)
+    th = g.get_params()
+    g.set_params(th + rng.normal(size=th.shape).astype(np.float32) * 0.05)
+    times = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        mix = g.infer(pos, nrm, wo)
+        times.append(time.perf_counter() - t0)
+    assert np.isfinite(mix).all()
+    print(json.dumps({"workload": f"guide_inference_{n}_samples", "calls": calls,
+                      "host_ms_per_call": [round(t * 1e3, 2) for t in times],
+                      "flop_per_sample": GUIDE_FLOP_PER_SAMPLE}), flush=True)
+
+
 def equal_time(trace_a, trace_b):
     """relMSE of trace b at the times of trace a's levels (log-log interpolation between b's levels),
     as (seconds, relMSE a, relMSE b) for every time both traces cover."""
@@ -544,6 +572,8 @@ def main():
                     help="estimators of the guided batch convergence (comma-separated)")
     ap.add_argument("--only-guided-batch", action="store_true", help="run the guided batch leg alone")
     ap.add_argument("--guided-profile", action="store_true", help="only the guided profiling workload (for rocprofv3)")
+    ap.add_argument("--inference-profile", action="store_true",
+                    help="only the guide-network inference workload (for rocprofv3)")
     ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -551,6 +581,9 @@ def main():
               "scenes with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.inference_profile:
+        inference_profile()
+        return
     if args.guided_profile:
         guided_profile()
         return
