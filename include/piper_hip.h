@@ -330,7 +330,9 @@ typedef struct pg_guide pg_guide;
 typedef struct pg_guide_params {
     uint32_t seed;           /* weight initialisation */
     float lr, beta1, beta2, eps; /* Adam */
-    float mis_fraction;      /* probability of drawing the continuation from the guide */
+    float mis_fraction;      /* probability of drawing the continuation from the guide; negative: the
+                                network learns it per vertex (an extra output in [0.05, 0.95], trained
+                                with the combined density mis_fraction p_guide + (1 - mis_fraction) p_bsdf) */
     uint32_t train_batch;    /* records per Adam step (>= 64) */
     uint32_t train_records;  /* records a guided render collects for training (0: none) */
 } pg_guide_params;
@@ -339,11 +341,14 @@ void pg_guide_destroy(pg_guide* guide);
 uint32_t pg_guide_param_count(void); /* 24880 FP32 parameters: per layer W[out][in] then b[out] */
 int pg_guide_get_params(pg_guide* guide, float* theta, uint32_t n);
 int pg_guide_set_params(pg_guide* guide, const float* theta, uint32_t n); /* also resets Adam */
-/* in[12 n] = per sample (pos01 xyz, -, normal xyz, -, wo xyz, -); mix[48 n] = 8 x (mu xyz, kappa),
- * then 8 lobe weights pi, then 8 x pi * kappa / (2 pi (1 - exp(-2 kappa))). */
+/* in[12 n] = per sample (pos01 xyz, -, normal xyz, -, wo xyz, -); mix[52 n] = 8 x (mu xyz, kappa),
+ * then 8 lobe weights pi, then 8 x pi * kappa / (2 pi (1 - exp(-2 kappa))), then the learned
+ * selection probability and three zeros. */
 int pg_guide_infer(pg_guide* guide, const float* in, uint32_t n, float* mix);
-/* Training records rec[16 n] = (pos01 xyz, weight c, normal, -, wo, -, sampled direction, -).
- * Loss = -sum c log p(direction) / sum c. gradients: FP32 dLoss/dtheta, no update. */
+/* Training records rec[16 n] = (pos01 xyz, weight c, normal, BSDF pdf, wo, -, sampled direction, -).
+ * Loss = -sum c log p(direction) / sum c, p the guide's density, or with a learned mis_fraction the
+ * combined density alpha p_guide + (1 - alpha) p_bsdf with alpha the network's selection output.
+ * gradients: FP32 dLoss/dtheta, no update. */
 int pg_guide_gradients(pg_guide* guide, const float* rec, uint32_t n, float* loss, float* grads);
 /* Adam steps over rec in batches of train_batch; *loss = loss of the last batch. */
 int pg_guide_train(pg_guide* guide, const float* rec, uint32_t n, float* loss);

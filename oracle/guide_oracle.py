@@ -13,6 +13,9 @@ kernels implement, so that their forward pass, loss and gradients can be checked
             mu = m / |m|, kappa = exp(clamp(r, KLO, KHI)), pi = softmax(logits).
   pdf     : p(w) = sum_i pi_i * kappa_i / (2 pi (1 - exp(-2 kappa_i))) * exp(kappa_i (mu_i . w - 1)).
   loss    : L = -sum_s c_s log p(w_s) / sum_s c_s   (c_s: the sample's radiance / sampling pdf).
+  learned selection (optional): output row ALPHA_ROW is the logit z of the probability of sampling
+            the guide, alpha = ALO + (AHI - ALO) sigmoid(z); the loss is then that of the one-sample
+            MIS density q = alpha p + (1 - alpha) p_bsdf, with p_bsdf given per sample.
 
 Weights and the inputs of every layer are rounded to bfloat16 (round to nearest even) exactly as
 the kernels feed the matrix cores; sums, biases and everything after the last layer are FP64.
@@ -27,6 +30,7 @@ NOUT = 40
 NOUT_PAD = 48
 NLOBES = 8
 KLO, KHI = -4.0, 9.0
+ALO, AHI = 0.05, 0.95
 LAYERS = [(NHID, NIN), (NHID, NHID), (NHID, NHID), (NHID, NHID), (NOUT_PAD, NHID)]
 
 
@@ -73,8 +77,11 @@ def output_row(l, k):
 
 
 LOBE_ROWS = np.array([[output_row(l, k) for k in range(5)] for l in range(NLOBES)])  # (8, 5)
-# Output rows that are always zero (slots 10 and 11 of every lane group).
+# Output rows outside the lobes (slots 10 and 11 of every lane group): zero, except slot 10 of lane
+# group 0, the selection logit, when a guide learns its selection probability.
 PAD_OUTPUTS = sorted(set(range(NOUT_PAD)) - set(LOBE_ROWS.reshape(-1).tolist()))
+ALPHA_ROW = 16 * (10 // 4) + 10 % 4  # lane group 0, slot 10
+assert ALPHA_ROW in PAD_OUTPUTS
 
 
 def encode(pos01, normal, wo):
@@ -146,12 +153,25 @@ def pdf(theta, feats, w, quant=True):
     return (pi * vmf(kappa, c)).sum(axis=1)
 
 
-def loss(theta, feats, w, c, quant=True):
-    p = pdf(theta, feats, w, quant)
+def selection(out):
+    """Learned probability of sampling the guide, from the selection logit row."""
+    return ALO + (AHI - ALO) / (1.0 + np.exp(-out[:, ALPHA_ROW]))
+
+
+def loss(theta, feats, w, c, quant=True, pb=None):
+    """pb: the BSDF's density of each w (learned selection: the loss of the combined density)."""
+    if pb is None:
+        p = pdf(theta, feats, w, quant)
+    else:
+        out, _ = forward(theta, feats, quant)
+        mu, kappa, pi, _ = mixture(out)
+        pg = (pi * vmf(kappa, np.einsum("nlk,nk->nl", mu, w))).sum(axis=1)
+        a = selection(out)
+        p = a * pg + (1.0 - a) * pb
     return float(-(c * np.log(p)).sum() / c.sum())
 
 
-def loss_and_grad(theta, feats, w, c, quant=True):
+def loss_and_grad(theta, feats, w, c, quant=True, pb=None):
     """Analytic gradient of loss() with respect to theta (bf16 rounding treated as identity)."""
     q = bf16 if quant else (lambda a: np.asarray(a, np.float64))
     layers = unpack(theta)
@@ -161,8 +181,17 @@ def loss_and_grad(theta, feats, w, c, quant=True):
     v = vmf(kappa, cosang)
     p = (pi * v).sum(axis=1)
     csum = c.sum()
-    L = float(-(c * np.log(p)).sum() / csum)
-    dLdp = -(c / csum) / p                            # (N,)
+    if pb is None:
+        L = float(-(c * np.log(p)).sum() / csum)
+        dLdp = -(c / csum) / p                        # (N,)
+    else:  # q = alpha p + (1 - alpha) pb, alpha = ALO + (AHI - ALO) sigmoid(z)
+        sg = 1.0 / (1.0 + np.exp(-out[:, ALPHA_ROW]))
+        a = ALO + (AHI - ALO) * sg
+        qd = a * p + (1.0 - a) * pb
+        L = float(-(c * np.log(qd)).sum() / csum)
+        dLdq = -(c / csum) / qd
+        dLdp = dLdq * a
+        dz_alpha = dLdq * (p - pb) * (AHI - ALO) * sg * (1.0 - sg)
     g = dLdp[:, None] * v                             # dL/dpi_i
     da = pi * (g - (pi * g).sum(axis=1, keepdims=True))
     dv = dLdp[:, None] * pi                           # dL/dv_i
@@ -177,6 +206,8 @@ def loss_and_grad(theta, feats, w, c, quant=True):
     dout[:, LOBE_ROWS[:, 0:3]] = dm
     dout[:, LOBE_ROWS[:, 3]] = dr
     dout[:, LOBE_ROWS[:, 4]] = da
+    if pb is not None:
+        dout[:, ALPHA_ROW] = dz_alpha
     grads = []
     dz = dout
     for li in range(len(layers) - 1, -1, -1):
@@ -190,9 +221,10 @@ def loss_and_grad(theta, feats, w, c, quant=True):
     flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
     # the zero rows of the output layer are never trained
     k = sum(o * i + o for o, i in LAYERS[:-1])
+    zero = [o for o in PAD_OUTPUTS if pb is None or o != ALPHA_ROW]
     Wlast = flat[k:k + NOUT_PAD * NHID].reshape(NOUT_PAD, NHID)
-    Wlast[PAD_OUTPUTS] = 0.0
-    flat[k + NOUT_PAD * NHID + np.array(PAD_OUTPUTS)] = 0.0
+    Wlast[zero] = 0.0
+    flat[k + NOUT_PAD * NHID + np.array(zero)] = 0.0
     return L, flat
 
 

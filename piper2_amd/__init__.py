@@ -441,15 +441,19 @@ class Guide:
         n = pos01.shape[0]
         inp = np.zeros((n, 12), np.float32)
         inp[:, 0:3], inp[:, 4:7], inp[:, 8:11] = pos01, normal, wo
-        mix = np.zeros((n, 48), np.float32)
+        mix = np.zeros((n, 52), np.float32)
         check(lib().pg_guide_infer(self.h, _ptr(inp), n, _ptr(mix)))
         return mix
 
     @staticmethod
-    def records(pos01, normal, wo, direction, c):
+    def records(pos01, normal, wo, direction, c, bsdf_pdf=None):
+        """Training records; bsdf_pdf: the BSDF's density of each direction (read with a learned
+        mis_fraction)."""
         n = pos01.shape[0]
         rec = np.zeros((n, 16), np.float32)
         rec[:, 0:3], rec[:, 3], rec[:, 4:7], rec[:, 8:11], rec[:, 12:15] = pos01, c, normal, wo, direction
+        if bsdf_pdf is not None:
+            rec[:, 7] = bsdf_pdf
         return rec
 
     def gradients(self, rec):

@@ -283,7 +283,7 @@ void guideStep(pg_guide* g, const float* rec, uint32_t n, bool update) {
     PG_HIP(hipMemsetAsync(g->loss.p, 0, 4, st));
     pg::launchGuideGradients(st, rec, n, b, g->wb.as<uint16_t>(), g->theta.as<float>(), g->scratch.p, g->csum.as<float>(),
                              g->partial.as<float>(), g->loss.as<float>(), g->tiles.as<pg::WGradTile>(),
-                             static_cast<uint32_t>(g->dev->numCUs));
+                             static_cast<uint32_t>(g->dev->numCUs), g->p.mis_fraction < 0.0f);
     pg::AdamParams ap{g->p.lr, g->p.beta1, g->p.beta2, g->p.eps, 1.0f, 1.0f, update ? 1u : 0u};
     if (update) {
         ++g->step;
@@ -1921,7 +1921,7 @@ int pg_debug_copy(pg_device* dev, uint64_t bytes, uint32_t reps, double* gbps) {
 int pg_guide_create(pg_device* dev, const pg_guide_params* params, pg_guide** out) {
     return guarded([&] {
         require(dev && params && out, "null argument");
-        require(params->mis_fraction >= 0.0f && params->mis_fraction < 1.0f, "mis_fraction must be in [0, 1)");
+        require(params->mis_fraction < 1.0f, "mis_fraction must be below 1 (negative: learned)");
         require(params->lr > 0.0f && params->train_batch >= 64, "bad learning rate or batch");
         std::lock_guard<std::mutex> lock(dev->mu);
         dev->checkFault();
@@ -1990,13 +1990,13 @@ int pg_guide_infer(pg_guide* g, const float* in, uint32_t n, float* mix) {
         hipStream_t st = g->dev->stream;
         DevBuf din, dmix;
         upload(din, in, static_cast<size_t>(n) * 48, st);
-        dmix.reserve(static_cast<size_t>(n) * 48 * 4);
+        dmix.reserve(static_cast<size_t>(n) * pg::kGMixFloatsHost * 4);
         guideSetLds();
         pg::launchGuideInferArray(st, din.as<float>(), n, g->wb.as<uint16_t>(), g->theta.as<float>(), dmix.as<float>(),
                                   static_cast<uint32_t>(g->dev->numCUs) * 2);
         PG_HIP(hipGetLastError());
         PG_HIP(hipStreamSynchronize(st));
-        PG_HIP(hipMemcpy(mix, dmix.p, static_cast<size_t>(n) * 48 * 4, hipMemcpyDeviceToHost));
+        PG_HIP(hipMemcpy(mix, dmix.p, static_cast<size_t>(n) * pg::kGMixFloatsHost * 4, hipMemcpyDeviceToHost));
     });
 }
 

@@ -94,6 +94,7 @@ struct PathSource {
 // kappa and weight logit). Only the softmax over the 8 weights crosses lanes (s, s+16, s+32, s+48).
 struct LobePair {
     float mx[2], my[2], mz[2], mn[2], r[2], kappa[2], pi[2];
+    float z;  // logit of the learned selection probability (lane group 0's slot 10, on every lane)
 };
 __device__ __forceinline__ float outSlot(const f32x4 (&out)[3], int u) { return out[u >> 2][u & 3]; }
 __device__ __forceinline__ LobePair lobesFromTile(const f32x4 (&out)[3]) {
@@ -119,6 +120,7 @@ __device__ __forceinline__ LobePair lobesFromTile(const f32x4 (&out)[3]) {
     sum += __shfl_xor(sum, 32);
     const float rs = __builtin_amdgcn_rcpf(sum);
     L.pi[0] = e0 * rs, L.pi[1] = e1 * rs;
+    L.z = __shfl(outSlot(out, kGAlphaSlot), static_cast<int>(__lane_id() & 15u));
     return L;
 }
 
@@ -149,6 +151,7 @@ __device__ __forceinline__ void storeLobes(uint32_t i, const LobePair& lp, int g
     reinterpret_cast<float2*>(o + 32)[grp] = make_float2(lp.pi[0], lp.pi[1]);
     reinterpret_cast<float2*>(o + 40)[grp] =
         make_float2(lp.pi[0] * vmfNormSampled(lp.kappa[0]), lp.pi[1] * vmfNormSampled(lp.kappa[1]));
+    if (grp == 0) reinterpret_cast<float4*>(o)[12] = make_float4(alphaFromLogit(lp.z), 0.0f, 0.0f, 0.0f);
 }
 
 // Mixtures of one wave's 64 samples: lane l's sample is queue/array entry `entry` (~0u: none) with
@@ -406,6 +409,7 @@ struct TrainBuffers {
     uint16_t* xh;  // 4 x 64 rows: inputs of layers 1..4
     uint16_t* dz;  // 4 x 64 rows (layers 0..3) + 48 rows (layer 4)
     uint32_t npad;
+    uint32_t learnAlpha;  // loss of the one-sample MIS density with the learned selection probability
 };
 
 __device__ __forceinline__ void storeRows(uint16_t* dst, uint32_t npad, int row0, int grp, uint32_t col, bf16x4 v) {
@@ -471,14 +475,28 @@ __global__ void __launch_bounds__(256) kGuideTrainFB(const float4* __restrict__ 
         p += __shfl_xor(p, 16);
         p += __shfl_xor(p, 32);
         p = fmaxf(p, 1e-30f);
-        if (grp == 0) lossPart += -c * logf(p);
-        const float dLdp = -c / p;
+        // learned selection: loss of q = alpha p + (1 - alpha) p_bsdf, alpha = alphaFromLogit(z)
+        float dLdp, dz10 = 0.0f;
+        if (tb.learnAlpha) {
+            const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-lp.z));
+            const float al = kGAlphaLo + (kGAlphaHi - kGAlphaLo) * sg;
+            const float pb = fmaxf(r1.w, 0.0f);
+            const float qd = fmaxf(al * p + (1.0f - al) * pb, 1e-30f);
+            if (grp == 0) lossPart += -c * logf(qd);
+            const float dLdq = -c / qd;
+            dLdp = dLdq * al;
+            dz10 = dLdq * (p - pb) * (kGAlphaHi - kGAlphaLo) * sg * (1.0f - sg);
+        } else {
+            if (grp == 0) lossPart += -c * logf(p);
+            dLdp = -c / p;
+        }
         // softmax: dL/da_i = pi_i (g_i - sum_j pi_j g_j), g_i = dL/dp v_i
         float gpi = lp.pi[0] * dLdp * v[0] + lp.pi[1] * dLdp * v[1];
         gpi += __shfl_xor(gpi, 16);
         gpi += __shfl_xor(gpi, 32);
-        float gs[12];  // dL/d(output slot u); slots 10, 11 are zero rows
-        gs[10] = gs[11] = 0.0f;
+        float gs[12];  // dL/d(output slot u); slot 10 of lane group 0 is the selection logit, the rest zero rows
+        gs[10] = grp == 0 ? dz10 : 0.0f;
+        gs[11] = 0.0f;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const float dv = dLdp * lp.pi[q];
@@ -727,10 +745,10 @@ void buildGuideTiles(void* scratch, const GuideBatchLayout& b, WGradTile* host) 
 
 void launchGuideGradients(hipStream_t st, const float* rec, uint32_t n, const GuideBatchLayout& b, const uint16_t* wb,
                           const float* theta, void* scratch, float* csum, float* partial, float* lossAcc,
-                          const WGradTile* tilesDev, uint32_t maxBlocks) {
+                          const WGradTile* tilesDev, uint32_t maxBlocks, bool learnAlpha) {
     uint16_t* base = static_cast<uint16_t*>(scratch);
     TrainBuffers tb{base, base + static_cast<size_t>(kGIn) * b.npad, base + static_cast<size_t>(kGIn + 4 * kGHid) * b.npad,
-                    b.npad};
+                    b.npad, learnAlpha ? 1u : 0u};
     hipLaunchKernelGGL(kGuideSumC, dim3(1), dim3(256), 0, st, reinterpret_cast<const float4*>(rec), n, csum);
     hipLaunchKernelGGL(kGuideTrainFB, dim3(gridFor(b.npad, 64, maxBlocks)), dim3(256), kTrainLds, st,
                        reinterpret_cast<const float4*>(rec), n, wb, theta, csum, tb, lossAcc);

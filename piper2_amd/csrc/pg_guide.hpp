@@ -26,7 +26,9 @@ namespace pg {
 constexpr int kGIn = 144, kGHid = 64, kGOut = 48, kGLobes = 8;
 // Output layout: lane group g holds rows 16 mt + 4 g + j (mt = 0..2, j = 0..3) of its sample; slot
 // u = 4 mt + j is lobe 2 g + u / 5, parameter u % 5 (mu x, mu y, mu z, log kappa, weight logit) for
-// u < 10. Slots 10 and 11 are the zero rows 34, 35, 38, 39, 42, 43, 46, 47.
+// u < 10. Slot 10 of lane group 0 (row 34) is the logit of the learned guide-vs-BSDF selection
+// probability (trained only when a guide learns it, zero otherwise); the other slots 10 and 11 are
+// the zero rows 35, 38, 39, 42, 43, 46, 47.
 constexpr int kGFreq = 12, kGBlob = 32;
 // FP32 parameter offsets (floats), layer l: W then b
 constexpr int kGW0 = 0, kGB0 = kGW0 + kGHid * kGIn;
@@ -37,8 +39,16 @@ constexpr int kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGOut * kGHid;
 constexpr int kGParams = kGB4 + kGOut;  // 24880
 static_assert(kGParams == 24880, "guide parameter count");
 constexpr float kGKappaLo = -4.0f, kGKappaHi = 9.0f;  // log-kappa clamp
-// Mixture record per sample: 8 x (mu xyz, kappa), 8 weights pi, 8 x pi * vMF normalisation
-constexpr int kGMixFloats = 48;
+// Mixture record per sample: 8 x (mu xyz, kappa), 8 weights pi, 8 x pi * vMF normalisation, then
+// the learned selection probability of the guide and three zeros.
+constexpr int kGMixFloats = 52;
+constexpr int kGAlphaSlot = 10;                     // output slot of lane group 0 holding its logit
+constexpr float kGAlphaLo = 0.05f, kGAlphaHi = 0.95f;  // a learned selection probability stays inside
+// Selection probability from its logit: kGAlphaLo + (kGAlphaHi - kGAlphaLo) sigmoid(z), so that both
+// the guide and the BSDF keep a share of the samples wherever the guide is used.
+__device__ __forceinline__ float alphaFromLogit(float z) {
+    return kGAlphaLo + (kGAlphaHi - kGAlphaLo) * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 
 // LDS row strides (bf16 elements): 4 extra elements keep the 16 rows of one A-operand read on
 // distinct banks.

@@ -12,7 +12,7 @@ namespace pg {
 constexpr uint32_t kGTiles = 96;       // 16x16 weight-gradient tiles over the five layers
 constexpr uint32_t kGMaxChunks = 64;   // sample chunks of the split-K weight gradient
 constexpr uint32_t kGParamCount = 24880;
-constexpr uint32_t kGMixFloatsHost = 48;  // mixture record floats per sample
+constexpr uint32_t kGMixFloatsHost = 52;  // mixture record floats per sample
 
 struct WGradTile {
     uint16_t rows, cols, mo, ki;  // layer shape and tile coordinates
@@ -64,11 +64,13 @@ void launchGuideInferGrouped(hipStream_t st, const BatchTable& bt, const PathSta
                              const uint32_t* count, uint32_t capacity, uint32_t nKeys, const GroupBuffers& gb,
                              const uint16_t* const* keyWb, const float* const* keyTheta, float* mix,
                              uint32_t maxBlocks);
-// Forward + backward over n training records (4 float4 each: (pos01, c) (normal) (wo) (direction)),
-// then the split-K weight gradients into partial[chunks][kGParamCount].
+// Forward + backward over n training records (4 float4 each: (pos01, c) (normal, BSDF pdf) (wo)
+// (direction)), then the split-K weight gradients into partial[chunks][kGParamCount]. learnAlpha: the
+// loss is that of the one-sample MIS density alpha p_guide + (1 - alpha) p_bsdf with alpha the
+// network's own selection output (trained too); otherwise that of p_guide alone.
 void launchGuideGradients(hipStream_t st, const float* rec, uint32_t n, const GuideBatchLayout& b, const uint16_t* wb,
                           const float* theta, void* scratch, float* csum, float* partial, float* lossAcc,
-                          const WGradTile* tilesDev, uint32_t maxBlocks);
+                          const WGradTile* tilesDev, uint32_t maxBlocks, bool learnAlpha);
 void launchGuideAdam(hipStream_t st, const float* partial, uint32_t chunks, float* theta, float* m, float* v,
                      uint16_t* wb, float* gradOut, const AdamParams& ap);
 void launchGuideToBf16(hipStream_t st, const float* theta, uint16_t* wb);

@@ -207,9 +207,11 @@ __device__ __forceinline__ bool isTrainPath(const ItemGuide& ig, uint32_t g) {
 // any 8-lobe mixture the network can express: such vertices draw from the BSDF alone (the guide
 // fraction is 0 there, which keeps the one-sample MIS estimator unbiased) and record no history.
 constexpr float kGuideMinAlpha = 0.04f;
-__device__ __forceinline__ float guideFraction(float alpha, const Mat& m) {
+// alpha < 0: the network's learned selection probability, the mixture record's float 48.
+__device__ __forceinline__ float guideFraction(float alpha, const Mat& m, const float* mix) {
     if (m.type() == kDevRoughGlass) return 0.0f;  // the guide proposes reflected directions only
-    return (m.type() != 1u || m.alpha >= kGuideMinAlpha) ? alpha : 0.0f;
+    if (m.type() == 1u && m.alpha < kGuideMinAlpha) return 0.0f;
+    return alpha < 0.0f ? mix[48] : alpha;
 }
 
 // Queue records of one shade block, staged in LDS until the block knows where its records go.
@@ -370,11 +372,12 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 const float uc1 = d9 ? c1 : c0, uc2 = d9 ? c0 : c1;
                 BsdfSample ss;
                 float pdfMix = 0.0f, a = 0.0f;
+                float pb = 0.0f;  // the BSDF's density of the drawn direction (training with a learned alpha)
                 if (!GUIDED || specV) {
                     ss = bsdfSample(mat, wo, ng, uc1, uc2);
                 } else {
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
-                    a = guideFraction(BATCH ? bt.guide[sceneOf(meta.z)].alpha : gs.alpha, mat);
+                    a = guideFraction(BATCH ? bt.guide[sceneOf(meta.z)].alpha : gs.alpha, mat, mix);
                     bool ok = true;
                     if (uc1 < a) {
                         const float3 w = mixtureSample(mix, uc1 / a, uc2);
@@ -388,7 +391,8 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     ss.pdf = 0.0f;
                     if (ok) {
                         const V3 ww = fr.toWorld(ss.wi);
-                        pdfMix = (1.0f - a) * bsdfPdf(mat, wo, ss.wi);
+                        pb = bsdfPdf(mat, wo, ss.wi);
+                        pdfMix = (1.0f - a) * pb;
                         if (a > 0.0f) pdfMix += a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z));
                         ss.f = bsdfEval(mat, wo, ss.wi, ng);
                         ss.pdf = pdfMix;
@@ -425,7 +429,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                             const float py = fminf(fmaxf((hp.y - lo.y) * ie.y, 0.0f), 1.0f);
                             const float pz = fminf(fmaxf((hp.z - lo.z) * ie.z, 0.0f), 1.0f);
                             h[0] = make_float4(px, py, pz, pdfMix);
-                            h[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+                            h[1] = make_float4(nrm.x, nrm.y, nrm.z, pb);
                             h[2] = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, 0.0f);
                             h[3] = make_float4(nd.x, nd.y, nd.z, 0.0f);
                             h[4] = make_float4(thr.x, thr.y, thr.z, 0.0f);

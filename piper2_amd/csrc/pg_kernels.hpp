@@ -32,9 +32,9 @@ constexpr uint32_t kMetaPending = 0x2u;
 
 // Guiding state handed to the shade and record stages.
 struct GuideShade {
-    const float* mix;  // 48 floats per extension-queue entry (pg_guide.hpp layout)
+    const float* mix;  // kGMixFloats per extension-queue entry (pg_guide.hpp layout)
     float4* hist;      // per path slot: histDepth records of 6 float4
-    float alpha;       // probability of sampling the guide
+    float alpha;       // probability of sampling the guide (negative: the network's learned one)
     float3 lo, invExt; // scene box -> [0,1]^3
     uint32_t histDepth, trainStride, trainPhase;
     uint32_t enabled, train;
@@ -55,7 +55,7 @@ struct ItemGuide {
     const float* theta;    // and its FP32 parameters (biases)
     GuideRecordOut tr;     // the guide's training records (capacity 0: no training)
     float3 lo, invExt;     // the scene's box -> [0,1]^3
-    float alpha;           // probability of sampling the guide
+    float alpha;           // probability of sampling the guide (negative: the network's learned one)
     uint32_t key;          // index of the guide among the batch's distinct guides
     uint32_t trainStride, trainPhase;
 };

@@ -126,6 +126,39 @@ def test_gradients_match_oracle(dev):
     assert np.array_equal(g.gradients(pg.Guide.records(p01, nrm, wo, d, c))[1], grad_g)
 
 
+def test_learned_selection_gradients_match_oracle(dev):
+    # mis_fraction < 0: the loss is that of alpha p_guide + (1 - alpha) p_bsdf with alpha the
+    # network's selection output; its gradients (the selection row's included) match the oracle's
+    rng = np.random.default_rng(6)
+    g = pg.Guide(dev, seed=6, mis_fraction=-1.0)
+    th = g.get_params().astype(np.float64)
+    k4 = sum(o * i + o for o, i in go.LAYERS[:-1])
+    th[k4 + go.ALPHA_ROW * go.NHID:k4 + (go.ALPHA_ROW + 1) * go.NHID] = rng.normal(0, 0.2, go.NHID)
+    g.set_params(th.astype(np.float32))
+    th = g.get_params().astype(np.float64)
+    n = 4096
+    p01, nrm, wo = _inputs(rng, n)
+    d = _unit(rng, n)
+    c = rng.uniform(0.0, 2.0, n).astype(np.float32)
+    pb = rng.uniform(0.05, 1.0, n).astype(np.float32)
+    loss_g, grad_g = g.gradients(pg.Guide.records(p01, nrm, wo, d, c, bsdf_pdf=pb))
+    L, grad_o = go.loss_and_grad(th, go.encode(p01.astype(np.float64), nrm.astype(np.float64), wo.astype(np.float64)),
+                                 d.astype(np.float64), c.astype(np.float64), pb=pb.astype(np.float64))
+    assert abs(loss_g - L) < 2e-3 * abs(L)
+    cos = np.dot(grad_g, grad_o) / (np.linalg.norm(grad_g) * np.linalg.norm(grad_o))
+    assert cos > 0.995, cos
+    arow = slice(k4 + go.ALPHA_ROW * go.NHID, k4 + (go.ALPHA_ROW + 1) * go.NHID)
+    a, b = grad_g[arow], grad_o[arow]
+    assert np.linalg.norm(b) > 0 and np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.99
+    # the mixtures carry the learned selection probability, inside its bounds
+    mix = g.infer(p01, nrm, wo)
+    al = go.selection(go.forward(th, go.encode(p01.astype(np.float64), nrm.astype(np.float64),
+                                               wo.astype(np.float64)))[0])
+    assert np.all((mix[:, 48] >= go.ALO) & (mix[:, 48] <= go.AHI))
+    assert np.median(np.abs(mix[:, 48] - al)) < 5e-3
+    assert not np.any(mix[:, 49:52])
+
+
 def test_adam_step_matches_oracle(dev):
     rng = np.random.default_rng(4)
     g = pg.Guide(dev, seed=4, lr=1e-2, train_batch=1024)

@@ -43,6 +43,33 @@ def test_guided_estimator_is_unbiased(dev):
     assert np.median(rel) < 0.02
 
 
+def test_learned_selection_estimator_is_unbiased(dev):
+    # a random network whose selection row is live: every vertex samples the guide with its own
+    # probability alpha in [0.05, 0.95], and the combined density keeps the estimator unbiased
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import guide_oracle as go
+    sc = scenes.hidden_cornell(width=64, height=64, spp=4096)
+    s = pg.Scene(dev, sc)
+    film = pg.Film(dev, 64, 64)
+    s.render(pg.render_params(sc, spp=4096, flags=pg.PG_FILM_FP64), film)
+    plain = img(film.download(fp64=True))
+    g = pg.Guide(dev, seed=12, train_records=0, mis_fraction=-1.0)
+    th = g.get_params()
+    k4 = sum(o * i + o for o, i in go.LAYERS[:-1])
+    rng = np.random.default_rng(12)
+    th[k4 + go.ALPHA_ROW * go.NHID:k4 + (go.ALPHA_ROW + 1) * go.NHID] = rng.normal(0, 0.5, go.NHID)
+    g.set_params(th)
+    film.clear()
+    s.render_guided(g, pg.render_params(sc, spp=4096, flags=pg.PG_FILM_FP64), film)
+    guided = img(film.download(fp64=True))
+    assert abs(guided.mean() - plain.mean()) / plain.mean() < 0.01
+    blocks = lambda a: a.reshape(8, 8, 8, 8, 3).mean(axis=(1, 3))  # noqa: E731
+    rel = np.abs(blocks(guided) - blocks(plain)) / blocks(plain)
+    assert np.median(rel) < 0.02
+
+
 def test_trained_guide_beats_plain_and_untrained(dev):
     sc = scenes.hidden_cornell(width=128, height=128, spp=4096)
     s = pg.Scene(dev, sc)

@@ -97,6 +97,52 @@ def test_loss_gradient_matches_finite_differences():
         assert abs(fd - grad[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, grad[k])
 
 
+def test_learned_selection_gradient_matches_finite_differences():
+    """With a learned selection probability the loss is that of alpha p_guide + (1 - alpha) p_bsdf:
+    its gradient (lobes and the selection logit row alike) matches central differences, and the
+    logit's gradient has the sign the closed form gives (alpha grows where the guide's density
+    exceeds the BSDF's at the sampled directions)."""
+    rng = np.random.default_rng(5)
+    theta = go.init_params(seed=5)
+    layers = go.unpack(theta)
+    for _, b in layers:
+        b[:] = rng.normal(0, 0.1, b.shape)
+    layers[4][1][go.NOUT:] = 0.0
+    W4, b4 = layers[4]
+    W4[go.ALPHA_ROW] = rng.normal(0, 0.2, W4.shape[1])  # a live selection row
+    b4[go.ALPHA_ROW] = 0.3
+    n = 64
+    feats = go.encode(rng.uniform(0, 1, (n, 3)), _unit(rng, n), _unit(rng, n))
+    w = _unit(rng, n)
+    c = rng.uniform(0.0, 2.0, n)
+    pb = rng.uniform(0.05, 1.0, n)
+    _, grad = go.loss_and_grad(theta, feats, w, c, quant=False, pb=pb)
+    k4 = sum(o * i + o for o, i in go.LAYERS[:-1])
+    arow = np.arange(k4 + go.ALPHA_ROW * go.NHID, k4 + (go.ALPHA_ROW + 1) * go.NHID)
+    abias = k4 + go.NOUT_PAD * go.NHID + go.ALPHA_ROW
+    assert np.any(grad[arow] != 0) and grad[abias] != 0
+    live = np.flatnonzero(grad != 0)
+    pick = np.concatenate([rng.choice(live, 50, replace=False), rng.choice(arow, 8, replace=False), [abias]])
+    h = 1e-6
+    for k in pick:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (go.loss(tp, feats, w, c, quant=False, pb=pb) - go.loss(tm, feats, w, c, quant=False, pb=pb)) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, grad[k])
+    # the logit bias moves alpha toward the better density: dL/dz = -sum c (p - pb) alpha' / q / sum c
+    out, _ = go.forward(theta, feats, quant=False)
+    mu, kappa, pi, _ = go.mixture(out)
+    pgd = (pi * go.vmf(kappa, np.einsum("nlk,nk->nl", mu, w))).sum(1)
+    sg = 1 / (1 + np.exp(-out[:, go.ALPHA_ROW]))
+    a = go.ALO + (go.AHI - go.ALO) * sg
+    closed = (-c * (pgd - pb) * (go.AHI - go.ALO) * sg * (1 - sg) / (a * pgd + (1 - a) * pb)).sum() / c.sum()
+    assert abs(grad[abias] - closed) <= 1e-9 + 1e-9 * abs(closed)
+    # without BSDF densities the selection row carries no gradient (it stays a zero row)
+    _, g0 = go.loss_and_grad(theta, feats, w, c, quant=False)
+    assert not np.any(g0[arow]) and g0[abias] == 0
+
+
 def test_adam_constant_gradient_closed_form():
     """With a constant gradient g, bias-corrected Adam moves every parameter by lr g / (|g| + eps)
     per step (m_t / (1 - b1^t) = g and v_t / (1 - b2^t) = g^2 exactly); without the correction the
