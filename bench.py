@@ -228,7 +228,7 @@ def pg_params(sc, spp_range, flags=0):
     return pg.render_params(sc, spp=SPP_DECLARED, spp_range=spp_range, flags=flags)
 
 
-def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_level=14):
+def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_level=14, learned=False):
     """Guided (online-trained, one-sample MIS at 0.5) against the plain tracer on one scene.
 
     The reference R is an FP64-film render of ref_spp samples by the guided estimator with a guide
@@ -236,7 +236,9 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
     for any guide; a trained one reaches a given error in fewer samples than the plain tracer). A
     plain FP64 render of the same samples is compared with it as a cross-check. Reported: time to
     1% relMSE from a fresh guide trained online during the progressive render (training included),
-    the plain tracer's time, and both estimators' relMSE at equal samples with the trained guide."""
+    the plain tracer's time, and both estimators' relMSE at equal samples with the trained guide.
+    learned=True adds the same two measurements for a guide whose selection probability is learned
+    per vertex (mis_fraction < 0) instead of fixed at 0.5."""
     import piper2_amd as pg
     scene = pg.Scene(dev, sc)
     film = pg.Film(dev, sc["width"], sc["height"])
@@ -283,6 +285,26 @@ def guided_vs_plain(dev, sc, ref_spp, label, train_passes=64, train_spp=16, max_
            "plain_trace_spp_s_relmse": tr_plain, "guided_trace_spp_s_relmse": tr_guided}
     if t_plain and t_guided:
         out["guided_speedup"] = round(t_plain / t_guided, 3)
+    if learned:  # the selection probability learned per vertex
+        tl = pg.Guide(dev, seed=1, mis_fraction=-1.0)
+        for p in range(train_passes):
+            scene.render_guided(tl, pg_params(sc, (2 * R + train_spp * p, 2 * R + train_spp * (p + 1))), None)
+        fl = pg.Guide(dev, seed=1, train_records=0, mis_fraction=-1.0)
+        fl.set_params(tl.get_params())
+        tl.close()
+        eq_l = {}
+        for spp in (256, 1024):
+            film.clear()
+            st_l = scene.render_guided(fl, pg_params(sc, (0, spp)), film)
+            eq_l[str(spp)] = {"relmse": round(_rel_mse(_img(film.download()), ref), 5), "ms": round(st_l["ms_total"], 2)}
+        fl.close()
+        gl = pg.Guide(dev, seed=1, mis_fraction=-1.0)
+        t_l, tr_l = time_to_relmse(sc, film, ref, lambda p: scene.render_guided(gl, p, film), max_level=max_level)
+        gl.close()
+        out["learned_selection"] = {"time_to_1pct_s": t_l and round(t_l, 4),
+                                    "speedup_vs_plain": round(t_plain / t_l, 3) if t_plain and t_l else None,
+                                    "relmse_at_equal_spp_trained": eq_l, "trace_spp_s_relmse": tr_l}
+        log(f"{label}: learned selection {t_l} s, equal spp {eq_l}")
     eq = equal_time(tr_plain, tr_guided)
     if eq:
         t, e_plain, e_guided = eq[-1]
@@ -720,7 +742,7 @@ def main():
         # time to 1% relMSE, guided (online training, one-sample MIS at 0.5) vs plain
         result["time_to_1pct_relmse"] = {
             "hard_room_256x256_depth12": guided_vs_plain(dev, scenes.hard_room(seed=0, width=256, height=256),
-                                                         args.ref_spp, "hard room"),
+                                                         args.ref_spp, "hard room", learned=True),
             "terrain_slit_1920x1080_depth8": guided_vs_plain(dev, scenes.terrain_slit(), args.ref_spp // 16,
                                                              "terrain slit", max_level=9),
         }

@@ -379,23 +379,28 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     const float* mix = gs.mix + static_cast<size_t>(i) * kGMixFloats;
                     a = guideFraction(BATCH ? bt.guide[sceneOf(meta.z)].alpha : gs.alpha, mat, mix);
                     bool ok = true;
-                    if (uc1 < a) {
+                    const bool fromGuide = uc1 < a;
+                    if (fromGuide) {
                         const float3 w = mixtureSample(mix, uc1 / a, uc2);
                         ss.wi = fr.toLocal(V3{w.x, w.y, w.z});
-                    } else {
+                    } else {  // the BSDF's own sample carries its value and density
                         const BsdfSample b = bsdfSample(mat, wo, ng, (uc1 - a) / (1.0f - a), uc2);
                         ss.wi = b.wi;
+                        ss.f = b.f;
+                        pb = b.pdf;
                         ok = b.pdf > 0.0f;
                     }
-                    ss.f = V3{0.0f, 0.0f, 0.0f};
-                    ss.pdf = 0.0f;
                     if (ok) {
-                        const V3 ww = fr.toWorld(ss.wi);
-                        pb = bsdfPdf(mat, wo, ss.wi);
+                        if (fromGuide) pb = bsdfPdf(mat, wo, ss.wi), ss.f = bsdfEval(mat, wo, ss.wi, ng);
                         pdfMix = (1.0f - a) * pb;
-                        if (a > 0.0f) pdfMix += a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z));
-                        ss.f = bsdfEval(mat, wo, ss.wi, ng);
+                        if (a > 0.0f) {
+                            const V3 ww = fr.toWorld(ss.wi);
+                            pdfMix += a * mixturePdf(mix, make_float3(ww.x, ww.y, ww.z));
+                        }
                         ss.pdf = pdfMix;
+                    } else {
+                        ss.f = V3{0.0f, 0.0f, 0.0f};
+                        ss.pdf = 0.0f;
                     }
                 }
                 if (!(ss.pdf > 0.0f && valid(ss.f))) {
