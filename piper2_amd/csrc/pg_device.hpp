@@ -70,11 +70,19 @@ __device__ __forceinline__ float sobolSample(const uint32_t* __restrict__ m, uin
 // row segment instead of 8 separate dependent chains. Bit-identical to sobolSample.
 __device__ __forceinline__ void sobolWindow(const DevScene& s, uint64_t index, uint32_t d0, float out[8]) {
     if (d0 + 8 <= kByteDims) {
-        // Byte-sliced table: one row of 8 dimensions per index byte (4 rows for a 28-bit index),
-        // all loads independent.
+        // Byte-sliced table: one row of 8 dimensions per index byte, all loads independent. The
+        // low four bytes are read whatever their value (a zero byte selects an all-zero row), so
+        // their loads go out together; higher bytes, set from 2^32 on, loop.
         uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint32_t* __restrict__ t = s.sobolB + d0;
-        for (uint32_t k = 0; index; ++k, index >>= 8) {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t* row = t + (k * 256 + static_cast<uint32_t>((index >> (8 * k)) & 255u)) * kByteDims;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] ^= row[j];
+        }
+        index >>= 32;
+        for (uint32_t k = 4; index; ++k, index >>= 8) {
             const uint32_t* row = t + (k * 256 + static_cast<uint32_t>(index & 255u)) * kByteDims;
 #pragma unroll
             for (int j = 0; j < 8; ++j) v[j] ^= row[j];
@@ -101,21 +109,38 @@ __device__ __forceinline__ void sobolWindow(const DevScene& s, uint64_t index, u
 }
 
 // sobolIntervalToIndex (ScrambledSobolSampler/Kernel.cpp:31-50) with tables from a GF(2) inverse.
+// sampleBits bounds the bit length of every sample number the caller passes.
 __device__ __forceinline__ uint64_t intervalToIndex(const DevScene& s, uint32_t m, uint32_t sample, uint32_t px,
-                                                    uint32_t py) {
+                                                    uint32_t py, uint32_t sampleBits) {
     if (m == 0) return 0;
     const uint32_t n = 2 * m;
     uint64_t index = static_cast<uint64_t>(sample) << n;
     uint64_t b = (static_cast<uint64_t>(px) << m) | py;
-    const uint64_t* delta = s.sobolDelta + m * 64;
-    const uint64_t* inv = s.sobolInv + m * 32;
-    while (sample) {
-        b ^= delta[__builtin_ctz(sample)];
-        sample &= sample - 1;
+    // The tables are read through the constant address space: with m and sampleBits the same on
+    // all lanes (one render's constants) every address is wave-uniform, so the entries come in as
+    // scalar loads, eight a batch, and each lane keeps those its bits select. A batch reads entries
+    // up to the next multiple of 8 past a bound; they exist (rows of 64 and 32 entries, sample <
+    // 2^32, n <= 2 kMaxLog2Res = 32) and are masked off by zero bits.
+    using CU64 = const __attribute__((address_space(4))) uint64_t*;
+    const CU64 delta = (CU64)(s.sobolDelta + m * 64);
+    const CU64 inv = (CU64)(s.sobolInv + m * 32);
+#pragma unroll
+    for (uint32_t j0 = 0; j0 < 32; j0 += 8) {
+        if (j0 >= sampleBits) break;
+        uint64_t t[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) t[k] = delta[j0 + k];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) b ^= t[k] & (0ull - ((sample >> (j0 + k)) & 1u));
     }
-    while (b) {
-        index ^= inv[__builtin_ctzll(b)];
-        b &= b - 1;
+#pragma unroll
+    for (uint32_t j0 = 0; j0 < 32; j0 += 8) {
+        if (j0 >= n) break;
+        uint64_t t[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) t[k] = inv[j0 + k];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) index ^= t[k] & (0ull - ((b >> (j0 + k)) & 1u));
     }
     return index;
 }

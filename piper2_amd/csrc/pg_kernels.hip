@@ -45,8 +45,9 @@ __device__ __forceinline__ uint64_t segmentPaths(const RenderConsts& rc) {
 }
 
 __device__ __forceinline__ Ray primaryRay(const DevScene& s, const RenderConsts& rc, uint32_t px, uint32_t py,
-                                          uint32_t si, uint64_t& idx, FilmPoint& fp, float& time) {
-    idx = intervalToIndex(s, rc.log2Res, si, px, py);
+                                          uint32_t si, uint32_t sampleBits, uint64_t& idx, FilmPoint& fp,
+                                          float& time) {
+    idx = intervalToIndex(s, rc.log2Res, si, px, py, sampleBits);
     const float res = static_cast<float>(1u << rc.log2Res);
     float u[8];
     sobolWindow(s, idx, 0, u);
@@ -144,7 +145,8 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
             uint64_t idx;
             FilmPoint fp;
             float time;
-            const Ray r = primaryRay(s, rc, px, py, si, idx, fp, time);
+            // si < rc.spp, a power of two; a batch's renders may differ in it
+            const Ray r = primaryRay(s, rc, px, py, si, BATCH ? 32u : __builtin_ctz(rc0.spp), idx, fp, time);
             ps.p[slot].o = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
             ps.p[slot].d = make_float4(r.d.x, r.d.y, r.d.z, time);  // d.w: the path's time, kept by every bounce
             ps.p[slot].thr = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -875,7 +877,7 @@ __global__ void kCameraRays(DevScene s, RenderConsts rc, const uint32_t* pix, ui
     uint64_t idx;
     FilmPoint fp;
     float time;
-    const Ray r = primaryRay(s, rc, pix[3 * i], pix[3 * i + 1], pix[3 * i + 2], idx, fp, time);
+    const Ray r = primaryRay(s, rc, pix[3 * i], pix[3 * i + 1], pix[3 * i + 2], __builtin_ctz(rc.spp), idx, fp, time);
     float* o = rays + 8 * static_cast<size_t>(i);
     o[0] = r.o.x, o[1] = r.o.y, o[2] = r.o.z, o[3] = r.d.x, o[4] = r.d.y, o[5] = r.d.z, o[6] = 1e-3f, o[7] = 1e5f;
     film[2 * i] = fp.x, film[2 * i + 1] = fp.y;
@@ -885,7 +887,7 @@ __global__ void kSobol(DevScene s, uint32_t m, uint32_t dims, const uint32_t* pi
                        uint64_t* indexOut) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t idx = intervalToIndex(s, m, pix[3 * i + 2], pix[3 * i], pix[3 * i + 1]);
+    const uint64_t idx = intervalToIndex(s, m, pix[3 * i + 2], pix[3 * i], pix[3 * i + 1], 32u);
     if (indexOut) indexOut[i] = idx;
     for (uint32_t d = 0; d < dims; ++d) out[static_cast<size_t>(i) * dims + d] = sobolSample(s.sobol, idx, d);
 }
