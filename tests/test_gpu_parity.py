@@ -86,6 +86,23 @@ def test_camera_rays_and_primary_hits_bit_exact(dev, which, request):
     assert np.array_equal(uv_g[tri], uv_c[tri])
 
 
+@pytest.mark.parametrize("spp", [1, 2, 100, 4096, 1 << 20])
+def test_camera_rays_bit_exact_across_sample_counts(dev, cornell, spp):
+    """Camera rays for sample numbers up to the render's power-of-two sample count (100 rounds up to
+    128): the interval-to-index loop bounds follow that count, so every sample-bit width from none
+    (1 spp) to 20 bits must give the oracle's rays bit for bit."""
+    o = OracleScene(cornell)
+    s = pg.Scene(dev, cornell)
+    rng = np.random.default_rng(spp)
+    top = 1 << (spp - 1).bit_length()
+    pix = _pix(rng, cornell, 4000, spp=top)
+    pix[:4, 2] = [0, top - 1, top // 2, max(top - 2, 0)]
+    rays_g, film_g = s.camera_rays(pg.render_params(cornell, spp=spp, spp_range=(0, 1)), pix)
+    rays_c, film_c = o.camera_rays(pix)
+    assert np.array_equal(rays_g, rays_c)
+    assert np.array_equal(film_g, film_c)
+
+
 def test_random_rays_match_bruteforce(dev, cornell):
     o = OracleScene(cornell)
     rng = np.random.default_rng(9)
