@@ -343,14 +343,30 @@ __device__ __forceinline__ bool traverse(const DevScene& s, const Ray& r, const 
         } else {
             const uint32_t first = ref & kLeafFirstMask;
             const uint32_t count = ((ref >> kLeafCountShift) & 0xFu) + 1u;
-            for (uint32_t k = first; k < first + count; ++k) {
-                if (COUNT) ++cnt.tris;
-                const float4 a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
-                float t, u, v;
-                if (!intersectTri(r, p, xyz(a), xyz(b), xyz(c), t, u, v)) continue;
-                if (ANY) {
-                    if (t > tmin && t < tmax) return true;
-                } else {
+            if constexpr (ANY) {
+                // Occlusion: the leaf's triangles two at a time, both loads issued before either
+                // test (a lone last triangle is loaded twice, in bounds), one memory wait per pair.
+                // The shadow stage takes 15.7 instead of 16.9 ms per 1080p64 frame this way; the
+                // closest-hit kernel is slower with it (1.656 against 1.622 ms per launch).
+                const uint32_t end = first + count;
+                for (uint32_t k = first; k < end; k += 2) {
+                    const bool two = k + 1 < end;
+                    const uint32_t k1 = two ? k + 1 : k;
+                    const float4 a0 = tris[3 * k], b0 = tris[3 * k + 1], c0 = tris[3 * k + 2];
+                    const float4 a1 = tris[3 * k1], b1 = tris[3 * k1 + 1], c1 = tris[3 * k1 + 2];
+                    __builtin_amdgcn_sched_barrier(0);  // keep the six loads ahead of their first use
+                    if (COUNT) cnt.tris += two ? 2u : 1u;
+                    float t, u, v;
+                    if (intersectTri(r, p, xyz(a0), xyz(b0), xyz(c0), t, u, v) && t > tmin && t < tmax) return true;
+                    if (two && intersectTri(r, p, xyz(a1), xyz(b1), xyz(c1), t, u, v) && t > tmin && t < tmax)
+                        return true;
+                }
+            } else {
+                for (uint32_t k = first; k < first + count; ++k) {
+                    if (COUNT) ++cnt.tris;
+                    const float4 a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
+                    float t, u, v;
+                    if (!intersectTri(r, p, xyz(a), xyz(b), xyz(c), t, u, v)) continue;
                     const uint32_t prim = __float_as_uint(a.w);
                     if (t > tmin && (t < h.t || (t == h.t && prim < h.prim))) h = HitRec{t, k, prim, u, v};
                 }
