@@ -33,6 +33,16 @@ __device__ __forceinline__ V3 normalize(V3 a) {
     const float l = length(a);
     return {a.x / l, a.y / l, a.z / l};
 }
+// Shading arithmetic (hit normals and frames, BSDFs, light sampling, MIS weights) on the hardware
+// reciprocal, square root and reciprocal square root (v_rcp_f32, v_sqrt_f32, v_rsq_f32, 1 ulp)
+// instead of the correctly rounded sequences of about ten instructions each: the shade kernel is
+// bound by instruction issue.
+// Results move by an ulp against IEEE division, inside the per-path tolerance against the oracle.
+// Camera rays, traversal and hit distances keep IEEE division and square root (compared bit for
+// bit), and so do the light-quad tests that repeat the closest-hit decision.
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ V3 fnormalize(V3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
 __device__ __forceinline__ float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 // Spectrum::valid (Kernel/Protocol.hpp:92-94)
 __device__ __forceinline__ bool valid(V3 a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
@@ -562,7 +572,7 @@ __device__ __forceinline__ Frame shadingFrame(V3 n, bool d3) {
     f.n = n;
     const V3 ex{1.0f, 0.0f, 0.0f}, ey{0.0f, 1.0f, 0.0f};
     f.t = fabsf(dot(n, ex)) < fabsf(dot(n, ey)) ? cross(n, ex) : cross(n, ey);
-    if (!d3) f.t = normalize(f.t);
+    if (!d3) f.t = fnormalize(f.t);
     f.b = cross(n, f.t);
     return f;
 }
@@ -582,7 +592,7 @@ __device__ __forceinline__ V3 triNormal(const DevScene& s, uint32_t pos, V3 dir,
                                         float time = 0.0f) {
     V3 v0, v1, v2;
     worldTriangle<MOTION>(s.inst, s.tris[3 * pos], s.tris[3 * pos + 1], s.tris[3 * pos + 2], time, v0, v1, v2);
-    const V3 ng = normalize(cross(v1 - v0, v2 - v0));
+    const V3 ng = fnormalize(cross(v1 - v0, v2 - v0));
     const bool f = dot(dir, ng) < 0.0f;  // Embree.cpp:336: Front when dot(dir, Ng) < 0
     if (front) *front = f;
     return f ? ng : -ng;
@@ -602,42 +612,42 @@ __device__ __forceinline__ V3 cosineHemisphere(float u1, float u2) {
     float x = 0.0f, y = 0.0f;
     if (!(fminf(au1, au2) < 1e-4f)) {
         const float rad = au1 > au2 ? u1 : u2;
-        const float theta = au1 > au2 ? (kPi / 4.0f) * (u2 / u1) : (kPi / 2.0f) - (kPi / 4.0f) * (u1 / u2);
+        const float theta = au1 > au2 ? (kPi / 4.0f) * fdiv(u2, u1) : (kPi / 2.0f) - (kPi / 4.0f) * fdiv(u1, u2);
         x = cosf(theta) * rad;
         y = sinf(theta) * rad;
     }
-    return {x, y, sqrtf(fmaxf(0.0f, 1.0f - (x * x + y * y)))};
+    return {x, y, fsqrt(fmaxf(0.0f, 1.0f - (x * x + y * y)))};
 }
 
 __device__ __forceinline__ float ggxLambda(V3 w, float alpha) {  // Classic/Kernel.cpp:326-341
     const float cos2 = w.z * w.z;
     const float sin2 = fmaxf(0.0f, 1.0f - cos2);
-    const float tan2 = sin2 / cos2;
+    const float tan2 = fdiv(sin2, cos2);
     if (isinf(tan2)) return 0.0f;
-    return (-1.0f + sqrtf(1.0f + alpha * alpha * tan2)) * 0.5f;
+    return (-1.0f + fsqrt(1.0f + alpha * alpha * tan2)) * 0.5f;
 }
 __device__ __forceinline__ float ggxD(V3 wh, float alpha) {  // Classic/Kernel.cpp:342-357
     const float cos2 = wh.z * wh.z;
     const float sin2 = fmaxf(0.0f, 1.0f - cos2);
-    const float tan2 = sin2 / cos2;
+    const float tan2 = fdiv(sin2, cos2);
     if (isinf(tan2)) return 0.0f;
-    const float e = cos2 * (1.0f + tan2 / (alpha * alpha));
-    return kInvPi / (alpha * alpha * e * e);
+    const float e = cos2 * (1.0f + fdiv(tan2, alpha * alpha));
+    return fdiv(kInvPi, alpha * alpha * e * e);
 }
 // Visible-normal sampling (Heitz, JCGT 2018).
 __device__ __forceinline__ V3 ggxSampleWh(V3 wo, float alpha, float u1, float u2) {
-    const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
+    const V3 vh = fnormalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
     const float lensq = vh.x * vh.x + vh.y * vh.y;
-    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / sqrtf(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * __builtin_amdgcn_rsqf(lensq) : V3{1.0f, 0.0f, 0.0f};
     const V3 t2 = cross(vh, t1);
-    const float r = sqrtf(u1);
+    const float r = fsqrt(u1);
     const float phi = 2.0f * kPi * u2;
     const float p1 = r * cosf(phi);
     float p2 = r * sinf(phi);
     const float sm = 0.5f * (1.0f + vh.z);
-    p2 = (1.0f - sm) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
-    const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
-    return normalize(V3{alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)});
+    p2 = (1.0f - sm) * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return fnormalize(V3{alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)});
 }
 // Anisotropic Trowbridge-Reitz for glass and substrate, whose RoughnessX and RoughnessY give the
 // alphas along the shading frame's x and y axes (MicrofacetDistribution, Classic/Kernel.cpp:289-357,
@@ -648,46 +658,46 @@ struct Ggx {
 __device__ __forceinline__ float ggxLambda(V3 w, Ggx g) {  // Classic/Kernel.cpp:326-341
     const float cos2 = w.z * w.z;
     const float sin2 = fmaxf(0.0f, 1.0f - cos2);
-    const float tan2 = sin2 / cos2;
+    const float tan2 = fdiv(sin2, cos2);
     if (isinf(tan2)) return 0.0f;
-    const float sinT = sqrtf(sin2);
-    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, w.x / sinT));
-    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, w.y / sinT));
+    const float sinT = fsqrt(sin2);
+    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, fdiv(w.x, sinT)));
+    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, fdiv(w.y, sinT)));
     const float p1 = cosPhi * g.ax, p2 = sinPhi * g.ay;
-    return (-1.0f + sqrtf(1.0f + (p1 * p1 + p2 * p2) * tan2)) * 0.5f;
+    return (-1.0f + fsqrt(1.0f + (p1 * p1 + p2 * p2) * tan2)) * 0.5f;
 }
 __device__ __forceinline__ float ggxD(V3 wh, Ggx g) {  // Classic/Kernel.cpp:342-357
     const float cos2 = wh.z * wh.z;
     const float sin2 = fmaxf(0.0f, 1.0f - cos2);
-    const float tan2 = sin2 / cos2;
+    const float tan2 = fdiv(sin2, cos2);
     if (isinf(tan2)) return 0.0f;
-    const float sinT = sqrtf(sin2);
-    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, wh.x / sinT));
-    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, wh.y / sinT));
-    const float p1 = cosPhi / g.ax, p2 = sinPhi / g.ay;
+    const float sinT = fsqrt(sin2);
+    const float cosPhi = sinT == 0.0f ? 1.0f : fminf(1.0f, fmaxf(-1.0f, fdiv(wh.x, sinT)));
+    const float sinPhi = sinT == 0.0f ? 0.0f : fminf(1.0f, fmaxf(-1.0f, fdiv(wh.y, sinT)));
+    const float p1 = fdiv(cosPhi, g.ax), p2 = fdiv(sinPhi, g.ay);
     const float e = cos2 * (1.0f + (p1 * p1 + p2 * p2) * tan2);
-    return kInvPi / (g.ax * g.ay * e * e);
+    return fdiv(kInvPi, g.ax * g.ay * e * e);
 }
 // Visible-normal sampling (Heitz, JCGT 2018) with the view direction stretched by (ax, ay).
 __device__ __forceinline__ V3 ggxSampleWh(V3 wo, Ggx g, float u1, float u2) {
-    const V3 vh = normalize(V3{g.ax * wo.x, g.ay * wo.y, wo.z});
+    const V3 vh = fnormalize(V3{g.ax * wo.x, g.ay * wo.y, wo.z});
     const float lensq = vh.x * vh.x + vh.y * vh.y;
-    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * (1.0f / sqrtf(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    const V3 t1 = lensq > 0.0f ? V3{-vh.y, vh.x, 0.0f} * __builtin_amdgcn_rsqf(lensq) : V3{1.0f, 0.0f, 0.0f};
     const V3 t2 = cross(vh, t1);
-    const float r = sqrtf(u1);
+    const float r = fsqrt(u1);
     const float phi = 2.0f * kPi * u2;
     const float p1 = r * cosf(phi);
     float p2 = r * sinf(phi);
     const float sm = 0.5f * (1.0f + vh.z);
-    p2 = (1.0f - sm) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
-    const V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
-    return normalize(V3{g.ax * nh.x, g.ay * nh.y, fmaxf(0.0f, nh.z)});
+    p2 = (1.0f - sm) * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1)) + sm * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return fnormalize(V3{g.ax * nh.x, g.ay * nh.y, fmaxf(0.0f, nh.z)});
 }
 // Density of a reflection drawn from the visible normals: D(wh) G1(wo) / (4 cos theta_o).
 __device__ __forceinline__ float ggxReflectPdf(Ggx g, V3 wo, V3 wi) {
     if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
-    const V3 wh = normalize(wo + wi);
-    return ggxD(wh, g) / (4.0f * wo.z * (1.0f + ggxLambda(wo, g)));
+    const V3 wh = fnormalize(wo + wi);
+    return fdiv(ggxD(wh, g), 4.0f * wo.z * (1.0f + ggxLambda(wo, g)));
 }
 __device__ __forceinline__ float remapRoughness(float roughness) {  // Classic/Kernel.cpp:378-381
     const float x = logf(fmaxf(1e-3f, roughness));
@@ -701,28 +711,28 @@ __device__ __forceinline__ V3 schlick(V3 f0, float c) {
 }
 __device__ __forceinline__ V3 conductorF(V3 f0, float alpha, V3 wo, V3 wi) {  // MicrofacetReflection::evaluate
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
-    const V3 wh = normalize(wo + wi);
+    const V3 wh = fnormalize(wo + wi);
     const float g = 1.0f + ggxLambda(wo, alpha) + ggxLambda(wi, alpha);
-    const float k = ggxD(wh, alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    const float k = fdiv(ggxD(wh, alpha), 4.0f * g * fabsf(wo.z * wi.z));
     return schlick(f0, dot(wi, wh)) * k;
 }
 __device__ __forceinline__ float conductorPdf(float alpha, V3 wo, V3 wi) {
     if (wi.z <= 0.0f || wo.z <= 0.0f) return 0.0f;
-    const V3 wh = normalize(wo + wi);
-    return ggxD(wh, alpha) / (4.0f * wo.z * (1.0f + ggxLambda(wo, alpha)));
+    const V3 wh = fnormalize(wo + wi);
+    return fdiv(ggxD(wh, alpha), 4.0f * wo.z * (1.0f + ggxLambda(wo, alpha)));
 }
 
 // FresnelDielectric (Classic/Kernel.cpp:271-286).
 __device__ __forceinline__ float fresnelDielectric(float cosI, float etaI, float etaT) {
     cosI = fmaxf(0.0f, fminf(1.0f, fabsf(cosI)));
-    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - cosI * cosI));
-    const float sinT = etaI / etaT * sinI;
+    const float sinI = fsqrt(fmaxf(0.0f, 1.0f - cosI * cosI));
+    const float sinT = fdiv(etaI, etaT) * sinI;
     if (sinT >= 1.0f) return 1.0f;
-    const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT * sinT));
+    const float cosT = fsqrt(fmaxf(0.0f, 1.0f - sinT * sinT));
     const float p1 = etaT * cosI, p2 = etaI * cosT;
-    const float par = (p1 - p2) / (p1 + p2);
+    const float par = fdiv(p1 - p2, p1 + p2);
     const float p3 = etaI * cosI, p4 = etaT * cosT;
-    const float perp = (p3 - p4) / (p3 + p4);
+    const float perp = fdiv(p3 - p4, p3 + p4);
     return 0.5f * (par * par + perp * perp);
 }
 
@@ -835,8 +845,8 @@ __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool fron
             if (b.w == 0u && rough > 0.0f) {  // matteInit: Oren-Nayar, sigma = roughness x pi
                 const float sigma = rough * kPi, s2 = sigma * sigma;
                 t = kDevOrenNayar;
-                r.alpha = 1.0f - s2 / (2.0f * (s2 + 0.33f));
-                r.aux = 0.45f * s2 / (s2 + 0.09f);
+                r.alpha = 1.0f - fdiv(s2, 2.0f * (s2 + 0.33f));
+                r.aux = fdiv(0.45f * s2, s2 + 0.09f);
             } else if (b.w == 1u) {
                 r.alpha = fmaxf(rough * rough, 1e-7f);
             } else if (b.w == 2u) {
@@ -863,9 +873,9 @@ __device__ __forceinline__ Mat loadMat(const DevScene& s, uint32_t id, bool fron
 // plasticInit :441-471, eta 1.46), with the exact visible-normal pdf as the conductor.
 __device__ __forceinline__ V3 plasticGlossyF(const Mat& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
-    const V3 wh = normalize(wo + wi);
+    const V3 wh = fnormalize(wo + wi);
     const float g = 1.0f + ggxLambda(wo, m.alpha) + ggxLambda(wi, m.alpha);
-    const float k = ggxD(wh, m.alpha) / (4.0f * g * fabsf(wo.z * wi.z));
+    const float k = fdiv(ggxD(wh, m.alpha), 4.0f * g * fabsf(wo.z * wi.z));
     const float etaI = m.front() ? 1.0f : 1.46f, etaT = m.front() ? 1.46f : 1.0f;
     return m.spec() * (fresnelDielectric(dot(wi, wh), etaI, etaT) * k);
 }
@@ -881,34 +891,34 @@ __device__ __forceinline__ float glassEtaI(const Mat& m) { return m.front() ? 1.
 __device__ __forceinline__ float glassEtaT(const Mat& m) { return m.front() ? 1.5f : 1.0f; }
 __device__ __forceinline__ V3 roughGlassR(const Mat& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.0f, 0.0f, 0.0f};
-    const V3 wh = normalize(wo + wi);
+    const V3 wh = fnormalize(wo + wi);
     const float g = 1.0f + ggxLambda(wo, m.ggx()) + ggxLambda(wi, m.ggx());
-    const float k = ggxD(wh, m.ggx()) / (4.0f * g * fabsf(wo.z * wi.z));
+    const float k = fdiv(ggxD(wh, m.ggx()), 4.0f * g * fabsf(wo.z * wi.z));
     return m.albedo * (fresnelDielectric(dot(wi, wh), glassEtaI(m), glassEtaT(m)) * k);
 }
 __device__ __forceinline__ V3 roughGlassT(const Mat& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z >= 0.0f) return {0.0f, 0.0f, 0.0f};
-    const float eta = glassEtaT(m) / glassEtaI(m);
-    V3 wh = normalize(wo + wi * eta);
+    const float eta = fdiv(glassEtaT(m), glassEtaI(m));
+    V3 wh = fnormalize(wo + wi * eta);
     if (wh.z < 0.0f) wh = -wh;
     const float ow = dot(wo, wh), iw = dot(wi, wh);
     if (!(ow > 0.0f && iw < 0.0f)) return {0.0f, 0.0f, 0.0f};  // Walter's chi+: a microfacet wo sees
     const float fr = fresnelDielectric(ow, glassEtaI(m), glassEtaT(m));
     const float den = ow + eta * iw;
     const float g = 1.0f + ggxLambda(wo, m.ggx()) + ggxLambda(wi, m.ggx());
-    const float v = (1.0f - fr) * ggxD(wh, m.ggx()) * fabsf(iw) * fabsf(ow) / (g * fabsf(wo.z * wi.z) * den * den);
+    const float v = fdiv((1.0f - fr) * ggxD(wh, m.ggx()) * fabsf(iw) * fabsf(ow), g * fabsf(wo.z * wi.z) * den * den);
     return m.spec() * v;
 }
 __device__ __forceinline__ float roughGlassTPdf(const Mat& m, V3 wo, V3 wi) {
     if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
-    const float eta = glassEtaT(m) / glassEtaI(m);
-    V3 wh = normalize(wo + wi * eta);
+    const float eta = fdiv(glassEtaT(m), glassEtaI(m));
+    V3 wh = fnormalize(wo + wi * eta);
     if (wh.z < 0.0f) wh = -wh;
     const float ow = dot(wo, wh), iw = dot(wi, wh);
     if (!(ow > 0.0f && iw < 0.0f)) return 0.0f;  // only normals the visible-normal sampler draws
     const float den = ow + eta * iw;
-    const float dwh = fabsf(eta * eta * iw / (den * den));  // d omega_h / d omega_i
-    return ggxD(wh, m.ggx()) * fabsf(ow) / (wo.z * (1.0f + ggxLambda(wo, m.ggx()))) * dwh;
+    const float dwh = fabsf(fdiv(eta * eta * iw, den * den));  // d omega_h / d omega_i
+    return fdiv(ggxD(wh, m.ggx()) * fabsf(ow), wo.z * (1.0f + ggxLambda(wo, m.ggx()))) * dwh;
 }
 __device__ __forceinline__ uint32_t glassLobes(const Mat& m) { return (valid(m.albedo) ? 1u : 0u) | (valid(m.spec()) ? 2u : 0u); }
 __device__ __forceinline__ V3 roughGlassEval(const Mat& m, V3 wo, V3 wi, V3 ng) {
@@ -927,15 +937,15 @@ __device__ __forceinline__ float roughGlassPdf(const Mat& m, V3 wo, V3 wi) {
 // OrenNayar::evaluate (Classic/Kernel.cpp:108-131): A and B are precomputed on upload; the value is
 // zero when either direction is within sin(theta) < 1e-4 of the normal, as the reference has it.
 __device__ __forceinline__ V3 orenNayarF(const Mat& m, V3 wo, V3 wi) {
-    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - wi.z * wi.z));
-    const float sinO = sqrtf(fmaxf(0.0f, 1.0f - wo.z * wo.z));
+    const float sinI = fsqrt(fmaxf(0.0f, 1.0f - wi.z * wi.z));
+    const float sinO = fsqrt(fmaxf(0.0f, 1.0f - wo.z * wo.z));
     if (sinI < 1e-4f || sinO < 1e-4f) return {0.0f, 0.0f, 0.0f};
-    const float cosPhiI = fminf(1.0f, fmaxf(-1.0f, wi.x / sinI)), sinPhiI = fminf(1.0f, fmaxf(-1.0f, wi.y / sinI));
-    const float cosPhiO = fminf(1.0f, fmaxf(-1.0f, wo.x / sinO)), sinPhiO = fminf(1.0f, fmaxf(-1.0f, wo.y / sinO));
+    const float cosPhiI = fminf(1.0f, fmaxf(-1.0f, fdiv(wi.x, sinI))), sinPhiI = fminf(1.0f, fmaxf(-1.0f, fdiv(wi.y, sinI)));
+    const float cosPhiO = fminf(1.0f, fmaxf(-1.0f, fdiv(wo.x, sinO))), sinPhiO = fminf(1.0f, fmaxf(-1.0f, fdiv(wo.y, sinO)));
     const float maxCos = fmaxf(0.0f, cosPhiI * cosPhiO + sinPhiI * sinPhiO);
     const bool iSteeper = fabsf(wi.z) > fabsf(wo.z);
     const float sinAlpha = iSteeper ? sinO : sinI;
-    const float tanBeta = iSteeper ? sinI / fabsf(wi.z) : sinO / fabsf(wo.z);
+    const float tanBeta = iSteeper ? fdiv(sinI, fabsf(wi.z)) : fdiv(sinO, fabsf(wo.z));
     const float b = (m.bits & kRoughFromTexture) ? m.aux : m.spec().x;
     return m.albedo * (kInvPi * (m.alpha + b * maxCos * sinAlpha * tanBeta));
 }
@@ -948,18 +958,18 @@ __device__ __forceinline__ V3 fresnelBlendF(const Mat& m, V3 wo, V3 wi) {
     const V3 negRem = rs - V3{1.0f, 1.0f, 1.0f};
     const float kd = (-28.0f / (23.0f * kPi)) * (1.0f - pow5(1.0f - 0.5f * fabsf(wi.z))) *
                      (1.0f - pow5(1.0f - 0.5f * fabsf(wo.z)));
-    const V3 wh = normalize(wo + wi);
+    const V3 wh = fnormalize(wo + wi);
     const float c = dot(wi, wh);
     const V3 schlick = rs + negRem * pow5(c - 1.0f);
-    const float ks = ggxD(wh, m.ggx()) / (4.0f * fmaxf(fabsf(wi.z), fabsf(wo.z)) * fabsf(c));
+    const float ks = fdiv(ggxD(wh, m.ggx()), 4.0f * fmaxf(fabsf(wi.z), fabsf(wo.z)) * fabsf(c));
     return mul(m.albedo, negRem) * kd + schlick * ks;
 }
 // FresnelBlend::pdf with the visible-normal density of its GGX half (the reference's pdf divides by
 // 1 + Lambda(wo) + Lambda(wh), which does not match the sampler it pairs with).
 __device__ __forceinline__ float fresnelBlendPdf(const Mat& m, V3 wo, V3 wi) {
     if (wi.z <= 0.0f) return 0.0f;
-    const V3 wh = normalize(wo + wi);
-    return 0.5f * (fabsf(wi.z) * kInvPi + ggxD(wh, m.ggx()) / (4.0f * fabsf(wo.z) * (1.0f + ggxLambda(wo, m.ggx()))));
+    const V3 wh = fnormalize(wo + wi);
+    return 0.5f * (fabsf(wi.z) * kInvPi + fdiv(ggxD(wh, m.ggx()), 4.0f * fabsf(wo.z) * (1.0f + ggxLambda(wo, m.ggx()))));
 }
 
 // Mirror and smooth glass are specular: their init clears noSpecular, so the vertex does no light
@@ -1028,7 +1038,7 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
     }
     if (m.type() == 3u) {  // mirrorSample (Classic/Kernel.cpp:678-692)
         s.wi = V3{-wo.x, -wo.y, wo.z};
-        s.f = m.albedo * (1.0f / fabsf(s.wi.z));
+        s.f = m.albedo * __builtin_amdgcn_rcpf(fabsf(s.wi.z));
         s.pdf = 1.0f;
         return s;
     }
@@ -1040,16 +1050,16 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
         if (u1 < fr) {
             s.wi = V3{-wo.x, -wo.y, wo.z};
             s.pdf = fr;
-            s.f = m.albedo * (fr / fabsf(s.wi.z));
+            s.f = m.albedo * fdiv(fr, fabsf(s.wi.z));
         } else {  // refract(wo, +z, etaI / etaT), Kernel/Transform.hpp:410-421
-            const float eta = etaI / etaT;
+            const float eta = fdiv(etaI, etaT);
             const float cosI = wo.z;
             const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - cosI * cosI);
             if (sin2T >= 1.0f) return s;
-            const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sin2T));
-            s.wi = normalize(V3{0.0f, 0.0f, eta * cosI - cosT} - wo * eta);
+            const float cosT = fsqrt(fmaxf(0.0f, 1.0f - sin2T));
+            s.wi = fnormalize(V3{0.0f, 0.0f, eta * cosI - cosT} - wo * eta);
             s.pdf = 1.0f - fr;
-            s.f = rt * (eta * eta * s.pdf / fabsf(s.wi.z));
+            s.f = rt * fdiv(eta * eta * s.pdf, fabsf(s.wi.z));
         }
         return s;
     }
@@ -1068,11 +1078,11 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
             const V3 wh = ggxSampleWh(wo, m.ggx(), u1, u2);
             const float c = dot(wo, wh);
             if (c < 0.0f) return s;
-            const float eta = glassEtaI(m) / glassEtaT(m);
+            const float eta = fdiv(glassEtaI(m), glassEtaT(m));
             const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - c * c);
             if (sin2T >= 1.0f) return s;
-            const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sin2T));
-            s.wi = normalize(wh * (eta * c - cosT) - wo * eta);
+            const float cosT = fsqrt(fmaxf(0.0f, 1.0f - sin2T));
+            s.wi = fnormalize(wh * (eta * c - cosT) - wo * eta);
             s.pdf = roughGlassTPdf(m, wo, s.wi);
             if (s.pdf > 0.0f) s.f = roughGlassT(m, wo, s.wi);
         }
@@ -1115,7 +1125,7 @@ __device__ __forceinline__ BsdfSample bsdfSample(const Mat& m, V3 wo, V3 ng, flo
     if (count > 1) {  // the other lobe: its value on the same side, its pdf always
         if (dot(wo, ng) * dot(s.wi, ng) > 0.0f) s.f = s.f + (diffuse ? plasticGlossyF(m, wo, s.wi) : m.albedo * kInvPi);
         s.pdf = s.pdf + (diffuse ? conductorPdf(m.alpha, wo, s.wi) : lambertPdf(s.wi));
-        s.pdf = s.pdf / static_cast<float>(count);
+        s.pdf = fdiv(s.pdf, static_cast<float>(count));
     }
     return s;
 }
@@ -1142,7 +1152,7 @@ __device__ __forceinline__ uint32_t selectPrimitive(const DevScene& s, const Dev
     r = min(r, L.count - 1u);
     const float hiCdf = s.quads[L.first + r].cdf;
     const float lower = d1 ? hiCdf : (r ? s.quads[L.first + r - 1u].cdf : 0.0f);
-    u = (u - lower) / s.quads[L.first + r].pdf;
+    u = fdiv(u - lower, s.quads[L.first + r].pdf);
     return L.first + r;
 }
 
@@ -1182,7 +1192,7 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
         const float st = sinf(theta);
         ls.dir = {st * cosf(phi), st * sinf(phi), cosf(theta)};
         ls.rad = ld3(s.env);
-        ls.pdf = st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
+        ls.pdf = st == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(2.0f * kPi * kPi * st);
         ls.dist = 1e5f;
         return ls;
     }
@@ -1190,9 +1200,11 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
         const float* p = s.points + 6 * (light - 1u - s.numLights);
         const V3 delta = V3{p[0], p[1], p[2]} - hit;
         const float dis2 = dot(delta, delta);
-        ls.rad = {p[3] / dis2, p[4] / dis2, p[5] / dis2};
-        ls.dist = sqrtf(fmaxf(dis2, 0.0f));
-        ls.dir = {delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
+        const float inv2 = __builtin_amdgcn_rcpf(dis2);
+        ls.rad = {p[3] * inv2, p[4] * inv2, p[5] * inv2};
+        ls.dist = fsqrt(fmaxf(dis2, 0.0f));
+        const float invD = __builtin_amdgcn_rcpf(ls.dist);
+        ls.dir = {delta.x * invD, delta.y * invD, delta.z * invD};
         ls.pdf = 1.0f;
         return ls;
     }
@@ -1201,24 +1213,25 @@ __device__ __forceinline__ LightSample sampleLight(const DevScene& s, uint32_t l
     const V3 src = ld3(q.origin) + ld3(q.u) * uu + ld3(q.v) * u2;
     V3 n = ld3(q.normal);
     if (dot(hit - src, n) <= 0.0f) n = -n;
-    const float pdfA = 1.0f / q.area;
+    const float pdfA = __builtin_amdgcn_rcpf(q.area);
     const V3 delta = src - hit;
     if (dot(delta, n) >= 0.0f) return ls;
     const float dis2 = dot(delta, delta);
     ls.rad = ld3(q.radiance);
-    ls.dist = sqrtf(dis2);
-    ls.dir = {delta.x / ls.dist, delta.y / ls.dist, delta.z / ls.dist};
-    ls.pdf = pdfA * dis2 / fabsf(dot(n, ls.dir));
+    ls.dist = fsqrt(dis2);
+    const float invD = __builtin_amdgcn_rcpf(ls.dist);
+    ls.dir = {delta.x * invD, delta.y * invD, delta.z * invD};
+    ls.pdf = fdiv(pdfA * dis2, fabsf(dot(n, ls.dir)));
     return ls;
 }
 __device__ __forceinline__ float envPdf(V3 dir) {  // SEPdf, BasicLight/Kernel.cpp:82-89
     const float theta = acosf(fminf(1.0f, fmaxf(-1.0f, dir.z)));
     const float st = sinf(theta);
-    return st == 0.0f ? 0.0f : 1.0f / (2.0f * kPi * kPi * st);
+    return st == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(2.0f * kPi * kPi * st);
 }
 __device__ __forceinline__ float powerHeuristic(float a, float b) {  // PathTracing/Kernel.cpp:27-31
     const float sa = a * a, sb = b * b;
-    return sa / (sa + sb);
+    return fdiv(sa, sa + sb);
 }
 
 // Wave-level compaction: every lane of the wave calls this together; lanes with pred get

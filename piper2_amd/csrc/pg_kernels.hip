@@ -310,7 +310,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     // light selection: uniformSelect over {environment, area lights, point lights}
                     const uint32_t count = 1u + s.numLights + s.numPoints;
                     const uint32_t sel = min(static_cast<uint32_t>(floorf(u[0] * static_cast<float>(count))), count - 1u);
-                    const float selInv = 1.0f / (1.0f / static_cast<float>(count));
+                    const float selInv = __builtin_amdgcn_rcpf(__builtin_amdgcn_rcpf(static_cast<float>(count)));
                     deltaSel = sel > s.numLights;
                     if (deltaSel) used = 5u;
                     const LightSample ls = sampleLight(s, sel, hp, d9 ? u[2] : u[1], d9 ? u[1] : u[2], d1);
@@ -318,7 +318,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                         const V3 wi = fr.toLocal(ls.dir);
                         const V3 f = bsdfEval(mat, wo, wi, ng);
                         if (valid(f)) {
-                            float w = fabsf(wi.z) / ls.pdf;
+                            float w = fdiv(fabsf(wi.z), ls.pdf);
                             if (!deltaSel) w = w * powerHeuristic(ls.pdf, bsdfPdf(mat, wo, wi));
                             const V3 c = mul(thr, mul(f, ls.rad) * w * selInv);
                             // staged where it is made, so its registers are free for the rest of the vertex
@@ -330,7 +330,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     BsdfSample bs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
                     if (!deltaSel) bs = bsdfSample(mat, wo, ng, d9 ? u[4] : u[3], d9 ? u[3] : u[4]);
                     if (bs.pdf > 0.0f && valid(bs.f)) {
-                        const float weight = fabsf(bs.wi.z) / bs.pdf;
+                        const float weight = fdiv(fabsf(bs.wi.z), bs.pdf);
                         const V3 dir = fr.toWorld(bs.wi);
                         const V3 c = mul(thr, bs.f * weight * selInv);
                         if (!envBlack) {
@@ -352,7 +352,7 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                                 if (noOtherLightFirst(s, sel - 1u, mray, tq, qi)) {
                                     const V3 qn = quadNormal(qd, dir);
                                     const V3 rad = dot(qn, dir) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
-                                    const float lpdf = tq * tq / (fabsf(dot(qn, dir)) * qd.area);
+                                    const float lpdf = fdiv(tq * tq, fabsf(dot(qn, dir)) * qd.area);
                                     if (valid(rad) && lpdf > 0.0f) {
                                         const V3 cm = mul(c, rad) * powerHeuristic(bs.pdf, lpdf);
                                         // tq > 0: the next float up makes the range (tmin, tq] inclusive
@@ -383,10 +383,10 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                     bool ok = true;
                     const bool fromGuide = uc1 < a;
                     if (fromGuide) {
-                        const float3 w = mixtureSample(mix, uc1 / a, uc2);
+                        const float3 w = mixtureSample(mix, fdiv(uc1, a), uc2);
                         ss.wi = fr.toLocal(V3{w.x, w.y, w.z});
                     } else {  // the BSDF's own sample carries its value and density
-                        const BsdfSample b = bsdfSample(mat, wo, ng, (uc1 - a) / (1.0f - a), uc2);
+                        const BsdfSample b = bsdfSample(mat, wo, ng, fdiv(uc1 - a, 1.0f - a), uc2);
                         ss.wi = b.wi;
                         ss.f = b.f;
                         pb = b.pdf;
@@ -408,13 +408,13 @@ kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, 
                 if (!(ss.pdf > 0.0f && valid(ss.f))) {
                     pushDone = true;
                 } else {
-                    thr = mul(thr, ss.f * (fabsf(ss.wi.z) / ss.pdf));
+                    thr = mul(thr, ss.f * fdiv(fabsf(ss.wi.z), ss.pdf));
                     bool alive = true;
                     if (meta.y > rc.minRR) {
                         const float qrr = fmaxf(0.05f, 1.0f - maxc(thr));
                         used += 1u;
                         if ((specV ? u[2] : (deltaSel ? u[5] : u[7])) < qrr) alive = false;
-                        else thr = thr * (1.0f / (1.0f - qrr));
+                        else thr = thr * __builtin_amdgcn_rcpf(1.0f - qrr);
                     }
                     if (alive) {
                         const V3 nd = fr.toWorld(ss.wi);
