@@ -24,7 +24,7 @@ __device__ __forceinline__ GuideInput hitInput(float4 oo, float4 dd, float t, fl
     const V3 p = V3{oo.x, oo.y, oo.z} + dir * t;
     V3 v0, v1, v2;  // a moving instance's triangle at the path's time (d.w)
     worldTriangle<MOTION>(inst, a, b, c, dd.w, v0, v1, v2);
-    const V3 ng = normalize(cross(v1 - v0, v2 - v0));
+    const V3 ng = fnormalize(cross(v1 - v0, v2 - v0));
     const V3 n = dot(dir, ng) < 0.0f ? ng : -ng;
     const float3 p01 = make_float3(fminf(fmaxf((p.x - lo.x) * invExt.x, 0.0f), 1.0f),
                                    fminf(fmaxf((p.y - lo.y) * invExt.y, 0.0f), 1.0f),

@@ -479,16 +479,16 @@ __device__ __forceinline__ float3 mixtureSample(const float* __restrict__ mix, f
         acc += wi;
     }
     if (sel < 0) sel = last, lo = loLast, wsel = mix[32 + last];
-    u1 = fminf(fmaxf((u1 - lo) / wsel, 0.0f), 1.0f);
+    u1 = fminf(fmaxf((u1 - lo) * __builtin_amdgcn_rcpf(wsel), 0.0f), 1.0f);
     const float4 l = reinterpret_cast<const float4*>(mix)[sel];
     const float k = l.w;
-    const float wz = fmaxf(-1.0f, 1.0f + logf(fmaxf(u1 + (1.0f - u1) * __expf(-2.0f * k), 1e-30f)) / k);
-    const float r = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
+    const float wz = fmaxf(-1.0f, 1.0f + logf(fmaxf(u1 + (1.0f - u1) * __expf(-2.0f * k), 1e-30f)) * __builtin_amdgcn_rcpf(k));
+    const float r = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
     float sp, cp;
     sincosf(2.0f * 3.14159265358979f * u2, &sp, &cp);
     // frame around mu (Duff et al. 2017 orthonormal basis)
     const float sign = copysignf(1.0f, l.z);
-    const float a = -1.0f / (sign + l.z), bb = l.x * l.y * a;
+    const float a = -__builtin_amdgcn_rcpf(sign + l.z), bb = l.x * l.y * a;
     const float3 t = make_float3(1.0f + sign * l.x * l.x * a, sign * bb, -sign * l.x);
     const float3 b = make_float3(bb, sign + l.y * l.y * a, -l.y);
     const float x = r * cp, y = r * sp;

@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
                 if (first && !occluded<W, MOTION>(s, ray, 1e-3f, tqUp, TravStack{lstack + lane, gs, ldsDepth}, time)) {
                     const V3 qn = quadNormal(qd, ray.d);
                     rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
-                    lpdf = tq * tq / (fabsf(dot(qn, ray.d)) * qd.area);
+                    lpdf = fdiv(tq * tq, fabsf(dot(qn, ray.d)) * qd.area);
                 }
             }
         } else {
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(64) kMis(DevScene s0, PathState ps, Queues q, 
                 const DevQuad& qd = s.quads[h.pos & ~kHitQuad];
                 const V3 qn = quadNormal(qd, ray.d);
                 rad = dot(qn, ray.d) < 0.0f ? ld3(qd.radiance) : V3{0.0f, 0.0f, 0.0f};
-                lpdf = h.t * h.t / (fabsf(dot(qn, ray.d)) * qd.area);
+                lpdf = fdiv(h.t * h.t, fabsf(dot(qn, ray.d)) * qd.area);
             }
         }
         if (valid(rad) && lpdf > 0.0f) {
