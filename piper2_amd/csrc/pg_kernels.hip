@@ -132,9 +132,10 @@ __global__ void __launch_bounds__(256) kRegen(DevScene s0, PathState ps, Queues 
                 sppBegin = bt.rc[item].sppBegin;
                 sppCount = bt.rc[item].sppCount;
             } else {
-                const uint64_t segPaths = segmentPaths(rc0);
-                item = static_cast<uint32_t>(G / segPaths);
-                g = static_cast<uint32_t>(G - static_cast<uint64_t>(item) * segPaths);
+                // one render's path numbers stay below 2^32 (checked on the host): 32-bit division
+                const uint32_t segPaths = static_cast<uint32_t>(segmentPaths(rc0)), G32 = static_cast<uint32_t>(G);
+                item = G32 / segPaths;
+                g = G32 - item * segPaths;
                 sppBegin = rc0.sppBegin + item * rc0.segSpp;
                 sppCount = min(rc0.segSpp, rc0.sppBegin + rc0.sppCount - sppBegin);
             }
@@ -230,7 +231,8 @@ static_assert(sizeof(MisRec) == sizeof(ShadowRec), "MIS rays share the staging s
 
 // Waves per SIMD: 4 (<= 128 VGPRs) for the plain and batched kernels: at 5 (<= 96 VGPRs) the
 // plastic / Oren-Nayar lobes spill 14 VGPRs and the stage takes 18.9 instead of 17.0 ms on the
-// 1080p64 terrain frame. The guided ones keep 3 for their extra state.
+// 1080p64 terrain frame. The guided ones keep 3 for their extra state (the untextured guided kernel
+// also fits 4 without spilling, and shades no faster).
 template <bool BATCH, bool GUIDED, bool TEX, bool MOTION, uint32_t BLOCK>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GUIDED ? 3 : 4)))
 kShade(DevScene s0, PathState ps, Queues q, RenderConsts rc0, uint32_t curSlot, uint32_t nextSlot, uint64_t* totals,
@@ -619,7 +621,8 @@ __global__ void __launch_bounds__(256) kRecord(PathState ps, Queues q, RenderCon
             const uint32_t g = meta.w;
             item = sceneOf(meta.z);
             const uint64_t G = (BATCH ? bt.pathOffsets[item] : static_cast<uint64_t>(item) * segmentPaths(rc0)) + g;
-            const uint64_t slot = G % ring.size;
+            // a single render's ring and path numbers stay below 2^32 (checked on the host)
+            const uint64_t slot = BATCH ? G % ring.size : static_cast<uint32_t>(G) % static_cast<uint32_t>(ring.size);
             ring.L[slot] = make_float4(L.x, L.y, L.z, fp.x);
             ring.y[slot] = fp.y;
             const bool train = BATCH ? (bt.guide != nullptr && isTrainPath(bt.guide[item], g)) : isTrainPath(gs, g);
