@@ -599,8 +599,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=1 << 16, help="reference samples per pixel of the guided comparison")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the render shards by tiles or "
-              "scenes with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a render would shard by film tiles "
+              "or by scenes, with no exchange between GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.inference_profile:
